@@ -1,0 +1,273 @@
+/*
+ * jdem.h -- C ABI of libjdem_hip.so, the MI355X (gfx950) DEM time step.
+ *
+ * Plain C: fixed-width integers, plain structs, raw pointers and sizes.  No
+ * C++, HIP or framework types cross this boundary; the HIP stream is passed
+ * as an opaque `void*` (a hipStream_t, or NULL for the null stream).
+ *
+ * Ownership: every array pointer in these structs is a DEVICE pointer owned
+ * by the caller (torch tensors through ctypes today; XLA buffers under a
+ * jax.ffi binding).  The library owns only the scratch memory of a context
+ * (`jdem_ctx`), which grows on demand and is reused across calls.
+ *
+ * Errors: every entry point returns an int status: JDEM_OK (0) or a negative
+ * JDEM_E* code; `jdem_last_error(ctx)` gives a message.  Runtime conditions
+ * of the reference that are flags and not exceptions (collider overflow,
+ * colliders/__init__.py:51-54; _partition.py:96-97, 149) are reported through
+ * the `overflow` fields, never as errors.
+ *
+ * Threading: one context per stream; contexts are independent.  Calls on one
+ * context are not thread-safe.
+ *
+ * Layout: (N, D) arrays are row-major float32 (D = 2 or 3, the spatial
+ * dimension); angular arrays are (N, 3) in 3D and (N, 1) in 2D; the
+ * quaternion is (N, 4) = [w, x, y, z]; neighbour lists are (N, M) int32
+ * padded with -1, as in JaxDEM (cell_list.py:500-595).
+ *
+ * Each entry point names the reference interface it replaces (file:line in
+ * cdelv/JaxDEM).
+ */
+#ifndef JDEM_H
+#define JDEM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define JDEM_ABI_VERSION 1
+
+enum {
+  JDEM_OK = 0,
+  JDEM_EINVAL = -1,       /* bad argument (shape, kind, null pointer)      */
+  JDEM_EHIP = -2,         /* a HIP runtime call failed                     */
+  JDEM_ENOMEM = -3,       /* scratch allocation failed                     */
+  JDEM_EUNSUPPORTED = -4  /* combination not implemented on the GPU path   */
+};
+
+/* Domain kinds: domains/free.py, domains/periodic.py, domains/reflect.py. */
+enum { JDEM_DOMAIN_FREE = 0, JDEM_DOMAIN_PERIODIC = 1, JDEM_DOMAIN_REFLECT = 2 };
+
+/* Pair laws: forces/spring.py, forces/hertz.py, forces/cundall_strack.py and
+ * two history-carrying laws with no reference counterpart. */
+enum {
+  JDEM_LAW_NONE = 0,
+  JDEM_LAW_SPRING = 1,
+  JDEM_LAW_HERTZ = 2,
+  JDEM_LAW_CUNDALL_STRACK = 3,
+  JDEM_LAW_SPRING_FRICTION = 4,
+  JDEM_LAW_HERTZ_FRICTION = 5
+};
+
+/* Integrators: integrators/velocity_verlet.py, direct_euler.py,
+ * velocity_verlet_spiral.py, spiral.py ("" = no-op). */
+enum { JDEM_LIN_NONE = 0, JDEM_LIN_VERLET = 1, JDEM_LIN_EULER = 2 };
+enum { JDEM_ROT_NONE = 0, JDEM_ROT_VERLET_SPIRAL = 1, JDEM_ROT_SPIRAL = 2 };
+
+/* Colliders: colliders/naive.py, cell_list.py, neighbor_list.py. */
+enum {
+  JDEM_COLLIDER_NONE = 0,
+  JDEM_COLLIDER_NAIVE = 1,
+  JDEM_COLLIDER_CELL_LIST = 2,
+  JDEM_COLLIDER_NEIGHBOR_LIST = 3
+};
+
+/* Pieces of one step for jdem_step_stage (system.py:60-82). */
+enum {
+  JDEM_STAGE_DOMAIN_APPLY = 1,
+  JDEM_STAGE_LIN_BEFORE = 2,
+  JDEM_STAGE_ROT_BEFORE = 3,
+  JDEM_STAGE_FORCE_MANAGER = 4,
+  JDEM_STAGE_LIN_AFTER = 5,
+  JDEM_STAGE_ROT_AFTER = 6
+};
+
+/* Particle state: the sphere fields of State (state.py:103-228). */
+typedef struct jdem_state {
+  int64_t n;          /* number of particles N                         */
+  int32_t dim;        /* 2 or 3                                        */
+  int32_t bond_width; /* W of bond_id (N, W); 0 if there are no bonds  */
+  float* pos;         /* pos_c (N, D), in/out                          */
+  float* vel;         /* (N, D), in/out                                */
+  float* ang_vel;     /* (N, 3) or (N, 1), in/out                      */
+  float* q;           /* (N, 4) [w x y z], in/out                      */
+  float* force;       /* (N, D), in/out                                */
+  float* torque;      /* (N, 3) or (N, 1), in/out                      */
+  const float* rad;     /* (N,)                                        */
+  const float* mass;    /* (N,)                                        */
+  const float* inertia; /* (N, 3) or (N, 1)                            */
+  const int32_t* mat_id;  /* (N,)                                      */
+  const int32_t* bond_id; /* (N, W), -1 padded, or NULL                */
+  const uint8_t* fixed;   /* (N,) 0/1                                  */
+} jdem_state;
+
+/* Domain parameters (domains/__init__.py:28-120). */
+typedef struct jdem_domain {
+  int32_t kind;
+  int32_t dim;
+  float box_size[3];
+  float inv_box_size[3];
+  float anchor[3];
+  float restitution; /* reflect.py:48 */
+} jdem_domain;
+
+/* Pair-law parameters.  Material-pair tables are (n_mat, n_mat) float32
+ * device arrays, material tables (n_mat,).  Which tables a law reads:
+ *   SPRING:          t0 = young_eff                    (spring.py:97-108)
+ *   HERTZ:           t0 = (4/3) E*                     (hertz.py:98-117)
+ *   CUNDALL_STRACK:  m0 = young, m1 = shear modulus G, m2 = mu, m3 = mu_r,
+ *                    t0 = beta(e_eff)          (cundall_strack.py:123-193)
+ *   SPRING_FRICTION: t0 = k_n, t1 = k_t, t2 = mu_eff; p0 = 2*damping ratio
+ *   HERTZ_FRICTION:  t0 = (4/3) E*, t1 = 8 G*, t2 = mu_eff; p0 = 2*damping
+ */
+typedef struct jdem_law {
+  int32_t kind;
+  int32_t n_mat;
+  const float* t0;
+  const float* t1;
+  const float* t2;
+  const float* m0;
+  const float* m1;
+  const float* m2;
+  const float* m3;
+  float p0;
+  float p1;
+} jdem_law;
+
+/* Cell grid of _grid_params (_partition.py:55-99), computed by the host in
+ * float32 exactly as the reference does. */
+typedef struct jdem_grid {
+  int32_t dims[3];
+  int32_t strides[3];
+  float cell_size[3];   /* per-axis effective cell size                 */
+  int32_t search_range; /* stencil half-width (cell_list.py:420-427)    */
+  int32_t periodic;
+  int64_t n_cells;
+} jdem_grid;
+
+/* NeighborList collider buffers (neighbor_list.py:137-286). */
+typedef struct jdem_nlist {
+  int32_t max_neighbors; /* M                                            */
+  float cutoff;          /* interaction cutoff                           */
+  float skin;            /* absolute skin                                */
+  int32_t* nl;           /* (N, M) in/out, -1 padded                     */
+  float* old_pos;        /* (N, D) in/out: positions at the last build   */
+  int32_t n_build_times; /* in/out (host value)                          */
+  int32_t overflow;      /* in/out (host value)                          */
+  int32_t secondary;     /* JDEM_COLLIDER_CELL_LIST or _NAIVE            */
+  jdem_grid grid;        /* secondary cell grid, already inflated so the  *
+                          * stencil reaches cutoff + skin                 */
+  float* hist_xi;        /* (N, M, D) tangential spring history or NULL  */
+  uint32_t* hist_live;   /* (N, ceil(M/32)) slot-carries-history bits     */
+} jdem_nlist;
+
+/* Everything one step reads, i.e. the System pytree (system.py:123-226). */
+typedef struct jdem_system {
+  jdem_state state;
+  jdem_domain domain;
+  jdem_law law;
+  int32_t lin_kind;
+  int32_t rot_kind;
+  float dt;
+  int32_t interact_same_bond_id;
+  /* ForceManager (force_manager.py:341-425), spheres only. */
+  float gravity[3];
+  const float* ext_force;     /* (N, D) or NULL */
+  const float* ext_force_com; /* (N, D) or NULL */
+  const float* ext_torque;    /* (N, A) or NULL */
+  /* collider */
+  int32_t collider_kind;
+  jdem_grid grid; /* CellList grid (compute_force)                        */
+  jdem_nlist nlist;
+} jdem_system;
+
+/* Counters filled by jdem_system_step. */
+typedef struct jdem_step_stats {
+  int64_t steps;             /* time steps advanced                      */
+  int64_t rebuilds;          /* neighbour-list rebuilds performed        */
+  int64_t host_syncs;        /* host synchronisations                    */
+  int32_t overflow;          /* collider overflow flag after the call    */
+  int32_t reserved;
+  double free_box[3];        /* FreeDomain box/anchor after the last step */
+  double free_anchor[3];
+} jdem_step_stats;
+
+typedef struct jdem_ctx jdem_ctx;
+
+/* ---- library ---------------------------------------------------------- */
+
+/* ABI version (JDEM_ABI_VERSION). */
+int jdem_abi_version(void);
+
+/* gfx target the library was built for, e.g. "gfx950". */
+const char* jdem_build_target(void);
+
+/* Create a context bound to `device` and `stream` (hipStream_t or NULL). */
+int jdem_ctx_create(int device, void* stream, jdem_ctx** out);
+int jdem_ctx_destroy(jdem_ctx* ctx);
+int jdem_ctx_set_stream(jdem_ctx* ctx, void* stream);
+/* Message for the last failed call on this context. */
+const char* jdem_last_error(const jdem_ctx* ctx);
+/* Bytes of scratch the context holds now. */
+int jdem_ctx_scratch_bytes(const jdem_ctx* ctx, size_t* bytes);
+
+/* ---- (a) cell list ------------------------------------------------------ */
+
+/* Hash + LDS-staged radix sort + cell start/end: the spatial partition of
+ * _get_spatial_partition (cell_list.py:37-87).  perm/sorted_hash are (N,)
+ * int32; cell_start/cell_end are (grid->n_cells,) int32 (empty cell:
+ * start == end). */
+int jdem_cell_build(jdem_ctx* ctx, const jdem_state* st, const jdem_domain* dom,
+                    const jdem_grid* grid, int32_t* perm, int32_t* sorted_hash,
+                    int32_t* cell_start, int32_t* cell_end);
+
+/* Candidate pairs within `cutoff`: DynamicCellList.create_neighbor_list
+ * (cell_list.py:500-595) when collider_kind is CELL_LIST (grid already
+ * inflated to the cutoff, cell_list.py:537-538), or
+ * NaiveSimulator.create_neighbor_list (naive.py:122-185) when NAIVE.
+ * nl is (N, M) int32 padded with -1; *overflow (host) receives the flag. */
+int jdem_neighbor_list(jdem_ctx* ctx, const jdem_system* sys, int32_t collider_kind,
+                       const jdem_grid* grid, float cutoff, int32_t max_neighbors,
+                       int32_t* nl, int32_t* overflow);
+
+/* ---- (b) forces ----------------------------------------------------------- */
+
+/* Collider.compute_force (colliders/__init__.py:56-80; naive.py:190-235;
+ * cell_list.py:437-464; neighbor_list.py:594-669): writes state.force and
+ * state.torque.  For NEIGHBOR_LIST it also performs _check_and_rebuild
+ * (neighbor_list.py:58-131) and updates sys->nlist in place. */
+int jdem_compute_force(jdem_ctx* ctx, jdem_system* sys);
+
+/* Collider.compute_potential_energy (cell_list.py:469-496; naive.py:76-117):
+ * 1/2 sum of pair energies, pair terms in float32, sum in float64.  Result
+ * written to *energy (host). */
+int jdem_potential_energy(jdem_ctx* ctx, jdem_system* sys, double* energy);
+
+/* Contact virial W = sum_{i<j} r_ij (x) F_ij (contacts.py:89-164) in float64,
+ * written to the host array w (dim*dim, row-major). */
+int jdem_contact_virial(jdem_ctx* ctx, jdem_system* sys, double* w);
+
+/* Kinetic energy, translational + rotational (utils/thermal.py:83-122), in
+ * float64 to *energy (host). */
+int jdem_kinetic_energy(jdem_ctx* ctx, const jdem_system* sys, double* energy);
+
+/* ---- (c) integration -------------------------------------------------------- */
+
+/* One stage of _step_once (system.py:60-82) on its own: domain.apply,
+ * {linear,rotation}_integrator.step_{before,after}_force, ForceManager.apply.
+ * For a FREE domain, DOMAIN_APPLY writes the new box/anchor into
+ * sys->domain (host). */
+int jdem_step_stage(jdem_ctx* ctx, jdem_system* sys, int32_t stage);
+
+/* System.step(state, system, n) (system.py:701-748): n fused steps.  State
+ * arrays are updated in place; sys->nlist is updated in place for the
+ * NeighborList collider.  stats may be NULL. */
+int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats* stats);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* JDEM_H */

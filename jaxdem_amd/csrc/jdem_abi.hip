@@ -1,0 +1,803 @@
+// jdem_abi.hip -- the C ABI of libjdem_hip.so (include/jdem.h): contexts,
+// scratch, and the host-side drivers that sequence the kernels of
+// jdem_cells.hip and jdem_step.hip on the context's stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+
+#include "jdem_internal.h"
+
+using namespace jdem;
+
+#ifndef JDEM_TARGET
+#define JDEM_TARGET "gfx950"
+#endif
+
+size_t jdem_ctx::scratch_bytes() const {
+  const Buf* all[] = {&pos2, &vel2, &ang2, &keys[0], &keys[1], &vals[0], &vals[1], &hist, &cstart, &cend, &spos,
+                      &nl_new, &xi_new, &live_new, &flags, &red, &nl_copy, &xi_copy, &live_copy};
+  size_t s = 0;
+  for (const Buf* b : all) s += b->bytes;
+  return s;
+}
+
+namespace jdem {
+bool ensure(jdem_ctx* ctx, Buf& b, size_t bytes) {
+  if (bytes == 0) bytes = 4;
+  if (b.bytes >= bytes) return true;
+  if (b.p) {
+    // the stream may still use the old buffer
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) return false;
+    (void)hipFree(b.p);
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  size_t want = bytes + bytes / 8;  // headroom for slowly growing callers
+  if (hipMalloc(&b.p, want) != hipSuccess) {
+    b.p = nullptr;
+    return false;
+  }
+  b.bytes = want;
+  return true;
+}
+}  // namespace jdem
+
+namespace {
+
+int fail(jdem_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                     \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return fail((ctx), e_ == hipErrorOutOfMemory ? JDEM_ENOMEM : JDEM_EHIP,                 \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                         \
+  } while (0)
+
+#define ENSURE(ctx, buf, bytes)                                                   \
+  do {                                                                            \
+    if (!ensure((ctx), (buf), (bytes)))                                           \
+      return fail((ctx), JDEM_ENOMEM, "scratch allocation failed: " #buf);        \
+  } while (0)
+
+DomP dom_of(const jdem_domain& d) {
+  DomP p;
+  p.kind = d.kind;
+  for (int a = 0; a < 3; ++a) {
+    p.box[a] = d.box_size[a];
+    p.inv_box[a] = d.inv_box_size[a];
+    p.anchor[a] = d.anchor[a];
+  }
+  p.restitution = d.restitution;
+  return p;
+}
+
+GridP grid_of(const jdem_grid& g) {
+  GridP p;
+  for (int a = 0; a < 3; ++a) {
+    p.dims[a] = g.dims[a] > 0 ? g.dims[a] : 1;
+    p.strides[a] = g.strides[a];
+    p.cell[a] = g.cell_size[a];
+  }
+  p.sr = g.search_range;
+  p.periodic = g.periodic;
+  p.n_cells = g.n_cells;
+  return p;
+}
+
+LawP law_of(const jdem_law& l) {
+  LawP p;
+  p.kind = l.kind;
+  p.n_mat = l.n_mat;
+  p.t0 = l.t0;
+  p.t1 = l.t1;
+  p.t2 = l.t2;
+  p.m0 = l.m0;
+  p.m1 = l.m1;
+  p.m2 = l.m2;
+  p.m3 = l.m3;
+  p.p0 = l.p0;
+  p.p1 = l.p1;
+  return p;
+}
+
+int check_system(jdem_ctx* ctx, const jdem_system* sys) {
+  if (!ctx) return JDEM_EINVAL;
+  if (!sys) return fail(ctx, JDEM_EINVAL, "null system");
+  const jdem_state& s = sys->state;
+  if (s.dim != 2 && s.dim != 3) return fail(ctx, JDEM_EINVAL, "dim must be 2 or 3");
+  if (s.n < 0 || s.n > (int64_t)INT32_MAX) return fail(ctx, JDEM_EINVAL, "particle count out of range");
+  if (s.n > 0 && (!s.pos || !s.vel || !s.ang_vel || !s.q || !s.force || !s.torque || !s.rad || !s.mass ||
+                  !s.inertia || !s.fixed))
+    return fail(ctx, JDEM_EINVAL, "null state array");
+  if (s.bond_width > 0 && !s.bond_id) return fail(ctx, JDEM_EINVAL, "bond_width > 0 but bond_id is null");
+  if (sys->law.n_mat > 1 && !s.mat_id) return fail(ctx, JDEM_EINVAL, "n_mat > 1 but mat_id is null");
+  const int lk = sys->law.kind;
+  if (sys->collider_kind != JDEM_COLLIDER_NONE &&
+      !(lk >= JDEM_LAW_SPRING && lk <= JDEM_LAW_HERTZ_FRICTION))
+    return fail(ctx, JDEM_EINVAL, "unknown law kind");
+  if (sys->law.n_mat < 1 || !sys->law.t0) {
+    if (sys->collider_kind != JDEM_COLLIDER_NONE && lk != JDEM_LAW_CUNDALL_STRACK)
+      return fail(ctx, JDEM_EINVAL, "law table t0 missing");
+  }
+  if (sys->domain.kind < JDEM_DOMAIN_FREE || sys->domain.kind > JDEM_DOMAIN_REFLECT)
+    return fail(ctx, JDEM_EINVAL, "unknown domain kind");
+  const bool hist_law = lk == JDEM_LAW_SPRING_FRICTION || lk == JDEM_LAW_HERTZ_FRICTION;
+  if (hist_law && sys->collider_kind != JDEM_COLLIDER_NONE) {
+    if (sys->collider_kind != JDEM_COLLIDER_NEIGHBOR_LIST)
+      return fail(ctx, JDEM_EINVAL, "history laws require the NeighborList collider");
+    if (!sys->nlist.hist_xi || !sys->nlist.hist_live)
+      return fail(ctx, JDEM_EINVAL, "history law without history buffers");
+  }
+  if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST) {
+    const jdem_nlist& L = sys->nlist;
+    if (L.max_neighbors < 0) return fail(ctx, JDEM_EINVAL, "max_neighbors < 0");
+    if (s.n > 0 && L.max_neighbors > 0 && (!L.nl || !L.old_pos))
+      return fail(ctx, JDEM_EINVAL, "neighbour-list buffers missing");
+    if (L.secondary != JDEM_COLLIDER_CELL_LIST && L.secondary != JDEM_COLLIDER_NAIVE)
+      return fail(ctx, JDEM_EINVAL, "secondary collider must be CellList or naive");
+  }
+  if (sys->collider_kind == JDEM_COLLIDER_CELL_LIST || (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST &&
+                                                        sys->nlist.secondary == JDEM_COLLIDER_CELL_LIST)) {
+    const jdem_grid& g = sys->collider_kind == JDEM_COLLIDER_CELL_LIST ? sys->grid : sys->nlist.grid;
+    if (g.n_cells <= 0 || g.n_cells > (int64_t)INT32_MAX)
+      return fail(ctx, JDEM_EUNSUPPORTED, "cell grid size out of the int32 hash range");
+    if (g.search_range < 1) return fail(ctx, JDEM_EINVAL, "search_range must be >= 1");
+  }
+  return JDEM_OK;
+}
+
+StepArgs base_args(const jdem_system* sys) {
+  StepArgs a;
+  std::memset(&a, 0, sizeof(a));
+  const jdem_state& s = sys->state;
+  a.n = s.n;
+  a.bond_w = s.bond_width;
+  a.isb = sys->interact_same_bond_id;
+  a.pos_in = s.pos;
+  a.vel_in = s.vel;
+  a.ang_in = s.ang_vel;
+  a.pos_out = s.pos;
+  a.vel_out = s.vel;
+  a.ang_out = s.ang_vel;
+  a.q = s.q;
+  a.force = s.force;
+  a.torque = s.torque;
+  a.rad = s.rad;
+  a.mass = s.mass;
+  a.inertia = s.inertia;
+  a.mat_id = s.mat_id;
+  a.bond_id = s.bond_id;
+  a.fixed = s.fixed;
+  a.ext_f = sys->ext_force;
+  a.ext_fc = sys->ext_force_com;
+  a.ext_t = sys->ext_torque;
+  for (int c = 0; c < 3; ++c) a.g[c] = sys->gravity[c];
+  a.has_fm = (sys->gravity[0] != 0.0f || sys->gravity[1] != 0.0f || sys->gravity[2] != 0.0f || a.ext_f ||
+              a.ext_fc || a.ext_t)
+                 ? 1
+                 : 0;
+  a.law = law_of(sys->law);
+  a.dom = dom_of(sys->domain);
+  a.dt = sys->dt;
+  a.lin = sys->lin_kind;
+  a.rot = sys->rot_kind;
+  a.M = sys->nlist.max_neighbors;
+  a.W32 = (a.M + 31) / 32;
+  a.nl = sys->nlist.nl;
+  a.hist_xi = sys->nlist.hist_xi;
+  a.hist_live = sys->nlist.hist_live;
+  return a;
+}
+
+struct Cells {
+  int32_t* perm;
+  int32_t* skeys;
+  int32_t* cstart;
+  int32_t* cend;
+  float4* spos;
+};
+
+int nbits_for(int64_t n_cells) {
+  int64_t m = n_cells > 1 ? n_cells - 1 : 1;
+  int b = 0;
+  while (m > 0) {
+    ++b;
+    m >>= 1;
+  }
+  return b;
+}
+
+// Hash + sort + cell ranges + sorted (x, r) of `pos` (cell_list.py:37-87).
+int build_cells(jdem_ctx* ctx, const float* pos, const float* rad, int64_t n, int dim, const DomP& dom,
+                const GridP& g, Cells* out) {
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  ENSURE(ctx, ctx->keys[0], 4 * nn);
+  ENSURE(ctx, ctx->keys[1], 4 * nn);
+  ENSURE(ctx, ctx->vals[0], 4 * nn);
+  ENSURE(ctx, ctx->vals[1], 4 * nn);
+  ENSURE(ctx, ctx->cstart, 4 * (size_t)g.n_cells);
+  ENSURE(ctx, ctx->cend, 4 * (size_t)g.n_cells);
+  ENSURE(ctx, ctx->spos, 16 * nn);
+  ENSURE(ctx, ctx->hist, 4 * 256 * (size_t)((n + 4095) / 4096 + 1));
+  uint32_t* hk = (uint32_t*)ctx->keys[1].p;
+  HIPCHK(ctx, launch_hash(pos, n, dim, dom, g, hk, ctx->stream));
+  uint32_t* sk = nullptr;
+  int32_t* perm = nullptr;
+  HIPCHK(ctx, radix_sort_pairs(ctx, hk, n, nbits_for(g.n_cells), &sk, &perm, ctx->stream));
+  int32_t* cs = (int32_t*)ctx->cstart.p;
+  int32_t* ce = (int32_t*)ctx->cend.p;
+  HIPCHK(ctx, launch_fill_i32(cs, g.n_cells, 0, ctx->stream));
+  HIPCHK(ctx, launch_fill_i32(ce, g.n_cells, 0, ctx->stream));
+  HIPCHK(ctx, launch_cell_ranges(sk, n, cs, ce, ctx->stream));
+  HIPCHK(ctx, launch_gather_spos(perm, pos, rad, n, dim, (float4*)ctx->spos.p, ctx->stream));
+  out->perm = perm;
+  out->skeys = (int32_t*)sk;
+  out->cstart = cs;
+  out->cend = ce;
+  out->spos = (float4*)ctx->spos.p;
+  return JDEM_OK;
+}
+
+// Device words in ctx->flags: [0], [1] rebuild-pending ping-pong, [2] list
+// overflow, [3] displacement flag of the standalone check.
+int ensure_flags(jdem_ctx* ctx) {
+  ENSURE(ctx, ctx->flags, 64);
+  if (!ctx->host_flags) {
+    if (hipHostMalloc((void**)&ctx->host_flags, 64, hipHostMallocDefault) != hipSuccess) {
+      ctx->host_flags = nullptr;
+      return fail(ctx, JDEM_ENOMEM, "pinned flag allocation failed");
+    }
+  }
+  return JDEM_OK;
+}
+
+int read_flags(jdem_ctx* ctx, int count) {
+  HIPCHK(ctx, hipMemcpyAsync(ctx->host_flags, ctx->flags.p, sizeof(int32_t) * count, hipMemcpyDeviceToHost,
+                             ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return JDEM_OK;
+}
+
+// Build a neighbour list of `pos` into nl_out (N, M) with the cell list or
+// the naive search; overflow accumulates into flags[2].
+int build_nlist(jdem_ctx* ctx, const jdem_system* sys, int kind, const jdem_grid& grid, const float* pos,
+                float cutoff, int M, int32_t* nl_out) {
+  const jdem_state& s = sys->state;
+  NLBuild b;
+  std::memset(&b, 0, sizeof(b));
+  b.n = s.n;
+  b.dim = s.dim;
+  b.M = M;
+  b.cutoff_sq = cutoff * cutoff;
+  b.isb = sys->interact_same_bond_id;
+  b.bond_w = s.bond_width;
+  b.bond_id = s.bond_id;
+  b.pos = pos;
+  b.dom = dom_of(sys->domain);
+  b.nl = nl_out;
+  b.overflow = (int32_t*)ctx->flags.p + 2;
+  if (kind == JDEM_COLLIDER_CELL_LIST) {
+    Cells c;
+    b.grid = grid_of(grid);
+    int rc = build_cells(ctx, pos, s.rad, s.n, s.dim, b.dom, b.grid, &c);
+    if (rc) return rc;
+    b.perm = c.perm;
+    b.skeys = c.skeys;
+    b.cstart = c.cstart;
+    b.cend = c.cend;
+    b.spos = c.spos;
+    HIPCHK(ctx, launch_nlist_cells(b, ctx->stream));
+  } else {
+    HIPCHK(ctx, launch_nlist_naive(b, ctx->stream));
+  }
+  return JDEM_OK;
+}
+
+// NeighborList rebuild at positions `pos` (neighbor_list.py:58-131, 545-589):
+// new list from the secondary collider, history remapped by partner id,
+// old_pos <- pos, n_build_times += 1.  flags[2] receives the overflow.
+int nl_rebuild(jdem_ctx* ctx, jdem_system* sys, const float* pos) {
+  jdem_nlist& L = sys->nlist;
+  const jdem_state& s = sys->state;
+  const int64_t n = s.n;
+  const int M = L.max_neighbors;
+  const int W32 = (M + 31) / 32;
+  const float lc = L.cutoff + L.skin;
+  HIPCHK(ctx, launch_fill_i32((int32_t*)ctx->flags.p + 2, 1, 0, ctx->stream));
+  if (M > 0 && n > 0) {
+    ENSURE(ctx, ctx->nl_new, 4 * (size_t)n * M);
+    int32_t* nl_new = (int32_t*)ctx->nl_new.p;
+    int rc = build_nlist(ctx, sys, L.secondary, L.grid, pos, lc, M, nl_new);
+    if (rc) return rc;
+    if (L.hist_xi && L.hist_live) {
+      if (L.n_build_times == 0) {
+        HIPCHK(ctx, launch_fill_i32((int32_t*)L.hist_live, n * W32, 0, ctx->stream));
+      } else {
+        ENSURE(ctx, ctx->xi_new, 4 * (size_t)n * M * s.dim);
+        ENSURE(ctx, ctx->live_new, 4 * (size_t)n * W32);
+        HIPCHK(ctx, launch_remap_hist(n, M, W32, s.dim, L.nl, L.hist_live, L.hist_xi, nl_new,
+                                      (uint32_t*)ctx->live_new.p, (float*)ctx->xi_new.p, ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(L.hist_xi, ctx->xi_new.p, 4 * (size_t)n * M * s.dim, hipMemcpyDeviceToDevice,
+                                   ctx->stream));
+        HIPCHK(ctx, hipMemcpyAsync(L.hist_live, ctx->live_new.p, 4 * (size_t)n * W32, hipMemcpyDeviceToDevice,
+                                   ctx->stream));
+      }
+    }
+    HIPCHK(ctx, hipMemcpyAsync(L.nl, nl_new, 4 * (size_t)n * M, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  if (n > 0) HIPCHK(ctx, launch_copy_f32(L.old_pos, pos, n * s.dim, ctx->stream));
+  L.n_build_times += 1;
+  return JDEM_OK;
+}
+
+__global__ void k_disp_check(const float* __restrict__ pos, const float* __restrict__ old, int64_t n, int dim,
+                             float trigger, int32_t* __restrict__ flag) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float d2;
+  const float dx = pos[dim * i] - old[dim * i];
+  const float dy = pos[dim * i + 1] - old[dim * i + 1];
+  if (dim == 3) {
+    const float dz = pos[dim * i + 2] - old[dim * i + 2];
+    d2 = (dx * dx + dy * dy) + dz * dz;
+  } else {
+    d2 = dx * dx + dy * dy;
+  }
+  if (d2 > trigger) *flag = 1;
+}
+
+float nl_trigger(const jdem_nlist& L) {
+  float sk = L.skin * L.skin;
+  return sk / 4.0f;
+}
+
+// _check_and_rebuild on its own (neighbor_list.py:58-131).
+int nl_check_and_rebuild(jdem_ctx* ctx, jdem_system* sys, bool* rebuilt) {
+  jdem_nlist& L = sys->nlist;
+  const jdem_state& s = sys->state;
+  *rebuilt = false;
+  int rc = ensure_flags(ctx);
+  if (rc) return rc;
+  bool need = L.n_build_times == 0;
+  if (!need && s.n > 0) {
+    HIPCHK(ctx, launch_fill_i32((int32_t*)ctx->flags.p + 3, 1, 0, ctx->stream));
+    hipLaunchKernelGGL(k_disp_check, dim3(nblocks(s.n)), dim3(kBlock), 0, ctx->stream, s.pos, L.old_pos, s.n, s.dim,
+                       nl_trigger(L), (int32_t*)ctx->flags.p + 3);
+    HIPCHK(ctx, hipGetLastError());
+    rc = read_flags(ctx, 4);
+    if (rc) return rc;
+    need = ctx->host_flags[3] != 0;
+  }
+  if (need) {
+    rc = nl_rebuild(ctx, sys, s.pos);
+    if (rc) return rc;
+    rc = read_flags(ctx, 3);
+    if (rc) return rc;
+    L.overflow = ctx->host_flags[2] != 0;
+    *rebuilt = true;
+  }
+  return JDEM_OK;
+}
+
+int src_of(int collider_kind) {
+  switch (collider_kind) {
+    case JDEM_COLLIDER_NAIVE:
+      return SRC_NAIVE;
+    case JDEM_COLLIDER_CELL_LIST:
+      return SRC_CELLS;
+    case JDEM_COLLIDER_NEIGHBOR_LIST:
+      return SRC_NLIST;
+    default:
+      return 0;
+  }
+}
+
+// Fill the source fields of `a` for the collider (building cells if needed).
+int prepare_source(jdem_ctx* ctx, jdem_system* sys, StepArgs& a, const float* pos) {
+  if (sys->collider_kind == JDEM_COLLIDER_CELL_LIST) {
+    Cells c;
+    a.grid = grid_of(sys->grid);
+    int rc = build_cells(ctx, pos, sys->state.rad, sys->state.n, sys->state.dim, a.dom, a.grid, &c);
+    if (rc) return rc;
+    a.perm = c.perm;
+    a.skeys = c.skeys;
+    a.cstart = c.cstart;
+    a.cend = c.cend;
+    a.spos = c.spos;
+  }
+  return JDEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jdem_abi_version(void) { return JDEM_ABI_VERSION; }
+
+const char* jdem_build_target(void) { return JDEM_TARGET; }
+
+int jdem_ctx_create(int device, void* stream, jdem_ctx** out) {
+  if (!out) return JDEM_EINVAL;
+  *out = nullptr;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return JDEM_EHIP;
+  if (device < 0 || device >= count) return JDEM_EINVAL;
+  if (hipSetDevice(device) != hipSuccess) return JDEM_EHIP;
+  jdem_ctx* c = new (std::nothrow) jdem_ctx();
+  if (!c) return JDEM_ENOMEM;
+  c->device = device;
+  c->stream = (hipStream_t)stream;
+  *out = c;
+  return JDEM_OK;
+}
+
+int jdem_ctx_destroy(jdem_ctx* ctx) {
+  if (!ctx) return JDEM_OK;
+  (void)hipSetDevice(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  Buf* all[] = {&ctx->pos2, &ctx->vel2, &ctx->ang2, &ctx->keys[0], &ctx->keys[1], &ctx->vals[0], &ctx->vals[1],
+                &ctx->hist, &ctx->cstart, &ctx->cend, &ctx->spos, &ctx->nl_new, &ctx->xi_new, &ctx->live_new,
+                &ctx->flags, &ctx->red, &ctx->nl_copy, &ctx->xi_copy, &ctx->live_copy};
+  for (Buf* b : all)
+    if (b->p) (void)hipFree(b->p);
+  if (ctx->host_flags) (void)hipHostFree(ctx->host_flags);
+  delete ctx;
+  return JDEM_OK;
+}
+
+int jdem_ctx_set_stream(jdem_ctx* ctx, void* stream) {
+  if (!ctx) return JDEM_EINVAL;
+  ctx->stream = (hipStream_t)stream;
+  return JDEM_OK;
+}
+
+const char* jdem_last_error(const jdem_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+int jdem_ctx_scratch_bytes(const jdem_ctx* ctx, size_t* bytes) {
+  if (!ctx || !bytes) return JDEM_EINVAL;
+  *bytes = ctx->scratch_bytes();
+  return JDEM_OK;
+}
+
+int jdem_cell_build(jdem_ctx* ctx, const jdem_state* st, const jdem_domain* dom, const jdem_grid* grid,
+                    int32_t* perm, int32_t* sorted_hash, int32_t* cell_start, int32_t* cell_end) {
+  if (!ctx) return JDEM_EINVAL;
+  if (!st || !dom || !grid || !perm || !sorted_hash || !cell_start || !cell_end)
+    return fail(ctx, JDEM_EINVAL, "null argument");
+  if (grid->n_cells <= 0 || grid->n_cells > (int64_t)INT32_MAX)
+    return fail(ctx, JDEM_EUNSUPPORTED, "cell grid size out of the int32 hash range");
+  (void)hipSetDevice(ctx->device);
+  Cells c;
+  int rc = build_cells(ctx, st->pos, st->rad, st->n, st->dim, dom_of(*dom), grid_of(*grid), &c);
+  if (rc) return rc;
+  const size_t n4 = 4 * (size_t)st->n;
+  if (st->n > 0) {
+    HIPCHK(ctx, hipMemcpyAsync(perm, c.perm, n4, hipMemcpyDeviceToDevice, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(sorted_hash, c.skeys, n4, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  HIPCHK(ctx, hipMemcpyAsync(cell_start, c.cstart, 4 * (size_t)grid->n_cells, hipMemcpyDeviceToDevice, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(cell_end, c.cend, 4 * (size_t)grid->n_cells, hipMemcpyDeviceToDevice, ctx->stream));
+  return JDEM_OK;
+}
+
+int jdem_neighbor_list(jdem_ctx* ctx, const jdem_system* sys, int32_t collider_kind, const jdem_grid* grid,
+                       float cutoff, int32_t max_neighbors, int32_t* nl, int32_t* overflow) {
+  if (!ctx) return JDEM_EINVAL;
+  jdem_system tmp = *sys;
+  tmp.collider_kind = JDEM_COLLIDER_NONE;
+  int rc = check_system(ctx, &tmp);
+  if (rc) return rc;
+  if (max_neighbors < 0 || (max_neighbors > 0 && sys->state.n > 0 && !nl))
+    return fail(ctx, JDEM_EINVAL, "bad neighbour-list output");
+  if (collider_kind != JDEM_COLLIDER_CELL_LIST && collider_kind != JDEM_COLLIDER_NAIVE)
+    return fail(ctx, JDEM_EINVAL, "collider must be CellList or naive");
+  if (collider_kind == JDEM_COLLIDER_CELL_LIST && (!grid || grid->n_cells <= 0 || grid->n_cells > (int64_t)INT32_MAX))
+    return fail(ctx, JDEM_EUNSUPPORTED, "cell grid size out of the int32 hash range");
+  (void)hipSetDevice(ctx->device);
+  rc = ensure_flags(ctx);
+  if (rc) return rc;
+  HIPCHK(ctx, launch_fill_i32((int32_t*)ctx->flags.p + 2, 1, 0, ctx->stream));
+  if (max_neighbors > 0 && sys->state.n > 0) {
+    jdem_grid g0;
+    std::memset(&g0, 0, sizeof(g0));
+    rc = build_nlist(ctx, sys, collider_kind, grid ? *grid : g0, sys->state.pos, cutoff, max_neighbors, nl);
+    if (rc) return rc;
+  }
+  rc = read_flags(ctx, 3);
+  if (rc) return rc;
+  if (overflow) *overflow = ctx->host_flags[2] != 0;
+  return JDEM_OK;
+}
+
+int jdem_compute_force(jdem_ctx* ctx, jdem_system* sys) {
+  int rc = check_system(ctx, sys);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  if (sys->collider_kind == JDEM_COLLIDER_NONE) {
+    const jdem_state& s = sys->state;
+    HIPCHK(ctx, hipMemsetAsync(s.force, 0, 4 * (size_t)s.n * s.dim, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(s.torque, 0, 4 * (size_t)s.n * (s.dim == 3 ? 3 : 1), ctx->stream));
+    return JDEM_OK;
+  }
+  if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST) {
+    bool rebuilt = false;
+    rc = nl_check_and_rebuild(ctx, sys, &rebuilt);
+    if (rc) return rc;
+  }
+  StepArgs a = base_args(sys);
+  rc = prepare_source(ctx, sys, a, sys->state.pos);
+  if (rc) return rc;
+  a.do_force = 1;
+  a.write_kin = 0;
+  HIPCHK(ctx, launch_step(a, sys->state.dim, sys->law.kind, src_of(sys->collider_kind), ctx->stream));
+  return JDEM_OK;
+}
+
+static int reduce_f64(jdem_ctx* ctx, const jdem_system* sys, int ncomp, double* host_out,
+                      hipError_t (*launch)(const StepArgs&, int, int, int, hipStream_t), bool by_collider) {
+  const jdem_state& s = sys->state;
+  const int64_t n = s.n;
+  const int nparts = 256;
+  ENSURE(ctx, ctx->red, 8 * ((size_t)(n > 0 ? n : 1) * ncomp + (size_t)nparts * ncomp + ncomp));
+  double* per = (double*)ctx->red.p;
+  double* part = per + (size_t)(n > 0 ? n : 1) * ncomp;
+  double* out = part + (size_t)nparts * ncomp;
+  StepArgs a = base_args(sys);
+  jdem_system* ms = const_cast<jdem_system*>(sys);
+  if (by_collider) {
+    int rc = prepare_source(ctx, ms, a, s.pos);
+    if (rc) return rc;
+  }
+  a.e_out = per;
+  if (n > 0) {
+    if (by_collider) {
+      HIPCHK(ctx, launch(a, s.dim, sys->law.kind, src_of(sys->collider_kind), ctx->stream));
+    } else {
+      HIPCHK(ctx, launch_kinetic(a, s.dim, ctx->stream));
+    }
+    HIPCHK(ctx, launch_sum_f64(per, n, ncomp, ncomp, part, out, nparts, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(host_out, out, 8 * (size_t)ncomp, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  } else {
+    for (int c = 0; c < ncomp; ++c) host_out[c] = 0.0;
+  }
+  return JDEM_OK;
+}
+
+int jdem_potential_energy(jdem_ctx* ctx, jdem_system* sys, double* energy) {
+  int rc = check_system(ctx, sys);
+  if (rc) return rc;
+  if (!energy) return fail(ctx, JDEM_EINVAL, "null output");
+  (void)hipSetDevice(ctx->device);
+  if (sys->collider_kind == JDEM_COLLIDER_NONE) {
+    *energy = 0.0;
+    return JDEM_OK;
+  }
+  if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST) {
+    bool rebuilt = false;
+    rc = nl_check_and_rebuild(ctx, sys, &rebuilt);
+    if (rc) return rc;
+  }
+  return reduce_f64(ctx, sys, 1, energy, launch_energy, true);
+}
+
+int jdem_contact_virial(jdem_ctx* ctx, jdem_system* sys, double* w) {
+  int rc = check_system(ctx, sys);
+  if (rc) return rc;
+  if (!w) return fail(ctx, JDEM_EINVAL, "null output");
+  (void)hipSetDevice(ctx->device);
+  const int D = sys->state.dim;
+  if (sys->collider_kind == JDEM_COLLIDER_NONE) {
+    for (int c = 0; c < D * D; ++c) w[c] = 0.0;
+    return JDEM_OK;
+  }
+  return reduce_f64(ctx, sys, D * D, w, launch_virial, true);
+}
+
+int jdem_kinetic_energy(jdem_ctx* ctx, const jdem_system* sys, double* energy) {
+  int rc = check_system(ctx, sys);
+  if (rc) return rc;
+  if (!energy) return fail(ctx, JDEM_EINVAL, "null output");
+  (void)hipSetDevice(ctx->device);
+  return reduce_f64(ctx, sys, 1, energy, nullptr, false);
+}
+
+static int free_box(jdem_ctx* ctx, const float* pos, const jdem_system* sys, float box[3], float anchor[3]) {
+  const jdem_state& s = sys->state;
+  const int nparts = 64;
+  ENSURE(ctx, ctx->red, 4 * 6 * (size_t)nparts);
+  HIPCHK(ctx, launch_box_minmax(pos, s.rad, s.n, s.dim, (float*)ctx->red.p, nparts, ctx->stream));
+  float hp[6 * 64];
+  HIPCHK(ctx, hipMemcpyAsync(hp, ctx->red.p, sizeof(hp), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int a = 0; a < s.dim; ++a) {
+    float lo = INFINITY, hi = -INFINITY;
+    for (int b = 0; b < nparts; ++b) {
+      lo = std::min(lo, hp[6 * b + a]);
+      hi = std::max(hi, hp[6 * b + 3 + a]);
+    }
+    box[a] = hi - lo;
+    anchor[a] = lo;
+  }
+  return JDEM_OK;
+}
+
+int jdem_step_stage(jdem_ctx* ctx, jdem_system* sys, int32_t stage) {
+  int rc = check_system(ctx, sys);
+  if (rc) return rc;
+  if (stage < JDEM_STAGE_DOMAIN_APPLY || stage > JDEM_STAGE_ROT_AFTER)
+    return fail(ctx, JDEM_EINVAL, "unknown stage");
+  (void)hipSetDevice(ctx->device);
+  if (stage == JDEM_STAGE_DOMAIN_APPLY && sys->domain.kind == JDEM_DOMAIN_FREE) {
+    if (sys->state.n == 0) return JDEM_OK;
+    float box[3], anc[3];
+    rc = free_box(ctx, sys->state.pos, sys, box, anc);
+    if (rc) return rc;
+    for (int a = 0; a < sys->state.dim; ++a) {
+      sys->domain.box_size[a] = box[a];
+      sys->domain.anchor[a] = anc[a];
+      sys->domain.inv_box_size[a] = 1.0f / box[a];
+    }
+    return JDEM_OK;
+  }
+  StepArgs a = base_args(sys);
+  HIPCHK(ctx, launch_stage(a, sys->state.dim, stage, ctx->stream));
+  return JDEM_OK;
+}
+
+int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats* stats) {
+  int rc = check_system(ctx, sys);
+  if (rc) return rc;
+  if (n < 0) return fail(ctx, JDEM_EINVAL, "n < 0");
+  jdem_step_stats local;
+  std::memset(&local, 0, sizeof(local));
+  jdem_step_stats* S = stats ? stats : &local;
+  std::memset(S, 0, sizeof(*S));
+  if (n == 0) return JDEM_OK;
+  if (sys->collider_kind == JDEM_COLLIDER_CELL_LIST && sys->domain.kind == JDEM_DOMAIN_FREE)
+    return fail(ctx, JDEM_EUNSUPPORTED, "CellList with a FreeDomain (per-step grid) is not implemented");
+  (void)hipSetDevice(ctx->device);
+  rc = ensure_flags(ctx);
+  if (rc) return rc;
+  jdem_state& s = sys->state;
+  const int64_t N = s.n;
+  const int D = s.dim;
+  const int A = D == 3 ? 3 : 1;
+  if (N == 0) {
+    S->steps = n;
+    return JDEM_OK;
+  }
+  ENSURE(ctx, ctx->pos2, 4 * (size_t)N * D);
+  ENSURE(ctx, ctx->vel2, 4 * (size_t)N * D);
+  ENSURE(ctx, ctx->ang2, 4 * (size_t)N * A);
+  float* P[2] = {s.pos, (float*)ctx->pos2.p};
+  float* V[2] = {s.vel, (float*)ctx->vel2.p};
+  float* W[2] = {s.ang_vel, (float*)ctx->ang2.p};
+  // kernel t (0..n) writes buffer wr(t) so that the last one writes the
+  // caller's arrays; it reads the buffer kernel t-1 wrote.
+  auto wr = [&](int64_t t) { return (int)((n - t) & 1); };
+  auto rd = [&](int64_t t) { return t == 0 ? 0 : wr(t - 1); };
+  const bool is_nl = sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST;
+  const int src = src_of(sys->collider_kind);
+  int32_t* flags = (int32_t*)ctx->flags.p;
+  HIPCHK(ctx, launch_fill_i32(flags, 4, 0, ctx->stream));
+  bool free_done = false;
+  float fbox[3] = {0, 0, 0}, fanc[3] = {0, 0, 0};
+
+  auto launch_t = [&](int64_t t) -> int {
+    StepArgs a = base_args(sys);
+    const int r = rd(t), w = wr(t);
+    a.pos_in = P[r];
+    a.vel_in = V[r];
+    a.ang_in = W[r];
+    a.pos_out = P[w];
+    a.vel_out = V[w];
+    a.ang_out = W[w];
+    a.do_force = t >= 1;
+    a.do_after = t >= 1;
+    a.do_before = t < n;
+    a.write_kin = 1;
+    a.step_idx = (int32_t)t;
+    if (sys->domain.kind == JDEM_DOMAIN_FREE && t == n - 1 && !free_done) {
+      // the last FreeDomain.apply (step n) sees the positions kernel n-1 reads
+      int rc2 = free_box(ctx, P[r], sys, fbox, fanc);
+      if (rc2) return rc2;
+      free_done = true;
+      S->host_syncs++;
+    }
+    if (a.do_force && sys->collider_kind != JDEM_COLLIDER_NONE) {
+      int rc2 = prepare_source(ctx, sys, a, P[r]);
+      if (rc2) return rc2;
+    }
+    int src_eff = src;
+    int law_eff = sys->law.kind;
+    if (sys->collider_kind == JDEM_COLLIDER_NONE) {
+      a.force_zero = 1;  // the "" collider (colliders/__init__.py:218-222)
+      src_eff = SRC_NAIVE;
+      law_eff = JDEM_LAW_SPRING;
+    }
+    if (is_nl) {
+      a.flag_rd = t >= 1 ? flags + (t & 1) : nullptr;
+      a.flag_wr = flags + ((t + 1) & 1);
+      a.check_disp = a.do_before ? 1 : 0;
+      a.old_pos = sys->nlist.old_pos;
+      a.trigger = nl_trigger(sys->nlist);
+    }
+    hipError_t e = launch_step(a, D, law_eff, src_eff, ctx->stream);
+    if (e != hipSuccess) return fail(ctx, JDEM_EHIP, std::string("step kernel: ") + hipGetErrorString(e));
+    return JDEM_OK;
+  };
+
+  if (!is_nl) {
+    for (int64_t t = 0; t <= n; ++t) {
+      rc = launch_t(t);
+      if (rc) return rc;
+    }
+  } else {
+    // Speculative chunks: every step kernel checks the rebuild word set by its
+    // predecessor and turns into a no-op when a rebuild is pending; the host
+    // reads the words once per chunk, rebuilds, and resumes at that step.
+    jdem_nlist& L = sys->nlist;
+    rc = launch_t(0);
+    if (rc) return rc;
+    bool pending = L.n_build_times == 0;
+    int64_t next = 1;
+    int64_t chunk_len = 8;
+    int64_t last_rebuild_at = 0;
+    int64_t interval = 0;
+    bool rebuilt_here = false;
+    while (next <= n) {
+      if (pending) {
+        rc = nl_rebuild(ctx, sys, P[rd(next)]);
+        if (rc) return rc;
+        HIPCHK(ctx, launch_fill_i32(flags, 2, 0, ctx->stream));
+        S->rebuilds++;
+        rebuilt_here = true;
+        if (last_rebuild_at > 0) interval = next - last_rebuild_at;
+        last_rebuild_at = next;
+        pending = false;
+        if (interval > 0) chunk_len = std::max<int64_t>(2, interval - 1);
+      }
+      const int64_t cnt = std::min<int64_t>(chunk_len, n - next + 1);
+      for (int64_t t = next; t < next + cnt; ++t) {
+        rc = launch_t(t);
+        if (rc) return rc;
+      }
+      rc = read_flags(ctx, 3);
+      if (rc) return rc;
+      S->host_syncs++;
+      const int32_t v = std::max(ctx->host_flags[0], ctx->host_flags[1]);
+      if (v == 0) {
+        next += cnt;
+        chunk_len = std::min<int64_t>(chunk_len * 2, 1024);
+      } else {
+        next = v;
+        pending = true;
+      }
+    }
+    // overflow of the most recent build (neighbor_list.py:582-589)
+    if (rebuilt_here) L.overflow = ctx->host_flags[2] != 0;
+    S->overflow = L.overflow;
+  }
+  if (free_done) {
+    for (int a = 0; a < D; ++a) {
+      sys->domain.box_size[a] = fbox[a];
+      sys->domain.anchor[a] = fanc[a];
+      sys->domain.inv_box_size[a] = 1.0f / fbox[a];
+      S->free_box[a] = fbox[a];
+      S->free_anchor[a] = fanc[a];
+    }
+  }
+  S->steps = n;
+  return JDEM_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,167 @@
+// jdem_internal.h -- context, scratch arena and kernel-argument structs shared
+// by the translation units of libjdem_hip.so.  Not part of the C ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+
+#include "jdem.h"
+#include "jdem_device.h"
+
+namespace jdem {
+
+constexpr int kBlock = 256;
+
+// Grid parameters on the device (jdem_grid).
+struct GridP {
+  int dims[3];
+  int strides[3];
+  float cell[3];
+  int sr;
+  int periodic;
+  int64_t n_cells;
+};
+
+// One grow-only device allocation of the context.
+struct Buf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+}  // namespace jdem
+
+struct jdem_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  // scratch, grown on demand and reused across calls
+  jdem::Buf pos2, vel2, ang2;          // ping-pong kinematics
+  jdem::Buf keys[2], vals[2], hist;    // radix sort
+  jdem::Buf cstart, cend, spos;        // cell tables, sorted (x, r)
+  jdem::Buf nl_new, xi_new, live_new;  // neighbour-list rebuild
+  jdem::Buf flags;                     // small device words
+  jdem::Buf red;                       // reduction partials
+  jdem::Buf nl_copy, xi_copy, live_copy;
+  int32_t* host_flags = nullptr;       // pinned host mirror of flags
+  size_t scratch_bytes() const;
+};
+
+namespace jdem {
+
+// Ensure b holds at least `bytes`; returns false on allocation failure.
+bool ensure(jdem_ctx* ctx, Buf& b, size_t bytes);
+
+// Kernel arguments of the fused step / force / energy kernels.
+struct StepArgs {
+  int64_t n;
+  int bond_w;
+  int isb;
+  const float* pos_in;
+  const float* vel_in;
+  const float* ang_in;
+  float* pos_out;
+  float* vel_out;
+  float* ang_out;
+  float* q;
+  float* force;
+  float* torque;
+  const float* rad;
+  const float* mass;
+  const float* inertia;
+  const int32_t* mat_id;
+  const int32_t* bond_id;
+  const uint8_t* fixed;
+  const float* ext_f;
+  const float* ext_fc;
+  const float* ext_t;
+  float g[3];
+  int has_fm;
+  LawP law;
+  DomP dom;
+  float dt;
+  int lin;
+  int rot;
+  // cell-list source
+  const int32_t* perm;
+  const int32_t* skeys;
+  const int32_t* cstart;
+  const int32_t* cend;
+  const float4* spos;
+  GridP grid;
+  // neighbour-list source
+  const int32_t* nl;
+  int M;
+  float* hist_xi;
+  uint32_t* hist_live;
+  int W32;
+  // phases
+  int do_force;
+  int force_zero;  // collider "": contact force and torque are zero
+  int do_after;
+  int do_before;
+  int write_kin;
+  // neighbour-list displacement check (neighbor_list.py:78-85)
+  const float* old_pos;
+  float trigger;
+  int check_disp;
+  const int32_t* flag_rd;
+  int32_t* flag_wr;
+  int32_t step_idx;
+  // energy / virial outputs (per particle, float64)
+  double* e_out;
+};
+
+enum Src { SRC_NAIVE = 1, SRC_CELLS = 2, SRC_NLIST = 3 };
+
+// launchers (jdem_step.hip)
+hipError_t launch_step(const StepArgs& a, int dim, int law, int src, hipStream_t s);
+hipError_t launch_energy(const StepArgs& a, int dim, int law, int src, hipStream_t s);
+hipError_t launch_virial(const StepArgs& a, int dim, int law, int src, hipStream_t s);
+
+// launchers (jdem_cells.hip)
+hipError_t launch_hash(const float* pos, int64_t n, int dim, const DomP& dom, const GridP& g, uint32_t* keys,
+                       hipStream_t s);
+// Sort (keys, iota) by key; results in (keys_out, vals_out).  nbits = key bits.
+hipError_t radix_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, int nbits, uint32_t** keys_out,
+                            int32_t** vals_out, hipStream_t s);
+hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
+hipError_t launch_cell_ranges(const uint32_t* skeys, int64_t n, int32_t* cstart, int32_t* cend, hipStream_t s);
+hipError_t launch_gather_spos(const int32_t* perm, const float* pos, const float* rad, int64_t n, int dim,
+                              float4* spos, hipStream_t s);
+struct NLBuild {
+  int64_t n;
+  int dim;
+  int M;
+  float cutoff_sq;
+  int isb;
+  int bond_w;
+  const int32_t* bond_id;
+  const float* pos;
+  const int32_t* perm;
+  const int32_t* skeys;
+  const int32_t* cstart;
+  const int32_t* cend;
+  const float4* spos;
+  GridP grid;
+  DomP dom;
+  int32_t* nl;
+  int32_t* overflow;
+};
+hipError_t launch_nlist_cells(const NLBuild& b, hipStream_t s);
+hipError_t launch_nlist_naive(const NLBuild& b, hipStream_t s);
+hipError_t launch_remap_hist(int64_t n, int M, int W32, int dim, const int32_t* old_nl, const uint32_t* old_live,
+                             const float* old_xi, const int32_t* new_nl, uint32_t* new_live, float* new_xi,
+                             hipStream_t s);
+hipError_t launch_box_minmax(const float* pos, const float* rad, int64_t n, int dim, float* part, int nparts,
+                             hipStream_t s);
+hipError_t launch_sum_f64(const double* in, int64_t n, int stride, int ncomp, double* partial, double* out,
+                          int nparts, hipStream_t s);
+hipError_t launch_kinetic(const StepArgs& a, int dim, hipStream_t s);
+hipError_t launch_stage(const StepArgs& a, int dim, int stage, hipStream_t s);
+hipError_t launch_copy_f32(float* dst, const float* src, int64_t n, hipStream_t s);
+
+inline unsigned nblocks(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
+
+}  // namespace jdem

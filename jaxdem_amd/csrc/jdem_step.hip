@@ -1,0 +1,595 @@
+// jdem_step.hip -- (b) neighbour traversal with the pair laws and (c) the
+// fused integration of one DEM time step on gfx950.
+//
+// One thread owns one particle i and accumulates F_i and T_i in registers
+// over its candidates in the reference's traversal order -- no atomics, so
+// results are deterministic run to run.  The same kernel then applies
+// ForceManager, step_after_force of step k, and domain.apply +
+// step_before_force of step k+1 (system.py:60-82), writing positions and
+// velocities to the other half of a ping-pong pair because neighbours of
+// i still read the step-k values.
+//
+// Reference: colliders/naive.py:190-235, cell_list.py:188-261, 437-496,
+// neighbor_list.py:594-669; forces/force_manager.py:341-425;
+// integrators/velocity_verlet.py:57-94, direct_euler.py:62-65,
+// velocity_verlet_spiral.py:83-180, spiral.py:60-144; domains/reflect.py:
+// 102-297.
+#include <hip/hip_runtime.h>
+
+#include "jdem_internal.h"
+
+namespace jdem {
+
+template <int D>
+JD Vec<D> v4s(float4 a);
+template <>
+JD Vec<3> v4s<3>(float4 a) { return {a.x, a.y, a.z}; }
+template <>
+JD Vec<2> v4s<2>(float4 a) { return {a.x, a.y}; }
+
+JD bool is_bonded(const StepArgs& a, int64_t i, int32_t j) {
+  for (int w = 0; w < a.bond_w; ++w)
+    if (a.bond_id[i * a.bond_w + w] == j) return true;
+  return false;
+}
+
+template <int D, int LAW>
+JD void fill_partner(const StepArgs& a, Side<D>& b, int32_t j) {
+  if (a.law.n_mat > 1) b.mat = a.mat_id[j];
+  if constexpr (LawTraits<LAW>::kin) {
+    b.v = ld<D>(a.vel_in, j);
+    b.w = lda<D>(a.ang_in, j);
+    b.m = a.mass[j];
+  }
+}
+
+// Evaluate the pair (i <- j) and accumulate.  Returns true if in contact.
+template <int D, int LAW, bool PER>
+JD bool visit(const StepArgs& a, const Side<D>& me, int64_t i, int32_t j, const Vec<D>& xj, float rj, Vec<D>& F,
+              Ang<D>& T, const Vec<D>& xi_old, Vec<D>& xi_new) {
+  const Vec<D> rij = disp_<PER>(me.x, xj, a.dom);
+  const float d_sq = dot(rij, rij);
+  const float R = me.r + rj;
+  // No contact is possible: delta of every law is exactly 0 (|rij| exceeds
+  // R by far more than the rounding of d_sq * rsqrt(d_sq)).
+  if (d_sq > (R * R) * 1.0001f) return false;
+  if (j == (int32_t)i) return false;
+  if (!a.isb && a.bond_w > 0 && is_bonded(a, i, j)) return false;
+  Side<D> b;
+  b.x = xj;
+  b.r = rj;
+  b.mat = 0;
+  fill_partner<D, LAW>(a, b, j);
+  PairOut<D> o = law_eval<D, LAW>(a.law, me, b, rij, d_sq, xi_old, a.dt);
+  F = F + o.F;
+  if constexpr (LawTraits<LAW>::torque) T = T + o.T;
+  if constexpr (LawTraits<LAW>::hist) xi_new = o.xi;
+  return o.contact;
+}
+
+// Per-particle force from the three candidate sources.
+template <int D, int LAW, int SRC, bool PER>
+JD void particle_force(const StepArgs& a, int64_t idx, int64_t i, const Side<D>& me, Vec<D>& F, Ang<D>& T) {
+  F = zero_vec<D>();
+  T = zero_ang<D>();
+  Vec<D> xo = zero_vec<D>(), xn;
+  if constexpr (SRC == SRC_NAIVE) {
+    for (int64_t j = 0; j < a.n; ++j) {
+      visit<D, LAW, PER>(a, me, i, (int32_t)j, ld<D>(a.pos_in, j), a.rad[j], F, T, xo, xn);
+    }
+  } else if constexpr (SRC == SRC_CELLS) {
+    const uint32_t key = (uint32_t)a.skeys[idx];
+    int c[3];
+    c[0] = (int)(key % (uint32_t)a.grid.dims[0]);
+    c[1] = (int)((key / (uint32_t)a.grid.dims[0]) % (uint32_t)a.grid.dims[1]);
+    c[2] = (D == 3) ? (int)(key / ((uint32_t)a.grid.dims[0] * (uint32_t)a.grid.dims[1])) : 0;
+    const int sr = a.grid.sr;
+    int hi[3];
+    for (int ax = 0; ax < D; ++ax) hi[ax] = PER ? min(sr, a.grid.dims[ax] - sr - 1) : sr;
+    for (int ox = -sr; ox <= hi[0]; ++ox) {
+      int cx = c[0] + ox;
+      if (PER) {
+        cx = cx % a.grid.dims[0];
+        if (cx < 0) cx += a.grid.dims[0];
+      } else if (cx < 0 || cx >= a.grid.dims[0])
+        continue;
+      for (int oy = -sr; oy <= hi[1]; ++oy) {
+        int cy = c[1] + oy;
+        if (PER) {
+          cy = cy % a.grid.dims[1];
+          if (cy < 0) cy += a.grid.dims[1];
+        } else if (cy < 0 || cy >= a.grid.dims[1])
+          continue;
+        const int zlo = (D == 3) ? -sr : 0, zhi = (D == 3) ? hi[2] : 0;
+        for (int oz = zlo; oz <= zhi; ++oz) {
+          int cz = 0;
+          if (D == 3) {
+            cz = c[2] + oz;
+            if (PER) {
+              cz = cz % a.grid.dims[2];
+              if (cz < 0) cz += a.grid.dims[2];
+            } else if (cz < 0 || cz >= a.grid.dims[2])
+              continue;
+          }
+          const int h = cx * a.grid.strides[0] + cy * a.grid.strides[1] + (D == 3 ? cz * a.grid.strides[2] : 0);
+          const int k1 = a.cend[h];
+          for (int k = a.cstart[h]; k < k1; ++k) {
+            const float4 p = a.spos[k];
+            visit<D, LAW, PER>(a, me, i, a.perm[k], v4s<D>(p), p.w, F, T, xo, xn);
+          }
+        }
+      }
+    }
+  } else {  // SRC_NLIST
+    const int32_t* row = a.nl + i * a.M;
+    for (int w = 0; w < a.W32; ++w) {
+      uint32_t live_old = 0, live_new = 0;
+      if constexpr (LawTraits<LAW>::hist) live_old = a.hist_live[i * a.W32 + w];
+      const int s_end = min(a.M, (w + 1) * 32);
+      bool done = false;
+      for (int sl = w * 32; sl < s_end; ++sl) {
+        const int32_t j = row[sl];
+        if (j < 0) {
+          done = true;
+          break;
+        }
+        Vec<D> xold = zero_vec<D>();
+        if constexpr (LawTraits<LAW>::hist) {
+          if ((live_old >> (sl & 31)) & 1u) xold = ld<D>(a.hist_xi, i * a.M + sl);
+        }
+        Vec<D> xnew;
+        const bool c = visit<D, LAW, PER>(a, me, i, j, ld<D>(a.pos_in, j), a.rad[j], F, T, xold, xnew);
+        if constexpr (LawTraits<LAW>::hist) {
+          if (c) {
+            st(a.hist_xi, i * a.M + sl, xnew);
+            live_new |= 1u << (sl & 31);
+          }
+        }
+      }
+      if constexpr (LawTraits<LAW>::hist) a.hist_live[i * a.W32 + w] = live_new;
+      if (done) {
+        if constexpr (LawTraits<LAW>::hist)
+          for (int w2 = w + 1; w2 < a.W32; ++w2) a.hist_live[i * a.W32 + w2] = 0u;
+        break;
+      }
+    }
+  }
+}
+
+template <int D, int LAW>
+JD void load_side(const StepArgs& a, int64_t i, Side<D>& me) {
+  me.x = ld<D>(a.pos_in, i);
+  me.r = a.rad[i];
+  me.mat = (a.law.n_mat > 1) ? a.mat_id[i] : 0;
+  me.v = ld<D>(a.vel_in, i);
+  me.w = lda<D>(a.ang_in, i);
+  me.m = a.mass[i];
+}
+
+// The fused step kernel.  Phases (flags): do_force computes F, T at the
+// current positions (else they are read from force/torque); do_after runs
+// ForceManager + step_after_force of step k; do_before runs domain.apply +
+// step_before_force of step k+1 and the NeighborList displacement check.
+template <int D, int LAW, int SRC, bool PER>
+__global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
+  if (a.flag_rd && *a.flag_rd) {  // a neighbour-list rebuild is pending
+    if (a.flag_wr) *a.flag_wr = *a.flag_rd;
+    return;
+  }
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.n) return;
+  const int64_t i = (SRC == SRC_CELLS && a.do_force) ? (int64_t)a.perm[idx] : idx;
+  Side<D> me;
+  load_side<D, LAW>(a, i, me);
+  Vec<D> F;
+  Ang<D> T;
+  if (a.do_force && a.force_zero) {
+    F = zero_vec<D>();
+    T = zero_ang<D>();
+  } else if (a.do_force) {
+    particle_force<D, LAW, SRC, PER>(a, idx, i, me, F, T);
+  } else {
+    F = ld<D>(a.force, i);
+    T = lda<D>(a.torque, i);
+  }
+  if (!a.write_kin) {  // Collider.compute_force only
+    st(a.force, i, F);
+    sta(a.torque, i, T);
+    return;
+  }
+  Vec<D> x = me.x, v = me.v;
+  Ang<D> w = me.w;
+  Quat q = ldq(a.q, i);
+  const Ang<D> I = lda<D>(a.inertia, i);
+  const float m = me.m;
+  const bool fixed = a.fixed[i] != 0;
+  const float freef = fixed ? 0.0f : 1.0f;
+  if (a.do_after) {
+    // ForceManager.apply for spheres (force_manager.py:358-424)
+    if (a.has_fm) {
+      Vec<D> fp = a.ext_f ? ld<D>(a.ext_f, i) : zero_vec<D>();
+      Vec<D> fc = a.ext_fc ? ld<D>(a.ext_fc, i) : zero_vec<D>();
+      Vec<D> g;
+      if constexpr (D == 3)
+        g = Vec<3>{a.g[0], a.g[1], a.g[2]};
+      else
+        g = Vec<2>{a.g[0], a.g[1]};
+      fc = fc + g * (m / 1.0f);
+      F = (F + fp) + fc;
+      if (a.ext_t) T = T + lda<D>(a.ext_t, i);
+    }
+    // step_after_force
+    if (a.lin == JDEM_LIN_VERLET) {
+      const float s = (a.dt * 0.5f) / m;
+      v = v + (F * s) * freef;
+    } else if (a.lin == JDEM_LIN_EULER) {
+      const float s = a.dt / m;
+      v = v + (F * s) * freef;
+      x = x + v * a.dt;
+    }
+    if (a.rot == JDEM_ROT_VERLET_SPIRAL)
+      vvs_after<D>(q, w, T, I, a.dt, freef);
+    else if (a.rot == JDEM_ROT_SPIRAL)
+      spiral_after<D>(q, w, T, I, a.dt, freef);
+  }
+  if (a.do_before) {
+    // domain.apply of step k+1 (reflect.py:102-297; free/periodic: no-op)
+    if (a.dom.kind == JDEM_DOMAIN_REFLECT) reflect_apply<D>(a.dom, x, v, q, F, m, me.r, fixed, a.dt);
+    // step_before_force of step k+1
+    if (a.lin == JDEM_LIN_VERLET) {
+      const float s = (a.dt * 0.5f) / m;
+      v = v + (F * s) * freef;
+      x = x + v * a.dt;
+    }
+    if (a.rot == JDEM_ROT_VERLET_SPIRAL) vvs_before<D>(q, w, T, I, a.dt, freef);
+  }
+  st(a.pos_out, i, x);
+  st(a.vel_out, i, v);
+  sta(a.ang_out, i, w);
+  stq(a.q, i, q);
+  st(a.force, i, F);
+  sta(a.torque, i, T);
+  if (a.do_before && a.check_disp) {
+    const Vec<D> d = x - ld<D>(a.old_pos, i);
+    if (dot(d, d) > a.trigger) *a.flag_wr = a.step_idx + 1;
+  }
+}
+
+// Potential energy: 1/2 sum of pair energies per particle, float64 sum.
+template <int D, int LAW, int SRC, bool PER>
+__global__ void __launch_bounds__(kBlock) k_energy(StepArgs a) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.n) return;
+  const int64_t i = (SRC == SRC_CELLS) ? (int64_t)a.perm[idx] : idx;
+  Side<D> me;
+  me.x = ld<D>(a.pos_in, i);
+  me.r = a.rad[i];
+  me.mat = (a.law.n_mat > 1) ? a.mat_id[i] : 0;
+  double acc = 0.0;
+  auto one = [&](int32_t j, Vec<D> xj, float rj) {
+    if (j == (int32_t)i) return;
+    if (!a.isb && a.bond_w > 0 && is_bonded(a, i, j)) return;
+    const Vec<D> rij = disp_<PER>(me.x, xj, a.dom);
+    const float d_sq = dot(rij, rij);
+    Side<D> b;
+    b.x = xj;
+    b.r = rj;
+    b.mat = (a.law.n_mat > 1) ? a.mat_id[j] : 0;
+    const float e = law_energy<D, LAW>(a.law, me, b, d_sq);
+    acc += 0.5 * (double)e;
+  };
+  if constexpr (SRC == SRC_NAIVE) {
+    for (int64_t j = 0; j < a.n; ++j) one((int32_t)j, ld<D>(a.pos_in, j), a.rad[j]);
+  } else if constexpr (SRC == SRC_NLIST) {
+    const int32_t* row = a.nl + i * a.M;
+    for (int sl = 0; sl < a.M; ++sl) {
+      const int32_t j = row[sl];
+      if (j < 0) break;
+      one(j, ld<D>(a.pos_in, j), a.rad[j]);
+    }
+  } else {
+    const uint32_t key = (uint32_t)a.skeys[idx];
+    int c[3];
+    c[0] = (int)(key % (uint32_t)a.grid.dims[0]);
+    c[1] = (int)((key / (uint32_t)a.grid.dims[0]) % (uint32_t)a.grid.dims[1]);
+    c[2] = (D == 3) ? (int)(key / ((uint32_t)a.grid.dims[0] * (uint32_t)a.grid.dims[1])) : 0;
+    const int sr = a.grid.sr;
+    int hi[3];
+    for (int ax = 0; ax < D; ++ax) hi[ax] = PER ? min(sr, a.grid.dims[ax] - sr - 1) : sr;
+    int o[3];
+    for (o[0] = -sr; o[0] <= hi[0]; ++o[0])
+      for (o[1] = -sr; o[1] <= hi[1]; ++o[1])
+        for (o[2] = (D == 3 ? -sr : 0); o[2] <= (D == 3 ? hi[2] : 0); ++o[2]) {
+          int h = 0;
+          bool ok = true;
+          for (int ax = 0; ax < D; ++ax) {
+            int cc = c[ax] + o[ax];
+            if (PER) {
+              cc = cc % a.grid.dims[ax];
+              if (cc < 0) cc += a.grid.dims[ax];
+            } else if (cc < 0 || cc >= a.grid.dims[ax]) {
+              ok = false;
+            }
+            h += cc * a.grid.strides[ax];
+          }
+          if (!ok) continue;
+          for (int k = a.cstart[h]; k < a.cend[h]; ++k) {
+            const float4 p = a.spos[k];
+            one(a.perm[k], v4s<D>(p), p.w);
+          }
+        }
+  }
+  a.e_out[i] = acc;
+}
+
+// Contact virial sum_{i<j} r_ij (x) F_ij (contacts.py:89-164): each i-side
+// pair contributes 1/2 r_ij (x) F_ij, float64.
+template <int D, int LAW, int SRC, bool PER>
+__global__ void __launch_bounds__(kBlock) k_virial(StepArgs a) {
+  const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= a.n) return;
+  const int64_t i = (SRC == SRC_CELLS) ? (int64_t)a.perm[idx] : idx;
+  Side<D> me;
+  load_side<D, LAW>(a, i, me);
+  double W[D * D];
+  for (int c = 0; c < D * D; ++c) W[c] = 0.0;
+  auto one = [&](int32_t j, Vec<D> xj, float rj, Vec<D> xold) {
+    if (j == (int32_t)i) return;
+    if (!a.isb && a.bond_w > 0 && is_bonded(a, i, j)) return;
+    const Vec<D> rij = disp_<PER>(me.x, xj, a.dom);
+    const float d_sq = dot(rij, rij);
+    Side<D> b;
+    b.x = xj;
+    b.r = rj;
+    b.mat = 0;
+    fill_partner<D, LAW>(a, b, j);
+    PairOut<D> o = law_eval<D, LAW>(a.law, me, b, rij, d_sq, xold, a.dt);
+    const float* r = &rij.x;
+    const float* f = &o.F.x;
+    for (int p = 0; p < D; ++p)
+      for (int c = 0; c < D; ++c) W[p * D + c] += 0.5 * ((double)r[p] * (double)f[c]);
+  };
+  if constexpr (SRC == SRC_NAIVE) {
+    for (int64_t j = 0; j < a.n; ++j) one((int32_t)j, ld<D>(a.pos_in, j), a.rad[j], zero_vec<D>());
+  } else if constexpr (SRC == SRC_NLIST) {
+    const int32_t* row = a.nl + i * a.M;
+    for (int sl = 0; sl < a.M; ++sl) {
+      const int32_t j = row[sl];
+      if (j < 0) break;
+      Vec<D> xold = zero_vec<D>();
+      if constexpr (LawTraits<LAW>::hist) {
+        if ((a.hist_live[i * a.W32 + (sl >> 5)] >> (sl & 31)) & 1u) xold = ld<D>(a.hist_xi, i * a.M + sl);
+      }
+      one(j, ld<D>(a.pos_in, j), a.rad[j], xold);
+    }
+  } else {
+    const uint32_t key = (uint32_t)a.skeys[idx];
+    int c[3];
+    c[0] = (int)(key % (uint32_t)a.grid.dims[0]);
+    c[1] = (int)((key / (uint32_t)a.grid.dims[0]) % (uint32_t)a.grid.dims[1]);
+    c[2] = (D == 3) ? (int)(key / ((uint32_t)a.grid.dims[0] * (uint32_t)a.grid.dims[1])) : 0;
+    const int sr = a.grid.sr;
+    int hi[3];
+    for (int ax = 0; ax < D; ++ax) hi[ax] = PER ? min(sr, a.grid.dims[ax] - sr - 1) : sr;
+    int o[3];
+    for (o[0] = -sr; o[0] <= hi[0]; ++o[0])
+      for (o[1] = -sr; o[1] <= hi[1]; ++o[1])
+        for (o[2] = (D == 3 ? -sr : 0); o[2] <= (D == 3 ? hi[2] : 0); ++o[2]) {
+          int h = 0;
+          bool ok = true;
+          for (int ax = 0; ax < D; ++ax) {
+            int cc = c[ax] + o[ax];
+            if (PER) {
+              cc = cc % a.grid.dims[ax];
+              if (cc < 0) cc += a.grid.dims[ax];
+            } else if (cc < 0 || cc >= a.grid.dims[ax]) {
+              ok = false;
+            }
+            h += cc * a.grid.strides[ax];
+          }
+          if (!ok) continue;
+          for (int k = a.cstart[h]; k < a.cend[h]; ++k) {
+            const float4 p = a.spos[k];
+            one(a.perm[k], v4s<D>(p), p.w, zero_vec<D>());
+          }
+        }
+  }
+  for (int c = 0; c < D * D; ++c) a.e_out[i * D * D + c] = W[c];
+}
+
+// Kinetic energy per particle (utils/thermal.py:83-122), float64.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_kinetic(StepArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const double m = a.mass[i];
+  double ke = 0.0;
+  double v2 = 0.0;
+  for (int c = 0; c < D; ++c) {
+    const double v = a.vel_in[D * i + c];
+    v2 += v * v;
+  }
+  ke = 0.5 * m * v2;
+  if constexpr (D == 3) {
+    // body-frame angular velocity in float64
+    const double qw = a.q[4 * i], qx = a.q[4 * i + 1], qy = a.q[4 * i + 2], qz = a.q[4 * i + 3];
+    const double n2 = qw * qw + ((qx * qx + qy * qy) + qz * qz);
+    const double inv = 1.0 / (n2 == 0.0 ? 1.0 : n2);
+    const double cw = qw * inv, cx = -qx * inv, cy = -qy * inv, cz = -qz * inv;
+    const double wx = a.ang_in[3 * i], wy = a.ang_in[3 * i + 1], wz = a.ang_in[3 * i + 2];
+    const double tx = 2.0 * (cy * wz - cz * wy), ty = 2.0 * (cz * wx - cx * wz), tz = 2.0 * (cx * wy - cy * wx);
+    const double bx = (wx + cw * tx) + (cy * tz - cz * ty);
+    const double by = (wy + cw * ty) + (cz * tx - cx * tz);
+    const double bz = (wz + cw * tz) + (cx * ty - cy * tx);
+    ke += 0.5 * ((double)a.inertia[3 * i] * bx * bx + (double)a.inertia[3 * i + 1] * by * by +
+                 (double)a.inertia[3 * i + 2] * bz * bz);
+  } else {
+    const double w = a.ang_in[i];
+    ke += 0.5 * (double)a.inertia[i] * w * w;
+  }
+  a.e_out[i] = ke;
+}
+
+// One stage of _step_once on its own (jdem_step_stage).
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_stage(StepArgs a, int stage) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  Vec<D> x = ld<D>(a.pos_out, i);
+  Vec<D> v = ld<D>(a.vel_out, i);
+  Ang<D> w = lda<D>(a.ang_out, i);
+  Quat q = ldq(a.q, i);
+  Vec<D> F = ld<D>(a.force, i);
+  Ang<D> T = lda<D>(a.torque, i);
+  const Ang<D> I = lda<D>(a.inertia, i);
+  const float m = a.mass[i];
+  const bool fixed = a.fixed[i] != 0;
+  const float freef = fixed ? 0.0f : 1.0f;
+  switch (stage) {
+    case JDEM_STAGE_DOMAIN_APPLY:
+      if (a.dom.kind == JDEM_DOMAIN_REFLECT) reflect_apply<D>(a.dom, x, v, q, F, m, a.rad[i], fixed, a.dt);
+      break;
+    case JDEM_STAGE_LIN_BEFORE:
+      if (a.lin == JDEM_LIN_VERLET) {
+        const float s = (a.dt * 0.5f) / m;
+        v = v + (F * s) * freef;
+        x = x + v * a.dt;
+      }
+      break;
+    case JDEM_STAGE_ROT_BEFORE:
+      if (a.rot == JDEM_ROT_VERLET_SPIRAL) vvs_before<D>(q, w, T, I, a.dt, freef);
+      break;
+    case JDEM_STAGE_FORCE_MANAGER: {
+      Vec<D> fp = a.ext_f ? ld<D>(a.ext_f, i) : zero_vec<D>();
+      Vec<D> fc = a.ext_fc ? ld<D>(a.ext_fc, i) : zero_vec<D>();
+      Vec<D> g;
+      if constexpr (D == 3)
+        g = Vec<3>{a.g[0], a.g[1], a.g[2]};
+      else
+        g = Vec<2>{a.g[0], a.g[1]};
+      fc = fc + g * (m / 1.0f);
+      F = (F + fp) + fc;
+      if (a.ext_t) T = T + lda<D>(a.ext_t, i);
+      break;
+    }
+    case JDEM_STAGE_LIN_AFTER:
+      if (a.lin == JDEM_LIN_VERLET) {
+        const float s = (a.dt * 0.5f) / m;
+        v = v + (F * s) * freef;
+      } else if (a.lin == JDEM_LIN_EULER) {
+        const float s = a.dt / m;
+        v = v + (F * s) * freef;
+        x = x + v * a.dt;
+      }
+      break;
+    case JDEM_STAGE_ROT_AFTER:
+      if (a.rot == JDEM_ROT_VERLET_SPIRAL)
+        vvs_after<D>(q, w, T, I, a.dt, freef);
+      else if (a.rot == JDEM_ROT_SPIRAL)
+        spiral_after<D>(q, w, T, I, a.dt, freef);
+      break;
+    default:
+      break;
+  }
+  st(a.pos_out, i, x);
+  st(a.vel_out, i, v);
+  sta(a.ang_out, i, w);
+  stq(a.q, i, q);
+  st(a.force, i, F);
+  sta(a.torque, i, T);
+}
+
+// ---------------------------------------------------------------------------
+// dispatch
+// ---------------------------------------------------------------------------
+template <template <int, int, int, bool> class K>
+struct Dispatch;
+
+#define JDEM_DISPATCH_BODY(KERNEL)                                                               \
+  template <int D, int LAW>                                                                      \
+  static hipError_t by_src(const StepArgs& a, int src, bool per, dim3 g, dim3 b, hipStream_t s) { \
+    switch (src) {                                                                               \
+      case SRC_NAIVE:                                                                            \
+        if (per)                                                                                 \
+          hipLaunchKernelGGL((KERNEL<D, LAW, SRC_NAIVE, true>), g, b, 0, s, a);                  \
+        else                                                                                     \
+          hipLaunchKernelGGL((KERNEL<D, LAW, SRC_NAIVE, false>), g, b, 0, s, a);                 \
+        break;                                                                                   \
+      case SRC_CELLS:                                                                            \
+        if (per)                                                                                 \
+          hipLaunchKernelGGL((KERNEL<D, LAW, SRC_CELLS, true>), g, b, 0, s, a);                  \
+        else                                                                                     \
+          hipLaunchKernelGGL((KERNEL<D, LAW, SRC_CELLS, false>), g, b, 0, s, a);                 \
+        break;                                                                                   \
+      case SRC_NLIST:                                                                            \
+        if (per)                                                                                 \
+          hipLaunchKernelGGL((KERNEL<D, LAW, SRC_NLIST, true>), g, b, 0, s, a);                  \
+        else                                                                                     \
+          hipLaunchKernelGGL((KERNEL<D, LAW, SRC_NLIST, false>), g, b, 0, s, a);                 \
+        break;                                                                                   \
+      default:                                                                                   \
+        return hipErrorInvalidValue;                                                             \
+    }                                                                                            \
+    return hipGetLastError();                                                                    \
+  }                                                                                              \
+  template <int D>                                                                               \
+  static hipError_t by_law(const StepArgs& a, int law, int src, bool per, dim3 g, dim3 b, hipStream_t s) { \
+    switch (law) {                                                                               \
+      case JDEM_LAW_SPRING:                                                                      \
+        return by_src<D, JDEM_LAW_SPRING>(a, src, per, g, b, s);                                 \
+      case JDEM_LAW_HERTZ:                                                                       \
+        return by_src<D, JDEM_LAW_HERTZ>(a, src, per, g, b, s);                                  \
+      case JDEM_LAW_CUNDALL_STRACK:                                                              \
+        return by_src<D, JDEM_LAW_CUNDALL_STRACK>(a, src, per, g, b, s);                         \
+      case JDEM_LAW_SPRING_FRICTION:                                                             \
+        return by_src<D, JDEM_LAW_SPRING_FRICTION>(a, src, per, g, b, s);                        \
+      case JDEM_LAW_HERTZ_FRICTION:                                                              \
+        return by_src<D, JDEM_LAW_HERTZ_FRICTION>(a, src, per, g, b, s);                         \
+      default:                                                                                   \
+        return hipErrorInvalidValue;                                                             \
+    }                                                                                            \
+  }                                                                                              \
+  static hipError_t run(const StepArgs& a, int dim, int law, int src, hipStream_t s) {           \
+    if (a.n == 0) return hipSuccess;                                                             \
+    dim3 g(nblocks(a.n)), b(kBlock);                                                             \
+    const bool per = a.dom.kind == JDEM_DOMAIN_PERIODIC;                                         \
+    return dim == 3 ? by_law<3>(a, law, src, per, g, b, s) : by_law<2>(a, law, src, per, g, b, s); \
+  }
+
+struct StepDispatch {
+  JDEM_DISPATCH_BODY(k_step)
+};
+struct EnergyDispatch {
+  JDEM_DISPATCH_BODY(k_energy)
+};
+struct VirialDispatch {
+  JDEM_DISPATCH_BODY(k_virial)
+};
+
+hipError_t launch_step(const StepArgs& a, int dim, int law, int src, hipStream_t s) {
+  return StepDispatch::run(a, dim, law, src, s);
+}
+hipError_t launch_energy(const StepArgs& a, int dim, int law, int src, hipStream_t s) {
+  return EnergyDispatch::run(a, dim, law, src, s);
+}
+hipError_t launch_virial(const StepArgs& a, int dim, int law, int src, hipStream_t s) {
+  return VirialDispatch::run(a, dim, law, src, s);
+}
+hipError_t launch_kinetic(const StepArgs& a, int dim, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  if (dim == 3)
+    hipLaunchKernelGGL(k_kinetic<3>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_kinetic<2>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_stage(const StepArgs& a, int dim, int stage, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  if (dim == 3)
+    hipLaunchKernelGGL(k_stage<3>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a, stage);
+  else
+    hipLaunchKernelGGL(k_stage<2>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a, stage);
+  return hipGetLastError();
+}
+
+}  // namespace jdem

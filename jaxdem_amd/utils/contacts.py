@@ -1,0 +1,25 @@
+"""Contact stress and pressure (jaxdem/utils/contacts.py:89-164).
+
+sigma = (1/V) sum_{i<j} r_ij (x) F_ij, P = tr(sigma) / dim, with the pair sum
+accumulated in float64 on the GPU.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def compute_contact_stress_tensor(state, system) -> np.ndarray:
+    from ..system import _contact_virial
+
+    W = _contact_virial(state, system)
+    V = float(np.prod(np.asarray(system.domain.box_size, np.float64)))
+    return W / V
+
+
+def compute_pressure(state, system) -> float:
+    s = compute_contact_stress_tensor(state, system)
+    return float(np.trace(s) / state.dim)
+
+
+__all__ = ["compute_contact_stress_tensor", "compute_pressure"]

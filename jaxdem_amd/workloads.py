@@ -1,0 +1,171 @@
+"""Seeded synthetic inputs of the DEM workloads, as host numpy arrays.
+
+Both the GPU path and any checker consume these same arrays.  numpy's PCG64
+stands in for JAX's PRNG.
+
+* ``spheres_1k``: 10x10x10 lattice, spacing 0.5, radius 0.1, unit density,
+  velocities N(0, 0.1^2) (seed 0), reflective 20^3 box at the origin,
+  linear spring k = 1e4, VV + VV-SPIRAL, dt = 1e-3, float32 (the README
+  example of JaxDEM, README.md:34-63).
+* ``spheres_1m_friction``: N = 2^20 bidisperse spheres (r = 0.5 / 0.7,
+  50:50 by seeded permutation) on the first N sites of a 102^3 lattice with
+  +/-5 % jitter, periodic cube at volume fraction 0.50, velocities N(0, 1)
+  with zero net momentum, unit density, spring k_n = 1e4 + tangential spring
+  (k_t = 2/7 k_n) with Coulomb mu = 0.5 and normal damping ratio 0.1,
+  NeighborList over a CellList with cell size 1.4 (the largest diameter),
+  VV + VV-SPIRAL, dt = t_c / 50 with t_c = pi sqrt(m_eff / k_n) of the
+  lightest pair, float32.
+"""
+
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+
+@dataclass
+class Workload:
+    name: str
+    pos: np.ndarray
+    vel: np.ndarray
+    rad: np.ndarray
+    mass: np.ndarray
+    box: np.ndarray
+    anchor: np.ndarray
+    domain: str
+    dt: float
+    law: str
+    young: float
+    mu: float = 0.0
+    damping: float = 0.0
+    kt_ratio: float = 2.0 / 7.0
+    collider: str = "naive"
+    cell_size: float | None = None
+    cutoff: float | None = None
+    skin: float | None = None
+    max_neighbors: int | None = None
+    extra: dict = field(default_factory=dict)
+
+    @property
+    def N(self) -> int:
+        return int(self.pos.shape[0])
+
+    @property
+    def dim(self) -> int:
+        return int(self.pos.shape[1])
+
+
+def sphere_volume(r: np.ndarray) -> np.ndarray:
+    return ((4.0 / 3.0) * math.pi * r.astype(np.float64) ** 3).astype(np.float32)
+
+
+def spheres_1k(seed: int = 0) -> Workload:
+    n = 10
+    sp = np.float32(0.5)
+    ax = np.arange(n, dtype=np.float32) * sp
+    pos = np.stack(np.meshgrid(ax, ax, ax, indexing="ij"), -1).reshape(-1, 3).astype(np.float32)
+    N = pos.shape[0]
+    rng = np.random.Generator(np.random.PCG64(seed))
+    vel = (0.1 * rng.standard_normal((N, 3))).astype(np.float32)
+    rad = np.full(N, 0.1, np.float32)
+    return Workload(
+        name="spheres_1k",
+        pos=pos,
+        vel=vel,
+        rad=rad,
+        mass=sphere_volume(rad),
+        box=np.full(3, 20.0, np.float32),
+        anchor=np.zeros(3, np.float32),
+        domain="reflect",
+        dt=1e-3,
+        law="spring",
+        young=1e4,
+        collider="naive",
+    )
+
+
+def spheres_friction(N: int = 1 << 20, seed: int = 0, phi: float = 0.50, skin: float = 0.1) -> Workload:
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rad = np.where(rng.permutation(N) < N // 2, 0.5, 0.7).astype(np.float32)
+    vbar = float(np.mean((4.0 / 3.0) * math.pi * rad.astype(np.float64) ** 3))
+    L = (N * vbar / phi) ** (1.0 / 3.0)
+    side = int(math.ceil(N ** (1.0 / 3.0)))
+    spacing = L / side
+    idx = np.arange(N)
+    ijk = np.stack([idx // (side * side), (idx // side) % side, idx % side], axis=1).astype(np.float64)
+    pos = (ijk + 0.5) * spacing
+    pos = pos + rng.uniform(-0.05 * spacing, 0.05 * spacing, pos.shape)
+    pos = pos.astype(np.float32)
+    vel = rng.standard_normal((N, 3))
+    vel = (vel - vel.mean(axis=0)).astype(np.float32)
+    mass = sphere_volume(rad)
+    k = 1e4
+    m_small = float(sphere_volume(np.array([0.5], np.float32))[0])
+    m_eff = 0.5 * m_small
+    t_c = math.pi * math.sqrt(m_eff / k)
+    return Workload(
+        name=f"spheres_{N}_friction",
+        pos=pos,
+        vel=vel,
+        rad=rad,
+        mass=mass,
+        box=np.full(3, L, np.float32),
+        anchor=np.zeros(3, np.float32),
+        domain="periodic",
+        dt=t_c / 50.0,
+        law="springfriction",
+        young=k,
+        mu=0.5,
+        damping=0.1,
+        collider="neighborlist",
+        cell_size=1.4,
+        cutoff=1.4,
+        skin=skin,
+        max_neighbors=None,
+        extra={"spacing": spacing, "L": L, "t_c": t_c},
+    )
+
+
+def make_gpu_system(w: Workload, device="cuda", max_neighbors: int | None = None, collider: str | None = None):
+    """(State, System) of the jaxdem_amd API for a workload."""
+    from . import Material, MaterialMatchmaker, MaterialTable, State, System
+
+    st = State.create(pos=w.pos, vel=w.vel, rad=w.rad, mass=w.mass, device=device)
+    col = collider or w.collider
+    if w.law in ("springfriction", "hertzfriction"):
+        mats = [Material.create("elasticfrict", density=1.0, young=w.young, poisson=0.3, mu=w.mu, e=1.0)]
+        fm_kw = {"kt_ratio": w.kt_ratio, "damping": w.damping}
+    else:
+        mats = [Material.create("elastic", density=1.0, young=w.young, poisson=0.3)]
+        fm_kw = {}
+    mt = MaterialTable.from_materials(mats, matcher=MaterialMatchmaker.create("harmonic"))
+    ckw: dict = {}
+    if col == "neighborlist":
+        ckw = dict(
+            state=st,
+            cutoff=w.cutoff,
+            skin=w.skin,
+            secondary_collider_type="CellList",
+            secondary_collider_kw=dict(state=st, cell_size=w.cell_size),
+        )
+        if max_neighbors is not None or w.max_neighbors is not None:
+            ckw["max_neighbors"] = max_neighbors or w.max_neighbors
+    elif col == "celllist":
+        ckw = dict(state=st) if w.cell_size is None else dict(state=st, cell_size=w.cell_size)
+    sysm = System.create(
+        st.shape,
+        dt=w.dt,
+        domain_type=w.domain,
+        domain_kw=dict(box_size=w.box, anchor=w.anchor),
+        collider_type={"naive": "naive", "celllist": "CellList", "neighborlist": "NeighborList"}[col],
+        collider_kw=ckw,
+        force_model_type=w.law,
+        force_model_kw=fm_kw,
+        mat_table=mt,
+    )
+    return st, sysm
+
+
+__all__ = ["Workload", "spheres_1k", "spheres_friction", "make_gpu_system", "sphere_volume"]

