@@ -1,0 +1,200 @@
+#!/usr/bin/env python3
+"""Throughput of the DEM time step on one MI355X.
+
+Workload (default): 2^20 bidisperse frictional spheres in a periodic box at
+volume fraction 0.50 -- spring k_n = 1e4, tangential spring with Coulomb
+mu = 0.5, normal damping ratio 0.1, NeighborList over a CellList of cell size
+1.4, velocity Verlet + VV-SPIRAL, float32, dt = t_c/50 (see
+jaxdem_amd/workloads.py).  One "step" is one DEM time step of the whole
+system; ``--steps`` of them are timed after ``--warmup`` untimed ones, inputs
+already resident in HBM.
+
+Prints one JSON line: particle-steps/s, the roofline of the fused step kernel
+(algorithmic bytes per launch / its mean HIP-event duration against the
+8 TB/s HBM3E peak), and the CPU baseline (the C++/OpenMP port of the oracle
+on the host cores, on a bounded sample of the same workload).
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "particle-steps/s at 1M spheres; p50/p99 rollout latency, batched vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def parse_args():
+    ap = argparse.ArgumentParser(description=__doc__)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=100)
+    ap.add_argument("--n", type=int, default=1 << 20)
+    ap.add_argument("--skin", type=float, default=0.1)
+    ap.add_argument("--seed", type=int, default=0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    return ap.parse_args()
+
+
+def live_contacts(sysm) -> int:
+    import torch
+
+    live = sysm.collider.history["live"].view(torch.int32)
+    # popcount of the 32-bit live words
+    x = live.to(torch.int64) & 0xFFFFFFFF
+    cnt = torch.zeros_like(x)
+    for b in range(32):
+        cnt += (x >> b) & 1
+    return int(cnt.sum().item())
+
+
+def traffic_from_profiles() -> tuple:
+    """HBM bytes per launch of the fused step kernel from the committed
+    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)."""
+    p = ROOT / "profiles" / "fused_step_spheres_1m_friction_pmc.txt"
+    if not p.exists():
+        return None, None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
+    return (float(m.group(1)), str(p.relative_to(ROOT))) if m else (None, None)
+
+
+def cpu_baseline(w, col, budget_s: float) -> dict:
+    import numpy as np
+
+    from oracle.port import PortRun
+
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
+    threads = max(1, min(threads, os.cpu_count() or threads))
+    run = PortRun(
+        w.pos, w.vel, w.rad, w.mass, w.box, w.anchor, domain=w.domain, dt=w.dt, law=w.law, kn=w.young,
+        kt=np.float32(w.kt_ratio) * np.float32(w.young), mu=w.mu, damping=w.damping, cutoff=col.cutoff,
+        skin=col.skin, M=col.max_neighbors, cell_size=col.secondary_collider.cell_size,
+        sr=col.secondary_collider.search_range,
+    )
+    run.steps(1, threads)  # first neighbour-list build, not timed
+    steps, secs = 0, 0.0
+    chunk = 2
+    while secs < budget_s and steps < 2000:
+        secs += run.steps(chunk, threads)
+        steps += chunk
+        chunk = min(chunk * 2, 64)
+    return {
+        "value": w.N * steps / secs,
+        "unit": "particle-steps/s",
+        "cores": threads,
+        "kind": "port",
+        "sample": f"{steps} steps of the same {w.N}-sphere workload after its first list build "
+        f"(C++/OpenMP port of the oracle, bit-identical to it; {secs:.1f} s)",
+    }
+
+
+def main():
+    args = parse_args()
+    if args.gpus != 1:
+        print(
+            f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build has no multi-GPU path; "
+            "see DESIGN.md for how the step would shard)",
+            file=sys.stderr,
+        )
+        sys.exit(2)
+    import torch
+
+    import jaxdem_amd as jd
+    from jaxdem_amd import _lib as L
+    from jaxdem_amd import workloads as W
+
+    dev = torch.device("cuda:0")
+    w = W.spheres_friction(N=args.n, seed=args.seed, skin=args.skin)
+    st, sy = W.make_gpu_system(w, device=dev)
+    print(f"[bench] N={w.N} dt={w.dt:.4e} M={sy.collider.max_neighbors} warmup={args.warmup}", file=sys.stderr, flush=True)
+    if args.warmup > 0:
+        st, sy = jd.System.step(st, sy, n=args.warmup)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    st1, sy1 = jd.System.step(st, sy, n=args.steps)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    secs = t1 - t0
+    value = w.N * args.steps / secs
+    stats = sy1.__dict__.get("_last_stats")
+    print(f"[bench] timed {args.steps} steps in {secs:.3f} s", file=sys.stderr, flush=True)
+
+    # roofline pass: the same steps with HIP events around every kernel
+    ctx = L.context_for(dev)
+    ctx.set_timing(True)
+    st2, sy2 = jd.System.step(st, sy, n=args.steps)
+    timing = ctx.get_timing()
+    ctx.set_timing(False)
+    torch.cuda.synchronize()
+    zh = live_contacts(sy2) / w.N
+    bytes_per_particle = 148.0 + 32.0 * zh
+    launches = max(timing["step_kernels"], 1)
+    avg_ms = timing["step_kernel_ms"] / launches
+    algo_bytes = w.N * bytes_per_particle
+    achieved = algo_bytes / (avg_ms * 1e-3) / 1e9
+    traffic, traffic_src = traffic_from_profiles()
+
+    out = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * secs / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {
+            "workload": f"spheres_{w.N}_friction: 2^20 bidisperse frictional spheres (r 0.5/0.7), periodic, phi 0.50",
+            "n_particles": w.N,
+            "collider": "NeighborList(CellList cell 1.4)",
+            "law": "springfriction (k_n 1e4, k_t 2/7 k_n, mu 0.5, damping ratio 0.1)",
+            "integrators": "verlet + verletspiral",
+            "dt": w.dt,
+            "skin": float(sy.collider.skin),
+            "max_neighbors": int(sy.collider.max_neighbors),
+        },
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": traffic,
+            "kernel": "k_step<3, SPRING_FRICTION, NLIST, periodic>",
+            "kernel_avg_us": 1e3 * avg_ms,
+            "algorithmic_bytes_per_particle": bytes_per_particle,
+            "mean_history_contacts": zh,
+        },
+        "breakdown_ms_per_step": {
+            "step_kernel": timing["step_kernel_ms"] / args.steps,
+            "idle_step_kernels": timing["idle_kernel_ms"] / args.steps,
+            "rebuilds": timing["rebuild_ms"] / args.steps,
+            "n_rebuilds": timing["rebuilds"],
+            "host_syncs": int(stats.host_syncs) if stats is not None else None,
+        },
+        "contact_evals_per_s": live_contacts(sy1) * args.steps / secs,
+        "traffic_source": traffic_src,
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(w, sy.collider, args.cpu_seconds)
+    else:
+        out["cpu_baseline"] = None
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

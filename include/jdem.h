@@ -194,6 +194,18 @@ typedef struct jdem_step_stats {
   double free_anchor[3];
 } jdem_step_stats;
 
+/* Kernel timing gathered with HIP events on the context's stream while
+ * timing is enabled (jdem_ctx_set_timing).  Step kernels that found a
+ * neighbour-list rebuild pending and did no work are counted apart. */
+typedef struct jdem_timing {
+  double step_kernel_ms;   /* fused step kernels that did work          */
+  int64_t step_kernels;
+  double idle_kernel_ms;   /* step kernels that returned early          */
+  int64_t idle_kernels;
+  double rebuild_ms;       /* neighbour-list / cell-list rebuilds       */
+  int64_t rebuilds;
+} jdem_timing;
+
 typedef struct jdem_ctx jdem_ctx;
 
 /* ---- library ---------------------------------------------------------- */
@@ -212,6 +224,12 @@ int jdem_ctx_set_stream(jdem_ctx* ctx, void* stream);
 const char* jdem_last_error(const jdem_ctx* ctx);
 /* Bytes of scratch the context holds now. */
 int jdem_ctx_scratch_bytes(const jdem_ctx* ctx, size_t* bytes);
+/* Enable (1) or disable (0) HIP-event timing of the kernels that
+ * jdem_system_step launches.  Has no counterpart in the reference (JaxDEM
+ * times whole jitted calls, benchmarks/run_benchmarks.py:27-51). */
+int jdem_ctx_set_timing(jdem_ctx* ctx, int enable);
+/* Synchronise the stream, return the accumulated timing and reset it. */
+int jdem_ctx_get_timing(jdem_ctx* ctx, jdem_timing* out);
 
 /* ---- (a) cell list ------------------------------------------------------ */
 

@@ -138,7 +138,19 @@ class JdemStepStats(ctypes.Structure):
     ]
 
 
+class JdemTiming(ctypes.Structure):
+    _fields_ = [
+        ("step_kernel_ms", c_double),
+        ("step_kernels", c_int64),
+        ("idle_kernel_ms", c_double),
+        ("idle_kernels", c_int64),
+        ("rebuild_ms", c_double),
+        ("rebuilds", c_int64),
+    ]
+
+
 STRUCTS = {
+    "jdem_timing": JdemTiming,
     "jdem_state": JdemState,
     "jdem_domain": JdemDomain,
     "jdem_law": JdemLaw,
@@ -157,6 +169,8 @@ SIGNATURES = {
     "jdem_ctx_set_stream": (c_int, [c_void_p, c_void_p]),
     "jdem_last_error": (ctypes.c_char_p, [c_void_p]),
     "jdem_ctx_scratch_bytes": (c_int, [c_void_p, POINTER(c_size_t)]),
+    "jdem_ctx_set_timing": (c_int, [c_void_p, c_int]),
+    "jdem_ctx_get_timing": (c_int, [c_void_p, POINTER(JdemTiming)]),
     "jdem_cell_build": (
         c_int,
         [c_void_p, POINTER(JdemState), POINTER(JdemDomain), POINTER(JdemGrid), c_void_p, c_void_p, c_void_p, c_void_p],
@@ -222,6 +236,14 @@ class Context:
         if stream != self.stream:
             self.check(self.lib.jdem_ctx_set_stream(self.ptr, c_void_p(stream)), "jdem_ctx_set_stream")
             self.stream = stream
+
+    def set_timing(self, on: bool) -> None:
+        self.check(self.lib.jdem_ctx_set_timing(self.ptr, 1 if on else 0), "jdem_ctx_set_timing")
+
+    def get_timing(self) -> dict:
+        t = JdemTiming()
+        self.check(self.lib.jdem_ctx_get_timing(self.ptr, ctypes.byref(t)), "jdem_ctx_get_timing")
+        return {name: getattr(t, name) for name, _ in JdemTiming._fields_}
 
     def scratch_bytes(self) -> int:
         n = c_size_t()

@@ -387,6 +387,37 @@ int nl_check_and_rebuild(jdem_ctx* ctx, jdem_system* sys, bool* rebuilt) {
   return JDEM_OK;
 }
 
+// ---- HIP-event timing ----------------------------------------------------
+hipEvent_t take_event(jdem_ctx* ctx) {
+  if (!ctx->event_pool.empty()) {
+    hipEvent_t e = ctx->event_pool.back();
+    ctx->event_pool.pop_back();
+    return e;
+  }
+  hipEvent_t e = nullptr;
+  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  return e;
+}
+
+// Returns the span index or -1 when timing is off.
+int span_begin(jdem_ctx* ctx, int kind, int64_t step) {
+  if (!ctx->timing) return -1;
+  jdem_ctx::TimedSpan s;
+  s.start = take_event(ctx);
+  s.stop = take_event(ctx);
+  s.kind = kind;
+  s.step = step;
+  if (!s.start || !s.stop) return -1;
+  (void)hipEventRecord(s.start, ctx->stream);
+  ctx->spans.push_back(s);
+  return (int)ctx->spans.size() - 1;
+}
+
+void span_end(jdem_ctx* ctx, int idx) {
+  if (idx < 0) return;
+  (void)hipEventRecord(ctx->spans[idx].stop, ctx->stream);
+}
+
 int src_of(int collider_kind) {
   switch (collider_kind) {
     case JDEM_COLLIDER_NAIVE:
@@ -449,6 +480,11 @@ int jdem_ctx_destroy(jdem_ctx* ctx) {
   for (Buf* b : all)
     if (b->p) (void)hipFree(b->p);
   if (ctx->host_flags) (void)hipHostFree(ctx->host_flags);
+  for (auto& s : ctx->spans) {
+    (void)hipEventDestroy(s.start);
+    (void)hipEventDestroy(s.stop);
+  }
+  for (hipEvent_t e : ctx->event_pool) (void)hipEventDestroy(e);
   delete ctx;
   return JDEM_OK;
 }
@@ -464,6 +500,36 @@ const char* jdem_last_error(const jdem_ctx* ctx) { return ctx ? ctx->err.c_str()
 int jdem_ctx_scratch_bytes(const jdem_ctx* ctx, size_t* bytes) {
   if (!ctx || !bytes) return JDEM_EINVAL;
   *bytes = ctx->scratch_bytes();
+  return JDEM_OK;
+}
+
+int jdem_ctx_set_timing(jdem_ctx* ctx, int enable) {
+  if (!ctx) return JDEM_EINVAL;
+  ctx->timing = enable != 0;
+  return JDEM_OK;
+}
+
+int jdem_ctx_get_timing(jdem_ctx* ctx, jdem_timing* out) {
+  if (!ctx || !out) return JDEM_EINVAL;
+  std::memset(out, 0, sizeof(*out));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (auto& s : ctx->spans) {
+    float ms = 0.0f;
+    HIPCHK(ctx, hipEventElapsedTime(&ms, s.start, s.stop));
+    if (s.kind == 0) {
+      out->step_kernel_ms += ms;
+      out->step_kernels++;
+    } else if (s.kind == 1) {
+      out->idle_kernel_ms += ms;
+      out->idle_kernels++;
+    } else {
+      out->rebuild_ms += ms;
+      out->rebuilds++;
+    }
+    ctx->event_pool.push_back(s.start);
+    ctx->event_pool.push_back(s.stop);
+  }
+  ctx->spans.clear();
   return JDEM_OK;
 }
 
@@ -714,8 +780,10 @@ int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       S->host_syncs++;
     }
     if (a.do_force && sys->collider_kind != JDEM_COLLIDER_NONE) {
+      const int sp = sys->collider_kind == JDEM_COLLIDER_CELL_LIST ? span_begin(ctx, 2, t) : -1;
       int rc2 = prepare_source(ctx, sys, a, P[r]);
       if (rc2) return rc2;
+      span_end(ctx, sp);
     }
     int src_eff = src;
     int law_eff = sys->law.kind;
@@ -731,7 +799,9 @@ int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       a.old_pos = sys->nlist.old_pos;
       a.trigger = nl_trigger(sys->nlist);
     }
+    const int sp = span_begin(ctx, 0, t);
     hipError_t e = launch_step(a, D, law_eff, src_eff, ctx->stream);
+    span_end(ctx, sp);
     if (e != hipSuccess) return fail(ctx, JDEM_EHIP, std::string("step kernel: ") + hipGetErrorString(e));
     return JDEM_OK;
   };
@@ -756,8 +826,10 @@ int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     bool rebuilt_here = false;
     while (next <= n) {
       if (pending) {
+        const int sp = span_begin(ctx, 2, next);
         rc = nl_rebuild(ctx, sys, P[rd(next)]);
         if (rc) return rc;
+        span_end(ctx, sp);
         HIPCHK(ctx, launch_fill_i32(flags, 2, 0, ctx->stream));
         S->rebuilds++;
         rebuilt_here = true;
@@ -767,6 +839,7 @@ int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
         if (interval > 0) chunk_len = std::max<int64_t>(2, interval - 1);
       }
       const int64_t cnt = std::min<int64_t>(chunk_len, n - next + 1);
+      const size_t chunk_spans = ctx->spans.size();
       for (int64_t t = next; t < next + cnt; ++t) {
         rc = launch_t(t);
         if (rc) return rc;
@@ -779,6 +852,11 @@ int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
         next += cnt;
         chunk_len = std::min<int64_t>(chunk_len * 2, 1024);
       } else {
+        // kernels v .. next+cnt-1 found the rebuild pending and did no work
+        for (size_t k = chunk_spans; k < ctx->spans.size(); ++k) {
+          auto& s = ctx->spans[k];
+          if (s.kind == 0 && s.step >= v) s.kind = 1;
+        }
         next = v;
         pending = true;
       }

@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <string>
+#include <vector>
 
 #include "jdem.h"
 #include "jdem_device.h"
@@ -45,6 +46,15 @@ struct jdem_ctx {
   jdem::Buf red;                       // reduction partials
   jdem::Buf nl_copy, xi_copy, live_copy;
   int32_t* host_flags = nullptr;       // pinned host mirror of flags
+  // HIP-event timing (jdem_ctx_set_timing)
+  struct TimedSpan {
+    hipEvent_t start, stop;
+    int kind;  // 0 step kernel, 1 idle step kernel, 2 rebuild
+    int64_t step;
+  };
+  bool timing = false;
+  std::vector<hipEvent_t> event_pool;
+  std::vector<TimedSpan> spans;
   size_t scratch_bytes() const;
 };
 

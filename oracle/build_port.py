@@ -1,0 +1,34 @@
+"""Build oracle/libjdem_port.so (TEST INFRASTRUCTURE: the CPU baseline port).
+
+g++ with OpenMP; FMA contraction off and no fast-math so that every float32
+operation rounds like the numpy oracle.
+"""
+
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+from pathlib import Path
+
+HERE = Path(__file__).resolve().parent
+SRC = HERE / "jdem_port.cpp"
+LIB = HERE / "libjdem_port.so"
+
+
+def build(verbose: bool = False) -> Path:
+    cxx = os.environ.get("CXX") or shutil.which("g++") or "g++"
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [cxx, "-O3", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
+           str(SRC), "-o", str(tmp)]
+    if verbose:
+        print(" ".join(cmd))
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed:\n{r.stdout}")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))
