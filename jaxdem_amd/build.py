@@ -20,7 +20,7 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 INCLUDE = ROOT / "include"
 LIB = PKG / "libjdem_hip.so"
-SOURCES = ["jdem_cells.hip", "jdem_step.hip", "jdem_abi.hip"]
+SOURCES = ["jdem_cells.hip", "jdem_step.hip", "jdem_resident.hip", "jdem_abi.hip"]
 ARCH = os.environ.get("JDEM_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [
@@ -67,7 +67,7 @@ def build(verbose: bool = False, jobs: int = 3) -> Path:
             _drain(procs)
     _drain(procs, wait=True)
     tmp = LIB.with_suffix(".so.tmp")
-    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)]
+    cmd = [cc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-Wl,--no-undefined", *map(str, objs), "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd), flush=True)
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)

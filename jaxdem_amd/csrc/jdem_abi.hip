@@ -418,6 +418,306 @@ void span_end(jdem_ctx* ctx, int idx) {
   (void)hipEventRecord(ctx->spans[idx].stop, ctx->stream);
 }
 
+// ---- resident NeighborList engine driver ---------------------------------
+// System.step with the NeighborList collider on the cell-sorted work set of
+// jdem_resident.hip; see that file for the layout.
+int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats* S) {
+  jdem_state& s = sys->state;
+  jdem_nlist& L = sys->nlist;
+  const int64_t N = s.n;
+  const int D = s.dim;
+  const int M = L.max_neighbors;
+  const int W64 = std::max(1, (M + 63) / 64);
+  const bool hist = L.hist_xi && L.hist_live;
+  const bool naive2 = L.secondary == JDEM_COLLIDER_NAIVE;
+  if (M > 256) return fail(ctx, JDEM_EUNSUPPORTED, "max_neighbors > 256 on the resident path");
+  if (!naive2 && sys->domain.kind == JDEM_DOMAIN_FREE)
+    return fail(ctx, JDEM_EUNSUPPORTED, "NeighborList over a CellList with a FreeDomain (per-step grid)");
+  const size_t n4 = 16 * (size_t)N, ni = 4 * (size_t)N;
+  for (int k = 0; k < 2; ++k) {
+    ENSURE(ctx, ctx->wP[k], n4);
+    ENSURE(ctx, ctx->wV[k], n4);
+    ENSURE(ctx, ctx->wW[k], n4);
+    ENSURE(ctx, ctx->wQ[k], n4);
+    ENSURE(ctx, ctx->wI[k], n4);
+    ENSURE(ctx, ctx->wOrig[k], ni);
+    ENSURE(ctx, ctx->wNl[k], 4 * (size_t)N * (size_t)std::max(M, 1));
+    ENSURE(ctx, ctx->wXi[k], hist ? 16 * (size_t)N * (size_t)std::max(M, 1) : 16);
+    ENSURE(ctx, ctx->wLive[k], 8 * (size_t)N * W64);
+  }
+  ENSURE(ctx, ctx->wF, n4);
+  ENSURE(ctx, ctx->wT, n4);
+  ENSURE(ctx, ctx->wOld, n4);
+  ENSURE(ctx, ctx->wWpos, ni);
+  ENSURE(ctx, ctx->wSrc, ni);
+  ENSURE(ctx, ctx->wKeys, ni);
+  ENSURE(ctx, ctx->keys[0], ni);
+  ENSURE(ctx, ctx->keys[1], ni);
+  ENSURE(ctx, ctx->vals[0], ni);
+  ENSURE(ctx, ctx->vals[1], ni);
+  ENSURE(ctx, ctx->hist, 4 * 256 * (size_t)((N + 4095) / 4096 + 1));
+  if (!naive2) {
+    ENSURE(ctx, ctx->cstart, 4 * (size_t)L.grid.n_cells);
+    ENSURE(ctx, ctx->cend, 4 * (size_t)L.grid.n_cells);
+  }
+  auto F4 = [](jdem::Buf& b) { return (float4*)b.p; };
+  auto I32 = [](jdem::Buf& b) { return (int32_t*)b.p; };
+  auto U64 = [](jdem::Buf& b) { return (uint64_t*)b.p; };
+  hipStream_t st = ctx->stream;
+  const DomP dom = dom_of(sys->domain);
+  const GridP g = grid_of(L.grid);
+  int cur = 0, set = 0, nlc = 0;
+
+  // --- entry: order the work set by (cell, original id), convert inputs ---
+  if (!naive2) {
+    HIPCHK(ctx, launch_hash(s.pos, N, D, dom, g, (uint32_t*)ctx->wKeys.p, st));
+    uint32_t* sk = nullptr;
+    int32_t* perm = nullptr;
+    HIPCHK(ctx, radix_sort_pairs(ctx, (uint32_t*)ctx->wKeys.p, N, nbits_for(g.n_cells), &sk, &perm, st));
+    HIPCHK(ctx, hipMemcpyAsync(ctx->wOrig[0].p, perm, ni, hipMemcpyDeviceToDevice, st));
+  } else {
+    HIPCHK(ctx, ws_launch_iota(I32(ctx->wOrig[0]), N, st));
+  }
+  HIPCHK(ctx, ws_launch_inverse(I32(ctx->wOrig[0]), N, I32(ctx->wWpos), st));
+  InOut io;
+  std::memset(&io, 0, sizeof(io));
+  io.n = N;
+  io.dim = D;
+  io.M = M;
+  io.W64 = W64;
+  io.pos = s.pos;
+  io.vel = s.vel;
+  io.ang = s.ang_vel;
+  io.q = s.q;
+  io.force = s.force;
+  io.torque = s.torque;
+  io.rad = s.rad;
+  io.mass = s.mass;
+  io.inertia = s.inertia;
+  io.mat_id = sys->law.n_mat > 1 ? s.mat_id : nullptr;
+  io.fixed = s.fixed;
+  io.nl_user = L.nl;
+  io.old_user = L.old_pos;
+  io.xi_user = hist ? L.hist_xi : nullptr;
+  io.live_user = hist ? L.hist_live : nullptr;
+  io.orig = I32(ctx->wOrig[0]);
+  io.wpos = I32(ctx->wWpos);
+  io.P4 = F4(ctx->wP[0]);
+  io.V4 = F4(ctx->wV[0]);
+  io.W4 = F4(ctx->wW[0]);
+  io.Q4 = F4(ctx->wQ[0]);
+  io.I4 = F4(ctx->wI[0]);
+  io.F4 = F4(ctx->wF);
+  io.T4 = F4(ctx->wT);
+  io.old4 = F4(ctx->wOld);
+  io.nl = I32(ctx->wNl[0]);
+  io.xi = F4(ctx->wXi[0]);
+  io.live = U64(ctx->wLive[0]);
+  HIPCHK(ctx, ws_launch_in(io, M > 0, st));
+
+  // --- rebuild at the current positions (neighbor_list.py:545-589) ---
+  auto rebuild = [&]() -> int {
+    const int sp = span_begin(ctx, 2, 0);
+    HIPCHK(ctx, launch_fill_i32((int32_t*)ctx->flags.p + 2, 1, 0, st));
+    int32_t* src = I32(ctx->wSrc);
+    uint32_t* sk = nullptr;
+    if (!naive2) {
+      HIPCHK(ctx, ws_launch_hash(F4(ctx->wP[cur]), I32(ctx->wOrig[set]), N, D, dom, g, (uint32_t*)ctx->wKeys.p,
+                                 I32(ctx->wWpos), st));
+      int32_t* new_orig = nullptr;
+      HIPCHK(ctx, radix_sort_pairs(ctx, (uint32_t*)ctx->wKeys.p, N, nbits_for(g.n_cells), &sk, &new_orig, st));
+      HIPCHK(ctx, ws_launch_permute(N, new_orig, I32(ctx->wWpos), F4(ctx->wP[cur]), F4(ctx->wV[cur]),
+                                    F4(ctx->wW[cur]), F4(ctx->wQ[set]), F4(ctx->wI[set]), F4(ctx->wP[1 - cur]),
+                                    F4(ctx->wV[1 - cur]), F4(ctx->wW[1 - cur]), F4(ctx->wQ[1 - set]),
+                                    F4(ctx->wI[1 - set]), I32(ctx->wOrig[1 - set]), src, st));
+      cur = 1 - cur;
+      set = 1 - set;
+      HIPCHK(ctx, launch_fill_i32(I32(ctx->cstart), g.n_cells, 0, st));
+      HIPCHK(ctx, launch_fill_i32(I32(ctx->cend), g.n_cells, 0, st));
+      HIPCHK(ctx, launch_cell_ranges(sk, N, I32(ctx->cstart), I32(ctx->cend), st));
+    } else {
+      HIPCHK(ctx, ws_launch_iota(src, N, st));
+    }
+    if (M > 0) {
+      NLW b;
+      std::memset(&b, 0, sizeof(b));
+      b.n = N;
+      b.M = M;
+      const float lc = L.cutoff + L.skin;
+      b.cutoff_sq = lc * lc;
+      b.isb = sys->interact_same_bond_id;
+      b.bond_w = s.bond_width;
+      b.bond_id = s.bond_id;
+      b.orig = I32(ctx->wOrig[set]);
+      b.P4 = F4(ctx->wP[cur]);
+      b.skeys = sk;
+      b.cstart = I32(ctx->cstart);
+      b.cend = I32(ctx->cend);
+      b.grid = g;
+      b.dom = dom;
+      b.nl = I32(ctx->wNl[1 - nlc]);
+      b.overflow = (int32_t*)ctx->flags.p + 2;
+      HIPCHK(ctx, ws_launch_nlist(b, D, naive2, st));
+      if (hist) {
+        if (L.n_build_times == 0) {
+          HIPCHK(ctx, ws_launch_zero64(U64(ctx->wLive[1 - nlc]), N * W64, st));
+        } else {
+          HIPCHK(ctx, ws_launch_remap(N, M, W64, src, I32(ctx->wNl[nlc]), F4(ctx->wXi[nlc]), U64(ctx->wLive[nlc]),
+                                      I32(ctx->wNl[1 - nlc]), F4(ctx->wXi[1 - nlc]), U64(ctx->wLive[1 - nlc]), st));
+        }
+      }
+      nlc = 1 - nlc;
+    }
+    HIPCHK(ctx, ws_launch_copy4(F4(ctx->wOld), F4(ctx->wP[cur]), N, st));
+    L.n_build_times += 1;
+    span_end(ctx, sp);
+    return JDEM_OK;
+  };
+
+  int32_t* flags = (int32_t*)ctx->flags.p;
+  HIPCHK(ctx, launch_fill_i32(flags, 4, 0, st));
+  float fbox[3] = {0, 0, 0}, fanc[3] = {0, 0, 0};
+  bool free_done = false;
+  std::vector<int> cur_at((size_t)n + 2, 0);
+
+  auto launch_t = [&](int64_t t) -> int {
+    if (sys->domain.kind == JDEM_DOMAIN_FREE && t == n - 1 && !free_done) {
+      const int nparts = 64;
+      ENSURE(ctx, ctx->red, 4 * 6 * (size_t)nparts);
+      HIPCHK(ctx, ws_launch_box(F4(ctx->wP[cur]), N, (float*)ctx->red.p, nparts, st));
+      float hp[6 * 64];
+      HIPCHK(ctx, hipMemcpyAsync(hp, ctx->red.p, sizeof(hp), hipMemcpyDeviceToHost, st));
+      HIPCHK(ctx, hipStreamSynchronize(st));
+      for (int a = 0; a < D; ++a) {
+        float lo = INFINITY, hi = -INFINITY;
+        for (int b = 0; b < nparts; ++b) {
+          lo = std::min(lo, hp[6 * b + a]);
+          hi = std::max(hi, hp[6 * b + 3 + a]);
+        }
+        fbox[a] = hi - lo;
+        fanc[a] = lo;
+      }
+      free_done = true;
+      S->host_syncs++;
+    }
+    WS a;
+    std::memset(&a, 0, sizeof(a));
+    a.n = N;
+    a.M = M;
+    a.W64 = W64;
+    a.law = law_of(sys->law);
+    a.dom = dom;
+    a.dt = sys->dt;
+    a.lin = sys->lin_kind;
+    a.rot = sys->rot_kind;
+    for (int c = 0; c < 3; ++c) a.g[c] = sys->gravity[c];
+    a.ext_f = sys->ext_force;
+    a.ext_fc = sys->ext_force_com;
+    a.ext_t = sys->ext_torque;
+    a.has_fm = (a.g[0] != 0.0f || a.g[1] != 0.0f || a.g[2] != 0.0f || a.ext_f || a.ext_fc || a.ext_t) ? 1 : 0;
+    a.orig = I32(ctx->wOrig[set]);
+    cur_at[t] = cur;
+    a.P = F4(ctx->wP[cur]);
+    a.V = F4(ctx->wV[cur]);
+    a.W = F4(ctx->wW[cur]);
+    a.P2 = F4(ctx->wP[1 - cur]);
+    a.V2 = F4(ctx->wV[1 - cur]);
+    a.W2 = F4(ctx->wW[1 - cur]);
+    a.Q = F4(ctx->wQ[set]);
+    a.I = F4(ctx->wI[set]);
+    a.F = F4(ctx->wF);
+    a.T = F4(ctx->wT);
+    a.old4 = F4(ctx->wOld);
+    a.nl = I32(ctx->wNl[nlc]);
+    a.xi = F4(ctx->wXi[nlc]);
+    a.live = U64(ctx->wLive[nlc]);
+    a.do_force = t >= 1;
+    a.do_after = t >= 1;
+    a.do_before = t < n;
+    a.store_ft = t == n;
+    a.trigger = nl_trigger(L);
+    a.check_disp = a.do_before;
+    a.flag_rd = t >= 1 ? flags + (t & 1) : nullptr;
+    a.flag_wr = flags + ((t + 1) & 1);
+    a.step_idx = (int32_t)t;
+    const int sp = span_begin(ctx, 0, t);
+    hipError_t e = ws_launch_step(a, D, sys->law.kind, st);
+    span_end(ctx, sp);
+    if (e != hipSuccess) return fail(ctx, JDEM_EHIP, std::string("resident step kernel: ") + hipGetErrorString(e));
+    cur = 1 - cur;
+    return JDEM_OK;
+  };
+
+  int rc = launch_t(0);
+  if (rc) return rc;
+  bool pending = L.n_build_times == 0;
+  int64_t next = 1;
+  int64_t chunk_len = 8;
+  int64_t last_rebuild_at = 0;
+  bool rebuilt_here = false;
+  while (next <= n) {
+    if (pending) {
+      rc = rebuild();
+      if (rc) return rc;
+      HIPCHK(ctx, launch_fill_i32(flags, 2, 0, st));
+      S->rebuilds++;
+      rebuilt_here = true;
+      if (last_rebuild_at > 0) chunk_len = std::max<int64_t>(2, next - last_rebuild_at + 1);
+      last_rebuild_at = next;
+      pending = false;
+    }
+    const int64_t cnt = std::min<int64_t>(chunk_len, n - next + 1);
+    const size_t chunk_spans = ctx->spans.size();
+    for (int64_t t = next; t < next + cnt; ++t) {
+      rc = launch_t(t);
+      if (rc) return rc;
+    }
+    rc = read_flags(ctx, 3);
+    if (rc) return rc;
+    S->host_syncs++;
+    const int32_t v = std::max(ctx->host_flags[0], ctx->host_flags[1]);
+    if (v == 0) {
+      next += cnt;
+      chunk_len = std::min<int64_t>(chunk_len * 2, 1024);
+    } else {
+      for (size_t k = chunk_spans; k < ctx->spans.size(); ++k) {
+        auto& sp = ctx->spans[k];
+        if (sp.kind == 0 && sp.step >= v) sp.kind = 1;
+      }
+      // kernels v.. of this chunk did no work: the data is where kernel v read
+      // it.  When the chunk's last kernel raised the flag (v == next + cnt),
+      // kernel v was never launched and `cur` already points at the data.
+      if (v < next + cnt) cur = cur_at[v];
+      next = v;
+      pending = true;
+    }
+  }
+  if (rebuilt_here) L.overflow = ctx->host_flags[2] != 0;
+  S->overflow = L.overflow;
+  // --- exit: back to the caller's order and layouts ---
+  io.orig = I32(ctx->wOrig[set]);
+  io.P4 = F4(ctx->wP[cur]);
+  io.V4 = F4(ctx->wV[cur]);
+  io.W4 = F4(ctx->wW[cur]);
+  io.Q4 = F4(ctx->wQ[set]);
+  io.I4 = F4(ctx->wI[set]);
+  io.nl = I32(ctx->wNl[nlc]);
+  io.xi = F4(ctx->wXi[nlc]);
+  io.live = U64(ctx->wLive[nlc]);
+  HIPCHK(ctx, ws_launch_out(io, M > 0, st));
+  if (free_done) {
+    for (int a = 0; a < D; ++a) {
+      sys->domain.box_size[a] = fbox[a];
+      sys->domain.anchor[a] = fanc[a];
+      sys->domain.inv_box_size[a] = 1.0f / fbox[a];
+      S->free_box[a] = fbox[a];
+      S->free_anchor[a] = fanc[a];
+    }
+  }
+  S->steps = n;
+  return JDEM_OK;
+}
+
 int src_of(int collider_kind) {
   switch (collider_kind) {
     case JDEM_COLLIDER_NAIVE:
@@ -741,6 +1041,7 @@ int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     S->steps = n;
     return JDEM_OK;
   }
+  if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST) return resident_nl_step(ctx, sys, n, S);
   ENSURE(ctx, ctx->pos2, 4 * (size_t)N * D);
   ENSURE(ctx, ctx->vel2, 4 * (size_t)N * D);
   ENSURE(ctx, ctx->ang2, 4 * (size_t)N * A);

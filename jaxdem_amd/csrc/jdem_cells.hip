@@ -88,7 +88,98 @@ __global__ void __launch_bounds__(kBlock) k_radix_hist(const uint32_t* __restric
   hist[(int64_t)tid * ntiles + blockIdx.x] = bins[tid];
 }
 
-// Exclusive scan of `total` uint32 values in one block of 1024 threads.
+// Parallel exclusive scan of `total` uint32 values in three launches:
+// per-chunk totals (coalesced), a one-block scan of the totals, then a
+// per-chunk scan in LDS plus the chunk offset.
+constexpr int SC_ITEMS = 16;
+constexpr int SC_CHUNK = kBlock * SC_ITEMS;
+
+__global__ void __launch_bounds__(kBlock) k_scan_totals(const uint32_t* __restrict__ data, int64_t total,
+                                                         uint32_t* __restrict__ sums) {
+  __shared__ uint32_t sh[kBlock];
+  const int tid = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * SC_CHUNK;
+  uint32_t s = 0;
+#pragma unroll
+  for (int it = 0; it < SC_ITEMS; ++it) {
+    const int64_t k = base + (int64_t)it * kBlock + tid;
+    if (k < total) s += data[k];
+  }
+  sh[tid] = s;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (tid < off) sh[tid] += sh[tid + off];
+    __syncthreads();
+  }
+  if (tid == 0) sums[blockIdx.x] = sh[0];
+}
+
+// exclusive scan of nsums (<= 1024 * 1024) values in one block
+__global__ void __launch_bounds__(1024) k_scan_sums(uint32_t* __restrict__ sums, int nsums) {
+  __shared__ uint32_t part[1024];
+  const int tid = threadIdx.x;
+  const int per = (nsums + 1023) / 1024;
+  const int b = tid * per;
+  const int e = min(b + per, nsums);
+  uint32_t s = 0;
+  for (int k = b; k < e; ++k) s += sums[k];
+  part[tid] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = (tid >= off) ? part[tid - off] : 0u;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  uint32_t run = (tid == 0) ? 0u : part[tid - 1];
+  for (int k = b; k < e; ++k) {
+    const uint32_t v = sums[k];
+    sums[k] = run;
+    run += v;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_apply(uint32_t* __restrict__ data, int64_t total,
+                                                        const uint32_t* __restrict__ sums) {
+  __shared__ uint32_t tile[SC_CHUNK];
+  __shared__ uint32_t tsum[kBlock];
+  const int tid = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * SC_CHUNK;
+#pragma unroll
+  for (int it = 0; it < SC_ITEMS; ++it) {
+    const int64_t k = base + (int64_t)it * kBlock + tid;
+    tile[it * kBlock + tid] = (k < total) ? data[k] : 0u;
+  }
+  __syncthreads();
+  // thread t scans its contiguous run tile[t*SC_ITEMS .. +SC_ITEMS)
+  uint32_t s = 0;
+#pragma unroll
+  for (int it = 0; it < SC_ITEMS; ++it) s += tile[tid * SC_ITEMS + it];
+  tsum[tid] = s;
+  __syncthreads();
+  for (int off = 1; off < kBlock; off <<= 1) {
+    const uint32_t v = (tid >= off) ? tsum[tid - off] : 0u;
+    __syncthreads();
+    tsum[tid] += v;
+    __syncthreads();
+  }
+  uint32_t run = sums[blockIdx.x] + ((tid == 0) ? 0u : tsum[tid - 1]);
+#pragma unroll
+  for (int it = 0; it < SC_ITEMS; ++it) {
+    const uint32_t v = tile[tid * SC_ITEMS + it];
+    tile[tid * SC_ITEMS + it] = run;
+    run += v;
+  }
+  __syncthreads();
+#pragma unroll
+  for (int it = 0; it < SC_ITEMS; ++it) {
+    const int64_t k = base + (int64_t)it * kBlock + tid;
+    if (k < total) data[k] = tile[it * kBlock + tid];
+  }
+}
+
+// Exclusive scan of `total` uint32 values in one block of 1024 threads
+// (kept for totals small enough that one launch is cheaper).
 __global__ void __launch_bounds__(1024) k_scan_exclusive(uint32_t* __restrict__ data, int64_t total) {
   __shared__ uint32_t part[1024];
   const int tid = threadIdx.x;
@@ -167,11 +258,12 @@ hipError_t radix_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, i
                             int32_t** vals_out, hipStream_t s) {
   const int ntiles = (int)((n + RS_TILE - 1) / RS_TILE);
   const int64_t total = (int64_t)ntiles * 256;
+  const int nchunks = (int)((total + SC_CHUNK - 1) / SC_CHUNK);
   if (!ensure(ctx, ctx->keys[0], sizeof(uint32_t) * (size_t)(n > 0 ? n : 1)) ||
       !ensure(ctx, ctx->keys[1], sizeof(uint32_t) * (size_t)(n > 0 ? n : 1)) ||
       !ensure(ctx, ctx->vals[0], sizeof(int32_t) * (size_t)(n > 0 ? n : 1)) ||
       !ensure(ctx, ctx->vals[1], sizeof(int32_t) * (size_t)(n > 0 ? n : 1)) ||
-      !ensure(ctx, ctx->hist, sizeof(uint32_t) * (size_t)(total > 0 ? total : 1)))
+      !ensure(ctx, ctx->hist, sizeof(uint32_t) * (size_t)(total > 0 ? total : 1) + sizeof(uint32_t) * (nchunks + 1)))
     return hipErrorOutOfMemory;
   uint32_t* kb[2] = {(uint32_t*)ctx->keys[0].p, (uint32_t*)ctx->keys[1].p};
   int32_t* vb[2] = {(int32_t*)ctx->vals[0].p, (int32_t*)ctx->vals[1].p};
@@ -185,7 +277,14 @@ hipError_t radix_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, i
     const int shift = 8 * p;
     if (n > 0) {
       hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, s, kcur, n, shift, hist, ntiles);
-      hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, hist, total);
+      if (nchunks <= 1) {
+        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, hist, total);
+      } else {
+        uint32_t* sums = hist + total;
+        hipLaunchKernelGGL(k_scan_totals, dim3(nchunks), dim3(kBlock), 0, s, hist, total, sums);
+        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, nchunks);
+        hipLaunchKernelGGL(k_scan_apply, dim3(nchunks), dim3(kBlock), 0, s, hist, total, sums);
+      }
       hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, s, kcur, vcur, kb[out], vb[out], n, shift,
                          hist, ntiles);
     }

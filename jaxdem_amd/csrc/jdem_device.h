@@ -531,46 +531,61 @@ JD float toc_alpha(float v, float acc, float delta, float ws, float dt) {
   return fminf(fmaxf(alpha, 0.0f), 1.0f);
 }
 
+JD void to_arr(Vec<3> v, float* a) {
+  a[0] = v.x;
+  a[1] = v.y;
+  a[2] = v.z;
+}
+JD void to_arr(Vec<2> v, float* a) {
+  a[0] = v.x;
+  a[1] = v.y;
+}
+JD void from_arr(Vec<3>& v, const float* a) { v = {a[0], a[1], a[2]}; }
+JD void from_arr(Vec<2>& v, const float* a) { v = {a[0], a[1]}; }
+
+// Register arrays with fully unrolled component loops (no stack traffic).
 template <int D>
 JD void reflect_apply(const DomP& dom, Vec<D>& x, Vec<D>& v, Quat& q, Vec<D> F, float m, float rad, bool fixed,
                       float dt) {
-  const float* xs = &x.x;
-  float* vs = &v.x;
-  float* ps = &x.x;
-  const float* fs = &F.x;
-  float e = dom.restitution;
-  float inv_mass = 1.0f / m;
+  float xs[D], vs[D], fs[D];
+  to_arr(x, xs);
+  to_arr(v, vs);
+  to_arr(F, fs);
+  const float e = dom.restitution;
+  const float inv_mass = 1.0f / m;
   float denom = inv_mass + 0.0f;
   denom = (denom == 0.0f) ? 1.0f : denom;
-  float ws[D], act[D], alpha[D], acc[D];
+  float ws[D], act[D], acc[D];
   float a_min = 1.0f;
+#pragma unroll
   for (int c = 0; c < D; ++c) {
-    float lo = dom.anchor[c] + rad;
-    float hi = (dom.anchor[c] + dom.box[c]) - rad;
-    float over_lo = fmaxf(0.0f, lo - xs[c]);
-    float over_hi = fmaxf(0.0f, xs[c] - hi);
-    float delta = fmaxf(over_lo, over_hi);
+    const float lo = dom.anchor[c] + rad;
+    const float hi = (dom.anchor[c] + dom.box[c]) - rad;
+    const float over_lo = fmaxf(0.0f, lo - xs[c]);
+    const float over_hi = fmaxf(0.0f, xs[c] - hi);
+    const float delta = fmaxf(over_lo, over_hi);
     ws[c] = ((over_lo > 0.0f) ? 1.0f : 0.0f) - ((over_hi > 0.0f) ? 1.0f : 0.0f);
     act[c] = fabsf(ws[c]);
     acc[c] = fs[c] * inv_mass;
-    alpha[c] = toc_alpha(vs[c], acc[c], delta, ws[c], dt);
-    if (act[c] > 0.0f) a_min = fminf(a_min, alpha[c]);
+    const float alpha = toc_alpha(vs[c], acc[c], delta, ws[c], dt);
+    if (act[c] > 0.0f) a_min = fminf(a_min, alpha);
   }
-  float dt_factor = (a_min - 1.0f) * dt;
-  float dt_rem = (1.0f - a_min) * dt;
-  float dvs[D];
+  const float dt_factor = (a_min - 1.0f) * dt;
+  const float dt_rem = (1.0f - a_min) * dt;
+#pragma unroll
   for (int c = 0; c < D; ++c) {
-    float v_contact = vs[c] + dt_factor * acc[c];
+    const float v_contact = vs[c] + dt_factor * acc[c];
     float j = ((-(1.0f + e)) * v_contact) / denom;
-    float closing = ((v_contact * ws[c]) < 0.0f) ? 1.0f : 0.0f;
-    float weight = act[c] / fmaxf(act[c], 1.0f);
+    const float closing = ((v_contact * ws[c]) < 0.0f) ? 1.0f : 0.0f;
+    const float weight = act[c] / fmaxf(act[c], 1.0f);
     j = j * (weight * closing);
-    float dv = j * inv_mass;
-    dvs[c] = fixed ? 0.0f : dv;
+    const float dv = fixed ? 0.0f : j * inv_mass;
+    vs[c] = vs[c] + dv;
+    xs[c] = xs[c] + dv * dt_rem;
   }
-  for (int c = 0; c < D; ++c) vs[c] = vs[c] + dvs[c];
+  from_arr(v, vs);
+  from_arr(x, xs);
   q = q_unit(q);
-  for (int c = 0; c < D; ++c) ps[c] = ps[c] + dvs[c] * dt_rem;
 }
 
 }  // namespace jdem

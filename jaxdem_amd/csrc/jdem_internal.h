@@ -45,6 +45,8 @@ struct jdem_ctx {
   jdem::Buf flags;                     // small device words
   jdem::Buf red;                       // reduction partials
   jdem::Buf nl_copy, xi_copy, live_copy;
+  // resident NeighborList working set (jdem_resident.hip)
+  jdem::Buf wP[2], wV[2], wW[2], wQ[2], wI[2], wF, wT, wOld, wOrig[2], wWpos, wSrc, wKeys, wNl[2], wXi[2], wLive[2];
   int32_t* host_flags = nullptr;       // pinned host mirror of flags
   // HIP-event timing (jdem_ctx_set_timing)
   struct TimedSpan {
@@ -171,6 +173,118 @@ hipError_t launch_sum_f64(const double* in, int64_t n, int stride, int ncomp, do
 hipError_t launch_kinetic(const StepArgs& a, int dim, hipStream_t s);
 hipError_t launch_stage(const StepArgs& a, int dim, int stage, hipStream_t s);
 hipError_t launch_copy_f32(float* dst, const float* src, int64_t n, hipStream_t s);
+
+
+// ---- resident NeighborList engine (jdem_resident.hip) ----------------------
+struct InOut {
+  int64_t n;
+  int dim;
+  int M;
+  int W64;  // live words per particle
+  // caller arrays (original order)
+  float* pos;
+  float* vel;
+  float* ang;
+  float* q;
+  float* force;
+  float* torque;
+  const float* rad;
+  const float* mass;
+  const float* inertia;
+  const int32_t* mat_id;
+  const uint8_t* fixed;
+  int32_t* nl_user;      // (N, M)
+  float* old_user;       // (N, D)
+  float* xi_user;        // (N, M, D) or null
+  uint32_t* live_user;   // (N, ceil(M/32)) or null
+  // work arrays
+  const int32_t* orig;  // work -> original
+  const int32_t* wpos;  // original -> work
+  float4* P4;
+  float4* V4;
+  float4* W4;
+  float4* Q4;
+  float4* I4;
+  float4* F4;
+  float4* T4;
+  float4* old4;
+  int32_t* nl;
+  float4* xi;
+  uint64_t* live;
+};
+
+struct NLW {
+  int64_t n;
+  int M;
+  float cutoff_sq;
+  int isb;
+  int bond_w;
+  const int32_t* bond_id;  // original indexing
+  const int32_t* orig;
+  const float4* P4;
+  const uint32_t* skeys;  // sorted keys = cell of work slot
+  const int32_t* cstart;
+  const int32_t* cend;
+  GridP grid;
+  DomP dom;
+  int32_t* nl;
+  int32_t* overflow;
+};
+
+struct WS {
+  int64_t n;
+  int M;
+  int W64;
+  LawP law;
+  DomP dom;
+  float dt;
+  int lin, rot;
+  float g[3];
+  int has_fm;
+  const float* ext_f;   // original order
+  const float* ext_fc;
+  const float* ext_t;
+  const int32_t* orig;
+  const float4* P;
+  const float4* V;
+  const float4* W;
+  float4* P2;
+  float4* V2;
+  float4* W2;
+  float4* Q;
+  const float4* I;
+  float4* F;
+  float4* T;
+  const float4* old4;
+  const int32_t* nl;
+  float4* xi;
+  uint64_t* live;
+  int do_force, do_after, do_before, store_ft;
+  float trigger;
+  int check_disp;
+  const int32_t* flag_rd;
+  int32_t* flag_wr;
+  int32_t step_idx;
+};
+
+hipError_t ws_launch_in(const InOut& io, bool with_nl, hipStream_t s);
+hipError_t ws_launch_out(const InOut& io, bool with_nl, hipStream_t s);
+hipError_t ws_launch_inverse(const int32_t* orig, int64_t n, int32_t* wpos, hipStream_t s);
+hipError_t ws_launch_iota(int32_t* p, int64_t n, hipStream_t s);
+hipError_t ws_launch_zero64(uint64_t* p, int64_t n, hipStream_t s);
+hipError_t ws_launch_copy4(float4* d, const float4* src, int64_t n, hipStream_t s);
+hipError_t ws_launch_hash(const float4* P4, const int32_t* orig, int64_t n, int dim, const DomP& dom, const GridP& g,
+                          uint32_t* keys, int32_t* wpos, hipStream_t s);
+hipError_t ws_launch_permute(int64_t n, const int32_t* new_orig, const int32_t* wpos, const float4* P,
+                             const float4* V, const float4* W, const float4* Q, const float4* I, float4* P2,
+                             float4* V2, float4* W2, float4* Q2, float4* I2, int32_t* orig2, int32_t* src,
+                             hipStream_t s);
+hipError_t ws_launch_nlist(const NLW& a, int dim, bool naive, hipStream_t s);
+hipError_t ws_launch_remap(int64_t n, int M, int W64, const int32_t* src, const int32_t* old_nl, const float4* old_xi,
+                           const uint64_t* old_live, const int32_t* new_nl, float4* new_xi, uint64_t* new_live,
+                           hipStream_t s);
+hipError_t ws_launch_step(const WS& a, int dim, int law, hipStream_t s);
+hipError_t ws_launch_box(const float4* P, int64_t n, float* part, int nparts, hipStream_t s);
 
 inline unsigned nblocks(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
 

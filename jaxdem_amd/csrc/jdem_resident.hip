@@ -1,0 +1,674 @@
+// jdem_resident.hip -- the NeighborList time step on a resident, cell-sorted
+// working set (the hot path of System.step with the NeighborList collider,
+// system.py:60-82 + neighbor_list.py:58-131, 594-669).
+//
+// Layout in HBM for the duration of one jdem_system_step call:
+//   P4[b] = (x, y, z, r)   V4[b] = (v, m)   W4[b] = (omega)   Q4[b] = q
+//   I4[b] = (inertia, bits: mat_id << 1 | fixed)
+// in "work" order b: particles sorted by (cell, original id) at every list
+// rebuild, so a particle's candidates are its memory neighbours.  orig[b] is
+// the original index of work slot b.  The neighbour list is slot-major
+// nl[s*N + b] (work indices), the tangential springs xi[s*N + b] float4, and
+// one 64-bit live mask per particle and 64 slots: every per-slot access of a
+// wavefront is contiguous.  Sorting by (cell, original id) makes each row the
+// reference's row (stencil order, ascending original id within a cell), so
+// sums and history match the CPU oracle bit for bit.
+//
+// The caller's arrays (original order, JaxDEM layouts) are read once on entry
+// and written once on exit.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+
+#include "jdem_internal.h"
+
+namespace jdem {
+
+// ---------------------------------------------------------------------------
+// float4 helpers
+// ---------------------------------------------------------------------------
+template <int D>
+JD Vec<D> xyz(float4 a);
+template <>
+JD Vec<3> xyz<3>(float4 a) { return {a.x, a.y, a.z}; }
+template <>
+JD Vec<2> xyz<2>(float4 a) { return {a.x, a.y}; }
+JD float4 f4(Vec<3> v, float w) { return make_float4(v.x, v.y, v.z, w); }
+JD float4 f4(Vec<2> v, float w) { return make_float4(v.x, v.y, 0.0f, w); }
+template <int D>
+JD Ang<D> ang4(float4 a);
+template <>
+JD Vec<3> ang4<3>(float4 a) { return {a.x, a.y, a.z}; }
+template <>
+JD Ang1 ang4<2>(float4 a) { return {a.x}; }
+JD float4 f4a(Vec<3> v) { return make_float4(v.x, v.y, v.z, 0.0f); }
+JD float4 f4a(Ang1 v) { return make_float4(v.z, 0.0f, 0.0f, 0.0f); }
+JD Quat q4(float4 a) { return {a.x, a.y, a.z, a.w}; }
+JD float4 f4q(Quat q) { return make_float4(q.w, q.x, q.y, q.z); }
+
+// ---------------------------------------------------------------------------
+// entry / exit conversions
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(kBlock) k_in_state(InOut a) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.n) return;
+  const int64_t i = a.orig[b];
+  const int D = a.dim;
+  const int A = D == 3 ? 3 : 1;
+  float p[3] = {0, 0, 0}, v[3] = {0, 0, 0}, w[3] = {0, 0, 0}, f[3] = {0, 0, 0}, t[3] = {0, 0, 0}, I[3] = {0, 0, 0},
+        o[3] = {0, 0, 0};
+  for (int c = 0; c < D; ++c) {
+    p[c] = a.pos[D * i + c];
+    v[c] = a.vel[D * i + c];
+    f[c] = a.force[D * i + c];
+    o[c] = a.old_user ? a.old_user[D * i + c] : 0.0f;
+  }
+  for (int c = 0; c < A; ++c) {
+    w[c] = a.ang[A * i + c];
+    t[c] = a.torque[A * i + c];
+    I[c] = a.inertia[A * i + c];
+  }
+  const int bits = ((a.mat_id ? a.mat_id[i] : 0) << 1) | (a.fixed[i] ? 1 : 0);
+  a.P4[b] = make_float4(p[0], p[1], p[2], a.rad[i]);
+  a.V4[b] = make_float4(v[0], v[1], v[2], a.mass[i]);
+  a.W4[b] = make_float4(w[0], w[1], w[2], 0.0f);
+  a.Q4[b] = reinterpret_cast<const float4*>(a.q)[i];
+  a.I4[b] = make_float4(I[0], I[1], I[2], __int_as_float(bits));
+  a.F4[b] = make_float4(f[0], f[1], f[2], 0.0f);
+  a.T4[b] = make_float4(t[0], t[1], t[2], 0.0f);
+  a.old4[b] = make_float4(o[0], o[1], o[2], 0.0f);
+}
+
+// user (N, M) list of original ids -> slot-major work list; history likewise
+__global__ void __launch_bounds__(kBlock) k_in_nlist(InOut a) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.n) return;
+  const int64_t i = a.orig[b];
+  const int W32 = (a.M + 31) / 32;
+  for (int w = 0; w < a.W64; ++w) a.live[(int64_t)w * a.n + b] = 0ull;
+  for (int s = 0; s < a.M; ++s) {
+    int32_t j = a.nl_user[i * a.M + s];
+    if (j >= a.n) j = -1;  // not a particle of this state: treated as padding
+    a.nl[(int64_t)s * a.n + b] = j >= 0 ? a.wpos[j] : -1;
+    if (a.xi_user && j >= 0) {
+      const bool lv = (a.live_user[i * W32 + (s >> 5)] >> (s & 31)) & 1u;
+      if (lv) {
+        float x[3] = {0, 0, 0};
+        for (int c = 0; c < a.dim; ++c) x[c] = a.xi_user[(i * a.M + s) * a.dim + c];
+        a.xi[(int64_t)s * a.n + b] = make_float4(x[0], x[1], x[2], 0.0f);
+        a.live[(int64_t)(s >> 6) * a.n + b] |= 1ull << (s & 63);
+      }
+    }
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_out_state(InOut a) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.n) return;
+  const int64_t i = a.orig[b];
+  const int D = a.dim;
+  const int A = D == 3 ? 3 : 1;
+  const float4 p = a.P4[b], v = a.V4[b], w = a.W4[b], f = a.F4[b], t = a.T4[b], o = a.old4[b];
+  const float pp[3] = {p.x, p.y, p.z}, vv[3] = {v.x, v.y, v.z}, ww[3] = {w.x, w.y, w.z}, ff[3] = {f.x, f.y, f.z},
+              tt[3] = {t.x, t.y, t.z}, oo[3] = {o.x, o.y, o.z};
+  for (int c = 0; c < D; ++c) {
+    a.pos[D * i + c] = pp[c];
+    a.vel[D * i + c] = vv[c];
+    a.force[D * i + c] = ff[c];
+    if (a.old_user) a.old_user[D * i + c] = oo[c];
+  }
+  for (int c = 0; c < A; ++c) {
+    a.ang[A * i + c] = ww[c];
+    a.torque[A * i + c] = tt[c];
+  }
+  reinterpret_cast<float4*>(a.q)[i] = a.Q4[b];
+}
+
+__global__ void __launch_bounds__(kBlock) k_out_nlist(InOut a) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.n) return;
+  const int64_t i = a.orig[b];
+  const int W32 = (a.M + 31) / 32;
+  uint32_t words[8];
+  for (int w = 0; w < W32 && w < 8; ++w) words[w] = 0u;
+  for (int s = 0; s < a.M; ++s) {
+    const int32_t j = a.nl[(int64_t)s * a.n + b];
+    a.nl_user[i * a.M + s] = j >= 0 ? a.orig[j] : -1;
+    if (a.xi_user) {
+      const bool lv = j >= 0 && ((a.live[(int64_t)(s >> 6) * a.n + b] >> (s & 63)) & 1ull);
+      if (lv) {
+        const float4 x = a.xi[(int64_t)s * a.n + b];
+        const float xx[3] = {x.x, x.y, x.z};
+        for (int c = 0; c < a.dim; ++c) a.xi_user[(i * a.M + s) * a.dim + c] = xx[c];
+        if ((s >> 5) < 8) words[s >> 5] |= 1u << (s & 31);
+      }
+    }
+  }
+  if (a.xi_user)
+    for (int w = 0; w < W32 && w < 8; ++w) a.live_user[i * W32 + w] = words[w];
+}
+
+// ---------------------------------------------------------------------------
+// rebuild: hash in original order, stable sort, permutation of the work set
+// ---------------------------------------------------------------------------
+template <bool PER>
+JD int ccoord(float x, int a, const DomP& dom, const GridP& g) {
+  float f;
+  if (PER) {
+    float t = (x - dom.anchor[a]) / dom.box[a];
+    t = t - floorf(t);
+    f = floorf(t * (float)g.dims[a]);
+  } else {
+    f = floorf((x - dom.anchor[a]) / g.cell[a]);
+  }
+  f = fminf(fmaxf(f, 0.0f), (float)(g.dims[a] - 1));
+  return (int)f;
+}
+
+// keys[orig[b]] = cell hash of P4[b]; wpos[orig[b]] = b
+template <int D, bool PER>
+__global__ void __launch_bounds__(kBlock) k_hash_work(const float4* __restrict__ P4, const int32_t* __restrict__ orig,
+                                                       int64_t n, DomP dom, GridP g, uint32_t* __restrict__ keys,
+                                                       int32_t* __restrict__ wpos) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  const float4 p = P4[b];
+  const float x[3] = {p.x, p.y, p.z};
+  int h = 0;
+#pragma unroll
+  for (int a = 0; a < D; ++a) h += ccoord<PER>(x[a], a, dom, g) * g.strides[a];
+  const int32_t i = orig[b];
+  keys[i] = (uint32_t)h;
+  wpos[i] = (int32_t)b;
+}
+
+// new work slot b' takes old slot src = wpos[new_orig[b']]
+__global__ void __launch_bounds__(kBlock) k_permute(int64_t n, const int32_t* __restrict__ new_orig,
+                                                     const int32_t* __restrict__ wpos, const float4* __restrict__ P,
+                                                     const float4* __restrict__ V, const float4* __restrict__ W,
+                                                     const float4* __restrict__ Q, const float4* __restrict__ I,
+                                                     float4* __restrict__ P2, float4* __restrict__ V2,
+                                                     float4* __restrict__ W2, float4* __restrict__ Q2,
+                                                     float4* __restrict__ I2, int32_t* __restrict__ orig2,
+                                                     int32_t* __restrict__ src_out) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  const int32_t i = new_orig[b];
+  const int32_t s = wpos[i];
+  P2[b] = P[s];
+  V2[b] = V[s];
+  W2[b] = W[s];
+  Q2[b] = Q[s];
+  I2[b] = I[s];
+  orig2[b] = i;
+  src_out[b] = s;
+}
+
+// neighbour list in work space (cell_list.py:500-595): candidates of a cell
+// are the contiguous work slots [cstart, cend).
+
+JD bool bonded_o(const int32_t* bond_id, int bond_w, int64_t i, int32_t j) {
+  for (int w = 0; w < bond_w; ++w)
+    if (bond_id[i * bond_w + w] == j) return true;
+  return false;
+}
+
+template <int D, bool PER>
+__global__ void __launch_bounds__(kBlock) k_nlist_work(NLW a) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.n) return;
+  const Vec<D> xi = xyz<D>(a.P4[b]);
+  const uint32_t key = a.skeys[b];
+  int c[3];
+  c[0] = (int)(key % (uint32_t)a.grid.dims[0]);
+  c[1] = (int)((key / (uint32_t)a.grid.dims[0]) % (uint32_t)a.grid.dims[1]);
+  c[2] = (D == 3) ? (int)(key / ((uint32_t)a.grid.dims[0] * (uint32_t)a.grid.dims[1])) : 0;
+  const int sr = a.grid.sr;
+  int hi[3];
+  for (int ax = 0; ax < D; ++ax) hi[ax] = PER ? min(sr, a.grid.dims[ax] - sr - 1) : sr;
+  const bool chk_bond = (!a.isb) && a.bond_w > 0;
+  const int64_t ib = chk_bond ? (int64_t)a.orig[b] : 0;
+  int count = 0;
+  for (int ox = -sr; ox <= hi[0]; ++ox) {
+    int cx = c[0] + ox;
+    if (PER) {
+      cx = cx % a.grid.dims[0];
+      if (cx < 0) cx += a.grid.dims[0];
+    } else if (cx < 0 || cx >= a.grid.dims[0])
+      continue;
+    for (int oy = -sr; oy <= hi[1]; ++oy) {
+      int cy = c[1] + oy;
+      if (PER) {
+        cy = cy % a.grid.dims[1];
+        if (cy < 0) cy += a.grid.dims[1];
+      } else if (cy < 0 || cy >= a.grid.dims[1])
+        continue;
+      const int zlo = (D == 3) ? -sr : 0, zhi = (D == 3) ? hi[2] : 0;
+      for (int oz = zlo; oz <= zhi; ++oz) {
+        int cz = 0;
+        if (D == 3) {
+          cz = c[2] + oz;
+          if (PER) {
+            cz = cz % a.grid.dims[2];
+            if (cz < 0) cz += a.grid.dims[2];
+          } else if (cz < 0 || cz >= a.grid.dims[2])
+            continue;
+        }
+        const int h = cx * a.grid.strides[0] + cy * a.grid.strides[1] + (D == 3 ? cz * a.grid.strides[2] : 0);
+        const int k1 = a.cend[h];
+        for (int k = a.cstart[h]; k < k1; ++k) {
+          if (k == b) continue;
+          const Vec<D> d = disp<PER>(xi, xyz<D>(a.P4[k]), a.dom);
+          if (!(dot(d, d) <= a.cutoff_sq)) continue;
+          if (chk_bond && bonded_o(a.bond_id, a.bond_w, ib, a.orig[k])) continue;
+          if (count < a.M) a.nl[(int64_t)count * a.n + b] = k;
+          ++count;
+        }
+      }
+    }
+  }
+  for (int t = count; t < a.M; ++t) a.nl[(int64_t)t * a.n + b] = -1;
+  if (count > a.M) atomicOr(a.overflow, 1);
+}
+
+// history remap (neighbor_list.py:29-54) between two work orders
+__global__ void __launch_bounds__(kBlock) k_remap_work(int64_t n, int M, int W64, const int32_t* __restrict__ src,
+                                                        const int32_t* __restrict__ old_nl,
+                                                        const float4* __restrict__ old_xi,
+                                                        const uint64_t* __restrict__ old_live,
+                                                        const int32_t* __restrict__ new_nl, float4* __restrict__ new_xi,
+                                                        uint64_t* __restrict__ new_live) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  const int64_t a = src[b];
+  for (int w = 0; w < W64; ++w) {
+    uint64_t bits = 0ull;
+    const int s_end = min(M, (w + 1) * 64);
+    for (int s = w * 64; s < s_end; ++s) {
+      const int32_t j = new_nl[(int64_t)s * n + b];
+      if (j < 0) break;
+      const int32_t aj = src[j];
+      int t = -1;
+      for (int u = 0; u < M; ++u) {
+        const int32_t o = old_nl[(int64_t)u * n + a];
+        if (o < 0) break;
+        if (o == aj) {
+          t = u;
+          break;
+        }
+      }
+      if (t < 0) continue;
+      if (!((old_live[(int64_t)(t >> 6) * n + a] >> (t & 63)) & 1ull)) continue;
+      new_xi[(int64_t)s * n + b] = old_xi[(int64_t)t * n + a];
+      bits |= 1ull << (s & 63);
+    }
+    new_live[(int64_t)w * n + b] = bits;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_copy_f4(float4* __restrict__ d, const float4* __restrict__ s, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] = s[i];
+}
+
+__global__ void __launch_bounds__(kBlock) k_iota(int32_t* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = (int32_t)i;
+}
+
+__global__ void __launch_bounds__(kBlock) k_fill_u64(uint64_t* __restrict__ p, int64_t n) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0ull;
+}
+
+// ---------------------------------------------------------------------------
+// the fused step on the work set
+// ---------------------------------------------------------------------------
+
+template <int D, int LAW, bool PER>
+__global__ void __launch_bounds__(kBlock) k_ws_step(WS a) {
+  if (a.flag_rd && *a.flag_rd) {
+    if (a.flag_wr) *a.flag_wr = *a.flag_rd;
+    return;
+  }
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.n) return;
+  const float4 p4 = a.P[b], v4 = a.V[b], w4 = a.W[b], i4 = a.I[b];
+  Side<D> me;
+  me.x = xyz<D>(p4);
+  me.r = p4.w;
+  me.v = xyz<D>(v4);
+  me.m = v4.w;
+  me.w = ang4<D>(w4);
+  const int bits = __float_as_int(i4.w);
+  me.mat = bits >> 1;
+  Vec<D> F;
+  Ang<D> T;
+  if (a.do_force) {
+    F = zero_vec<D>();
+    T = zero_ang<D>();
+    // Batches of PF slots: the list entries, then their positions, are
+    // loaded back to back before any of them is used.
+    constexpr int PF = 4;
+    int w_cur = 0;
+    uint64_t live_old = 0ull, live_new = 0ull;
+    if constexpr (LawTraits<LAW>::hist) live_old = a.live[b];
+    bool done = false;
+    for (int s0 = 0; s0 < a.M && !done; s0 += PF) {
+      int32_t js[PF];
+      float4 pjs[PF];
+#pragma unroll
+      for (int k = 0; k < PF; ++k) js[k] = (s0 + k < a.M) ? a.nl[(int64_t)(s0 + k) * a.n + b] : -1;
+#pragma unroll
+      for (int k = 0; k < PF; ++k) pjs[k] = (js[k] >= 0) ? a.P[js[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+      for (int k = 0; k < PF; ++k) {
+        if (done) break;
+        const int32_t j = js[k];
+        if (j < 0) {
+          done = true;
+          break;
+        }
+        const int s = s0 + k;
+        if constexpr (LawTraits<LAW>::hist) {
+          if ((s >> 6) != w_cur) {
+            a.live[(int64_t)w_cur * a.n + b] = live_new;
+            w_cur = s >> 6;
+            live_old = a.live[(int64_t)w_cur * a.n + b];
+            live_new = 0ull;
+          }
+        }
+        const float4 pj = pjs[k];
+        const Vec<D> rij = disp_<PER>(me.x, xyz<D>(pj), a.dom);
+        const float d_sq = dot(rij, rij);
+        const float R = me.r + pj.w;
+        if (d_sq > (R * R) * 1.0001f) continue;
+        Side<D> o;
+        o.x = xyz<D>(pj);
+        o.r = pj.w;
+        o.mat = (a.law.n_mat > 1) ? (__float_as_int(a.I[j].w) >> 1) : 0;
+        if constexpr (LawTraits<LAW>::kin) {
+          const float4 vj = a.V[j];
+          o.v = xyz<D>(vj);
+          o.m = vj.w;
+          o.w = ang4<D>(a.W[j]);
+        }
+        Vec<D> xold = zero_vec<D>();
+        if constexpr (LawTraits<LAW>::hist) {
+          if ((live_old >> (s & 63)) & 1ull) xold = xyz<D>(a.xi[(int64_t)s * a.n + b]);
+        }
+        PairOut<D> r = law_eval<D, LAW>(a.law, me, o, rij, d_sq, xold, a.dt);
+        F = F + r.F;
+        if constexpr (LawTraits<LAW>::torque) T = T + r.T;
+        if constexpr (LawTraits<LAW>::hist) {
+          if (r.contact) {
+            a.xi[(int64_t)s * a.n + b] = f4(r.xi, 0.0f);
+            live_new |= 1ull << (s & 63);
+          }
+        }
+      }
+    }
+    if constexpr (LawTraits<LAW>::hist) {
+      a.live[(int64_t)w_cur * a.n + b] = live_new;
+      for (int w2 = w_cur + 1; w2 < a.W64; ++w2) a.live[(int64_t)w2 * a.n + b] = 0ull;
+    }
+  } else {
+    F = xyz<D>(a.F[b]);
+    T = ang4<D>(a.T[b]);
+  }
+  Vec<D> x = me.x, v = me.v;
+  Ang<D> w = me.w;
+  Quat q = q4(a.Q[b]);
+  const Ang<D> I = ang4<D>(i4);
+  const float m = me.m;
+  const bool fixed = (bits & 1) != 0;
+  const float freef = fixed ? 0.0f : 1.0f;
+  if (a.do_after) {
+    if (a.has_fm) {
+      const int64_t i = a.orig[b];
+      Vec<D> fp = a.ext_f ? ld<D>(a.ext_f, i) : zero_vec<D>();
+      Vec<D> fc = a.ext_fc ? ld<D>(a.ext_fc, i) : zero_vec<D>();
+      Vec<D> g;
+      if constexpr (D == 3)
+        g = Vec<3>{a.g[0], a.g[1], a.g[2]};
+      else
+        g = Vec<2>{a.g[0], a.g[1]};
+      fc = fc + g * (m / 1.0f);
+      F = (F + fp) + fc;
+      if (a.ext_t) T = T + lda<D>(a.ext_t, i);
+    }
+    if (a.lin == JDEM_LIN_VERLET) {
+      const float s = (a.dt * 0.5f) / m;
+      v = v + (F * s) * freef;
+    } else if (a.lin == JDEM_LIN_EULER) {
+      const float s = a.dt / m;
+      v = v + (F * s) * freef;
+      x = x + v * a.dt;
+    }
+    if (a.rot == JDEM_ROT_VERLET_SPIRAL)
+      vvs_after<D>(q, w, T, I, a.dt, freef);
+    else if (a.rot == JDEM_ROT_SPIRAL)
+      spiral_after<D>(q, w, T, I, a.dt, freef);
+  }
+  if (a.do_before) {
+    if (a.dom.kind == JDEM_DOMAIN_REFLECT) reflect_apply<D>(a.dom, x, v, q, F, m, me.r, fixed, a.dt);
+    if (a.lin == JDEM_LIN_VERLET) {
+      const float s = (a.dt * 0.5f) / m;
+      v = v + (F * s) * freef;
+      x = x + v * a.dt;
+    }
+    if (a.rot == JDEM_ROT_VERLET_SPIRAL) vvs_before<D>(q, w, T, I, a.dt, freef);
+  }
+  a.P2[b] = f4(x, me.r);
+  a.V2[b] = f4(v, m);
+  a.W2[b] = f4a(w);
+  a.Q[b] = f4q(q);
+  if (a.store_ft) {
+    a.F[b] = f4(F, 0.0f);
+    a.T[b] = f4a(T);
+  }
+  if (a.do_before && a.check_disp) {
+    const Vec<D> d = x - xyz<D>(a.old4[b]);
+    if (dot(d, d) > a.trigger) *a.flag_wr = a.step_idx + 1;
+  }
+}
+
+template <int D, int LAW>
+hipError_t launch_ws_law(const WS& a, bool per, hipStream_t s) {
+  dim3 g(nblocks(a.n)), b(kBlock);
+  if (per)
+    hipLaunchKernelGGL((k_ws_step<D, LAW, true>), g, b, 0, s, a);
+  else
+    hipLaunchKernelGGL((k_ws_step<D, LAW, false>), g, b, 0, s, a);
+  return hipGetLastError();
+}
+
+template <int D>
+hipError_t launch_ws_dim(const WS& a, int law, bool per, hipStream_t s) {
+  switch (law) {
+    case JDEM_LAW_SPRING:
+      return launch_ws_law<D, JDEM_LAW_SPRING>(a, per, s);
+    case JDEM_LAW_HERTZ:
+      return launch_ws_law<D, JDEM_LAW_HERTZ>(a, per, s);
+    case JDEM_LAW_CUNDALL_STRACK:
+      return launch_ws_law<D, JDEM_LAW_CUNDALL_STRACK>(a, per, s);
+    case JDEM_LAW_SPRING_FRICTION:
+      return launch_ws_law<D, JDEM_LAW_SPRING_FRICTION>(a, per, s);
+    case JDEM_LAW_HERTZ_FRICTION:
+      return launch_ws_law<D, JDEM_LAW_HERTZ_FRICTION>(a, per, s);
+    default:
+      return hipErrorInvalidValue;
+  }
+}
+
+hipError_t ws_launch_step(const WS& a, int dim, int law, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  const bool per = a.dom.kind == JDEM_DOMAIN_PERIODIC;
+  return dim == 3 ? launch_ws_dim<3>(a, law, per, s) : launch_ws_dim<2>(a, law, per, s);
+}
+
+}  // namespace jdem
+
+// ---------------------------------------------------------------------------
+// launch wrappers of the conversion and rebuild kernels
+// ---------------------------------------------------------------------------
+namespace jdem {
+
+__global__ void __launch_bounds__(kBlock) k_inverse(const int32_t* __restrict__ orig, int64_t n,
+                                                     int32_t* __restrict__ wpos) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < n) wpos[orig[b]] = (int32_t)b;
+}
+
+// NaiveSimulator.create_neighbor_list in work space (descending original id;
+// the work order is the original order when the secondary collider is naive)
+template <int D, bool PER>
+__global__ void __launch_bounds__(kBlock) k_nlist_work_naive(NLW a) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.n) return;
+  const Vec<D> xi = xyz<D>(a.P4[b]);
+  const bool chk_bond = (!a.isb) && a.bond_w > 0;
+  int count = 0;
+  for (int64_t k = a.n - 1; k >= 0; --k) {
+    if (k == b) continue;
+    const Vec<D> d = disp_<PER>(xi, xyz<D>(a.P4[k]), a.dom);
+    if (!(dot(d, d) <= a.cutoff_sq)) continue;
+    if (chk_bond && bonded_o(a.bond_id, a.bond_w, a.orig[b], a.orig[k])) continue;
+    if (count < a.M) a.nl[(int64_t)count * a.n + b] = (int32_t)k;
+    ++count;
+  }
+  for (int t = count; t < a.M; ++t) a.nl[(int64_t)t * a.n + b] = -1;
+  if (count > a.M) atomicOr(a.overflow, 1);
+}
+
+hipError_t ws_launch_in(const InOut& io, bool with_nl, hipStream_t s) {
+  dim3 g(nblocks(io.n)), b(kBlock);
+  hipLaunchKernelGGL(k_in_state, g, b, 0, s, io);
+  if (with_nl) hipLaunchKernelGGL(k_in_nlist, g, b, 0, s, io);
+  return hipGetLastError();
+}
+hipError_t ws_launch_out(const InOut& io, bool with_nl, hipStream_t s) {
+  dim3 g(nblocks(io.n)), b(kBlock);
+  hipLaunchKernelGGL(k_out_state, g, b, 0, s, io);
+  if (with_nl) hipLaunchKernelGGL(k_out_nlist, g, b, 0, s, io);
+  return hipGetLastError();
+}
+hipError_t ws_launch_inverse(const int32_t* orig, int64_t n, int32_t* wpos, hipStream_t s) {
+  hipLaunchKernelGGL(k_inverse, dim3(nblocks(n)), dim3(kBlock), 0, s, orig, n, wpos);
+  return hipGetLastError();
+}
+hipError_t ws_launch_iota(int32_t* p, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_iota, dim3(nblocks(n)), dim3(kBlock), 0, s, p, n);
+  return hipGetLastError();
+}
+hipError_t ws_launch_zero64(uint64_t* p, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_fill_u64, dim3(nblocks(n)), dim3(kBlock), 0, s, p, n);
+  return hipGetLastError();
+}
+hipError_t ws_launch_copy4(float4* d, const float4* src, int64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(k_copy_f4, dim3(nblocks(n)), dim3(kBlock), 0, s, d, src, n);
+  return hipGetLastError();
+}
+hipError_t ws_launch_hash(const float4* P4, const int32_t* orig, int64_t n, int dim, const DomP& dom, const GridP& g,
+                          uint32_t* keys, int32_t* wpos, hipStream_t s) {
+  dim3 gr(nblocks(n)), b(kBlock);
+  const bool per = g.periodic != 0;
+  if (dim == 3) {
+    if (per)
+      hipLaunchKernelGGL((k_hash_work<3, true>), gr, b, 0, s, P4, orig, n, dom, g, keys, wpos);
+    else
+      hipLaunchKernelGGL((k_hash_work<3, false>), gr, b, 0, s, P4, orig, n, dom, g, keys, wpos);
+  } else {
+    if (per)
+      hipLaunchKernelGGL((k_hash_work<2, true>), gr, b, 0, s, P4, orig, n, dom, g, keys, wpos);
+    else
+      hipLaunchKernelGGL((k_hash_work<2, false>), gr, b, 0, s, P4, orig, n, dom, g, keys, wpos);
+  }
+  return hipGetLastError();
+}
+hipError_t ws_launch_permute(int64_t n, const int32_t* new_orig, const int32_t* wpos, const float4* P,
+                             const float4* V, const float4* W, const float4* Q, const float4* I, float4* P2,
+                             float4* V2, float4* W2, float4* Q2, float4* I2, int32_t* orig2, int32_t* src,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_permute, dim3(nblocks(n)), dim3(kBlock), 0, s, n, new_orig, wpos, P, V, W, Q, I, P2, V2, W2,
+                     Q2, I2, orig2, src);
+  return hipGetLastError();
+}
+hipError_t ws_launch_nlist(const NLW& a, int dim, bool naive, hipStream_t s) {
+  dim3 g(nblocks(a.n)), b(kBlock);
+  const bool per = naive ? a.dom.kind == JDEM_DOMAIN_PERIODIC : a.grid.periodic != 0;
+  if (naive) {
+    if (dim == 3) {
+      if (per)
+        hipLaunchKernelGGL((k_nlist_work_naive<3, true>), g, b, 0, s, a);
+      else
+        hipLaunchKernelGGL((k_nlist_work_naive<3, false>), g, b, 0, s, a);
+    } else {
+      if (per)
+        hipLaunchKernelGGL((k_nlist_work_naive<2, true>), g, b, 0, s, a);
+      else
+        hipLaunchKernelGGL((k_nlist_work_naive<2, false>), g, b, 0, s, a);
+    }
+  } else {
+    if (dim == 3) {
+      if (per)
+        hipLaunchKernelGGL((k_nlist_work<3, true>), g, b, 0, s, a);
+      else
+        hipLaunchKernelGGL((k_nlist_work<3, false>), g, b, 0, s, a);
+    } else {
+      if (per)
+        hipLaunchKernelGGL((k_nlist_work<2, true>), g, b, 0, s, a);
+      else
+        hipLaunchKernelGGL((k_nlist_work<2, false>), g, b, 0, s, a);
+    }
+  }
+  return hipGetLastError();
+}
+hipError_t ws_launch_remap(int64_t n, int M, int W64, const int32_t* src, const int32_t* old_nl, const float4* old_xi,
+                           const uint64_t* old_live, const int32_t* new_nl, float4* new_xi, uint64_t* new_live,
+                           hipStream_t s) {
+  hipLaunchKernelGGL(k_remap_work, dim3(nblocks(n)), dim3(kBlock), 0, s, n, M, W64, src, old_nl, old_xi, old_live,
+                     new_nl, new_xi, new_live);
+  return hipGetLastError();
+}
+
+// FreeDomain box of float4 positions (free.py:42-58)
+__global__ void __launch_bounds__(kBlock) k_box_minmax4(const float4* __restrict__ P, int64_t n,
+                                                         float* __restrict__ part) {
+  __shared__ float smin[3][kBlock], smax[3][kBlock];
+  const int tid = threadIdx.x;
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + tid; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const float4 p = P[i];
+    const float x[3] = {p.x, p.y, p.z};
+    for (int a = 0; a < 3; ++a) {
+      lo[a] = fminf(lo[a], x[a] - p.w);
+      hi[a] = fmaxf(hi[a], x[a] + p.w);
+    }
+  }
+  for (int a = 0; a < 3; ++a) {
+    smin[a][tid] = lo[a];
+    smax[a][tid] = hi[a];
+  }
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if (tid < off)
+      for (int a = 0; a < 3; ++a) {
+        smin[a][tid] = fminf(smin[a][tid], smin[a][tid + off]);
+        smax[a][tid] = fmaxf(smax[a][tid], smax[a][tid + off]);
+      }
+    __syncthreads();
+  }
+  if (tid == 0)
+    for (int a = 0; a < 3; ++a) {
+      part[6 * blockIdx.x + a] = smin[a][0];
+      part[6 * blockIdx.x + 3 + a] = smax[a][0];
+    }
+}
+hipError_t ws_launch_box(const float4* P, int64_t n, float* part, int nparts, hipStream_t s) {
+  hipLaunchKernelGGL(k_box_minmax4, dim3(nparts), dim3(kBlock), 0, s, P, n, part);
+  return hipGetLastError();
+}
+
+}  // namespace jdem

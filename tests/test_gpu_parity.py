@@ -323,3 +323,54 @@ def test_potential_energy(device, law, collider):
     assert e == pytest.approx(want, rel=1e-12)
     ke = jd.utils.compute_kinetic_energy(st, sy)
     assert ke == pytest.approx(O.kinetic_energy(ost), rel=1e-12)
+
+
+# ---------------------------------------------------------------------------
+# resident NeighborList engine
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("case", ["spring-periodic", "hertzfriction-2d", "springfriction-reflect", "naive-secondary"])
+def test_neighborlist_variants_bitwise(device, case):
+    print(f"[step] neighborlist {case}", flush=True)
+    dim = 2 if case.endswith("2d") else 3
+    domain = "reflect" if "reflect" in case else "periodic"
+    law = {"spring-periodic": "spring", "hertzfriction-2d": "hertzfriction",
+           "springfriction-reflect": "springfriction", "naive-secondary": "springfriction"}[case]
+    N = 600 if case == "naive-secondary" else 3000
+    st, box = packed(N, dim, seed=17, domain=domain, phi=0.5)
+    fkw = {"force_model_kw": {"kt_ratio": 0.3, "damping": 0.05}} if "friction" in law else {}
+    sy = make_system(st, box, domain, law, "NeighborList", dt=2e-4, skin=0.05, **fkw)
+    if case == "naive-secondary":
+        col = jd.NeighborList.Create(st, cutoff=float(2 * st.rad.max().item()), skin=0.05,
+                                     secondary_collider_type="naive")
+        from dataclasses import replace as _r
+        sy = _r(sy, collider=jd.system._fresh_collider(_r(col, history=jd.system._alloc_history(col, st.shape, sy.force_model))))
+    n = 25
+    st2, sy2 = jd.System.step(st, sy, n=n)
+    nls = B.nl_state_of(st, sy)
+    ost = O.step(B.state_of(st), B.params_of(st, sy), n, nls)
+    _compare_states(st2, ost, f"nl/{case}")
+    assert sy2.collider.n_build_times == nls.n_build
+    assert_bitwise("nl", sy2.collider.neighbor_list.cpu().numpy().astype(np.int64), nls.nl)
+    if sy2.collider.history is not None:
+        xi, live = B.history_of(sy2.collider)
+        assert_bitwise("live", live, nls.live)
+        assert_bitwise("xi", xi, nls.xi)
+
+
+def test_neighborlist_split_calls_equal_one_call(device):
+    """step(n=30) == step(n=12) then step(n=18): the list and the history
+    survive the round trip through the caller's layouts."""
+    w = W.spheres_friction(N=4096, seed=3, skin=0.05)
+    st, sy = W.make_gpu_system(w, device=device)
+    a, ya = jd.System.step(st, sy, n=30)
+    b1, yb = jd.System.step(st, sy, n=12)
+    b, yb = jd.System.step(b1, yb, n=18)
+    _compare_states(a, B.state_of(b), "split")
+    assert ya.collider.n_build_times == yb.collider.n_build_times
+    assert torch.equal(ya.collider.neighbor_list, yb.collider.neighbor_list)
+    xa, la = B.history_of(ya.collider)
+    xb, lb = B.history_of(yb.collider)
+    assert_bitwise("live", la, lb)
+    assert_bitwise("xi", xa, xb)
+    # the inputs are left untouched (functional semantics)
+    assert int(sy.collider.n_build_times) == 0
