@@ -43,6 +43,13 @@ def parse_args():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument(
+        "--copy-probe-mib",
+        type=int,
+        default=0,
+        help="also run three plain device-to-device copies of this many MiB before the timed steps "
+        "(a kernel of known traffic, to scale rocprofv3 FETCH_SIZE / WRITE_SIZE)",
+    )
     return ap.parse_args()
 
 
@@ -102,8 +109,8 @@ def main():
     args = parse_args()
     if args.gpus != 1:
         print(
-            f"bench.py measures one GPU: --gpus {args.gpus} is not supported (this build has no multi-GPU path; "
-            "see DESIGN.md for how the step would shard)",
+            f"bench.py measures one GPU: --gpus {args.gpus} is not supported (the step runs on one GPU; "
+            "there is no multi-GPU path)",
             file=sys.stderr,
         )
         sys.exit(2)
@@ -119,6 +126,14 @@ def main():
     print(f"[bench] N={w.N} dt={w.dt:.4e} M={sy.collider.max_neighbors} warmup={args.warmup}", file=sys.stderr, flush=True)
     if args.warmup > 0:
         st, sy = jd.System.step(st, sy, n=args.warmup)
+    if args.copy_probe_mib > 0:
+        nel = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.rand(nel, device=dev)
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            dst.copy_(src)
+        print(f"[bench] copy probe: 3 x {nel * 4} bytes read and written", file=sys.stderr, flush=True)
+        del src, dst
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     st1, sy1 = jd.System.step(st, sy, n=args.steps)
@@ -174,7 +189,7 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": traffic,
-            "kernel": "k_step<3, SPRING_FRICTION, NLIST, periodic>",
+            "kernel": "k_ws_step<3, SPRING_FRICTION, periodic>",
             "kernel_avg_us": 1e3 * avg_ms,
             "algorithmic_bytes_per_particle": bytes_per_particle,
             "mean_history_contacts": zh,

@@ -17,11 +17,19 @@ using namespace jdem;
 #define JDEM_TARGET "gfx950"
 #endif
 
-size_t jdem_ctx::scratch_bytes() const {
-  const Buf* all[] = {&pos2, &vel2, &ang2, &keys[0], &keys[1], &vals[0], &vals[1], &hist, &cstart, &cend, &spos,
-                      &nl_new, &xi_new, &live_new, &flags, &red, &nl_copy, &xi_copy, &live_copy};
+std::vector<Buf*> jdem_ctx::bufs() {
+  std::vector<Buf*> all = {&pos2,    &vel2,   &ang2,     &keys[0], &keys[1], &vals[0], &vals[1], &hist,
+                           &cstart,  &cend,   &spos,     &nl_new,  &xi_new,  &live_new, &flags,  &red,
+                           &nl_copy, &xi_copy, &live_copy, &wF,     &wT,      &wOld,     &wWpos, &wSrc,
+                           &wInv,    &wKeys};
+  for (int k = 0; k < 2; ++k)
+    for (Buf* b : {&wP[k], &wV[k], &wW[k], &wQ[k], &wI[k], &wOrig[k], &wNl[k], &wXi[k], &wLive[k]}) all.push_back(b);
+  return all;
+}
+
+size_t jdem_ctx::scratch_bytes() {
   size_t s = 0;
-  for (const Buf* b : all) s += b->bytes;
+  for (const Buf* b : bufs()) s += b->bytes;
   return s;
 }
 
@@ -394,8 +402,11 @@ hipEvent_t take_event(jdem_ctx* ctx) {
     ctx->event_pool.pop_back();
     return e;
   }
+  // Timing-only events: without the system-scope fence, recording one does
+  // not write back and invalidate the caches, which would slow the kernel
+  // that follows and distort what is measured.
   hipEvent_t e = nullptr;
-  if (hipEventCreate(&e) != hipSuccess) return nullptr;
+  if (hipEventCreateWithFlags(&e, hipEventDisableSystemFence) != hipSuccess) return nullptr;
   return e;
 }
 
@@ -450,16 +461,14 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   ENSURE(ctx, ctx->wOld, n4);
   ENSURE(ctx, ctx->wWpos, ni);
   ENSURE(ctx, ctx->wSrc, ni);
+  ENSURE(ctx, ctx->wInv, ni);
   ENSURE(ctx, ctx->wKeys, ni);
   ENSURE(ctx, ctx->keys[0], ni);
   ENSURE(ctx, ctx->keys[1], ni);
   ENSURE(ctx, ctx->vals[0], ni);
   ENSURE(ctx, ctx->vals[1], ni);
   ENSURE(ctx, ctx->hist, 4 * 256 * (size_t)((N + 4095) / 4096 + 1));
-  if (!naive2) {
-    ENSURE(ctx, ctx->cstart, 4 * (size_t)L.grid.n_cells);
-    ENSURE(ctx, ctx->cend, 4 * (size_t)L.grid.n_cells);
-  }
+  if (!naive2) ENSURE(ctx, ctx->cstart, 4 * ((size_t)L.grid.n_cells + 1));
   auto F4 = [](jdem::Buf& b) { return (float4*)b.p; };
   auto I32 = [](jdem::Buf& b) { return (int32_t*)b.p; };
   auto U64 = [](jdem::Buf& b) { return (uint64_t*)b.p; };
@@ -520,6 +529,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     const int sp = span_begin(ctx, 2, 0);
     HIPCHK(ctx, launch_fill_i32((int32_t*)ctx->flags.p + 2, 1, 0, st));
     int32_t* src = I32(ctx->wSrc);
+    int32_t* inv = I32(ctx->wInv);
     uint32_t* sk = nullptr;
     if (!naive2) {
       HIPCHK(ctx, ws_launch_hash(F4(ctx->wP[cur]), I32(ctx->wOrig[set]), N, D, dom, g, (uint32_t*)ctx->wKeys.p,
@@ -529,14 +539,13 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       HIPCHK(ctx, ws_launch_permute(N, new_orig, I32(ctx->wWpos), F4(ctx->wP[cur]), F4(ctx->wV[cur]),
                                     F4(ctx->wW[cur]), F4(ctx->wQ[set]), F4(ctx->wI[set]), F4(ctx->wP[1 - cur]),
                                     F4(ctx->wV[1 - cur]), F4(ctx->wW[1 - cur]), F4(ctx->wQ[1 - set]),
-                                    F4(ctx->wI[1 - set]), I32(ctx->wOrig[1 - set]), src, st));
+                                    F4(ctx->wI[1 - set]), I32(ctx->wOrig[1 - set]), src, inv, st));
       cur = 1 - cur;
       set = 1 - set;
-      HIPCHK(ctx, launch_fill_i32(I32(ctx->cstart), g.n_cells, 0, st));
-      HIPCHK(ctx, launch_fill_i32(I32(ctx->cend), g.n_cells, 0, st));
-      HIPCHK(ctx, launch_cell_ranges(sk, N, I32(ctx->cstart), I32(ctx->cend), st));
+      HIPCHK(ctx, ws_launch_cell_starts(sk, N, g.n_cells, I32(ctx->cstart), st));
     } else {
       HIPCHK(ctx, ws_launch_iota(src, N, st));
+      HIPCHK(ctx, ws_launch_iota(inv, N, st));
     }
     if (M > 0) {
       NLW b;
@@ -551,21 +560,23 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       b.orig = I32(ctx->wOrig[set]);
       b.P4 = F4(ctx->wP[cur]);
       b.skeys = sk;
-      b.cstart = I32(ctx->cstart);
-      b.cend = I32(ctx->cend);
+      b.starts = I32(ctx->cstart);
       b.grid = g;
       b.dom = dom;
       b.nl = I32(ctx->wNl[1 - nlc]);
       b.overflow = (int32_t*)ctx->flags.p + 2;
-      HIPCHK(ctx, ws_launch_nlist(b, D, naive2, st));
       if (hist) {
-        if (L.n_build_times == 0) {
-          HIPCHK(ctx, ws_launch_zero64(U64(ctx->wLive[1 - nlc]), N * W64, st));
-        } else {
-          HIPCHK(ctx, ws_launch_remap(N, M, W64, src, I32(ctx->wNl[nlc]), F4(ctx->wXi[nlc]), U64(ctx->wLive[nlc]),
-                                      I32(ctx->wNl[1 - nlc]), F4(ctx->wXi[1 - nlc]), U64(ctx->wLive[1 - nlc]), st));
-        }
+        b.W64 = W64;
+        b.remap = L.n_build_times > 0;
+        b.src = src;
+        b.inv = inv;
+        b.old_nl = I32(ctx->wNl[nlc]);
+        b.old_xi = F4(ctx->wXi[nlc]);
+        b.old_live = U64(ctx->wLive[nlc]);
+        b.xi = F4(ctx->wXi[1 - nlc]);
+        b.live = U64(ctx->wLive[1 - nlc]);
       }
+      HIPCHK(ctx, ws_launch_nlist(b, D, naive2, st));
       nlc = 1 - nlc;
     }
     HIPCHK(ctx, ws_launch_copy4(F4(ctx->wOld), F4(ctx->wP[cur]), N, st));
@@ -774,10 +785,7 @@ int jdem_ctx_destroy(jdem_ctx* ctx) {
   if (!ctx) return JDEM_OK;
   (void)hipSetDevice(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
-  Buf* all[] = {&ctx->pos2, &ctx->vel2, &ctx->ang2, &ctx->keys[0], &ctx->keys[1], &ctx->vals[0], &ctx->vals[1],
-                &ctx->hist, &ctx->cstart, &ctx->cend, &ctx->spos, &ctx->nl_new, &ctx->xi_new, &ctx->live_new,
-                &ctx->flags, &ctx->red, &ctx->nl_copy, &ctx->xi_copy, &ctx->live_copy};
-  for (Buf* b : all)
+  for (Buf* b : ctx->bufs())
     if (b->p) (void)hipFree(b->p);
   if (ctx->host_flags) (void)hipHostFree(ctx->host_flags);
   for (auto& s : ctx->spans) {
@@ -799,7 +807,7 @@ const char* jdem_last_error(const jdem_ctx* ctx) { return ctx ? ctx->err.c_str()
 
 int jdem_ctx_scratch_bytes(const jdem_ctx* ctx, size_t* bytes) {
   if (!ctx || !bytes) return JDEM_EINVAL;
-  *bytes = ctx->scratch_bytes();
+  *bytes = const_cast<jdem_ctx*>(ctx)->scratch_bytes();
   return JDEM_OK;
 }
 

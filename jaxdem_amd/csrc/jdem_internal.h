@@ -46,7 +46,7 @@ struct jdem_ctx {
   jdem::Buf red;                       // reduction partials
   jdem::Buf nl_copy, xi_copy, live_copy;
   // resident NeighborList working set (jdem_resident.hip)
-  jdem::Buf wP[2], wV[2], wW[2], wQ[2], wI[2], wF, wT, wOld, wOrig[2], wWpos, wSrc, wKeys, wNl[2], wXi[2], wLive[2];
+  jdem::Buf wP[2], wV[2], wW[2], wQ[2], wI[2], wF, wT, wOld, wOrig[2], wWpos, wSrc, wInv, wKeys, wNl[2], wXi[2], wLive[2];
   int32_t* host_flags = nullptr;       // pinned host mirror of flags
   // HIP-event timing (jdem_ctx_set_timing)
   struct TimedSpan {
@@ -57,7 +57,9 @@ struct jdem_ctx {
   bool timing = false;
   std::vector<hipEvent_t> event_pool;
   std::vector<TimedSpan> spans;
-  size_t scratch_bytes() const;
+  // every scratch buffer of the context, for accounting and release
+  std::vector<jdem::Buf*> bufs();
+  size_t scratch_bytes();
 };
 
 namespace jdem {
@@ -223,13 +225,31 @@ struct NLW {
   const int32_t* orig;
   const float4* P4;
   const uint32_t* skeys;  // sorted keys = cell of work slot
-  const int32_t* cstart;
-  const int32_t* cend;
+  const int32_t* starts;  // (n_cells + 1) first slot of each cell
   GridP grid;
   DomP dom;
   int32_t* nl;
   int32_t* overflow;
+  // history carried over from the previous list (null live: no history)
+  int W64;
+  int remap;                // 0: start with empty history, 1: remap old
+  const int32_t* src;       // new work slot -> old work slot
+  const int32_t* inv;       // old work slot -> new work slot
+  const int32_t* old_nl;
+  const float4* old_xi;
+  const uint64_t* old_live;
+  float4* xi;
+  uint64_t* live;
 };
+
+// Workgroups are dispatched round-robin over the 8 XCDs of the chip, each
+// with its own L2: workgroup w runs on XCD w % 8.  xcd_tile maps them so
+// that XCD x processes one contiguous run of tiles; in the cell-sorted work
+// order, spatial neighbours are then mostly served by the same L2.
+__device__ inline unsigned xcd_tile(unsigned w, unsigned nw) {
+  const unsigned q = nw / 8u, r = nw % 8u, x = w % 8u, l = w / 8u;
+  return (x < r ? x * (q + 1u) : r * (q + 1u) + (x - r) * q) + l;
+}
 
 struct WS {
   int64_t n;
@@ -271,18 +291,15 @@ hipError_t ws_launch_in(const InOut& io, bool with_nl, hipStream_t s);
 hipError_t ws_launch_out(const InOut& io, bool with_nl, hipStream_t s);
 hipError_t ws_launch_inverse(const int32_t* orig, int64_t n, int32_t* wpos, hipStream_t s);
 hipError_t ws_launch_iota(int32_t* p, int64_t n, hipStream_t s);
-hipError_t ws_launch_zero64(uint64_t* p, int64_t n, hipStream_t s);
 hipError_t ws_launch_copy4(float4* d, const float4* src, int64_t n, hipStream_t s);
 hipError_t ws_launch_hash(const float4* P4, const int32_t* orig, int64_t n, int dim, const DomP& dom, const GridP& g,
                           uint32_t* keys, int32_t* wpos, hipStream_t s);
 hipError_t ws_launch_permute(int64_t n, const int32_t* new_orig, const int32_t* wpos, const float4* P,
                              const float4* V, const float4* W, const float4* Q, const float4* I, float4* P2,
                              float4* V2, float4* W2, float4* Q2, float4* I2, int32_t* orig2, int32_t* src,
-                             hipStream_t s);
+                             int32_t* inv, hipStream_t s);
+hipError_t ws_launch_cell_starts(const uint32_t* skeys, int64_t n, int64_t n_cells, int32_t* starts, hipStream_t s);
 hipError_t ws_launch_nlist(const NLW& a, int dim, bool naive, hipStream_t s);
-hipError_t ws_launch_remap(int64_t n, int M, int W64, const int32_t* src, const int32_t* old_nl, const float4* old_xi,
-                           const uint64_t* old_live, const int32_t* new_nl, float4* new_xi, uint64_t* new_live,
-                           hipStream_t s);
 hipError_t ws_launch_step(const WS& a, int dim, int law, hipStream_t s);
 hipError_t ws_launch_box(const float4* P, int64_t n, float* part, int nparts, hipStream_t s);
 

@@ -132,8 +132,10 @@ __global__ void __launch_bounds__(kBlock) k_out_nlist(InOut a) {
   const int W32 = (a.M + 31) / 32;
   uint32_t words[8];
   for (int w = 0; w < W32 && w < 8; ++w) words[w] = 0u;
+  bool ended = false;  // work rows end at their first -1
   for (int s = 0; s < a.M; ++s) {
-    const int32_t j = a.nl[(int64_t)s * a.n + b];
+    const int32_t j = ended ? -1 : a.nl[(int64_t)s * a.n + b];
+    ended = j < 0;
     a.nl_user[i * a.M + s] = j >= 0 ? a.orig[j] : -1;
     if (a.xi_user) {
       const bool lv = j >= 0 && ((a.live[(int64_t)(s >> 6) * a.n + b] >> (s & 63)) & 1ull);
@@ -166,7 +168,18 @@ JD int ccoord(float x, int a, const DomP& dom, const GridP& g) {
   return (int)f;
 }
 
-// keys[orig[b]] = cell hash of P4[b]; wpos[orig[b]] = b
+// Work-space cell key: the LAST axis varies fastest (z in 3D, y in 2D), so
+// the stencil's fastest axis ('ij' meshgrid order, cell_list.py:425-427)
+// walks consecutive keys.  Any linearisation gives the same rows (stencil
+// order, ascending id within a cell); this one makes a stencil column one
+// contiguous range of the sorted work set.
+template <int D>
+JD uint32_t wkey(const int* c, const GridP& g) {
+  if (D == 3) return (uint32_t)((c[0] * g.dims[1] + c[1]) * g.dims[2] + c[2]);
+  return (uint32_t)(c[0] * g.dims[1] + c[1]);
+}
+
+// keys[orig[b]] = work cell key of P4[b]; wpos[orig[b]] = b
 template <int D, bool PER>
 __global__ void __launch_bounds__(kBlock) k_hash_work(const float4* __restrict__ P4, const int32_t* __restrict__ orig,
                                                        int64_t n, DomP dom, GridP g, uint32_t* __restrict__ keys,
@@ -175,15 +188,34 @@ __global__ void __launch_bounds__(kBlock) k_hash_work(const float4* __restrict__
   if (b >= n) return;
   const float4 p = P4[b];
   const float x[3] = {p.x, p.y, p.z};
-  int h = 0;
+  int c[3] = {0, 0, 0};
 #pragma unroll
-  for (int a = 0; a < D; ++a) h += ccoord<PER>(x[a], a, dom, g) * g.strides[a];
+  for (int a = 0; a < D; ++a) c[a] = ccoord<PER>(x[a], a, dom, g);
   const int32_t i = orig[b];
-  keys[i] = (uint32_t)h;
+  keys[i] = wkey<D>(c, g);
   wpos[i] = (int32_t)b;
 }
 
-// new work slot b' takes old slot src = wpos[new_orig[b']]
+// Dense cell starts: starts[h] = first sorted slot with key >= h (a lower
+// bound in the sorted keys), for h in [0, n_cells]; an empty cell gets the
+// start of the next occupied one, so consecutive cells h1..h2 hold the slots
+// [starts[h1], starts[h2 + 1]).
+__global__ void __launch_bounds__(kBlock) k_cell_starts(const uint32_t* __restrict__ sk, int64_t n, int64_t n_cells,
+                                                         int32_t* __restrict__ starts) {
+  const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (h > n_cells) return;
+  int64_t lo = 0, hi = n;
+  while (lo < hi) {
+    const int64_t mid = (lo + hi) >> 1;
+    if ((int64_t)sk[mid] < h)
+      lo = mid + 1;
+    else
+      hi = mid;
+  }
+  starts[h] = (int32_t)lo;
+}
+
+// new work slot b' takes old slot src = wpos[new_orig[b']]; inv[src] = b'
 __global__ void __launch_bounds__(kBlock) k_permute(int64_t n, const int32_t* __restrict__ new_orig,
                                                      const int32_t* __restrict__ wpos, const float4* __restrict__ P,
                                                      const float4* __restrict__ V, const float4* __restrict__ W,
@@ -191,7 +223,7 @@ __global__ void __launch_bounds__(kBlock) k_permute(int64_t n, const int32_t* __
                                                      float4* __restrict__ P2, float4* __restrict__ V2,
                                                      float4* __restrict__ W2, float4* __restrict__ Q2,
                                                      float4* __restrict__ I2, int32_t* __restrict__ orig2,
-                                                     int32_t* __restrict__ src_out) {
+                                                     int32_t* __restrict__ src_out, int32_t* __restrict__ inv_out) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n) return;
   const int32_t i = new_orig[b];
@@ -203,10 +235,15 @@ __global__ void __launch_bounds__(kBlock) k_permute(int64_t n, const int32_t* __
   I2[b] = I[s];
   orig2[b] = i;
   src_out[b] = s;
+  inv_out[s] = (int32_t)b;
 }
 
-// neighbour list in work space (cell_list.py:500-595): candidates of a cell
-// are the contiguous work slots [cstart, cend).
+// neighbour list in work space (cell_list.py:500-595).  The stencil is
+// walked in the reference's 'ij' order (cell_list.py:425-427), last axis
+// fastest; a run of stencil cells along that axis is one contiguous range of
+// work slots, [starts[h_first], starts[h_last + 1]), split in two where a
+// periodic axis wraps.  Candidates within a cell come in ascending original
+// id, the order of the reference's stable hash sort.
 
 JD bool bonded_o(const int32_t* bond_id, int bond_w, int64_t i, int32_t j) {
   for (int w = 0; w < bond_w; ++w)
@@ -214,97 +251,184 @@ JD bool bonded_o(const int32_t* bond_id, int bond_w, int64_t i, int32_t j) {
   return false;
 }
 
-template <int D, bool PER>
-__global__ void __launch_bounds__(kBlock) k_nlist_work(NLW a) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= a.n) return;
-  const Vec<D> xi = xyz<D>(a.P4[b]);
-  const uint32_t key = a.skeys[b];
-  int c[3];
-  c[0] = (int)(key % (uint32_t)a.grid.dims[0]);
-  c[1] = (int)((key / (uint32_t)a.grid.dims[0]) % (uint32_t)a.grid.dims[1]);
-  c[2] = (D == 3) ? (int)(key / ((uint32_t)a.grid.dims[0] * (uint32_t)a.grid.dims[1])) : 0;
-  const int sr = a.grid.sr;
-  int hi[3];
-  for (int ax = 0; ax < D; ++ax) hi[ax] = PER ? min(sr, a.grid.dims[ax] - sr - 1) : sr;
-  const bool chk_bond = (!a.isb) && a.bond_w > 0;
-  const int64_t ib = chk_bond ? (int64_t)a.orig[b] : 0;
-  int count = 0;
-  for (int ox = -sr; ox <= hi[0]; ++ox) {
-    int cx = c[0] + ox;
-    if (PER) {
-      cx = cx % a.grid.dims[0];
-      if (cx < 0) cx += a.grid.dims[0];
-    } else if (cx < 0 || cx >= a.grid.dims[0])
-      continue;
-    for (int oy = -sr; oy <= hi[1]; ++oy) {
-      int cy = c[1] + oy;
-      if (PER) {
-        cy = cy % a.grid.dims[1];
-        if (cy < 0) cy += a.grid.dims[1];
-      } else if (cy < 0 || cy >= a.grid.dims[1])
-        continue;
-      const int zlo = (D == 3) ? -sr : 0, zhi = (D == 3) ? hi[2] : 0;
-      for (int oz = zlo; oz <= zhi; ++oz) {
-        int cz = 0;
-        if (D == 3) {
-          cz = c[2] + oz;
-          if (PER) {
-            cz = cz % a.grid.dims[2];
-            if (cz < 0) cz += a.grid.dims[2];
-          } else if (cz < 0 || cz >= a.grid.dims[2])
-            continue;
-        }
-        const int h = cx * a.grid.strides[0] + cy * a.grid.strides[1] + (D == 3 ? cz * a.grid.strides[2] : 0);
-        const int k1 = a.cend[h];
-        for (int k = a.cstart[h]; k < k1; ++k) {
-          if (k == b) continue;
-          const Vec<D> d = disp<PER>(xi, xyz<D>(a.P4[k]), a.dom);
-          if (!(dot(d, d) <= a.cutoff_sq)) continue;
-          if (chk_bond && bonded_o(a.bond_id, a.bond_w, ib, a.orig[k])) continue;
-          if (count < a.M) a.nl[(int64_t)count * a.n + b] = k;
-          ++count;
-        }
-      }
-    }
+// Cells of the stencil along one axis, in stencil order: the offsets
+// -sr..hi (hi = min(sr, dims - sr - 1) on a periodic axis, which drops the
+// offsets that would revisit a cell) give at most two runs of consecutive
+// cells, [r0a, r0b] then [r1a, r1b] (empty when a > b).
+template <bool PER>
+JD void stencil_runs(int c, int sr, int dims, int& r0a, int& r0b, int& r1a, int& r1b) {
+  if (PER) {
+    const int cnt = sr + min(sr, dims - sr - 1) + 1;  // in [1, dims]
+    int start = (c - sr) % dims;
+    if (start < 0) start += dims;
+    const int last = start + cnt - 1;
+    r0a = start;
+    r0b = min(last, dims - 1);
+    r1a = 0;
+    r1b = last - dims;
+  } else {
+    r0a = max(c - sr, 0);
+    r0b = min(c + sr, dims - 1);
+    r1a = 0;
+    r1b = -1;
   }
-  for (int t = count; t < a.M; ++t) a.nl[(int64_t)t * a.n + b] = -1;
-  if (count > a.M) atomicOr(a.overflow, 1);
 }
 
-// history remap (neighbor_list.py:29-54) between two work orders
-__global__ void __launch_bounds__(kBlock) k_remap_work(int64_t n, int M, int W64, const int32_t* __restrict__ src,
-                                                        const int32_t* __restrict__ old_nl,
-                                                        const float4* __restrict__ old_xi,
-                                                        const uint64_t* __restrict__ old_live,
-                                                        const int32_t* __restrict__ new_nl, float4* __restrict__ new_xi,
-                                                        uint64_t* __restrict__ new_live) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= n) return;
-  const int64_t a = src[b];
-  for (int w = 0; w < W64; ++w) {
-    uint64_t bits = 0ull;
-    const int s_end = min(M, (w + 1) * 64);
-    for (int s = w * 64; s < s_end; ++s) {
-      const int32_t j = new_nl[(int64_t)s * n + b];
-      if (j < 0) break;
-      const int32_t aj = src[j];
-      int t = -1;
-      for (int u = 0; u < M; ++u) {
-        const int32_t o = old_nl[(int64_t)u * n + a];
-        if (o < 0) break;
-        if (o == aj) {
-          t = u;
+// Offsets along a slow axis; returns the cell or -1 when it is off the grid.
+template <bool PER>
+JD int stencil_cell(int c, int o, int dims) {
+  int x = c + o;
+  if (PER) {
+    x %= dims;
+    if (x < 0) x += dims;
+    return x;
+  }
+  return (x < 0 || x >= dims) ? -1 : x;
+}
+
+// History of a freshly built row (neighbor_list.py:29-54): the history of
+// new pair (b, j) is the old history of the same two particles, zero when
+// they were not listed together.  Only live old slots carry history, so each
+// one is looked up in the new row (row(s), s < nrow): old slot t of src[b]
+// lists old work slot o, now at inv[o].  A pair occurs at most once per row,
+// so the lookups are one-to-one.  Rows hold at most 256 slots (four live
+// words).
+template <class Row>
+JD void remap_row(const NLW& a, int64_t b, int nrow, Row row) {
+  if (!a.live) return;
+  const int64_t n = a.n;
+  uint64_t out0 = 0ull, out1 = 0ull, out2 = 0ull, out3 = 0ull;
+  if (a.remap) {
+    const int64_t ob = a.src[b];
+    for (int w = 0; w < a.W64; ++w) {
+      uint64_t bits = a.old_live[(int64_t)w * n + ob];
+      while (bits) {
+        const int t = w * 64 + (__ffsll((unsigned long long)bits) - 1);
+        bits &= bits - 1ull;
+        const int32_t o = a.old_nl[(int64_t)t * n + ob];
+        if (o < 0) continue;
+        const int32_t tgt = a.inv[o];
+        for (int s = 0; s < nrow; ++s) {
+          if (row(s) != tgt) continue;
+          a.xi[(int64_t)s * n + b] = a.old_xi[(int64_t)t * n + ob];
+          const uint64_t bit = 1ull << (s & 63);
+          const int ws = s >> 6;
+          out0 |= ws == 0 ? bit : 0ull;
+          out1 |= ws == 1 ? bit : 0ull;
+          out2 |= ws == 2 ? bit : 0ull;
+          out3 |= ws == 3 ? bit : 0ull;
           break;
         }
       }
-      if (t < 0) continue;
-      if (!((old_live[(int64_t)(t >> 6) * n + a] >> (t & 63)) & 1ull)) continue;
-      new_xi[(int64_t)s * n + b] = old_xi[(int64_t)t * n + a];
-      bits |= 1ull << (s & 63);
     }
-    new_live[(int64_t)w * n + b] = bits;
   }
+  a.live[b] = out0;
+  if (a.W64 > 1) a.live[n + b] = out1;
+  if (a.W64 > 2) a.live[2 * n + b] = out2;
+  if (a.W64 > 3) a.live[3 * n + b] = out3;
+}
+
+// A row ends at its first -1 (or at M slots); the slots after the
+// terminator are not written.
+JD void end_row(const NLW& a, int64_t b, int count) {
+  if (count < a.M) a.nl[(int64_t)count * a.n + b] = -1;
+  if (count > a.M) atomicOr(a.overflow, 1);
+  remap_row(a, b, min(count, a.M), [&](int s) { return a.nl[(int64_t)s * a.n + b]; });
+}
+
+// The first kRowLds slots of each row are collected in LDS, slot-major, and
+// written out one slot at a time for the whole workgroup, so the stores of a
+// slot are contiguous; later slots go straight to memory.
+constexpr int kRowLds = 24;
+
+template <int D, bool PER>
+__global__ void __launch_bounds__(kBlock) k_nlist_work(NLW a) {
+  __shared__ int32_t rows[kRowLds][kBlock];
+  __shared__ int row_max;
+  const int tid = threadIdx.x;
+  if (tid == 0) row_max = 0;
+  __syncthreads();
+  const int64_t b = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + tid;
+  const bool active = b < a.n;
+  int count = 0;
+  if (active) {
+    const Vec<D> xi = xyz<D>(a.P4[b]);
+    const uint32_t key = a.skeys[b];
+    const int d0 = a.grid.dims[0], d1 = a.grid.dims[1], d2 = (D == 3) ? a.grid.dims[2] : 1;
+    int c0, c1, cf;
+    if (D == 3) {
+      cf = (int)(key % (uint32_t)d2);
+      const uint32_t r = key / (uint32_t)d2;
+      c1 = (int)(r % (uint32_t)d1);
+      c0 = (int)(r / (uint32_t)d1);
+    } else {
+      cf = (int)(key % (uint32_t)d1);
+      c0 = (int)(key / (uint32_t)d1);
+      c1 = 0;
+    }
+    const int sr = a.grid.sr;
+    const int dF = (D == 3) ? d2 : d1;
+    int r0a, r0b, r1a, r1b;
+    stencil_runs<PER>(cf, sr, dF, r0a, r0b, r1a, r1b);
+    const int hi0 = PER ? min(sr, d0 - sr - 1) : sr;
+    const int lo1 = (D == 3) ? -sr : 0;
+    const int hi1 = (D == 3) ? (PER ? min(sr, d1 - sr - 1) : sr) : 0;
+    const bool chk_bond = (!a.isb) && a.bond_w > 0;
+    const int64_t ib = chk_bond ? (int64_t)a.orig[b] : 0;
+    const int32_t* __restrict__ starts = a.starts;
+    for (int ox = -sr; ox <= hi0; ++ox) {
+      const int cx = stencil_cell<PER>(c0, ox, d0);
+      if (cx < 0) continue;
+      for (int oy = lo1; oy <= hi1; ++oy) {
+        int base;
+        if (D == 3) {
+          const int cy = stencil_cell<PER>(c1, oy, d1);
+          if (cy < 0) continue;
+          base = (cx * d1 + cy) * d2;
+        } else {
+          base = cx * d1;
+        }
+        // both runs' bounds are loaded before either is walked
+        const int k0 = starts[base + r0a], k0e = starts[base + r0b + 1];
+        const int k1 = (r1b >= r1a) ? starts[base + r1a] : 0;
+        const int k1e = (r1b >= r1a) ? starts[base + r1b + 1] : 0;
+#pragma unroll
+        for (int run = 0; run < 2; ++run) {
+          const int kb = run ? k1 : k0, ke = run ? k1e : k0e;
+          // candidates in groups of four, loaded back to back
+          for (int kg = kb; kg < ke; kg += 4) {
+            float4 pk[4];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) pk[u] = (kg + u < ke) ? a.P4[kg + u] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+            for (int u = 0; u < 4; ++u) {
+              const int k = kg + u;
+              if (k >= ke) break;
+              if (k == b) continue;
+              const Vec<D> d = disp<PER>(xi, xyz<D>(pk[u]), a.dom);
+              if (!(dot(d, d) <= a.cutoff_sq)) continue;
+              if (chk_bond && bonded_o(a.bond_id, a.bond_w, ib, a.orig[k])) continue;
+              if (count < kRowLds)
+                rows[count][tid] = k;
+              else if (count < a.M)
+                a.nl[(int64_t)count * a.n + b] = k;
+              ++count;
+            }
+          }
+        }
+      }
+    }
+    atomicMax(&row_max, min(count, kRowLds));
+  }
+  __syncthreads();
+  if (!active) return;
+  const int c = min(count, a.M);
+  const int staged = min(row_max, a.M);
+  for (int s = 0; s < staged; ++s)
+    if (s < c) a.nl[(int64_t)s * a.n + b] = rows[s][tid];
+  if (count < a.M) a.nl[(int64_t)count * a.n + b] = -1;
+  if (count > a.M) atomicOr(a.overflow, 1);
+  remap_row(a, b, c, [&](int s) { return s < kRowLds ? rows[s][tid] : a.nl[(int64_t)s * a.n + b]; });
 }
 
 __global__ void __launch_bounds__(kBlock) k_copy_f4(float4* __restrict__ d, const float4* __restrict__ s, int64_t n) {
@@ -317,22 +441,26 @@ __global__ void __launch_bounds__(kBlock) k_iota(int32_t* __restrict__ p, int64_
   if (i < n) p[i] = (int32_t)i;
 }
 
-__global__ void __launch_bounds__(kBlock) k_fill_u64(uint64_t* __restrict__ p, int64_t n) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0ull;
-}
 
 // ---------------------------------------------------------------------------
 // the fused step on the work set
 // ---------------------------------------------------------------------------
 
+// Minimum waves per SIMD asked of the compiler: six (80 VGPRs) holds the
+// spring-friction law and the integrators without spills; the Cundall-Strack
+// and Hertz-friction laws need five.
+template <int D, int LAW>
+constexpr int ws_waves() {
+  return (D == 3 && (LAW == JDEM_LAW_CUNDALL_STRACK || LAW == JDEM_LAW_HERTZ_FRICTION)) ? 5 : 6;
+}
+
 template <int D, int LAW, bool PER>
-__global__ void __launch_bounds__(kBlock) k_ws_step(WS a) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_waves<D, LAW>(), 8))) k_ws_step(WS a) {
   if (a.flag_rd && *a.flag_rd) {
     if (a.flag_wr) *a.flag_wr = *a.flag_rd;
     return;
   }
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t b = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (b >= a.n) return;
   const float4 p4 = a.P[b], v4 = a.V[b], w4 = a.W[b], i4 = a.I[b];
   Side<D> me;
@@ -348,42 +476,56 @@ __global__ void __launch_bounds__(kBlock) k_ws_step(WS a) {
   if (a.do_force) {
     F = zero_vec<D>();
     T = zero_ang<D>();
-    // Batches of PF slots: the list entries, then their positions, are
-    // loaded back to back before any of them is used.
+    // Pass 1 marks the slots whose pair is within reach of a contact,
+    // d^2 <= R^2 * 1.0001 (a margin over the law's own overlap test); pass 2
+    // evaluates the law on the marked slots only, in slot order, so a wave
+    // runs the law as often as its busiest lane has contacts.  Pass 1 loads
+    // PF list entries, then their positions, back to back before using any.
     constexpr int PF = 4;
-    int w_cur = 0;
-    uint64_t live_old = 0ull, live_new = 0ull;
-    if constexpr (LawTraits<LAW>::hist) live_old = a.live[b];
+    uint64_t cm0 = 0ull, cm1 = 0ull, cm2 = 0ull, cm3 = 0ull;
     bool done = false;
     for (int s0 = 0; s0 < a.M && !done; s0 += PF) {
       int32_t js[PF];
       float4 pjs[PF];
 #pragma unroll
       for (int k = 0; k < PF; ++k) js[k] = (s0 + k < a.M) ? a.nl[(int64_t)(s0 + k) * a.n + b] : -1;
+      // the row ends at its first -1: what follows is stale
+#pragma unroll
+      for (int k = 1; k < PF; ++k) js[k] = js[k - 1] < 0 ? -1 : js[k];
 #pragma unroll
       for (int k = 0; k < PF; ++k) pjs[k] = (js[k] >= 0) ? a.P[js[k]] : make_float4(0.f, 0.f, 0.f, 0.f);
 #pragma unroll
       for (int k = 0; k < PF; ++k) {
-        if (done) break;
-        const int32_t j = js[k];
-        if (j < 0) {
+        if (js[k] < 0) {
           done = true;
           break;
-        }
-        const int s = s0 + k;
-        if constexpr (LawTraits<LAW>::hist) {
-          if ((s >> 6) != w_cur) {
-            a.live[(int64_t)w_cur * a.n + b] = live_new;
-            w_cur = s >> 6;
-            live_old = a.live[(int64_t)w_cur * a.n + b];
-            live_new = 0ull;
-          }
         }
         const float4 pj = pjs[k];
         const Vec<D> rij = disp_<PER>(me.x, xyz<D>(pj), a.dom);
         const float d_sq = dot(rij, rij);
         const float R = me.r + pj.w;
         if (d_sq > (R * R) * 1.0001f) continue;
+        const int s = s0 + k;
+        const uint64_t bit = 1ull << (s & 63);
+        const int ws = s >> 6;
+        cm0 |= ws == 0 ? bit : 0ull;
+        cm1 |= ws == 1 ? bit : 0ull;
+        cm2 |= ws == 2 ? bit : 0ull;
+        cm3 |= ws == 3 ? bit : 0ull;
+      }
+    }
+    const int n_words = LawTraits<LAW>::hist ? a.W64 : (a.M + 63) >> 6;
+    for (int w = 0; w < n_words; ++w) {
+      uint64_t cm = w == 0 ? cm0 : (w == 1 ? cm1 : (w == 2 ? cm2 : cm3));
+      uint64_t live_old = 0ull, live_new = 0ull;
+      if constexpr (LawTraits<LAW>::hist) live_old = a.live[(int64_t)w * a.n + b];
+      while (cm) {
+        const int s = w * 64 + (__ffsll((unsigned long long)cm) - 1);
+        cm &= cm - 1ull;
+        const int32_t j = a.nl[(int64_t)s * a.n + b];
+        const float4 pj = a.P[j];
+        const Vec<D> rij = disp_<PER>(me.x, xyz<D>(pj), a.dom);
+        const float d_sq = dot(rij, rij);
         Side<D> o;
         o.x = xyz<D>(pj);
         o.r = pj.w;
@@ -408,10 +550,7 @@ __global__ void __launch_bounds__(kBlock) k_ws_step(WS a) {
           }
         }
       }
-    }
-    if constexpr (LawTraits<LAW>::hist) {
-      a.live[(int64_t)w_cur * a.n + b] = live_new;
-      for (int w2 = w_cur + 1; w2 < a.W64; ++w2) a.live[(int64_t)w2 * a.n + b] = 0ull;
+      if constexpr (LawTraits<LAW>::hist) a.live[(int64_t)w * a.n + b] = live_new;
     }
   } else {
     F = xyz<D>(a.F[b]);
@@ -538,8 +677,7 @@ __global__ void __launch_bounds__(kBlock) k_nlist_work_naive(NLW a) {
     if (count < a.M) a.nl[(int64_t)count * a.n + b] = (int32_t)k;
     ++count;
   }
-  for (int t = count; t < a.M; ++t) a.nl[(int64_t)t * a.n + b] = -1;
-  if (count > a.M) atomicOr(a.overflow, 1);
+  end_row(a, b, count);
 }
 
 hipError_t ws_launch_in(const InOut& io, bool with_nl, hipStream_t s) {
@@ -560,10 +698,6 @@ hipError_t ws_launch_inverse(const int32_t* orig, int64_t n, int32_t* wpos, hipS
 }
 hipError_t ws_launch_iota(int32_t* p, int64_t n, hipStream_t s) {
   hipLaunchKernelGGL(k_iota, dim3(nblocks(n)), dim3(kBlock), 0, s, p, n);
-  return hipGetLastError();
-}
-hipError_t ws_launch_zero64(uint64_t* p, int64_t n, hipStream_t s) {
-  hipLaunchKernelGGL(k_fill_u64, dim3(nblocks(n)), dim3(kBlock), 0, s, p, n);
   return hipGetLastError();
 }
 hipError_t ws_launch_copy4(float4* d, const float4* src, int64_t n, hipStream_t s) {
@@ -590,9 +724,13 @@ hipError_t ws_launch_hash(const float4* P4, const int32_t* orig, int64_t n, int 
 hipError_t ws_launch_permute(int64_t n, const int32_t* new_orig, const int32_t* wpos, const float4* P,
                              const float4* V, const float4* W, const float4* Q, const float4* I, float4* P2,
                              float4* V2, float4* W2, float4* Q2, float4* I2, int32_t* orig2, int32_t* src,
-                             hipStream_t s) {
+                             int32_t* inv, hipStream_t s) {
   hipLaunchKernelGGL(k_permute, dim3(nblocks(n)), dim3(kBlock), 0, s, n, new_orig, wpos, P, V, W, Q, I, P2, V2, W2,
-                     Q2, I2, orig2, src);
+                     Q2, I2, orig2, src, inv);
+  return hipGetLastError();
+}
+hipError_t ws_launch_cell_starts(const uint32_t* skeys, int64_t n, int64_t n_cells, int32_t* starts, hipStream_t s) {
+  hipLaunchKernelGGL(k_cell_starts, dim3(nblocks(n_cells + 1)), dim3(kBlock), 0, s, skeys, n, n_cells, starts);
   return hipGetLastError();
 }
 hipError_t ws_launch_nlist(const NLW& a, int dim, bool naive, hipStream_t s) {
@@ -625,14 +763,6 @@ hipError_t ws_launch_nlist(const NLW& a, int dim, bool naive, hipStream_t s) {
   }
   return hipGetLastError();
 }
-hipError_t ws_launch_remap(int64_t n, int M, int W64, const int32_t* src, const int32_t* old_nl, const float4* old_xi,
-                           const uint64_t* old_live, const int32_t* new_nl, float4* new_xi, uint64_t* new_live,
-                           hipStream_t s) {
-  hipLaunchKernelGGL(k_remap_work, dim3(nblocks(n)), dim3(kBlock), 0, s, n, M, W64, src, old_nl, old_xi, old_live,
-                     new_nl, new_xi, new_live);
-  return hipGetLastError();
-}
-
 // FreeDomain box of float4 positions (free.py:42-58)
 __global__ void __launch_bounds__(kBlock) k_box_minmax4(const float4* __restrict__ P, int64_t n,
                                                          float* __restrict__ part) {
