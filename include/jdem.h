@@ -194,6 +194,34 @@ typedef struct jdem_step_stats {
   double free_anchor[3];
 } jdem_step_stats;
 
+/* Arguments of the FIRE optimiser, fire(dt, alpha_init, f_inc, f_dec,
+ * f_alpha, N_min, N_bad_max, dt_max_scale, dt_min_scale)
+ * (minimizers/optimizers.py:150-160); the reference's defaults are
+ * 0.1, 1.1, 0.5, 0.99, 5, 10, 10.0, 1e-3.  Given as Python floats (double):
+ * the optimiser scalars are float32, except that dt * dt_{max,min}_scale is
+ * a double product rounded once, as in the reference. */
+typedef struct jdem_fire {
+  double dt;
+  double alpha_init;
+  double f_inc;
+  double f_dec;
+  double f_alpha;
+  int32_t n_min;
+  int32_t n_bad_max;
+  double dt_max_scale;
+  double dt_min_scale;
+} jdem_fire;
+
+/* Result of jdem_minimize. */
+typedef struct jdem_minimize_stats {
+  int64_t steps;    /* iterations taken (routines.py:150-383 `steps`)      */
+  double pe;        /* final potential energy per particle (float64 sum)  */
+  double max_grad;  /* final max |force| / |torque| component             */
+  double power;     /* FIRE power F . v of the last iteration (float64)   */
+  int32_t overflow; /* collider overflow flag after the call               */
+  int32_t reserved;
+} jdem_minimize_stats;
+
 /* Kernel timing gathered with HIP events on the context's stream while
  * timing is enabled (jdem_ctx_set_timing).  Step kernels that found a
  * neighbour-list rebuild pending and did no work are counted apart. */
@@ -283,6 +311,20 @@ int jdem_step_stage(jdem_ctx* ctx, jdem_system* sys, int32_t stage);
  * arrays are updated in place; sys->nlist is updated in place for the
  * NeighborList collider.  stats may be NULL. */
 int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats* stats);
+
+/* ---- (d) energy minimisation -------------------------------------------------- */
+
+/* minimize(state, system, max_steps, pe_tol, pe_diff_tol, force_tol) with the
+ * FIRE optimiser (minimizers/routines.py:150-383, optimizers.py:150-339):
+ * one force/energy evaluation per iteration; stops when max_steps is
+ * reached, |pe|/N <= pe_tol, the relative change of pe drops below
+ * pe_diff_tol, or max |grad| <= force_tol.  Positions and orientations are
+ * updated in place (fixed particles keep their position), force/torque hold
+ * the last evaluation.  The potential energy and the FIRE power are float64
+ * sums on the GPU (the reference sums in the state dtype).  Gravity is
+ * applied and its energy counted; sys->ext_* must be NULL. */
+int jdem_minimize(jdem_ctx* ctx, jdem_system* sys, const jdem_fire* fire, int64_t max_steps, double pe_tol,
+                  double pe_diff_tol, double force_tol, jdem_minimize_stats* stats);
 
 #ifdef __cplusplus
 }

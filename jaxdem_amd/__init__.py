@@ -31,7 +31,8 @@ from .integrators import (
 from .materials import Material, MaterialMatchmaker, MaterialTable
 from .state import Quaternion, State
 from .system import System
-from . import utils
+from . import minimizers, utils
+from .minimizers import fire, minimize
 
 __version__ = "0.1.0"
 
@@ -65,5 +66,8 @@ __all__ = [
     "State",
     "Quaternion",
     "System",
+    "fire",
+    "minimize",
+    "minimizers",
     "utils",
 ]

@@ -149,6 +149,31 @@ class JdemTiming(ctypes.Structure):
     ]
 
 
+class JdemFire(ctypes.Structure):
+    _fields_ = [
+        ("dt", c_double),
+        ("alpha_init", c_double),
+        ("f_inc", c_double),
+        ("f_dec", c_double),
+        ("f_alpha", c_double),
+        ("n_min", c_int32),
+        ("n_bad_max", c_int32),
+        ("dt_max_scale", c_double),
+        ("dt_min_scale", c_double),
+    ]
+
+
+class JdemMinimizeStats(ctypes.Structure):
+    _fields_ = [
+        ("steps", c_int64),
+        ("pe", c_double),
+        ("max_grad", c_double),
+        ("power", c_double),
+        ("overflow", c_int32),
+        ("reserved", c_int32),
+    ]
+
+
 STRUCTS = {
     "jdem_timing": JdemTiming,
     "jdem_state": JdemState,
@@ -158,6 +183,8 @@ STRUCTS = {
     "jdem_nlist": JdemNlist,
     "jdem_system": JdemSystem,
     "jdem_step_stats": JdemStepStats,
+    "jdem_fire": JdemFire,
+    "jdem_minimize_stats": JdemMinimizeStats,
 }
 
 # name -> (restype, argtypes) for every entry point of include/jdem.h
@@ -185,6 +212,11 @@ SIGNATURES = {
     "jdem_kinetic_energy": (c_int, [c_void_p, POINTER(JdemSystem), POINTER(c_double)]),
     "jdem_step_stage": (c_int, [c_void_p, POINTER(JdemSystem), c_int32]),
     "jdem_system_step": (c_int, [c_void_p, POINTER(JdemSystem), c_int64, POINTER(JdemStepStats)]),
+    "jdem_minimize": (
+        c_int,
+        [c_void_p, POINTER(JdemSystem), POINTER(JdemFire), c_int64, c_double, c_double, c_double,
+         POINTER(JdemMinimizeStats)],
+    ),
 }
 
 _lib = None

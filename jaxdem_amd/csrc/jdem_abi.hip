@@ -21,7 +21,7 @@ std::vector<Buf*> jdem_ctx::bufs() {
   std::vector<Buf*> all = {&pos2,    &vel2,   &ang2,     &keys[0], &keys[1], &vals[0], &vals[1], &hist,
                            &cstart,  &cend,   &spos,     &nl_new,  &xi_new,  &live_new, &flags,  &red,
                            &nl_copy, &xi_copy, &live_copy, &wF,     &wT,      &wOld,     &wWpos, &wSrc,
-                           &wInv,    &wKeys};
+                           &wInv,    &wKeys,  &fireV,    &fireDev};
   for (int k = 0; k < 2; ++k)
     for (Buf* b : {&wP[k], &wV[k], &wW[k], &wQ[k], &wI[k], &wOrig[k], &wNl[k], &wXi[k], &wLive[k]}) all.push_back(b);
   return all;
@@ -476,6 +476,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   const DomP dom = dom_of(sys->domain);
   const GridP g = grid_of(L.grid);
   int cur = 0, set = 0, nlc = 0;
+  int xc = 0;  // springs: read wXi[xc], write wXi[1 - xc]
 
   // --- entry: order the work set by (cell, original id), convert inputs ---
   if (!naive2) {
@@ -571,13 +572,14 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
         b.src = src;
         b.inv = inv;
         b.old_nl = I32(ctx->wNl[nlc]);
-        b.old_xi = F4(ctx->wXi[nlc]);
+        b.old_xi = F4(ctx->wXi[xc]);
         b.old_live = U64(ctx->wLive[nlc]);
-        b.xi = F4(ctx->wXi[1 - nlc]);
+        b.xi = F4(ctx->wXi[1 - xc]);
         b.live = U64(ctx->wLive[1 - nlc]);
       }
       HIPCHK(ctx, ws_launch_nlist(b, D, naive2, st));
       nlc = 1 - nlc;
+      xc = 1 - xc;
     }
     HIPCHK(ctx, ws_launch_copy4(F4(ctx->wOld), F4(ctx->wP[cur]), N, st));
     L.n_build_times += 1;
@@ -589,7 +591,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   HIPCHK(ctx, launch_fill_i32(flags, 4, 0, st));
   float fbox[3] = {0, 0, 0}, fanc[3] = {0, 0, 0};
   bool free_done = false;
-  std::vector<int> cur_at((size_t)n + 2, 0);
+  std::vector<int> cur_at((size_t)n + 2, 0), xc_at((size_t)n + 2, 0);
 
   auto launch_t = [&](int64_t t) -> int {
     if (sys->domain.kind == JDEM_DOMAIN_FREE && t == n - 1 && !free_done) {
@@ -640,7 +642,9 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     a.T = F4(ctx->wT);
     a.old4 = F4(ctx->wOld);
     a.nl = I32(ctx->wNl[nlc]);
-    a.xi = F4(ctx->wXi[nlc]);
+    xc_at[t] = xc;
+    a.xi_rd = F4(ctx->wXi[xc]);
+    a.xi_wr = F4(ctx->wXi[1 - xc]);
     a.live = U64(ctx->wLive[nlc]);
     a.do_force = t >= 1;
     a.do_after = t >= 1;
@@ -656,6 +660,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     span_end(ctx, sp);
     if (e != hipSuccess) return fail(ctx, JDEM_EHIP, std::string("resident step kernel: ") + hipGetErrorString(e));
     cur = 1 - cur;
+    if (a.do_force) xc = 1 - xc;  // only a force pass rewrites the springs
     return JDEM_OK;
   };
 
@@ -698,7 +703,10 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       // kernels v.. of this chunk did no work: the data is where kernel v read
       // it.  When the chunk's last kernel raised the flag (v == next + cnt),
       // kernel v was never launched and `cur` already points at the data.
-      if (v < next + cnt) cur = cur_at[v];
+      if (v < next + cnt) {
+        cur = cur_at[v];
+        xc = xc_at[v];
+      }
       next = v;
       pending = true;
     }
@@ -713,7 +721,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   io.Q4 = F4(ctx->wQ[set]);
   io.I4 = F4(ctx->wI[set]);
   io.nl = I32(ctx->wNl[nlc]);
-  io.xi = F4(ctx->wXi[nlc]);
+  io.xi = F4(ctx->wXi[xc]);
   io.live = U64(ctx->wLive[nlc]);
   HIPCHK(ctx, ws_launch_out(io, M > 0, st));
   if (free_done) {
@@ -1024,6 +1032,141 @@ int jdem_step_stage(jdem_ctx* ctx, jdem_system* sys, int32_t stage) {
   }
   StepArgs a = base_args(sys);
   HIPCHK(ctx, launch_stage(a, sys->state.dim, stage, ctx->stream));
+  return JDEM_OK;
+}
+
+int jdem_minimize(jdem_ctx* ctx, jdem_system* sys, const jdem_fire* fire, int64_t max_steps, double pe_tol,
+                  double pe_diff_tol, double force_tol, jdem_minimize_stats* stats) {
+  int rc = check_system(ctx, sys);
+  if (rc) return rc;
+  if (!fire) return fail(ctx, JDEM_EINVAL, "null FIRE arguments");
+  if (max_steps < 0) return fail(ctx, JDEM_EINVAL, "max_steps < 0");
+  const bool hist_law = sys->law.kind == JDEM_LAW_SPRING_FRICTION || sys->law.kind == JDEM_LAW_HERTZ_FRICTION;
+  if (hist_law) return fail(ctx, JDEM_EUNSUPPORTED, "minimize with a history law");
+  if (sys->ext_force || sys->ext_force_com || sys->ext_torque)
+    return fail(ctx, JDEM_EUNSUPPORTED, "minimize with external force arrays");
+  (void)hipSetDevice(ctx->device);
+  jdem_state& s = sys->state;
+  const int64_t N = s.n;
+  const int D = s.dim, A = D == 3 ? 3 : 1;
+  hipStream_t st = ctx->stream;
+  const int nparts = std::max(1u, nblocks(N));
+  ENSURE(ctx, ctx->fireV, 4 * (size_t)std::max<int64_t>(N, 1) * (D + A));
+  ENSURE(ctx, ctx->fireDev, sizeof(FireDev) + 8 * (size_t)nparts);
+  ENSURE(ctx, ctx->red, 8 * ((size_t)std::max<int64_t>(N, 1) + 256 + 1));
+  FireArgs f;
+  std::memset(&f, 0, sizeof(f));
+  f.n = N;
+  f.dim = D;
+  f.p.dt0 = (float)fire->dt;
+  f.p.alpha_init = (float)fire->alpha_init;
+  f.p.f_inc = (float)fire->f_inc;
+  f.p.f_dec = (float)fire->f_dec;
+  f.p.f_alpha = (float)fire->f_alpha;
+  // dt * dt_{max,min}_scale are Python floats in the reference, rounded once
+  f.p.dt_max = (float)(fire->dt * fire->dt_max_scale);
+  f.p.dt_min = (float)(fire->dt * fire->dt_min_scale);
+  f.p.n_min = fire->n_min;
+  f.p.n_bad_max = fire->n_bad_max;
+  f.st = (FireDev*)ctx->fireDev.p;
+  f.part = (double*)((char*)ctx->fireDev.p + sizeof(FireDev));
+  f.nparts = nparts;
+  f.force = s.force;
+  f.torque = s.torque;
+  f.fixed = s.fixed;
+  f.vel_pos = (float*)ctx->fireV.p;
+  f.vel_rot = f.vel_pos + (size_t)std::max<int64_t>(N, 1) * D;
+  f.pos = s.pos;
+  f.q = s.q;
+  for (int c = 0; c < 3; ++c) f.g[c] = sys->gravity[c];
+  f.mass = s.mass;
+  double* per = (double*)ctx->red.p;
+  double* part = per + std::max<int64_t>(N, 1);
+  double* out = part + 256;
+  f.e_out = per;
+  const bool grav = sys->gravity[0] != 0.0f || sys->gravity[1] != 0.0f || sys->gravity[2] != 0.0f;
+  // optimiser state: zero velocities, dt = dt0, alpha = alpha_init (optimizers.py:231-239)
+  HIPCHK(ctx, hipMemsetAsync(ctx->fireV.p, 0, 4 * (size_t)std::max<int64_t>(N, 1) * (D + A), st));
+  FireDev h0;
+  std::memset(&h0, 0, sizeof(h0));
+  h0.dt = (float)fire->dt;
+  h0.alpha = (float)fire->alpha_init;
+  HIPCHK(ctx, hipMemcpyAsync(f.st, &h0, sizeof(h0), hipMemcpyHostToDevice, st));
+
+  // eval_step (routines.py:228-245): force, ForceManager, energy, max |grad|
+  double pe = 0.0, gmax = 0.0;
+  auto eval = [&]() -> int {
+    if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST) {
+      bool rebuilt = false;
+      int r = nl_check_and_rebuild(ctx, sys, &rebuilt);
+      if (r) return r;
+    }
+    StepArgs a = base_args(sys);
+    if (sys->collider_kind == JDEM_COLLIDER_NONE) {
+      HIPCHK(ctx, hipMemsetAsync(s.force, 0, 4 * (size_t)N * D, st));
+      HIPCHK(ctx, hipMemsetAsync(s.torque, 0, 4 * (size_t)N * A, st));
+    } else if (N > 0) {
+      int r = prepare_source(ctx, sys, a, s.pos);
+      if (r) return r;
+      a.do_force = 1;
+      HIPCHK(ctx, launch_step(a, D, sys->law.kind, src_of(sys->collider_kind), st));
+    }
+    if (N > 0) HIPCHK(ctx, launch_stage(a, D, JDEM_STAGE_FORCE_MANAGER, st));
+    HIPCHK(ctx, hipMemsetAsync(&f.st->max_grad_bits, 0, sizeof(unsigned), st));
+    HIPCHK(ctx, launch_fire_grad_max(f, st));
+    double h[2] = {0.0, 0.0};
+    if (N > 0) {
+      if (sys->collider_kind != JDEM_COLLIDER_NONE) {
+        a.e_out = per;
+        HIPCHK(ctx, launch_energy(a, D, sys->law.kind, src_of(sys->collider_kind), st));
+      } else {
+        HIPCHK(ctx, hipMemsetAsync(per, 0, 8 * (size_t)N, st));
+      }
+      if (grav) HIPCHK(ctx, launch_fire_gravity_energy(f, st));
+      HIPCHK(ctx, launch_sum_f64(per, N, 1, 1, part, out, 256, st));
+      HIPCHK(ctx, hipMemcpyAsync(&h[0], out, 8, hipMemcpyDeviceToHost, st));
+    }
+    unsigned bits = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&bits, &f.st->max_grad_bits, sizeof(bits), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    float g;
+    std::memcpy(&g, &bits, sizeof(g));
+    pe = h[0];
+    gmax = (double)g;
+    return JDEM_OK;
+  };
+
+  rc = eval();
+  if (rc) return rc;
+  double prev = INFINITY;
+  int64_t steps = 0;
+  const double tiny = 2.2250738585072014e-308;  // smallest normal float64
+  const double dN = (double)std::max<int64_t>(N, 1);
+  while (steps < max_steps) {
+    const bool conv_pe = std::fabs(pe / dN) <= pe_tol;
+    const double denom = std::max(std::max(std::fabs(pe), std::fabs(prev)), tiny);
+    const bool conv_rel = std::fabs(pe - prev) / denom < pe_diff_tol;
+    const bool conv_f = gmax <= force_tol;
+    if (conv_pe || conv_rel || conv_f) break;
+    HIPCHK(ctx, launch_fire_power(f, st));
+    HIPCHK(ctx, launch_fire_decide(f, st));
+    HIPCHK(ctx, launch_fire_move(f, st));
+    HIPCHK(ctx, launch_fire_commit(f, st));  // dt, alpha <- the values just decided
+    prev = pe;
+    rc = eval();
+    if (rc) return rc;
+    ++steps;
+  }
+  if (stats) {
+    FireDev h;
+    HIPCHK(ctx, hipMemcpyAsync(&h, f.st, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    stats->steps = steps;
+    stats->pe = pe / dN;
+    stats->max_grad = gmax;
+    stats->power = h.power;
+    stats->overflow = sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST ? sys->nlist.overflow : 0;
+  }
   return JDEM_OK;
 }
 

@@ -428,8 +428,7 @@ __global__ void __launch_bounds__(kBlock) k_nlist_cells(NLBuild b) {
         for (int k = k0; k < k1; ++k) {
           const int32_t j = b.perm[k];
           if (j == i) continue;
-          const Vec<D> d = disp<PER>(xi, v4<D>(b.spos[k]), b.dom);
-          if (!(dot(d, d) <= b.cutoff_sq)) continue;
+          if (!(dist_sq<PER>(xi, v4<D>(b.spos[k]), b.dom) <= b.cutoff_sq)) continue;
           if (chk_bond && bonded(b.bond_id, b.bond_w, i, j)) continue;
           if (count < b.M) row[count] = j;
           ++count;

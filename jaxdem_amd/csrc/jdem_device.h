@@ -194,6 +194,27 @@ JD float disp1(float a, float b, float B) {
   if (PER) r = r - B * rintf(r / B);
   return r;
 }
+// Domain.displacement for squared distances only.  When |r| <= B/4, r/B
+// rounds to a value of magnitude <= 1/4, round(r/B) is +-0 and the result is
+// r up to the sign of a zero, which a square does not see; the division is
+// needed only for pairs across the periodic boundary.
+template <bool PER>
+JD float disp1_sq(float a, float b, float B) {
+  float r = a - b;
+  if (PER && !(fabsf(r) <= 0.25f * B)) r = r - B * rintf(r / B);
+  return r;
+}
+template <bool PER>
+JD float dist_sq(Vec<3> a, Vec<3> b, const DomP& d) {
+  const Vec<3> r = {disp1_sq<PER>(a.x, b.x, d.box[0]), disp1_sq<PER>(a.y, b.y, d.box[1]),
+                    disp1_sq<PER>(a.z, b.z, d.box[2])};
+  return dot(r, r);
+}
+template <bool PER>
+JD float dist_sq(Vec<2> a, Vec<2> b, const DomP& d) {
+  const Vec<2> r = {disp1_sq<PER>(a.x, b.x, d.box[0]), disp1_sq<PER>(a.y, b.y, d.box[1])};
+  return dot(r, r);
+}
 template <bool PER>
 JD Vec<3> disp_(Vec<3> a, Vec<3> b, const DomP& d) {
   return {disp1_<PER>(a.x, b.x, d.box[0], d.inv_box[0]), disp1_<PER>(a.y, b.y, d.box[1], d.inv_box[1]),

@@ -1,0 +1,257 @@
+// jdem_fire.hip -- the FIRE energy minimiser on the GPU
+// (minimizers/optimizers.py:150-339, minimizers/routines.py:150-383).
+//
+// One iteration is four launches around the force evaluation:
+//   k_fire_power   per particle: F = -grad (masked by ~fixed), v_old =
+//                  v + F dt/2; block partial sums of F . v_old in float64
+//   k_fire_decide  one block: the power, then the downhill / uphill update of
+//                  dt, alpha, N_good, N_bad (optimizers.py:250-296)
+//   k_fire_move    per particle: velocity mixing, the parameter update and
+//                  the new orientation (routines.py:341-360, 37-65)
+//   k_fire_commit  dt, alpha <- the values just decided
+// The optimiser scalars stay on the device (FireDev); the host reads only the
+// energy and the largest gradient component once per iteration.  Every
+// float32 expression follows the reference's operation order, so positions
+// match the float32 oracle bit for bit while the power decisions agree.
+#include <hip/hip_runtime.h>
+
+#include "jdem_internal.h"
+
+namespace jdem {
+
+namespace {
+
+// optax.safe_norm(x, 1e-16, axis=-1): the row norm, or 1e-16 where the norm
+// is not above it
+template <int A>
+JD float safe_norm(const float* x) {
+  float n2;
+  if (A == 3)
+    n2 = (x[0] * x[0] + x[1] * x[1]) + x[2] * x[2];
+  else if (A == 2)
+    n2 = x[0] * x[0] + x[1] * x[1];
+  else
+    n2 = x[0] * x[0];
+  const float n = sqrtf(n2);
+  return n <= 1e-16f ? 1e-16f : n;
+}
+
+// Quaternion.from_rotvec (quaternion.py:99-127)
+JD Quat q_from_rotvec(Vec<3> r) {
+  const float n2 = dot(r, r);
+  const float safe = fmaxf(n2, 1e-16f);
+  const float theta = sqrtf(safe);
+  const float half = 0.5f * theta;
+  const float ch = cosf(half);
+  const float sc = sinf(half) / theta;
+  return {ch, r.x * sc, r.y * sc, r.z * sc};
+}
+
+// One parameter leaf of one particle (A components): F, v_old and the
+// velocity-mixing update of optimizers.py:245-318.
+template <int A>
+struct Leaf {
+  float f[A], v_old[A];
+  JD void load(const float* F, const float* V, int64_t i, float m, float dt) {
+#pragma unroll
+    for (int c = 0; c < A; ++c) {
+      f[c] = -((-F[A * i + c]) * m);  // F = -(grad * mask), grad = -force
+      v_old[c] = V[A * i + c] + (f[c] * dt) / 2.0f;
+    }
+  }
+  JD double power() const {
+    double p = 0.0;
+#pragma unroll
+    for (int c = 0; c < A; ++c) p += (double)f[c] * (double)v_old[c];
+    return p;
+  }
+  // returns the update; writes the new velocity
+  JD void mix(const FireDev& s, float* V, int64_t i, float* upd) const {
+    float vh[A];
+#pragma unroll
+    for (int c = 0; c < A; ++c) vh[c] = v_old[c] * s.vs + (f[c] * s.new_dt) / 2.0f;
+    const float vn = safe_norm<A>(vh);
+    const float fn = safe_norm<A>(f);
+    const float mixr = fn > 1e-16f ? (vn / fn) * s.new_alpha : 0.0f;
+#pragma unroll
+    for (int c = 0; c < A; ++c) {
+      vh[c] = (vh[c] * (1.0f - s.new_alpha) + f[c] * mixr) * s.vs;
+      upd[c] = (v_old[c] * s.dt_rev) / 2.0f + (vh[c] * s.new_dt) / 2.0f;
+      V[A * i + c] = vh[c];
+    }
+  }
+};
+
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_fire_power(FireArgs a) {
+  constexpr int A = D == 3 ? 3 : 1;
+  __shared__ double sh[kBlock];
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  double p = 0.0;
+  if (i < a.n) {
+    const float m = a.fixed[i] ? 0.0f : 1.0f;
+    const float dt = a.st->dt;
+    Leaf<D> lp;
+    Leaf<A> lr;
+    lp.load(a.force, a.vel_pos, i, m, dt);
+    lr.load(a.torque, a.vel_rot, i, m, dt);
+    p = lp.power() + lr.power();
+  }
+  sh[threadIdx.x] = p;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) a.part[blockIdx.x] = sh[0];
+}
+
+__global__ void __launch_bounds__(kBlock) k_fire_decide(FireArgs a) {
+  __shared__ double sh[kBlock];
+  double acc = 0.0;
+  for (int b = threadIdx.x; b < a.nparts; b += kBlock) acc += a.part[b];
+  sh[threadIdx.x] = acc;
+  __syncthreads();
+  for (int off = kBlock / 2; off > 0; off >>= 1) {
+    if ((int)threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+    __syncthreads();
+  }
+  if (threadIdx.x != 0) return;
+  FireDev s = *a.st;
+  const double power = sh[0];
+  const float cand_inc = fminf(s.dt * a.p.f_inc, a.p.dt_max);
+  const float cand_dec = fmaxf(s.dt * a.p.f_dec, a.p.dt_min);
+  if (power > 0.0) {  // downhill (optimizers.py:278-292)
+    const int ng = s.n_good + 1;
+    s.new_dt = ng > a.p.n_min ? cand_inc : s.dt;
+    s.new_alpha = ng > a.p.n_min ? s.alpha * a.p.f_alpha : s.alpha;
+    s.n_good = ng;
+    s.n_bad = 0;
+    s.dt_rev = 0.0f;
+    s.vs = 1.0f;
+  } else {  // uphill (optimizers.py:258-276)
+    int nb = s.n_bad + 1;
+    const bool exceeded = nb > a.p.n_bad_max;
+    s.new_dt = exceeded ? a.p.dt0 : cand_dec;
+    s.n_bad = exceeded ? 0 : nb;
+    s.new_alpha = a.p.alpha_init;
+    s.n_good = 0;
+    s.dt_rev = -s.new_dt;
+    s.vs = 0.0f;
+  }
+  s.power = power;
+  *a.st = s;
+}
+
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_fire_move(FireArgs a) {
+  constexpr int A = D == 3 ? 3 : 1;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const FireDev s = *a.st;
+  const bool fixed = a.fixed[i] != 0;
+  const float m = fixed ? 0.0f : 1.0f;
+  Leaf<D> lp;
+  Leaf<A> lr;
+  lp.load(a.force, a.vel_pos, i, m, s.dt);
+  lr.load(a.torque, a.vel_rot, i, m, s.dt);
+  float up[D], ur[A];
+  lp.mix(s, a.vel_pos, i, up);
+  lr.mix(s, a.vel_rot, i, ur);
+  // updates * mask, then where(mask, params + updates, params)
+  if (!fixed)
+#pragma unroll
+    for (int c = 0; c < D; ++c) a.pos[D * i + c] = a.pos[D * i + c] + up[c] * m;
+  float rv[A];
+#pragma unroll
+  for (int c = 0; c < A; ++c) rv[c] = fixed ? 0.0f : 0.0f + ur[c] * m;
+  const Vec<3> r3 = (D == 3) ? Vec<3>{rv[0], rv[A > 1 ? 1 : 0], rv[A > 2 ? 2 : 0]} : Vec<3>{0.0f, 0.0f, rv[0]};
+  const float4 q4 = reinterpret_cast<const float4*>(a.q)[i];
+  const Quat q = {q4.x, q4.y, q4.z, q4.w};
+  const Quat qn = q_unit(q_mul(q_from_rotvec(r3), q));
+  reinterpret_cast<float4*>(a.q)[i] = make_float4(qn.w, qn.x, qn.y, qn.z);
+}
+
+__global__ void k_fire_commit(FireArgs a) {
+  if (threadIdx.x != 0) return;
+  a.st->dt = a.st->new_dt;
+  a.st->alpha = a.st->new_alpha;
+}
+
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_fire_grad_max(FireArgs a) {
+  constexpr int A = D == 3 ? 3 : 1;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  float m = 0.0f;
+  if (i < a.n) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) m = fmaxf(m, fabsf(a.force[D * i + c]));
+#pragma unroll
+    for (int c = 0; c < A; ++c) m = fmaxf(m, fabsf(a.torque[A * i + c]));
+    // a NaN component makes the maximum NaN, as jnp.max does
+    for (int c = 0; c < D; ++c)
+      if (a.force[D * i + c] != a.force[D * i + c]) m = __int_as_float(0x7fc00000);
+    for (int c = 0; c < A; ++c)
+      if (a.torque[A * i + c] != a.torque[A * i + c]) m = __int_as_float(0x7fc00000);
+  }
+  // non-negative floats (and the quiet NaN above) order like their bits
+  unsigned bits = (unsigned)__float_as_int(m);
+  for (int off = 32; off > 0; off >>= 1) bits = max(bits, (unsigned)__shfl_xor((int)bits, off));
+  if ((threadIdx.x & 63) == 0) atomicMax(&a.st->max_grad_bits, bits);
+}
+
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_fire_gravity_energy(FireArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  float gx;
+  if (D == 3)
+    gx = (a.g[0] * a.pos[3 * i] + a.g[1] * a.pos[3 * i + 1]) + a.g[2] * a.pos[3 * i + 2];
+  else
+    gx = a.g[0] * a.pos[2 * i] + a.g[1] * a.pos[2 * i + 1];
+  a.e_out[i] += (double)(-(gx * a.mass[i]));
+}
+
+}  // namespace
+
+hipError_t launch_fire_power(const FireArgs& a, hipStream_t s) {
+  if (a.dim == 3)
+    hipLaunchKernelGGL(k_fire_power<3>, dim3(a.nparts), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_fire_power<2>, dim3(a.nparts), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_fire_decide(const FireArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_fire_decide, dim3(1), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_fire_move(const FireArgs& a, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  if (a.dim == 3)
+    hipLaunchKernelGGL(k_fire_move<3>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_fire_move<2>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_fire_commit(const FireArgs& a, hipStream_t s) {
+  hipLaunchKernelGGL(k_fire_commit, dim3(1), dim3(64), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_fire_grad_max(const FireArgs& a, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  if (a.dim == 3)
+    hipLaunchKernelGGL(k_fire_grad_max<3>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_fire_grad_max<2>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_fire_gravity_energy(const FireArgs& a, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  if (a.dim == 3)
+    hipLaunchKernelGGL(k_fire_gravity_energy<3>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_fire_gravity_energy<2>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+
+}  // namespace jdem

@@ -46,7 +46,7 @@ struct jdem_ctx {
   jdem::Buf red;                       // reduction partials
   jdem::Buf nl_copy, xi_copy, live_copy;
   // resident NeighborList working set (jdem_resident.hip)
-  jdem::Buf wP[2], wV[2], wW[2], wQ[2], wI[2], wF, wT, wOld, wOrig[2], wWpos, wSrc, wInv, wKeys, wNl[2], wXi[2], wLive[2];
+  jdem::Buf wP[2], wV[2], wW[2], wQ[2], wI[2], wF, wT, wOld, wOrig[2], wWpos, wSrc, wInv, wKeys, fireV, fireDev, wNl[2], wXi[2], wLive[2];
   int32_t* host_flags = nullptr;       // pinned host mirror of flags
   // HIP-event timing (jdem_ctx_set_timing)
   struct TimedSpan {
@@ -277,7 +277,8 @@ struct WS {
   float4* T;
   const float4* old4;
   const int32_t* nl;
-  float4* xi;
+  const float4* xi_rd;  // springs by rank, read
+  float4* xi_wr;        // springs by rank, written (another buffer)
   uint64_t* live;
   int do_force, do_after, do_before, store_ft;
   float trigger;
@@ -302,6 +303,49 @@ hipError_t ws_launch_cell_starts(const uint32_t* skeys, int64_t n, int64_t n_cel
 hipError_t ws_launch_nlist(const NLW& a, int dim, bool naive, hipStream_t s);
 hipError_t ws_launch_step(const WS& a, int dim, int law, hipStream_t s);
 hipError_t ws_launch_box(const float4* P, int64_t n, float* part, int nparts, hipStream_t s);
+
+// ---- FIRE minimiser (jdem_fire.hip) -----------------------------------------
+struct FireP {
+  float dt0, alpha_init, f_inc, f_dec, f_alpha;
+  float dt_max, dt_min;  // dt0 * dt_{max,min}_scale, rounded once to float
+  int n_min, n_bad_max;
+};
+// Optimiser scalars, kept on the device between the kernels of an iteration.
+struct FireDev {
+  float dt, alpha;
+  int n_good, n_bad;
+  float new_dt, new_alpha, dt_rev, vs;
+  double power;
+  unsigned max_grad_bits;  // max |grad| component as float bits
+  int pad;
+};
+struct FireArgs {
+  int64_t n;
+  int dim;
+  FireP p;
+  FireDev* st;
+  const float* force;
+  const float* torque;
+  const uint8_t* fixed;
+  float* vel_pos;
+  float* vel_rot;
+  float* pos;
+  float* q;
+  double* part;
+  int nparts;
+  // potential energy of gravity (force_manager.py:450-459)
+  float g[3];
+  const float* mass;
+  double* e_out;
+};
+hipError_t launch_fire_power(const FireArgs& a, hipStream_t s);
+hipError_t launch_fire_decide(const FireArgs& a, hipStream_t s);
+hipError_t launch_fire_move(const FireArgs& a, hipStream_t s);
+hipError_t launch_fire_commit(const FireArgs& a, hipStream_t s);
+// max |force| / |torque| component into st->max_grad_bits; gravity energy
+// added into e_out (per particle) when g != 0
+hipError_t launch_fire_grad_max(const FireArgs& a, hipStream_t s);
+hipError_t launch_fire_gravity_energy(const FireArgs& a, hipStream_t s);
 
 inline unsigned nblocks(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
 

@@ -8,9 +8,12 @@
 // in "work" order b: particles sorted by (cell, original id) at every list
 // rebuild, so a particle's candidates are its memory neighbours.  orig[b] is
 // the original index of work slot b.  The neighbour list is slot-major
-// nl[s*N + b] (work indices), the tangential springs xi[s*N + b] float4, and
-// one 64-bit live mask per particle and 64 slots: every per-slot access of a
-// wavefront is contiguous.  Sorting by (cell, original id) makes each row the
+// nl[s*N + b] (work indices) and ends at its first -1; one 64-bit live mask
+// per particle and 64 slots marks the slots that carry a tangential spring.
+// The springs themselves are stored by rank: xi[k*N + b] is the spring of
+// the k-th live slot of particle b, so the rows a wavefront touches are the
+// first few, densely used ones.  xi is double-buffered (read one, write the
+// other) because a step's new ranks can overtake its old ones.  Sorting by (cell, original id) makes each row the
 // reference's row (stencil order, ascending original id within a cell), so
 // sums and history match the CPU oracle bit for bit.
 //
@@ -87,6 +90,7 @@ __global__ void __launch_bounds__(kBlock) k_in_nlist(InOut a) {
   const int64_t i = a.orig[b];
   const int W32 = (a.M + 31) / 32;
   for (int w = 0; w < a.W64; ++w) a.live[(int64_t)w * a.n + b] = 0ull;
+  int rank = 0;
   for (int s = 0; s < a.M; ++s) {
     int32_t j = a.nl_user[i * a.M + s];
     if (j >= a.n) j = -1;  // not a particle of this state: treated as padding
@@ -96,7 +100,8 @@ __global__ void __launch_bounds__(kBlock) k_in_nlist(InOut a) {
       if (lv) {
         float x[3] = {0, 0, 0};
         for (int c = 0; c < a.dim; ++c) x[c] = a.xi_user[(i * a.M + s) * a.dim + c];
-        a.xi[(int64_t)s * a.n + b] = make_float4(x[0], x[1], x[2], 0.0f);
+        a.xi[(int64_t)rank * a.n + b] = make_float4(x[0], x[1], x[2], 0.0f);
+        ++rank;
         a.live[(int64_t)(s >> 6) * a.n + b] |= 1ull << (s & 63);
       }
     }
@@ -133,6 +138,7 @@ __global__ void __launch_bounds__(kBlock) k_out_nlist(InOut a) {
   uint32_t words[8];
   for (int w = 0; w < W32 && w < 8; ++w) words[w] = 0u;
   bool ended = false;  // work rows end at their first -1
+  int rank = 0;
   for (int s = 0; s < a.M; ++s) {
     const int32_t j = ended ? -1 : a.nl[(int64_t)s * a.n + b];
     ended = j < 0;
@@ -140,7 +146,8 @@ __global__ void __launch_bounds__(kBlock) k_out_nlist(InOut a) {
     if (a.xi_user) {
       const bool lv = j >= 0 && ((a.live[(int64_t)(s >> 6) * a.n + b] >> (s & 63)) & 1ull);
       if (lv) {
-        const float4 x = a.xi[(int64_t)s * a.n + b];
+        const float4 x = a.xi[(int64_t)rank * a.n + b];
+        ++rank;
         const float xx[3] = {x.x, x.y, x.z};
         for (int c = 0; c < a.dim; ++c) a.xi_user[(i * a.M + s) * a.dim + c] = xx[c];
         if ((s >> 5) < 8) words[s >> 5] |= 1u << (s & 31);
@@ -286,13 +293,33 @@ JD int stencil_cell(int c, int o, int dims) {
   return (x < 0 || x >= dims) ? -1 : x;
 }
 
+// Number of set bits of the live words (w0..w3) below slot s.
+JD int rank_below(uint64_t w0, uint64_t w1, uint64_t w2, uint64_t w3, int s) {
+  const int ws = s >> 6;
+  const uint64_t below = (1ull << (s & 63)) - 1ull;
+  int r = 0;
+  r += ws > 0 ? __popcll(w0) : __popcll(w0 & below);
+  if (ws >= 1) r += ws > 1 ? __popcll(w1) : __popcll(w1 & below);
+  if (ws >= 2) r += ws > 2 ? __popcll(w2) : __popcll(w2 & below);
+  if (ws >= 3) r += __popcll(w3 & below);
+  return r;
+}
+
 // History of a freshly built row (neighbor_list.py:29-54): the history of
 // new pair (b, j) is the old history of the same two particles, zero when
 // they were not listed together.  Only live old slots carry history, so each
 // one is looked up in the new row (row(s), s < nrow): old slot t of src[b]
 // lists old work slot o, now at inv[o].  A pair occurs at most once per row,
-// so the lookups are one-to-one.  Rows hold at most 256 slots (four live
-// words).
+// so the lookups are one-to-one.  The first sweep finds the new live slots,
+// the second copies each spring from its old rank to its new one.  Rows hold
+// at most 256 slots (four live words).
+template <class Row>
+JD int find_in_row(int32_t tgt, int nrow, Row row) {
+  for (int s = 0; s < nrow; ++s)
+    if (row(s) == tgt) return s;
+  return -1;
+}
+
 template <class Row>
 JD void remap_row(const NLW& a, int64_t b, int nrow, Row row) {
   if (!a.live) return;
@@ -300,24 +327,29 @@ JD void remap_row(const NLW& a, int64_t b, int nrow, Row row) {
   uint64_t out0 = 0ull, out1 = 0ull, out2 = 0ull, out3 = 0ull;
   if (a.remap) {
     const int64_t ob = a.src[b];
-    for (int w = 0; w < a.W64; ++w) {
-      uint64_t bits = a.old_live[(int64_t)w * n + ob];
-      while (bits) {
-        const int t = w * 64 + (__ffsll((unsigned long long)bits) - 1);
-        bits &= bits - 1ull;
-        const int32_t o = a.old_nl[(int64_t)t * n + ob];
-        if (o < 0) continue;
-        const int32_t tgt = a.inv[o];
-        for (int s = 0; s < nrow; ++s) {
-          if (row(s) != tgt) continue;
-          a.xi[(int64_t)s * n + b] = a.old_xi[(int64_t)t * n + ob];
-          const uint64_t bit = 1ull << (s & 63);
-          const int ws = s >> 6;
-          out0 |= ws == 0 ? bit : 0ull;
-          out1 |= ws == 1 ? bit : 0ull;
-          out2 |= ws == 2 ? bit : 0ull;
-          out3 |= ws == 3 ? bit : 0ull;
-          break;
+    for (int sweep = 0; sweep < 2; ++sweep) {
+      int rank_old = 0;
+      for (int w = 0; w < a.W64; ++w) {
+        uint64_t bits = a.old_live[(int64_t)w * n + ob];
+        while (bits) {
+          const int t = w * 64 + (__ffsll((unsigned long long)bits) - 1);
+          bits &= bits - 1ull;
+          const int32_t o = a.old_nl[(int64_t)t * n + ob];
+          const int s = o < 0 ? -1 : find_in_row(a.inv[o], nrow, row);
+          if (s >= 0) {
+            if (sweep == 0) {
+              const uint64_t bit = 1ull << (s & 63);
+              const int ws = s >> 6;
+              out0 |= ws == 0 ? bit : 0ull;
+              out1 |= ws == 1 ? bit : 0ull;
+              out2 |= ws == 2 ? bit : 0ull;
+              out3 |= ws == 3 ? bit : 0ull;
+            } else {
+              const int rank_new = rank_below(out0, out1, out2, out3, s);
+              a.xi[(int64_t)rank_new * n + b] = a.old_xi[(int64_t)rank_old * n + ob];
+            }
+          }
+          ++rank_old;
         }
       }
     }
@@ -405,8 +437,7 @@ __global__ void __launch_bounds__(kBlock) k_nlist_work(NLW a) {
               const int k = kg + u;
               if (k >= ke) break;
               if (k == b) continue;
-              const Vec<D> d = disp<PER>(xi, xyz<D>(pk[u]), a.dom);
-              if (!(dot(d, d) <= a.cutoff_sq)) continue;
+              if (!(dist_sq<PER>(xi, xyz<D>(pk[u]), a.dom) <= a.cutoff_sq)) continue;
               if (chk_bond && bonded_o(a.bond_id, a.bond_w, ib, a.orig[k])) continue;
               if (count < kRowLds)
                 rows[count][tid] = k;
@@ -515,10 +546,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
       }
     }
     const int n_words = LawTraits<LAW>::hist ? a.W64 : (a.M + 63) >> 6;
+    int rank_old_base = 0, rank_new = 0;  // springs are stored by rank
     for (int w = 0; w < n_words; ++w) {
       uint64_t cm = w == 0 ? cm0 : (w == 1 ? cm1 : (w == 2 ? cm2 : cm3));
       uint64_t live_old = 0ull, live_new = 0ull;
       if constexpr (LawTraits<LAW>::hist) live_old = a.live[(int64_t)w * a.n + b];
+      const int rank_base = rank_old_base;
+      if constexpr (LawTraits<LAW>::hist) rank_old_base += __popcll(live_old);
       while (cm) {
         const int s = w * 64 + (__ffsll((unsigned long long)cm) - 1);
         cm &= cm - 1ull;
@@ -538,14 +572,18 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
         }
         Vec<D> xold = zero_vec<D>();
         if constexpr (LawTraits<LAW>::hist) {
-          if ((live_old >> (s & 63)) & 1ull) xold = xyz<D>(a.xi[(int64_t)s * a.n + b]);
+          if ((live_old >> (s & 63)) & 1ull) {
+            const int k = rank_base + __popcll(live_old & ((1ull << (s & 63)) - 1ull));
+            xold = xyz<D>(a.xi_rd[(int64_t)k * a.n + b]);
+          }
         }
         PairOut<D> r = law_eval<D, LAW>(a.law, me, o, rij, d_sq, xold, a.dt);
         F = F + r.F;
         if constexpr (LawTraits<LAW>::torque) T = T + r.T;
         if constexpr (LawTraits<LAW>::hist) {
           if (r.contact) {
-            a.xi[(int64_t)s * a.n + b] = f4(r.xi, 0.0f);
+            a.xi_wr[(int64_t)rank_new * a.n + b] = f4(r.xi, 0.0f);
+            ++rank_new;
             live_new |= 1ull << (s & 63);
           }
         }

@@ -1212,6 +1212,189 @@ def pair_virial(st: State, P: Params, C: np.ndarray) -> np.ndarray:
 __all__ = [name for name in dir() if not name.startswith("_")]
 
 
+# --------------------------------------------------------------------------
+# FIRE energy minimisation (minimizers/optimizers.py:150-339,
+# minimizers/routines.py:24-383), spheres
+# --------------------------------------------------------------------------
+
+
+@dataclass
+class FireParams:
+    """Arguments of ``fire(dt, ...)`` (optimizers.py:150-160)."""
+
+    dt: float
+    alpha_init: float = 0.1
+    f_inc: float = 1.1
+    f_dec: float = 0.5
+    f_alpha: float = 0.99
+    N_min: int = 5
+    N_bad_max: int = 10
+    dt_max_scale: float = 10.0
+    dt_min_scale: float = 1e-3
+
+
+@dataclass
+class FireState:
+    """``FIREState`` (optimizers.py:222-230): velocities of the two parameter
+    leaves (pos_c, rotvec) and the scalars, in the parameter dtype."""
+
+    vel_pos: np.ndarray
+    vel_rot: np.ndarray
+    dt: Any
+    alpha: Any
+    N_good: int = 0
+    N_bad: int = 0
+
+
+def fire_init(st: State, fp: FireParams) -> FireState:
+    T_ = st.pos.dtype
+    return FireState(
+        vel_pos=np.zeros_like(st.pos),
+        vel_rot=np.zeros_like(st.torque),
+        dt=T_.type(fp.dt),
+        alpha=T_.type(fp.alpha_init),
+    )
+
+
+def safe_norm_rows(x: np.ndarray, min_norm: float) -> np.ndarray:
+    """``optax.safe_norm(x, min_norm, axis=-1, keepdims=True)``: the row norm,
+    or ``min_norm`` where the norm is not above it."""
+    T_ = x.dtype
+    if x.shape[-1] == 1:
+        n = np.sqrt(x[..., 0] * x[..., 0])
+    elif x.shape[-1] == 2:
+        n = np.sqrt(x[..., 0] * x[..., 0] + x[..., 1] * x[..., 1])
+    else:
+        n = np.sqrt(dot(x, x))
+    return np.where(n <= T_.type(min_norm), T_.type(min_norm), n)[..., None]
+
+
+def fire_power(Fp: np.ndarray, Fr: np.ndarray, fs: FireState) -> float:
+    """Power ``sum(F . v_old)`` over both leaves (optimizers.py:245-248).
+    Each float32 product is exact in float64; the sum is taken in float64."""
+    dt = fs.dt
+    vp = fs.vel_pos + Fp * dt / Fp.dtype.type(2.0)
+    vr = fs.vel_rot + Fr * dt / Fr.dtype.type(2.0)
+    return float(np.sum(Fp.astype(np.float64) * vp.astype(np.float64)) +
+                 np.sum(Fr.astype(np.float64) * vr.astype(np.float64)))
+
+
+def fire_update(Fp: np.ndarray, Fr: np.ndarray, fs: FireState, fp: FireParams, power: float | None = None):
+    """``fire().update`` (optimizers.py:241-318) with F = -grad for the two
+    leaves.  Returns (update_pos, update_rot, new FireState)."""
+    T_ = Fp.dtype.type
+    two = T_(2.0)
+    dt = fs.dt
+    v_old_p = fs.vel_pos + Fp * dt / two
+    v_old_r = fs.vel_rot + Fr * dt / two
+    if power is None:
+        power = fire_power(Fp, Fr, fs)
+    dt_cand_inc = np.minimum(dt * T_(fp.f_inc), T_(fp.dt * fp.dt_max_scale))
+    dt_cand_dec = np.maximum(dt * T_(fp.f_dec), T_(fp.dt * fp.dt_min_scale))
+    if power > 0.0:  # downhill (optimizers.py:278-292)
+        N_good = fs.N_good + 1
+        new_dt = dt_cand_inc if N_good > fp.N_min else dt
+        new_alpha = fs.alpha * T_(fp.f_alpha) if N_good > fp.N_min else fs.alpha
+        N_bad = 0
+        dt_rev = T_(0.0)
+        vs = T_(1.0)
+    else:  # uphill (optimizers.py:258-276)
+        N_bad = fs.N_bad + 1
+        exceeded = N_bad > fp.N_bad_max
+        new_dt = T_(fp.dt) if exceeded else dt_cand_dec
+        N_bad = 0 if exceeded else N_bad
+        new_alpha = T_(fp.alpha_init)
+        N_good = 0
+        dt_rev = -new_dt
+        vs = T_(0.0)
+    out = []
+    for F, v_old in ((Fp, v_old_p), (Fr, v_old_r)):
+        v_half = v_old * vs + F * new_dt / two
+        vn = safe_norm_rows(v_half, 1e-16)
+        fn = safe_norm_rows(F, 1e-16)
+        mix = np.where(fn > T_(1e-16), vn / fn * new_alpha, T_(0.0))
+        v_half = (v_half * (T_(1.0) - new_alpha) + F * mix) * vs
+        upd = v_old * dt_rev / two + v_half * new_dt / two
+        out.append((upd, v_half))
+    ns = FireState(vel_pos=out[0][1], vel_rot=out[1][1], dt=new_dt, alpha=new_alpha, N_good=N_good, N_bad=N_bad)
+    return out[0][0], out[1][0], ns
+
+
+def q_from_rotvec(r: np.ndarray) -> np.ndarray:
+    """``Quaternion.from_rotvec`` (quaternion.py:99-127)."""
+    T_ = r.dtype.type
+    n2 = dot(r, r)
+    safe = np.maximum(n2, T_(1e-16))
+    theta = np.sqrt(safe)
+    half = T_(0.5) * theta
+    ch = np.cos(half)
+    sc = np.sin(half) / theta
+    return np.concatenate([ch[..., None], r * sc[..., None]], axis=-1)
+
+
+def apply_fire_params(st: State, upd_pos: np.ndarray, upd_rot: np.ndarray) -> None:
+    """``optax.apply_updates`` + ``jnp.where(mask, new, old)`` (routines.py:
+    351-360) and ``_delta_params_to_state`` (routines.py:37-65): positions
+    move, the orientation turns by the delta rotation vector about itself."""
+    m = (~st.fixed)[:, None]
+    st.pos = np.where(m, st.pos + upd_pos * m.astype(st.pos.dtype), st.pos)
+    zero = np.zeros_like(upd_rot)
+    rv = np.where(m, zero + upd_rot * m.astype(st.pos.dtype), zero)
+    if st.dim == 2:
+        rv = np.concatenate([np.zeros_like(st.pos), rv], axis=-1)
+    st.q = q_unit(q_mul(q_from_rotvec(rv), st.q))
+
+
+def gravity_energy(st: State, P: Params) -> float:
+    """``ForceManager.compute_potential_energy`` gravity term
+    (force_manager.py:450-459) for spheres: -sum (g . x) m."""
+    if P.gravity is None:
+        return 0.0
+    g = np.asarray(P.gravity, st.pos.dtype)
+    e = -(dot(np.broadcast_to(g, st.pos.shape), st.pos) * st.mass)
+    return float(e.astype(np.float64).sum())
+
+
+def fire_eval(st: State, P: Params, nls: NLState | None = None) -> tuple[float, float]:
+    """``eval_step`` (routines.py:228-245): forces, torques and the potential
+    energy at the current parameters; returns (pe, max |grad| component)."""
+    compute_force(st, P, nls)
+    force_manager_apply(st, P)
+    if P.collider == "neighborlist":
+        pe = potential_energy(st, P, nls.nl)
+    else:
+        pe = potential_energy(st, P)
+    pe += gravity_energy(st, P)
+    g = max(float(np.max(np.abs(st.force), initial=0.0)), float(np.max(np.abs(st.torque), initial=0.0)))
+    return pe, g
+
+
+def minimize(st: State, P: Params, fp: FireParams, max_steps: int = 10000, pe_tol: float = 1e-16,
+             pe_diff_tol: float = 1e-16, force_tol: float = 0.0, nls: NLState | None = None):
+    """``minimize`` (routines.py:150-383) with FIRE; the potential energy and
+    the power are float64 sums.  Returns (steps, pe per particle)."""
+    N = st.N
+    fs = fire_init(st, fp)
+    pe, gmax = fire_eval(st, P, nls)
+    prev = math.inf
+    steps = 0
+    tiny = np.finfo(np.float64).tiny
+    while steps < max_steps:
+        conv_pe = abs(pe / N) <= pe_tol
+        denom = max(abs(pe), abs(prev), tiny)
+        conv_rel = abs(pe - prev) / denom < pe_diff_tol
+        conv_f = gmax <= force_tol
+        if conv_pe or conv_rel or conv_f:
+            break
+        m = (~st.fixed)[:, None].astype(st.pos.dtype)
+        up, ur, fs = fire_update(st.force * m, st.torque * m, fs, fp)
+        apply_fire_params(st, up, ur)
+        prev = pe
+        pe, gmax = fire_eval(st, P, nls)
+        steps += 1
+    return steps, pe / N
+
+
 def _selfcheck() -> None:  # pragma: no cover - manual use
     st = make_state([[0.0, 0.0, 0.0], [0.9, 0.0, 0.0]], [0.5, 0.5])
     P = Params(dim=3, dt=1e-3, domain=Domain("free", np.ones(3, np.float32), np.zeros(3, np.float32)),

@@ -374,3 +374,51 @@ def test_neighborlist_split_calls_equal_one_call(device):
     assert_bitwise("xi", xa, xb)
     # the inputs are left untouched (functional semantics)
     assert int(sy.collider.n_build_times) == 0
+
+
+# ---------------------------------------------------------------------------
+# (d) FIRE energy minimisation (minimizers/routines.py:150-383)
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("collider", ["naive", "CellList", "NeighborList"])
+def test_minimize_fire_bitwise(device, collider):
+    """A fixed number of FIRE iterations on a jammed periodic packing: every
+    position, orientation, force and torque equals the float32 oracle's; the
+    float64 energy agrees to rounding of the sum order."""
+    print(f"[fire] collider={collider}", flush=True)
+    st, box = packed(700, 3, seed=9, phi=0.70, ang=False)
+    sy = make_system(st, box, "periodic", "spring", collider, dt=1e-3)
+    n = 40
+    pos0 = st.pos_c.clone()
+    st2, sy2, steps, pe = jd.minimize(st, sy, max_steps=n, pe_tol=0.0, pe_diff_tol=-1.0, force_tol=-1.0)
+    assert steps == n
+    ost = B.state_of(st)
+    P = B.params_of(st, sy)
+    nls = B.nl_state_of(st, sy) if collider == "NeighborList" else None
+    osteps, ope = O.minimize(ost, P, O.FireParams(dt=float(np.float32(1e-3))), max_steps=n, pe_tol=0.0,
+                             pe_diff_tol=-1.0, force_tol=-1.0, nls=nls)
+    assert osteps == n
+    assert_bitwise("fire pos", st2.pos_c.cpu().numpy(), ost.pos)
+    assert_bitwise("fire q", st2.q.data.cpu().numpy(), ost.q)
+    assert_bitwise("fire force", st2.force.cpu().numpy(), ost.force)
+    assert_bitwise("fire torque", st2.torque.cpu().numpy(), ost.torque)
+    assert pe == pytest.approx(ope, rel=1e-12)
+    assert torch.equal(st.pos_c, pos0)  # the inputs are untouched
+
+
+def test_minimize_fire_converges_and_stops(device):
+    """Stopping rules of routines.py:284-314 on a 20k-sphere jammed packing:
+    force_tol stops the loop once max |F| <= tol; pe_tol stops at once when
+    |pe|/N is already below it; max_steps caps the count."""
+    print("[fire] convergence", flush=True)
+    st, box = packed(20000, 3, seed=4, phi=0.68, ang=False)
+    sy = make_system(st, box, "periodic", "spring", "CellList", dt=1e-3)
+    s1, y1, steps, pe = jd.minimize(st, sy, max_steps=20000, pe_tol=0.0, pe_diff_tol=-1.0, force_tol=1e-3)
+    res = y1.__dict__["_last_minimize"]
+    assert 0 < steps < 20000 and res.max_grad <= 1e-3
+    assert float(s1.force.abs().max()) == pytest.approx(res.max_grad)
+    e0 = sy.potential_energy(st) / st.N
+    assert 0.0 < pe < e0
+    _, _, s0, _ = jd.minimize(st, sy, max_steps=100, pe_tol=1e30)
+    assert s0 == 0
+    _, _, s3, _ = jd.minimize(st, sy, max_steps=3, pe_tol=0.0, pe_diff_tol=-1.0)
+    assert s3 == 3

@@ -403,3 +403,102 @@ def test_cpu_port_matches_numpy_oracle():
                        ("ang", run.angv, ref.ang_vel), ("force", run.force, ref.force)]:
         assert np.array_equal(a, b), name
     assert run.n_build.value == nls.n_build
+
+
+# --------------------------------------------------------------------------
+# FIRE minimisation (minimizers/optimizers.py:150-339, routines.py:150-383)
+# --------------------------------------------------------------------------
+def _packing(N=64, phi=0.7, seed=0, dtype=F64):
+    rng = np.random.default_rng(seed)
+    rad = np.where(np.arange(N) % 2 == 0, 0.5, 0.7)
+    L = float((np.sum(4.0 / 3.0 * np.pi * rad**3) / phi) ** (1.0 / 3.0))
+    st = O.make_state(rng.uniform(0, L, (N, 3)), rad, dtype=dtype)
+    d = O.Domain("periodic", np.full(3, L, dtype), np.zeros(3, dtype))
+    P = O.Params(dim=3, dt=0.05, domain=d, law=O.Law("spring", young_eff=np.ones((1, 1), dtype)), collider="naive")
+    return st, P
+
+
+def test_fire_separates_an_overlapping_pair_to_zero_energy():
+    """The minimum of a harmonic overlap is any separation >= r_i + r_j: FIRE
+    reaches pe = 0 (the pe_tol stop) with the pair out of contact."""
+    st = two(0.6, r=0.5)
+    P = params("spring", young_eff=np.array([[1.0]]))
+    steps, pe = O.minimize(st, P, O.FireParams(dt=0.1), max_steps=10000, pe_tol=0.0, pe_diff_tol=-1.0)
+    assert pe == 0.0 and 0 < steps < 10000
+    assert abs(st.pos[1, 0] - st.pos[0, 0]) >= 1.0
+    # momentum-free: both spheres moved symmetrically
+    assert st.pos[0, 0] + st.pos[1, 0] == pytest.approx(0.6, abs=1e-12)
+
+
+def test_fire_fixed_particle_stays_put():
+    st = two(0.6, r=0.5, fixed=[True, False])
+    P = params("spring", young_eff=np.array([[1.0]]))
+    O.minimize(st, P, O.FireParams(dt=0.1), max_steps=10000, pe_tol=0.0, pe_diff_tol=-1.0)
+    assert st.pos[0, 0] == 0.0 and st.pos[1, 0] >= 1.0
+
+
+def test_fire_uphill_and_downhill_rules():
+    """Bitzek et al. (2006) with the reference's uphill-counter reset: P > 0
+    counts downhill steps and, past N_min, grows dt by f_inc (capped at
+    dt_max) and shrinks alpha by f_alpha; P <= 0 shrinks dt by f_dec (not
+    below dt_min), resets alpha and zeroes the velocity; more than N_bad_max
+    uphill steps in a row restore dt0."""
+    fp = O.FireParams(dt=0.1)
+    F = np.array([[1.0, 0.0, 0.0]])
+    Z = np.zeros((1, 3))
+    fs = O.FireState(vel_pos=np.array([[0.5, 0.0, 0.0]]), vel_rot=Z.copy(), dt=F64(0.1), alpha=F64(0.1))
+    for k in range(1, 8):
+        _, _, fs = O.fire_update(F, Z, fs, fp, power=1.0)
+        assert fs.N_good == k and fs.N_bad == 0
+        if k <= fp.N_min:
+            assert fs.dt == pytest.approx(0.1) and fs.alpha == pytest.approx(0.1)
+    assert fs.dt == pytest.approx(0.1 * 1.1**2) and fs.alpha == pytest.approx(0.1 * 0.99**2)
+    for _ in range(40):
+        _, _, fs = O.fire_update(F, Z, fs, fp, power=1.0)
+    assert fs.dt == pytest.approx(1.0)  # dt_max = 10 dt0
+    dts = []
+    for k in range(1, 13):
+        up, _, fs = O.fire_update(F, Z, fs, fp, power=-1.0)
+        assert fs.N_good == 0 and fs.alpha == pytest.approx(0.1)
+        np.testing.assert_array_equal(fs.vel_pos, 0.0)
+        dts.append(float(fs.dt))
+    assert dts[:10] == pytest.approx([max(1.0 * 0.5**k, 1e-4) for k in range(1, 11)])
+    assert dts[10] == pytest.approx(0.1)  # the 11th uphill step in a row restores dt0
+    assert dts[11] == pytest.approx(0.05)
+
+
+def test_fire_velocity_mixing_is_bitzek():
+    """Downhill, v <- (1 - alpha) v + alpha |v| F_hat with v the half-kicked
+    velocity (Bitzek et al. 2006, eq. 3)."""
+    fp = O.FireParams(dt=0.2)
+    F = np.array([[0.0, 2.0, 0.0]])
+    v0 = np.array([[1.0, 0.0, 0.0]])
+    fs = O.FireState(vel_pos=v0, vel_rot=np.zeros((1, 3)), dt=F64(0.2), alpha=F64(0.1))
+    _, _, ns = O.fire_update(F, np.zeros((1, 3)), fs, fp, power=1.0)
+    v_half = v0 + F * 0.2  # two half kicks of dt/2 each
+    want = 0.9 * v_half + 0.1 * np.linalg.norm(v_half) * F / 2.0
+    np.testing.assert_allclose(ns.vel_pos, want, rtol=1e-14)
+
+
+def test_fire_relaxes_a_jammed_packing():
+    """phi = 0.7 is above jamming: FIRE ends in mechanical equilibrium
+    (max |F| below 1e-10) at positive energy lower than the start."""
+    st, P = _packing()
+    O.compute_force(st, P)
+    e0 = O.potential_energy(st, P)
+    f0 = np.abs(st.force).max()
+    steps, pe = O.minimize(st, P, O.FireParams(dt=0.05), max_steps=20000, pe_tol=0.0, pe_diff_tol=-1.0,
+                           force_tol=1e-10)
+    assert np.abs(st.force).max() <= 1e-10 < f0
+    assert 0.0 < pe * st.N < e0
+    # equilibrium: the net force on the packing vanishes
+    assert np.abs(st.force.sum(axis=0)).max() < 1e-9
+
+
+def test_fire_float32_tracks_float64():
+    s32, P32 = _packing(N=32, dtype=F32)
+    s64, P64 = _packing(N=32, dtype=F64)
+    _, pe32 = O.minimize(s32, P32, O.FireParams(dt=0.05), max_steps=60, pe_tol=0.0, pe_diff_tol=-1.0)
+    _, pe64 = O.minimize(s64, P64, O.FireParams(dt=0.05), max_steps=60, pe_tol=0.0, pe_diff_tol=-1.0)
+    assert pe32 == pytest.approx(pe64, rel=1e-4)
+    np.testing.assert_allclose(s32.pos, s64.pos, atol=1e-4)
