@@ -312,7 +312,40 @@ int jdem_step_stage(jdem_ctx* ctx, jdem_system* sys, int32_t stage);
  * NeighborList collider.  stats may be NULL. */
 int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats* stats);
 
-/* ---- (d) energy minimisation -------------------------------------------------- */
+/* ---- (d) batches of independent systems --------------------------------------- */
+
+/* A segmented batch: n_systems independent systems whose particles are
+ * concatenated in sys->state (system k owns particles offsets[k] ..
+ * offsets[k+1]-1).  Every array here is a HOST array of n_systems entries
+ * (offsets: n_systems + 1).  Per system: its domain (PERIODIC or REFLECT),
+ * the grid of its secondary CellList (inflated to cutoff + skin as
+ * cell_list.py:537-538), dt, and the NeighborList cutoff and skin. */
+typedef struct jdem_batch {
+  int32_t n_systems;
+  int32_t rebuild_first; /* build all lists at the first step: some system was
+                            never built (its rows are all -1) */
+  const int64_t* offsets;
+  const jdem_domain* domains;
+  const jdem_grid* grids;
+  const float* dt;
+  const float* cutoff;
+  const float* skin;
+} jdem_batch;
+
+/* System.step for a batch of systems of unequal size in one segmented
+ * layout -- what vmap over a padded batch of equal-size systems does in the
+ * reference (system.py:743-744).  sys carries the concatenated state, the
+ * shared law (material tables; mat_id selects a system's material),
+ * integrators, gravity and the NeighborList buffers (one max_neighbors for
+ * all systems; secondary = CELL_LIST); sys->dt, sys->domain and
+ * sys->nlist.{grid,cutoff,skin} are ignored.  The systems' lists are rebuilt
+ * together whenever one of them needs it; every trajectory equals that of
+ * the same system stepped alone (jdem_system_step), bit for bit, because a
+ * list rebuild changes no force.  sys->nlist.n_build_times counts the
+ * batch's builds. */
+int jdem_batch_step(jdem_ctx* ctx, jdem_system* sys, const jdem_batch* batch, int64_t n, jdem_step_stats* stats);
+
+/* ---- (e) energy minimisation -------------------------------------------------- */
 
 /* minimize(state, system, max_steps, pe_tol, pe_diff_tol, force_tol) with the
  * FIRE optimiser (minimizers/routines.py:150-383, optimizers.py:150-339):

@@ -31,7 +31,8 @@ from .integrators import (
 from .materials import Material, MaterialMatchmaker, MaterialTable
 from .state import Quaternion, State
 from .system import System
-from . import minimizers, utils
+from . import batching, minimizers, utils
+from .batching import Batch
 from .minimizers import fire, minimize
 
 __version__ = "0.1.0"
@@ -66,6 +67,8 @@ __all__ = [
     "State",
     "Quaternion",
     "System",
+    "Batch",
+    "batching",
     "fire",
     "minimize",
     "minimizers",

@@ -149,6 +149,19 @@ class JdemTiming(ctypes.Structure):
     ]
 
 
+class JdemBatch(ctypes.Structure):
+    _fields_ = [
+        ("n_systems", c_int32),
+        ("rebuild_first", c_int32),
+        ("offsets", c_void_p),
+        ("domains", c_void_p),
+        ("grids", c_void_p),
+        ("dt", c_void_p),
+        ("cutoff", c_void_p),
+        ("skin", c_void_p),
+    ]
+
+
 class JdemFire(ctypes.Structure):
     _fields_ = [
         ("dt", c_double),
@@ -183,6 +196,7 @@ STRUCTS = {
     "jdem_nlist": JdemNlist,
     "jdem_system": JdemSystem,
     "jdem_step_stats": JdemStepStats,
+    "jdem_batch": JdemBatch,
     "jdem_fire": JdemFire,
     "jdem_minimize_stats": JdemMinimizeStats,
 }
@@ -212,6 +226,7 @@ SIGNATURES = {
     "jdem_kinetic_energy": (c_int, [c_void_p, POINTER(JdemSystem), POINTER(c_double)]),
     "jdem_step_stage": (c_int, [c_void_p, POINTER(JdemSystem), c_int32]),
     "jdem_system_step": (c_int, [c_void_p, POINTER(JdemSystem), c_int64, POINTER(JdemStepStats)]),
+    "jdem_batch_step": (c_int, [c_void_p, POINTER(JdemSystem), POINTER(JdemBatch), c_int64, POINTER(JdemStepStats)]),
     "jdem_minimize": (
         c_int,
         [c_void_p, POINTER(JdemSystem), POINTER(JdemFire), c_int64, c_double, c_double, c_double,

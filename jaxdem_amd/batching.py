@@ -1,0 +1,272 @@
+"""Independent systems of unequal size advanced together in one segmented
+layout (``jdem_batch_step``).
+
+The reference batches by ``vmap`` over systems padded to one particle count
+(system.py:743-744, State.stack).  Here the systems' particles are
+concatenated instead: every particle knows its system, cells never mix
+systems, and one kernel per time step advances all of them.  Each system
+keeps its own domain, cell grid, time step, cutoff and skin; the force
+model, integrators and gravity are shared.  A system's trajectory in the
+batch equals its trajectory alone, bit for bit.
+
+    batch = Batch.pack([(state_0, system_0), (state_1, system_1), ...])
+    batch = batch.step(n)
+    pairs = batch.unpack()
+"""
+
+from __future__ import annotations
+
+import ctypes
+from dataclasses import dataclass, fields, replace
+from typing import Sequence
+
+import numpy as np
+import torch
+
+from . import _lib as L
+from .colliders import DynamicCellList, NeighborList
+from .domains import PeriodicDomain, ReflectDomain
+from .materials import MaterialTable
+from .state import Quaternion, State
+
+F32 = np.float32
+
+
+def _merge_tables(tables: Sequence[MaterialTable]) -> tuple[MaterialTable, np.ndarray]:
+    """Block-diagonal union of material tables: system k's material m becomes
+    mat_offsets[k] + m; pair entries across systems are never used."""
+    sizes = [len(t) for t in tables]
+    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    keys = sorted(set().union(*[set(t.props) for t in tables]))
+    props = {}
+    for k in keys:
+        props[k] = np.concatenate(
+            [np.asarray(t.props[k], np.float32) if k in t.props else np.zeros(len(t), np.float32) for t in tables]
+        )
+    pkeys = sorted(set().union(*[set(t.pair) for t in tables]))
+    M = int(offs[-1])
+    pair = {}
+    for k in pkeys:
+        a = np.zeros((M, M), np.float32)
+        for t, o in zip(tables, offs[:-1]):
+            if k in t.pair:
+                n = len(t)
+                a[o : o + n, o : o + n] = np.asarray(t.pair[k], np.float32)
+        pair[k] = a
+    return MaterialTable(props, pair, tables[0].matcher), offs
+
+
+def _pad_cols(t: torch.Tensor, width: int, fill) -> torch.Tensor:
+    if t.shape[1] == width:
+        return t
+    pad = torch.full((t.shape[0], width - t.shape[1]) + tuple(t.shape[2:]), fill, dtype=t.dtype, device=t.device)
+    return torch.cat([t, pad], dim=1)
+
+
+def _shift(t: torch.Tensor, off: int) -> torch.Tensor:
+    return torch.where(t >= 0, t + off, t)
+
+
+@dataclass
+class Batch:
+    """A packed batch; ``systems`` keeps each system's own parameters."""
+
+    state: State
+    system: object  # the packed System (shared plugins, merged materials)
+    offsets: np.ndarray
+    mat_offsets: np.ndarray
+    systems: list
+    bond_widths: list
+
+    def __len__(self) -> int:
+        return len(self.systems)
+
+    @property
+    def n_particles(self) -> int:
+        return int(self.offsets[-1])
+
+    # ------------------------------------------------------------------
+    @staticmethod
+    def pack(pairs: Sequence[tuple]) -> "Batch":
+        from .system import _fusable
+
+        if not pairs:
+            raise ValueError("Batch.pack needs at least one (state, system) pair")
+        states = [p[0] for p in pairs]
+        systems = [p[1] for p in pairs]
+        s0, y0 = states[0], systems[0]
+        dev = s0.device
+        for st, sy in pairs:
+            if st.device != dev or st.dim != s0.dim:
+                raise ValueError("batched systems must share the device and the dimension")
+            if not st.is_sphere_state:
+                raise NotImplementedError("batches hold sphere states only")
+            if not _fusable(st, sy):
+                raise NotImplementedError("batched systems must use built-in plugins without force functions")
+            col = sy.collider
+            if not isinstance(col, NeighborList) or not isinstance(col.secondary_collider, DynamicCellList):
+                raise NotImplementedError("batched systems run a NeighborList over a CellList")
+            if not isinstance(sy.domain, (PeriodicDomain, ReflectDomain)):
+                raise NotImplementedError("batched systems need a periodic or reflect domain")
+            same = (
+                type(sy.force_model) is type(y0.force_model)
+                and getattr(sy.force_model, "kt_ratio", None) == getattr(y0.force_model, "kt_ratio", None)
+                and getattr(sy.force_model, "damping", None) == getattr(y0.force_model, "damping", None)
+                and type(sy.linear_integrator) is type(y0.linear_integrator)
+                and type(sy.rotation_integrator) is type(y0.rotation_integrator)
+                and sy.interact_same_bond_id == y0.interact_same_bond_id
+                and np.array_equal(np.asarray(sy.force_manager.gravity), np.asarray(y0.force_manager.gravity))
+                and (col.history is None) == (y0.collider.history is None)
+            )
+            if not same:
+                raise ValueError("batched systems must share the force model, integrators and gravity")
+            if any(getattr(sy.force_manager, k) is not None for k in ("external_force", "external_force_com",
+                                                                          "external_torque")):
+                raise NotImplementedError("batched systems with pending external forces")
+        sizes = [st.N for st in states]
+        offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        mt, moff = _merge_tables([sy.mat_table for sy in systems])
+        # concatenated state
+        bw = [int(st.bond_id.shape[-1]) for st in states]
+        width = max(bw)
+        kw = {}
+        for f in fields(s0):
+            v = getattr(s0, f.name)
+            if isinstance(v, Quaternion):
+                kw[f.name] = Quaternion(torch.cat([getattr(s, f.name).data for s in states]).contiguous())
+            elif isinstance(v, torch.Tensor):
+                parts = []
+                for k, s in enumerate(states):
+                    t = getattr(s, f.name)
+                    if f.name == "bond_id":
+                        t = _shift(_pad_cols(t, width, -1), int(offsets[k]))
+                    elif f.name == "clump_id":
+                        t = t + int(offsets[k])
+                    elif f.name == "mat_id":
+                        t = t + int(moff[k])
+                    parts.append(t)
+                kw[f.name] = torch.cat(parts).contiguous()
+            else:
+                kw[f.name] = v
+        kw["has_bonds"] = any(s.has_bonds for s in states)
+        state = State(**kw)
+        # packed NeighborList buffers (one row width M for every system)
+        M = max(sy.collider.max_neighbors for sy in systems)
+        W32 = max((M + 31) // 32, 1)
+        nl = torch.cat(
+            [_shift(_pad_cols(sy.collider.neighbor_list, M, -1), int(offsets[k])) for k, sy in enumerate(systems)]
+        ).contiguous()
+        old = torch.cat([sy.collider.old_pos for sy in systems]).contiguous()
+        hist = None
+        if y0.collider.history is not None:
+            hist = {
+                "xi": torch.cat([_pad_cols(sy.collider.history["xi"], M, 0.0) for sy in systems]).contiguous(),
+                "live": torch.cat([_pad_cols(sy.collider.history["live"], W32, 0) for sy in systems]).contiguous(),
+            }
+        built = [int(sy.collider.n_build_times) for sy in systems]
+        col = replace(
+            y0.collider, neighbor_list=nl, old_pos=old, history=hist, max_neighbors=M,
+            n_build_times=max(built), overflow=any(sy.collider.overflow for sy in systems),
+        )
+        packed = replace(y0, collider=col, mat_table=mt)
+        packed.__dict__["_rebuild_first"] = min(built) == 0 and max(built) > 0
+        return Batch(state, packed, offsets, moff, list(systems), bw)
+
+    # ------------------------------------------------------------------
+    def _c_arrays(self, keep: list) -> L.JdemBatch:
+        B = len(self.systems)
+        off = np.ascontiguousarray(self.offsets, np.int64)
+        doms = (L.JdemDomain * B)()
+        grids = (L.JdemGrid * B)()
+        dt = np.zeros(B, F32)
+        cut = np.zeros(B, F32)
+        skin = np.zeros(B, F32)
+        for k, sy in enumerate(self.systems):
+            doms[k] = sy.domain.to_c(self.state.dim)
+            g, ovf = sy.collider.secondary_grid(sy)
+            if ovf:
+                raise NotImplementedError("secondary cell grid exceeds the int32 hash range")
+            grids[k] = g
+            dt[k] = F32(sy.dt)
+            cut[k] = F32(sy.collider.cutoff)
+            skin[k] = F32(sy.collider.skin)
+        keep += [off, doms, grids, dt, cut, skin]
+        b = L.JdemBatch()
+        b.n_systems = B
+        b.rebuild_first = 1 if self.system.__dict__.get("_rebuild_first") else 0
+        b.offsets = off.ctypes.data
+        b.domains = ctypes.addressof(doms)
+        b.grids = ctypes.addressof(grids)
+        b.dt = dt.ctypes.data
+        b.cutoff = cut.ctypes.data
+        b.skin = skin.ctypes.data
+        return b
+
+    def step(self, n: int = 1) -> "Batch":
+        """Advance every system ``n`` steps; the batch itself is left
+        untouched (functional semantics, like System.step)."""
+        from .system import _advance_clock, _ctx, _fresh_collider, _system_struct
+
+        st = self.state.with_kinematics()
+        col = _fresh_collider(self.system.collider)
+        sy = replace(self.system, collider=col)
+        keep: list = []
+        S = _system_struct(st, sy, keep)
+        Bc = self._c_arrays(keep)
+        stats = L.JdemStepStats()
+        ctx = _ctx(st)
+        ctx.check(ctx.lib.jdem_batch_step(ctx.ptr, ctypes.byref(S), ctypes.byref(Bc), int(n), ctypes.byref(stats)),
+                  "jdem_batch_step")
+        col2 = replace(col, n_build_times=int(S.nlist.n_build_times), overflow=bool(S.nlist.overflow))
+        sy2 = replace(sy, collider=col2)
+        sy2.__dict__["_last_stats"] = stats
+        systems = [_advance_clock(s, int(n)) for s in self.systems]
+        return Batch(st, sy2, self.offsets, self.mat_offsets, systems, self.bond_widths)
+
+    def unpack(self) -> list:
+        """The systems again, as (State, System) pairs.  A system's list keeps
+        its own row width; entries past it set its overflow flag."""
+        out = []
+        st, col = self.state, self.system.collider
+        for k, sy in enumerate(self.systems):
+            a, b = int(self.offsets[k]), int(self.offsets[k + 1])
+            kw = {}
+            for f in fields(st):
+                v = getattr(st, f.name)
+                if isinstance(v, Quaternion):
+                    kw[f.name] = Quaternion(v.data[a:b].clone())
+                elif isinstance(v, torch.Tensor):
+                    t = v[a:b].clone()
+                    if f.name == "bond_id":
+                        t = _shift(t, -a)[:, : self.bond_widths[k]].contiguous()
+                    elif f.name == "clump_id":
+                        t = t - a
+                    elif f.name == "mat_id":
+                        t = t - int(self.mat_offsets[k])
+                    kw[f.name] = t
+                else:
+                    kw[f.name] = v
+            kw["has_bonds"] = bool(self.bond_widths[k] > 0 and (kw["bond_id"] >= 0).any())
+            sk = State(**kw)
+            c0 = sy.collider
+            Mk = c0.max_neighbors
+            rows = _shift(col.neighbor_list[a:b], -a)
+            overflow = bool(col.overflow) or bool((rows[:, Mk:] >= 0).any()) if rows.shape[1] > Mk else bool(col.overflow)
+            hist = None
+            if col.history is not None:
+                w32 = c0.history["live"].shape[1]
+                hist = {
+                    "xi": col.history["xi"][a:b, :Mk].contiguous().clone(),
+                    "live": col.history["live"][a:b, :w32].contiguous().clone(),
+                }
+                if Mk % 32:
+                    hist["live"][:, -1] &= (1 << (Mk % 32)) - 1
+            ck = replace(
+                c0, neighbor_list=rows[:, :Mk].contiguous().clone(), old_pos=col.old_pos[a:b].clone(),
+                history=hist, n_build_times=int(col.n_build_times), overflow=overflow,
+            )
+            out.append((sk, replace(sy, collider=ck)))
+        return out
+
+
+__all__ = ["Batch"]

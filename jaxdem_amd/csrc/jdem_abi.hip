@@ -21,7 +21,7 @@ std::vector<Buf*> jdem_ctx::bufs() {
   std::vector<Buf*> all = {&pos2,    &vel2,   &ang2,     &keys[0], &keys[1], &vals[0], &vals[1], &hist,
                            &cstart,  &cend,   &spos,     &nl_new,  &xi_new,  &live_new, &flags,  &red,
                            &nl_copy, &xi_copy, &live_copy, &wF,     &wT,      &wOld,     &wWpos, &wSrc,
-                           &wInv,    &wKeys,  &fireV,    &fireDev};
+                           &wInv,    &wKeys,  &fireV,    &fireDev, &segp,  &segid};
   for (int k = 0; k < 2; ++k)
     for (Buf* b : {&wP[k], &wV[k], &wW[k], &wQ[k], &wI[k], &wOrig[k], &wNl[k], &wXi[k], &wLive[k]}) all.push_back(b);
   return all;
@@ -116,7 +116,7 @@ LawP law_of(const jdem_law& l) {
   return p;
 }
 
-int check_system(jdem_ctx* ctx, const jdem_system* sys) {
+int check_system(jdem_ctx* ctx, const jdem_system* sys, bool batch = false) {
   if (!ctx) return JDEM_EINVAL;
   if (!sys) return fail(ctx, JDEM_EINVAL, "null system");
   const jdem_state& s = sys->state;
@@ -152,8 +152,8 @@ int check_system(jdem_ctx* ctx, const jdem_system* sys) {
     if (L.secondary != JDEM_COLLIDER_CELL_LIST && L.secondary != JDEM_COLLIDER_NAIVE)
       return fail(ctx, JDEM_EINVAL, "secondary collider must be CellList or naive");
   }
-  if (sys->collider_kind == JDEM_COLLIDER_CELL_LIST || (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST &&
-                                                        sys->nlist.secondary == JDEM_COLLIDER_CELL_LIST)) {
+  if (!batch && (sys->collider_kind == JDEM_COLLIDER_CELL_LIST ||
+                 (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST && sys->nlist.secondary == JDEM_COLLIDER_CELL_LIST))) {
     const jdem_grid& g = sys->collider_kind == JDEM_COLLIDER_CELL_LIST ? sys->grid : sys->nlist.grid;
     if (g.n_cells <= 0 || g.n_cells > (int64_t)INT32_MAX)
       return fail(ctx, JDEM_EUNSUPPORTED, "cell grid size out of the int32 hash range");
@@ -432,7 +432,17 @@ void span_end(jdem_ctx* ctx, int idx) {
 // ---- resident NeighborList engine driver ---------------------------------
 // System.step with the NeighborList collider on the cell-sorted work set of
 // jdem_resident.hip; see that file for the layout.
-int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats* S) {
+// A segmented batch of independent systems (jdem_batch_step): the system of
+// every particle and the systems' parameters, on the device.
+struct SegRun {
+  int n_seg;
+  const int32_t* seg;
+  const SegP* segp;
+  int64_t total_cells;
+  bool rebuild_first;  // build at the first step although lists exist
+};
+
+int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats* S, const SegRun* sg = nullptr) {
   jdem_state& s = sys->state;
   jdem_nlist& L = sys->nlist;
   const int64_t N = s.n;
@@ -442,8 +452,9 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   const bool hist = L.hist_xi && L.hist_live;
   const bool naive2 = L.secondary == JDEM_COLLIDER_NAIVE;
   if (M > 256) return fail(ctx, JDEM_EUNSUPPORTED, "max_neighbors > 256 on the resident path");
-  if (!naive2 && sys->domain.kind == JDEM_DOMAIN_FREE)
+  if (!naive2 && !sg && sys->domain.kind == JDEM_DOMAIN_FREE)
     return fail(ctx, JDEM_EUNSUPPORTED, "NeighborList over a CellList with a FreeDomain (per-step grid)");
+  const int64_t n_cells = sg ? sg->total_cells : L.grid.n_cells;
   const size_t n4 = 16 * (size_t)N, ni = 4 * (size_t)N;
   for (int k = 0; k < 2; ++k) {
     ENSURE(ctx, ctx->wP[k], n4);
@@ -468,7 +479,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   ENSURE(ctx, ctx->vals[0], ni);
   ENSURE(ctx, ctx->vals[1], ni);
   ENSURE(ctx, ctx->hist, 4 * 256 * (size_t)((N + 4095) / 4096 + 1));
-  if (!naive2) ENSURE(ctx, ctx->cstart, 4 * ((size_t)L.grid.n_cells + 1));
+  if (!naive2) ENSURE(ctx, ctx->cstart, 4 * ((size_t)n_cells + 1));
   auto F4 = [](jdem::Buf& b) { return (float4*)b.p; };
   auto I32 = [](jdem::Buf& b) { return (int32_t*)b.p; };
   auto U64 = [](jdem::Buf& b) { return (uint64_t*)b.p; };
@@ -479,7 +490,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   int xc = 0;  // springs: read wXi[xc], write wXi[1 - xc]
 
   // --- entry: order the work set by (cell, original id), convert inputs ---
-  if (!naive2) {
+  if (!naive2 && !sg) {
     HIPCHK(ctx, launch_hash(s.pos, N, D, dom, g, (uint32_t*)ctx->wKeys.p, st));
     uint32_t* sk = nullptr;
     int32_t* perm = nullptr;
@@ -533,17 +544,22 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     int32_t* inv = I32(ctx->wInv);
     uint32_t* sk = nullptr;
     if (!naive2) {
-      HIPCHK(ctx, ws_launch_hash(F4(ctx->wP[cur]), I32(ctx->wOrig[set]), N, D, dom, g, (uint32_t*)ctx->wKeys.p,
-                                 I32(ctx->wWpos), st));
+      if (sg) {
+        HIPCHK(ctx, ws_launch_hash_seg(F4(ctx->wP[cur]), I32(ctx->wOrig[set]), N, D, sg->seg, sg->segp,
+                                       (uint32_t*)ctx->wKeys.p, I32(ctx->wWpos), st));
+      } else {
+        HIPCHK(ctx, ws_launch_hash(F4(ctx->wP[cur]), I32(ctx->wOrig[set]), N, D, dom, g, (uint32_t*)ctx->wKeys.p,
+                                   I32(ctx->wWpos), st));
+      }
       int32_t* new_orig = nullptr;
-      HIPCHK(ctx, radix_sort_pairs(ctx, (uint32_t*)ctx->wKeys.p, N, nbits_for(g.n_cells), &sk, &new_orig, st));
+      HIPCHK(ctx, radix_sort_pairs(ctx, (uint32_t*)ctx->wKeys.p, N, nbits_for(n_cells), &sk, &new_orig, st));
       HIPCHK(ctx, ws_launch_permute(N, new_orig, I32(ctx->wWpos), F4(ctx->wP[cur]), F4(ctx->wV[cur]),
                                     F4(ctx->wW[cur]), F4(ctx->wQ[set]), F4(ctx->wI[set]), F4(ctx->wP[1 - cur]),
                                     F4(ctx->wV[1 - cur]), F4(ctx->wW[1 - cur]), F4(ctx->wQ[1 - set]),
                                     F4(ctx->wI[1 - set]), I32(ctx->wOrig[1 - set]), src, inv, st));
       cur = 1 - cur;
       set = 1 - set;
-      HIPCHK(ctx, ws_launch_cell_starts(sk, N, g.n_cells, I32(ctx->cstart), st));
+      HIPCHK(ctx, ws_launch_cell_starts(sk, N, n_cells, I32(ctx->cstart), st));
     } else {
       HIPCHK(ctx, ws_launch_iota(src, N, st));
       HIPCHK(ctx, ws_launch_iota(inv, N, st));
@@ -563,6 +579,10 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       b.skeys = sk;
       b.starts = I32(ctx->cstart);
       b.grid = g;
+      if (sg) {
+        b.seg = sg->seg;
+        b.segp = sg->segp;
+      }
       b.dom = dom;
       b.nl = I32(ctx->wNl[1 - nlc]);
       b.overflow = (int32_t*)ctx->flags.p + 2;
@@ -594,7 +614,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   std::vector<int> cur_at((size_t)n + 2, 0), xc_at((size_t)n + 2, 0);
 
   auto launch_t = [&](int64_t t) -> int {
-    if (sys->domain.kind == JDEM_DOMAIN_FREE && t == n - 1 && !free_done) {
+    if (!sg && sys->domain.kind == JDEM_DOMAIN_FREE && t == n - 1 && !free_done) {
       const int nparts = 64;
       ENSURE(ctx, ctx->red, 4 * 6 * (size_t)nparts);
       HIPCHK(ctx, ws_launch_box(F4(ctx->wP[cur]), N, (float*)ctx->red.p, nparts, st));
@@ -655,6 +675,10 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     a.flag_rd = t >= 1 ? flags + (t & 1) : nullptr;
     a.flag_wr = flags + ((t + 1) & 1);
     a.step_idx = (int32_t)t;
+    if (sg) {
+      a.seg = sg->seg;
+      a.segp = sg->segp;
+    }
     const int sp = span_begin(ctx, 0, t);
     hipError_t e = ws_launch_step(a, D, sys->law.kind, st);
     span_end(ctx, sp);
@@ -666,7 +690,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
 
   int rc = launch_t(0);
   if (rc) return rc;
-  bool pending = L.n_build_times == 0;
+  bool pending = L.n_build_times == 0 || (sg && sg->rebuild_first);
   int64_t next = 1;
   int64_t chunk_len = 8;
   int64_t last_rebuild_at = 0;
@@ -1033,6 +1057,69 @@ int jdem_step_stage(jdem_ctx* ctx, jdem_system* sys, int32_t stage) {
   StepArgs a = base_args(sys);
   HIPCHK(ctx, launch_stage(a, sys->state.dim, stage, ctx->stream));
   return JDEM_OK;
+}
+
+int jdem_batch_step(jdem_ctx* ctx, jdem_system* sys, const jdem_batch* batch, int64_t n, jdem_step_stats* stats) {
+  int rc = check_system(ctx, sys, true);
+  if (rc) return rc;
+  if (!batch || batch->n_systems < 1 || !batch->offsets || !batch->domains || !batch->grids || !batch->dt ||
+      !batch->cutoff || !batch->skin)
+    return fail(ctx, JDEM_EINVAL, "incomplete batch description");
+  if (n < 0) return fail(ctx, JDEM_EINVAL, "negative step count");
+  if (sys->collider_kind != JDEM_COLLIDER_NEIGHBOR_LIST || sys->nlist.secondary != JDEM_COLLIDER_CELL_LIST)
+    return fail(ctx, JDEM_EUNSUPPORTED, "batches run the NeighborList collider over a CellList");
+  const int B = batch->n_systems;
+  const int64_t N = sys->state.n;
+  if (batch->offsets[0] != 0 || batch->offsets[B] != N) return fail(ctx, JDEM_EINVAL, "offsets do not cover the state");
+  std::vector<SegP> host((size_t)B);
+  int64_t base = 0;
+  for (int k = 0; k < B; ++k) {
+    if (batch->offsets[k + 1] < batch->offsets[k]) return fail(ctx, JDEM_EINVAL, "offsets must not decrease");
+    const jdem_domain& d = batch->domains[k];
+    if (d.kind != JDEM_DOMAIN_PERIODIC && d.kind != JDEM_DOMAIN_REFLECT)
+      return fail(ctx, JDEM_EUNSUPPORTED, "batched systems need a periodic or reflect domain");
+    const jdem_grid& g = batch->grids[k];
+    if (g.n_cells <= 0 || g.search_range < 1) return fail(ctx, JDEM_EINVAL, "bad cell grid of a batched system");
+    SegP& p = host[k];
+    std::memset(&p, 0, sizeof(p));
+    p.dom = dom_of(d);
+    p.grid = grid_of(g);
+    p.cell_base = (uint32_t)base;
+    p.dt = batch->dt[k];
+    const float lc = batch->cutoff[k] + batch->skin[k];
+    p.cutoff_sq = lc * lc;
+    const float sk = batch->skin[k] * batch->skin[k];
+    p.trigger = sk / 4.0f;
+    base += g.n_cells;
+    if (base > (int64_t)UINT32_MAX) return fail(ctx, JDEM_EUNSUPPORTED, "batch has more than 2^32 cells");
+  }
+  (void)hipSetDevice(ctx->device);
+  rc = ensure_flags(ctx);
+  if (rc) return rc;
+  hipStream_t st = ctx->stream;
+  ENSURE(ctx, ctx->segp, sizeof(SegP) * (size_t)B + 8 * ((size_t)B + 1));
+  ENSURE(ctx, ctx->segid, 4 * (size_t)std::max<int64_t>(N, 1));
+  SegP* dsegp = (SegP*)ctx->segp.p;
+  int64_t* doff = (int64_t*)((char*)ctx->segp.p + sizeof(SegP) * (size_t)B);
+  HIPCHK(ctx, hipMemcpyAsync(dsegp, host.data(), sizeof(SegP) * (size_t)B, hipMemcpyHostToDevice, st));
+  HIPCHK(ctx, hipMemcpyAsync(doff, batch->offsets, 8 * ((size_t)B + 1), hipMemcpyHostToDevice, st));
+  HIPCHK(ctx, ws_launch_seg_ids(doff, B, N, (int32_t*)ctx->segid.p, st));
+  // the host arrays above are read by the copies before this call returns
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  SegRun sg;
+  sg.n_seg = B;
+  sg.seg = (const int32_t*)ctx->segid.p;
+  sg.segp = dsegp;
+  sg.total_cells = base;
+  sg.rebuild_first = batch->rebuild_first != 0;
+  jdem_step_stats local;
+  jdem_step_stats* S = stats ? stats : &local;
+  std::memset(S, 0, sizeof(*S));
+  if (N == 0 || n == 0) {
+    S->steps = n;
+    return JDEM_OK;
+  }
+  return resident_nl_step(ctx, sys, n, S, &sg);
 }
 
 int jdem_minimize(jdem_ctx* ctx, jdem_system* sys, const jdem_fire* fire, int64_t max_steps, double pe_tol,

@@ -46,7 +46,7 @@ struct jdem_ctx {
   jdem::Buf red;                       // reduction partials
   jdem::Buf nl_copy, xi_copy, live_copy;
   // resident NeighborList working set (jdem_resident.hip)
-  jdem::Buf wP[2], wV[2], wW[2], wQ[2], wI[2], wF, wT, wOld, wOrig[2], wWpos, wSrc, wInv, wKeys, fireV, fireDev, wNl[2], wXi[2], wLive[2];
+  jdem::Buf wP[2], wV[2], wW[2], wQ[2], wI[2], wF, wT, wOld, wOrig[2], wWpos, wSrc, wInv, wKeys, fireV, fireDev, segp, segid, wNl[2], wXi[2], wLive[2];
   int32_t* host_flags = nullptr;       // pinned host mirror of flags
   // HIP-event timing (jdem_ctx_set_timing)
   struct TimedSpan {
@@ -215,6 +215,19 @@ struct InOut {
   uint64_t* live;
 };
 
+// Parameters of one system of a segmented batch (jdem_batch_step).  The
+// systems' particles occupy consecutive ranges of the work set in both the
+// caller's order and the cell order, because each system owns a consecutive
+// range of work cell keys starting at cell_base.
+struct SegP {
+  DomP dom;
+  GridP grid;
+  uint32_t cell_base;
+  float dt;
+  float cutoff_sq;  // (cutoff + skin)^2 of the list build
+  float trigger;    // rebuild threshold on |x - x_build|^2
+};
+
 struct NLW {
   int64_t n;
   int M;
@@ -240,6 +253,9 @@ struct NLW {
   const uint64_t* old_live;
   float4* xi;
   uint64_t* live;
+  // segmented batch: system of each work slot and the systems' parameters
+  const int32_t* seg;
+  const SegP* segp;
 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs of the chip, each
@@ -286,6 +302,9 @@ struct WS {
   const int32_t* flag_rd;
   int32_t* flag_wr;
   int32_t step_idx;
+  // segmented batch (null for one system)
+  const int32_t* seg;
+  const SegP* segp;
 };
 
 hipError_t ws_launch_in(const InOut& io, bool with_nl, hipStream_t s);
@@ -295,6 +314,10 @@ hipError_t ws_launch_iota(int32_t* p, int64_t n, hipStream_t s);
 hipError_t ws_launch_copy4(float4* d, const float4* src, int64_t n, hipStream_t s);
 hipError_t ws_launch_hash(const float4* P4, const int32_t* orig, int64_t n, int dim, const DomP& dom, const GridP& g,
                           uint32_t* keys, int32_t* wpos, hipStream_t s);
+hipError_t ws_launch_hash_seg(const float4* P4, const int32_t* orig, int64_t n, int dim, const int32_t* seg,
+                              const SegP* segp, uint32_t* keys, int32_t* wpos, hipStream_t s);
+// seg[b] = system of particle b for the ranges offsets[0..n_seg]
+hipError_t ws_launch_seg_ids(const int64_t* offsets, int n_seg, int64_t n, int32_t* seg, hipStream_t s);
 hipError_t ws_launch_permute(int64_t n, const int32_t* new_orig, const int32_t* wpos, const float4* P,
                              const float4* V, const float4* W, const float4* Q, const float4* I, float4* P2,
                              float4* V2, float4* W2, float4* Q2, float4* I2, int32_t* orig2, int32_t* src,

@@ -203,23 +203,56 @@ __global__ void __launch_bounds__(kBlock) k_hash_work(const float4* __restrict__
   wpos[i] = (int32_t)b;
 }
 
-// Dense cell starts: starts[h] = first sorted slot with key >= h (a lower
-// bound in the sorted keys), for h in [0, n_cells]; an empty cell gets the
-// start of the next occupied one, so consecutive cells h1..h2 hold the slots
-// [starts[h1], starts[h2 + 1]).
-__global__ void __launch_bounds__(kBlock) k_cell_starts(const uint32_t* __restrict__ sk, int64_t n, int64_t n_cells,
-                                                         int32_t* __restrict__ starts) {
-  const int64_t h = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (h > n_cells) return;
-  int64_t lo = 0, hi = n;
-  while (lo < hi) {
-    const int64_t mid = (lo + hi) >> 1;
-    if ((int64_t)sk[mid] < h)
-      lo = mid + 1;
+// The same for a segmented batch: the cell of slot b in its system's grid,
+// offset by the system's first key.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_hash_seg(const float4* __restrict__ P4, const int32_t* __restrict__ orig,
+                                                      int64_t n, const int32_t* __restrict__ seg,
+                                                      const SegP* __restrict__ segp, uint32_t* __restrict__ keys,
+                                                      int32_t* __restrict__ wpos) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  const SegP& sp = segp[seg[b]];
+  const float4 p = P4[b];
+  const float x[3] = {p.x, p.y, p.z};
+  int c[3] = {0, 0, 0};
+  const bool per = sp.dom.kind == JDEM_DOMAIN_PERIODIC;
+#pragma unroll
+  for (int a = 0; a < D; ++a) c[a] = per ? ccoord<true>(x[a], a, sp.dom, sp.grid) : ccoord<false>(x[a], a, sp.dom, sp.grid);
+  const int32_t i = orig[b];
+  keys[i] = sp.cell_base + wkey<D>(c, sp.grid);
+  wpos[i] = (int32_t)b;
+}
+
+// seg[b] = the system whose particle range [offsets[s], offsets[s + 1])
+// holds b
+__global__ void __launch_bounds__(kBlock) k_seg_ids(const int64_t* __restrict__ offsets, int n_seg, int64_t n,
+                                                     int32_t* __restrict__ seg) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  int lo = 0, hi = n_seg;  // largest s with offsets[s] <= b
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (offsets[mid] <= b)
+      lo = mid;
     else
       hi = mid;
   }
-  starts[h] = (int32_t)lo;
+  seg[b] = lo;
+}
+
+// Dense cell starts: starts[h] = first sorted slot with key >= h, for h in
+// [0, n_cells]; an empty cell gets the start of the next occupied one, so
+// consecutive cells h1..h2 hold the slots [starts[h1], starts[h2 + 1]).
+// Thread s (0 <= s <= n) writes the cells in (sk[s-1], sk[s]], the gap
+// before slot s, so every cell is written once.
+__global__ void __launch_bounds__(kBlock) k_cell_starts(const uint32_t* __restrict__ sk, int64_t n, int64_t n_cells,
+                                                         int32_t* __restrict__ starts) {
+  const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (s > n) return;
+  const int64_t lo = (s == 0) ? 0 : (int64_t)sk[s - 1] + 1;
+  const int64_t hi = (s == n) ? n_cells : (int64_t)sk[s];
+  for (int64_t h = lo; h <= hi; ++h) starts[h] = (int32_t)s;
 }
 
 // new work slot b' takes old slot src = wpos[new_orig[b']]; inv[src] = b'
@@ -262,9 +295,8 @@ JD bool bonded_o(const int32_t* bond_id, int bond_w, int64_t i, int32_t j) {
 // -sr..hi (hi = min(sr, dims - sr - 1) on a periodic axis, which drops the
 // offsets that would revisit a cell) give at most two runs of consecutive
 // cells, [r0a, r0b] then [r1a, r1b] (empty when a > b).
-template <bool PER>
-JD void stencil_runs(int c, int sr, int dims, int& r0a, int& r0b, int& r1a, int& r1b) {
-  if (PER) {
+JD void stencil_runs(bool per, int c, int sr, int dims, int& r0a, int& r0b, int& r1a, int& r1b) {
+  if (per) {
     const int cnt = sr + min(sr, dims - sr - 1) + 1;  // in [1, dims]
     int start = (c - sr) % dims;
     if (start < 0) start += dims;
@@ -282,10 +314,9 @@ JD void stencil_runs(int c, int sr, int dims, int& r0a, int& r0b, int& r1a, int&
 }
 
 // Offsets along a slow axis; returns the cell or -1 when it is off the grid.
-template <bool PER>
-JD int stencil_cell(int c, int o, int dims) {
+JD int stencil_cell(bool per, int c, int o, int dims) {
   int x = c + o;
-  if (PER) {
+  if (per) {
     x %= dims;
     if (x < 0) x += dims;
     return x;
@@ -373,7 +404,9 @@ JD void end_row(const NLW& a, int64_t b, int count) {
 // slot are contiguous; later slots go straight to memory.
 constexpr int kRowLds = 24;
 
-template <int D, bool PER>
+// PM: 0 = one non-periodic system, 1 = one periodic system, 2 = a segmented
+// batch, where each work slot takes the grid, domain and cutoff of its system.
+template <int D, int PM>
 __global__ void __launch_bounds__(kBlock) k_nlist_work(NLW a) {
   __shared__ int32_t rows[kRowLds][kBlock];
   __shared__ int row_max;
@@ -384,9 +417,25 @@ __global__ void __launch_bounds__(kBlock) k_nlist_work(NLW a) {
   const bool active = b < a.n;
   int count = 0;
   if (active) {
+    GridP grid;
+    DomP dom;
+    float cutoff_sq;
+    uint32_t cbase = 0;
+    if (PM == 2) {
+      const SegP& sp = a.segp[a.seg[b]];
+      grid = sp.grid;
+      dom = sp.dom;
+      cutoff_sq = sp.cutoff_sq;
+      cbase = sp.cell_base;
+    } else {
+      grid = a.grid;
+      dom = a.dom;
+      cutoff_sq = a.cutoff_sq;
+    }
+    const bool PER = PM == 2 ? dom.kind == JDEM_DOMAIN_PERIODIC : PM == 1;
     const Vec<D> xi = xyz<D>(a.P4[b]);
-    const uint32_t key = a.skeys[b];
-    const int d0 = a.grid.dims[0], d1 = a.grid.dims[1], d2 = (D == 3) ? a.grid.dims[2] : 1;
+    const uint32_t key = a.skeys[b] - cbase;
+    const int d0 = grid.dims[0], d1 = grid.dims[1], d2 = (D == 3) ? grid.dims[2] : 1;
     int c0, c1, cf;
     if (D == 3) {
       cf = (int)(key % (uint32_t)d2);
@@ -398,10 +447,10 @@ __global__ void __launch_bounds__(kBlock) k_nlist_work(NLW a) {
       c0 = (int)(key / (uint32_t)d1);
       c1 = 0;
     }
-    const int sr = a.grid.sr;
+    const int sr = grid.sr;
     const int dF = (D == 3) ? d2 : d1;
     int r0a, r0b, r1a, r1b;
-    stencil_runs<PER>(cf, sr, dF, r0a, r0b, r1a, r1b);
+    stencil_runs(PER, cf, sr, dF, r0a, r0b, r1a, r1b);
     const int hi0 = PER ? min(sr, d0 - sr - 1) : sr;
     const int lo1 = (D == 3) ? -sr : 0;
     const int hi1 = (D == 3) ? (PER ? min(sr, d1 - sr - 1) : sr) : 0;
@@ -409,16 +458,16 @@ __global__ void __launch_bounds__(kBlock) k_nlist_work(NLW a) {
     const int64_t ib = chk_bond ? (int64_t)a.orig[b] : 0;
     const int32_t* __restrict__ starts = a.starts;
     for (int ox = -sr; ox <= hi0; ++ox) {
-      const int cx = stencil_cell<PER>(c0, ox, d0);
+      const int cx = stencil_cell(PER, c0, ox, d0);
       if (cx < 0) continue;
       for (int oy = lo1; oy <= hi1; ++oy) {
-        int base;
+        int64_t base = cbase;
         if (D == 3) {
-          const int cy = stencil_cell<PER>(c1, oy, d1);
+          const int cy = stencil_cell(PER, c1, oy, d1);
           if (cy < 0) continue;
-          base = (cx * d1 + cy) * d2;
+          base += (int64_t)(cx * d1 + cy) * d2;
         } else {
-          base = cx * d1;
+          base += (int64_t)cx * d1;
         }
         // both runs' bounds are loaded before either is walked
         const int k0 = starts[base + r0a], k0e = starts[base + r0b + 1];
@@ -437,7 +486,9 @@ __global__ void __launch_bounds__(kBlock) k_nlist_work(NLW a) {
               const int k = kg + u;
               if (k >= ke) break;
               if (k == b) continue;
-              if (!(dist_sq<PER>(xi, xyz<D>(pk[u]), a.dom) <= a.cutoff_sq)) continue;
+              const float dsq =
+                  PER ? dist_sq<true>(xi, xyz<D>(pk[u]), dom) : dist_sq<false>(xi, xyz<D>(pk[u]), dom);
+              if (!(dsq <= cutoff_sq)) continue;
               if (chk_bond && bonded_o(a.bond_id, a.bond_w, ib, a.orig[k])) continue;
               if (count < kRowLds)
                 rows[count][tid] = k;
@@ -478,21 +529,44 @@ __global__ void __launch_bounds__(kBlock) k_iota(int32_t* __restrict__ p, int64_
 // ---------------------------------------------------------------------------
 
 // Minimum waves per SIMD asked of the compiler: six (80 VGPRs) holds the
-// spring-friction law and the integrators without spills; the Cundall-Strack
-// and Hertz-friction laws need five.
-template <int D, int LAW>
+// spring-friction law and the integrators; the Cundall-Strack and
+// Hertz-friction laws in 3D need five, and one fewer in a segmented batch,
+// whose per-system domain takes registers too.
+template <int D, int LAW, int PM>
 constexpr int ws_waves() {
-  return (D == 3 && (LAW == JDEM_LAW_CUNDALL_STRACK || LAW == JDEM_LAW_HERTZ_FRICTION)) ? 5 : 6;
+  const bool heavy = D == 3 && (LAW == JDEM_LAW_CUNDALL_STRACK || LAW == JDEM_LAW_HERTZ_FRICTION);
+  const bool seg3 = D == 3 && PM == 2 && LAW != JDEM_LAW_SPRING && LAW != JDEM_LAW_HERTZ;
+  return heavy ? (PM == 2 ? 4 : 5) : (seg3 ? 5 : 6);
 }
 
-template <int D, int LAW, bool PER>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_waves<D, LAW>(), 8))) k_ws_step(WS a) {
+// Domain._displacement for mode PM: 0 / 1 = one non-periodic / periodic
+// system, 2 = the domain of the slot's system in a segmented batch.
+template <int PM, int D>
+JD Vec<D> disp_pm(Vec<D> a, Vec<D> b, const DomP& d) {
+  if (PM == 2) return d.kind == JDEM_DOMAIN_PERIODIC ? disp_<true>(a, b, d) : disp_<false>(a, b, d);
+  return disp_<PM == 1>(a, b, d);
+}
+
+template <int D, int LAW, int PM>
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_waves<D, LAW, PM>(), 8))) k_ws_step(WS a) {
   if (a.flag_rd && *a.flag_rd) {
     if (a.flag_wr) *a.flag_wr = *a.flag_rd;
     return;
   }
   const int64_t b = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
   if (b >= a.n) return;
+  DomP dom;
+  float dt, trigger;
+  if (PM == 2) {
+    const SegP& sp = a.segp[a.seg[b]];
+    dom = sp.dom;
+    dt = sp.dt;
+    trigger = sp.trigger;
+  } else {
+    dom = a.dom;
+    dt = a.dt;
+    trigger = a.trigger;
+  }
   const float4 p4 = a.P[b], v4 = a.V[b], w4 = a.W[b], i4 = a.I[b];
   Side<D> me;
   me.x = xyz<D>(p4);
@@ -532,7 +606,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
           break;
         }
         const float4 pj = pjs[k];
-        const Vec<D> rij = disp_<PER>(me.x, xyz<D>(pj), a.dom);
+        const Vec<D> rij = disp_pm<PM>(me.x, xyz<D>(pj), dom);
         const float d_sq = dot(rij, rij);
         const float R = me.r + pj.w;
         if (d_sq > (R * R) * 1.0001f) continue;
@@ -558,7 +632,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
         cm &= cm - 1ull;
         const int32_t j = a.nl[(int64_t)s * a.n + b];
         const float4 pj = a.P[j];
-        const Vec<D> rij = disp_<PER>(me.x, xyz<D>(pj), a.dom);
+        const Vec<D> rij = disp_pm<PM>(me.x, xyz<D>(pj), dom);
         const float d_sq = dot(rij, rij);
         Side<D> o;
         o.x = xyz<D>(pj);
@@ -577,7 +651,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
             xold = xyz<D>(a.xi_rd[(int64_t)k * a.n + b]);
           }
         }
-        PairOut<D> r = law_eval<D, LAW>(a.law, me, o, rij, d_sq, xold, a.dt);
+        PairOut<D> r = law_eval<D, LAW>(a.law, me, o, rij, d_sq, xold, dt);
         F = F + r.F;
         if constexpr (LawTraits<LAW>::torque) T = T + r.T;
         if constexpr (LawTraits<LAW>::hist) {
@@ -616,26 +690,26 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
       if (a.ext_t) T = T + lda<D>(a.ext_t, i);
     }
     if (a.lin == JDEM_LIN_VERLET) {
-      const float s = (a.dt * 0.5f) / m;
+      const float s = (dt * 0.5f) / m;
       v = v + (F * s) * freef;
     } else if (a.lin == JDEM_LIN_EULER) {
-      const float s = a.dt / m;
+      const float s = dt / m;
       v = v + (F * s) * freef;
-      x = x + v * a.dt;
+      x = x + v * dt;
     }
     if (a.rot == JDEM_ROT_VERLET_SPIRAL)
-      vvs_after<D>(q, w, T, I, a.dt, freef);
+      vvs_after<D>(q, w, T, I, dt, freef);
     else if (a.rot == JDEM_ROT_SPIRAL)
-      spiral_after<D>(q, w, T, I, a.dt, freef);
+      spiral_after<D>(q, w, T, I, dt, freef);
   }
   if (a.do_before) {
-    if (a.dom.kind == JDEM_DOMAIN_REFLECT) reflect_apply<D>(a.dom, x, v, q, F, m, me.r, fixed, a.dt);
+    if (dom.kind == JDEM_DOMAIN_REFLECT) reflect_apply<D>(dom, x, v, q, F, m, me.r, fixed, dt);
     if (a.lin == JDEM_LIN_VERLET) {
-      const float s = (a.dt * 0.5f) / m;
+      const float s = (dt * 0.5f) / m;
       v = v + (F * s) * freef;
-      x = x + v * a.dt;
+      x = x + v * dt;
     }
-    if (a.rot == JDEM_ROT_VERLET_SPIRAL) vvs_before<D>(q, w, T, I, a.dt, freef);
+    if (a.rot == JDEM_ROT_VERLET_SPIRAL) vvs_before<D>(q, w, T, I, dt, freef);
   }
   a.P2[b] = f4(x, me.r);
   a.V2[b] = f4(v, m);
@@ -647,17 +721,19 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
   }
   if (a.do_before && a.check_disp) {
     const Vec<D> d = x - xyz<D>(a.old4[b]);
-    if (dot(d, d) > a.trigger) *a.flag_wr = a.step_idx + 1;
+    if (dot(d, d) > trigger) *a.flag_wr = a.step_idx + 1;
   }
 }
 
 template <int D, int LAW>
 hipError_t launch_ws_law(const WS& a, bool per, hipStream_t s) {
   dim3 g(nblocks(a.n)), b(kBlock);
-  if (per)
-    hipLaunchKernelGGL((k_ws_step<D, LAW, true>), g, b, 0, s, a);
+  if (a.segp)
+    hipLaunchKernelGGL((k_ws_step<D, LAW, 2>), g, b, 0, s, a);
+  else if (per)
+    hipLaunchKernelGGL((k_ws_step<D, LAW, 1>), g, b, 0, s, a);
   else
-    hipLaunchKernelGGL((k_ws_step<D, LAW, false>), g, b, 0, s, a);
+    hipLaunchKernelGGL((k_ws_step<D, LAW, 0>), g, b, 0, s, a);
   return hipGetLastError();
 }
 
@@ -759,6 +835,20 @@ hipError_t ws_launch_hash(const float4* P4, const int32_t* orig, int64_t n, int 
   }
   return hipGetLastError();
 }
+hipError_t ws_launch_hash_seg(const float4* P4, const int32_t* orig, int64_t n, int dim, const int32_t* seg,
+                              const SegP* segp, uint32_t* keys, int32_t* wpos, hipStream_t s) {
+  dim3 gr(nblocks(n)), b(kBlock);
+  if (dim == 3)
+    hipLaunchKernelGGL(k_hash_seg<3>, gr, b, 0, s, P4, orig, n, seg, segp, keys, wpos);
+  else
+    hipLaunchKernelGGL(k_hash_seg<2>, gr, b, 0, s, P4, orig, n, seg, segp, keys, wpos);
+  return hipGetLastError();
+}
+hipError_t ws_launch_seg_ids(const int64_t* offsets, int n_seg, int64_t n, int32_t* seg, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_seg_ids, dim3(nblocks(n)), dim3(kBlock), 0, s, offsets, n_seg, n, seg);
+  return hipGetLastError();
+}
 hipError_t ws_launch_permute(int64_t n, const int32_t* new_orig, const int32_t* wpos, const float4* P,
                              const float4* V, const float4* W, const float4* Q, const float4* I, float4* P2,
                              float4* V2, float4* W2, float4* Q2, float4* I2, int32_t* orig2, int32_t* src,
@@ -768,7 +858,7 @@ hipError_t ws_launch_permute(int64_t n, const int32_t* new_orig, const int32_t* 
   return hipGetLastError();
 }
 hipError_t ws_launch_cell_starts(const uint32_t* skeys, int64_t n, int64_t n_cells, int32_t* starts, hipStream_t s) {
-  hipLaunchKernelGGL(k_cell_starts, dim3(nblocks(n_cells + 1)), dim3(kBlock), 0, s, skeys, n, n_cells, starts);
+  hipLaunchKernelGGL(k_cell_starts, dim3(nblocks(n + 1)), dim3(kBlock), 0, s, skeys, n, n_cells, starts);
   return hipGetLastError();
 }
 hipError_t ws_launch_nlist(const NLW& a, int dim, bool naive, hipStream_t s) {
@@ -786,17 +876,22 @@ hipError_t ws_launch_nlist(const NLW& a, int dim, bool naive, hipStream_t s) {
       else
         hipLaunchKernelGGL((k_nlist_work_naive<2, false>), g, b, 0, s, a);
     }
+  } else if (a.segp) {
+    if (dim == 3)
+      hipLaunchKernelGGL((k_nlist_work<3, 2>), g, b, 0, s, a);
+    else
+      hipLaunchKernelGGL((k_nlist_work<2, 2>), g, b, 0, s, a);
   } else {
     if (dim == 3) {
       if (per)
-        hipLaunchKernelGGL((k_nlist_work<3, true>), g, b, 0, s, a);
+        hipLaunchKernelGGL((k_nlist_work<3, 1>), g, b, 0, s, a);
       else
-        hipLaunchKernelGGL((k_nlist_work<3, false>), g, b, 0, s, a);
+        hipLaunchKernelGGL((k_nlist_work<3, 0>), g, b, 0, s, a);
     } else {
       if (per)
-        hipLaunchKernelGGL((k_nlist_work<2, true>), g, b, 0, s, a);
+        hipLaunchKernelGGL((k_nlist_work<2, 1>), g, b, 0, s, a);
       else
-        hipLaunchKernelGGL((k_nlist_work<2, false>), g, b, 0, s, a);
+        hipLaunchKernelGGL((k_nlist_work<2, 0>), g, b, 0, s, a);
     }
   }
   return hipGetLastError();

@@ -128,6 +128,58 @@ def spheres_friction(N: int = 1 << 20, seed: int = 0, phi: float = 0.50, skin: f
     )
 
 
+def disks_hertz_friction(N: int, seed: int, phi: float, skin: float = 0.1, E: float = 1e4, mu: float = 0.3,
+                         dt: float = 2e-4) -> Workload:
+    """One 2D system of the batching workload: N disks, radii U[0.4, 0.6],
+    periodic square box at area fraction phi, on the first N sites of a
+    jittered square lattice (+/-5 % of the spacing), velocities N(0, 1) with
+    zero net momentum, unit density, Hertz normal contact (E, nu = 0.3) with
+    the tangential spring and Coulomb friction mu, NeighborList over a
+    CellList, VV + VV-SPIRAL, float32."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rad = rng.uniform(0.4, 0.6, N).astype(np.float32)
+    area = float(np.sum(math.pi * rad.astype(np.float64) ** 2))
+    L = math.sqrt(area / phi)
+    side = int(math.ceil(math.sqrt(N)))
+    spacing = L / side
+    idx = np.arange(N)
+    ij = np.stack([idx // side, idx % side], axis=1).astype(np.float64)
+    pos = (ij + 0.5) * spacing + rng.uniform(-0.05 * spacing, 0.05 * spacing, (N, 2))
+    vel = rng.standard_normal((N, 2))
+    vel = (vel - vel.mean(axis=0)).astype(np.float32)
+    mass = (math.pi * rad.astype(np.float64) ** 2).astype(np.float32)
+    return Workload(
+        name=f"disks_{N}_hertzfriction",
+        pos=pos.astype(np.float32),
+        vel=vel,
+        rad=rad,
+        mass=mass,
+        box=np.full(2, L, np.float32),
+        anchor=np.zeros(2, np.float32),
+        domain="periodic",
+        dt=dt,
+        law="hertzfriction",
+        young=E,
+        mu=mu,
+        damping=0.0,
+        collider="neighborlist",
+        cell_size=1.2,
+        cutoff=1.2,
+        skin=skin,
+        extra={"phi": phi, "L": L},
+    )
+
+
+def batch_disks(n_systems: int = 4096, seed: int = 0, n_min: int = 256, n_max: int = 16384) -> list:
+    """The batching workload: n_systems independent disk systems, N
+    log-uniform in [n_min, n_max] and area fraction uniform in [0.3, 0.8]
+    (both seeded); system k uses seed (seed, k) for its own arrays."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    Ns = np.rint(np.exp(rng.uniform(math.log(n_min), math.log(n_max), n_systems))).astype(int)
+    phis = rng.uniform(0.3, 0.8, n_systems)
+    return [disks_hertz_friction(int(N), seed=seed * 1000003 + k, phi=float(p)) for k, (N, p) in enumerate(zip(Ns, phis))]
+
+
 def make_gpu_system(w: Workload, device="cuda", max_neighbors: int | None = None, collider: str | None = None):
     """(State, System) of the jaxdem_amd API for a workload."""
     from . import Material, MaterialMatchmaker, MaterialTable, State, System
@@ -168,4 +220,12 @@ def make_gpu_system(w: Workload, device="cuda", max_neighbors: int | None = None
     return st, sysm
 
 
-__all__ = ["Workload", "spheres_1k", "spheres_friction", "make_gpu_system", "sphere_volume"]
+__all__ = [
+    "Workload",
+    "spheres_1k",
+    "spheres_friction",
+    "disks_hertz_friction",
+    "batch_disks",
+    "make_gpu_system",
+    "sphere_volume",
+]

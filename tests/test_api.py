@@ -191,3 +191,27 @@ def test_bench_refuses_more_than_one_gpu():
     root = Path(__file__).resolve().parent.parent
     r = subprocess.run([sys.executable, str(root / "bench.py"), "--gpus", "2"], capture_output=True, text=True)
     assert r.returncode == 2 and "one GPU" in r.stderr
+
+
+def test_batch_pack_unpack_round_trip():
+    """Batch.pack concatenates states, merges material tables block-diagonally
+    and pads the neighbour lists; unpack gives every system back unchanged."""
+    import torch
+
+    from jaxdem_amd import workloads as W
+
+    ws = [W.disks_hertz_friction(N, seed=k, phi=p) for k, (N, p) in enumerate([(300, 0.5), (700, 0.7), (256, 0.3)])]
+    pairs = [W.make_gpu_system(w, device="cpu") for w in ws]
+    b = jd.Batch.pack(pairs)
+    assert b.n_particles == 1256 and list(b.offsets) == [0, 300, 1000, 1256]
+    assert len(b.system.mat_table) == 3
+    assert b.system.collider.max_neighbors == max(p[1].collider.max_neighbors for p in pairs)
+    assert int(b.state.mat_id.max()) == 2
+    for (s0, y0), (s1, y1) in zip(pairs, b.unpack()):
+        for f in ("pos_c", "vel", "rad", "mass", "mat_id", "clump_id", "bond_id", "fixed"):
+            assert torch.equal(getattr(s0, f), getattr(s1, f)), f
+        assert torch.equal(s0.q.data, s1.q.data)
+        assert torch.equal(y0.collider.neighbor_list, y1.collider.neighbor_list)
+        assert torch.equal(y0.collider.history["xi"], y1.collider.history["xi"])
+    with pytest.raises(RuntimeError, match="GPU"):
+        b.step(1)
