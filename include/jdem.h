@@ -322,8 +322,8 @@ int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
  * cell_list.py:537-538), dt, and the NeighborList cutoff and skin. */
 typedef struct jdem_batch {
   int32_t n_systems;
-  int32_t rebuild_first; /* build all lists at the first step: some system was
-                            never built (its rows are all -1) */
+  int32_t reserved;
+  int32_t* n_build_times; /* in/out: NeighborList builds of each system       */
   const int64_t* offsets;
   const jdem_domain* domains;
   const jdem_grid* grids;
@@ -338,11 +338,11 @@ typedef struct jdem_batch {
  * shared law (material tables; mat_id selects a system's material),
  * integrators, gravity and the NeighborList buffers (one max_neighbors for
  * all systems; secondary = CELL_LIST); sys->dt, sys->domain and
- * sys->nlist.{grid,cutoff,skin} are ignored.  The systems' lists are rebuilt
- * together whenever one of them needs it; every trajectory equals that of
- * the same system stepped alone (jdem_system_step), bit for bit, because a
- * list rebuild changes no force.  sys->nlist.n_build_times counts the
- * batch's builds. */
+ * sys->nlist.{grid,cutoff,skin,n_build_times} are ignored.  Each system
+ * rebuilds its list when its own particles moved more than its skin/2
+ * (neighbor_list.py:78-85), at the same steps as alone, so every trajectory
+ * equals that of the same system stepped alone (jdem_system_step) bit for
+ * bit.  sys->nlist.overflow is the OR over the systems. */
 int jdem_batch_step(jdem_ctx* ctx, jdem_system* sys, const jdem_batch* batch, int64_t n, jdem_step_stats* stats);
 
 /* ---- (e) energy minimisation -------------------------------------------------- */

@@ -152,7 +152,8 @@ class JdemTiming(ctypes.Structure):
 class JdemBatch(ctypes.Structure):
     _fields_ = [
         ("n_systems", c_int32),
-        ("rebuild_first", c_int32),
+        ("reserved", c_int32),
+        ("n_build_times", c_void_p),
         ("offsets", c_void_p),
         ("domains", c_void_p),
         ("grids", c_void_p),

@@ -77,6 +77,7 @@ class Batch:
     mat_offsets: np.ndarray
     systems: list
     bond_widths: list
+    n_build: list  # NeighborList builds of each system
 
     def __len__(self) -> int:
         return len(self.systems)
@@ -169,8 +170,7 @@ class Batch:
             n_build_times=max(built), overflow=any(sy.collider.overflow for sy in systems),
         )
         packed = replace(y0, collider=col, mat_table=mt)
-        packed.__dict__["_rebuild_first"] = min(built) == 0 and max(built) > 0
-        return Batch(state, packed, offsets, moff, list(systems), bw)
+        return Batch(state, packed, offsets, moff, list(systems), bw, built)
 
     # ------------------------------------------------------------------
     def _c_arrays(self, keep: list) -> L.JdemBatch:
@@ -190,10 +190,11 @@ class Batch:
             dt[k] = F32(sy.dt)
             cut[k] = F32(sy.collider.cutoff)
             skin[k] = F32(sy.collider.skin)
-        keep += [off, doms, grids, dt, cut, skin]
+        nb = np.ascontiguousarray(self.n_build, np.int32)
+        keep += [off, doms, grids, dt, cut, skin, nb]
         b = L.JdemBatch()
         b.n_systems = B
-        b.rebuild_first = 1 if self.system.__dict__.get("_rebuild_first") else 0
+        b.n_build_times = nb.ctypes.data
         b.offsets = off.ctypes.data
         b.domains = ctypes.addressof(doms)
         b.grids = ctypes.addressof(grids)
@@ -217,11 +218,12 @@ class Batch:
         ctx = _ctx(st)
         ctx.check(ctx.lib.jdem_batch_step(ctx.ptr, ctypes.byref(S), ctypes.byref(Bc), int(n), ctypes.byref(stats)),
                   "jdem_batch_step")
-        col2 = replace(col, n_build_times=int(S.nlist.n_build_times), overflow=bool(S.nlist.overflow))
+        n_build = [int(v) for v in keep[-1]]
+        col2 = replace(col, n_build_times=max(n_build), overflow=bool(S.nlist.overflow))
         sy2 = replace(sy, collider=col2)
         sy2.__dict__["_last_stats"] = stats
         systems = [_advance_clock(s, int(n)) for s in self.systems]
-        return Batch(st, sy2, self.offsets, self.mat_offsets, systems, self.bond_widths)
+        return Batch(st, sy2, self.offsets, self.mat_offsets, systems, self.bond_widths, n_build)
 
     def unpack(self) -> list:
         """The systems again, as (State, System) pairs.  A system's list keeps
@@ -263,7 +265,7 @@ class Batch:
                     hist["live"][:, -1] &= (1 << (Mk % 32)) - 1
             ck = replace(
                 c0, neighbor_list=rows[:, :Mk].contiguous().clone(), old_pos=col.old_pos[a:b].clone(),
-                history=hist, n_build_times=int(col.n_build_times), overflow=overflow,
+                history=hist, n_build_times=int(self.n_build[k]), overflow=overflow,
             )
             out.append((sk, replace(sy, collider=ck)))
         return out

@@ -21,7 +21,7 @@ std::vector<Buf*> jdem_ctx::bufs() {
   std::vector<Buf*> all = {&pos2,    &vel2,   &ang2,     &keys[0], &keys[1], &vals[0], &vals[1], &hist,
                            &cstart,  &cend,   &spos,     &nl_new,  &xi_new,  &live_new, &flags,  &red,
                            &nl_copy, &xi_copy, &live_copy, &wF,     &wT,      &wOld,     &wWpos, &wSrc,
-                           &wInv,    &wKeys,  &fireV,    &fireDev, &segp,  &segid};
+                           &wInv,    &wKeys,  &fireV,    &fireDev, &segp,  &segid, &segflag, &wOld2};
   for (int k = 0; k < 2; ++k)
     for (Buf* b : {&wP[k], &wV[k], &wW[k], &wQ[k], &wI[k], &wOrig[k], &wNl[k], &wXi[k], &wLive[k]}) all.push_back(b);
   return all;
@@ -439,7 +439,10 @@ struct SegRun {
   const int32_t* seg;
   const SegP* segp;
   int64_t total_cells;
-  bool rebuild_first;  // build at the first step although lists exist
+  int32_t* n_build;   // builds of each system (device)
+  int32_t* seg_flag;  // system asks for a rebuild (device)
+  int32_t* seg_mode;  // what the current rebuild does to it (device)
+  bool any_unbuilt;   // some system has no list yet
 };
 
 int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats* S, const SegRun* sg = nullptr) {
@@ -470,6 +473,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   ENSURE(ctx, ctx->wF, n4);
   ENSURE(ctx, ctx->wT, n4);
   ENSURE(ctx, ctx->wOld, n4);
+  if (sg) ENSURE(ctx, ctx->wOld2, n4);
   ENSURE(ctx, ctx->wWpos, ni);
   ENSURE(ctx, ctx->wSrc, ni);
   ENSURE(ctx, ctx->wInv, ni);
@@ -543,6 +547,8 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     int32_t* src = I32(ctx->wSrc);
     int32_t* inv = I32(ctx->wInv);
     uint32_t* sk = nullptr;
+    // systems of a batch: which ones rebuild now (the others keep their rows)
+    if (sg) HIPCHK(ctx, ws_launch_seg_modes(sg->n_seg, sg->seg_flag, sg->n_build, sg->seg_mode, st));
     if (!naive2) {
       if (sg) {
         HIPCHK(ctx, ws_launch_hash_seg(F4(ctx->wP[cur]), I32(ctx->wOrig[set]), N, D, sg->seg, sg->segp,
@@ -556,7 +562,8 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       HIPCHK(ctx, ws_launch_permute(N, new_orig, I32(ctx->wWpos), F4(ctx->wP[cur]), F4(ctx->wV[cur]),
                                     F4(ctx->wW[cur]), F4(ctx->wQ[set]), F4(ctx->wI[set]), F4(ctx->wP[1 - cur]),
                                     F4(ctx->wV[1 - cur]), F4(ctx->wW[1 - cur]), F4(ctx->wQ[1 - set]),
-                                    F4(ctx->wI[1 - set]), I32(ctx->wOrig[1 - set]), src, inv, st));
+                                    F4(ctx->wI[1 - set]), I32(ctx->wOrig[1 - set]), src, inv,
+                                    sg ? F4(ctx->wOld) : nullptr, sg ? F4(ctx->wOld2) : nullptr, st));
       cur = 1 - cur;
       set = 1 - set;
       HIPCHK(ctx, ws_launch_cell_starts(sk, N, n_cells, I32(ctx->cstart), st));
@@ -582,6 +589,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       if (sg) {
         b.seg = sg->seg;
         b.segp = sg->segp;
+        b.seg_mode = sg->seg_mode;
       }
       b.dom = dom;
       b.nl = I32(ctx->wNl[1 - nlc]);
@@ -601,7 +609,13 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       nlc = 1 - nlc;
       xc = 1 - xc;
     }
-    HIPCHK(ctx, ws_launch_copy4(F4(ctx->wOld), F4(ctx->wP[cur]), N, st));
+    if (sg) {
+      // rebuilt systems take the current positions as their build positions
+      HIPCHK(ctx, ws_launch_old_update(N, sg->seg, sg->seg_mode, F4(ctx->wP[cur]), F4(ctx->wOld2), st));
+      std::swap(ctx->wOld, ctx->wOld2);
+    } else {
+      HIPCHK(ctx, ws_launch_copy4(F4(ctx->wOld), F4(ctx->wP[cur]), N, st));
+    }
     L.n_build_times += 1;
     span_end(ctx, sp);
     return JDEM_OK;
@@ -678,6 +692,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     if (sg) {
       a.seg = sg->seg;
       a.segp = sg->segp;
+      a.seg_flag = sg->seg_flag;
     }
     const int sp = span_begin(ctx, 0, t);
     hipError_t e = ws_launch_step(a, D, sys->law.kind, st);
@@ -690,7 +705,8 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
 
   int rc = launch_t(0);
   if (rc) return rc;
-  bool pending = L.n_build_times == 0 || (sg && sg->rebuild_first);
+  bool pending = sg ? sg->any_unbuilt : L.n_build_times == 0;
+  if (sg) HIPCHK(ctx, ws_launch_seg_init_flags(sg->n_seg, sg->n_build, sg->seg_flag, st));
   int64_t next = 1;
   int64_t chunk_len = 8;
   int64_t last_rebuild_at = 0;
@@ -739,6 +755,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   S->overflow = L.overflow;
   // --- exit: back to the caller's order and layouts ---
   io.orig = I32(ctx->wOrig[set]);
+  io.old4 = F4(ctx->wOld);
   io.P4 = F4(ctx->wP[cur]);
   io.V4 = F4(ctx->wV[cur]);
   io.W4 = F4(ctx->wW[cur]);
@@ -1063,7 +1080,7 @@ int jdem_batch_step(jdem_ctx* ctx, jdem_system* sys, const jdem_batch* batch, in
   int rc = check_system(ctx, sys, true);
   if (rc) return rc;
   if (!batch || batch->n_systems < 1 || !batch->offsets || !batch->domains || !batch->grids || !batch->dt ||
-      !batch->cutoff || !batch->skin)
+      !batch->cutoff || !batch->skin || !batch->n_build_times)
     return fail(ctx, JDEM_EINVAL, "incomplete batch description");
   if (n < 0) return fail(ctx, JDEM_EINVAL, "negative step count");
   if (sys->collider_kind != JDEM_COLLIDER_NEIGHBOR_LIST || sys->nlist.secondary != JDEM_COLLIDER_CELL_LIST)
@@ -1099,10 +1116,15 @@ int jdem_batch_step(jdem_ctx* ctx, jdem_system* sys, const jdem_batch* batch, in
   hipStream_t st = ctx->stream;
   ENSURE(ctx, ctx->segp, sizeof(SegP) * (size_t)B + 8 * ((size_t)B + 1));
   ENSURE(ctx, ctx->segid, 4 * (size_t)std::max<int64_t>(N, 1));
+  ENSURE(ctx, ctx->segflag, 3 * 4 * (size_t)B);
   SegP* dsegp = (SegP*)ctx->segp.p;
   int64_t* doff = (int64_t*)((char*)ctx->segp.p + sizeof(SegP) * (size_t)B);
+  int32_t* dbuild = (int32_t*)ctx->segflag.p;
+  bool any_unbuilt = false;
+  for (int k = 0; k < B; ++k) any_unbuilt |= batch->n_build_times[k] == 0;
   HIPCHK(ctx, hipMemcpyAsync(dsegp, host.data(), sizeof(SegP) * (size_t)B, hipMemcpyHostToDevice, st));
   HIPCHK(ctx, hipMemcpyAsync(doff, batch->offsets, 8 * ((size_t)B + 1), hipMemcpyHostToDevice, st));
+  HIPCHK(ctx, hipMemcpyAsync(dbuild, batch->n_build_times, 4 * (size_t)B, hipMemcpyHostToDevice, st));
   HIPCHK(ctx, ws_launch_seg_ids(doff, B, N, (int32_t*)ctx->segid.p, st));
   // the host arrays above are read by the copies before this call returns
   HIPCHK(ctx, hipStreamSynchronize(st));
@@ -1111,7 +1133,10 @@ int jdem_batch_step(jdem_ctx* ctx, jdem_system* sys, const jdem_batch* batch, in
   sg.seg = (const int32_t*)ctx->segid.p;
   sg.segp = dsegp;
   sg.total_cells = base;
-  sg.rebuild_first = batch->rebuild_first != 0;
+  sg.n_build = dbuild;
+  sg.seg_flag = dbuild + B;
+  sg.seg_mode = dbuild + 2 * B;
+  sg.any_unbuilt = any_unbuilt;
   jdem_step_stats local;
   jdem_step_stats* S = stats ? stats : &local;
   std::memset(S, 0, sizeof(*S));
@@ -1119,7 +1144,11 @@ int jdem_batch_step(jdem_ctx* ctx, jdem_system* sys, const jdem_batch* batch, in
     S->steps = n;
     return JDEM_OK;
   }
-  return resident_nl_step(ctx, sys, n, S, &sg);
+  rc = resident_nl_step(ctx, sys, n, S, &sg);
+  if (rc) return rc;
+  HIPCHK(ctx, hipMemcpyAsync(batch->n_build_times, dbuild, 4 * (size_t)B, hipMemcpyDeviceToHost, st));
+  HIPCHK(ctx, hipStreamSynchronize(st));
+  return JDEM_OK;
 }
 
 int jdem_minimize(jdem_ctx* ctx, jdem_system* sys, const jdem_fire* fire, int64_t max_steps, double pe_tol,

@@ -46,7 +46,7 @@ struct jdem_ctx {
   jdem::Buf red;                       // reduction partials
   jdem::Buf nl_copy, xi_copy, live_copy;
   // resident NeighborList working set (jdem_resident.hip)
-  jdem::Buf wP[2], wV[2], wW[2], wQ[2], wI[2], wF, wT, wOld, wOrig[2], wWpos, wSrc, wInv, wKeys, fireV, fireDev, segp, segid, wNl[2], wXi[2], wLive[2];
+  jdem::Buf wP[2], wV[2], wW[2], wQ[2], wI[2], wF, wT, wOld, wOrig[2], wWpos, wSrc, wInv, wKeys, fireV, fireDev, segp, segid, segflag, wOld2, wNl[2], wXi[2], wLive[2];
   int32_t* host_flags = nullptr;       // pinned host mirror of flags
   // HIP-event timing (jdem_ctx_set_timing)
   struct TimedSpan {
@@ -253,10 +253,14 @@ struct NLW {
   const uint64_t* old_live;
   float4* xi;
   uint64_t* live;
-  // segmented batch: system of each work slot and the systems' parameters
+  // segmented batch: system of each work slot, the systems' parameters and
+  // what this rebuild does to each system's rows (SEG_KEEP / SEG_FRESH /
+  // SEG_REMAP)
   const int32_t* seg;
   const SegP* segp;
+  const int32_t* seg_mode;
 };
+enum { SEG_KEEP = 0, SEG_FRESH = 1, SEG_REMAP = 2 };
 
 // Workgroups are dispatched round-robin over the 8 XCDs of the chip, each
 // with its own L2: workgroup w runs on XCD w % 8.  xcd_tile maps them so
@@ -302,9 +306,11 @@ struct WS {
   const int32_t* flag_rd;
   int32_t* flag_wr;
   int32_t step_idx;
-  // segmented batch (null for one system)
+  // segmented batch (null for one system); seg_flag[s] = 1 when system s
+  // needs a rebuild
   const int32_t* seg;
   const SegP* segp;
+  int32_t* seg_flag;
 };
 
 hipError_t ws_launch_in(const InOut& io, bool with_nl, hipStream_t s);
@@ -321,7 +327,13 @@ hipError_t ws_launch_seg_ids(const int64_t* offsets, int n_seg, int64_t n, int32
 hipError_t ws_launch_permute(int64_t n, const int32_t* new_orig, const int32_t* wpos, const float4* P,
                              const float4* V, const float4* W, const float4* Q, const float4* I, float4* P2,
                              float4* V2, float4* W2, float4* Q2, float4* I2, int32_t* orig2, int32_t* src,
-                             int32_t* inv, hipStream_t s);
+                             int32_t* inv, const float4* O, float4* O2, hipStream_t s);
+// batch rebuilds: per-system mode from the flags (and bump the build counts),
+// the flags of never-built systems, and old positions of rebuilt systems
+hipError_t ws_launch_seg_modes(int n_seg, int32_t* seg_flag, int32_t* n_build, int32_t* seg_mode, hipStream_t s);
+hipError_t ws_launch_seg_init_flags(int n_seg, const int32_t* n_build, int32_t* seg_flag, hipStream_t s);
+hipError_t ws_launch_old_update(int64_t n, const int32_t* seg, const int32_t* seg_mode, const float4* P, float4* old4,
+                                hipStream_t s);
 hipError_t ws_launch_cell_starts(const uint32_t* skeys, int64_t n, int64_t n_cells, int32_t* starts, hipStream_t s);
 hipError_t ws_launch_nlist(const NLW& a, int dim, bool naive, hipStream_t s);
 hipError_t ws_launch_step(const WS& a, int dim, int law, hipStream_t s);

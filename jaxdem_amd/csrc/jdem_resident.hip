@@ -263,7 +263,8 @@ __global__ void __launch_bounds__(kBlock) k_permute(int64_t n, const int32_t* __
                                                      float4* __restrict__ P2, float4* __restrict__ V2,
                                                      float4* __restrict__ W2, float4* __restrict__ Q2,
                                                      float4* __restrict__ I2, int32_t* __restrict__ orig2,
-                                                     int32_t* __restrict__ src_out, int32_t* __restrict__ inv_out) {
+                                                     int32_t* __restrict__ src_out, int32_t* __restrict__ inv_out,
+                                                     const float4* __restrict__ O, float4* __restrict__ O2) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n) return;
   const int32_t i = new_orig[b];
@@ -273,9 +274,33 @@ __global__ void __launch_bounds__(kBlock) k_permute(int64_t n, const int32_t* __
   W2[b] = W[s];
   Q2[b] = Q[s];
   I2[b] = I[s];
+  if (O) O2[b] = O[s];  // positions at the last build (kept by systems that do not rebuild)
   orig2[b] = i;
   src_out[b] = s;
   inv_out[s] = (int32_t)b;
+}
+
+// Batch rebuilds.  mode of system s: KEEP when it did not ask for a
+// rebuild, FRESH for its first build, REMAP otherwise; its build count grows
+// and its flag is cleared.
+__global__ void k_seg_modes(int n_seg, int32_t* __restrict__ seg_flag, int32_t* __restrict__ n_build,
+                            int32_t* __restrict__ seg_mode) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n_seg) return;
+  const int f = seg_flag[s] != 0;
+  seg_mode[s] = f ? (n_build[s] > 0 ? SEG_REMAP : SEG_FRESH) : SEG_KEEP;
+  n_build[s] += f;
+  seg_flag[s] = 0;
+}
+__global__ void k_seg_init_flags(int n_seg, const int32_t* __restrict__ n_build, int32_t* __restrict__ seg_flag) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n_seg) seg_flag[s] = n_build[s] == 0 ? 1 : 0;
+}
+__global__ void __launch_bounds__(kBlock) k_old_update(int64_t n, const int32_t* __restrict__ seg,
+                                                        const int32_t* __restrict__ seg_mode,
+                                                        const float4* __restrict__ P, float4* __restrict__ old4) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < n && seg_mode[seg[b]] != SEG_KEEP) old4[b] = P[b];
 }
 
 // neighbour list in work space (cell_list.py:500-595).  The stencil is
@@ -352,11 +377,11 @@ JD int find_in_row(int32_t tgt, int nrow, Row row) {
 }
 
 template <class Row>
-JD void remap_row(const NLW& a, int64_t b, int nrow, Row row) {
+JD void remap_row(const NLW& a, int64_t b, int nrow, Row row, bool remap) {
   if (!a.live) return;
   const int64_t n = a.n;
   uint64_t out0 = 0ull, out1 = 0ull, out2 = 0ull, out3 = 0ull;
-  if (a.remap) {
+  if (remap) {
     const int64_t ob = a.src[b];
     for (int sweep = 0; sweep < 2; ++sweep) {
       int rank_old = 0;
@@ -396,7 +421,29 @@ JD void remap_row(const NLW& a, int64_t b, int nrow, Row row) {
 JD void end_row(const NLW& a, int64_t b, int count) {
   if (count < a.M) a.nl[(int64_t)count * a.n + b] = -1;
   if (count > a.M) atomicOr(a.overflow, 1);
-  remap_row(a, b, min(count, a.M), [&](int s) { return a.nl[(int64_t)s * a.n + b]; });
+  remap_row(a, b, min(count, a.M), [&](int s) { return a.nl[(int64_t)s * a.n + b]; }, a.remap != 0);
+}
+
+// A system of a batch that keeps its list across another system's rebuild:
+// its row moves with the work order (entries through inv), its springs and
+// live words move unchanged.
+JD void keep_row(const NLW& a, int64_t b) {
+  const int64_t n = a.n, ob = a.src[b];
+  int c = 0;
+  for (; c < a.M; ++c) {
+    const int32_t j = a.old_nl[(int64_t)c * n + ob];
+    if (j < 0) break;
+    a.nl[(int64_t)c * n + b] = a.inv[j];
+  }
+  if (c < a.M) a.nl[(int64_t)c * n + b] = -1;
+  if (!a.live) return;
+  int live = 0;
+  for (int w = 0; w < a.W64; ++w) {
+    const uint64_t bits = a.old_live[(int64_t)w * n + ob];
+    a.live[(int64_t)w * n + b] = bits;
+    live += __popcll(bits);
+  }
+  for (int k = 0; k < live; ++k) a.xi[(int64_t)k * n + b] = a.old_xi[(int64_t)k * n + ob];
 }
 
 // The first kRowLds slots of each row are collected in LDS, slot-major, and
@@ -413,7 +460,16 @@ __global__ void __launch_bounds__(kBlock) k_nlist_work(NLW a) {
   const int tid = threadIdx.x;
   if (tid == 0) row_max = 0;
   __syncthreads();
-  const int64_t b = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + tid;
+  int64_t b = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + tid;
+  bool remap = a.remap != 0;
+  if (PM == 2 && b < a.n) {
+    const int mode = a.seg_mode[a.seg[b]];
+    if (mode == SEG_KEEP) {
+      keep_row(a, b);
+      b = a.n;  // no new row for this slot (the barrier below is still reached)
+    }
+    remap = mode == SEG_REMAP;
+  }
   const bool active = b < a.n;
   int count = 0;
   if (active) {
@@ -510,8 +566,10 @@ __global__ void __launch_bounds__(kBlock) k_nlist_work(NLW a) {
     if (s < c) a.nl[(int64_t)s * a.n + b] = rows[s][tid];
   if (count < a.M) a.nl[(int64_t)count * a.n + b] = -1;
   if (count > a.M) atomicOr(a.overflow, 1);
-  remap_row(a, b, c, [&](int s) { return s < kRowLds ? rows[s][tid] : a.nl[(int64_t)s * a.n + b]; });
+  remap_row(a, b, c, [&](int s) { return s < kRowLds ? rows[s][tid] : a.nl[(int64_t)s * a.n + b]; }, remap);
 }
+
+
 
 __global__ void __launch_bounds__(kBlock) k_copy_f4(float4* __restrict__ d, const float4* __restrict__ s, int64_t n) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -721,7 +779,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
   }
   if (a.do_before && a.check_disp) {
     const Vec<D> d = x - xyz<D>(a.old4[b]);
-    if (dot(d, d) > trigger) *a.flag_wr = a.step_idx + 1;
+    if (dot(d, d) > trigger) {
+      *a.flag_wr = a.step_idx + 1;
+      if (PM == 2) a.seg_flag[a.seg[b]] = 1;
+    }
   }
 }
 
@@ -852,9 +913,22 @@ hipError_t ws_launch_seg_ids(const int64_t* offsets, int n_seg, int64_t n, int32
 hipError_t ws_launch_permute(int64_t n, const int32_t* new_orig, const int32_t* wpos, const float4* P,
                              const float4* V, const float4* W, const float4* Q, const float4* I, float4* P2,
                              float4* V2, float4* W2, float4* Q2, float4* I2, int32_t* orig2, int32_t* src,
-                             int32_t* inv, hipStream_t s) {
+                             int32_t* inv, const float4* O, float4* O2, hipStream_t s) {
   hipLaunchKernelGGL(k_permute, dim3(nblocks(n)), dim3(kBlock), 0, s, n, new_orig, wpos, P, V, W, Q, I, P2, V2, W2,
-                     Q2, I2, orig2, src, inv);
+                     Q2, I2, orig2, src, inv, O, O2);
+  return hipGetLastError();
+}
+hipError_t ws_launch_seg_modes(int n_seg, int32_t* seg_flag, int32_t* n_build, int32_t* seg_mode, hipStream_t s) {
+  hipLaunchKernelGGL(k_seg_modes, dim3(nblocks(n_seg)), dim3(kBlock), 0, s, n_seg, seg_flag, n_build, seg_mode);
+  return hipGetLastError();
+}
+hipError_t ws_launch_seg_init_flags(int n_seg, const int32_t* n_build, int32_t* seg_flag, hipStream_t s) {
+  hipLaunchKernelGGL(k_seg_init_flags, dim3(nblocks(n_seg)), dim3(kBlock), 0, s, n_seg, n_build, seg_flag);
+  return hipGetLastError();
+}
+hipError_t ws_launch_old_update(int64_t n, const int32_t* seg, const int32_t* seg_mode, const float4* P, float4* old4,
+                                hipStream_t s) {
+  hipLaunchKernelGGL(k_old_update, dim3(nblocks(n)), dim3(kBlock), 0, s, n, seg, seg_mode, P, old4);
   return hipGetLastError();
 }
 hipError_t ws_launch_cell_starts(const uint32_t* skeys, int64_t n, int64_t n_cells, int32_t* starts, hipStream_t s) {
