@@ -43,6 +43,10 @@ def parse_args():
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--batch-systems", type=int, default=4096, help="systems of the batching workload")
+    ap.add_argument("--batch-steps", type=int, default=2000)
+    ap.add_argument("--plain-sample", type=int, default=16, help="systems stepped one at a time for the comparison")
+    ap.add_argument("--no-batch", action="store_true")
     ap.add_argument(
         "--copy-probe-mib",
         type=int,
@@ -102,6 +106,64 @@ def cpu_baseline(w, col, budget_s: float) -> dict:
         "kind": "port",
         "sample": f"{steps} steps of the same {w.N}-sphere workload after its first list build "
         f"(C++/OpenMP port of the oracle, bit-identical to it; {secs:.1f} s)",
+    }
+
+
+def batch_bench(args, dev) -> dict:
+    """Batching workload: independent 2D Hertz-friction disk systems of
+    log-uniform size in one segmented layout, against the same systems
+    stepped one at a time with System.step (a size-stratified sample)."""
+    import numpy as np
+    import torch
+
+    import jaxdem_amd as jd
+    from jaxdem_amd import workloads as W
+
+    t0 = time.perf_counter()
+    ws = W.batch_disks(args.batch_systems, seed=args.seed)
+    pairs = [W.make_gpu_system(w, device="cpu") for w in ws]
+    b = jd.Batch.pack(pairs).to(dev)
+    del pairs
+    prep = time.perf_counter() - t0
+    N = b.n_particles
+    print(f"[bench] batch: {len(ws)} systems, {N} disks, packed in {prep:.1f} s", file=sys.stderr, flush=True)
+    b = b.step(20)  # first list builds, not timed
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b2 = b.step(args.batch_steps)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    batched = N * args.batch_steps / secs
+    stats = b2.system.__dict__.get("_last_stats")
+    # one at a time: a sample stratified by size, the same number of steps
+    order = np.argsort([w.N for w in ws])
+    pick = order[np.linspace(0, len(ws) - 1, min(args.plain_sample, len(ws))).astype(int)]
+    n_tot, t_tot = 0, 0.0
+    for k in pick:
+        st, sy = W.make_gpu_system(ws[int(k)], device=dev)
+        st, sy = jd.System.step(st, sy, n=20)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        jd.System.step(st, sy, n=args.batch_steps)
+        torch.cuda.synchronize()
+        t_tot += time.perf_counter() - t1
+        n_tot += ws[int(k)].N * args.batch_steps
+    plain = n_tot / t_tot
+    print(f"[bench] batch: {batched:.3e} particle-steps/s batched, {plain:.3e} one at a time", file=sys.stderr,
+          flush=True)
+    return {
+        "workload": f"batch_disks: {len(ws)} 2D systems, N log-uniform in [256, 16384] (total {N}), "
+        "area fraction U[0.3, 0.8], Hertz + tangential spring, mu 0.3, NeighborList over CellList",
+        "n_systems": len(ws),
+        "n_particles": N,
+        "steps": args.batch_steps,
+        "value": batched,
+        "unit": "particle-steps/s",
+        "ms_per_step": 1e3 * secs / args.batch_steps,
+        "plain_one_at_a_time": plain,
+        "plain_sample": f"{len(pick)} systems stratified by size, {args.batch_steps} steps each with System.step",
+        "batched_over_plain": batched / plain,
+        "rebuilds": int(stats.rebuilds) if stats is not None else None,
     }
 
 
@@ -208,6 +270,9 @@ def main():
         out["cpu_baseline"] = cpu_baseline(w, sy.collider, args.cpu_seconds)
     else:
         out["cpu_baseline"] = None
+    del st1, sy1, st2, sy2
+    if not args.no_batch:
+        out["batched"] = batch_bench(args, dev)
     print(json.dumps(out), flush=True)
 
 

@@ -32,27 +32,44 @@ from .state import Quaternion, State
 F32 = np.float32
 
 
+def _table_key(t: MaterialTable) -> tuple:
+    items = [(k, np.asarray(v, np.float32).tobytes()) for k, v in sorted(t.props.items())]
+    items += [(k, np.asarray(v, np.float32).tobytes()) for k, v in sorted(t.pair.items())]
+    return (type(t.matcher).__name__, tuple(items))
+
+
 def _merge_tables(tables: Sequence[MaterialTable]) -> tuple[MaterialTable, np.ndarray]:
-    """Block-diagonal union of material tables: system k's material m becomes
-    mat_offsets[k] + m; pair entries across systems are never used."""
-    sizes = [len(t) for t in tables]
-    offs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
-    keys = sorted(set().union(*[set(t.props) for t in tables]))
+    """Block-diagonal union of the distinct material tables: system k's
+    material m becomes mat_offsets[k] + m; systems with equal tables share a
+    block, and pair entries across blocks are never used."""
+    uniq: dict = {}
+    blocks: list = []
+    which = []
+    for t in tables:
+        key = _table_key(t)
+        if key not in uniq:
+            uniq[key] = len(blocks)
+            blocks.append(t)
+        which.append(uniq[key])
+    sizes = [len(t) for t in blocks]
+    boffs = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    keys = sorted(set().union(*[set(t.props) for t in blocks]))
     props = {}
     for k in keys:
         props[k] = np.concatenate(
-            [np.asarray(t.props[k], np.float32) if k in t.props else np.zeros(len(t), np.float32) for t in tables]
+            [np.asarray(t.props[k], np.float32) if k in t.props else np.zeros(len(t), np.float32) for t in blocks]
         )
-    pkeys = sorted(set().union(*[set(t.pair) for t in tables]))
-    M = int(offs[-1])
+    pkeys = sorted(set().union(*[set(t.pair) for t in blocks]))
+    M = int(boffs[-1])
     pair = {}
     for k in pkeys:
         a = np.zeros((M, M), np.float32)
-        for t, o in zip(tables, offs[:-1]):
+        for t, o in zip(blocks, boffs[:-1]):
             if k in t.pair:
                 n = len(t)
                 a[o : o + n, o : o + n] = np.asarray(t.pair[k], np.float32)
         pair[k] = a
+    offs = np.array([boffs[w] for w in which] + [M], np.int64)
     return MaterialTable(props, pair, tables[0].matcher), offs
 
 
@@ -173,6 +190,25 @@ class Batch:
         return Batch(state, packed, offsets, moff, list(systems), bw, built)
 
     # ------------------------------------------------------------------
+    def to(self, device) -> "Batch":
+        """The batch with every tensor on ``device`` (pack on the host, then
+        move once)."""
+        st = self.state
+        kw = {}
+        for f in fields(st):
+            v = getattr(st, f.name)
+            if isinstance(v, Quaternion):
+                kw[f.name] = Quaternion(v.data.to(device))
+            elif isinstance(v, torch.Tensor):
+                kw[f.name] = v.to(device)
+            else:
+                kw[f.name] = v
+        col = self.system.collider
+        hist = None if col.history is None else {k: v.to(device) for k, v in col.history.items()}
+        col2 = replace(col, neighbor_list=col.neighbor_list.to(device), old_pos=col.old_pos.to(device), history=hist)
+        return Batch(State(**kw), replace(self.system, collider=col2), self.offsets, self.mat_offsets, self.systems,
+                     self.bond_widths, list(self.n_build))
+
     def _c_arrays(self, keep: list) -> L.JdemBatch:
         B = len(self.systems)
         off = np.ascontiguousarray(self.offsets, np.int64)

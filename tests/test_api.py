@@ -204,9 +204,13 @@ def test_batch_pack_unpack_round_trip():
     pairs = [W.make_gpu_system(w, device="cpu") for w in ws]
     b = jd.Batch.pack(pairs)
     assert b.n_particles == 1256 and list(b.offsets) == [0, 300, 1000, 1256]
-    assert len(b.system.mat_table) == 3
+    assert len(b.system.mat_table) == 1  # the three systems share one material: one block
     assert b.system.collider.max_neighbors == max(p[1].collider.max_neighbors for p in pairs)
-    assert int(b.state.mat_id.max()) == 2
+    assert int(b.state.mat_id.max()) == 0
+    w3 = W.disks_hertz_friction(200, seed=9, phi=0.5, mu=0.1)  # another material
+    b2 = jd.Batch.pack(pairs + [W.make_gpu_system(w3, device="cpu")])
+    assert len(b2.system.mat_table) == 2 and list(b2.mat_offsets) == [0, 0, 0, 1, 2]
+    assert float(b2.system.mat_table.mu_eff[1, 1]) == pytest.approx(0.1)
     for (s0, y0), (s1, y1) in zip(pairs, b.unpack()):
         for f in ("pos_c", "vel", "rad", "mass", "mat_id", "clump_id", "bond_id", "fixed"):
             assert torch.equal(getattr(s0, f), getattr(s1, f)), f
