@@ -47,6 +47,9 @@ def parse_args():
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--plain-sample", type=int, default=16, help="systems stepped one at a time for the comparison")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--serve-requests", type=int, default=32)
+    ap.add_argument("--serve-rate", type=float, default=16.0, help="Poisson arrival rate, requests per second")
+    ap.add_argument("--no-serve", action="store_true")
     ap.add_argument(
         "--copy-probe-mib",
         type=int,
@@ -167,6 +170,41 @@ def batch_bench(args, dev) -> dict:
     }
 
 
+def serving_bench(args, dev) -> dict:
+    """Serving workload: Poisson-arriving rollout requests (N log-uniform in
+    [1e3, 2e5], 100-10000 steps, polydisperse spheres with friction, periodic
+    or reflect) served by continuous batching, against the same requests run
+    one at a time with System.step in arrival order."""
+    import torch
+
+    from jaxdem_amd import serving as SV
+    from jaxdem_amd import workloads as W
+
+    spec = W.serving_requests(args.serve_requests, rate=args.serve_rate, seed=args.seed)
+    reqs = []
+    for k, (t, steps, w) in enumerate(spec):
+        st, sy = W.make_gpu_system(w, device=dev)
+        reqs.append(SV.Request(k, st, sy, steps, arrival=t))
+    torch.cuda.synchronize()
+    work = sum(r.n * r.steps for r in reqs)
+    print(f"[bench] serving: {len(reqs)} requests, {work:.3e} particle-steps, rate {args.serve_rate}/s",
+          file=sys.stderr, flush=True)
+    server = SV.Server(dev, chunk=50)
+    batched = SV.latency_stats(server.run(reqs, realtime=True))
+    print(f"[bench] serving batched: {batched}", file=sys.stderr, flush=True)
+    plain = SV.latency_stats(SV.run_one_at_a_time(reqs, dev, realtime=True))
+    print(f"[bench] serving one at a time: {plain}", file=sys.stderr, flush=True)
+    return {
+        "workload": f"serving_requests: {len(reqs)} Poisson arrivals at {args.serve_rate}/s, N log-uniform in "
+        "[1e3, 2e5], steps U[100, 10000], log-normal radii (mean 0.5, sigma 0.2), periodic/reflect, "
+        "phi U[0.3, 0.55], spring + friction mu U[0, 0.5]",
+        "continuous_batching": batched,
+        "one_at_a_time": plain,
+        "throughput_ratio": batched["particle_steps_per_s"] / plain["particle_steps_per_s"],
+        "p99_ratio": plain["p99_latency_s"] / batched["p99_latency_s"],
+    }
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -273,6 +311,9 @@ def main():
     del st1, sy1, st2, sy2
     if not args.no_batch:
         out["batched"] = batch_bench(args, dev)
+        torch.cuda.empty_cache()
+    if not args.no_serve:
+        out["serving"] = serving_bench(args, dev)
     print(json.dumps(out), flush=True)
 
 

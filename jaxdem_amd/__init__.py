@@ -31,7 +31,7 @@ from .integrators import (
 from .materials import Material, MaterialMatchmaker, MaterialTable
 from .state import Quaternion, State
 from .system import System
-from . import batching, minimizers, utils
+from . import batching, minimizers, serving, utils
 from .batching import Batch
 from .minimizers import fire, minimize
 
@@ -69,6 +69,7 @@ __all__ = [
     "System",
     "Batch",
     "batching",
+    "serving",
     "fire",
     "minimize",
     "minimizers",

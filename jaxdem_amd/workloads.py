@@ -180,6 +180,75 @@ def batch_disks(n_systems: int = 4096, seed: int = 0, n_min: int = 256, n_max: i
     return [disks_hertz_friction(int(N), seed=seed * 1000003 + k, phi=float(p)) for k, (N, p) in enumerate(zip(Ns, phis))]
 
 
+def polydisperse_spheres(N: int, seed: int, phi: float, domain: str, mu: float, skin: float = 0.1,
+                         k: float = 1e4, damping: float = 0.1) -> Workload:
+    """One request of the serving workload: N spheres with log-normal radii
+    (mean 0.5, log-space sigma 0.2, clipped to [0.25, 1.0]) on the first N
+    sites of a jittered cubic lattice in a cube at volume fraction phi
+    (periodic or reflect), velocities N(0, 1) with zero net momentum, unit
+    density, spring k_n = k with the tangential spring, Coulomb mu and normal
+    damping ratio, NeighborList over a CellList, VV + VV-SPIRAL, dt = t_c/50
+    of the lightest pair, float32."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    s2 = 0.2
+    rad = np.exp(rng.normal(math.log(0.5) - 0.5 * s2 * s2, s2, N)).clip(0.25, 1.0).astype(np.float32)
+    vol = float(np.sum((4.0 / 3.0) * math.pi * rad.astype(np.float64) ** 3))
+    L = (vol / phi) ** (1.0 / 3.0)
+    side = int(math.ceil(N ** (1.0 / 3.0)))
+    spacing = L / side
+    idx = np.arange(N)
+    ijk = np.stack([idx // (side * side), (idx // side) % side, idx % side], axis=1).astype(np.float64)
+    pos = (ijk + 0.5) * spacing + rng.uniform(-0.05 * spacing, 0.05 * spacing, (N, 3))
+    vel = rng.standard_normal((N, 3))
+    vel = (vel - vel.mean(axis=0)).astype(np.float32)
+    mass = sphere_volume(rad)
+    m_min = float(mass.min())
+    t_c = math.pi * math.sqrt(0.5 * m_min / k)
+    rmax = float(rad.max())
+    return Workload(
+        name=f"poly_{N}_{domain}",
+        pos=pos.astype(np.float32),
+        vel=vel,
+        rad=rad,
+        mass=mass,
+        box=np.full(3, L, np.float32),
+        anchor=np.zeros(3, np.float32),
+        domain=domain,
+        dt=t_c / 50.0,
+        law="springfriction",
+        young=k,
+        mu=mu,
+        damping=damping,
+        collider="neighborlist",
+        cell_size=2.0 * rmax,
+        cutoff=2.0 * rmax,
+        skin=skin,
+        extra={"phi": phi, "L": L},
+    )
+
+
+def serving_requests(n_requests: int, rate: float, seed: int = 0, n_min: int = 1000, n_max: int = 200000,
+                     steps_min: int = 100, steps_max: int = 10000) -> list:
+    """The serving workload: Poisson arrivals at ``rate`` per second; per
+    request N log-uniform in [n_min, n_max], steps uniform in [steps_min,
+    steps_max], periodic or reflect (50:50), volume fraction U[0.3, 0.55],
+    friction mu U[0, 0.5].  Returns (arrival time, steps, Workload) tuples."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    t = np.cumsum(rng.exponential(1.0 / rate, n_requests))
+    out = []
+    for k in range(n_requests):
+        N = int(np.rint(np.exp(rng.uniform(math.log(n_min), math.log(n_max)))))
+        steps = int(rng.integers(steps_min, steps_max + 1))
+        dom = "periodic" if rng.random() < 0.5 else "reflect"
+        phi = float(rng.uniform(0.3, 0.55))
+        mu = float(rng.uniform(0.0, 0.5))
+        w = polydisperse_spheres(N, seed=seed * 1000003 + 7 * k + 1, phi=phi, domain=dom, mu=mu)
+        if dom == "reflect":
+            w.box = w.box * np.float32(1.02)  # lattice sites stay inside the walls
+        out.append((float(t[k]), steps, w))
+    return out
+
+
 def make_gpu_system(w: Workload, device="cuda", max_neighbors: int | None = None, collider: str | None = None):
     """(State, System) of the jaxdem_amd API for a workload."""
     from . import Material, MaterialMatchmaker, MaterialTable, State, System
@@ -226,6 +295,8 @@ __all__ = [
     "spheres_friction",
     "disks_hertz_friction",
     "batch_disks",
+    "polydisperse_spheres",
+    "serving_requests",
     "make_gpu_system",
     "sphere_volume",
 ]

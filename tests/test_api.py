@@ -219,3 +219,21 @@ def test_batch_pack_unpack_round_trip():
         assert torch.equal(y0.collider.history["xi"], y1.collider.history["xi"])
     with pytest.raises(RuntimeError, match="GPU"):
         b.step(1)
+
+
+def test_serving_footprint_and_latency_stats():
+    from jaxdem_amd import serving as SV
+    from jaxdem_amd import workloads as W
+
+    small = W.make_gpu_system(W.polydisperse_spheres(500, seed=1, phi=0.4, domain="periodic", mu=0.2), device="cpu")
+    big = W.make_gpu_system(W.polydisperse_spheres(4000, seed=1, phi=0.4, domain="periodic", mu=0.2), device="cpu")
+    assert 0 < SV.footprint_bytes(*small) < SV.footprint_bytes(*big)
+    rs = []
+    for k, (a, f) in enumerate([(0.0, 1.0), (0.5, 1.5), (1.0, 4.0), (1.0, 2.0)]):
+        r = SV.Request(k, small[0], small[1], 10, arrival=a)
+        r.finished = f
+        rs.append(r)
+    s = SV.latency_stats(rs)
+    assert s["requests"] == 4 and s["makespan_s"] == 4.0
+    assert s["p50_latency_s"] == pytest.approx(1.0) and s["p99_latency_s"] == pytest.approx(2.94)
+    assert s["particle_steps"] == 4 * 500 * 10

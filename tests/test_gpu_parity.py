@@ -479,3 +479,39 @@ def test_batch_split_steps(device):
     two = b.step(12).step(18).unpack()
     for k, ((sa, _), (sb, _)) in enumerate(zip(one, two)):
         _kinematics_equal(sa, sb, f"sys{k}")
+
+
+# ---------------------------------------------------------------------------
+# (f) serving: continuous batching of rollout requests
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("budget", [None, 1])
+def test_serving_results_equal_alone(device, budget):
+    """Requests of different sizes, lengths, domains and friction served
+    together (and, with a one-byte budget, one at a time by admission): every
+    result equals the request run alone with System.step, bit for bit."""
+    from jaxdem_amd import serving as SV
+
+    print(f"[serving] budget={budget}", flush=True)
+    spec = [(1200, 37, "periodic", 0.4, 0.1), (2500, 80, "reflect", 0.5, 0.3), (900, 15, "reflect", 0.35, 0.0),
+            (3000, 64, "periodic", 0.45, 0.45), (1500, 50, "periodic", 0.3, 0.2)]
+    reqs = []
+    for k, (N, steps, dom, phi, mu) in enumerate(spec):
+        w = W.polydisperse_spheres(N, seed=100 + k, phi=phi, domain=dom, mu=mu)
+        if dom == "reflect":
+            w.box = w.box * np.float32(1.02)
+        st, sy = W.make_gpu_system(w, device=device)
+        reqs.append(SV.Request(k, st, sy, steps, arrival=0.0))
+    server = SV.Server(device, hbm_budget=budget if budget is not None else 1 << 36, chunk=16)
+    done = server.run(reqs, realtime=False)
+    assert sorted(r.rid for r in done) == list(range(len(spec)))
+    if budget == 1:
+        assert max(n for _, n, _, _ in server.log) == 1  # one request at a time
+    else:
+        assert max(n for _, n, _, _ in server.log) == len(spec)
+    for r in done:
+        sa, ya = jd.System.step(reqs[r.rid].state, reqs[r.rid].system, n=r.steps)
+        sb, yb = r.result
+        _kinematics_equal(sa, sb, f"request {r.rid}")
+        assert yb.step_count == r.steps
+    st = SV.latency_stats(done)
+    assert st["p50_latency_s"] <= st["p99_latency_s"] and st["particle_steps"] == sum(N * s for N, s, *_ in spec)
