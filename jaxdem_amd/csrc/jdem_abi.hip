@@ -21,7 +21,7 @@ std::vector<Buf*> jdem_ctx::bufs() {
   std::vector<Buf*> all = {&pos2,    &vel2,   &ang2,     &keys[0], &keys[1], &vals[0], &vals[1], &hist,
                            &cstart,  &cend,   &spos,     &nl_new,  &xi_new,  &live_new, &flags,  &red,
                            &nl_copy, &xi_copy, &live_copy, &wF,     &wT,      &wOld,     &wWpos, &wSrc,
-                           &wInv,    &wKeys,  &fireV,    &fireDev, &segp,  &segid, &segflag, &wOld2};
+                           &wInv,    &wKeys,  &fireV,    &fireDev, &segp,  &segid, &segflag};
   for (int k = 0; k < 2; ++k)
     for (Buf* b : {&wP[k], &wV[k], &wW[k], &wQ[k], &wI[k], &wOrig[k], &wNl[k], &wXi[k], &wLive[k]}) all.push_back(b);
   return all;
@@ -438,6 +438,9 @@ struct SegRun {
   int n_seg;
   const int32_t* seg;
   const SegP* segp;
+  const int64_t* offsets;  // particle ranges (device, n_seg + 1)
+  int64_t* cpref;          // flagged-particle prefix (device, n_seg + 1)
+  int64_t* ccref;          // flagged-cell prefix (device, n_seg + 1)
   int64_t total_cells;
   int32_t* n_build;   // builds of each system (device)
   int32_t* seg_flag;  // system asks for a rebuild (device)
@@ -473,7 +476,6 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   ENSURE(ctx, ctx->wF, n4);
   ENSURE(ctx, ctx->wT, n4);
   ENSURE(ctx, ctx->wOld, n4);
-  if (sg) ENSURE(ctx, ctx->wOld2, n4);
   ENSURE(ctx, ctx->wWpos, ni);
   ENSURE(ctx, ctx->wSrc, ni);
   ENSURE(ctx, ctx->wInv, ni);
@@ -541,29 +543,119 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   HIPCHK(ctx, ws_launch_in(io, M > 0, st));
 
   // --- rebuild at the current positions (neighbor_list.py:545-589) ---
+  // Batches: only the systems that asked for a rebuild are re-sorted and
+  // rebuilt, each within its own range of work slots; the work order, rows,
+  // springs and build positions of the other systems stay where they are.
+  auto rebuild_batch = [&]() -> int {
+    const int sp = span_begin(ctx, 2, 0);
+    HIPCHK(ctx, launch_fill_i32((int32_t*)ctx->flags.p + 2, 1, 0, st));
+    HIPCHK(ctx, ws_launch_seg_modes(sg->n_seg, sg->seg_flag, sg->n_build, sg->seg_mode, st));
+    HIPCHK(ctx, ws_launch_seg_prefix(sg->n_seg, sg->seg_mode, sg->offsets, sg->segp, sg->cpref, sg->ccref, st));
+    int64_t tot[2];
+    HIPCHK(ctx, hipMemcpyAsync(&tot[0], sg->cpref + sg->n_seg, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipMemcpyAsync(&tot[1], sg->ccref + sg->n_seg, 8, hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    S->host_syncs++;
+    const int64_t fn = tot[0];
+    if (fn > 0) {
+      RB r;
+      std::memset(&r, 0, sizeof(r));
+      r.n = N;
+      r.fn = fn;
+      r.M = M;
+      r.W64 = W64;
+      r.seg = sg->seg;
+      r.segp = sg->segp;
+      r.seg_mode = sg->seg_mode;
+      r.offsets = sg->offsets;
+      r.cpref = sg->cpref;
+      r.ccref = sg->ccref;
+      r.P = F4(ctx->wP[cur]);
+      r.V = F4(ctx->wV[cur]);
+      r.W = F4(ctx->wW[cur]);
+      r.Q = F4(ctx->wQ[set]);
+      r.I = F4(ctx->wI[set]);
+      r.old4 = F4(ctx->wOld);
+      r.orig = I32(ctx->wOrig[set]);
+      r.nl = I32(ctx->wNl[nlc]);
+      r.live = hist ? U64(ctx->wLive[nlc]) : nullptr;
+      r.xi = F4(ctx->wXi[xc]);
+      r.keys_c = (uint32_t*)ctx->wKeys.p;
+      r.slot_c = I32(ctx->cend);
+      // staging: buffers that hold nothing live during a batch rebuild
+      r.tP = F4(ctx->wP[1 - cur]);
+      r.tV = F4(ctx->wV[1 - cur]);
+      r.tW = F4(ctx->wW[1 - cur]);
+      r.tQ = F4(ctx->wQ[1 - set]);
+      r.tI = F4(ctx->wI[1 - set]);
+      r.torig = I32(ctx->wOrig[1 - set]);
+      r.cs = I32(ctx->wWpos);
+      r.tnl = I32(ctx->wNl[1 - nlc]);
+      r.tlive = U64(ctx->wLive[1 - nlc]);
+      r.txi = F4(ctx->wXi[1 - xc]);
+      r.inv = I32(ctx->wInv);
+      r.cidx = I32(ctx->wSrc);
+      HIPCHK(ctx, ws_launch_hash_flagged(r, D, st));
+      uint32_t* sk = nullptr;
+      int32_t* perm = nullptr;
+      HIPCHK(ctx, radix_sort_pairs(ctx, r.keys_c, fn, nbits_for(std::max<int64_t>(tot[1], 1)), &sk, &perm, st));
+      r.perm = perm;
+      HIPCHK(ctx, ws_launch_place(r, sk, I32(ctx->cstart), st));
+      if (M > 0) {
+        NLW b;
+        std::memset(&b, 0, sizeof(b));
+        b.n = N;
+        b.M = M;
+        b.isb = sys->interact_same_bond_id;
+        b.bond_w = s.bond_width;
+        b.bond_id = s.bond_id;
+        b.orig = I32(ctx->wOrig[set]);
+        b.P4 = F4(ctx->wP[cur]);
+        b.skeys = sk;  // compact sorted keys: slot b's key is sk[cidx[b]]
+        b.ccref = sg->ccref;
+        b.cidx = I32(ctx->wSrc);
+        b.starts = I32(ctx->cstart);
+        b.grid = g;
+        b.dom = dom;
+        b.seg = sg->seg;
+        b.segp = sg->segp;
+        b.seg_mode = sg->seg_mode;
+        b.nl = I32(ctx->wNl[nlc]);
+        b.overflow = (int32_t*)ctx->flags.p + 2;
+        if (hist) {
+          b.W64 = W64;
+          b.inv = I32(ctx->wInv);
+          b.ostride = fn;
+          b.old_nl = r.tnl;
+          b.old_xi = r.txi;
+          b.old_live = r.tlive;
+          b.xi = F4(ctx->wXi[xc]);
+          b.live = U64(ctx->wLive[nlc]);
+        }
+        HIPCHK(ctx, ws_launch_nlist(b, D, false, st));
+      }
+    }
+    L.n_build_times += 1;
+    span_end(ctx, sp);
+    return JDEM_OK;
+  };
+
   auto rebuild = [&]() -> int {
+    if (sg) return rebuild_batch();
     const int sp = span_begin(ctx, 2, 0);
     HIPCHK(ctx, launch_fill_i32((int32_t*)ctx->flags.p + 2, 1, 0, st));
     int32_t* src = I32(ctx->wSrc);
     int32_t* inv = I32(ctx->wInv);
     uint32_t* sk = nullptr;
-    // systems of a batch: which ones rebuild now (the others keep their rows)
-    if (sg) HIPCHK(ctx, ws_launch_seg_modes(sg->n_seg, sg->seg_flag, sg->n_build, sg->seg_mode, st));
     if (!naive2) {
-      if (sg) {
-        HIPCHK(ctx, ws_launch_hash_seg(F4(ctx->wP[cur]), I32(ctx->wOrig[set]), N, D, sg->seg, sg->segp,
-                                       (uint32_t*)ctx->wKeys.p, I32(ctx->wWpos), st));
-      } else {
-        HIPCHK(ctx, ws_launch_hash(F4(ctx->wP[cur]), I32(ctx->wOrig[set]), N, D, dom, g, (uint32_t*)ctx->wKeys.p,
-                                   I32(ctx->wWpos), st));
-      }
+      HIPCHK(ctx, ws_launch_hash(F4(ctx->wP[cur]), I32(ctx->wOrig[set]), N, D, dom, g, (uint32_t*)ctx->wKeys.p,
+                                 I32(ctx->wWpos), st));
       int32_t* new_orig = nullptr;
       HIPCHK(ctx, radix_sort_pairs(ctx, (uint32_t*)ctx->wKeys.p, N, nbits_for(n_cells), &sk, &new_orig, st));
       HIPCHK(ctx, ws_launch_permute(N, new_orig, I32(ctx->wWpos), F4(ctx->wP[cur]), F4(ctx->wV[cur]),
                                     F4(ctx->wW[cur]), F4(ctx->wQ[set]), F4(ctx->wI[set]), F4(ctx->wP[1 - cur]),
                                     F4(ctx->wV[1 - cur]), F4(ctx->wW[1 - cur]), F4(ctx->wQ[1 - set]),
-                                    F4(ctx->wI[1 - set]), I32(ctx->wOrig[1 - set]), src, inv,
-                                    sg ? F4(ctx->wOld) : nullptr, sg ? F4(ctx->wOld2) : nullptr, st));
+                                    F4(ctx->wI[1 - set]), I32(ctx->wOrig[1 - set]), src, inv, st));
       cur = 1 - cur;
       set = 1 - set;
       HIPCHK(ctx, ws_launch_cell_starts(sk, N, n_cells, I32(ctx->cstart), st));
@@ -586,11 +678,6 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       b.skeys = sk;
       b.starts = I32(ctx->cstart);
       b.grid = g;
-      if (sg) {
-        b.seg = sg->seg;
-        b.segp = sg->segp;
-        b.seg_mode = sg->seg_mode;
-      }
       b.dom = dom;
       b.nl = I32(ctx->wNl[1 - nlc]);
       b.overflow = (int32_t*)ctx->flags.p + 2;
@@ -609,13 +696,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       nlc = 1 - nlc;
       xc = 1 - xc;
     }
-    if (sg) {
-      // rebuilt systems take the current positions as their build positions
-      HIPCHK(ctx, ws_launch_old_update(N, sg->seg, sg->seg_mode, F4(ctx->wP[cur]), F4(ctx->wOld2), st));
-      std::swap(ctx->wOld, ctx->wOld2);
-    } else {
-      HIPCHK(ctx, ws_launch_copy4(F4(ctx->wOld), F4(ctx->wP[cur]), N, st));
-    }
+    HIPCHK(ctx, ws_launch_copy4(F4(ctx->wOld), F4(ctx->wP[cur]), N, st));
     L.n_build_times += 1;
     span_end(ctx, sp);
     return JDEM_OK;
@@ -1114,7 +1195,8 @@ int jdem_batch_step(jdem_ctx* ctx, jdem_system* sys, const jdem_batch* batch, in
   rc = ensure_flags(ctx);
   if (rc) return rc;
   hipStream_t st = ctx->stream;
-  ENSURE(ctx, ctx->segp, sizeof(SegP) * (size_t)B + 8 * ((size_t)B + 1));
+  ENSURE(ctx, ctx->segp, sizeof(SegP) * (size_t)B + 3 * 8 * ((size_t)B + 1));
+  ENSURE(ctx, ctx->cend, 4 * (size_t)std::max<int64_t>(N, 1));
   ENSURE(ctx, ctx->segid, 4 * (size_t)std::max<int64_t>(N, 1));
   ENSURE(ctx, ctx->segflag, 3 * 4 * (size_t)B);
   SegP* dsegp = (SegP*)ctx->segp.p;
@@ -1133,6 +1215,9 @@ int jdem_batch_step(jdem_ctx* ctx, jdem_system* sys, const jdem_batch* batch, in
   sg.seg = (const int32_t*)ctx->segid.p;
   sg.segp = dsegp;
   sg.total_cells = base;
+  sg.offsets = doff;
+  sg.cpref = doff + (B + 1);
+  sg.ccref = doff + 2 * (B + 1);
   sg.n_build = dbuild;
   sg.seg_flag = dbuild + B;
   sg.seg_mode = dbuild + 2 * B;

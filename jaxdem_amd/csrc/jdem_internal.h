@@ -46,7 +46,7 @@ struct jdem_ctx {
   jdem::Buf red;                       // reduction partials
   jdem::Buf nl_copy, xi_copy, live_copy;
   // resident NeighborList working set (jdem_resident.hip)
-  jdem::Buf wP[2], wV[2], wW[2], wQ[2], wI[2], wF, wT, wOld, wOrig[2], wWpos, wSrc, wInv, wKeys, fireV, fireDev, segp, segid, segflag, wOld2, wNl[2], wXi[2], wLive[2];
+  jdem::Buf wP[2], wV[2], wW[2], wQ[2], wI[2], wF, wT, wOld, wOrig[2], wWpos, wSrc, wInv, wKeys, fireV, fireDev, segp, segid, segflag, wNl[2], wXi[2], wLive[2];
   int32_t* host_flags = nullptr;       // pinned host mirror of flags
   // HIP-event timing (jdem_ctx_set_timing)
   struct TimedSpan {
@@ -259,8 +259,49 @@ struct NLW {
   const int32_t* seg;
   const SegP* segp;
   const int32_t* seg_mode;
+  // batches rebuild in place: the old rows, springs and live words of the
+  // particle now at b are at compact index cidx[b] (stride ostride)
+  const int32_t* cidx;
+  int64_t ostride;
+  const int64_t* ccref;  // compact key of system s's first cell
 };
 enum { SEG_KEEP = 0, SEG_FRESH = 1, SEG_REMAP = 2 };
+
+// In-place rebuild of the flagged systems of a batch (jdem_resident.hip).
+struct RB {
+  int64_t n, fn;  // particles; flagged particles
+  int M, W64;
+  const int32_t* seg;
+  const SegP* segp;
+  const int32_t* seg_mode;
+  const int64_t* offsets;
+  const int64_t* cpref;
+  const int64_t* ccref;
+  // work set (updated in place)
+  float4 *P, *V, *W, *Q, *I, *old4;
+  int32_t* orig;
+  const int32_t* nl;
+  const uint64_t* live;
+  const float4* xi;
+  // compact sort input / output
+  uint32_t* keys_c;
+  int32_t* slot_c;
+  const int32_t* perm;
+  // staging, compact index
+  float4 *tP, *tV, *tW, *tQ, *tI;
+  int32_t* torig;
+  int32_t* cs;
+  int32_t* tnl;
+  uint64_t* tlive;
+  float4* txi;
+  // maps
+  int32_t* inv;
+  int32_t* cidx;
+};
+hipError_t ws_launch_seg_prefix(int n_seg, const int32_t* seg_mode, const int64_t* offsets, const SegP* segp,
+                               int64_t* cpref, int64_t* ccref, hipStream_t s);
+hipError_t ws_launch_hash_flagged(const RB& r, int dim, hipStream_t s);
+hipError_t ws_launch_place(const RB& r, const uint32_t* sk, int32_t* starts, hipStream_t s);
 
 // Workgroups are dispatched round-robin over the 8 XCDs of the chip, each
 // with its own L2: workgroup w runs on XCD w % 8.  xcd_tile maps them so
@@ -320,20 +361,17 @@ hipError_t ws_launch_iota(int32_t* p, int64_t n, hipStream_t s);
 hipError_t ws_launch_copy4(float4* d, const float4* src, int64_t n, hipStream_t s);
 hipError_t ws_launch_hash(const float4* P4, const int32_t* orig, int64_t n, int dim, const DomP& dom, const GridP& g,
                           uint32_t* keys, int32_t* wpos, hipStream_t s);
-hipError_t ws_launch_hash_seg(const float4* P4, const int32_t* orig, int64_t n, int dim, const int32_t* seg,
-                              const SegP* segp, uint32_t* keys, int32_t* wpos, hipStream_t s);
 // seg[b] = system of particle b for the ranges offsets[0..n_seg]
 hipError_t ws_launch_seg_ids(const int64_t* offsets, int n_seg, int64_t n, int32_t* seg, hipStream_t s);
 hipError_t ws_launch_permute(int64_t n, const int32_t* new_orig, const int32_t* wpos, const float4* P,
                              const float4* V, const float4* W, const float4* Q, const float4* I, float4* P2,
                              float4* V2, float4* W2, float4* Q2, float4* I2, int32_t* orig2, int32_t* src,
-                             int32_t* inv, const float4* O, float4* O2, hipStream_t s);
+                             int32_t* inv, hipStream_t s);
 // batch rebuilds: per-system mode from the flags (and bump the build counts),
-// the flags of never-built systems, and old positions of rebuilt systems
+// and the flags of never-built systems
 hipError_t ws_launch_seg_modes(int n_seg, int32_t* seg_flag, int32_t* n_build, int32_t* seg_mode, hipStream_t s);
 hipError_t ws_launch_seg_init_flags(int n_seg, const int32_t* n_build, int32_t* seg_flag, hipStream_t s);
-hipError_t ws_launch_old_update(int64_t n, const int32_t* seg, const int32_t* seg_mode, const float4* P, float4* old4,
-                                hipStream_t s);
+
 hipError_t ws_launch_cell_starts(const uint32_t* skeys, int64_t n, int64_t n_cells, int32_t* starts, hipStream_t s);
 hipError_t ws_launch_nlist(const NLW& a, int dim, bool naive, hipStream_t s);
 hipError_t ws_launch_step(const WS& a, int dim, int law, hipStream_t s);

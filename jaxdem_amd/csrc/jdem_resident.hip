@@ -203,27 +203,6 @@ __global__ void __launch_bounds__(kBlock) k_hash_work(const float4* __restrict__
   wpos[i] = (int32_t)b;
 }
 
-// The same for a segmented batch: the cell of slot b in its system's grid,
-// offset by the system's first key.
-template <int D>
-__global__ void __launch_bounds__(kBlock) k_hash_seg(const float4* __restrict__ P4, const int32_t* __restrict__ orig,
-                                                      int64_t n, const int32_t* __restrict__ seg,
-                                                      const SegP* __restrict__ segp, uint32_t* __restrict__ keys,
-                                                      int32_t* __restrict__ wpos) {
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= n) return;
-  const SegP& sp = segp[seg[b]];
-  const float4 p = P4[b];
-  const float x[3] = {p.x, p.y, p.z};
-  int c[3] = {0, 0, 0};
-  const bool per = sp.dom.kind == JDEM_DOMAIN_PERIODIC;
-#pragma unroll
-  for (int a = 0; a < D; ++a) c[a] = per ? ccoord<true>(x[a], a, sp.dom, sp.grid) : ccoord<false>(x[a], a, sp.dom, sp.grid);
-  const int32_t i = orig[b];
-  keys[i] = sp.cell_base + wkey<D>(c, sp.grid);
-  wpos[i] = (int32_t)b;
-}
-
 // seg[b] = the system whose particle range [offsets[s], offsets[s + 1])
 // holds b
 __global__ void __launch_bounds__(kBlock) k_seg_ids(const int64_t* __restrict__ offsets, int n_seg, int64_t n,
@@ -263,8 +242,7 @@ __global__ void __launch_bounds__(kBlock) k_permute(int64_t n, const int32_t* __
                                                      float4* __restrict__ P2, float4* __restrict__ V2,
                                                      float4* __restrict__ W2, float4* __restrict__ Q2,
                                                      float4* __restrict__ I2, int32_t* __restrict__ orig2,
-                                                     int32_t* __restrict__ src_out, int32_t* __restrict__ inv_out,
-                                                     const float4* __restrict__ O, float4* __restrict__ O2) {
+                                                     int32_t* __restrict__ src_out, int32_t* __restrict__ inv_out) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n) return;
   const int32_t i = new_orig[b];
@@ -274,7 +252,6 @@ __global__ void __launch_bounds__(kBlock) k_permute(int64_t n, const int32_t* __
   W2[b] = W[s];
   Q2[b] = Q[s];
   I2[b] = I[s];
-  if (O) O2[b] = O[s];  // positions at the last build (kept by systems that do not rebuild)
   orig2[b] = i;
   src_out[b] = s;
   inv_out[s] = (int32_t)b;
@@ -296,11 +273,141 @@ __global__ void k_seg_init_flags(int n_seg, const int32_t* __restrict__ n_build,
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
   if (s < n_seg) seg_flag[s] = n_build[s] == 0 ? 1 : 0;
 }
-__global__ void __launch_bounds__(kBlock) k_old_update(int64_t n, const int32_t* __restrict__ seg,
-                                                        const int32_t* __restrict__ seg_mode,
-                                                        const float4* __restrict__ P, float4* __restrict__ old4) {
+
+// ---- batch rebuilds in place -----------------------------------------------
+// Only the systems that asked for a rebuild are re-sorted and rebuilt, each
+// inside its own range of work slots; the others are not touched.  Flagged
+// particles are numbered compactly: system s's particles take the compact
+// range [cpref[s], cpref[s + 1]) in original order, its cells the compact
+// key range starting at ccref[s].
+
+// Exclusive prefix sums over the systems of the flagged particle and cell
+// counts (one block).
+__global__ void __launch_bounds__(1024) k_seg_prefix(int n_seg, const int32_t* __restrict__ seg_mode,
+                                                      const int64_t* __restrict__ offsets,
+                                                      const SegP* __restrict__ segp, int64_t* __restrict__ cpref,
+                                                      int64_t* __restrict__ ccref) {
+  __shared__ int64_t sp[1024], sc[1024];
+  const int t = threadIdx.x;
+  const int per = (n_seg + 1023) / 1024;
+  const int s0 = t * per, s1 = min(n_seg, s0 + per);
+  int64_t np = 0, nc = 0;
+  for (int s = s0; s < s1; ++s)
+    if (seg_mode[s] != SEG_KEEP) {
+      np += offsets[s + 1] - offsets[s];
+      nc += segp[s].grid.n_cells;
+    }
+  sp[t] = np;
+  sc[t] = nc;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {  // inclusive scan
+    const int64_t ap = t >= off ? sp[t - off] : 0, ac = t >= off ? sc[t - off] : 0;
+    __syncthreads();
+    sp[t] += ap;
+    sc[t] += ac;
+    __syncthreads();
+  }
+  int64_t p = sp[t] - np, c = sc[t] - nc;
+  for (int s = s0; s < s1; ++s) {
+    cpref[s] = p;
+    ccref[s] = c;
+    if (seg_mode[s] != SEG_KEEP) {
+      p += offsets[s + 1] - offsets[s];
+      c += segp[s].grid.n_cells;
+    }
+  }
+  if (t == 1023) {
+    cpref[n_seg] = sp[t];
+    ccref[n_seg] = sc[t];
+  }
+}
+
+// compact key of every particle of a flagged system, at its compact index
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_hash_flagged(RB r) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b < n && seg_mode[seg[b]] != SEG_KEEP) old4[b] = P[b];
+  if (b >= r.n) return;
+  const int s = r.seg[b];
+  if (r.seg_mode[s] == SEG_KEEP) return;
+  const SegP& sp = r.segp[s];
+  const float4 p = r.P[b];
+  const float x[3] = {p.x, p.y, p.z};
+  int c[3] = {0, 0, 0};
+  const bool per = sp.dom.kind == JDEM_DOMAIN_PERIODIC;
+#pragma unroll
+  for (int a = 0; a < D; ++a) c[a] = per ? ccoord<true>(x[a], a, sp.dom, sp.grid) : ccoord<false>(x[a], a, sp.dom, sp.grid);
+  const int64_t i = r.cpref[s] + (r.orig[b] - r.offsets[s]);
+  r.keys_c[i] = (uint32_t)(r.ccref[s] + wkey<D>(c, sp.grid));
+  r.slot_c[i] = (int32_t)b;
+}
+
+// sorted position k -> staging (k) of the particle's fields and old list
+__global__ void __launch_bounds__(kBlock) k_place_gather(RB r) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= r.fn) return;
+  const int32_t ob = r.slot_c[r.perm[k]];
+  const int s = r.seg[ob];
+  const int64_t nb = r.offsets[s] + (k - r.cpref[s]);
+  r.tP[k] = r.P[ob];
+  r.tV[k] = r.V[ob];
+  r.tW[k] = r.W[ob];
+  r.tQ[k] = r.Q[ob];
+  r.tI[k] = r.I[ob];
+  r.torig[k] = r.orig[ob];
+  r.cs[k] = s;
+  r.inv[ob] = (int32_t)nb;
+  if (r.M > 0) {
+    int c = 0;
+    for (; c < r.M; ++c) {
+      const int32_t j = r.nl[(int64_t)c * r.n + ob];
+      r.tnl[(int64_t)c * r.fn + k] = j;
+      if (j < 0) break;
+    }
+    if (r.live) {
+      int nlive = 0;
+      for (int w = 0; w < r.W64; ++w) {
+        const uint64_t bits = r.live[(int64_t)w * r.n + ob];
+        r.tlive[(int64_t)w * r.fn + k] = bits;
+        nlive += __popcll(bits);
+      }
+      for (int q = 0; q < nlive; ++q) r.txi[(int64_t)q * r.fn + k] = r.xi[(int64_t)q * r.n + ob];
+    }
+  }
+}
+
+// staging (k) -> the particle's new work slot in its system's range
+__global__ void __launch_bounds__(kBlock) k_place_scatter(RB r) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= r.fn) return;
+  const int s = r.cs[k];
+  const int64_t nb = r.offsets[s] + (k - r.cpref[s]);
+  const float4 p = r.tP[k];
+  r.P[nb] = p;
+  r.V[nb] = r.tV[k];
+  r.W[nb] = r.tW[k];
+  r.Q[nb] = r.tQ[k];
+  r.I[nb] = r.tI[k];
+  r.orig[nb] = r.torig[k];
+  r.old4[nb] = p;  // build position
+  r.cidx[nb] = (int32_t)k;
+}
+
+// cell starts of the flagged systems: thread k fills the cells of its system
+// from the one after its predecessor's key to its own key (the last particle
+// of a system also fills to the end of the system's grid)
+__global__ void __launch_bounds__(kBlock) k_cell_starts_flagged(RB r, const uint32_t* __restrict__ sk,
+                                                                 int32_t* __restrict__ starts) {
+  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= r.fn) return;
+  const int s = r.cs[k];
+  const SegP& sp = r.segp[s];
+  const int64_t first = r.cpref[s], last = r.cpref[s + 1] - 1;
+  const int64_t key = (int64_t)sk[k] - r.ccref[s];
+  const int64_t lo = (k == first) ? 0 : (int64_t)sk[k - 1] - r.ccref[s] + 1;
+  const int32_t slot = (int32_t)(r.offsets[s] + (k - first));
+  for (int64_t h = lo; h <= key; ++h) starts[sp.cell_base + h] = slot;
+  if (k == last)
+    for (int64_t h = key + 1; h <= sp.grid.n_cells; ++h) starts[sp.cell_base + h] = (int32_t)r.offsets[s + 1];
 }
 
 // neighbour list in work space (cell_list.py:500-595).  The stencil is
@@ -382,15 +489,18 @@ JD void remap_row(const NLW& a, int64_t b, int nrow, Row row, bool remap) {
   const int64_t n = a.n;
   uint64_t out0 = 0ull, out1 = 0ull, out2 = 0ull, out3 = 0ull;
   if (remap) {
-    const int64_t ob = a.src[b];
+    // the old row of the particle now at b: at slot src[b] of the previous
+    // buffers, or (batches) at compact index cidx[b] of the staged rows
+    const int64_t ob = a.cidx ? (int64_t)a.cidx[b] : (int64_t)a.src[b];
+    const int64_t os = a.cidx ? a.ostride : n;
     for (int sweep = 0; sweep < 2; ++sweep) {
       int rank_old = 0;
       for (int w = 0; w < a.W64; ++w) {
-        uint64_t bits = a.old_live[(int64_t)w * n + ob];
+        uint64_t bits = a.old_live[(int64_t)w * os + ob];
         while (bits) {
           const int t = w * 64 + (__ffsll((unsigned long long)bits) - 1);
           bits &= bits - 1ull;
-          const int32_t o = a.old_nl[(int64_t)t * n + ob];
+          const int32_t o = a.old_nl[(int64_t)t * os + ob];
           const int s = o < 0 ? -1 : find_in_row(a.inv[o], nrow, row);
           if (s >= 0) {
             if (sweep == 0) {
@@ -402,7 +512,7 @@ JD void remap_row(const NLW& a, int64_t b, int nrow, Row row, bool remap) {
               out3 |= ws == 3 ? bit : 0ull;
             } else {
               const int rank_new = rank_below(out0, out1, out2, out3, s);
-              a.xi[(int64_t)rank_new * n + b] = a.old_xi[(int64_t)rank_old * n + ob];
+              a.xi[(int64_t)rank_new * n + b] = a.old_xi[(int64_t)rank_old * os + ob];
             }
           }
           ++rank_old;
@@ -424,27 +534,7 @@ JD void end_row(const NLW& a, int64_t b, int count) {
   remap_row(a, b, min(count, a.M), [&](int s) { return a.nl[(int64_t)s * a.n + b]; }, a.remap != 0);
 }
 
-// A system of a batch that keeps its list across another system's rebuild:
-// its row moves with the work order (entries through inv), its springs and
-// live words move unchanged.
-JD void keep_row(const NLW& a, int64_t b) {
-  const int64_t n = a.n, ob = a.src[b];
-  int c = 0;
-  for (; c < a.M; ++c) {
-    const int32_t j = a.old_nl[(int64_t)c * n + ob];
-    if (j < 0) break;
-    a.nl[(int64_t)c * n + b] = a.inv[j];
-  }
-  if (c < a.M) a.nl[(int64_t)c * n + b] = -1;
-  if (!a.live) return;
-  int live = 0;
-  for (int w = 0; w < a.W64; ++w) {
-    const uint64_t bits = a.old_live[(int64_t)w * n + ob];
-    a.live[(int64_t)w * n + b] = bits;
-    live += __popcll(bits);
-  }
-  for (int k = 0; k < live; ++k) a.xi[(int64_t)k * n + b] = a.old_xi[(int64_t)k * n + ob];
-}
+
 
 // The first kRowLds slots of each row are collected in LDS, slot-major, and
 // written out one slot at a time for the whole workgroup, so the stores of a
@@ -464,10 +554,7 @@ __global__ void __launch_bounds__(kBlock) k_nlist_work(NLW a) {
   bool remap = a.remap != 0;
   if (PM == 2 && b < a.n) {
     const int mode = a.seg_mode[a.seg[b]];
-    if (mode == SEG_KEEP) {
-      keep_row(a, b);
-      b = a.n;  // no new row for this slot (the barrier below is still reached)
-    }
+    if (mode == SEG_KEEP) b = a.n;  // the system keeps its rows (the barrier below is still reached)
     remap = mode == SEG_REMAP;
   }
   const bool active = b < a.n;
@@ -476,21 +563,23 @@ __global__ void __launch_bounds__(kBlock) k_nlist_work(NLW a) {
     GridP grid;
     DomP dom;
     float cutoff_sq;
-    uint32_t cbase = 0;
+    uint32_t cbase = 0, key;
     if (PM == 2) {
-      const SegP& sp = a.segp[a.seg[b]];
+      const int sg = a.seg[b];
+      const SegP& sp = a.segp[sg];
       grid = sp.grid;
       dom = sp.dom;
       cutoff_sq = sp.cutoff_sq;
       cbase = sp.cell_base;
+      key = a.skeys[a.cidx[b]] - (uint32_t)a.ccref[sg];  // batches sort the flagged systems compactly
     } else {
       grid = a.grid;
       dom = a.dom;
       cutoff_sq = a.cutoff_sq;
+      key = a.skeys[b];
     }
     const bool PER = PM == 2 ? dom.kind == JDEM_DOMAIN_PERIODIC : PM == 1;
     const Vec<D> xi = xyz<D>(a.P4[b]);
-    const uint32_t key = a.skeys[b] - cbase;
     const int d0 = grid.dims[0], d1 = grid.dims[1], d2 = (D == 3) ? grid.dims[2] : 1;
     int c0, c1, cf;
     if (D == 3) {
@@ -896,15 +985,6 @@ hipError_t ws_launch_hash(const float4* P4, const int32_t* orig, int64_t n, int 
   }
   return hipGetLastError();
 }
-hipError_t ws_launch_hash_seg(const float4* P4, const int32_t* orig, int64_t n, int dim, const int32_t* seg,
-                              const SegP* segp, uint32_t* keys, int32_t* wpos, hipStream_t s) {
-  dim3 gr(nblocks(n)), b(kBlock);
-  if (dim == 3)
-    hipLaunchKernelGGL(k_hash_seg<3>, gr, b, 0, s, P4, orig, n, seg, segp, keys, wpos);
-  else
-    hipLaunchKernelGGL(k_hash_seg<2>, gr, b, 0, s, P4, orig, n, seg, segp, keys, wpos);
-  return hipGetLastError();
-}
 hipError_t ws_launch_seg_ids(const int64_t* offsets, int n_seg, int64_t n, int32_t* seg, hipStream_t s) {
   if (n == 0) return hipSuccess;
   hipLaunchKernelGGL(k_seg_ids, dim3(nblocks(n)), dim3(kBlock), 0, s, offsets, n_seg, n, seg);
@@ -913,9 +993,30 @@ hipError_t ws_launch_seg_ids(const int64_t* offsets, int n_seg, int64_t n, int32
 hipError_t ws_launch_permute(int64_t n, const int32_t* new_orig, const int32_t* wpos, const float4* P,
                              const float4* V, const float4* W, const float4* Q, const float4* I, float4* P2,
                              float4* V2, float4* W2, float4* Q2, float4* I2, int32_t* orig2, int32_t* src,
-                             int32_t* inv, const float4* O, float4* O2, hipStream_t s) {
+                             int32_t* inv, hipStream_t s) {
   hipLaunchKernelGGL(k_permute, dim3(nblocks(n)), dim3(kBlock), 0, s, n, new_orig, wpos, P, V, W, Q, I, P2, V2, W2,
-                     Q2, I2, orig2, src, inv, O, O2);
+                     Q2, I2, orig2, src, inv);
+  return hipGetLastError();
+}
+hipError_t ws_launch_seg_prefix(int n_seg, const int32_t* seg_mode, const int64_t* offsets, const SegP* segp,
+                               int64_t* cpref, int64_t* ccref, hipStream_t s) {
+  hipLaunchKernelGGL(k_seg_prefix, dim3(1), dim3(1024), 0, s, n_seg, seg_mode, offsets, segp, cpref, ccref);
+  return hipGetLastError();
+}
+hipError_t ws_launch_hash_flagged(const RB& r, int dim, hipStream_t s) {
+  if (r.n == 0) return hipSuccess;
+  if (dim == 3)
+    hipLaunchKernelGGL(k_hash_flagged<3>, dim3(nblocks(r.n)), dim3(kBlock), 0, s, r);
+  else
+    hipLaunchKernelGGL(k_hash_flagged<2>, dim3(nblocks(r.n)), dim3(kBlock), 0, s, r);
+  return hipGetLastError();
+}
+hipError_t ws_launch_place(const RB& r, const uint32_t* sk, int32_t* starts, hipStream_t s) {
+  if (r.fn == 0) return hipSuccess;
+  const dim3 g(nblocks(r.fn)), b(kBlock);
+  hipLaunchKernelGGL(k_place_gather, g, b, 0, s, r);
+  hipLaunchKernelGGL(k_place_scatter, g, b, 0, s, r);
+  hipLaunchKernelGGL(k_cell_starts_flagged, g, b, 0, s, r, sk, starts);
   return hipGetLastError();
 }
 hipError_t ws_launch_seg_modes(int n_seg, int32_t* seg_flag, int32_t* n_build, int32_t* seg_mode, hipStream_t s) {
@@ -924,11 +1025,6 @@ hipError_t ws_launch_seg_modes(int n_seg, int32_t* seg_flag, int32_t* n_build, i
 }
 hipError_t ws_launch_seg_init_flags(int n_seg, const int32_t* n_build, int32_t* seg_flag, hipStream_t s) {
   hipLaunchKernelGGL(k_seg_init_flags, dim3(nblocks(n_seg)), dim3(kBlock), 0, s, n_seg, n_build, seg_flag);
-  return hipGetLastError();
-}
-hipError_t ws_launch_old_update(int64_t n, const int32_t* seg, const int32_t* seg_mode, const float4* P, float4* old4,
-                                hipStream_t s) {
-  hipLaunchKernelGGL(k_old_update, dim3(nblocks(n)), dim3(kBlock), 0, s, n, seg, seg_mode, P, old4);
   return hipGetLastError();
 }
 hipError_t ws_launch_cell_starts(const uint32_t* skeys, int64_t n, int64_t n_cells, int32_t* starts, hipStream_t s) {
