@@ -359,6 +359,33 @@ int jdem_batch_step(jdem_ctx* ctx, jdem_system* sys, const jdem_batch* batch, in
 int jdem_minimize(jdem_ctx* ctx, jdem_system* sys, const jdem_fire* fire, int64_t max_steps, double pe_tol,
                   double pe_diff_tol, double force_tol, jdem_minimize_stats* stats);
 
+/* ---- (f) precision form ---------------------------------------------------------- */
+
+/* Harmonic spheres (the spring law with one spring constant) in a periodic
+ * box whose positions are held as an integer cell index plus a float32
+ * offset inside the cell: x = cidx * cell + off per axis, 0 <= off < cell.
+ * Requirements: dims[a] >= 3 and cell[a] >= the largest diameter.  No
+ * counterpart in the reference, whose positions are absolute floats. */
+typedef struct jdem_precise {
+  int64_t n;
+  int32_t dim;
+  int32_t dims[3];  /* cells per axis; the box edge is dims * cell     */
+  float cell[3];
+  int32_t* cidx;    /* (N, D) device, in/out                           */
+  float* off;       /* (N, D) device, in/out                           */
+  const float* rad; /* (N,) device                                     */
+  float* force;     /* (N, D) device, out: forces of the last evaluation */
+  float k;          /* spring constant                                 */
+  int32_t reserved;
+} jdem_precise;
+
+/* minimize (minimizers/routines.py:150-383) with FIRE on a jdem_precise
+ * system: float32 pair arithmetic, float64 energy, virial, FIRE power and
+ * residual.  virial (D*D, may be NULL) receives the float64 contact virial
+ * sum over pairs r_ij (x) F_ij of the final configuration. */
+int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire, int64_t max_steps, double pe_tol,
+                          double pe_diff_tol, double force_tol, jdem_minimize_stats* stats, double* virial);
+
 #ifdef __cplusplus
 }
 #endif

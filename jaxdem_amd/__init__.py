@@ -31,7 +31,7 @@ from .integrators import (
 from .materials import Material, MaterialMatchmaker, MaterialTable
 from .state import Quaternion, State
 from .system import System
-from . import batching, minimizers, serving, utils
+from . import batching, minimizers, precision, serving, utils
 from .batching import Batch
 from .minimizers import fire, minimize
 
@@ -70,6 +70,7 @@ __all__ = [
     "Batch",
     "batching",
     "serving",
+    "precision",
     "fire",
     "minimize",
     "minimizers",

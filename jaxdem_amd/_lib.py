@@ -188,6 +188,21 @@ class JdemMinimizeStats(ctypes.Structure):
     ]
 
 
+class JdemPrecise(ctypes.Structure):
+    _fields_ = [
+        ("n", c_int64),
+        ("dim", c_int32),
+        ("dims", c_int32 * 3),
+        ("cell", c_float * 3),
+        ("cidx", c_void_p),
+        ("off", c_void_p),
+        ("rad", c_void_p),
+        ("force", c_void_p),
+        ("k", c_float),
+        ("reserved", c_int32),
+    ]
+
+
 STRUCTS = {
     "jdem_timing": JdemTiming,
     "jdem_state": JdemState,
@@ -200,6 +215,7 @@ STRUCTS = {
     "jdem_batch": JdemBatch,
     "jdem_fire": JdemFire,
     "jdem_minimize_stats": JdemMinimizeStats,
+    "jdem_precise": JdemPrecise,
 }
 
 # name -> (restype, argtypes) for every entry point of include/jdem.h
@@ -228,6 +244,11 @@ SIGNATURES = {
     "jdem_step_stage": (c_int, [c_void_p, POINTER(JdemSystem), c_int32]),
     "jdem_system_step": (c_int, [c_void_p, POINTER(JdemSystem), c_int64, POINTER(JdemStepStats)]),
     "jdem_batch_step": (c_int, [c_void_p, POINTER(JdemSystem), POINTER(JdemBatch), c_int64, POINTER(JdemStepStats)]),
+    "jdem_precise_minimize": (
+        c_int,
+        [c_void_p, POINTER(JdemPrecise), POINTER(JdemFire), c_int64, c_double, c_double, c_double,
+         POINTER(JdemMinimizeStats), c_void_p],
+    ),
     "jdem_minimize": (
         c_int,
         [c_void_p, POINTER(JdemSystem), POINTER(JdemFire), c_int64, c_double, c_double, c_double,

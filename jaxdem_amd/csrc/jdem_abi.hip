@@ -1371,6 +1371,170 @@ int jdem_minimize(jdem_ctx* ctx, jdem_system* sys, const jdem_fire* fire, int64_
   return JDEM_OK;
 }
 
+int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire, int64_t max_steps, double pe_tol,
+                          double pe_diff_tol, double force_tol, jdem_minimize_stats* stats, double* virial) {
+  if (!ctx) return JDEM_EINVAL;
+  if (!p || !fire) return fail(ctx, JDEM_EINVAL, "null argument");
+  const int D = p->dim;
+  const int64_t N = p->n;
+  if ((D != 2 && D != 3) || N < 0 || N > (int64_t)INT32_MAX) return fail(ctx, JDEM_EINVAL, "bad dim or n");
+  if (N > 0 && (!p->cidx || !p->off || !p->rad || !p->force)) return fail(ctx, JDEM_EINVAL, "null array");
+  int64_t n_cells = 1;
+  for (int a = 0; a < D; ++a) {
+    if (p->dims[a] < 3 || !(p->cell[a] > 0.0f)) return fail(ctx, JDEM_EINVAL, "need dims >= 3 and cell > 0");
+    n_cells *= p->dims[a];
+  }
+  if (n_cells > (int64_t)INT32_MAX) return fail(ctx, JDEM_EUNSUPPORTED, "too many cells");
+  if (max_steps < 0) return fail(ctx, JDEM_EINVAL, "max_steps < 0");
+  (void)hipSetDevice(ctx->device);
+  hipStream_t st = ctx->stream;
+  const int A = D == 3 ? 3 : 1;
+  const size_t Nn = (size_t)std::max<int64_t>(N, 1);
+  const int nparts = std::max(1u, nblocks(N));
+  // scratch: FIRE velocities and zero torque / fixed / rotation velocities,
+  // per-sphere energies and virials, cell keys and starts
+  ENSURE(ctx, ctx->fireV, 4 * Nn * (D + 2 * A) + Nn);
+  ENSURE(ctx, ctx->fireDev, sizeof(FireDev) + 8 * (size_t)nparts + 64);
+  ENSURE(ctx, ctx->red, 8 * (Nn * (1 + D * D) + 256 * D * D + D * D + 8));
+  ENSURE(ctx, ctx->wKeys, 4 * Nn);
+  ENSURE(ctx, ctx->cstart, 4 * ((size_t)n_cells + 1));
+  float* vpos = (float*)ctx->fireV.p;
+  float* vrot = vpos + Nn * D;
+  float* zt = vrot + Nn * A;
+  uint8_t* zfix = (uint8_t*)(zt + Nn * A);
+  HIPCHK(ctx, hipMemsetAsync(ctx->fireV.p, 0, 4 * Nn * (D + 2 * A) + Nn, st));
+  FireArgs f;
+  std::memset(&f, 0, sizeof(f));
+  f.n = N;
+  f.dim = D;
+  f.p.dt0 = (float)fire->dt;
+  f.p.alpha_init = (float)fire->alpha_init;
+  f.p.f_inc = (float)fire->f_inc;
+  f.p.f_dec = (float)fire->f_dec;
+  f.p.f_alpha = (float)fire->f_alpha;
+  f.p.dt_max = (float)(fire->dt * fire->dt_max_scale);
+  f.p.dt_min = (float)(fire->dt * fire->dt_min_scale);
+  f.p.n_min = fire->n_min;
+  f.p.n_bad_max = fire->n_bad_max;
+  f.st = (FireDev*)ctx->fireDev.p;
+  f.part = (double*)((char*)ctx->fireDev.p + sizeof(FireDev));
+  f.nparts = nparts;
+  f.force = p->force;
+  f.torque = zt;
+  f.fixed = zfix;
+  f.vel_pos = vpos;
+  f.vel_rot = vrot;
+  int32_t* moved = (int32_t*)((char*)ctx->fireDev.p + sizeof(FireDev) + 8 * (size_t)nparts);
+  FireDev h0;
+  std::memset(&h0, 0, sizeof(h0));
+  h0.dt = (float)fire->dt;
+  h0.alpha = (float)fire->alpha_init;
+  HIPCHK(ctx, hipMemcpyAsync(f.st, &h0, sizeof(h0), hipMemcpyHostToDevice, st));
+  double* per = (double*)ctx->red.p;
+  double* wper = per + Nn;
+  double* part = wper + Nn * D * D;
+  double* out = part + 256 * D * D;
+  PrecArgs a;
+  std::memset(&a, 0, sizeof(a));
+  a.n = N;
+  a.dim = D;
+  for (int q = 0; q < 3; ++q) {
+    a.dims[q] = q < D ? p->dims[q] : 1;
+    a.h[q] = q < D ? p->cell[q] : 1.0f;
+  }
+  a.cidx = p->cidx;
+  a.off = p->off;
+  a.rad = p->rad;
+  a.k = p->k;
+  a.keys = (uint32_t*)ctx->wKeys.p;
+  a.starts = (int32_t*)ctx->cstart.p;
+  a.force = p->force;
+  a.e_out = per;
+  a.w_out = nullptr;
+  a.moved = moved;
+  bool sorted = false;
+  double pe = 0.0, gmax = 0.0;
+  auto eval = [&](bool with_virial) -> int {
+    if (!sorted && N > 0) {  // the cell list, rebuilt when a sphere changed cell
+      HIPCHK(ctx, launch_prec_keys(a, st));
+      uint32_t* sk = nullptr;
+      int32_t* perm = nullptr;
+      HIPCHK(ctx, radix_sort_pairs(ctx, a.keys, N, nbits_for(n_cells), &sk, &perm, st));
+      a.perm = perm;
+      HIPCHK(ctx, ws_launch_cell_starts(sk, N, n_cells, (int32_t*)ctx->cstart.p, st));
+      sorted = true;
+    }
+    a.w_out = with_virial ? wper : nullptr;
+    HIPCHK(ctx, launch_prec_force(a, st));
+    HIPCHK(ctx, hipMemsetAsync(&f.st->max_grad_bits, 0, sizeof(unsigned), st));
+    HIPCHK(ctx, launch_fire_grad_max(f, st));
+    double h = 0.0;
+    if (N > 0) {
+      HIPCHK(ctx, launch_sum_f64(per, N, 1, 1, part, out, 256, st));
+      HIPCHK(ctx, hipMemcpyAsync(&h, out, 8, hipMemcpyDeviceToHost, st));
+    }
+    unsigned bits = 0;
+    int32_t mv = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&bits, &f.st->max_grad_bits, sizeof(bits), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipMemcpyAsync(&mv, moved, sizeof(mv), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    float g;
+    std::memcpy(&g, &bits, sizeof(g));
+    pe = h;
+    gmax = (double)g;
+    return JDEM_OK;
+  };
+  HIPCHK(ctx, hipMemsetAsync(moved, 0, sizeof(int32_t), st));
+  int rc = eval(false);
+  if (rc) return rc;
+  double prev = INFINITY;
+  int64_t steps = 0;
+  const double tiny = 2.2250738585072014e-308;
+  const double dN = (double)std::max<int64_t>(N, 1);
+  while (steps < max_steps) {
+    const bool conv_pe = std::fabs(pe / dN) <= pe_tol;
+    const double denom = std::max(std::max(std::fabs(pe), std::fabs(prev)), tiny);
+    const bool conv_rel = std::fabs(pe - prev) / denom < pe_diff_tol;
+    if (conv_pe || conv_rel || gmax <= force_tol) break;
+    HIPCHK(ctx, launch_fire_power(f, st));
+    HIPCHK(ctx, launch_fire_decide(f, st));
+    HIPCHK(ctx, hipMemsetAsync(moved, 0, sizeof(int32_t), st));
+    HIPCHK(ctx, launch_prec_move(f, a, st));
+    HIPCHK(ctx, launch_fire_commit(f, st));
+    prev = pe;
+    // a sphere that left its cell needs a new cell list before the evaluation
+    int32_t mv = 0;
+    HIPCHK(ctx, hipMemcpyAsync(&mv, moved, sizeof(mv), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    if (mv) sorted = false;
+    rc = eval(false);
+    if (rc) return rc;
+    ++steps;
+  }
+  if (virial) {
+    rc = eval(true);  // same configuration: the virial of the final state
+    if (rc) return rc;
+    double w9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    if (N > 0) {
+      HIPCHK(ctx, launch_sum_f64(wper, N, D * D, D * D, part, out, 256, st));
+      HIPCHK(ctx, hipMemcpyAsync(w9, out, 8 * (size_t)(D * D), hipMemcpyDeviceToHost, st));
+      HIPCHK(ctx, hipStreamSynchronize(st));
+    }
+    for (int q = 0; q < D * D; ++q) virial[q] = w9[q];
+  }
+  if (stats) {
+    FireDev h;
+    HIPCHK(ctx, hipMemcpyAsync(&h, f.st, sizeof(h), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    stats->steps = steps;
+    stats->pe = pe / dN;
+    stats->max_grad = gmax;
+    stats->power = h.power;
+    stats->overflow = 0;
+  }
+  return JDEM_OK;
+}
+
 int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats* stats) {
   int rc = check_system(ctx, sys);
   if (rc) return rc;

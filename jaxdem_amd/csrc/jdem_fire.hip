@@ -212,7 +212,131 @@ __global__ void __launch_bounds__(kBlock) k_fire_gravity_energy(FireArgs a) {
   a.e_out[i] += (double)(-(gx * a.mass[i]));
 }
 
+// ---- precision form ---------------------------------------------------------
+// Positions are x = c h + o per axis, c an integer cell and o in [0, h): the
+// float32 offset keeps ~1e-7 h of resolution anywhere in the box, where an
+// absolute float32 coordinate keeps ~1e-7 L.  The displacement of two spheres
+// in neighbouring cells, (c_i - c_j) h + (o_i - o_j), is formed from the
+// stencil offset (an exact small integer) and the offset difference.
+
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_prec_keys(PrecArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int32_t* c = a.cidx + D * i;
+  a.keys[i] = D == 3 ? (uint32_t)((c[0] * a.dims[1] + c[1]) * a.dims[2] + c[2]) : (uint32_t)(c[0] * a.dims[1] + c[1]);
+}
+
+// spring law (spring.py:97-108, 138-147) on the cell stencil; float32 pair
+// arithmetic, float64 energy and virial
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_prec_force(PrecArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  int ci[3] = {0, 0, 0};
+  float oi[3] = {0, 0, 0};
+  for (int q = 0; q < D; ++q) {
+    ci[q] = a.cidx[D * i + q];
+    oi[q] = a.off[D * i + q];
+  }
+  const float ri = a.rad[i];
+  float F[3] = {0, 0, 0};
+  double e = 0.0, w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int zr = D == 3 ? 1 : 0;
+  for (int ox = -1; ox <= 1; ++ox)
+    for (int oy = -1; oy <= 1; ++oy)
+      for (int oz = -zr; oz <= zr; ++oz) {
+        const int o[3] = {ox, oy, oz};
+        int cc[3] = {0, 0, 0};
+        for (int q = 0; q < D; ++q) {
+          cc[q] = (ci[q] + o[q]) % a.dims[q];
+          if (cc[q] < 0) cc[q] += a.dims[q];
+        }
+        const int64_t h = D == 3 ? ((int64_t)cc[0] * a.dims[1] + cc[1]) * a.dims[2] + cc[2]
+                                 : (int64_t)cc[0] * a.dims[1] + cc[1];
+        const int k1 = a.starts[h + 1];
+        for (int kk = a.starts[h]; kk < k1; ++kk) {
+          const int64_t j = a.perm[kk];
+          if (j == i) continue;
+          float r[3] = {0, 0, 0};
+          for (int q = 0; q < D; ++q) r[q] = (float)(-o[q]) * a.h[q] + (oi[q] - a.off[D * j + q]);
+          const float d2 = D == 3 ? (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
+          const float R = ri + a.rad[j];
+          if (!(d2 < R * R) || d2 == 0.0f) continue;
+          const float inv = 1.0f / sqrtf(d2);
+          const float delta = R - d2 * inv;
+          const float s = (a.k * delta) * inv;
+          for (int q = 0; q < D; ++q) F[q] = F[q] + s * r[q];
+          const double dd = (double)delta;
+          e += 0.25 * (double)a.k * dd * dd;
+          if (a.w_out)
+            for (int p = 0; p < D; ++p)
+              for (int q = 0; q < D; ++q) w[D * p + q] += 0.5 * (double)r[p] * (double)(s * r[q]);
+        }
+      }
+  for (int q = 0; q < D; ++q) a.force[D * i + q] = F[q];
+  a.e_out[i] = e;
+  if (a.w_out)
+    for (int q = 0; q < D * D; ++q) a.w_out[(int64_t)D * D * i + q] = w[q];
+}
+
+// the FIRE update of the position leaf (no rotation: the spring law has no
+// torque), then the offset is folded back into [0, h)
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_prec_move(FireArgs f, PrecArgs a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const FireDev s = *f.st;
+  Leaf<D> lp;
+  lp.load(f.force, f.vel_pos, i, 1.0f, s.dt);
+  float up[D];
+  lp.mix(s, f.vel_pos, i, up);
+  bool moved = false;
+#pragma unroll
+  for (int q = 0; q < D; ++q) {
+    float o = a.off[D * i + q] + up[q];
+    int c = a.cidx[D * i + q];
+    if (!(o >= 0.0f && o < a.h[q])) {
+      const float m = floorf(o / a.h[q]);
+      o = o - m * a.h[q];
+      if (o < 0.0f) o = 0.0f;  // rounding of o - m h
+      if (o >= a.h[q]) o = 0.0f, c += 1;
+      c = (c + (int)m) % a.dims[q];
+      if (c < 0) c += a.dims[q];
+      a.cidx[D * i + q] = c;
+      moved = true;
+    }
+    a.off[D * i + q] = o;
+  }
+  if (moved) *a.moved = 1;
+}
+
 }  // namespace
+
+hipError_t launch_prec_keys(const PrecArgs& a, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  if (a.dim == 3)
+    hipLaunchKernelGGL(k_prec_keys<3>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_prec_keys<2>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_prec_force(const PrecArgs& a, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  if (a.dim == 3)
+    hipLaunchKernelGGL(k_prec_force<3>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_prec_force<2>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_prec_move(const FireArgs& f, const PrecArgs& a, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  if (a.dim == 3)
+    hipLaunchKernelGGL(k_prec_move<3>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, f, a);
+  else
+    hipLaunchKernelGGL(k_prec_move<2>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, f, a);
+  return hipGetLastError();
+}
 
 hipError_t launch_fire_power(const FireArgs& a, hipStream_t s) {
   if (a.dim == 3)

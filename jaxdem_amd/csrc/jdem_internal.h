@@ -420,6 +420,28 @@ hipError_t launch_fire_commit(const FireArgs& a, hipStream_t s);
 hipError_t launch_fire_grad_max(const FireArgs& a, hipStream_t s);
 hipError_t launch_fire_gravity_energy(const FireArgs& a, hipStream_t s);
 
+// ---- precision form: integer cell + float32 offset (jdem_fire.hip) ----------
+struct PrecArgs {
+  int64_t n;
+  int dim;
+  int dims[3];   // cells per axis (periodic)
+  float h[3];    // cell edges
+  int32_t* cidx; // (N, D) cell of each sphere
+  float* off;    // (N, D) position inside the cell, [0, h)
+  const float* rad;
+  float k;
+  uint32_t* keys;        // cell key of each sphere
+  const int32_t* perm;   // cell-sorted position -> sphere
+  const int32_t* starts; // first sorted position of each cell (n_cells + 1)
+  float* force;          // (N, D)
+  double* e_out;         // (N) half the pair energies of each sphere
+  double* w_out;         // (N, D*D) half the pair virials, or null
+  int32_t* moved;        // set when a sphere changed cell
+};
+hipError_t launch_prec_keys(const PrecArgs& a, hipStream_t s);
+hipError_t launch_prec_force(const PrecArgs& a, hipStream_t s);
+hipError_t launch_prec_move(const FireArgs& f, const PrecArgs& a, hipStream_t s);
+
 inline unsigned nblocks(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }
 
 }  // namespace jdem

@@ -1,4 +1,5 @@
-"""Build oracle/libjdem_port.so (TEST INFRASTRUCTURE: the CPU baseline port).
+"""Build oracle/libjdem_port.so (TEST INFRASTRUCTURE: the CPU baseline port
+and the float64 FIRE reference).
 
 g++ with OpenMP; FMA contraction off and no fast-math so that every float32
 operation rounds like the numpy oracle.
@@ -12,7 +13,7 @@ import subprocess
 from pathlib import Path
 
 HERE = Path(__file__).resolve().parent
-SRC = HERE / "jdem_port.cpp"
+SRC = [HERE / "jdem_port.cpp", HERE / "jdem_fire64.cpp"]
 LIB = HERE / "libjdem_port.so"
 
 
@@ -20,7 +21,7 @@ def build(verbose: bool = False) -> Path:
     cxx = os.environ.get("CXX") or shutil.which("g++") or "g++"
     tmp = LIB.with_suffix(".so.tmp")
     cmd = [cxx, "-O3", "-std=c++17", "-fopenmp", "-fPIC", "-shared", "-ffp-contract=off", "-fno-fast-math",
-           str(SRC), "-o", str(tmp)]
+           *map(str, SRC), "-o", str(tmp)]
     if verbose:
         print(" ".join(cmd))
     r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)

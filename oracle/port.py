@@ -117,3 +117,59 @@ class PortRun:
         if rc != 0:
             raise RuntimeError(f"jdem_port_steps returned {rc}")
         return dt
+
+
+class Fire64Params(ctypes.Structure):
+    _fields_ = [
+        ("box", ctypes.c_double),
+        ("k", ctypes.c_double),
+        ("dt", ctypes.c_double),
+        ("alpha_init", ctypes.c_double),
+        ("f_inc", ctypes.c_double),
+        ("f_dec", ctypes.c_double),
+        ("f_alpha", ctypes.c_double),
+        ("dt_max_scale", ctypes.c_double),
+        ("dt_min_scale", ctypes.c_double),
+        ("n_min", c_int),
+        ("n_bad_max", c_int),
+        ("max_steps", c_int64),
+        ("pe_tol", ctypes.c_double),
+        ("pe_diff_tol", ctypes.c_double),
+        ("force_tol", ctypes.c_double),
+    ]
+
+
+class Fire64Result(ctypes.Structure):
+    _fields_ = [
+        ("steps", c_int64),
+        ("pe", ctypes.c_double),
+        ("max_grad", ctypes.c_double),
+        ("virial", ctypes.c_double * 9),
+    ]
+
+
+def fire64(pos, rad, box: float, k: float = 1.0, dt: float = 1e-2, max_steps: int = 10000, pe_tol: float = 1e-16,
+           pe_diff_tol: float = 1e-16, force_tol: float = 0.0, threads: int = 0, **fire_kw) -> dict:
+    """float64 FIRE of harmonic spheres in a periodic cube (jdem_fire64.cpp)."""
+    lib = ctypes.CDLL(str(LIB))
+    lib.fire64_minimize.restype = c_int
+    lib.fire64_minimize.argtypes = [ctypes.POINTER(Fire64Params), c_int64, c_void_p, c_void_p,
+                                    ctypes.POINTER(Fire64Result), c_int]
+    p = Fire64Params()
+    p.box, p.k, p.dt = float(box), float(k), float(dt)
+    p.alpha_init = fire_kw.get("alpha_init", 0.1)
+    p.f_inc = fire_kw.get("f_inc", 1.1)
+    p.f_dec = fire_kw.get("f_dec", 0.5)
+    p.f_alpha = fire_kw.get("f_alpha", 0.99)
+    p.dt_max_scale = fire_kw.get("dt_max_scale", 10.0)
+    p.dt_min_scale = fire_kw.get("dt_min_scale", 1e-3)
+    p.n_min = fire_kw.get("N_min", 5)
+    p.n_bad_max = fire_kw.get("N_bad_max", 10)
+    p.max_steps, p.pe_tol, p.pe_diff_tol, p.force_tol = int(max_steps), pe_tol, pe_diff_tol, force_tol
+    x = np.ascontiguousarray(pos, np.float64).copy()
+    r = np.ascontiguousarray(rad, np.float64)
+    out = Fire64Result()
+    t = time.perf_counter()
+    lib.fire64_minimize(ctypes.byref(p), x.shape[0], x.ctypes.data, r.ctypes.data, ctypes.byref(out), int(threads))
+    return {"pos": x, "steps": int(out.steps), "pe": float(out.pe), "max_grad": float(out.max_grad),
+            "virial": np.array(out.virial[:]).reshape(3, 3), "seconds": time.perf_counter() - t}

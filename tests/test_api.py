@@ -237,3 +237,21 @@ def test_serving_footprint_and_latency_stats():
     assert s["requests"] == 4 and s["makespan_s"] == 4.0
     assert s["p50_latency_s"] == pytest.approx(1.0) and s["p99_latency_s"] == pytest.approx(2.94)
     assert s["particle_steps"] == 4 * 500 * 10
+
+
+def test_precision_cell_offset_round_trip():
+    """x = c h + o with 0 <= o < h; the float64 reconstruction is within one
+    float32 rounding of the offset."""
+    from jaxdem_amd import precision as PR
+
+    rng = np.random.default_rng(3)
+    box = np.array([73.0, 73.0, 73.0])
+    pos = rng.uniform(-80.0, 150.0, (5000, 3))
+    dims, h = PR.grid_for(box, np.array([0.5, 0.7]))
+    assert list(dims) == [52, 52, 52]
+    c, o = PR.to_cell_offset(pos, box, dims)
+    assert c.dtype == np.int32 and o.dtype == np.float32
+    assert (c >= 0).all() and (c < 52).all() and (o >= 0).all() and (o < h.astype(np.float32)).all()
+    back = PR.from_cell_offset(c, o, box, dims)
+    err = np.abs(PR.min_image_diff(back, pos, box))
+    assert err.max() < 2e-7  # float32 resolution of an offset below 1.41, not of a coordinate near 73

@@ -502,3 +502,25 @@ def test_fire_float32_tracks_float64():
     _, pe64 = O.minimize(s64, P64, O.FireParams(dt=0.05), max_steps=60, pe_tol=0.0, pe_diff_tol=-1.0)
     assert pe32 == pytest.approx(pe64, rel=1e-4)
     np.testing.assert_allclose(s32.pos, s64.pos, atol=1e-4)
+
+
+def test_fire64_reference_matches_numpy_oracle():
+    """The C++ float64 FIRE (the precision form's full-FP64 reference) takes
+    the same iterates as the numpy oracle in float64."""
+    from oracle import build_port, port
+
+    build_port.build()
+    rng = np.random.default_rng(0)
+    N = 64
+    rad = np.where(np.arange(N) % 2 == 0, 0.5, 0.7)
+    L = float((np.sum(4.0 / 3.0 * np.pi * rad**3) / 0.7) ** (1.0 / 3.0))
+    pos = rng.uniform(0, L, (N, 3))
+    r = port.fire64(pos, rad, L, k=1.0, dt=0.05, max_steps=20000, pe_tol=0.0, pe_diff_tol=-1.0, force_tol=1e-12)
+    st = O.make_state(pos, rad, dtype=F64)
+    d = O.Domain("periodic", np.full(3, L), np.zeros(3))
+    P = O.Params(dim=3, dt=0.05, domain=d, law=O.Law("spring", young_eff=np.ones((1, 1))), collider="naive")
+    steps, pe = O.minimize(st, P, O.FireParams(dt=0.05), max_steps=20000, pe_tol=0.0, pe_diff_tol=-1.0,
+                           force_tol=1e-12)
+    assert r["steps"] == steps and r["max_grad"] <= 1e-12
+    assert r["pe"] == pytest.approx(pe, rel=1e-12)
+    np.testing.assert_allclose(r["pos"], st.pos, atol=1e-9)
