@@ -371,12 +371,15 @@ typedef struct jdem_precise {
   int32_t dim;
   int32_t dims[3];  /* cells per axis; the box edge is dims * cell     */
   float cell[3];
+  float cell_lo[3]; /* low parts: the edge is cell + cell_lo exactly     */
   int32_t* cidx;    /* (N, D) device, in/out                           */
   float* off;       /* (N, D) device, in/out                           */
   const float* rad; /* (N,) device                                     */
   float* force;     /* (N, D) device, out: forces of the last evaluation */
   float k;          /* spring constant                                 */
-  int32_t reserved;
+  int32_t stall_steps; /* stop once the largest force component has not
+                          reached a new low for this many steps (0: never);
+                          float32 pair arithmetic has a residual floor    */
 } jdem_precise;
 
 /* minimize (minimizers/routines.py:150-383) with FIRE on a jdem_precise

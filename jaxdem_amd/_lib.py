@@ -194,12 +194,13 @@ class JdemPrecise(ctypes.Structure):
         ("dim", c_int32),
         ("dims", c_int32 * 3),
         ("cell", c_float * 3),
+        ("cell_lo", c_float * 3),
         ("cidx", c_void_p),
         ("off", c_void_p),
         ("rad", c_void_p),
         ("force", c_void_p),
         ("k", c_float),
-        ("reserved", c_int32),
+        ("stall_steps", c_int32),
     ]
 
 

@@ -1441,6 +1441,7 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
   for (int q = 0; q < 3; ++q) {
     a.dims[q] = q < D ? p->dims[q] : 1;
     a.h[q] = q < D ? p->cell[q] : 1.0f;
+    a.hlo[q] = q < D ? p->cell_lo[q] : 0.0f;
   }
   a.cidx = p->cidx;
   a.off = p->off;
@@ -1491,11 +1492,15 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
   int64_t steps = 0;
   const double tiny = 2.2250738585072014e-308;
   const double dN = (double)std::max<int64_t>(N, 1);
+  double best = gmax;
+  int64_t best_at = 0;
   while (steps < max_steps) {
     const bool conv_pe = std::fabs(pe / dN) <= pe_tol;
     const double denom = std::max(std::max(std::fabs(pe), std::fabs(prev)), tiny);
     const bool conv_rel = std::fabs(pe - prev) / denom < pe_diff_tol;
-    if (conv_pe || conv_rel || gmax <= force_tol) break;
+    if (gmax < best) best = gmax, best_at = steps;
+    const bool stalled = p->stall_steps > 0 && steps - best_at >= p->stall_steps;
+    if (conv_pe || conv_rel || gmax <= force_tol || stalled) break;
     HIPCHK(ctx, launch_fire_power(f, st));
     HIPCHK(ctx, launch_fire_decide(f, st));
     HIPCHK(ctx, hipMemsetAsync(moved, 0, sizeof(int32_t), st));

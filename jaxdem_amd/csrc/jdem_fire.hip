@@ -215,9 +215,11 @@ __global__ void __launch_bounds__(kBlock) k_fire_gravity_energy(FireArgs a) {
 // ---- precision form ---------------------------------------------------------
 // Positions are x = c h + o per axis, c an integer cell and o in [0, h): the
 // float32 offset keeps ~1e-7 h of resolution anywhere in the box, where an
-// absolute float32 coordinate keeps ~1e-7 L.  The displacement of two spheres
-// in neighbouring cells, (c_i - c_j) h + (o_i - o_j), is formed from the
-// stencil offset (an exact small integer) and the offset difference.
+// absolute float32 coordinate keeps ~1e-7 L.  The cell edge is the float pair
+// h + hlo, so the lattice matches the box to float64 accuracy.  The
+// displacement of two spheres in neighbouring cells, (c_i - c_j) h + (o_i -
+// o_j), is formed as a float pair too (error-free sums), so it is exact to
+// ~1e-14 |r| rather than rounded to ~6e-8 |r|.
 
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_prec_keys(PrecArgs a) {
@@ -258,13 +260,43 @@ __global__ void __launch_bounds__(kBlock) k_prec_force(PrecArgs a) {
         for (int kk = a.starts[h]; kk < k1; ++kk) {
           const int64_t j = a.perm[kk];
           if (j == i) continue;
-          float r[3] = {0, 0, 0};
-          for (int q = 0; q < D; ++q) r[q] = (float)(-o[q]) * a.h[q] + (oi[q] - a.off[D * j + q]);
+          // r = rh + rl, both float32: two_sum of the cell term (exact, the
+          // stencil offset is -1, 0 or 1) and the offset difference, whose
+          // own rounding error two_sum recovers too
+          float r[3] = {0, 0, 0}, rl[3] = {0, 0, 0};
+          for (int q = 0; q < D; ++q) {
+            const float oj = a.off[D * j + q];
+            const float b = oi[q] - oj;
+            const float bv = b - oi[q];
+            const float be = (oi[q] - (b - bv)) + (-oj - bv);
+            const float c = (float)(-o[q]) * a.h[q];
+            const float t = c + b;
+            const float tv = t - c;
+            const float te = (c - (t - tv)) + (b - tv);
+            r[q] = t;
+            rl[q] = (te + be) + (float)(-o[q]) * a.hlo[q];
+          }
           const float d2 = D == 3 ? (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
           const float R = ri + a.rad[j];
           if (!(d2 < R * R) || d2 == 0.0f) continue;
           const float inv = 1.0f / sqrtf(d2);
-          const float delta = R - d2 * inv;
+          // delta = (R^2 - |r|^2) / (R + |r|) with |r|^2 = sum rh^2 + 2 rh rl
+          // and the squares split exactly into hi + lo with fma, so the
+          // cancellation of R against |r| at small overlaps loses nothing;
+          // R - |r| directly would keep only ~1e-7 |r| / delta of relative
+          // accuracy
+          float sh = 0.0f, sl = 0.0f;
+          for (int q = 0; q < D; ++q) {
+            const float p = r[q] * r[q];
+            const float pe = fmaf(r[q], r[q], -p);
+            const float t = sh + p;
+            const float bp = t - sh;
+            sl = sl + ((((sh - (t - bp)) + (p - bp)) + pe) + 2.0f * r[q] * rl[q]);
+            sh = t;
+          }
+          const float RR = R * R;
+          const float diff = (RR - sh) + (fmaf(R, R, -RR) - sl);
+          const float delta = diff / (R + d2 * inv);
           const float s = (a.k * delta) * inv;
           for (int q = 0; q < D; ++q) F[q] = F[q] + s * r[q];
           const double dd = (double)delta;
@@ -298,7 +330,7 @@ __global__ void __launch_bounds__(kBlock) k_prec_move(FireArgs f, PrecArgs a) {
     int c = a.cidx[D * i + q];
     if (!(o >= 0.0f && o < a.h[q])) {
       const float m = floorf(o / a.h[q]);
-      o = o - m * a.h[q];
+      o = (o - m * a.h[q]) - m * a.hlo[q];
       if (o < 0.0f) o = 0.0f;  // rounding of o - m h
       if (o >= a.h[q]) o = 0.0f, c += 1;
       c = (c + (int)m) % a.dims[q];

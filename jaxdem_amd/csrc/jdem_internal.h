@@ -425,7 +425,8 @@ struct PrecArgs {
   int64_t n;
   int dim;
   int dims[3];   // cells per axis (periodic)
-  float h[3];    // cell edges
+  float h[3];    // cell edges: h + hlo as an unevaluated float pair
+  float hlo[3];
   int32_t* cidx; // (N, D) cell of each sphere
   float* off;    // (N, D) position inside the cell, [0, h)
   const float* rad;

@@ -32,18 +32,27 @@ def grid_for(box, rad) -> tuple:
     return dims, (box / dims)
 
 
+def cell_edges(box, dims) -> tuple:
+    """The cell edge box / dims as a float32 pair (hi, lo) and their sum in
+    float64, the lattice constant both the GPU and the conversions use."""
+    h = np.asarray(box, np.float64) / np.asarray(dims, np.int64)
+    hi = h.astype(F32)
+    lo = (h - hi.astype(np.float64)).astype(F32)
+    return hi, lo, hi.astype(np.float64) + lo.astype(np.float64)
+
+
 def to_cell_offset(pos, box, dims) -> tuple:
     """float64 positions -> (cell index int32, offset float32), the offset
     rounded once and kept in [0, h)."""
     pos = np.asarray(pos, np.float64)
     box = np.asarray(box, np.float64)
     dims = np.asarray(dims, np.int64)
-    h = box / dims
-    x = pos - box * np.floor(pos / box)
+    hi, _, h = cell_edges(box, dims)
+    L = dims * h
+    x = pos - L * np.floor(pos / L)
     c = np.clip(np.floor(x / h).astype(np.int64), 0, dims - 1)
     o = (x - c * h).astype(F32)
-    h32 = h.astype(F32)
-    over = o >= h32
+    over = o >= hi
     o = np.where(over, F32(0.0), np.maximum(o, F32(0.0))).astype(F32)
     c = np.where(over, (c + 1) % dims, c)
     return c.astype(np.int32), o
@@ -51,22 +60,28 @@ def to_cell_offset(pos, box, dims) -> tuple:
 
 def from_cell_offset(cidx, off, box, dims) -> np.ndarray:
     """Positions in [0, L) as float64 from the cell index and offset."""
-    h = np.asarray(box, np.float64) / np.asarray(dims, np.int64)
+    _, _, h = cell_edges(box, dims)
     return np.asarray(cidx, np.float64) * h + np.asarray(off, np.float64)
 
 
 def minimize_precise(pos, rad, box, k: float = 1.0, fire_cfg: FireConfig | None = None, dt: float = 1e-2,
                      max_steps: int = 10000, pe_tol: float = 1e-16, pe_diff_tol: float = 1e-16,
-                     force_tol: float = 0.0, device="cuda") -> dict:
+                     force_tol: float = 0.0, stall_steps: int = 0, device="cuda") -> dict:
     """FIRE (minimizers/routines.py:150-383) of harmonic spheres in a periodic
     box on the precision-form representation.  Returns float64 positions, the
     energy per particle, the largest force component, the contact virial and
-    the pressure tr(virial)/(D V), the step count and the forces."""
+    the pressure tr(virial)/(D V), the step count and the forces.
+
+    The loop stops on the reference's three tests (pe_tol, pe_diff_tol,
+    force_tol) or at ``max_steps``; with ``stall_steps`` > 0 it also stops
+    once max |F| has not reached a new low for that many steps.  The float32
+    pair arithmetic and the float32 offsets leave a residual floor near
+    1e-7 k h, below which force_tol cannot be met."""
     pos = np.asarray(pos, np.float64)
     N, D = pos.shape
     rad = np.asarray(rad, F32)
     box = np.broadcast_to(np.asarray(box, np.float64), (D,)).copy()
-    dims, h = grid_for(box, rad)
+    dims, _ = grid_for(box, rad)
     c, o = to_cell_offset(pos, box, dims)
     dev = torch.device(device)
     tc = torch.as_tensor(c, device=dev).contiguous()
@@ -76,14 +91,17 @@ def minimize_precise(pos, rad, box, k: float = 1.0, fire_cfg: FireConfig | None 
     p = L.JdemPrecise()
     p.n = N
     p.dim = D
+    hi, lo, _ = cell_edges(box, dims)
     for a in range(3):
         p.dims[a] = int(dims[a]) if a < D else 1
-        p.cell[a] = float(F32(h[a])) if a < D else 1.0
+        p.cell[a] = float(hi[a]) if a < D else 1.0
+        p.cell_lo[a] = float(lo[a]) if a < D else 0.0
     p.cidx = tc.data_ptr()
     p.off = to.data_ptr()
     p.rad = tr.data_ptr()
     p.force = tf.data_ptr()
     p.k = float(F32(k))
+    p.stall_steps = int(stall_steps)
     cfg = fire_cfg or fire(dt)
     fs = cfg.struct()
     stats = L.JdemMinimizeStats()
@@ -117,4 +135,4 @@ def min_image_diff(a, b, box) -> np.ndarray:
     return d - box * np.rint(d / box)
 
 
-__all__ = ["grid_for", "to_cell_offset", "from_cell_offset", "minimize_precise", "min_image_diff"]
+__all__ = ["grid_for", "cell_edges", "to_cell_offset", "from_cell_offset", "minimize_precise", "min_image_diff"]

@@ -249,6 +249,17 @@ def serving_requests(n_requests: int, rate: float, seed: int = 0, n_min: int = 1
     return out
 
 
+def bidisperse_jamming(N: int = 1 << 18, seed: int = 0, phi: float = 0.66) -> tuple:
+    """Energy-minimisation start: N spheres, radii 0.5 and 0.7 (1 : 1.4)
+    alternating 50 : 50, uniform random positions (float64) in a periodic
+    cube at volume fraction phi (N = 2^18 at 0.66 gives L = 73.02).
+    Returns (pos, rad, L)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    rad = np.where(np.arange(N) % 2 == 0, 0.5, 0.7)
+    L = float((np.sum(4.0 / 3.0 * math.pi * rad**3) / phi) ** (1.0 / 3.0))
+    return rng.uniform(0.0, L, (N, 3)), rad, L
+
+
 def make_gpu_system(w: Workload, device="cuda", max_neighbors: int | None = None, collider: str | None = None):
     """(State, System) of the jaxdem_amd API for a workload."""
     from . import Material, MaterialMatchmaker, MaterialTable, State, System

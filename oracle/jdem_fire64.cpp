@@ -7,8 +7,9 @@
 // minimize), i.e. minimizers/optimizers.py:241-318 and
 // minimizers/routines.py:150-383, with the spring law of spring.py:97-108 and
 // 138-147 (F = k delta rij / d, E = k delta^2 / 2), all in float64, and the
-// periodic minimum image of periodic.py:39-79.  Pairs are found through a
-// cell list rebuilt at every evaluation (cells >= the largest diameter).
+// periodic minimum image of periodic.py:39-79.  Pairs come from a Verlet list
+// (cutoff R_ij + skin) built over a cell list and rebuilt once a sphere has
+// moved half the skin; its order is the cell-stencil traversal's.
 #include <cmath>
 #include <cstdint>
 #include <cstring>
@@ -74,10 +75,69 @@ void build_cells(Cells& c, int64_t n, const double* pos, double L, double rmax) 
   for (int64_t i = 0; i < n; ++i) c.order[fill[key[i]]++] = i;
 }
 
-// forces, energy per particle (half of each pair), virial
-void evaluate(const Cells& c, int64_t n, const double* pos, const double* rad, double L, double k, double* F,
-              double* E, double* W9) {
+// Verlet list: for every sphere, the spheres within R_ij + skin, in the
+// order of the cell-stencil traversal (so the force sums run in that order)
+struct Verlet {
+  std::vector<int64_t> start, nbr;
+  std::vector<double> pos0;  // positions at the build
+};
+
+void build_verlet(Verlet& v, const Cells& c, int64_t n, const double* pos, const double* rad, double L, double skin) {
   const int d = c.dims;
+  std::vector<int64_t> cnt(n + 1, 0);
+  auto visit = [&](int64_t i, auto&& emit) {
+    int ci[3];
+    for (int a = 0; a < 3; ++a) {
+      double t = pos[3 * i + a] / L;
+      t -= std::floor(t);
+      ci[a] = std::min(std::max((int)std::floor(t * d), 0), d - 1);
+    }
+    // distinct neighbour cells (fewer than 3 per axis are visited once)
+    int offs[3][3], no[3];
+    for (int a = 0; a < 3; ++a) {
+      no[a] = 0;
+      for (int o = -1; o <= 1; ++o) {
+        const int cc = ((ci[a] + o) % d + d) % d;
+        bool seen = false;
+        for (int q = 0; q < no[a]; ++q) seen |= offs[a][q] == cc;
+        if (!seen) offs[a][no[a]++] = cc;
+      }
+    }
+    for (int x = 0; x < no[0]; ++x)
+      for (int y = 0; y < no[1]; ++y)
+        for (int z = 0; z < no[2]; ++z) {
+          const int64_t h = ((int64_t)offs[0][x] * d + offs[1][y]) * d + offs[2][z];
+          for (int64_t q = c.start[h]; q < c.start[h + 1]; ++q) {
+            const int64_t j = c.order[q];
+            if (j == i) continue;
+            double r[3];
+            for (int a = 0; a < 3; ++a) r[a] = wrap(pos[3 * i + a] - pos[3 * j + a], L);
+            const double d2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+            const double R = rad[i] + rad[j] + skin;
+            if (d2 < R * R) emit(j);
+          }
+        }
+  };
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t m = 0;
+    visit(i, [&](int64_t) { ++m; });
+    cnt[i + 1] = m;
+  }
+  for (int64_t i = 0; i < n; ++i) cnt[i + 1] += cnt[i];
+  v.start = cnt;
+  v.nbr.assign(cnt[n], 0);
+#pragma omp parallel for schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    int64_t w = v.start[i];
+    visit(i, [&](int64_t j) { v.nbr[w++] = j; });
+  }
+  v.pos0.assign(pos, pos + 3 * n);
+}
+
+// forces, energy per particle (half of each pair), virial
+void evaluate(const Verlet& v, int64_t n, const double* pos, const double* rad, double L, double k, double* F,
+              double* E, double* W9) {
   double vir[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma omp parallel
   {
@@ -85,44 +145,21 @@ void evaluate(const Cells& c, int64_t n, const double* pos, const double* rad, d
 #pragma omp for schedule(static)
     for (int64_t i = 0; i < n; ++i) {
       double f[3] = {0, 0, 0}, e = 0.0;
-      int ci[3];
-      for (int a = 0; a < 3; ++a) {
-        double t = pos[3 * i + a] / L;
-        t -= std::floor(t);
-        ci[a] = std::min(std::max((int)std::floor(t * d), 0), d - 1);
+      for (int64_t q = v.start[i]; q < v.start[i + 1]; ++q) {
+        const int64_t j = v.nbr[q];
+        double r[3];
+        for (int a = 0; a < 3; ++a) r[a] = wrap(pos[3 * i + a] - pos[3 * j + a], L);
+        const double d2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
+        const double R = rad[i] + rad[j];
+        if (d2 >= R * R || d2 == 0.0) continue;
+        const double dist = std::sqrt(d2);
+        const double delta = R - dist;
+        const double s = k * delta / dist;
+        for (int a = 0; a < 3; ++a) f[a] += s * r[a];
+        e += 0.25 * k * delta * delta;  // half of the pair energy, counted from both sides
+        for (int a = 0; a < 3; ++a)
+          for (int b = 0; b < 3; ++b) lv[3 * a + b] += 0.5 * r[a] * (s * r[b]);
       }
-      // distinct neighbour cells (fewer than 3 per axis are visited once)
-      int offs[3][3], no[3];
-      for (int a = 0; a < 3; ++a) {
-        no[a] = 0;
-        for (int o = -1; o <= 1; ++o) {
-          const int cc = ((ci[a] + o) % d + d) % d;
-          bool seen = false;
-          for (int q = 0; q < no[a]; ++q) seen |= offs[a][q] == cc;
-          if (!seen) offs[a][no[a]++] = cc;
-        }
-      }
-      for (int x = 0; x < no[0]; ++x)
-        for (int y = 0; y < no[1]; ++y)
-          for (int z = 0; z < no[2]; ++z) {
-            const int64_t h = ((int64_t)offs[0][x] * d + offs[1][y]) * d + offs[2][z];
-            for (int64_t q = c.start[h]; q < c.start[h + 1]; ++q) {
-              const int64_t j = c.order[q];
-              if (j == i) continue;
-              double r[3];
-              for (int a = 0; a < 3; ++a) r[a] = wrap(pos[3 * i + a] - pos[3 * j + a], L);
-              const double d2 = r[0] * r[0] + r[1] * r[1] + r[2] * r[2];
-              const double R = rad[i] + rad[j];
-              if (d2 >= R * R || d2 == 0.0) continue;
-              const double dist = std::sqrt(d2);
-              const double delta = R - dist;
-              const double s = k * delta / dist;
-              for (int a = 0; a < 3; ++a) f[a] += s * r[a];
-              e += 0.25 * k * delta * delta;  // half of the pair energy, counted from both sides
-              for (int a = 0; a < 3; ++a)
-                for (int b = 0; b < 3; ++b) lv[3 * a + b] += 0.5 * r[a] * (s * r[b]);
-            }
-          }
       for (int a = 0; a < 3; ++a) F[3 * i + a] = f[a];
       E[i] = e;
     }
@@ -130,6 +167,21 @@ void evaluate(const Cells& c, int64_t n, const double* pos, const double* rad, d
     for (int q = 0; q < 9; ++q) vir[q] += lv[q];
   }
   for (int q = 0; q < 9; ++q) W9[q] = vir[q];
+}
+
+// largest displacement since the list was built
+double max_disp2(const Verlet& v, int64_t n, const double* pos) {
+  double m = 0.0;
+#pragma omp parallel for reduction(max : m) schedule(static)
+  for (int64_t i = 0; i < n; ++i) {
+    double s = 0.0;
+    for (int a = 0; a < 3; ++a) {
+      const double t = pos[3 * i + a] - v.pos0[3 * i + a];
+      s += t * t;
+    }
+    m = std::max(m, s);
+  }
+  return m;
 }
 
 double safe_norm3(const double* x) {
@@ -149,10 +201,18 @@ extern "C" int fire64_minimize(const fire64_params* p, int64_t n, double* pos, c
   for (int64_t i = 0; i < n; ++i) rmax = std::max(rmax, rad[i]);
   std::vector<double> F(3 * n), E(n), v(3 * n, 0.0), vold(3 * n);
   Cells c;
+  Verlet vl;
   double W9[9];
+  const double skin = 0.1 * rmax;
+  bool built = false;
   auto eval = [&](double& pe, double& gmax) {
-    build_cells(c, n, pos, L, rmax);
-    evaluate(c, n, pos, rad, L, p->k, F.data(), E.data(), W9);
+    // the list holds every pair within R_ij while no sphere moved skin/2
+    if (!built || max_disp2(vl, n, pos) > 0.25 * skin * skin) {
+      build_cells(c, n, pos, L, rmax + 0.5 * skin);
+      build_verlet(vl, c, n, pos, rad, L, skin);
+      built = true;
+    }
+    evaluate(vl, n, pos, rad, L, p->k, F.data(), E.data(), W9);
     pe = 0.0;
     gmax = 0.0;
     for (int64_t i = 0; i < n; ++i) pe += E[i];

@@ -64,7 +64,7 @@ STRUCT_FIELDS = {
     "jdem_fire": ["dt", "alpha_init", "f_inc", "f_dec", "f_alpha", "n_min", "n_bad_max", "dt_max_scale",
                   "dt_min_scale"],
     "jdem_minimize_stats": ["steps", "pe", "max_grad", "power", "overflow", "reserved"],
-    "jdem_precise": ["n", "dim", "dims", "cell", "cidx", "off", "rad", "force", "k", "reserved"],
+    "jdem_precise": ["n", "dim", "dims", "cell", "cell_lo", "cidx", "off", "rad", "force", "k", "stall_steps"],
 }
 
 

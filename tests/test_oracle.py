@@ -10,6 +10,7 @@ test_rotation_integrators.py:25-97.
 from __future__ import annotations
 
 import math
+from pathlib import Path
 
 import numpy as np
 import pytest
@@ -524,3 +525,76 @@ def test_fire64_reference_matches_numpy_oracle():
     assert r["steps"] == steps and r["max_grad"] <= 1e-12
     assert r["pe"] == pytest.approx(pe, rel=1e-12)
     np.testing.assert_allclose(r["pos"], st.pos, atol=1e-9)
+
+
+# ---------------------------------------------------------------------------
+# float64 evaluation and rattler rule of the precision-form comparison
+# ---------------------------------------------------------------------------
+def _fcc(cells, a):
+    base = np.array([[0, 0, 0], [0.5, 0.5, 0], [0.5, 0, 0.5], [0, 0.5, 0.5]])
+    g = np.stack(np.meshgrid(*[np.arange(cells)] * 3, indexing="ij"), -1).reshape(-1, 1, 3)
+    return ((g + base[None]) * a).reshape(-1, 3)
+
+
+def test_evaluate64_fcc_closed_form():
+    """A compressed FCC crystal: every sphere has 12 contacts at distance
+    d = a/sqrt(2), the forces cancel by symmetry, E/N = 6 (k/2) delta^2 and
+    P = (N 12/2) d k delta / (3 V) with delta = 2r - d."""
+    from tests.precision_vs_fp64 import evaluate64, rattlers
+
+    cells, a = 4, 1.0
+    d = a / math.sqrt(2.0)
+    r = 0.5 * d * 1.01
+    pos = _fcc(cells, a) + 0.1
+    N = len(pos)
+    rad = np.full(N, r)
+    L = cells * a
+    e = evaluate64(pos, rad, L, k=2.0)
+    delta = 2 * r - d
+    assert len(e["contacts"][0]) == N * 6
+    assert e["max_grad"] < 1e-13
+    assert e["pe"] == pytest.approx(6 * 0.5 * 2.0 * delta * delta, rel=1e-12)
+    assert e["pressure"] == pytest.approx(N * 6 * d * 2.0 * delta / (3 * L**3), rel=1e-12)
+    assert not rattlers(pos, rad, L).any()
+
+
+def test_rattler_rule_iterates():
+    """A sphere touching four others that touch nothing else: the four have
+    one contact each and go first, which leaves the centre with none."""
+    from tests.precision_vs_fp64 import rattlers
+
+    L = 20.0
+    c = np.array([10.0, 10.0, 10.0])
+    dirs = np.array([[1, 1, 1], [1, -1, -1], [-1, 1, -1], [-1, -1, 1]]) / math.sqrt(3.0)
+    pos = np.vstack([c, c + 0.95 * dirs, [[2.0, 2.0, 2.0]]])
+    rad = np.full(len(pos), 0.5)
+    assert rattlers(pos, rad, L).all()
+
+
+def test_jamming_fixture_is_a_float64_minimum():
+    """The fixture's float64 minimum, re-evaluated independently in numpy:
+    max|F| < 1e-12, and E/N and P as the C++ FIRE reported them."""
+    from tests.precision_vs_fp64 import evaluate64, rattlers
+
+    fx = np.load(Path(__file__).parent / "golden" / "jamming_n1024.npz")
+    e = evaluate64(fx["ref"], fx["rad"], float(fx["L"]))
+    assert e["max_grad"] < 1e-12
+    assert e["pe"] == pytest.approx(float(fx["ref_pe"]), rel=1e-12)
+    assert e["pressure"] == pytest.approx(float(fx["ref_pressure"]), rel=1e-12)
+    assert 0 < rattlers(fx["ref"], fx["rad"], float(fx["L"])).sum() < 100
+
+
+def test_fp64_minimisation_is_chaotic_near_jamming():
+    """The reason positions are compared only at the minimum: from the
+    fixture's start (float64-relaxed to max|F| <= 1e-4), a 1e-9 perturbation
+    sends the float64 FIRE to a different minimum, several percent away in
+    energy."""
+    from tests.precision_vs_fp64 import evaluate64, fp64_run
+
+    fx = np.load(Path(__file__).parent / "golden" / "jamming_n1024.npz")
+    rad, L = fx["rad"], float(fx["L"])
+    rng = np.random.default_rng(5)
+    r = fp64_run(fx["start"] + 1e-9 * rng.standard_normal(fx["start"].shape), rad, L, max_steps=200000)
+    assert r["max_grad"] < 1e-12
+    e = evaluate64(r["pos"], rad, L)
+    assert abs(e["pe"] - float(fx["ref_pe"])) > 0.02 * float(fx["ref_pe"])
