@@ -377,9 +377,10 @@ typedef struct jdem_precise {
   const float* rad; /* (N,) device                                     */
   float* force;     /* (N, D) device, out: forces of the last evaluation */
   float k;          /* spring constant                                 */
-  int32_t stall_steps; /* stop once the largest force component has not
+  int32_t stall_steps; /* stop once neither the largest force component
+                          nor the energy (by more than 1e-13 of it) has
                           reached a new low for this many steps (0: never);
-                          float32 pair arithmetic has a residual floor    */
+                          float32 positions have a residual floor         */
 } jdem_precise;
 
 /* minimize (minimizers/routines.py:150-383) with FIRE on a jdem_precise

@@ -1492,13 +1492,15 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
   int64_t steps = 0;
   const double tiny = 2.2250738585072014e-308;
   const double dN = (double)std::max<int64_t>(N, 1);
-  double best = gmax;
+  // progress: a new low of max|F|, or of the energy by more than 1e-13 of it
+  double best_g = gmax, best_e = pe;
   int64_t best_at = 0;
   while (steps < max_steps) {
     const bool conv_pe = std::fabs(pe / dN) <= pe_tol;
     const double denom = std::max(std::max(std::fabs(pe), std::fabs(prev)), tiny);
     const bool conv_rel = std::fabs(pe - prev) / denom < pe_diff_tol;
-    if (gmax < best) best = gmax, best_at = steps;
+    if (gmax < best_g) best_g = gmax, best_at = steps;
+    if (pe < best_e - 1e-13 * std::fabs(best_e)) best_e = pe, best_at = steps;
     const bool stalled = p->stall_steps > 0 && steps - best_at >= p->stall_steps;
     if (conv_pe || conv_rel || gmax <= force_tol || stalled) break;
     HIPCHK(ctx, launch_fire_power(f, st));

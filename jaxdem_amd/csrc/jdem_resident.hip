@@ -694,14 +694,37 @@ JD Vec<D> disp_pm(Vec<D> a, Vec<D> b, const DomP& d) {
   return disp_<PM == 1>(a, b, d);
 }
 
-template <int D, int LAW, int PM>
+// LDS of the lane-balanced force pass: per wave, kPairCap pair descriptors
+// (a wave with more marked slots takes them in several chunks) and one round
+// of 64 pair results (force + contact flag, torque, spring).
+constexpr int kPairCap = 128;
+// pass 1 keeps the partner index of a lane's first kStash marked slots in LDS,
+// so the pair evaluation need not reload it from the list
+constexpr int kStash = 4;
+
+// orders the LDS traffic of one wave around the exchange of pair results
+JD void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// MW = 64-slot words of a row the kernel handles (1 when M <= 64, else 4):
+// the per-lane slot masks then take 2 * MW registers
+template <int D, int LAW, int PM, int MW>
 __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_waves<D, LAW, PM>(), 8))) k_ws_step(WS a) {
+  __shared__ int pdesc[kBlock / 64][kPairCap];
+  __shared__ int pstash[kBlock / 64][kStash][64];
+  __shared__ float4 pres[kBlock / 64][3][64];
   if (a.flag_rd && *a.flag_rd) {
     if (a.flag_wr) *a.flag_wr = *a.flag_rd;
     return;
   }
-  const int64_t b = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
-  if (b >= a.n) return;
+  // every lane of a wave stays to the end of the force pass, which works
+  // wave-wide; a lane past the end reads the last particle and writes nothing
+  const int64_t b0 = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
+  const bool active = b0 < a.n;
+  const int64_t b = active ? b0 : a.n - 1;
   DomP dom;
   float dt, trigger;
   if (PM == 2) {
@@ -714,15 +737,13 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
     dt = a.dt;
     trigger = a.trigger;
   }
-  const float4 p4 = a.P[b], v4 = a.V[b], w4 = a.W[b], i4 = a.I[b];
+  // the force pass needs only this particle's position and radius; the
+  // integration reloads them with the velocities and inertia after it,
+  // which keeps them out of the registers the pair evaluation needs
+  const float4 p4 = a.P[b];
   Side<D> me;
   me.x = xyz<D>(p4);
   me.r = p4.w;
-  me.v = xyz<D>(v4);
-  me.m = v4.w;
-  me.w = ang4<D>(w4);
-  const int bits = __float_as_int(i4.w);
-  me.mat = bits >> 1;
   Vec<D> F;
   Ang<D> T;
   if (a.do_force) {
@@ -734,7 +755,11 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
     // runs the law as often as its busiest lane has contacts.  Pass 1 loads
     // PF list entries, then their positions, back to back before using any.
     constexpr int PF = 4;
-    uint64_t cm0 = 0ull, cm1 = 0ull, cm2 = 0ull, cm3 = 0ull;
+    uint64_t cm[MW];
+#pragma unroll
+    for (int q = 0; q < MW; ++q) cm[q] = 0ull;
+    int n_marked = 0;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     bool done = false;
     for (int s0 = 0; s0 < a.M && !done; s0 += PF) {
       int32_t js[PF];
@@ -757,69 +782,155 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
         const float d_sq = dot(rij, rij);
         const float R = me.r + pj.w;
         if (d_sq > (R * R) * 1.0001f) continue;
+        if (n_marked < kStash) pstash[wv][n_marked][lane] = js[k];
+        ++n_marked;
         const int s = s0 + k;
         const uint64_t bit = 1ull << (s & 63);
         const int ws = s >> 6;
-        cm0 |= ws == 0 ? bit : 0ull;
-        cm1 |= ws == 1 ? bit : 0ull;
-        cm2 |= ws == 2 ? bit : 0ull;
-        cm3 |= ws == 3 ? bit : 0ull;
+#pragma unroll
+        for (int q = 0; q < MW; ++q) cm[q] |= ws == q ? bit : 0ull;
       }
     }
     const int n_words = LawTraits<LAW>::hist ? a.W64 : (a.M + 63) >> 6;
-    int rank_old_base = 0, rank_new = 0;  // springs are stored by rank
-    for (int w = 0; w < n_words; ++w) {
-      uint64_t cm = w == 0 ? cm0 : (w == 1 ? cm1 : (w == 2 ? cm2 : cm3));
-      uint64_t live_old = 0ull, live_new = 0ull;
-      if constexpr (LawTraits<LAW>::hist) live_old = a.live[(int64_t)w * a.n + b];
-      const int rank_base = rank_old_base;
-      if constexpr (LawTraits<LAW>::hist) rank_old_base += __popcll(live_old);
-      while (cm) {
-        const int s = w * 64 + (__ffsll((unsigned long long)cm) - 1);
-        cm &= cm - 1ull;
-        const int32_t j = a.nl[(int64_t)s * a.n + b];
-        const float4 pj = a.P[j];
-        const Vec<D> rij = disp_pm<PM>(me.x, xyz<D>(pj), dom);
-        const float d_sq = dot(rij, rij);
-        Side<D> o;
-        o.x = xyz<D>(pj);
-        o.r = pj.w;
-        o.mat = (a.law.n_mat > 1) ? (__float_as_int(a.I[j].w) >> 1) : 0;
-        if constexpr (LawTraits<LAW>::kin) {
-          const float4 vj = a.V[j];
-          o.v = xyz<D>(vj);
-          o.m = vj.w;
-          o.w = ang4<D>(a.W[j]);
-        }
-        Vec<D> xold = zero_vec<D>();
-        if constexpr (LawTraits<LAW>::hist) {
-          if ((live_old >> (s & 63)) & 1ull) {
-            const int k = rank_base + __popcll(live_old & ((1ull << (s & 63)) - 1ull));
-            xold = xyz<D>(a.xi_rd[(int64_t)k * a.n + b]);
+    int nc = 0;
+#pragma unroll
+    for (int q = 0; q < MW; ++q) {
+      if (!active) cm[q] = 0ull;
+      nc += __popcll(cm[q]);
+    }
+    // Pass 2, lane-balanced: the wave's marked slots are numbered lane by
+    // lane (slot order within a lane) and dealt out 64 at a time, one pair
+    // per lane, so the law runs ceil(total / 64) times per wave instead of
+    // as often as the busiest lane has contacts.  Each owner then adds its
+    // pairs' results in slot order: the sums, springs and live bits are
+    // those of one lane walking its own row.
+    int incl = nc;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const int t = __shfl_up(incl, d);
+      if (lane >= d) incl += t;
+    }
+    const int total = __shfl(incl, 63);
+    const int off = incl - nc;
+    uint64_t ln[MW];
+#pragma unroll
+    for (int q = 0; q < MW; ++q) ln[q] = 0ull;
+    int rank_new = 0;  // springs are stored by rank
+    for (int cb = 0; cb < total; cb += kPairCap) {  // wave-uniform
+      // descriptors of pairs cb .. cb + kPairCap - 1: owner lane | slot << 6
+      // | (old spring rank + 1) << 14 | (mark number, kStash if not stashed) << 23
+      if (off < cb + kPairCap && off + nc > cb) {
+        int idx = off, rank_base = 0, c = 0;
+#pragma unroll
+        for (int w = 0; w < MW; ++w) {
+          uint64_t cmw = w < n_words ? cm[w] : 0ull;
+          uint64_t live_old = 0ull;
+          if constexpr (LawTraits<LAW>::hist)
+            if (w < n_words) live_old = a.live[(int64_t)w * a.n + b];
+#pragma unroll 1
+          while (cmw) {
+            const int sl = __ffsll((unsigned long long)cmw) - 1;
+            cmw &= cmw - 1ull;
+            int k = -1;
+            if constexpr (LawTraits<LAW>::hist)
+              if ((live_old >> sl) & 1ull) k = rank_base + __popcll(live_old & ((1ull << sl) - 1ull));
+            if (idx >= cb && idx < cb + kPairCap)
+              pdesc[wv][idx - cb] = lane | ((w * 64 + sl) << 6) | ((k + 1) << 14) | (min(c, kStash) << 23);
+            ++idx;
+            ++c;
           }
+          if constexpr (LawTraits<LAW>::hist) rank_base += __popcll(live_old);
         }
-        PairOut<D> r = law_eval<D, LAW>(a.law, me, o, rij, d_sq, xold, dt);
-        F = F + r.F;
-        if constexpr (LawTraits<LAW>::torque) T = T + r.T;
-        if constexpr (LawTraits<LAW>::hist) {
-          if (r.contact) {
-            a.xi_wr[(int64_t)rank_new * a.n + b] = f4(r.xi, 0.0f);
-            ++rank_new;
-            live_new |= 1ull << (s & 63);
+      }
+      const int tc = min(total - cb, kPairCap);
+      for (int r0 = 0; r0 < tc; r0 += 64) {
+        wave_sync();
+        if (r0 + lane < tc) {
+          const int dsc = pdesc[wv][r0 + lane];
+          const int64_t bo = b0 - lane + (dsc & 63);
+          const int k = ((dsc >> 14) & 511) - 1, c = dsc >> 23;
+          DomP dom_o;
+          float dt_o;
+          if (PM == 2) {
+            const SegP& sp = a.segp[a.seg[bo]];
+            dom_o = sp.dom;
+            dt_o = sp.dt;
+          } else {
+            dom_o = dom;
+            dt_o = dt;
+          }
+          const float4 po = a.P[bo];
+          Side<D> mo;
+          mo.x = xyz<D>(po);
+          mo.r = po.w;
+          mo.mat = (a.law.n_mat > 1) ? (__float_as_int(a.I[bo].w) >> 1) : 0;
+          // the LDS read is unconditional, so the two sources stay two loads
+          // (not one flat load through a selected pointer)
+          int32_t j = pstash[wv][min(c, kStash - 1)][dsc & 63];
+          if (c >= kStash) j = a.nl[(int64_t)((dsc >> 6) & 255) * a.n + bo];
+          const float4 pj = a.P[j];
+          Side<D> o;
+          o.x = xyz<D>(pj);
+          o.r = pj.w;
+          o.mat = (a.law.n_mat > 1) ? (__float_as_int(a.I[j].w) >> 1) : 0;
+          if constexpr (LawTraits<LAW>::kin) {
+            const float4 vo = a.V[bo], vj = a.V[j];
+            mo.v = xyz<D>(vo);
+            mo.m = vo.w;
+            mo.w = ang4<D>(a.W[bo]);
+            o.v = xyz<D>(vj);
+            o.m = vj.w;
+            o.w = ang4<D>(a.W[j]);
+          }
+          Vec<D> xold = zero_vec<D>();
+          if constexpr (LawTraits<LAW>::hist)
+            if (k >= 0) xold = xyz<D>(a.xi_rd[(int64_t)k * a.n + bo]);
+          const Vec<D> rij = disp_pm<PM>(mo.x, o.x, dom_o);
+          const float d_sq = dot(rij, rij);
+          const PairOut<D> r = law_eval<D, LAW>(a.law, mo, o, rij, d_sq, xold, dt_o);
+          pres[wv][0][lane] = f4(r.F, r.contact ? 1.0f : 0.0f);
+          if constexpr (LawTraits<LAW>::torque) pres[wv][1][lane] = f4a(r.T);
+          if constexpr (LawTraits<LAW>::hist) pres[wv][2][lane] = f4(r.xi, 0.0f);
+        }
+        wave_sync();
+        const int q0 = cb + r0, q1 = min(off + nc, q0 + 64);
+#pragma unroll 1
+        for (int q = max(off, q0); q < q1; ++q) {
+          const float4 fr = pres[wv][0][q - q0];
+          F = F + xyz<D>(fr);
+          if constexpr (LawTraits<LAW>::torque) T = T + ang4<D>(pres[wv][1][q - q0]);
+          if constexpr (LawTraits<LAW>::hist) {
+            if (fr.w != 0.0f) {
+              const int sl = (pdesc[wv][q - cb] >> 6) & 255;
+              a.xi_wr[(int64_t)rank_new * a.n + b] = pres[wv][2][q - q0];
+              ++rank_new;
+              const uint64_t bit = 1ull << (sl & 63);
+              const int ws = sl >> 6;
+#pragma unroll
+              for (int qq = 0; qq < MW; ++qq) ln[qq] |= ws == qq ? bit : 0ull;
+            }
           }
         }
       }
-      if constexpr (LawTraits<LAW>::hist) a.live[(int64_t)w * a.n + b] = live_new;
+      wave_sync();  // the next chunk's descriptors overwrite these
     }
+    if constexpr (LawTraits<LAW>::hist)
+      if (active)
+#pragma unroll
+        for (int w = 0; w < MW; ++w)
+          if (w < n_words) a.live[(int64_t)w * a.n + b] = ln[w];
   } else {
     F = xyz<D>(a.F[b]);
     T = ang4<D>(a.T[b]);
   }
-  Vec<D> x = me.x, v = me.v;
-  Ang<D> w = me.w;
+  if (!active) return;
+  const float4 p4b = a.P[b], v4 = a.V[b], w4 = a.W[b], i4 = a.I[b];
+  const int bits = __float_as_int(i4.w);
+  Vec<D> x = xyz<D>(p4b), v = xyz<D>(v4);
+  Ang<D> w = ang4<D>(w4);
   Quat q = q4(a.Q[b]);
   const Ang<D> I = ang4<D>(i4);
-  const float m = me.m;
+  const float m = v4.w;
   const bool fixed = (bits & 1) != 0;
   const float freef = fixed ? 0.0f : 1.0f;
   if (a.do_after) {
@@ -850,7 +961,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
       spiral_after<D>(q, w, T, I, dt, freef);
   }
   if (a.do_before) {
-    if (dom.kind == JDEM_DOMAIN_REFLECT) reflect_apply<D>(dom, x, v, q, F, m, me.r, fixed, dt);
+    if (dom.kind == JDEM_DOMAIN_REFLECT) reflect_apply<D>(dom, x, v, q, F, m, p4b.w, fixed, dt);
     if (a.lin == JDEM_LIN_VERLET) {
       const float s = (dt * 0.5f) / m;
       v = v + (F * s) * freef;
@@ -858,7 +969,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
     }
     if (a.rot == JDEM_ROT_VERLET_SPIRAL) vvs_before<D>(q, w, T, I, dt, freef);
   }
-  a.P2[b] = f4(x, me.r);
+  a.P2[b] = f4(x, p4b.w);
   a.V2[b] = f4(v, m);
   a.W2[b] = f4a(w);
   a.Q[b] = f4q(q);
@@ -875,15 +986,23 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
   }
 }
 
-template <int D, int LAW>
-hipError_t launch_ws_law(const WS& a, bool per, hipStream_t s) {
+template <int D, int LAW, int MW>
+void launch_ws_mw(const WS& a, bool per, hipStream_t s) {
   dim3 g(nblocks(a.n)), b(kBlock);
   if (a.segp)
-    hipLaunchKernelGGL((k_ws_step<D, LAW, 2>), g, b, 0, s, a);
+    hipLaunchKernelGGL((k_ws_step<D, LAW, 2, MW>), g, b, 0, s, a);
   else if (per)
-    hipLaunchKernelGGL((k_ws_step<D, LAW, 1>), g, b, 0, s, a);
+    hipLaunchKernelGGL((k_ws_step<D, LAW, 1, MW>), g, b, 0, s, a);
   else
-    hipLaunchKernelGGL((k_ws_step<D, LAW, 0>), g, b, 0, s, a);
+    hipLaunchKernelGGL((k_ws_step<D, LAW, 0, MW>), g, b, 0, s, a);
+}
+template <int D, int LAW>
+hipError_t launch_ws_law(const WS& a, bool per, hipStream_t s) {
+  const int words = max(a.W64, (a.M + 63) >> 6);
+  if (words <= 1)
+    launch_ws_mw<D, LAW, 1>(a, per, s);
+  else
+    launch_ws_mw<D, LAW, 4>(a, per, s);
   return hipGetLastError();
 }
 

@@ -74,7 +74,8 @@ def minimize_precise(pos, rad, box, k: float = 1.0, fire_cfg: FireConfig | None 
 
     The loop stops on the reference's three tests (pe_tol, pe_diff_tol,
     force_tol) or at ``max_steps``; with ``stall_steps`` > 0 it also stops
-    once max |F| has not reached a new low for that many steps.  The float32
+    once neither max |F| nor the energy (by more than 1e-13 of it) has
+    reached a new low for that many steps.  The float32
     pair arithmetic and the float32 offsets leave a residual floor near
     1e-7 k h, below which force_tol cannot be met."""
     pos = np.asarray(pos, np.float64)

@@ -530,7 +530,7 @@ def test_precise_fire_against_fp64(device):
     P 2e-5 relative; the run's own float64 reductions of float32 pair terms
     E/N and P 2e-5 (the float32 radii alone shift E/N by 4.4e-6 and P by
     3.3e-6); non-rattler positions 1e-5 after the mean shift; residual floor
-    (no new low of max|F| for 5000 steps) below 1e-5."""
+    (no new low of max|F| or of the energy for 5000 steps) below 1e-5."""
     from tests import precision_vs_fp64 as PV
 
     fx = np.load(Path(__file__).parent / "golden" / "jamming_n1024.npz")
