@@ -678,12 +678,15 @@ __global__ void __launch_bounds__(kBlock) k_iota(int32_t* __restrict__ p, int64_
 // Minimum waves per SIMD asked of the compiler: six (80 VGPRs) holds the
 // spring-friction law and the integrators; the Cundall-Strack and
 // Hertz-friction laws in 3D need five, and one fewer in a segmented batch,
-// whose per-system domain takes registers too.
-template <int D, int LAW, int PM>
+// whose per-system domain takes registers too, or with rows of more than 64
+// slots, whose slot masks take four words.
+template <int D, int LAW, int PM, int MW>
 constexpr int ws_waves() {
   const bool heavy = D == 3 && (LAW == JDEM_LAW_CUNDALL_STRACK || LAW == JDEM_LAW_HERTZ_FRICTION);
   const bool seg3 = D == 3 && PM == 2 && LAW != JDEM_LAW_SPRING && LAW != JDEM_LAW_HERTZ;
-  return heavy ? (PM == 2 ? 4 : 5) : (seg3 ? 5 : 6);
+  const bool wide = MW > 1 && LawTraits<LAW>::hist;
+  const int w = heavy ? (PM == 2 ? 4 : 5) : (seg3 ? 5 : 6);
+  return wide ? w - 1 : w;
 }
 
 // Domain._displacement for mode PM: 0 / 1 = one non-periodic / periodic
@@ -712,8 +715,9 @@ JD void wave_sync() {
 // MW = 64-slot words of a row the kernel handles (1 when M <= 64, else 4):
 // the per-lane slot masks then take 2 * MW registers
 template <int D, int LAW, int PM, int MW>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_waves<D, LAW, PM>(), 8))) k_ws_step(WS a) {
+__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_waves<D, LAW, PM, MW>(), 8))) k_ws_step(WS a) {
   __shared__ int pdesc[kBlock / 64][kPairCap];
+  __shared__ int pjx[kBlock / 64][kPairCap];
   __shared__ int pstash[kBlock / 64][kStash][64];
   __shared__ float4 pres[kBlock / 64][3][64];
   if (a.flag_rd && *a.flag_rd) {
@@ -818,7 +822,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
     int rank_new = 0;  // springs are stored by rank
     for (int cb = 0; cb < total; cb += kPairCap) {  // wave-uniform
       // descriptors of pairs cb .. cb + kPairCap - 1: owner lane | slot << 6
-      // | (old spring rank + 1) << 14 | (mark number, kStash if not stashed) << 23
+      // | (old spring rank + 1) << 14, and the partner index
       if (off < cb + kPairCap && off + nc > cb) {
         int idx = off, rank_base = 0, c = 0;
 #pragma unroll
@@ -834,8 +838,14 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
             int k = -1;
             if constexpr (LawTraits<LAW>::hist)
               if ((live_old >> sl) & 1ull) k = rank_base + __popcll(live_old & ((1ull << sl) - 1ull));
-            if (idx >= cb && idx < cb + kPairCap)
-              pdesc[wv][idx - cb] = lane | ((w * 64 + sl) << 6) | ((k + 1) << 14) | (min(c, kStash) << 23);
+            if (idx >= cb && idx < cb + kPairCap) {
+              pdesc[wv][idx - cb] = lane | ((w * 64 + sl) << 6) | ((k + 1) << 14);
+              // the LDS read is unconditional, so the two sources stay two
+              // loads (not one flat load through a selected pointer)
+              int jj = pstash[wv][min(c, kStash - 1)][lane];
+              if (c >= kStash) jj = a.nl[(int64_t)(w * 64 + sl) * a.n + b];
+              pjx[wv][idx - cb] = jj;
+            }
             ++idx;
             ++c;
           }
@@ -847,8 +857,19 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
         wave_sync();
         if (r0 + lane < tc) {
           const int dsc = pdesc[wv][r0 + lane];
+          const int32_t j = pjx[wv][r0 + lane];
           const int64_t bo = b0 - lane + (dsc & 63);
-          const int k = ((dsc >> 14) & 511) - 1, c = dsc >> 23;
+          const int k = (dsc >> 14) - 1;
+          // every load of the pair is issued before any is used
+          const float4 po = a.P[bo], pj = a.P[j];
+          float4 vo, vj, wo, wj, xo;
+          if constexpr (LawTraits<LAW>::kin) {
+            vo = a.V[bo];
+            vj = a.V[j];
+            wo = a.W[bo];
+            wj = a.W[j];
+          }
+          if constexpr (LawTraits<LAW>::hist) xo = a.xi_rd[(int64_t)(k >= 0 ? k : 0) * a.n + bo];
           DomP dom_o;
           float dt_o;
           if (PM == 2) {
@@ -859,32 +880,28 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
             dom_o = dom;
             dt_o = dt;
           }
-          const float4 po = a.P[bo];
-          Side<D> mo;
+          Side<D> mo, o;
           mo.x = xyz<D>(po);
           mo.r = po.w;
-          mo.mat = (a.law.n_mat > 1) ? (__float_as_int(a.I[bo].w) >> 1) : 0;
-          // the LDS read is unconditional, so the two sources stay two loads
-          // (not one flat load through a selected pointer)
-          int32_t j = pstash[wv][min(c, kStash - 1)][dsc & 63];
-          if (c >= kStash) j = a.nl[(int64_t)((dsc >> 6) & 255) * a.n + bo];
-          const float4 pj = a.P[j];
-          Side<D> o;
           o.x = xyz<D>(pj);
           o.r = pj.w;
-          o.mat = (a.law.n_mat > 1) ? (__float_as_int(a.I[j].w) >> 1) : 0;
+          mo.mat = 0;
+          o.mat = 0;
+          if (a.law.n_mat > 1) {
+            mo.mat = __float_as_int(a.I[bo].w) >> 1;
+            o.mat = __float_as_int(a.I[j].w) >> 1;
+          }
           if constexpr (LawTraits<LAW>::kin) {
-            const float4 vo = a.V[bo], vj = a.V[j];
             mo.v = xyz<D>(vo);
             mo.m = vo.w;
-            mo.w = ang4<D>(a.W[bo]);
+            mo.w = ang4<D>(wo);
             o.v = xyz<D>(vj);
             o.m = vj.w;
-            o.w = ang4<D>(a.W[j]);
+            o.w = ang4<D>(wj);
           }
           Vec<D> xold = zero_vec<D>();
           if constexpr (LawTraits<LAW>::hist)
-            if (k >= 0) xold = xyz<D>(a.xi_rd[(int64_t)k * a.n + bo]);
+            if (k >= 0) xold = xyz<D>(xo);
           const Vec<D> rij = disp_pm<PM>(mo.x, o.x, dom_o);
           const float d_sq = dot(rij, rij);
           const PairOut<D> r = law_eval<D, LAW>(a.law, mo, o, rij, d_sq, xold, dt_o);
