@@ -50,6 +50,10 @@ def parse_args():
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=16.0, help="Poisson arrival rate, requests per second")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--precision-n", type=int, default=1 << 18, help="spheres of the precision-form minimisation")
+    ap.add_argument("--precision-steps", type=int, default=20000,
+                    help="FIRE steps timed (0: run to the float32 floor, which stalls after ~1e5 steps)")
+    ap.add_argument("--no-precision", action="store_true")
     ap.add_argument(
         "--copy-probe-mib",
         type=int,
@@ -205,6 +209,44 @@ def serving_bench(args, dev) -> dict:
     }
 
 
+def precision_bench(args, dev) -> dict:
+    """Precision form: FIRE energy minimisation of the 3D bidisperse jamming
+    start (radii 0.5 / 0.7, phi 0.66, harmonic k = 1, dt0 = 1e-2) with
+    positions as integer cell + float32 offset and float64 energy, virial,
+    power and residual, for --precision-steps steps from the random start."""
+    import torch
+
+    from jaxdem_amd import precision as PR
+    from jaxdem_amd import workloads as W
+
+    pos, rad, L = W.bidisperse_jamming(args.precision_n, seed=args.seed, phi=0.66)
+    steps = args.precision_steps if args.precision_steps > 0 else 400000
+    stall = 0 if args.precision_steps > 0 else 20000
+    # warm-up: a few steps, so the timed run does not include the first allocations
+    PR.minimize_precise(pos, rad, L, k=1.0, dt=1e-2, max_steps=10, pe_tol=0.0, pe_diff_tol=-1.0, force_tol=0.0,
+                        device=dev)
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    r = PR.minimize_precise(pos, rad, L, k=1.0, dt=1e-2, max_steps=steps, pe_tol=0.0, pe_diff_tol=-1.0,
+                            force_tol=1e-12, stall_steps=stall, device=dev)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t
+    print(f"[bench] precision: {r['steps']} steps in {secs:.2f} s, max|F| {r['max_grad']:.3e}", file=sys.stderr,
+          flush=True)
+    return {
+        "workload": f"bidisperse_jamming: {args.precision_n} spheres (r 0.5/0.7, 50:50), periodic cube L = {L:.2f} "
+        "at phi 0.66, harmonic k = 1, FIRE dt0 = 1e-2, integer cell + float32 offset positions, float64 "
+        "energy / virial / power / residual",
+        "steps": r["steps"],
+        "seconds": secs,
+        "ms_per_step": secs * 1e3 / max(r["steps"], 1),
+        "particle_steps_per_s": args.precision_n * r["steps"] / secs,
+        "max_grad": r["max_grad"],
+        "pe_per_particle": r["pe"],
+        "pressure": r["pressure"],
+    }
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -314,6 +356,9 @@ def main():
         torch.cuda.empty_cache()
     if not args.no_serve:
         out["serving"] = serving_bench(args, dev)
+        torch.cuda.empty_cache()
+    if not args.no_precision:
+        out["precision"] = precision_bench(args, dev)
     print(json.dumps(out), flush=True)
 
 
