@@ -50,6 +50,9 @@ def parse_args():
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=16.0, help="Poisson arrival rate, requests per second")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-loads", type=str, default="",
+                    help="also sweep offered loads, as fractions of the one-at-a-time capacity (e.g. 0.3,0.6,0.9)")
+    ap.add_argument("--serve-sweep-requests", type=int, default=24)
     ap.add_argument("--precision-n", type=int, default=1 << 18, help="spheres of the precision-form minimisation")
     ap.add_argument("--precision-steps", type=int, default=20000,
                     help="FIRE steps timed (0: run to the float32 floor, which stalls after ~1e5 steps)")
@@ -74,6 +77,11 @@ def live_contacts(sysm) -> int:
     for b in range(32):
         cnt += (x >> b) & 1
     return int(cnt.sum().item())
+
+
+def listed_pairs(sysm) -> int:
+    """Entries of the neighbour list (i-side candidates a step tests)."""
+    return int((sysm.collider.neighbor_list >= 0).sum().item())
 
 
 def traffic_from_profiles() -> tuple:
@@ -198,7 +206,7 @@ def serving_bench(args, dev) -> dict:
     print(f"[bench] serving batched: {batched}", file=sys.stderr, flush=True)
     plain = SV.latency_stats(SV.run_one_at_a_time(reqs, dev, realtime=True))
     print(f"[bench] serving one at a time: {plain}", file=sys.stderr, flush=True)
-    return {
+    out = {
         "workload": f"serving_requests: {len(reqs)} Poisson arrivals at {args.serve_rate}/s, N log-uniform in "
         "[1e3, 2e5], steps U[100, 10000], log-normal radii (mean 0.5, sigma 0.2), periodic/reflect, "
         "phi U[0.3, 0.55], spring + friction mu U[0, 0.5]",
@@ -207,6 +215,27 @@ def serving_bench(args, dev) -> dict:
         "throughput_ratio": batched["particle_steps_per_s"] / plain["particle_steps_per_s"],
         "p99_ratio": plain["p99_latency_s"] / batched["p99_latency_s"],
     }
+    if args.serve_loads:
+        # offered load = arrival rate x mean work per request / one-at-a-time throughput
+        mean_work = work / len(reqs)
+        sweep = []
+        for load in [float(x) for x in args.serve_loads.split(",") if x]:
+            rate = load * plain["particle_steps_per_s"] / mean_work
+            spec = W.serving_requests(args.serve_sweep_requests, rate=rate, seed=args.seed + 1)
+            rs = []
+            for k, (t, steps, w) in enumerate(spec):
+                st, sy = W.make_gpu_system(w, device=dev)
+                rs.append(SV.Request(k, st, sy, steps, arrival=t))
+            torch.cuda.synchronize()
+            b = SV.latency_stats(SV.Server(dev, chunk=50).run(rs, realtime=True))
+            p = SV.latency_stats(SV.run_one_at_a_time(rs, dev, realtime=True))
+            print(f"[bench] serving load {load}: rate {rate:.2f}/s batched {b} one at a time {p}", file=sys.stderr,
+                  flush=True)
+            sweep.append({"offered_load": load, "rate_per_s": rate, "continuous_batching": b, "one_at_a_time": p})
+            del rs
+            torch.cuda.empty_cache()
+        out["load_sweep"] = sweep
+    return out
 
 
 def precision_bench(args, dev) -> dict:
@@ -344,6 +373,7 @@ def main():
             "host_syncs": int(stats.host_syncs) if stats is not None else None,
         },
         "contact_evals_per_s": live_contacts(sy1) * args.steps / secs,
+        "candidate_tests_per_s": listed_pairs(sy1) * args.steps / secs,
         "traffic_source": traffic_src,
     }
     if not args.no_cpu_baseline:

@@ -6,8 +6,12 @@ Every active request is one system of a segmented batch
 history stay in device memory between step chunks.  At each chunk boundary
 finished requests leave and arrived requests are admitted first come, first
 served, while their device footprint fits the memory budget.  A chunk runs
-until the next request finishes (at most ``chunk`` steps), so no request runs
-past its own step count.  Because a system advances in a batch exactly as it
+until the next request finishes, so no request runs past its own step
+count, and for about ``chunk_ms`` of GPU time (estimated from the previous
+chunks' time per step; ``chunk`` steps for the first): arrivals wait at most
+one chunk, while the per-chunk costs (packing the systems' arrays, the
+entry and exit conversions of the batched step, a host synchronisation) stay
+a small fraction of each chunk.  Because a system advances in a batch exactly as it
 does alone, each result is bit-identical to running the request by itself.
 
     server = Server(device, hbm_budget=...)
@@ -66,13 +70,14 @@ class Server:
     """First-come-first-served continuous batching of rollout requests."""
 
     def __init__(self, device, hbm_budget: int | None = None, chunk: int = 50, max_active: int = 4096,
-                 clock: Callable[[], float] = time.perf_counter):
+                 clock: Callable[[], float] = time.perf_counter, chunk_ms: float = 20.0):
         self.device = torch.device(device)
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info(self.device)
             hbm_budget = int(0.8 * free)
         self.hbm_budget = int(hbm_budget)
         self.chunk = int(chunk)
+        self.chunk_ms = float(chunk_ms)
         self.max_active = int(max_active)
         self.clock = clock
         self.log: list = []
@@ -90,6 +95,7 @@ class Server:
         now = lambda: self.clock() - t0  # noqa: E731
         used = 0
         done: list = []
+        ms_per_step = None  # of the current composition, from the last chunk
         while pending or queue or active:
             t = now()
             while pending and (not realtime or pending[0].arrival <= t):
@@ -108,13 +114,17 @@ class Server:
                 pairs = (batch.unpack() if batch is not None else []) + [(r.state, r.system) for r in admit]
                 active = active + admit
                 batch = Batch.pack(pairs)
+                ms_per_step = None
             if not active:
                 if pending and realtime:
                     time.sleep(max(0.0, min(pending[0].arrival - now(), 0.01)))
                 continue
-            n = min(self.chunk, min(r.steps - r.done_steps for r in active))
+            want = self.chunk if ms_per_step is None else max(1, int(self.chunk_ms / max(ms_per_step, 1e-6)))
+            n = min(want, min(r.steps - r.done_steps for r in active))
+            t_c = self.clock()
             batch = batch.step(n)
             torch.cuda.synchronize(self.device)
+            ms_per_step = (self.clock() - t_c) * 1e3 / n
             for r in active:
                 r.done_steps += n
             self.log.append((now(), len(active), int(batch.n_particles), n))
@@ -132,6 +142,7 @@ class Server:
                         keep.append((r, pair))
                 active = [r for r, _ in keep]
                 batch = Batch.pack([p for _, p in keep]) if keep else None
+                ms_per_step = None
         return done
 
 
