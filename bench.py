@@ -47,6 +47,10 @@ def parse_args():
     ap.add_argument("--batch-steps", type=int, default=2000)
     ap.add_argument("--plain-sample", type=int, default=16, help="systems stepped one at a time for the comparison")
     ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--batch-skins", type=str, default="0.1,0.2,0.3,0.45,0.6",
+                    help="candidate skins of the batch autotuner (empty: keep the workload's 0.1)")
+    ap.add_argument("--batch-probe-every", type=int, default=4, help="the autotuner probes every k-th system")
+    ap.add_argument("--batch-probe-steps", type=int, default=200)
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=16.0, help="Poisson arrival rate, requests per second")
     ap.add_argument("--no-serve", action="store_true")
@@ -134,8 +138,19 @@ def batch_bench(args, dev) -> dict:
     import jaxdem_amd as jd
     from jaxdem_amd import workloads as W
 
+    skin, tune = 0.1, []
+    if args.batch_skins:
+        from jaxdem_amd.batching import autotune_skin
+
+        def sample(sk):
+            return [W.make_gpu_system(w, device="cpu")
+                    for w in W.batch_disks(args.batch_systems, seed=args.seed, skin=sk, every=args.batch_probe_every)]
+
+        skin, tune = autotune_skin(sample, [float(x) for x in args.batch_skins.split(",") if x],
+                                   probe_steps=args.batch_probe_steps, device=dev)
+        print(f"[bench] batch autotuner: skin {skin} from {tune}", file=sys.stderr, flush=True)
     t0 = time.perf_counter()
-    ws = W.batch_disks(args.batch_systems, seed=args.seed)
+    ws = W.batch_disks(args.batch_systems, seed=args.seed, skin=skin)
     pairs = [W.make_gpu_system(w, device="cpu") for w in ws]
     b = jd.Batch.pack(pairs).to(dev)
     del pairs
@@ -168,7 +183,8 @@ def batch_bench(args, dev) -> dict:
           flush=True)
     return {
         "workload": f"batch_disks: {len(ws)} 2D systems, N log-uniform in [256, 16384] (total {N}), "
-        "area fraction U[0.3, 0.8], Hertz + tangential spring, mu 0.3, NeighborList over CellList",
+        f"area fraction U[0.3, 0.8], Hertz + tangential spring, mu 0.3, NeighborList over CellList, skin {skin}"
+        + (" (autotuned)" if tune else ""),
         "n_systems": len(ws),
         "n_particles": N,
         "steps": args.batch_steps,
@@ -179,6 +195,10 @@ def batch_bench(args, dev) -> dict:
         "plain_sample": f"{len(pick)} systems stratified by size, {args.batch_steps} steps each with System.step",
         "batched_over_plain": batched / plain,
         "rebuilds": int(stats.rebuilds) if stats is not None else None,
+        "skin": skin,
+        "autotuner": {"candidates_skin_value_rebuilds": tune,
+                      "probe": f"every {args.batch_probe_every}th system, {args.batch_probe_steps} steps"}
+        if tune else None,
     }
 
 

@@ -17,8 +17,9 @@ batch equals its trajectory alone, bit for bit.
 from __future__ import annotations
 
 import ctypes
+import time
 from dataclasses import dataclass, fields, replace
-from typing import Sequence
+from typing import Callable, Sequence
 
 import numpy as np
 import torch
@@ -307,4 +308,35 @@ class Batch:
         return out
 
 
-__all__ = ["Batch"]
+def autotune_skin(make_pairs: Callable[[float], list], skins: Sequence[float], probe_steps: int = 200,
+                  warm_steps: int = 20, device="cuda") -> tuple:
+    """Rebuild-cadence autotuner of a batch.  For every candidate skin, packs
+    the systems ``make_pairs(skin)`` returns, steps ``warm_steps`` untimed
+    (first list builds) and ``probe_steps`` timed, and keeps the skin with
+    the highest particle-steps/s.  The skin sets the list radius (cutoff +
+    skin), hence the cell size of the list build and the rebuild cadence (a
+    list is rebuilt once a sphere has moved skin/2): a small skin rebuilds
+    often, a large one tests more listed pairs at every step.  It does not
+    change which pairs touch, so every candidate gives the same forces up to
+    the order of the sums.  Returns (best skin, [(skin, particle-steps/s,
+    rebuilds), ...])."""
+    table = []
+    for skin in skins:
+        pairs = make_pairs(float(skin))
+        b = Batch.pack(pairs).to(device)
+        del pairs
+        b = b.step(warm_steps)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        b2 = b.step(probe_steps)
+        torch.cuda.synchronize()
+        secs = time.perf_counter() - t0
+        st = b2.system.__dict__.get("_last_stats")
+        table.append((float(skin), b.n_particles * probe_steps / secs, int(st.rebuilds) if st is not None else -1))
+        del b, b2
+        torch.cuda.empty_cache()
+    best = max(table, key=lambda r: r[1])[0]
+    return best, table
+
+
+__all__ = ["Batch", "autotune_skin"]

@@ -170,14 +170,18 @@ def disks_hertz_friction(N: int, seed: int, phi: float, skin: float = 0.1, E: fl
     )
 
 
-def batch_disks(n_systems: int = 4096, seed: int = 0, n_min: int = 256, n_max: int = 16384) -> list:
+def batch_disks(n_systems: int = 4096, seed: int = 0, n_min: int = 256, n_max: int = 16384, skin: float = 0.1,
+                every: int = 1) -> list:
     """The batching workload: n_systems independent disk systems, N
     log-uniform in [n_min, n_max] and area fraction uniform in [0.3, 0.8]
-    (both seeded); system k uses seed (seed, k) for its own arrays."""
+    (both seeded); system k uses seed (seed, k) for its own arrays.  With
+    ``every`` > 1 only systems 0, every, 2 every, ... are made (a sample of
+    the same systems)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     Ns = np.rint(np.exp(rng.uniform(math.log(n_min), math.log(n_max), n_systems))).astype(int)
     phis = rng.uniform(0.3, 0.8, n_systems)
-    return [disks_hertz_friction(int(N), seed=seed * 1000003 + k, phi=float(p)) for k, (N, p) in enumerate(zip(Ns, phis))]
+    return [disks_hertz_friction(int(N), seed=seed * 1000003 + k, phi=float(p), skin=skin)
+            for k, (N, p) in enumerate(zip(Ns, phis)) if k % every == 0]
 
 
 def polydisperse_spheres(N: int, seed: int, phi: float, domain: str, mu: float, skin: float = 0.1,
