@@ -21,7 +21,7 @@ std::vector<Buf*> jdem_ctx::bufs() {
   std::vector<Buf*> all = {&pos2,    &vel2,   &ang2,     &keys[0], &keys[1], &vals[0], &vals[1], &hist,
                            &cstart,  &cend,   &spos,     &nl_new,  &xi_new,  &live_new, &flags,  &red,
                            &nl_copy, &xi_copy, &live_copy, &wF,     &wT,      &wOld,     &wWpos, &wSrc,
-                           &wInv,    &wKeys,  &fireV,    &fireDev, &segp,  &segid, &segflag};
+                           &wInv,    &wKeys,  &fireV,    &fireDev, &segp,  &segid, &segflag, &wXiRank};
   for (int k = 0; k < 2; ++k)
     for (Buf* b : {&wP[k], &wV[k], &wW[k], &wQ[k], &wI[k], &wOrig[k], &wNl[k], &wXi[k], &wLive[k]}) all.push_back(b);
   return all;
@@ -462,6 +462,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     return fail(ctx, JDEM_EUNSUPPORTED, "NeighborList over a CellList with a FreeDomain (per-step grid)");
   const int64_t n_cells = sg ? sg->total_cells : L.grid.n_cells;
   const size_t n4 = 16 * (size_t)N, ni = 4 * (size_t)N;
+  const size_t Npad = ((size_t)N + 63) & ~(size_t)63;  // whole wavefronts of spring regions
   for (int k = 0; k < 2; ++k) {
     ENSURE(ctx, ctx->wP[k], n4);
     ENSURE(ctx, ctx->wV[k], n4);
@@ -470,9 +471,10 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     ENSURE(ctx, ctx->wI[k], n4);
     ENSURE(ctx, ctx->wOrig[k], ni);
     ENSURE(ctx, ctx->wNl[k], 4 * (size_t)N * (size_t)std::max(M, 1));
-    ENSURE(ctx, ctx->wXi[k], hist ? 16 * (size_t)N * (size_t)std::max(M, 1) : 16);
+    ENSURE(ctx, ctx->wXi[k], hist ? 16 * Npad * (size_t)std::max(M, 1) : 16);
     ENSURE(ctx, ctx->wLive[k], 8 * (size_t)N * W64);
   }
+  ENSURE(ctx, ctx->wXiRank, hist ? 16 * (size_t)N * (size_t)std::max(M, 1) : 16);
   ENSURE(ctx, ctx->wF, n4);
   ENSURE(ctx, ctx->wT, n4);
   ENSURE(ctx, ctx->wOld, n4);
@@ -538,9 +540,13 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   io.T4 = F4(ctx->wT);
   io.old4 = F4(ctx->wOld);
   io.nl = I32(ctx->wNl[0]);
-  io.xi = F4(ctx->wXi[0]);
+  io.xi = F4(ctx->wXiRank);
   io.live = U64(ctx->wLive[0]);
   HIPCHK(ctx, ws_launch_in(io, M > 0, st));
+  // the step kernel keeps springs per wavefront
+  if (hist && M > 0)
+    HIPCHK(ctx, ws_launch_pack_springs(N, M, W64, F4(ctx->wXiRank), U64(ctx->wLive[0]), F4(ctx->wXi[0]),
+                                       F4(ctx->wW[0]), nullptr, nullptr, st));
 
   // --- rebuild at the current positions (neighbor_list.py:545-589) ---
   // Batches: only the systems that asked for a rebuild are re-sorted and
@@ -551,6 +557,10 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     HIPCHK(ctx, launch_fill_i32((int32_t*)ctx->flags.p + 2, 1, 0, st));
     HIPCHK(ctx, ws_launch_seg_modes(sg->n_seg, sg->seg_flag, sg->n_build, sg->seg_mode, st));
     HIPCHK(ctx, ws_launch_seg_prefix(sg->n_seg, sg->seg_mode, sg->offsets, sg->segp, sg->cpref, sg->ccref, st));
+    // springs of the wavefronts that hold a rebuilt system: per wave -> by rank
+    if (hist && M > 0)
+      HIPCHK(ctx, ws_launch_unpack_springs(N, M, W64, F4(ctx->wXi[xc]), U64(ctx->wLive[nlc]), F4(ctx->wW[cur]),
+                                           F4(ctx->wXiRank), sg->seg, sg->seg_mode, st));
     int64_t tot[2];
     HIPCHK(ctx, hipMemcpyAsync(&tot[0], sg->cpref + sg->n_seg, 8, hipMemcpyDeviceToHost, st));
     HIPCHK(ctx, hipMemcpyAsync(&tot[1], sg->ccref + sg->n_seg, 8, hipMemcpyDeviceToHost, st));
@@ -579,7 +589,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       r.orig = I32(ctx->wOrig[set]);
       r.nl = I32(ctx->wNl[nlc]);
       r.live = hist ? U64(ctx->wLive[nlc]) : nullptr;
-      r.xi = F4(ctx->wXi[xc]);
+      r.xi = F4(ctx->wXiRank);
       r.keys_c = (uint32_t*)ctx->wKeys.p;
       r.slot_c = I32(ctx->cend);
       // staging: buffers that hold nothing live during a batch rebuild
@@ -629,12 +639,15 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
           b.old_nl = r.tnl;
           b.old_xi = r.txi;
           b.old_live = r.tlive;
-          b.xi = F4(ctx->wXi[xc]);
+          b.xi = F4(ctx->wXiRank);
           b.live = U64(ctx->wLive[nlc]);
         }
         HIPCHK(ctx, ws_launch_nlist(b, D, false, st));
       }
     }
+    if (hist && M > 0)
+      HIPCHK(ctx, ws_launch_pack_springs(N, M, W64, F4(ctx->wXiRank), U64(ctx->wLive[nlc]), F4(ctx->wXi[xc]),
+                                         F4(ctx->wW[cur]), sg->seg, sg->seg_mode, st));
     L.n_build_times += 1;
     span_end(ctx, sp);
     return JDEM_OK;
@@ -644,6 +657,10 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     if (sg) return rebuild_batch();
     const int sp = span_begin(ctx, 2, 0);
     HIPCHK(ctx, launch_fill_i32((int32_t*)ctx->flags.p + 2, 1, 0, st));
+    // springs per wave -> by rank, at the slots before the re-sort
+    if (hist && M > 0)
+      HIPCHK(ctx, ws_launch_unpack_springs(N, M, W64, F4(ctx->wXi[xc]), U64(ctx->wLive[nlc]), F4(ctx->wW[cur]),
+                                           F4(ctx->wXiRank), nullptr, nullptr, st));
     int32_t* src = I32(ctx->wSrc);
     int32_t* inv = I32(ctx->wInv);
     uint32_t* sk = nullptr;
@@ -687,14 +704,18 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
         b.src = src;
         b.inv = inv;
         b.old_nl = I32(ctx->wNl[nlc]);
-        b.old_xi = F4(ctx->wXi[xc]);
+        b.old_xi = F4(ctx->wXiRank);
         b.old_live = U64(ctx->wLive[nlc]);
-        b.xi = F4(ctx->wXi[1 - xc]);
+        b.xi = F4(ctx->wXi[1 - xc]);  // by rank here, per wave after the pack below
         b.live = U64(ctx->wLive[1 - nlc]);
       }
       HIPCHK(ctx, ws_launch_nlist(b, D, naive2, st));
       nlc = 1 - nlc;
-      xc = 1 - xc;
+      if (hist) {
+        // new springs by rank (wXi[1 - xc]) -> per wave (wXi[xc], free now)
+        HIPCHK(ctx, ws_launch_pack_springs(N, M, W64, F4(ctx->wXi[1 - xc]), U64(ctx->wLive[nlc]), F4(ctx->wXi[xc]),
+                                           F4(ctx->wW[cur]), nullptr, nullptr, st));
+      }
     }
     HIPCHK(ctx, ws_launch_copy4(F4(ctx->wOld), F4(ctx->wP[cur]), N, st));
     L.n_build_times += 1;
@@ -843,8 +864,11 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   io.Q4 = F4(ctx->wQ[set]);
   io.I4 = F4(ctx->wI[set]);
   io.nl = I32(ctx->wNl[nlc]);
-  io.xi = F4(ctx->wXi[xc]);
   io.live = U64(ctx->wLive[nlc]);
+  io.xi = F4(ctx->wXiRank);
+  if (hist && M > 0)
+    HIPCHK(ctx, ws_launch_unpack_springs(N, M, W64, F4(ctx->wXi[xc]), U64(ctx->wLive[nlc]), F4(ctx->wW[cur]),
+                                         F4(ctx->wXiRank), nullptr, nullptr, st));
   HIPCHK(ctx, ws_launch_out(io, M > 0, st));
   if (free_done) {
     for (int a = 0; a < D; ++a) {

@@ -46,7 +46,7 @@ struct jdem_ctx {
   jdem::Buf red;                       // reduction partials
   jdem::Buf nl_copy, xi_copy, live_copy;
   // resident NeighborList working set (jdem_resident.hip)
-  jdem::Buf wP[2], wV[2], wW[2], wQ[2], wI[2], wF, wT, wOld, wOrig[2], wWpos, wSrc, wInv, wKeys, fireV, fireDev, segp, segid, segflag, wNl[2], wXi[2], wLive[2];
+  jdem::Buf wP[2], wV[2], wW[2], wQ[2], wI[2], wF, wT, wOld, wOrig[2], wWpos, wSrc, wInv, wKeys, fireV, fireDev, segp, segid, segflag, wNl[2], wXi[2], wLive[2], wXiRank;
   int32_t* host_flags = nullptr;       // pinned host mirror of flags
   // HIP-event timing (jdem_ctx_set_timing)
   struct TimedSpan {
@@ -356,6 +356,19 @@ struct WS {
 
 hipError_t ws_launch_in(const InOut& io, bool with_nl, hipStream_t s);
 hipError_t ws_launch_out(const InOut& io, bool with_nl, hipStream_t s);
+// Tangential springs of the step kernel, per wavefront: the 64 work slots
+// of wave w own the region [64 w M, 64 (w + 1) M) of the spring buffer, and
+// slot b's springs (in slot order of its live bits) start at offset W[b].w
+// (an int in the float bits) of that region.  The list rebuild, the entry
+// and the exit keep springs by rank instead ([k N + b], k-th live slot).
+// pack: by rank -> per wave (offsets = prefix of the live counts);
+// unpack: per wave -> by rank.  With seg_mode, only wavefronts holding a
+// slot of a system that is rebuilt (mode != SEG_KEEP) are converted.
+hipError_t ws_launch_pack_springs(int64_t n, int M, int W64, const float4* rank, const uint64_t* live, float4* wave,
+                                  float4* W, const int32_t* seg, const int32_t* seg_mode, hipStream_t s);
+hipError_t ws_launch_unpack_springs(int64_t n, int M, int W64, const float4* wave, const uint64_t* live,
+                                    const float4* W, float4* rank, const int32_t* seg, const int32_t* seg_mode,
+                                    hipStream_t s);
 hipError_t ws_launch_inverse(const int32_t* orig, int64_t n, int32_t* wpos, hipStream_t s);
 hipError_t ws_launch_iota(int32_t* p, int64_t n, hipStream_t s);
 hipError_t ws_launch_copy4(float4* d, const float4* src, int64_t n, hipStream_t s);

@@ -672,6 +672,57 @@ __global__ void __launch_bounds__(kBlock) k_iota(int32_t* __restrict__ p, int64_
 
 
 // ---------------------------------------------------------------------------
+// spring layouts: by rank <-> per wavefront
+// ---------------------------------------------------------------------------
+// Every lane of a wave runs to the end (the scan is wave-wide); lanes past n
+// count zero springs.
+JD int wave_exclusive_scan(int v, int lane) {
+  int incl = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int t = __shfl_up(incl, d);
+    if (lane >= d) incl += t;
+  }
+  return incl - v;
+}
+
+JD bool wave_touched(int64_t b, int64_t n, const int32_t* seg, const int32_t* seg_mode) {
+  if (!seg_mode) return true;
+  const bool mine = b < n && seg_mode[seg[b]] != SEG_KEEP;
+  return __ballot(mine) != 0ull;
+}
+
+__global__ void __launch_bounds__(kBlock) k_pack_springs(int64_t n, int M, int W64, const float4* __restrict__ rank,
+                                                          const uint64_t* __restrict__ live, float4* __restrict__ wave,
+                                                          float4* __restrict__ W, const int32_t* __restrict__ seg,
+                                                          const int32_t* __restrict__ seg_mode) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int lane = threadIdx.x & 63;
+  if (!wave_touched(b, n, seg, seg_mode)) return;  // wave-uniform
+  int cnt = 0;
+  if (b < n)
+    for (int w = 0; w < W64; ++w) cnt += __popcll(live[(int64_t)w * n + b]);
+  const int off = wave_exclusive_scan(cnt, lane);
+  if (b >= n) return;
+  float4* dst = wave + (b & ~(int64_t)63) * M + off;
+  for (int k = 0; k < cnt; ++k) dst[k] = rank[(int64_t)k * n + b];
+  W[b].w = __int_as_float(off);
+}
+
+__global__ void __launch_bounds__(kBlock) k_unpack_springs(int64_t n, int M, int W64, const float4* __restrict__ wave,
+                                                            const uint64_t* __restrict__ live,
+                                                            const float4* __restrict__ W, float4* __restrict__ rank,
+                                                            const int32_t* __restrict__ seg,
+                                                            const int32_t* __restrict__ seg_mode) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (!wave_touched(b, n, seg, seg_mode) || b >= n) return;
+  int cnt = 0;
+  for (int w = 0; w < W64; ++w) cnt += __popcll(live[(int64_t)w * n + b]);
+  const float4* src = wave + (b & ~(int64_t)63) * M + __float_as_int(W[b].w);
+  for (int k = 0; k < cnt; ++k) rank[(int64_t)k * n + b] = src[k];
+}
+
+// ---------------------------------------------------------------------------
 // the fused step on the work set
 // ---------------------------------------------------------------------------
 
@@ -750,6 +801,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
   me.r = p4.w;
   Vec<D> F;
   Ang<D> T;
+  int spring_off = -1;  // start of this slot's new springs in its wave's region
   if (a.do_force) {
     F = zero_vec<D>();
     T = zero_ang<D>();
@@ -816,15 +868,18 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
     }
     const int total = __shfl(incl, 63);
     const int off = incl - nc;
+    spring_off = off;
     uint64_t ln[MW];
 #pragma unroll
     for (int q = 0; q < MW; ++q) ln[q] = 0ull;
     int rank_new = 0;  // springs are stored by rank
     for (int cb = 0; cb < total; cb += kPairCap) {  // wave-uniform
       // descriptors of pairs cb .. cb + kPairCap - 1: owner lane | slot << 6
-      // | (old spring rank + 1) << 14, and the partner index
+      // | (old spring's place in the wave's region + 1) << 14, and the
+      // partner index
       if (off < cb + kPairCap && off + nc > cb) {
-        int idx = off, rank_base = 0, c = 0;
+        // old springs: this slot's start in the wave's spring region
+        int idx = off, rank_base = LawTraits<LAW>::hist ? __float_as_int(a.W[b].w) : 0, c = 0;
 #pragma unroll
         for (int w = 0; w < MW; ++w) {
           uint64_t cmw = w < n_words ? cm[w] : 0ull;
@@ -869,7 +924,7 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
             wo = a.W[bo];
             wj = a.W[j];
           }
-          if constexpr (LawTraits<LAW>::hist) xo = a.xi_rd[(int64_t)(k >= 0 ? k : 0) * a.n + bo];
+          if constexpr (LawTraits<LAW>::hist) xo = a.xi_rd[(b0 - lane) * a.M + (k >= 0 ? k : 0)];
           DomP dom_o;
           float dt_o;
           if (PM == 2) {
@@ -919,7 +974,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
           if constexpr (LawTraits<LAW>::hist) {
             if (fr.w != 0.0f) {
               const int sl = (pdesc[wv][q - cb] >> 6) & 255;
-              a.xi_wr[(int64_t)rank_new * a.n + b] = pres[wv][2][q - q0];
+              // new springs from the slot's marked-pair prefix: the wave's
+              // springs fill one contiguous run of its region
+              a.xi_wr[(b0 - lane) * a.M + off + rank_new] = pres[wv][2][q - q0];
               ++rank_new;
               const uint64_t bit = 1ull << (sl & 63);
               const int ws = sl >> 6;
@@ -988,7 +1045,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
   }
   a.P2[b] = f4(x, p4b.w);
   a.V2[b] = f4(v, m);
-  a.W2[b] = f4a(w);
+  float4 w_out = f4a(w);
+  // the springs' place travels with the angular velocity (its free lane)
+  w_out.w = __int_as_float(spring_off >= 0 ? spring_off : __float_as_int(w4.w));
+  a.W2[b] = w_out;
   a.Q[b] = f4q(q);
   if (a.store_ft) {
     a.F[b] = f4(F, 0.0f);
@@ -1039,6 +1099,22 @@ hipError_t launch_ws_dim(const WS& a, int law, bool per, hipStream_t s) {
     default:
       return hipErrorInvalidValue;
   }
+}
+
+hipError_t ws_launch_pack_springs(int64_t n, int M, int W64, const float4* rank, const uint64_t* live, float4* wave,
+                                  float4* W, const int32_t* seg, const int32_t* seg_mode, hipStream_t s) {
+  if (n == 0 || M == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_pack_springs, dim3(nblocks(n)), dim3(kBlock), 0, s, n, M, W64, rank, live, wave, W, seg,
+                     seg_mode);
+  return hipGetLastError();
+}
+hipError_t ws_launch_unpack_springs(int64_t n, int M, int W64, const float4* wave, const uint64_t* live,
+                                    const float4* W, float4* rank, const int32_t* seg, const int32_t* seg_mode,
+                                    hipStream_t s) {
+  if (n == 0 || M == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_unpack_springs, dim3(nblocks(n)), dim3(kBlock), 0, s, n, M, W64, wave, live, W, rank, seg,
+                     seg_mode);
+  return hipGetLastError();
 }
 
 hipError_t ws_launch_step(const WS& a, int dim, int law, hipStream_t s) {
