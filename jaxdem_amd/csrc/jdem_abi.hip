@@ -657,10 +657,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     if (sg) return rebuild_batch();
     const int sp = span_begin(ctx, 2, 0);
     HIPCHK(ctx, launch_fill_i32((int32_t*)ctx->flags.p + 2, 1, 0, st));
-    // springs per wave -> by rank, at the slots before the re-sort
-    if (hist && M > 0)
-      HIPCHK(ctx, ws_launch_unpack_springs(N, M, W64, F4(ctx->wXi[xc]), U64(ctx->wLive[nlc]), F4(ctx->wW[cur]),
-                                           F4(ctx->wXiRank), nullptr, nullptr, st));
+    const int cur_before = cur;  // the W buffer that holds the springs' offsets
     int32_t* src = I32(ctx->wSrc);
     int32_t* inv = I32(ctx->wInv);
     uint32_t* sk = nullptr;
@@ -704,7 +701,8 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
         b.src = src;
         b.inv = inv;
         b.old_nl = I32(ctx->wNl[nlc]);
-        b.old_xi = F4(ctx->wXiRank);
+        b.old_xi = F4(ctx->wXi[xc]);  // per wave, read through the old offsets
+        b.old_W = F4(ctx->wW[cur_before]);
         b.old_live = U64(ctx->wLive[nlc]);
         b.xi = F4(ctx->wXi[1 - xc]);  // by rank here, per wave after the pack below
         b.live = U64(ctx->wLive[1 - nlc]);

@@ -250,6 +250,9 @@ struct NLW {
   const int32_t* inv;       // old work slot -> new work slot
   const int32_t* old_nl;
   const float4* old_xi;
+  // old springs per wavefront (the step kernel's layout), their offsets in
+  // old_W[.].w; null: old_xi is by rank
+  const float4* old_W;
   const uint64_t* old_live;
   float4* xi;
   uint64_t* live;

@@ -512,7 +512,9 @@ JD void remap_row(const NLW& a, int64_t b, int nrow, Row row, bool remap) {
               out3 |= ws == 3 ? bit : 0ull;
             } else {
               const int rank_new = rank_below(out0, out1, out2, out3, s);
-              a.xi[(int64_t)rank_new * n + b] = a.old_xi[(int64_t)rank_old * os + ob];
+              const int64_t oi = a.old_W ? (ob & ~(int64_t)63) * a.M + __float_as_int(a.old_W[ob].w) + rank_old
+                                         : (int64_t)rank_old * os + ob;
+              a.xi[(int64_t)rank_new * n + b] = a.old_xi[oi];
             }
           }
           ++rank_old;
