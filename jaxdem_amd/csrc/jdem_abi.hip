@@ -704,16 +704,13 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
         b.old_xi = F4(ctx->wXi[xc]);  // per wave, read through the old offsets
         b.old_W = F4(ctx->wW[cur_before]);
         b.old_live = U64(ctx->wLive[nlc]);
-        b.xi = F4(ctx->wXi[1 - xc]);  // by rank here, per wave after the pack below
+        b.xi = F4(ctx->wXi[1 - xc]);  // per wave, offsets into the new W
+        b.new_W = F4(ctx->wW[cur]);
         b.live = U64(ctx->wLive[1 - nlc]);
       }
       HIPCHK(ctx, ws_launch_nlist(b, D, naive2, st));
       nlc = 1 - nlc;
-      if (hist) {
-        // new springs by rank (wXi[1 - xc]) -> per wave (wXi[xc], free now)
-        HIPCHK(ctx, ws_launch_pack_springs(N, M, W64, F4(ctx->wXi[1 - xc]), U64(ctx->wLive[nlc]), F4(ctx->wXi[xc]),
-                                           F4(ctx->wW[cur]), nullptr, nullptr, st));
-      }
+      if (hist) xc = 1 - xc;
     }
     HIPCHK(ctx, ws_launch_copy4(F4(ctx->wOld), F4(ctx->wP[cur]), N, st));
     L.n_build_times += 1;

@@ -253,6 +253,9 @@ struct NLW {
   // old springs per wavefront (the step kernel's layout), their offsets in
   // old_W[.].w; null: old_xi is by rank
   const float4* old_W;
+  // new springs per wavefront with their offsets written to new_W[.].w
+  // (one wave-wide scan of the new counts); null: xi is by rank
+  float4* new_W;
   const uint64_t* old_live;
   float4* xi;
   uint64_t* live;

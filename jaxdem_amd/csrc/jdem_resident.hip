@@ -483,45 +483,72 @@ JD int find_in_row(int32_t tgt, int nrow, Row row) {
   return -1;
 }
 
+// exclusive prefix sum over the 64 lanes of a wave; every lane must call it
+JD int wave_exclusive_scan(int v, int lane) {
+  int incl = v;
+#pragma unroll
+  for (int d = 1; d < 64; d <<= 1) {
+    const int t = __shfl_up(incl, d);
+    if (lane >= d) incl += t;
+  }
+  return incl - v;
+}
+
+// With new_W set, every lane of the wave must call it (inactive lanes with
+// active = false): the new springs are placed per wavefront after a
+// wave-wide scan of the new counts.
 template <class Row>
-JD void remap_row(const NLW& a, int64_t b, int nrow, Row row, bool remap) {
+JD void remap_row(const NLW& a, int64_t b, bool active, int nrow, Row row, bool remap) {
   if (!a.live) return;
   const int64_t n = a.n;
   uint64_t out0 = 0ull, out1 = 0ull, out2 = 0ull, out3 = 0ull;
-  if (remap) {
-    // the old row of the particle now at b: at slot src[b] of the previous
-    // buffers, or (batches) at compact index cidx[b] of the staged rows
-    const int64_t ob = a.cidx ? (int64_t)a.cidx[b] : (int64_t)a.src[b];
-    const int64_t os = a.cidx ? a.ostride : n;
-    for (int sweep = 0; sweep < 2; ++sweep) {
-      int rank_old = 0;
-      for (int w = 0; w < a.W64; ++w) {
-        uint64_t bits = a.old_live[(int64_t)w * os + ob];
-        while (bits) {
-          const int t = w * 64 + (__ffsll((unsigned long long)bits) - 1);
-          bits &= bits - 1ull;
-          const int32_t o = a.old_nl[(int64_t)t * os + ob];
-          const int s = o < 0 ? -1 : find_in_row(a.inv[o], nrow, row);
-          if (s >= 0) {
-            if (sweep == 0) {
-              const uint64_t bit = 1ull << (s & 63);
-              const int ws = s >> 6;
-              out0 |= ws == 0 ? bit : 0ull;
-              out1 |= ws == 1 ? bit : 0ull;
-              out2 |= ws == 2 ? bit : 0ull;
-              out3 |= ws == 3 ? bit : 0ull;
-            } else {
-              const int rank_new = rank_below(out0, out1, out2, out3, s);
-              const int64_t oi = a.old_W ? (ob & ~(int64_t)63) * a.M + __float_as_int(a.old_W[ob].w) + rank_old
-                                         : (int64_t)rank_old * os + ob;
-              a.xi[(int64_t)rank_new * n + b] = a.old_xi[oi];
-            }
-          }
-          ++rank_old;
-        }
+  // the old row of the particle now at b: at slot src[b] of the previous
+  // buffers, or (batches) at compact index cidx[b] of the staged rows
+  const bool go = active && remap;
+  const int64_t ob = go ? (a.cidx ? (int64_t)a.cidx[b] : (int64_t)a.src[b]) : 0;
+  const int64_t os = a.cidx ? a.ostride : n;
+  // start of the old springs, read before any lane writes new offsets (old_W
+  // and new_W are one buffer when the rebuild does not re-sort)
+  const int64_t obase = (go && a.old_W) ? (ob & ~(int64_t)63) * a.M + __float_as_int(a.old_W[ob].w) : 0;
+  // visit(t, o_slot_new): each live old slot t whose partner is listed
+  // again, with the rank of t among the live old slots
+  auto sweep = [&](auto&& f) {
+    int rank_old = 0;
+    for (int w = 0; w < a.W64; ++w) {
+      uint64_t bits = a.old_live[(int64_t)w * os + ob];
+      while (bits) {
+        const int t = w * 64 + (__ffsll((unsigned long long)bits) - 1);
+        bits &= bits - 1ull;
+        const int32_t o = a.old_nl[(int64_t)t * os + ob];
+        const int s = o < 0 ? -1 : find_in_row(a.inv[o], nrow, row);
+        if (s >= 0) f(s, rank_old);
+        ++rank_old;
       }
     }
+  };
+  if (go)
+    sweep([&](int s, int) {
+      const uint64_t bit = 1ull << (s & 63);
+      const int ws = s >> 6;
+      out0 |= ws == 0 ? bit : 0ull;
+      out1 |= ws == 1 ? bit : 0ull;
+      out2 |= ws == 2 ? bit : 0ull;
+      out3 |= ws == 3 ? bit : 0ull;
+    });
+  int64_t wbase = 0;
+  if (a.new_W) {
+    const int cnt = __popcll(out0) + __popcll(out1) + __popcll(out2) + __popcll(out3);
+    const int off = wave_exclusive_scan(cnt, (int)(threadIdx.x & 63));
+    if (active) a.new_W[b].w = __int_as_float(off);
+    wbase = (b & ~(int64_t)63) * a.M + off;
   }
+  if (go)
+    sweep([&](int s, int rank_old) {
+      const int rank_new = rank_below(out0, out1, out2, out3, s);
+      const int64_t oi = a.old_W ? obase + rank_old : (int64_t)rank_old * os + ob;
+      a.xi[a.new_W ? wbase + rank_new : (int64_t)rank_new * n + b] = a.old_xi[oi];
+    });
+  if (!active) return;
   a.live[b] = out0;
   if (a.W64 > 1) a.live[n + b] = out1;
   if (a.W64 > 2) a.live[2 * n + b] = out2;
@@ -530,10 +557,10 @@ JD void remap_row(const NLW& a, int64_t b, int nrow, Row row, bool remap) {
 
 // A row ends at its first -1 (or at M slots); the slots after the
 // terminator are not written.
-JD void end_row(const NLW& a, int64_t b, int count) {
-  if (count < a.M) a.nl[(int64_t)count * a.n + b] = -1;
-  if (count > a.M) atomicOr(a.overflow, 1);
-  remap_row(a, b, min(count, a.M), [&](int s) { return a.nl[(int64_t)s * a.n + b]; }, a.remap != 0);
+JD void end_row(const NLW& a, int64_t b, bool active, int count) {
+  if (active && count < a.M) a.nl[(int64_t)count * a.n + b] = -1;
+  if (active && count > a.M) atomicOr(a.overflow, 1);
+  remap_row(a, b, active, min(count, a.M), [&](int s) { return a.nl[(int64_t)s * a.n + b]; }, a.remap != 0);
 }
 
 
@@ -650,14 +677,16 @@ __global__ void __launch_bounds__(kBlock) k_nlist_work(NLW a) {
     atomicMax(&row_max, min(count, kRowLds));
   }
   __syncthreads();
-  if (!active) return;
   const int c = min(count, a.M);
-  const int staged = min(row_max, a.M);
-  for (int s = 0; s < staged; ++s)
-    if (s < c) a.nl[(int64_t)s * a.n + b] = rows[s][tid];
-  if (count < a.M) a.nl[(int64_t)count * a.n + b] = -1;
-  if (count > a.M) atomicOr(a.overflow, 1);
-  remap_row(a, b, c, [&](int s) { return s < kRowLds ? rows[s][tid] : a.nl[(int64_t)s * a.n + b]; }, remap);
+  if (active) {
+    const int staged = min(row_max, a.M);
+    for (int s = 0; s < staged; ++s)
+      if (s < c) a.nl[(int64_t)s * a.n + b] = rows[s][tid];
+    if (count < a.M) a.nl[(int64_t)count * a.n + b] = -1;
+    if (count > a.M) atomicOr(a.overflow, 1);
+  }
+  remap_row(a, b, active, c, [&](int s) { return s < kRowLds ? rows[s][tid] : a.nl[(int64_t)s * a.n + b]; },
+            remap);
 }
 
 
@@ -676,17 +705,6 @@ __global__ void __launch_bounds__(kBlock) k_iota(int32_t* __restrict__ p, int64_
 // ---------------------------------------------------------------------------
 // spring layouts: by rank <-> per wavefront
 // ---------------------------------------------------------------------------
-// Every lane of a wave runs to the end (the scan is wave-wide); lanes past n
-// count zero springs.
-JD int wave_exclusive_scan(int v, int lane) {
-  int incl = v;
-#pragma unroll
-  for (int d = 1; d < 64; d <<= 1) {
-    const int t = __shfl_up(incl, d);
-    if (lane >= d) incl += t;
-  }
-  return incl - v;
-}
 
 JD bool wave_touched(int64_t b, int64_t n, const int32_t* seg, const int32_t* seg_mode) {
   if (!seg_mode) return true;
@@ -1143,19 +1161,21 @@ __global__ void __launch_bounds__(kBlock) k_inverse(const int32_t* __restrict__ 
 template <int D, bool PER>
 __global__ void __launch_bounds__(kBlock) k_nlist_work_naive(NLW a) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= a.n) return;
-  const Vec<D> xi = xyz<D>(a.P4[b]);
-  const bool chk_bond = (!a.isb) && a.bond_w > 0;
+  const bool active = b < a.n;  // every lane reaches end_row (wave-wide scan)
   int count = 0;
-  for (int64_t k = a.n - 1; k >= 0; --k) {
-    if (k == b) continue;
-    const Vec<D> d = disp_<PER>(xi, xyz<D>(a.P4[k]), a.dom);
-    if (!(dot(d, d) <= a.cutoff_sq)) continue;
-    if (chk_bond && bonded_o(a.bond_id, a.bond_w, a.orig[b], a.orig[k])) continue;
-    if (count < a.M) a.nl[(int64_t)count * a.n + b] = (int32_t)k;
-    ++count;
+  if (active) {
+    const Vec<D> xi = xyz<D>(a.P4[b]);
+    const bool chk_bond = (!a.isb) && a.bond_w > 0;
+    for (int64_t k = a.n - 1; k >= 0; --k) {
+      if (k == b) continue;
+      const Vec<D> d = disp_<PER>(xi, xyz<D>(a.P4[k]), a.dom);
+      if (!(dot(d, d) <= a.cutoff_sq)) continue;
+      if (chk_bond && bonded_o(a.bond_id, a.bond_w, a.orig[b], a.orig[k])) continue;
+      if (count < a.M) a.nl[(int64_t)count * a.n + b] = (int32_t)k;
+      ++count;
+    }
   }
-  end_row(a, b, count);
+  end_row(a, b, active, count);
 }
 
 hipError_t ws_launch_in(const InOut& io, bool with_nl, hipStream_t s) {
