@@ -568,7 +568,7 @@ JD void end_row(const NLW& a, int64_t b, bool active, int count) {
 // The first kRowLds slots of each row are collected in LDS, slot-major, and
 // written out one slot at a time for the whole workgroup, so the stores of a
 // slot are contiguous; later slots go straight to memory.
-constexpr int kRowLds = 24;
+constexpr int kRowLds = 16;
 
 // PM: 0 = one non-periodic system, 1 = one periodic system, 2 = a segmented
 // batch, where each work slot takes the grid, domain and cutoff of its system.

@@ -204,6 +204,8 @@ def main(argv=None):
         b.add_argument("--out", required=True)
         b.add_argument("--max-steps", type=int, default=400000)
     sub.choices["gpu"].add_argument("--stall-steps", type=int, default=20000)
+    sub.choices["gpu"].add_argument("--chunk-steps", type=int, default=0,
+                                    help="restart FIRE every this many steps, printing progress (0: one run)")
     sub.choices["gpu"].add_argument("--no-absolute", action="store_true")
     c = sub.add_parser("compare")
     c.add_argument("ref")
@@ -229,7 +231,18 @@ def main(argv=None):
         _save(args.out, {**r, "rad": s["rad"], "L": s["L"]})
     elif args.cmd == "gpu":
         s = _load(args.start)
-        r = gpu_precise(s["pos"], s["rad"], s["L"], max_steps=args.max_steps, stall_steps=args.stall_steps)
+        if args.chunk_steps > 0:
+            pos, done, secs = s["pos"], 0, 0.0
+            while done < args.max_steps:
+                r = gpu_precise(pos, s["rad"], s["L"], max_steps=min(args.chunk_steps, args.max_steps - done),
+                                stall_steps=args.stall_steps)
+                pos, done, secs = r["pos"], done + r["steps"], secs + r["seconds"]
+                print(f"  {done} steps, max|F| {r['max_grad']:.3e}, E/N {r['pe']:.12g}", flush=True)
+                if r["steps"] < args.chunk_steps:
+                    break  # stalled or converged within the chunk
+            r["steps"], r["seconds"] = done, secs
+        else:
+            r = gpu_precise(s["pos"], s["rad"], s["L"], max_steps=args.max_steps, stall_steps=args.stall_steps)
         print(f"precise: {r['steps']} steps, max|F| {r['max_grad']:.3e}, E/N {r['pe']:.17g}, "
               f"P {r['pressure']:.17g}, {r['seconds']:.2f} s", flush=True)
         _save(args.out, {"pos": r["pos"], "pe": r["pe"], "pressure": r["pressure"], "max_grad": r["max_grad"],
