@@ -1417,6 +1417,7 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
   ENSURE(ctx, ctx->red, 8 * (Nn * (1 + D * D) + 256 * D * D + D * D + 8));
   ENSURE(ctx, ctx->wKeys, 4 * Nn);
   ENSURE(ctx, ctx->cstart, 4 * ((size_t)n_cells + 1));
+  ENSURE(ctx, ctx->spos, 16 * Nn);
   float* vpos = (float*)ctx->fireV.p;
   float* vrot = vpos + Nn * D;
   float* zt = vrot + Nn * A;
@@ -1468,6 +1469,7 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
   a.k = p->k;
   a.keys = (uint32_t*)ctx->wKeys.p;
   a.starts = (int32_t*)ctx->cstart.p;
+  a.sp = (float4*)ctx->spos.p;
   a.force = p->force;
   a.e_out = per;
   a.w_out = nullptr;
@@ -1485,6 +1487,7 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
       sorted = true;
     }
     a.w_out = with_virial ? wper : nullptr;
+    HIPCHK(ctx, launch_prec_gather(a, st));
     HIPCHK(ctx, launch_prec_force(a, st));
     HIPCHK(ctx, hipMemsetAsync(&f.st->max_grad_bits, 0, sizeof(unsigned), st));
     HIPCHK(ctx, launch_fire_grad_max(f, st));

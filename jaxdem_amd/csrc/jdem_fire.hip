@@ -194,10 +194,18 @@ __global__ void __launch_bounds__(kBlock) k_fire_grad_max(FireArgs a) {
     for (int c = 0; c < A; ++c)
       if (a.torque[A * i + c] != a.torque[A * i + c]) m = __int_as_float(0x7fc00000);
   }
-  // non-negative floats (and the quiet NaN above) order like their bits
+  // non-negative floats (and the quiet NaN above) order like their bits;
+  // one atomic per block
+  __shared__ unsigned wmax[kBlock / 64];
   unsigned bits = (unsigned)__float_as_int(m);
   for (int off = 32; off > 0; off >>= 1) bits = max(bits, (unsigned)__shfl_xor((int)bits, off));
-  if ((threadIdx.x & 63) == 0) atomicMax(&a.st->max_grad_bits, bits);
+  if ((threadIdx.x & 63) == 0) wmax[threadIdx.x >> 6] = bits;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    unsigned b = wmax[0];
+    for (int w = 1; w < kBlock / 64; ++w) b = max(b, wmax[w]);
+    atomicMax(&a.st->max_grad_bits, b);
+  }
 }
 
 template <int D>
@@ -229,6 +237,16 @@ __global__ void __launch_bounds__(kBlock) k_prec_keys(PrecArgs a) {
   a.keys[i] = D == 3 ? (uint32_t)((c[0] * a.dims[1] + c[1]) * a.dims[2] + c[2]) : (uint32_t)(c[0] * a.dims[1] + c[1]);
 }
 
+// (offset, radius) of every sphere in cell order, so the stencil reads each
+// cell's candidates as one contiguous run
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_prec_gather(PrecArgs a) {
+  const int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (kk >= a.n) return;
+  const int64_t j = a.perm[kk];
+  a.sp[kk] = make_float4(a.off[D * j], a.off[D * j + 1], D == 3 ? a.off[D * j + 2] : 0.0f, a.rad[j]);
+}
+
 // spring law (spring.py:97-108, 138-147) on the cell stencil; float32 pair
 // arithmetic, float64 energy and virial
 template <int D>
@@ -258,14 +276,15 @@ __global__ void __launch_bounds__(kBlock) k_prec_force(PrecArgs a) {
                                  : (int64_t)cc[0] * a.dims[1] + cc[1];
         const int k1 = a.starts[h + 1];
         for (int kk = a.starts[h]; kk < k1; ++kk) {
-          const int64_t j = a.perm[kk];
-          if (j == i) continue;
+          if (a.perm[kk] == i) continue;
+          const float4 sj = a.sp[kk];
+          const float osj[3] = {sj.x, sj.y, sj.z};
           // r = rh + rl, both float32: two_sum of the cell term (exact, the
           // stencil offset is -1, 0 or 1) and the offset difference, whose
           // own rounding error two_sum recovers too
           float r[3] = {0, 0, 0}, rl[3] = {0, 0, 0};
           for (int q = 0; q < D; ++q) {
-            const float oj = a.off[D * j + q];
+            const float oj = osj[q];
             const float b = oi[q] - oj;
             const float bv = b - oi[q];
             const float be = (oi[q] - (b - bv)) + (-oj - bv);
@@ -277,7 +296,7 @@ __global__ void __launch_bounds__(kBlock) k_prec_force(PrecArgs a) {
             rl[q] = (te + be) + (float)(-o[q]) * a.hlo[q];
           }
           const float d2 = D == 3 ? (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
-          const float R = ri + a.rad[j];
+          const float R = ri + sj.w;
           if (!(d2 < R * R) || d2 == 0.0f) continue;
           const float inv = 1.0f / sqrtf(d2);
           // delta = (R^2 - |r|^2) / (R + |r|) with |r|^2 = sum rh^2 + 2 rh rl
@@ -351,6 +370,14 @@ hipError_t launch_prec_keys(const PrecArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_prec_keys<3>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
   else
     hipLaunchKernelGGL(k_prec_keys<2>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_prec_gather(const PrecArgs& a, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  if (a.dim == 3)
+    hipLaunchKernelGGL(k_prec_gather<3>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_prec_gather<2>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 hipError_t launch_prec_force(const PrecArgs& a, hipStream_t s) {

@@ -453,12 +453,14 @@ struct PrecArgs {
   uint32_t* keys;        // cell key of each sphere
   const int32_t* perm;   // cell-sorted position -> sphere
   const int32_t* starts; // first sorted position of each cell (n_cells + 1)
+  float4* sp;            // cell-sorted (offset, radius), gathered per evaluation
   float* force;          // (N, D)
   double* e_out;         // (N) half the pair energies of each sphere
   double* w_out;         // (N, D*D) half the pair virials, or null
   int32_t* moved;        // set when a sphere changed cell
 };
 hipError_t launch_prec_keys(const PrecArgs& a, hipStream_t s);
+hipError_t launch_prec_gather(const PrecArgs& a, hipStream_t s);
 hipError_t launch_prec_force(const PrecArgs& a, hipStream_t s);
 hipError_t launch_prec_move(const FireArgs& f, const PrecArgs& a, hipStream_t s);
 
