@@ -312,6 +312,29 @@ int jdem_step_stage(jdem_ctx* ctx, jdem_system* sys, int32_t stage);
  * NeighborList collider.  stats may be NULL. */
 int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats* stats);
 
+/* Frames of a rollout: frame f is the state after steps[0] + ... + steps[f]
+ * steps.  Each array holds n_frames consecutive copies of the matching state
+ * array (pos/vel/force: n*dim floats a frame; ang_vel/torque: n*(1 or 3);
+ * q: n*4), device memory.  steps is a HOST array. */
+typedef struct jdem_frames {
+  int64_t n_frames;
+  const int64_t* steps;
+  float* pos;
+  float* vel;
+  float* ang_vel;
+  float* q;
+  float* force;
+  float* torque;
+} jdem_frames;
+
+/* System.trajectory_rollout(state, system, n, stride/strides, save_fn)
+ * (system.py:607-700, scan body system.py:101-120) with the state saved at
+ * every frame: n_frames segments of System.step, each segment's end state
+ * written to its frame without leaving the resident working set in between.
+ * NeighborList collider only, not with a FREE domain.  The state and
+ * sys->nlist end as after jdem_system_step(sum of steps). */
+int jdem_rollout(jdem_ctx* ctx, jdem_system* sys, const jdem_frames* frames, jdem_step_stats* stats);
+
 /* ---- (d) batches of independent systems --------------------------------------- */
 
 /* A segmented batch: n_systems independent systems whose particles are

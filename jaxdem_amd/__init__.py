@@ -30,7 +30,7 @@ from .integrators import (
 )
 from .materials import Material, MaterialMatchmaker, MaterialTable
 from .state import Quaternion, State
-from .system import System
+from .system import System, save_state
 from . import batching, minimizers, precision, serving, utils
 from .batching import Batch
 from .minimizers import fire, minimize
@@ -67,6 +67,7 @@ __all__ = [
     "State",
     "Quaternion",
     "System",
+    "save_state",
     "Batch",
     "batching",
     "serving",

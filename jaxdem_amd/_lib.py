@@ -138,6 +138,19 @@ class JdemStepStats(ctypes.Structure):
     ]
 
 
+class JdemFrames(ctypes.Structure):
+    _fields_ = [
+        ("n_frames", c_int64),
+        ("steps", POINTER(c_int64)),
+        ("pos", c_void_p),
+        ("vel", c_void_p),
+        ("ang_vel", c_void_p),
+        ("q", c_void_p),
+        ("force", c_void_p),
+        ("torque", c_void_p),
+    ]
+
+
 class JdemTiming(ctypes.Structure):
     _fields_ = [
         ("step_kernel_ms", c_double),
@@ -244,6 +257,7 @@ SIGNATURES = {
     "jdem_kinetic_energy": (c_int, [c_void_p, POINTER(JdemSystem), POINTER(c_double)]),
     "jdem_step_stage": (c_int, [c_void_p, POINTER(JdemSystem), c_int32]),
     "jdem_system_step": (c_int, [c_void_p, POINTER(JdemSystem), c_int64, POINTER(JdemStepStats)]),
+    "jdem_rollout": (c_int, [c_void_p, POINTER(JdemSystem), POINTER(JdemFrames), POINTER(JdemStepStats)]),
     "jdem_batch_step": (c_int, [c_void_p, POINTER(JdemSystem), POINTER(JdemBatch), c_int64, POINTER(JdemStepStats)]),
     "jdem_precise_minimize": (
         c_int,

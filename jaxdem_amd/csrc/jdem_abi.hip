@@ -448,7 +448,12 @@ struct SegRun {
   bool any_unbuilt;   // some system has no list yet
 };
 
-int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats* S, const SegRun* sg = nullptr) {
+// n steps of a NeighborList system on the resident working set.  With
+// frames, the steps are split into fr->n_frames segments of fr->steps[f]
+// steps, each ended as System.step ends, and the state after segment f is
+// written to frame f (caller order) before the next segment starts.
+int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats* S, const SegRun* sg = nullptr,
+                     const jdem_frames* fr = nullptr) {
   jdem_state& s = sys->state;
   jdem_nlist& L = sys->nlist;
   const int64_t N = s.n;
@@ -719,13 +724,13 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   };
 
   int32_t* flags = (int32_t*)ctx->flags.p;
-  HIPCHK(ctx, launch_fill_i32(flags, 4, 0, st));
   float fbox[3] = {0, 0, 0}, fanc[3] = {0, 0, 0};
   bool free_done = false;
-  std::vector<int> cur_at((size_t)n + 2, 0), xc_at((size_t)n + 2, 0);
+  std::vector<int> cur_at, xc_at;
+  int64_t nseg = n;  // steps of the current segment
 
   auto launch_t = [&](int64_t t) -> int {
-    if (!sg && sys->domain.kind == JDEM_DOMAIN_FREE && t == n - 1 && !free_done) {
+    if (!sg && sys->domain.kind == JDEM_DOMAIN_FREE && t == nseg - 1 && !free_done) {
       const int nparts = 64;
       ENSURE(ctx, ctx->red, 4 * 6 * (size_t)nparts);
       HIPCHK(ctx, ws_launch_box(F4(ctx->wP[cur]), N, (float*)ctx->red.p, nparts, st));
@@ -779,8 +784,8 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     a.live = U64(ctx->wLive[nlc]);
     a.do_force = t >= 1;
     a.do_after = t >= 1;
-    a.do_before = t < n;
-    a.store_ft = t == n;
+    a.do_before = t < nseg;
+    a.store_ft = t == nseg;
     a.trigger = nl_trigger(L);
     a.check_disp = a.do_before;
     a.flag_rd = t >= 1 ? flags + (t & 1) : nullptr;
@@ -800,6 +805,12 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     return JDEM_OK;
   };
 
+  bool rebuilt_here = false;
+  auto run_segment = [&](int64_t ns) -> int {
+  nseg = ns;
+  cur_at.assign((size_t)ns + 2, 0);
+  xc_at.assign((size_t)ns + 2, 0);
+  HIPCHK(ctx, launch_fill_i32(flags, 4, 0, st));
   int rc = launch_t(0);
   if (rc) return rc;
   bool pending = sg ? sg->any_unbuilt : L.n_build_times == 0;
@@ -807,8 +818,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   int64_t next = 1;
   int64_t chunk_len = 8;
   int64_t last_rebuild_at = 0;
-  bool rebuilt_here = false;
-  while (next <= n) {
+  while (next <= ns) {
     if (pending) {
       rc = rebuild();
       if (rc) return rc;
@@ -819,7 +829,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       last_rebuild_at = next;
       pending = false;
     }
-    const int64_t cnt = std::min<int64_t>(chunk_len, n - next + 1);
+    const int64_t cnt = std::min<int64_t>(chunk_len, ns - next + 1);
     const size_t chunk_spans = ctx->spans.size();
     for (int64_t t = next; t < next + cnt; ++t) {
       rc = launch_t(t);
@@ -846,6 +856,44 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       }
       next = v;
       pending = true;
+    }
+  }
+  return JDEM_OK;
+  };
+
+  // the state after each segment, written to its frame in caller order
+  auto export_frame = [&](int64_t f) -> int {
+    const int A = D == 3 ? 3 : 1;
+    InOut fo = io;
+    fo.pos = fr->pos + (size_t)f * N * D;
+    fo.vel = fr->vel + (size_t)f * N * D;
+    fo.ang = fr->ang_vel + (size_t)f * N * A;
+    fo.q = fr->q + (size_t)f * N * 4;
+    fo.force = fr->force + (size_t)f * N * D;
+    fo.torque = fr->torque + (size_t)f * N * A;
+    fo.old_user = nullptr;
+    fo.orig = I32(ctx->wOrig[set]);
+    fo.P4 = F4(ctx->wP[cur]);
+    fo.V4 = F4(ctx->wV[cur]);
+    fo.W4 = F4(ctx->wW[cur]);
+    fo.Q4 = F4(ctx->wQ[set]);
+    fo.I4 = F4(ctx->wI[set]);
+    fo.old4 = F4(ctx->wOld);
+    HIPCHK(ctx, ws_launch_out(fo, false, st));
+    return JDEM_OK;
+  };
+  int rc = JDEM_OK;
+  if (!fr) {
+    rc = run_segment(n);
+    if (rc) return rc;
+  } else {
+    for (int64_t f = 0; f < fr->n_frames; ++f) {
+      if (fr->steps[f] > 0) {
+        rc = run_segment(fr->steps[f]);
+        if (rc) return rc;
+      }
+      rc = export_frame(f);
+      if (rc) return rc;
     }
   }
   if (rebuilt_here) L.overflow = ctx->host_flags[2] != 0;
@@ -1722,6 +1770,36 @@ int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   }
   S->steps = n;
   return JDEM_OK;
+}
+
+int jdem_rollout(jdem_ctx* ctx, jdem_system* sys, const jdem_frames* fr, jdem_step_stats* stats) {
+  int rc = check_system(ctx, sys);
+  if (rc) return rc;
+  if (!fr || fr->n_frames < 0 || (fr->n_frames > 0 && !fr->steps)) return fail(ctx, JDEM_EINVAL, "bad frames");
+  if (sys->collider_kind != JDEM_COLLIDER_NEIGHBOR_LIST)
+    return fail(ctx, JDEM_EUNSUPPORTED, "rollout frames need the NeighborList collider");
+  if (sys->domain.kind == JDEM_DOMAIN_FREE) return fail(ctx, JDEM_EUNSUPPORTED, "rollout frames with a FreeDomain");
+  int64_t n = 0;
+  for (int64_t f = 0; f < fr->n_frames; ++f) {
+    if (fr->steps[f] < 0) return fail(ctx, JDEM_EINVAL, "negative frame stride");
+    n += fr->steps[f];
+  }
+  jdem_state& s = sys->state;
+  if (fr->n_frames > 0 && s.n > 0 &&
+      (!fr->pos || !fr->vel || !fr->ang_vel || !fr->q || !fr->force || !fr->torque))
+    return fail(ctx, JDEM_EINVAL, "null frame array");
+  jdem_step_stats local;
+  std::memset(&local, 0, sizeof(local));
+  jdem_step_stats* S = stats ? stats : &local;
+  std::memset(S, 0, sizeof(*S));
+  if (s.n == 0 || fr->n_frames == 0) {
+    S->steps = n;
+    return JDEM_OK;
+  }
+  (void)hipSetDevice(ctx->device);
+  rc = ensure_flags(ctx);
+  if (rc) return rc;
+  return resident_nl_step(ctx, sys, n, S, nullptr, fr);
 }
 
 }  // extern "C"

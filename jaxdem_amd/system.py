@@ -45,13 +45,22 @@ from .integrators import (
     VelocityVerletSpiral,
 )
 from .materials import Material, MaterialTable
-from .state import State
+from .state import Quaternion, State
 
 F32 = np.float32
 
 
 def _save_state_system(state, system):
     return state, system
+
+
+def save_state(state, system):
+    """A ``save_fn`` for ``System.trajectory_rollout`` that keeps the state
+    of every frame and not the system.  With the fused step (built-in
+    integrators, domain other than FreeDomain, NeighborList collider, no
+    user hooks) the frames are written on the device by one
+    ``jdem_rollout`` call, with no host round trip between frames."""
+    return state
 
 
 @dataclass
@@ -233,6 +242,17 @@ class System:
             if n is None:
                 raise ValueError("`n` must be provided when `strides` is None.")
             ks = [stride] * int(n)
+        if (
+            save_fn is save_state
+            and ks
+            and min(ks) >= 0
+            and state.batch_size == 1
+            and state.N > 0
+            and isinstance(system.collider, NeighborList)
+            and not isinstance(system.domain, FreeDomain)
+            and _fusable(state, system)
+        ):
+            return _fused_rollout(state, system, ks)
         frames = []
         for k in ks:
             state, system = System.step(state, system, n=k)
@@ -452,6 +472,59 @@ def _fused_step(state: State, system: System, n: int) -> tuple:
     return new_state, out_sys
 
 
+def _fused_rollout(state: State, system: System, ks: list) -> tuple:
+    """trajectory_rollout with ``save_state`` frames in one jdem_rollout call:
+    segment f runs System.step(ks[f]) on the resident working set and its
+    end state is written to frame f."""
+    new_state = state.with_kinematics()
+    col = _fresh_collider(system.collider)
+    sys2 = replace(system, collider=col, domain=replace(system.domain))
+    keep: list = []
+    S = _system_struct(new_state, sys2, keep)
+    nf = len(ks)
+
+    def frames_like(t: torch.Tensor) -> torch.Tensor:
+        return torch.empty((nf,) + tuple(t.shape), dtype=t.dtype, device=t.device)
+
+    out = {
+        "pos_c": frames_like(new_state.pos_c),
+        "vel": frames_like(new_state.vel),
+        "ang_vel": frames_like(new_state.ang_vel),
+        "q": frames_like(new_state.q.data),
+        "force": frames_like(new_state.force),
+        "torque": frames_like(new_state.torque),
+    }
+    steps = (ctypes.c_int64 * nf)(*ks)
+    fr = L.JdemFrames()
+    fr.n_frames = nf
+    fr.steps = ctypes.cast(steps, ctypes.POINTER(ctypes.c_int64))
+    fr.pos = out["pos_c"].data_ptr()
+    fr.vel = out["vel"].data_ptr()
+    fr.ang_vel = out["ang_vel"].data_ptr()
+    fr.q = out["q"].data_ptr()
+    fr.force = out["force"].data_ptr()
+    fr.torque = out["torque"].data_ptr()
+    stats = L.JdemStepStats()
+    ctx = _ctx(new_state)
+    ctx.check(ctx.lib.jdem_rollout(ctx.ptr, ctypes.byref(S), ctypes.byref(fr), ctypes.byref(stats)), "jdem_rollout")
+    kw = {}
+    for f in dataclasses.fields(new_state):
+        v = getattr(new_state, f.name)
+        if f.name in out:
+            kw[f.name] = Quaternion(out["q"]) if f.name == "q" else out[f.name]
+        elif isinstance(v, torch.Tensor):
+            kw[f.name] = v.unsqueeze(0).expand((nf,) + tuple(v.shape)).contiguous()
+        else:
+            kw[f.name] = v
+    traj = State(**kw)
+    dom = replace(sys2.domain, inv_box_size=(F32(1.0) / np.asarray(sys2.domain.box_size, F32)).astype(F32))
+    fm = replace(sys2.force_manager, external_force=None, external_force_com=None, external_torque=None)
+    out_sys = replace(sys2, collider=_collider_after(col, S), domain=dom, force_manager=fm)
+    out_sys = _advance_clock(out_sys, sum(ks))
+    out_sys.__dict__["_last_stats"] = stats
+    return new_state, out_sys, traj
+
+
 def _step_once(state: State, system: System) -> tuple:
     """_step_once (system.py:60-82) stage by stage."""
     system = _advance_clock(system, 1)
@@ -553,4 +626,4 @@ def _nl_refresh(state: State, system: System) -> tuple:
     return state, sys2
 
 
-__all__ = ["System"]
+__all__ = ["System", "save_state"]
