@@ -61,6 +61,8 @@ def parse_args():
     ap.add_argument("--precision-steps", type=int, default=20000,
                     help="FIRE steps timed (0: run to the float32 floor, which stalls after ~1e5 steps)")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--rollout-stride", type=int, default=50, help="steps between saved frames of the rollout line")
+    ap.add_argument("--no-rollout", action="store_true")
     ap.add_argument(
         "--copy-probe-mib",
         type=int,
@@ -296,6 +298,31 @@ def precision_bench(args, dev) -> dict:
     }
 
 
+def rollout_bench(args, st, sy, n_particles: int, plain_value: float) -> dict:
+    """trajectory_rollout of the same workload with a frame saved every
+    --rollout-stride steps: frames written on the device by one jdem_rollout
+    call (save_fn=save_state), and the same frames through one System.step
+    call per frame."""
+    import torch
+
+    import jaxdem_amd as jd
+
+    k = max(1, args.rollout_stride)
+    nf = max(1, args.steps // k)
+    out = {"stride": k, "frames": nf, "steps": nf * k}
+    for name, fn in (("device_frames", jd.save_state), ("step_per_frame", lambda s, y: (s, y))):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        _, _, traj = jd.System.trajectory_rollout(st, sy, n=nf, stride=k, save_fn=fn)
+        torch.cuda.synchronize()
+        secs = time.perf_counter() - t0
+        out[name] = {"particle_steps_per_s": n_particles * nf * k / secs, "ms_per_frame": 1e3 * secs / nf}
+        del traj
+    out["device_frames_vs_plain_step"] = out["device_frames"]["particle_steps_per_s"] / plain_value
+    print(f"[bench] rollout: {out}", file=sys.stderr, flush=True)
+    return out
+
+
 def main():
     args = parse_args()
     if args.gpus != 1:
@@ -401,6 +428,9 @@ def main():
     else:
         out["cpu_baseline"] = None
     del st1, sy1, st2, sy2
+    if not args.no_rollout:
+        out["rollout"] = rollout_bench(args, st, sy, w.N, value)
+        torch.cuda.empty_cache()
     if not args.no_batch:
         out["batched"] = batch_bench(args, dev)
         torch.cuda.empty_cache()
