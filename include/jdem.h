@@ -261,7 +261,8 @@ int jdem_ctx_get_timing(jdem_ctx* ctx, jdem_timing* out);
 
 /* ---- (a) cell list ------------------------------------------------------ */
 
-/* Hash + LDS-staged radix sort + cell start/end: the spatial partition of
+/* Hash + stable sort by cell (a counting sort; an LSD radix sort for grids
+ * much larger than N) + cell start/end: the spatial partition of
  * _get_spatial_partition (cell_list.py:37-87).  perm/sorted_hash are (N,)
  * int32; cell_start/cell_end are (grid->n_cells,) int32 (empty cell:
  * start == end). */

@@ -231,7 +231,7 @@ int build_cells(jdem_ctx* ctx, const float* pos, const float* rad, int64_t n, in
   ENSURE(ctx, ctx->keys[1], 4 * nn);
   ENSURE(ctx, ctx->vals[0], 4 * nn);
   ENSURE(ctx, ctx->vals[1], 4 * nn);
-  ENSURE(ctx, ctx->cstart, 4 * (size_t)g.n_cells);
+  ENSURE(ctx, ctx->cstart, 4 * ((size_t)g.n_cells + 1));
   ENSURE(ctx, ctx->cend, 4 * (size_t)g.n_cells);
   ENSURE(ctx, ctx->spos, 16 * nn);
   ENSURE(ctx, ctx->hist, 4 * 256 * (size_t)((n + 4095) / 4096 + 1));
@@ -239,12 +239,18 @@ int build_cells(jdem_ctx* ctx, const float* pos, const float* rad, int64_t n, in
   HIPCHK(ctx, launch_hash(pos, n, dim, dom, g, hk, ctx->stream));
   uint32_t* sk = nullptr;
   int32_t* perm = nullptr;
-  HIPCHK(ctx, radix_sort_pairs(ctx, hk, n, nbits_for(g.n_cells), &sk, &perm, ctx->stream));
   int32_t* cs = (int32_t*)ctx->cstart.p;
   int32_t* ce = (int32_t*)ctx->cend.p;
-  HIPCHK(ctx, launch_fill_i32(cs, g.n_cells, 0, ctx->stream));
-  HIPCHK(ctx, launch_fill_i32(ce, g.n_cells, 0, ctx->stream));
-  HIPCHK(ctx, launch_cell_ranges(sk, n, cs, ce, ctx->stream));
+  if (use_cell_sort(n, g.n_cells)) {
+    // starts of every cell (an empty cell starts where the next one does)
+    HIPCHK(ctx, cell_sort_pairs(ctx, hk, n, g.n_cells, &sk, &perm, cs, ctx->stream));
+    HIPCHK(ctx, launch_starts_to_ends(cs, g.n_cells, ce, ctx->stream));
+  } else {
+    HIPCHK(ctx, radix_sort_pairs(ctx, hk, n, nbits_for(g.n_cells), &sk, &perm, ctx->stream));
+    HIPCHK(ctx, launch_fill_i32(cs, g.n_cells, 0, ctx->stream));
+    HIPCHK(ctx, launch_fill_i32(ce, g.n_cells, 0, ctx->stream));
+    HIPCHK(ctx, launch_cell_ranges(sk, n, cs, ce, ctx->stream));
+  }
   HIPCHK(ctx, launch_gather_spos(perm, pos, rad, n, dim, (float4*)ctx->spos.p, ctx->stream));
   out->perm = perm;
   out->skeys = (int32_t*)sk;
@@ -507,7 +513,10 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     HIPCHK(ctx, launch_hash(s.pos, N, D, dom, g, (uint32_t*)ctx->wKeys.p, st));
     uint32_t* sk = nullptr;
     int32_t* perm = nullptr;
-    HIPCHK(ctx, radix_sort_pairs(ctx, (uint32_t*)ctx->wKeys.p, N, nbits_for(g.n_cells), &sk, &perm, st));
+    if (use_cell_sort(N, g.n_cells))
+      HIPCHK(ctx, cell_sort_pairs(ctx, (uint32_t*)ctx->wKeys.p, N, g.n_cells, &sk, &perm, I32(ctx->cstart), st));
+    else
+      HIPCHK(ctx, radix_sort_pairs(ctx, (uint32_t*)ctx->wKeys.p, N, nbits_for(g.n_cells), &sk, &perm, st));
     HIPCHK(ctx, hipMemcpyAsync(ctx->wOrig[0].p, perm, ni, hipMemcpyDeviceToDevice, st));
   } else {
     HIPCHK(ctx, ws_launch_iota(I32(ctx->wOrig[0]), N, st));
@@ -670,14 +679,19 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       HIPCHK(ctx, ws_launch_hash(F4(ctx->wP[cur]), I32(ctx->wOrig[set]), N, D, dom, g, (uint32_t*)ctx->wKeys.p,
                                  I32(ctx->wWpos), st));
       int32_t* new_orig = nullptr;
-      HIPCHK(ctx, radix_sort_pairs(ctx, (uint32_t*)ctx->wKeys.p, N, nbits_for(n_cells), &sk, &new_orig, st));
+      // the counting sort also writes the cell starts
+      const bool counted = use_cell_sort(N, n_cells);
+      if (counted)
+        HIPCHK(ctx, cell_sort_pairs(ctx, (uint32_t*)ctx->wKeys.p, N, n_cells, &sk, &new_orig, I32(ctx->cstart), st));
+      else
+        HIPCHK(ctx, radix_sort_pairs(ctx, (uint32_t*)ctx->wKeys.p, N, nbits_for(n_cells), &sk, &new_orig, st));
       HIPCHK(ctx, ws_launch_permute(N, new_orig, I32(ctx->wWpos), F4(ctx->wP[cur]), F4(ctx->wV[cur]),
                                     F4(ctx->wW[cur]), F4(ctx->wQ[set]), F4(ctx->wI[set]), F4(ctx->wP[1 - cur]),
                                     F4(ctx->wV[1 - cur]), F4(ctx->wW[1 - cur]), F4(ctx->wQ[1 - set]),
                                     F4(ctx->wI[1 - set]), I32(ctx->wOrig[1 - set]), src, inv, st));
       cur = 1 - cur;
       set = 1 - set;
-      HIPCHK(ctx, ws_launch_cell_starts(sk, N, n_cells, I32(ctx->cstart), st));
+      if (!counted) HIPCHK(ctx, ws_launch_cell_starts(sk, N, n_cells, I32(ctx->cstart), st));
     } else {
       HIPCHK(ctx, ws_launch_iota(src, N, st));
       HIPCHK(ctx, ws_launch_iota(inv, N, st));
@@ -1529,9 +1543,13 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
       HIPCHK(ctx, launch_prec_keys(a, st));
       uint32_t* sk = nullptr;
       int32_t* perm = nullptr;
-      HIPCHK(ctx, radix_sort_pairs(ctx, a.keys, N, nbits_for(n_cells), &sk, &perm, st));
+      if (use_cell_sort(N, n_cells)) {
+        HIPCHK(ctx, cell_sort_pairs(ctx, a.keys, N, n_cells, &sk, &perm, (int32_t*)ctx->cstart.p, st));
+      } else {
+        HIPCHK(ctx, radix_sort_pairs(ctx, a.keys, N, nbits_for(n_cells), &sk, &perm, st));
+        HIPCHK(ctx, ws_launch_cell_starts(sk, N, n_cells, (int32_t*)ctx->cstart.p, st));
+      }
       a.perm = perm;
-      HIPCHK(ctx, ws_launch_cell_starts(sk, N, n_cells, (int32_t*)ctx->cstart.p, st));
       sorted = true;
     }
     a.w_out = with_virial ? wper : nullptr;

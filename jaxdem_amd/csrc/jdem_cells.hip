@@ -300,6 +300,101 @@ hipError_t radix_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, i
 }
 
 // ---------------------------------------------------------------------------
+// counting sort by cell.  Placement through atomic cursors puts a cell's ids
+// in arbitrary order inside the cell's range; the rank pass then writes each
+// id at its cell start + the number of smaller ids in the cell, which is the
+// order of a stable sort of (key, id).  Cells hold a few particles, so the
+// rank pass reads a handful of ids per element.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(kBlock) k_cs_zero(int32_t* __restrict__ cnt, int32_t* __restrict__ cur,
+                                                     int64_t n_cells) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c > n_cells) return;
+  cnt[c] = 0;
+  if (c < n_cells) cur[c] = 0;
+}
+
+__global__ void __launch_bounds__(kBlock) k_cs_count(const uint32_t* __restrict__ keys, int64_t n,
+                                                      int32_t* __restrict__ cnt) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  atomicAdd(&cnt[keys[i]], 1);
+}
+
+__global__ void __launch_bounds__(kBlock) k_cs_place(const uint32_t* __restrict__ keys, int64_t n,
+                                                      const int32_t* __restrict__ starts, int32_t* __restrict__ cur,
+                                                      uint32_t* __restrict__ tkey, int32_t* __restrict__ tid) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const uint32_t k = keys[i];
+  const int32_t slot = starts[k] + atomicAdd(&cur[k], 1);
+  tkey[slot] = k;
+  tid[slot] = (int32_t)i;
+}
+
+__global__ void __launch_bounds__(kBlock) k_cs_rank(const uint32_t* __restrict__ tkey, const int32_t* __restrict__ tid,
+                                                     int64_t n, const int32_t* __restrict__ starts,
+                                                     uint32_t* __restrict__ sk, int32_t* __restrict__ perm) {
+  const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (j >= n) return;
+  const uint32_t k = tkey[j];
+  const int32_t id = tid[j];
+  const int32_t s = starts[k], e = starts[k + 1];
+  int32_t r = 0;
+  for (int32_t m = s; m < e; ++m) r += tid[m] < id ? 1 : 0;
+  sk[s + r] = k;
+  perm[s + r] = id;
+}
+
+__global__ void __launch_bounds__(kBlock) k_cs_ends(const int32_t* __restrict__ starts, int64_t n_cells,
+                                                     int32_t* __restrict__ cend) {
+  const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n_cells) cend[c] = starts[c + 1];
+}
+hipError_t launch_starts_to_ends(const int32_t* starts, int64_t n_cells, int32_t* cend, hipStream_t s) {
+  if (n_cells <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_cs_ends, dim3(nblocks(n_cells)), dim3(kBlock), 0, s, starts, n_cells, cend);
+  return hipGetLastError();
+}
+
+// keys_in may be ctx->keys[1]: it is read only before the rank pass writes
+// the sorted keys there.
+hipError_t cell_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, int64_t n_cells, uint32_t** keys_out,
+                           int32_t** vals_out, int32_t* starts, hipStream_t s) {
+  const int64_t total = n_cells + 1;
+  const int nchunks = (int)((total + SC_CHUNK - 1) / SC_CHUNK);
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  if (!ensure(ctx, ctx->keys[0], sizeof(uint32_t) * nn) || !ensure(ctx, ctx->keys[1], sizeof(uint32_t) * nn) ||
+      !ensure(ctx, ctx->vals[0], sizeof(int32_t) * nn) || !ensure(ctx, ctx->vals[1], sizeof(int32_t) * nn) ||
+      !ensure(ctx, ctx->ccur, sizeof(int32_t) * (size_t)(n_cells > 0 ? n_cells : 1)) ||
+      !ensure(ctx, ctx->hist, sizeof(uint32_t) * (size_t)(nchunks + 1)))
+    return hipErrorOutOfMemory;
+  uint32_t* tkey = (uint32_t*)ctx->keys[0].p;
+  int32_t* tid = (int32_t*)ctx->vals[0].p;
+  uint32_t* sk = (uint32_t*)ctx->keys[1].p;
+  int32_t* perm = (int32_t*)ctx->vals[1].p;
+  int32_t* cur = (int32_t*)ctx->ccur.p;
+  hipLaunchKernelGGL(k_cs_zero, dim3(nblocks(total)), dim3(kBlock), 0, s, starts, cur, n_cells);
+  if (n > 0) hipLaunchKernelGGL(k_cs_count, dim3(nblocks(n)), dim3(kBlock), 0, s, keys_in, n, starts);
+  uint32_t* cnt = (uint32_t*)starts;  // counts are non-negative: scan them as uint32
+  if (nchunks <= 1) {
+    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, cnt, total);
+  } else {
+    uint32_t* sums = (uint32_t*)ctx->hist.p;
+    hipLaunchKernelGGL(k_scan_totals, dim3(nchunks), dim3(kBlock), 0, s, cnt, total, sums);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, nchunks);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nchunks), dim3(kBlock), 0, s, cnt, total, sums);
+  }
+  if (n > 0) {
+    hipLaunchKernelGGL(k_cs_place, dim3(nblocks(n)), dim3(kBlock), 0, s, keys_in, n, starts, cur, tkey, tid);
+    hipLaunchKernelGGL(k_cs_rank, dim3(nblocks(n)), dim3(kBlock), 0, s, tkey, tid, n, starts, sk, perm);
+  }
+  *keys_out = sk;
+  *vals_out = perm;
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // cell tables
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_fill_i32(int32_t* __restrict__ p, int64_t n, int32_t v) {

@@ -40,6 +40,7 @@ struct jdem_ctx {
   // scratch, grown on demand and reused across calls
   jdem::Buf pos2, vel2, ang2;          // ping-pong kinematics
   jdem::Buf keys[2], vals[2], hist;    // radix sort
+  jdem::Buf ccur;                      // counting sort: per-cell cursors
   jdem::Buf cstart, cend, spos;        // cell tables, sorted (x, r)
   jdem::Buf nl_new, xi_new, live_new;  // neighbour-list rebuild
   jdem::Buf flags;                     // small device words
@@ -140,6 +141,16 @@ hipError_t launch_hash(const float* pos, int64_t n, int dim, const DomP& dom, co
 // Sort (keys, iota) by key; results in (keys_out, vals_out).  nbits = key bits.
 hipError_t radix_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, int nbits, uint32_t** keys_out,
                             int32_t** vals_out, hipStream_t s);
+// The same result as radix_sort_pairs for keys < n_cells, by counting:
+// per-cell counts, their exclusive scan (written to starts[0..n_cells],
+// starts[n_cells] = n), placement through per-cell cursors, then each
+// element's rank among its cell's ids.  Cost grows with n + n_cells.
+hipError_t cell_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, int64_t n_cells, uint32_t** keys_out,
+                           int32_t** vals_out, int32_t* starts, hipStream_t s);
+// cend[c] = starts[c + 1] for c < n_cells
+hipError_t launch_starts_to_ends(const int32_t* starts, int64_t n_cells, int32_t* cend, hipStream_t s);
+// whether cell_sort_pairs is the cheaper sort for n keys below n_cells
+inline bool use_cell_sort(int64_t n, int64_t n_cells) { return n > 0 && n_cells <= 4 * n + 65536; }
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
 hipError_t launch_cell_ranges(const uint32_t* skeys, int64_t n, int32_t* cstart, int32_t* cend, hipStream_t s);
 hipError_t launch_gather_spos(const int32_t* perm, const float* pos, const float* rad, int64_t n, int dim,
