@@ -1012,7 +1012,10 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
       if (active)
 #pragma unroll
         for (int w = 0; w < MW; ++w)
-          if (w < n_words) a.live[(int64_t)w * a.n + b] = ln[w];
+          if (w < n_words) {  // most words are unchanged from the last step: skip their stores
+            const int64_t li = (int64_t)w * a.n + b;
+            if (a.live[li] != ln[w]) a.live[li] = ln[w];
+          }
   } else {
     F = xyz<D>(a.F[b]);
     T = ang4<D>(a.T[b]);
