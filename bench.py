@@ -39,7 +39,10 @@ def parse_args():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=100)
     ap.add_argument("--n", type=int, default=1 << 20)
-    ap.add_argument("--skin", type=float, default=0.1)
+    ap.add_argument("--skin", type=float, default=None,
+                    help="NeighborList skin of the plain workload (default: autotuned over --plain-skins)")
+    ap.add_argument("--plain-skins", type=str, default="0.1,0.15,0.2",
+                    help="candidate skins of the plain-workload autotuner")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="budget of the CPU-baseline sample")
     ap.add_argument("--no-cpu-baseline", action="store_true")
@@ -339,7 +342,16 @@ def main():
     from jaxdem_amd import workloads as W
 
     dev = torch.device("cuda:0")
-    w = W.spheres_friction(N=args.n, seed=args.seed, skin=args.skin)
+    skin, skin_tune = args.skin, None
+    if skin is None:
+        from jaxdem_amd.tuning import autotune_skin_system
+
+        def make_pair(sk):
+            return W.make_gpu_system(W.spheres_friction(N=args.n, seed=args.seed, skin=sk), device=dev)
+
+        skin, skin_tune = autotune_skin_system(make_pair, [float(x) for x in args.plain_skins.split(",") if x])
+        print(f"[bench] plain autotuner: skin {skin} from {skin_tune}", file=sys.stderr, flush=True)
+    w = W.spheres_friction(N=args.n, seed=args.seed, skin=skin)
     st, sy = W.make_gpu_system(w, device=dev)
     print(f"[bench] N={w.N} dt={w.dt:.4e} M={sy.collider.max_neighbors} warmup={args.warmup}", file=sys.stderr, flush=True)
     if args.warmup > 0:
@@ -393,11 +405,13 @@ def main():
         "config": {
             "workload": f"spheres_{w.N}_friction: 2^20 bidisperse frictional spheres (r 0.5/0.7), periodic, phi 0.50",
             "n_particles": w.N,
-            "collider": "NeighborList(CellList cell 1.4)",
+            "collider": f"NeighborList(CellList cell 1.4, skin {skin:g})",
             "law": "springfriction (k_n 1e4, k_t 2/7 k_n, mu 0.5, damping ratio 0.1)",
             "integrators": "verlet + verletspiral",
             "dt": w.dt,
             "skin": float(sy.collider.skin),
+            "skin_autotuner": {"candidates_skin_value_rebuilds_M": skin_tune,
+                               "probe": "200 steps after 20, System.step"} if skin_tune else None,
             "max_neighbors": int(sy.collider.max_neighbors),
         },
         "roofline": {
@@ -418,6 +432,7 @@ def main():
             "rebuilds": timing["rebuild_ms"] / args.steps,
             "n_rebuilds": timing["rebuilds"],
             "host_syncs": int(stats.host_syncs) if stats is not None else None,
+            "list_overflow": bool(stats.overflow) if stats is not None else None,
         },
         "contact_evals_per_s": live_contacts(sy1) * args.steps / secs,
         "candidate_tests_per_s": listed_pairs(sy1) * args.steps / secs,
