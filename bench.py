@@ -57,6 +57,8 @@ def parse_args():
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=16.0, help="Poisson arrival rate, requests per second")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-max-active", type=int, default=4, help="requests in the serving batch at most")
+    ap.add_argument("--serve-policy", type=str, default="fcfs", help="serving admission: fcfs or sjf")
     ap.add_argument("--serve-loads", type=str, default="",
                     help="also sweep offered loads, as fractions of the one-at-a-time capacity (e.g. 0.3,0.6,0.9)")
     ap.add_argument("--serve-sweep-requests", type=int, default=24)
@@ -226,7 +228,7 @@ def serving_bench(args, dev) -> dict:
     work = sum(r.n * r.steps for r in reqs)
     print(f"[bench] serving: {len(reqs)} requests, {work:.3e} particle-steps, rate {args.serve_rate}/s",
           file=sys.stderr, flush=True)
-    server = SV.Server(dev, chunk=50)
+    server = SV.Server(dev, chunk=50, max_active=args.serve_max_active, policy=args.serve_policy)
     batched = SV.latency_stats(server.run(reqs, realtime=True))
     print(f"[bench] serving batched: {batched}", file=sys.stderr, flush=True)
     plain = SV.latency_stats(SV.run_one_at_a_time(reqs, dev, realtime=True))
@@ -235,6 +237,7 @@ def serving_bench(args, dev) -> dict:
         "workload": f"serving_requests: {len(reqs)} Poisson arrivals at {args.serve_rate}/s, N log-uniform in "
         "[1e3, 2e5], steps U[100, 10000], log-normal radii (mean 0.5, sigma 0.2), periodic/reflect, "
         "phi U[0.3, 0.55], spring + friction mu U[0, 0.5]",
+        "server": {"policy": args.serve_policy, "max_active": args.serve_max_active, "chunk_ms": 20},
         "continuous_batching": batched,
         "one_at_a_time": plain,
         "throughput_ratio": batched["particle_steps_per_s"] / plain["particle_steps_per_s"],
@@ -252,7 +255,8 @@ def serving_bench(args, dev) -> dict:
                 st, sy = W.make_gpu_system(w, device=dev)
                 rs.append(SV.Request(k, st, sy, steps, arrival=t))
             torch.cuda.synchronize()
-            b = SV.latency_stats(SV.Server(dev, chunk=50).run(rs, realtime=True))
+            b = SV.latency_stats(SV.Server(dev, chunk=50, max_active=args.serve_max_active,
+                                           policy=args.serve_policy).run(rs, realtime=True))
             p = SV.latency_stats(SV.run_one_at_a_time(rs, dev, realtime=True))
             print(f"[bench] serving load {load}: rate {rate:.2f}/s batched {b} one at a time {p}", file=sys.stderr,
                   flush=True)

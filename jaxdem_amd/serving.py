@@ -4,8 +4,9 @@ arrive and advanced together on the GPU (continuous batching).
 Every active request is one system of a segmented batch
 (``jaxdem_amd.batching``); its state, cell list and tangential-spring
 history stay in device memory between step chunks.  At each chunk boundary
-finished requests leave and arrived requests are admitted first come, first
-served, while their device footprint fits the memory budget.  A chunk runs
+finished requests leave and arrived requests are admitted (first come first
+served, or least work first) while their device footprint fits the memory
+budget and the batch holds fewer than ``max_active`` requests.  A chunk runs
 until the next request finishes, so no request runs past its own step
 count, and for about ``chunk_ms`` of GPU time (estimated from the previous
 chunks' time per step; ``chunk`` steps for the first): arrivals wait at most
@@ -67,10 +68,22 @@ def footprint_bytes(state, system) -> int:
 
 
 class Server:
-    """First-come-first-served continuous batching of rollout requests."""
+    """Continuous batching of rollout requests.
+
+    ``policy`` picks which waiting request is admitted next when a place in
+    the batch is free: "fcfs" the earliest arrival, "sjf" the least work
+    (particles x steps; ties by arrival).  ``max_active`` caps the requests
+    in the batch: with a cap, requests share the GPU with a few others
+    instead of with every arrival, and "sjf" lets short requests pass long
+    ones in the queue."""
 
     def __init__(self, device, hbm_budget: int | None = None, chunk: int = 50, max_active: int = 4096,
-                 clock: Callable[[], float] = time.perf_counter, chunk_ms: float = 20.0):
+                 clock: Callable[[], float] = time.perf_counter, chunk_ms: float = 20.0, policy: str = "fcfs"):
+        if policy not in ("fcfs", "sjf"):
+            raise ValueError(f"unknown admission policy {policy!r} (fcfs, sjf)")
+        if int(max_active) < 1:
+            raise ValueError("max_active must be >= 1")
+        self.policy = policy
         self.device = torch.device(device)
         if hbm_budget is None:
             free, _ = torch.cuda.mem_get_info(self.device)
@@ -88,7 +101,7 @@ class Server:
         clock; otherwise every request is present from the start.  Returns
         the requests with their results and timestamps."""
         pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
-        queue: list = []  # arrived, not admitted yet (FIFO)
+        queue: list = []  # arrived, not admitted yet, in arrival order
         active: list = []  # in the batch, in batch order
         batch: Batch | None = None
         t0 = self.clock()
@@ -102,10 +115,14 @@ class Server:
                 queue.append(pending.pop(0))
             admit = []
             while queue and len(active) + len(admit) < self.max_active:
-                fp = footprint_bytes(queue[0].state, queue[0].system)
+                k = 0
+                if self.policy == "sjf":
+                    k = min(range(len(queue)), key=lambda i: (queue[i].n * queue[i].steps, queue[i].arrival,
+                                                             queue[i].rid))
+                fp = footprint_bytes(queue[k].state, queue[k].system)
                 if used + fp > self.hbm_budget and (active or admit):
                     break
-                r = queue.pop(0)
+                r = queue.pop(k)
                 r.admitted = now()
                 r.__dict__["_fp"] = fp
                 used += fp

@@ -81,7 +81,7 @@ def make_system(st, box, domain, law, collider, dt=1e-4, cell_size=None, **kw):
 
 
 # ---------------------------------------------------------------------------
-# (a) cell list: hash + radix sort + cell ranges
+# (a) cell list: hash + stable sort by cell + cell ranges
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("n", [1, 7, 4095, 4096, 4097, 70001])
 @pytest.mark.parametrize("periodic", [True, False])
@@ -305,6 +305,39 @@ def test_trajectory_rollout_frames(device):
     assert torch.equal(t0[0].pos_c[0], st.pos_c)
 
 
+def test_rollout_abi_rejects(device):
+    """jdem_rollout refuses what it does not cover, with an error and no
+    launch: another collider, a negative stride, missing frame arrays."""
+    print("[rollout] abi errors", flush=True)
+    st, box = packed(500, 3, seed=4)
+    sy = make_system(st, box, "periodic", "spring", "CellList")
+    keep: list = []
+    S = jd.system._system_struct(st, sy, keep)
+    w = W.spheres_friction(N=1024, seed=5, skin=0.05)
+    st2, sy2 = W.make_gpu_system(w, device=device)
+    S2 = jd.system._system_struct(st2, sy2, keep)
+    # frames sized for the larger system, so no call below could write past them
+    frames = {k: torch.empty((2,) + tuple(v.shape), device=device)
+              for k, v in (("pos", st2.pos_c), ("vel", st2.vel), ("ang_vel", st2.ang_vel), ("q", st2.q.data),
+                           ("force", st2.force), ("torque", st2.torque))}
+    steps = (ctypes.c_int64 * 2)(3, 4)
+    fr = L.JdemFrames()
+    fr.n_frames = 2
+    fr.steps = ctypes.cast(steps, ctypes.POINTER(ctypes.c_int64))
+    for k, t in frames.items():
+        setattr(fr, k, t.data_ptr())
+    ctx = L.context_for(st.device)
+    with pytest.raises(L.JdemError, match="NeighborList"):
+        ctx.check(ctx.lib.jdem_rollout(ctx.ptr, ctypes.byref(S), ctypes.byref(fr), None), "jdem_rollout")
+    steps[1] = -1
+    with pytest.raises(L.JdemError, match="negative"):
+        ctx.check(ctx.lib.jdem_rollout(ctx.ptr, ctypes.byref(S2), ctypes.byref(fr), None), "jdem_rollout")
+    steps[1] = 4
+    fr.q = None
+    with pytest.raises(L.JdemError, match="null frame"):
+        ctx.check(ctx.lib.jdem_rollout(ctx.ptr, ctypes.byref(S2), ctypes.byref(fr), None), "jdem_rollout")
+
+
 @pytest.mark.parametrize("case", ["spheres_friction", "disks_friction"])
 def test_device_rollout_frames_bitwise(device, case):
     """save_state frames written on the device by jdem_rollout equal, bit for
@@ -522,14 +555,16 @@ def test_batch_split_steps(device):
 # ---------------------------------------------------------------------------
 # (f) serving: continuous batching of rollout requests
 # ---------------------------------------------------------------------------
-@pytest.mark.parametrize("budget", [None, 1])
-def test_serving_results_equal_alone(device, budget):
+@pytest.mark.parametrize("budget,policy", [(None, "fcfs"), (1, "fcfs"), (None, "sjf")])
+def test_serving_results_equal_alone(device, budget, policy):
     """Requests of different sizes, lengths, domains and friction served
-    together (and, with a one-byte budget, one at a time by admission): every
-    result equals the request run alone with System.step, bit for bit."""
+    together (and, with a one-byte budget, one at a time by admission; with
+    least-work-first admission, at most two at a time in order of work):
+    every result equals the request run alone with System.step, bit for
+    bit."""
     from jaxdem_amd import serving as SV
 
-    print(f"[serving] budget={budget}", flush=True)
+    print(f"[serving] budget={budget} policy={policy}", flush=True)
     spec = [(1200, 37, "periodic", 0.4, 0.1), (2500, 80, "reflect", 0.5, 0.3), (900, 15, "reflect", 0.35, 0.0),
             (3000, 64, "periodic", 0.45, 0.45), (1500, 50, "periodic", 0.3, 0.2)]
     reqs = []
@@ -539,11 +574,17 @@ def test_serving_results_equal_alone(device, budget):
             w.box = w.box * np.float32(1.02)
         st, sy = W.make_gpu_system(w, device=device)
         reqs.append(SV.Request(k, st, sy, steps, arrival=0.0))
-    server = SV.Server(device, hbm_budget=budget if budget is not None else 1 << 36, chunk=16)
+    server = SV.Server(device, hbm_budget=budget if budget is not None else 1 << 36, chunk=16, policy=policy,
+                       max_active=2 if policy == "sjf" else 4096)
     done = server.run(reqs, realtime=False)
     assert sorted(r.rid for r in done) == list(range(len(spec)))
     if budget == 1:
         assert max(n for _, n, _, _ in server.log) == 1  # one request at a time
+    elif policy == "sjf":
+        assert max(n for _, n, _, _ in server.log) == 2
+        # the two least-work requests were admitted first
+        first = sorted(range(len(spec)), key=lambda k: spec[k][0] * spec[k][1])[:2]
+        assert sorted(r.rid for r in sorted(done, key=lambda r: r.admitted)[:2]) == sorted(first)
     else:
         assert max(n for _, n, _, _ in server.log) == len(spec)
     for r in done:
