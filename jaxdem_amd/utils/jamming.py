@@ -1,0 +1,135 @@
+"""Jamming by bisection on the packing fraction (utils/jamming.py:28-267,
+utils/packing_utils.py:21-113), driving the GPU ``minimize``.
+
+The protocol (O'Hern et al., PRE 68, 011306 (2003)): minimise; while the
+minimum has no overlaps (potential energy per particle <= ``pe_tol``),
+compress affinely by ``packing_fraction_increment``; once a compression
+jams, bisect between the last unjammed and the first jammed packing
+fraction, restarting each trial from the last unjammed state, until the two
+agree to ``packing_fraction_tolerance``.  Every minimisation is one
+``jdem_minimize`` call on the device; the bisection bookkeeping is host
+arithmetic in float32, as the reference's jitted loop computes it.
+Spheres only: a body is a single sphere, so the affine rescale moves each
+sphere centre (packing_utils.py:91-102 with one sphere per group).
+"""
+
+from __future__ import annotations
+
+from dataclasses import replace
+from typing import Any, NamedTuple
+
+import numpy as np
+
+F32 = np.float32
+
+
+class JamResult(NamedTuple):
+    """Result of ``bisection_jam`` (jamming.py:28-50), in the reference's
+    field order."""
+
+    unjammed_state: Any
+    unjammed_system: Any
+    jammed_state: Any
+    jammed_system: Any
+    packing_fraction: float
+    potential_energy: float
+
+
+def compute_particle_volume(state) -> F32:
+    """Total particle volume (packing_utils.py:21-26; one sphere per body)."""
+    if not state.is_sphere_state:
+        raise NotImplementedError("jamming on the GPU path handles spheres only")
+    return F32(state.volume.sum().item())
+
+
+def compute_packing_fraction(state, system) -> F32:
+    """Particle volume over the box volume, anchor at the origin
+    (packing_utils.py:29-36)."""
+    return F32(compute_particle_volume(state) / F32(np.prod(np.asarray(system.domain.box_size, F32), dtype=F32)))
+
+
+def scale_to_packing_fraction(state, system, new_packing_fraction) -> tuple:
+    """Rescale the box by one common factor, keeping its aspect ratio, and the
+    sphere centres with it (packing_utils.py:67-113).  A NeighborList is
+    marked for a rebuild.  The domain's inverse box size is refreshed with
+    the box (the reference keeps the old one, which changes no contact
+    displacement while the box is wider than twice the contact distance)."""
+    from ..colliders import NeighborList
+
+    box = np.asarray(system.domain.box_size, F32)
+    vol = compute_particle_volume(state)
+    scale = F32(F32(vol / (F32(new_packing_fraction) * F32(np.prod(box, dtype=F32)))) ** F32(1.0 / state.dim))
+    new_box = (box * scale).astype(F32)
+    # offset = com * scale - com, added to the centre (one sphere per body)
+    s = replace(state, pos_c=state.pos_c + (state.pos_c * float(scale) - state.pos_c))
+    dom = replace(system.domain, box_size=new_box, inv_box_size=(F32(1.0) / new_box).astype(F32))
+    sys2 = replace(system, domain=dom)
+    if isinstance(sys2.collider, NeighborList):
+        sys2 = replace(sys2, collider=replace(sys2.collider, n_build_times=0))
+    return s, sys2
+
+
+def bisection_jam(
+    state,
+    system,
+    n_minimization_steps: int = 1000000,
+    pe_tol: float = 1e-16,
+    pe_diff_tol: float = 1e-16,
+    n_jamming_steps: int = 10000,
+    packing_fraction_tolerance: float = 1e-10,
+    packing_fraction_increment: float = 1e-3,
+    verbose: bool = True,
+) -> JamResult:
+    """Nearest jammed state by bisection with state reversion
+    (jamming.py:53-267).  Returns a ``JamResult``; ``packing_fraction`` is
+    the packing fraction of the last trial, as in the reference."""
+    from ..minimizers import minimize
+
+    def relax(s, y):
+        return minimize(s, y, max_steps=n_minimization_steps, pe_tol=pe_tol, pe_diff_tol=pe_diff_tol)
+
+    state, system, n_steps, final_pe = relax(state, system)
+    is_jammed = final_pe > pe_tol
+    if verbose:
+        if is_jammed:
+            print(f"Warning: Initial state is already jammed (PE={final_pe} > tol={pe_tol}). Skipping.")
+        print(f"Initial minimization took {n_steps} steps.")
+    pf = compute_packing_fraction(state, system)
+    pf_low, pf_high = pf, F32(-1.0)
+    cur_state, cur_system = state, system
+    last_state, last_system = state, system
+    inc = F32(packing_fraction_increment)
+    i = 0
+    while i < n_jamming_steps and not is_jammed:
+        s, y, n_steps, final_pe = relax(cur_state, cur_system)
+        if final_pe > pe_tol:  # jammed: revert to the last unjammed state, bisect below this fraction
+            new_pf_high = pf
+            new_pf_low = pf_low
+            new_pf = F32((new_pf_high + pf_low) / F32(2.0))
+            new_state, new_system = last_state, last_system
+        else:  # unjammed: keep it, then bisect (a jammed fraction is known) or compress further
+            last_state, last_system = s, y
+            new_pf_low = pf
+            new_pf_high = pf_high
+            new_pf = F32((pf_high + new_pf_low) / F32(2.0)) if pf_high > 0 else F32(new_pf_low + inc)
+            new_state, new_system = s, y
+        ratio = F32(new_pf_high / new_pf_low)
+        is_jammed = bool(abs(float(ratio) - 1.0) < packing_fraction_tolerance and new_pf_high > 0)
+        if verbose:
+            print(f"Step: {i + 1} -  phi={pf}, PE={final_pe} after {n_steps} steps")
+        cur_state, cur_system = scale_to_packing_fraction(new_state, new_system, new_pf)
+        pf, pf_low, pf_high = new_pf, new_pf_low, new_pf_high
+        i += 1
+    last_jammed_pf = pf_high if pf_high > 0 else pf
+    js, jy = scale_to_packing_fraction(last_state, last_system, last_jammed_pf)
+    js, jy, _, final_pe = relax(js, jy)
+    return JamResult(last_state, last_system, js, jy, float(pf), float(final_pe))
+
+
+__all__ = [
+    "JamResult",
+    "bisection_jam",
+    "compute_packing_fraction",
+    "compute_particle_volume",
+    "scale_to_packing_fraction",
+]

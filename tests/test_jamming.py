@@ -1,0 +1,66 @@
+"""Jamming by bisection (utils/jamming.py:53-267) on the GPU minimiser.
+
+Pinned by a closed form: a triangular lattice of equal disks is the densest
+packing of the plane and stays triangular under an affine compression, so it
+jams at exactly pi / (2 sqrt 3) = 0.9068997 (Thue), to float32 resolution."""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+import jaxdem_amd as jd
+from jaxdem_amd.utils import jamming as J
+
+
+def triangular(nx: int, ny: int, phi: float, device):
+    """nx x ny disks of diameter 1 on a triangular lattice (ny even) filling a
+    periodic box at packing fraction phi."""
+    a = math.sqrt(math.pi / (2.0 * math.sqrt(3.0) * phi))
+    h = a * math.sqrt(3.0) / 2.0
+    pos = np.array([[(i + 0.5 + 0.5 * (j % 2)) * a, (j + 0.5) * h] for j in range(ny) for i in range(nx)], np.float32)
+    n = len(pos)
+    st = jd.State.create(pos=pos, vel=np.zeros_like(pos), rad=np.full(n, 0.5, np.float32),
+                         mass=np.ones(n, np.float32), device=device)
+    box = np.array([nx * a, ny * h], np.float32)
+    return st, box
+
+
+def make(st, box, collider="naive"):
+    mt = jd.MaterialTable.from_materials([jd.Material.create("elastic", density=1.0, young=1e3, poisson=0.3)])
+    ckw = {}
+    if collider == "CellList":
+        ckw = dict(state=st)
+    return jd.System.create(st.shape, dt=1e-3, domain_type="periodic",
+                            domain_kw=dict(box_size=box, anchor=np.zeros(2, np.float32)),
+                            collider_type=collider, collider_kw=ckw, force_model_type="spring", mat_table=mt)
+
+
+def test_scale_to_packing_fraction_cpu():
+    st, box = triangular(6, 4, 0.5, "cpu")
+    sy = make(st, box)
+    phi0 = J.compute_packing_fraction(st, sy)
+    assert abs(float(phi0) - 0.5) < 1e-6
+    s2, y2 = J.scale_to_packing_fraction(st, sy, 0.7)
+    assert abs(float(J.compute_packing_fraction(s2, y2)) - 0.7) < 1e-6
+    b0, b1 = np.asarray(sy.domain.box_size), np.asarray(y2.domain.box_size)
+    assert abs(b1[0] / b1[1] - b0[0] / b0[1]) < 1e-6  # aspect ratio kept
+    scale = b1[0] / b0[0]
+    np.testing.assert_allclose(s2.pos_c.numpy(), st.pos_c.numpy() * scale, rtol=2e-6, atol=1e-6)
+    np.testing.assert_allclose(np.asarray(y2.domain.inv_box_size), 1.0 / b1, rtol=1e-7)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("collider", ["naive", "CellList"])
+def test_bisection_jam_triangular_lattice(device, collider):
+    print(f"[jamming] triangular lattice, {collider}", flush=True)
+    st, box = triangular(8, 8, 0.85, device)
+    sy = make(st, box, collider)
+    res = J.bisection_jam(st, sy, n_minimization_steps=2000, packing_fraction_tolerance=1e-6,
+                          packing_fraction_increment=1e-2, verbose=False)
+    phi_j = math.pi / (2.0 * math.sqrt(3.0))
+    assert abs(res.packing_fraction - phi_j) < 2e-5, res.packing_fraction
+    assert res.potential_energy > 1e-16  # the returned jammed state overlaps
+    assert float(J.compute_packing_fraction(res.unjammed_state, res.unjammed_system)) <= phi_j + 2e-6
+    assert float(J.compute_packing_fraction(res.jammed_state, res.jammed_system)) >= phi_j - 2e-6
