@@ -1,5 +1,5 @@
-"""Jamming by bisection on the packing fraction (utils/jamming.py:28-267,
-utils/packing_utils.py:21-113), driving the GPU ``minimize``.
+"""Jamming protocols over the packing fraction (utils/jamming.py:28-267,
+585-793; utils/packing_utils.py:21-113), driving the GPU ``minimize``.
 
 The protocol (O'Hern et al., PRE 68, 011306 (2003)): minimise; while the
 minimum has no overlaps (potential energy per particle <= ``pe_tol``),
@@ -126,9 +126,65 @@ def bisection_jam(
     return JamResult(last_state, last_system, js, jy, float(pf), float(final_pe))
 
 
+def pe_band_jam(
+    state,
+    system,
+    n_minimization_steps: int = 1_000_000,
+    pe_tol: float = 1e-16,
+    pe_diff_tol: float = 1e-16,
+    pe_band_factor: float = 2.0,
+    packing_fraction_increment: float = 1e-3,
+    n_jamming_steps: int = 10_000,
+    verbose: bool = True,
+) -> JamResult:
+    """Jammed state in a potential-energy band (jamming.py:585-793): minimise;
+    below the band [pe_tol, pe_band_factor * pe_tol] compress by the current
+    increment, above it expand, inside it stop.  The increment halves at every
+    reversal of direction.  There is no reversion: each trial rescales the
+    state just minimised.  ``unjammed_state`` is the last state seen below the
+    band (the input if none); the result's packing fraction is that of the
+    final, re-minimised state."""
+    from ..minimizers import minimize
+
+    def relax(s, y):
+        return minimize(s, y, max_steps=n_minimization_steps, pe_tol=pe_tol, pe_diff_tol=pe_diff_tol)
+
+    pe_lo = pe_tol
+    pe_hi = pe_band_factor * pe_tol
+    pf = compute_packing_fraction(state, system)
+    inc = F32(packing_fraction_increment)
+    prev_dir = 0
+    last_state, last_system = state, system
+    i = 0
+    done = False
+    while i < n_jamming_steps and not done:
+        state, system, n_steps, pe = relax(state, system)
+        below, above = pe < pe_lo, pe > pe_hi
+        done = not (below or above)
+        direction = 1 if below else (-1 if above else 0)
+        if prev_dir != 0 and direction != 0 and direction != prev_dir:
+            inc = F32(F32(0.5) * inc)
+        new_pf = F32(pf + F32(direction) * inc)
+        if below:
+            last_state, last_system = state, system
+        if verbose:
+            print(f"Step: {i + 1} - phi={pf}, increment={inc}, PE/N={pe} after {n_steps} steps")
+        if not done:
+            prev_dir = direction
+            state, system = scale_to_packing_fraction(state, system, new_pf)
+            pf = new_pf
+        i += 1
+    # relaxed on return (a no-op after an in-band exit; the last rescale is
+    # still unminimised when the step cap ends the loop)
+    state, system, _, final_pe = relax(state, system)
+    return JamResult(last_state, last_system, state, system, float(compute_packing_fraction(state, system)),
+                     float(final_pe))
+
+
 __all__ = [
     "JamResult",
     "bisection_jam",
+    "pe_band_jam",
     "compute_packing_fraction",
     "compute_particle_volume",
     "scale_to_packing_fraction",

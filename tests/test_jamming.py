@@ -64,3 +64,23 @@ def test_bisection_jam_triangular_lattice(device, collider):
     assert res.potential_energy > 1e-16  # the returned jammed state overlaps
     assert float(J.compute_packing_fraction(res.unjammed_state, res.unjammed_system)) <= phi_j + 2e-6
     assert float(J.compute_packing_fraction(res.jammed_state, res.jammed_system)) >= phi_j - 2e-6
+
+
+@pytest.mark.gpu
+def test_pe_band_jam_triangular_lattice(device):
+    """On the compressed lattice every disk keeps six equal overlaps
+    delta = 1 - a(phi), so PE/N = 3 * k delta^2 / 2 with a = sqrt(phi_J / phi)
+    (disk diameter 1): the returned state must sit in the band and on that
+    curve (tolerance 2 %: float32 resolution of delta ~ 3e-4)."""
+    print("[jamming] pe band, triangular lattice", flush=True)
+    st, box = triangular(8, 8, 0.85, device)
+    sy = make(st, box)
+    pe_tol, k = 1e-4, 1e3
+    res = J.pe_band_jam(st, sy, n_minimization_steps=2000, pe_tol=pe_tol, pe_band_factor=2.0,
+                        packing_fraction_increment=1e-2, n_jamming_steps=200, verbose=False)
+    assert pe_tol <= res.potential_energy <= 2 * pe_tol, res.potential_energy
+    phi_j = math.pi / (2.0 * math.sqrt(3.0))
+    delta = 1.0 - math.sqrt(phi_j / res.packing_fraction)
+    want = 1.5 * k * delta**2
+    assert abs(res.potential_energy - want) < 0.02 * want, (res.potential_energy, want)
+    assert float(J.compute_packing_fraction(res.unjammed_state, res.unjammed_system)) <= res.packing_fraction
