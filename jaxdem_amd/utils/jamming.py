@@ -1,5 +1,5 @@
-"""Jamming protocols over the packing fraction (utils/jamming.py:28-267,
-585-793; utils/packing_utils.py:21-113), driving the GPU ``minimize``.
+"""Jamming protocols (utils/jamming.py:28-793; utils/packing_utils.py:21-113),
+driving the GPU ``minimize``.
 
 The protocol (O'Hern et al., PRE 68, 011306 (2003)): minimise; while the
 minimum has no overlaps (potential energy per particle <= ``pe_tol``),
@@ -181,10 +181,114 @@ def pe_band_jam(
                      float(final_pe))
 
 
+def pressure_bisection_jam(
+    state,
+    system,
+    *,
+    n_minimization_steps: int = 1_000_000,
+    pe_tol: float = 1e-16,
+    pe_diff_tol: float = 1e-16,
+    pressure_threshold: float = 1e-7,
+    pressure_band_factor: float = 1.01,
+    growth_rate: float = 1.001,
+    fine_growth_rate: float = 1.000001,
+    length_ratio_tolerance: float = 1e-14,
+    n_jamming_steps: int = 10_000,
+    verbose: bool = True,
+) -> JamResult:
+    """Jammed state in a pressure band (jamming.py:270-583).  The control
+    variable is the box length L = prod(box)^(1/dim), in float64 on the host;
+    the pressure is the GPU contact virial's (contacts.py:89-164).  A coarse
+    phase divides L by ``growth_rate`` while the pressure is below
+    [P_lo, P_hi] = [pressure_threshold, pressure_band_factor * P_lo] and
+    stops at the first over-compression; a fine phase bisects in L (growing by
+    ``fine_growth_rate`` when no over-compressed bound is known) until the
+    pressure is in the band or |L_hi / L_lo - 1| < ``length_ratio_tolerance``.
+    Every trial box rescales the last under-compressed state."""
+    from ..minimizers import minimize
+    from .contacts import compute_pressure
+
+    def relax(s, y):
+        return minimize(s, y, max_steps=n_minimization_steps, pe_tol=pe_tol, pe_diff_tol=pe_diff_tol)
+
+    p_lo = float(pressure_threshold)
+    p_hi = float(pressure_band_factor) * p_lo
+    dim = int(state.dim)
+    volume = float(compute_particle_volume(state))
+
+    def length_of(y) -> float:
+        return float(np.prod(np.asarray(y.domain.box_size, F32), dtype=F32)) ** (1.0 / dim)
+
+    def pf_for_length(length: float) -> float:
+        return volume / (length**dim)
+
+    state, system, _, final_pe = relax(state, system)
+    pressure = compute_pressure(state, system)
+    if pressure > p_hi:
+        if verbose:
+            print(f"Warning: Initial state is already over-compressed (P={pressure:.5e} > P_hi={p_hi:.5e}). Skipping.")
+        return JamResult(state, system, state, system, float(compute_packing_fraction(state, system)),
+                         float(final_pe))
+    length = length_of(system)
+    length_hi = -1.0  # largest under-compressed box seen
+    length_lo = -1.0  # smallest over-compressed box seen
+    last_state, last_system = state, system
+    iteration = 0
+
+    def do_step(state, system, rate, break_on_over, check_convergence):
+        nonlocal last_state, last_system, length, length_hi, length_lo, final_pe
+        state, system, _, pe = relax(state, system)
+        p = compute_pressure(state, system)
+        final_pe = float(pe)
+        if verbose:
+            print(f"Step {iteration}: L={length:.8e}, phi={pf_for_length(length):.8e}, P={p:.5e}, PE={final_pe:.5e}")
+        status = "continue"
+        if p < p_lo:  # under-compressed: compress (bisect once a bound is known)
+            last_state, last_system = state, system
+            length_hi = length
+            if length_lo > 0.0:
+                length = 0.5 * (length_hi + length_lo)
+                length_lo = -1.0
+            else:
+                length /= rate
+        elif p > p_hi:  # over-compressed: record the bound and bisect
+            length_lo = length
+            length = 0.5 * (length_hi + length_lo)
+            if break_on_over:
+                status = "break"
+        else:
+            status = "success"
+        if check_convergence and length_hi > 0.0 and length_lo > 0.0 and abs(length_hi / length_lo - 1.0) < \
+                length_ratio_tolerance:
+            status = "converged"
+        if status in ("continue", "break"):
+            state, system = scale_to_packing_fraction(last_state, last_system, pf_for_length(length))
+        return state, system, status
+
+    status = "continue"
+    while iteration < n_jamming_steps and status == "continue":
+        state, system, status = do_step(state, system, growth_rate, True, False)
+        iteration += 1
+    success = status == "success"
+    if not success:
+        status = "continue"
+        while iteration < n_jamming_steps and status == "continue":
+            state, system, status = do_step(state, system, fine_growth_rate, False, True)
+            iteration += 1
+        success = status == "success"
+    if verbose and not success:
+        print("Warning: pressure band not reached; returning the marginally jammed bracket bound.")
+    jammed_length = length if success else (length_lo if length_lo > 0.0 else length)
+    js, jy = scale_to_packing_fraction(last_state, last_system, pf_for_length(jammed_length))
+    js, jy, _, final_pe = relax(js, jy)
+    return JamResult(last_state, last_system, js, jy, float(compute_packing_fraction(js, jy)), float(final_pe))
+
+
 __all__ = [
     "JamResult",
     "bisection_jam",
     "pe_band_jam",
+    "pressure_bisection_jam",
     "compute_packing_fraction",
     "compute_particle_volume",
     "scale_to_packing_fraction",

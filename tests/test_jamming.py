@@ -84,3 +84,27 @@ def test_pe_band_jam_triangular_lattice(device):
     want = 1.5 * k * delta**2
     assert abs(res.potential_energy - want) < 0.02 * want, (res.potential_energy, want)
     assert float(J.compute_packing_fraction(res.unjammed_state, res.unjammed_system)) <= res.packing_fraction
+
+
+@pytest.mark.gpu
+def test_pressure_bisection_jam_triangular_lattice(device):
+    """Contact virial of the compressed lattice: three contacts per disk of
+    length a carrying k delta over the area a^2 sqrt(3) / 2 per disk give
+    P = sqrt(3) k delta / a (2D pressure = trace / 2).  The returned state must
+    sit in the band [P_lo, 1.01 P_lo] and on that curve (tolerance 2 %)."""
+    from jaxdem_amd.utils.contacts import compute_pressure
+
+    print("[jamming] pressure band, triangular lattice", flush=True)
+    st, box = triangular(8, 8, 0.85, device)
+    sy = make(st, box)
+    k, p_lo = 1e3, 1.0
+    res = J.pressure_bisection_jam(st, sy, n_minimization_steps=2000, pressure_threshold=p_lo,
+                                   growth_rate=1.01, fine_growth_rate=1.0001, n_jamming_steps=500,
+                                   verbose=False)
+    p = compute_pressure(res.jammed_state, res.jammed_system)
+    assert p_lo * 0.999 <= p <= 1.01 * p_lo * 1.001, p
+    phi_j = math.pi / (2.0 * math.sqrt(3.0))
+    a = math.sqrt(phi_j / res.packing_fraction)  # lattice spacing, disk diameter 1
+    want = math.sqrt(3.0) * k * (1.0 - a) / a
+    assert abs(p - want) < 0.02 * want, (p, want)
+    assert compute_pressure(res.unjammed_state, res.unjammed_system) < p_lo
