@@ -113,8 +113,8 @@ class NaiveSimulator(Collider):
 @Collider.register("CellList")
 @dataclass
 class DynamicCellList(Collider):
-    """Implicit cell list (cell_list.py:267-595): hash + radix sort + cell
-    ranges built on the GPU every call; 'ij'-ordered stencil."""
+    """Implicit cell list (cell_list.py:267-595): hash + stable sort by cell +
+    cell ranges built on the GPU every call; 'ij'-ordered stencil."""
 
     neighbor_mask: np.ndarray = field(default_factory=lambda: np.zeros((0, 3), np.int64))
     cell_size: float = 1.0
