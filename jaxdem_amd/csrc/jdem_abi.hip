@@ -688,7 +688,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       HIPCHK(ctx, ws_launch_permute(N, new_orig, I32(ctx->wWpos), F4(ctx->wP[cur]), F4(ctx->wV[cur]),
                                     F4(ctx->wW[cur]), F4(ctx->wQ[set]), F4(ctx->wI[set]), F4(ctx->wP[1 - cur]),
                                     F4(ctx->wV[1 - cur]), F4(ctx->wW[1 - cur]), F4(ctx->wQ[1 - set]),
-                                    F4(ctx->wI[1 - set]), I32(ctx->wOrig[1 - set]), src, inv, st));
+                                    F4(ctx->wI[1 - set]), I32(ctx->wOrig[1 - set]), src, inv, st, F4(ctx->wOld)));
       cur = 1 - cur;
       set = 1 - set;
       if (!counted) HIPCHK(ctx, ws_launch_cell_starts(sk, N, n_cells, I32(ctx->cstart), st));
@@ -731,7 +731,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       nlc = 1 - nlc;
       if (hist) xc = 1 - xc;
     }
-    HIPCHK(ctx, ws_launch_copy4(F4(ctx->wOld), F4(ctx->wP[cur]), N, st));
+    if (naive2) HIPCHK(ctx, ws_launch_copy4(F4(ctx->wOld), F4(ctx->wP[cur]), N, st));  // else the permute wrote them
     L.n_build_times += 1;
     span_end(ctx, sp);
     return JDEM_OK;

@@ -242,12 +242,15 @@ __global__ void __launch_bounds__(kBlock) k_permute(int64_t n, const int32_t* __
                                                      float4* __restrict__ P2, float4* __restrict__ V2,
                                                      float4* __restrict__ W2, float4* __restrict__ Q2,
                                                      float4* __restrict__ I2, int32_t* __restrict__ orig2,
-                                                     int32_t* __restrict__ src_out, int32_t* __restrict__ inv_out) {
+                                                     int32_t* __restrict__ src_out, int32_t* __restrict__ inv_out,
+                                                     float4* __restrict__ old2) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n) return;
   const int32_t i = new_orig[b];
   const int32_t s = wpos[i];
-  P2[b] = P[s];
+  const float4 p = P[s];
+  P2[b] = p;
+  if (old2) old2[b] = p;  // the positions of this build
   V2[b] = V[s];
   W2[b] = W[s];
   Q2[b] = Q[s];
@@ -1230,9 +1233,9 @@ hipError_t ws_launch_seg_ids(const int64_t* offsets, int n_seg, int64_t n, int32
 hipError_t ws_launch_permute(int64_t n, const int32_t* new_orig, const int32_t* wpos, const float4* P,
                              const float4* V, const float4* W, const float4* Q, const float4* I, float4* P2,
                              float4* V2, float4* W2, float4* Q2, float4* I2, int32_t* orig2, int32_t* src,
-                             int32_t* inv, hipStream_t s) {
+                             int32_t* inv, hipStream_t s, float4* old2) {
   hipLaunchKernelGGL(k_permute, dim3(nblocks(n)), dim3(kBlock), 0, s, n, new_orig, wpos, P, V, W, Q, I, P2, V2, W2,
-                     Q2, I2, orig2, src, inv);
+                     Q2, I2, orig2, src, inv, old2);
   return hipGetLastError();
 }
 hipError_t ws_launch_seg_prefix(int n_seg, const int32_t* seg_mode, const int64_t* offsets, const SegP* segp,
