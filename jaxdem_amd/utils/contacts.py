@@ -22,4 +22,17 @@ def compute_pressure(state, system) -> float:
     return float(np.trace(s) / state.dim)
 
 
-__all__ = ["compute_contact_stress_tensor", "compute_pressure"]
+def compute_contact_pressure(state, system, cutoff=None, max_neighbors=None, volume=None) -> tuple:
+    """``compute_contact_pressure`` (contacts.py:149-164): trace of the contact
+    stress over dim, positive for repulsive overlaps; returns (state, system,
+    pressure).  The contacts come from the system's own collider, so
+    ``cutoff`` and ``max_neighbors`` (the reference's neighbour-list sizing
+    for this call) are not needed; ``volume`` replaces the box volume."""
+    from ..system import _contact_virial
+
+    W = _contact_virial(state, system)
+    V = float(np.prod(np.asarray(system.domain.box_size, np.float64))) if volume is None else float(volume)
+    return state, system, float(np.trace(W / V) / state.dim)
+
+
+__all__ = ["compute_contact_pressure", "compute_contact_stress_tensor", "compute_pressure"]

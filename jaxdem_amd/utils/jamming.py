@@ -20,6 +20,8 @@ from typing import Any, NamedTuple
 
 import numpy as np
 
+from .packing_utils import compute_packing_fraction, compute_particle_volume, scale_to_packing_fraction
+
 F32 = np.float32
 
 
@@ -33,40 +35,6 @@ class JamResult(NamedTuple):
     jammed_system: Any
     packing_fraction: float
     potential_energy: float
-
-
-def compute_particle_volume(state) -> F32:
-    """Total particle volume (packing_utils.py:21-26; one sphere per body)."""
-    if not state.is_sphere_state:
-        raise NotImplementedError("jamming on the GPU path handles spheres only")
-    return F32(state.volume.sum().item())
-
-
-def compute_packing_fraction(state, system) -> F32:
-    """Particle volume over the box volume, anchor at the origin
-    (packing_utils.py:29-36)."""
-    return F32(compute_particle_volume(state) / F32(np.prod(np.asarray(system.domain.box_size, F32), dtype=F32)))
-
-
-def scale_to_packing_fraction(state, system, new_packing_fraction) -> tuple:
-    """Rescale the box by one common factor, keeping its aspect ratio, and the
-    sphere centres with it (packing_utils.py:67-113).  A NeighborList is
-    marked for a rebuild.  The domain's inverse box size is refreshed with
-    the box (the reference keeps the old one, which changes no contact
-    displacement while the box is wider than twice the contact distance)."""
-    from ..colliders import NeighborList
-
-    box = np.asarray(system.domain.box_size, F32)
-    vol = compute_particle_volume(state)
-    scale = F32(F32(vol / (F32(new_packing_fraction) * F32(np.prod(box, dtype=F32)))) ** F32(1.0 / state.dim))
-    new_box = (box * scale).astype(F32)
-    # offset = com * scale - com, added to the centre (one sphere per body)
-    s = replace(state, pos_c=state.pos_c + (state.pos_c * float(scale) - state.pos_c))
-    dom = replace(system.domain, box_size=new_box, inv_box_size=(F32(1.0) / new_box).astype(F32))
-    sys2 = replace(system, domain=dom)
-    if isinstance(sys2.collider, NeighborList):
-        sys2 = replace(sys2, collider=replace(sys2.collider, n_build_times=0))
-    return s, sys2
 
 
 def bisection_jam(

@@ -108,3 +108,28 @@ def test_pressure_bisection_jam_triangular_lattice(device):
     want = math.sqrt(3.0) * k * (1.0 - a) / a
     assert abs(p - want) < 0.02 * want, (p, want)
     assert compute_pressure(res.unjammed_state, res.unjammed_system) < p_lo
+
+
+@pytest.mark.gpu
+def test_quasistatic_compression_triangular_lattice(device):
+    """Compressing the lattice below its jamming point leaves it free of
+    overlaps; past it, PE/N = 3 k delta^2 / 2 and P = sqrt(3) k delta / a
+    (float32 delta resolution: 2 %); expanding back removes every overlap."""
+    from jaxdem_amd.utils.contacts import compute_contact_pressure
+    from jaxdem_amd.utils.packing_utils import quasistatic_compress_to_packing_fraction as compress
+
+    print("[jamming] quasistatic compression, triangular lattice", flush=True)
+    st, box = triangular(8, 8, 0.80, device)
+    sy = make(st, box)
+    k = 1e3
+    phi_j = math.pi / (2.0 * math.sqrt(3.0))
+    s1, y1, phi1, pe1 = compress(st, sy, 0.88, step=0.02, max_n_min_steps_per_outer=2000)
+    assert abs(phi1 - 0.88) < 1e-10 and pe1 == 0.0
+    assert abs(float(J.compute_packing_fraction(s1, y1)) - 0.88) < 1e-6
+    s2, y2, phi2, pe2 = compress(s1, y1, 0.92, step=0.01, max_n_min_steps_per_outer=2000)
+    a = math.sqrt(phi_j / phi2)
+    assert abs(pe2 - 1.5 * k * (1.0 - a) ** 2) < 0.02 * 1.5 * k * (1.0 - a) ** 2, pe2
+    _, _, p = compute_contact_pressure(s2, y2)
+    assert abs(p - math.sqrt(3.0) * k * (1.0 - a) / a) < 0.02 * math.sqrt(3.0) * k * (1.0 - a) / a, p
+    s3, y3, phi3, pe3 = compress(s2, y2, 0.85, step=0.05, max_n_min_steps_per_outer=2000)
+    assert abs(phi3 - 0.85) < 1e-10 and pe3 == 0.0
