@@ -255,3 +255,19 @@ def test_precision_cell_offset_round_trip():
     back = PR.from_cell_offset(c, o, box, dims)
     err = np.abs(PR.min_image_diff(back, pos, box))
     assert err.max() < 2e-7  # float32 resolution of an offset below 1.41, not of a coordinate near 73
+
+
+def test_server_rejects_bad_policy_and_cap():
+    from jaxdem_amd import serving as SV
+
+    with pytest.raises(ValueError, match="policy"):
+        SV.Server("cpu", hbm_budget=1, policy="lifo")
+    with pytest.raises(ValueError, match="max_active"):
+        SV.Server("cpu", hbm_budget=1, max_active=0)
+    assert SV.Server("cpu", hbm_budget=1, policy="sjf", max_active=3).policy == "sjf"
+
+
+def test_trajectory_rollout_save_state_signature():
+    # save_state keeps the state of a frame and drops the system
+    assert jd.save_state("s", "y") == "s"
+    assert jd.system.save_state is jd.save_state

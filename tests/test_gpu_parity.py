@@ -622,3 +622,19 @@ def test_precise_fire_against_fp64(device):
     assert res["reported_pe_rel_err"] < 2e-5 and res["reported_pressure_rel_err"] < 2e-5
     assert res["pos_max_err"] < 1e-5
     assert abs(res["ref_pe"] - float(fx["ref_pe"])) <= 1e-12 * float(fx["ref_pe"])
+
+
+def test_skin_autotuner_system(device):
+    """The plain-system skin autotuner probes every candidate and keeps the
+    fastest; every candidate's list fits its rows."""
+    from jaxdem_amd.tuning import autotune_skin_system
+
+    print("[tuning] skin autotuner", flush=True)
+
+    def make_pair(sk):
+        return W.make_gpu_system(W.spheres_friction(N=4096, seed=6, skin=sk), device=device)
+
+    best, table = autotune_skin_system(make_pair, [0.05, 0.1], probe_steps=30, warm_steps=5)
+    assert [r[0] for r in table] == [0.05, 0.1]
+    assert best in (0.05, 0.1) and best == max(table, key=lambda r: r[1])[0]
+    assert all(r[1] > 0 and r[2] >= 0 and r[3] > 0 for r in table)
