@@ -15,7 +15,6 @@ sphere centre (packing_utils.py:91-102 with one sphere per group).
 
 from __future__ import annotations
 
-from dataclasses import replace
 from typing import Any, NamedTuple
 
 import numpy as np
