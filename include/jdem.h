@@ -300,6 +300,11 @@ int jdem_contact_virial(jdem_ctx* ctx, jdem_system* sys, double* w);
  * float64 to *energy (host). */
 int jdem_kinetic_energy(jdem_ctx* ctx, const jdem_system* sys, double* energy);
 
+/* The two parts of it, each a float64 sum: translational (thermal.py:83-102)
+ * and rotational with the body-frame angular velocity (thermal.py:105-122),
+ * to host doubles. */
+int jdem_kinetic_energy_parts(jdem_ctx* ctx, const jdem_system* sys, double* translational, double* rotational);
+
 /* ---- (c) integration -------------------------------------------------------- */
 
 /* One stage of _step_once (system.py:60-82) on its own: domain.apply,

@@ -230,6 +230,7 @@ STRUCTS = {
     "jdem_fire": JdemFire,
     "jdem_minimize_stats": JdemMinimizeStats,
     "jdem_precise": JdemPrecise,
+    "jdem_frames": JdemFrames,
 }
 
 # name -> (restype, argtypes) for every entry point of include/jdem.h
@@ -255,6 +256,7 @@ SIGNATURES = {
     "jdem_potential_energy": (c_int, [c_void_p, POINTER(JdemSystem), POINTER(c_double)]),
     "jdem_contact_virial": (c_int, [c_void_p, POINTER(JdemSystem), POINTER(c_double)]),
     "jdem_kinetic_energy": (c_int, [c_void_p, POINTER(JdemSystem), POINTER(c_double)]),
+    "jdem_kinetic_energy_parts": (c_int, [c_void_p, POINTER(JdemSystem), POINTER(c_double), POINTER(c_double)]),
     "jdem_step_stage": (c_int, [c_void_p, POINTER(JdemSystem), c_int32]),
     "jdem_system_step": (c_int, [c_void_p, POINTER(JdemSystem), c_int64, POINTER(JdemStepStats)]),
     "jdem_rollout": (c_int, [c_void_p, POINTER(JdemSystem), POINTER(JdemFrames), POINTER(JdemStepStats)]),

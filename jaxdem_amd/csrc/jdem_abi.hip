@@ -1187,12 +1187,26 @@ int jdem_contact_virial(jdem_ctx* ctx, jdem_system* sys, double* w) {
   return reduce_f64(ctx, sys, D * D, w, launch_virial, true);
 }
 
-int jdem_kinetic_energy(jdem_ctx* ctx, const jdem_system* sys, double* energy) {
+int jdem_kinetic_energy_parts(jdem_ctx* ctx, const jdem_system* sys, double* translational, double* rotational) {
   int rc = check_system(ctx, sys);
   if (rc) return rc;
-  if (!energy) return fail(ctx, JDEM_EINVAL, "null output");
+  if (!translational || !rotational) return fail(ctx, JDEM_EINVAL, "null output");
   (void)hipSetDevice(ctx->device);
-  return reduce_f64(ctx, sys, 1, energy, nullptr, false);
+  double parts[2] = {0.0, 0.0};
+  rc = reduce_f64(ctx, sys, 2, parts, nullptr, false);
+  if (rc) return rc;
+  *translational = parts[0];
+  *rotational = parts[1];
+  return JDEM_OK;
+}
+
+int jdem_kinetic_energy(jdem_ctx* ctx, const jdem_system* sys, double* energy) {
+  if (!energy) return ctx ? fail(ctx, JDEM_EINVAL, "null output") : JDEM_EINVAL;
+  double t = 0.0, r = 0.0;
+  const int rc = jdem_kinetic_energy_parts(ctx, sys, &t, &r);
+  if (rc) return rc;
+  *energy = t + r;
+  return JDEM_OK;
 }
 
 static int free_box(jdem_ctx* ctx, const float* pos, const jdem_system* sys, float box[3], float anchor[3]) {

@@ -403,13 +403,13 @@ __global__ void __launch_bounds__(kBlock) k_kinetic(StepArgs a) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const double m = a.mass[i];
-  double ke = 0.0;
+  double ke = 0.0;  // rotational part
   double v2 = 0.0;
   for (int c = 0; c < D; ++c) {
     const double v = a.vel_in[D * i + c];
     v2 += v * v;
   }
-  ke = 0.5 * m * v2;
+  const double ke_t = 0.5 * m * v2;
   if constexpr (D == 3) {
     // body-frame angular velocity in float64
     const double qw = a.q[4 * i], qx = a.q[4 * i + 1], qy = a.q[4 * i + 2], qz = a.q[4 * i + 3];
@@ -427,7 +427,9 @@ __global__ void __launch_bounds__(kBlock) k_kinetic(StepArgs a) {
     const double w = a.ang_in[i];
     ke += 0.5 * (double)a.inertia[i] * w * w;
   }
-  a.e_out[i] = ke;
+  // translational and rotational parts, summed separately
+  a.e_out[2 * i] = ke_t;
+  a.e_out[2 * i + 1] = ke;
 }
 
 // One stage of _step_once on its own (jdem_step_stage).

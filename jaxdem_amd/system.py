@@ -591,6 +591,16 @@ def _contact_virial(state: State, system: System) -> np.ndarray:
     return np.array(list(w), dtype=np.float64).reshape(D, D)
 
 
+def _kinetic_energy_parts(state: State, system: System) -> tuple:
+    keep: list = []
+    S = _system_struct(state, system, keep)
+    ctx = _ctx(state)
+    t, r = ctypes.c_double(), ctypes.c_double()
+    ctx.check(ctx.lib.jdem_kinetic_energy_parts(ctx.ptr, ctypes.byref(S), ctypes.byref(t), ctypes.byref(r)),
+              "jdem_kinetic_energy_parts")
+    return float(t.value), float(r.value)
+
+
 def _kinetic_energy(state: State, system: System) -> float:
     keep: list = []
     S = _system_struct(state, system, keep)

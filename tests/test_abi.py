@@ -65,6 +65,7 @@ STRUCT_FIELDS = {
                   "dt_min_scale"],
     "jdem_minimize_stats": ["steps", "pe", "max_grad", "power", "overflow", "reserved"],
     "jdem_precise": ["n", "dim", "dims", "cell", "cell_lo", "cidx", "off", "rad", "force", "k", "stall_steps"],
+    "jdem_frames": ["n_frames", "steps", "pos", "vel", "ang_vel", "q", "force", "torque"],
 }
 
 

@@ -638,3 +638,32 @@ def test_skin_autotuner_system(device):
     assert [r[0] for r in table] == [0.05, 0.1]
     assert best in (0.05, 0.1) and best == max(table, key=lambda r: r[1])[0]
     assert all(r[1] > 0 and r[2] >= 0 and r[3] > 0 for r in table)
+
+
+def test_thermal_parts_and_temperature(device):
+    """Translational and rotational kinetic energy against float64 sums of
+    the same state (rotational: the body-frame form, which for the isotropic
+    inertia of a sphere equals 1/2 I |w|^2), temperature = 2 KE / n_dof, and
+    scale_to_temperature hits its target."""
+    from jaxdem_amd.utils import thermal as T
+
+    print("[thermal] kinetic parts", flush=True)
+    st, _ = packed(3000, 3, seed=21, domain="reflect")
+    m = st.mass.double().cpu().numpy()
+    v = st.vel.double().cpu().numpy()
+    w = st.ang_vel.double().cpu().numpy()
+    inert = st.inertia.double().cpu().numpy()
+    assert np.allclose(inert[:, 0], inert[:, 1]) and np.allclose(inert[:, 0], inert[:, 2])
+    kt = 0.5 * float(np.sum(m * np.sum(v * v, axis=1)))
+    kr = 0.5 * float(np.sum(inert[:, 0] * np.sum(w * w, axis=1)))
+    assert abs(T.compute_translational_kinetic_energy(st) - kt) < 1e-12 * kt
+    assert abs(T.compute_rotational_kinetic_energy(st) - kr) < 1e-6 * kr  # float32 quaternion rotation
+    n_dof, n_v, n_w = T.count_dynamic_dofs(st, can_rotate=True, subtract_drift=True)
+    assert (n_v, n_w) == ((3000 - 1) * 3, 3000 * 3)
+    temp = T.compute_temperature(st, can_rotate=True, subtract_drift=True)
+    assert abs(temp - 2.0 * (T.compute_translational_kinetic_energy(st) + T.compute_rotational_kinetic_energy(st))
+               / n_dof) < 1e-12 * temp
+    s2 = T.scale_to_temperature(st, 0.25, can_rotate=True, subtract_drift=True)
+    assert abs(T.compute_temperature(s2, can_rotate=True, subtract_drift=True) - 0.25) < 1e-5
+    drift = (s2.vel.double() * s2.mass.double()[:, None]).sum(dim=0).cpu().numpy()
+    assert np.all(np.abs(drift) < 1e-3)
