@@ -93,7 +93,40 @@ def scale_to_temperature(state, target_temperature: float, can_rotate: bool, sub
     return replace(state, vel=(vel * s).contiguous(), ang_vel=ang)
 
 
+def set_temperature(state, target_temperature: float, can_rotate: bool, subtract_drift: bool, seed: int | None = 0,
+                    k_B: float = 1.0):
+    """Random velocities at ``target_temperature`` (thermal.py:206-303): one
+    normal draw per free body (clump) for the velocity and for the body-frame
+    angular velocity, the free bodies' mean velocity removed with
+    ``subtract_drift``, the angular velocity rotated to the lab frame (3D),
+    then ``scale_to_temperature``.  The draws come from numpy's PCG64 with
+    ``seed`` (JAX's PRNG streams cannot be reproduced without JAX)."""
+    from dataclasses import replace
+
+    import torch
+
+    from ..state import Quaternion
+
+    rng = np.random.Generator(np.random.PCG64(0 if seed is None else int(seed)))
+    n, dim, a = state.N, state.dim, int(state.ang_vel.shape[-1])
+    clump = state.clump_id.reshape(-1).long().cpu().numpy()
+    counts = np.bincount(clump, minlength=n)
+    fixed = np.bincount(clump, weights=state.fixed.reshape(-1).cpu().numpy().astype(np.float64), minlength=n)
+    free = (counts > 0) & (fixed == 0)
+    v = rng.standard_normal((n, dim)) * free[:, None]
+    if subtract_drift:
+        v = v - (v.sum(axis=0) / max(int(free.sum()), 1)) * free[:, None]
+    w = rng.standard_normal((n, a)) * free[:, None]
+    dev = state.vel.device
+    vel = torch.as_tensor(v[clump], dtype=torch.float32, device=dev).contiguous()
+    wb = torch.as_tensor(w[clump], dtype=torch.float32, device=dev).contiguous()
+    ang = wb if dim == 2 else Quaternion.rotate(state.q, wb).contiguous()
+    return scale_to_temperature(replace(state, vel=vel, ang_vel=ang), target_temperature, can_rotate,
+                                subtract_drift, k_B)
+
+
 __all__ = [
+    "set_temperature",
     "compute_energy",
     "compute_kinetic_energy",
     "compute_potential_energy",

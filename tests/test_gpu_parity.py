@@ -667,3 +667,8 @@ def test_thermal_parts_and_temperature(device):
     assert abs(T.compute_temperature(s2, can_rotate=True, subtract_drift=True) - 0.25) < 1e-5
     drift = (s2.vel.double() * s2.mass.double()[:, None]).sum(dim=0).cpu().numpy()
     assert np.all(np.abs(drift) < 1e-3)
+    s3 = T.set_temperature(st, 1.5, can_rotate=True, subtract_drift=True, seed=3)
+    assert abs(T.compute_temperature(s3, can_rotate=True, subtract_drift=True) - 1.5) < 1e-5 * 1.5
+    assert not torch.equal(s3.vel, st.vel)
+    s4 = T.set_temperature(st, 1.5, can_rotate=False, subtract_drift=False, seed=3)
+    assert abs(T.compute_temperature(s4, can_rotate=False, subtract_drift=False) - 1.5) < 1e-5 * 1.5
