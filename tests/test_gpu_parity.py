@@ -338,6 +338,27 @@ def test_rollout_abi_rejects(device):
         ctx.check(ctx.lib.jdem_rollout(ctx.ptr, ctypes.byref(S2), ctypes.byref(fr), None), "jdem_rollout")
 
 
+@pytest.mark.parametrize("strides", [[4, 0, 7, 0, 0, 12], [25]])
+def test_device_rollout_frames_reflect(device, strides):
+    """Device frames in a reflecting box with friction, repeated frames
+    (zero strides) and a single frame: each equals System.step of its
+    cumulative steps, bit for bit."""
+    print(f"[step] device rollout reflect {strides}", flush=True)
+    w = W.polydisperse_spheres(2500, seed=41, phi=0.45, domain="reflect", mu=0.3)
+    w.box = w.box * np.float32(1.02)
+    st, sy = W.make_gpu_system(w, device=device)
+    _, fy, traj = jd.System.trajectory_rollout(st, sy, strides=np.array(strides), save_fn=jd.save_state)
+    assert traj.pos_c.shape[0] == len(strides)
+    total = 0
+    for f, k in enumerate(strides):
+        total += k
+        ref = jd.System.step(st, sy, n=total)[0]
+        for name in ("pos_c", "vel", "ang_vel", "force", "torque"):
+            assert_bitwise(f"frame {f} {name}", getattr(traj, name)[f].cpu().numpy(), getattr(ref, name).cpu().numpy())
+        assert_bitwise(f"frame {f} q", traj.q.data[f].cpu().numpy(), ref.q.data.cpu().numpy())
+    assert fy.step_count == sy.step_count + total
+
+
 @pytest.mark.parametrize("case", ["spheres_friction", "disks_friction"])
 def test_device_rollout_frames_bitwise(device, case):
     """save_state frames written on the device by jdem_rollout equal, bit for
