@@ -54,7 +54,7 @@ def grid_state(
 
 
 from . import contacts, thermal  # noqa: E402
-from .contacts import compute_contact_stress_tensor, compute_pressure  # noqa: E402
+from .contacts import compute_contact_pressure, compute_contact_stress_tensor, compute_pressure  # noqa: E402
 from .thermal import (  # noqa: E402
     compute_energy,
     compute_kinetic_energy,
@@ -68,6 +68,7 @@ __all__ = [
     "compute_energy",
     "compute_kinetic_energy",
     "compute_potential_energy",
+    "compute_contact_pressure",
     "compute_contact_stress_tensor",
     "compute_pressure",
 ]

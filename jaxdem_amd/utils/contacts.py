@@ -9,30 +9,30 @@ from __future__ import annotations
 import numpy as np
 
 
-def compute_contact_stress_tensor(state, system) -> np.ndarray:
+def compute_contact_stress_tensor(state, system, cutoff=None, max_neighbors=None, volume=None) -> tuple:
+    """``compute_contact_stress_tensor`` (contacts.py:89-147): the contact
+    virial over the volume (the box volume unless ``volume`` is given), as
+    (state, system, stress (dim, dim) float64).  The contacts come from the
+    system's own collider, so ``cutoff`` and ``max_neighbors`` (the
+    reference's neighbour-list sizing for this call) are not needed."""
     from ..system import _contact_virial
 
     W = _contact_virial(state, system)
-    V = float(np.prod(np.asarray(system.domain.box_size, np.float64)))
-    return W / V
+    V = float(np.prod(np.asarray(system.domain.box_size, np.float64))) if volume is None else float(volume)
+    return state, system, W / V
 
 
 def compute_pressure(state, system) -> float:
-    s = compute_contact_stress_tensor(state, system)
-    return float(np.trace(s) / state.dim)
+    """trace(stress) / dim as a float."""
+    return compute_contact_pressure(state, system)[2]
 
 
 def compute_contact_pressure(state, system, cutoff=None, max_neighbors=None, volume=None) -> tuple:
     """``compute_contact_pressure`` (contacts.py:149-164): trace of the contact
     stress over dim, positive for repulsive overlaps; returns (state, system,
-    pressure).  The contacts come from the system's own collider, so
-    ``cutoff`` and ``max_neighbors`` (the reference's neighbour-list sizing
-    for this call) are not needed; ``volume`` replaces the box volume."""
-    from ..system import _contact_virial
-
-    W = _contact_virial(state, system)
-    V = float(np.prod(np.asarray(system.domain.box_size, np.float64))) if volume is None else float(volume)
-    return state, system, float(np.trace(W / V) / state.dim)
+    pressure)."""
+    state, system, stress = compute_contact_stress_tensor(state, system, cutoff, max_neighbors, volume)
+    return state, system, float(np.trace(stress) / state.dim)
 
 
 __all__ = ["compute_contact_pressure", "compute_contact_stress_tensor", "compute_pressure"]

@@ -417,6 +417,33 @@ def test_potential_energy(device, law, collider):
     assert ke == pytest.approx(O.kinetic_energy(ost), rel=1e-12)
 
 
+@pytest.mark.parametrize("collider", ["naive", "CellList"])
+@pytest.mark.parametrize("dim", [3, 2])
+def test_contact_stress_and_pressure(device, collider, dim):
+    """Contact stress (contacts.py:89-164) against the oracle's float64
+    virial sum of the same float32 pair forces: 1e-12 of the largest
+    component (different summation order)."""
+    from jaxdem_amd.utils.contacts import compute_contact_pressure, compute_contact_stress_tensor
+
+    st, box = packed(1500 if dim == 3 else 1200, dim, seed=31 + dim, domain="periodic")
+    sy = make_system(st, box, "periodic", "hertz", collider)
+    _, _, stress = compute_contact_stress_tensor(st, sy)
+    ost = B.state_of(st)
+    P = B.params_of(st, sy)
+    if collider == "naive":
+        C = O.naive_candidates(st.N)
+    else:
+        C, _ = O.cell_candidates(ost.pos, P.domain, P.cell_size, P.search_range)
+    want = O.pair_virial(ost, P, C) / float(np.prod(box.astype(np.float64)))
+    assert stress.shape == (dim, dim)
+    assert np.max(np.abs(stress - want)) <= 1e-12 * np.max(np.abs(want))
+    assert np.trace(want) > 0  # repulsive overlaps: positive pressure
+    _, _, p = compute_contact_pressure(st, sy)
+    assert p == pytest.approx(np.trace(want) / dim, rel=1e-12)
+    _, _, s2 = compute_contact_stress_tensor(st, sy, volume=2.0)
+    assert np.allclose(s2 * 2.0, stress * float(np.prod(box.astype(np.float64))), rtol=1e-12)
+
+
 # ---------------------------------------------------------------------------
 # resident NeighborList engine
 # ---------------------------------------------------------------------------
