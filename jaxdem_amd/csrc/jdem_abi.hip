@@ -1550,15 +1550,21 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
   a.e_out = per;
   a.w_out = nullptr;
   a.moved = moved;
+  a.gmax_bits = &f.st->max_grad_bits;
+  // With the counting sort the re-sort is decided on the device: its kernels
+  // run only while a sphere has changed cell, so a step needs no host read
+  // of the flag.  The radix fallback keeps the host-side decision.
+  const bool gated = use_cell_sort(N, n_cells);
+  a.gated = gated ? 1 : 0;
   bool sorted = false;
   double pe = 0.0, gmax = 0.0;
   auto eval = [&](bool with_virial) -> int {
-    if (!sorted && N > 0) {  // the cell list, rebuilt when a sphere changed cell
+    if ((gated || !sorted) && N > 0) {  // the cell list, rebuilt when a sphere changed cell
       HIPCHK(ctx, launch_prec_keys(a, st));
       uint32_t* sk = nullptr;
       int32_t* perm = nullptr;
-      if (use_cell_sort(N, n_cells)) {
-        HIPCHK(ctx, cell_sort_pairs(ctx, a.keys, N, n_cells, &sk, &perm, (int32_t*)ctx->cstart.p, st));
+      if (gated) {
+        HIPCHK(ctx, cell_sort_pairs(ctx, a.keys, N, n_cells, &sk, &perm, (int32_t*)ctx->cstart.p, st, moved));
       } else {
         HIPCHK(ctx, radix_sort_pairs(ctx, a.keys, N, nbits_for(n_cells), &sk, &perm, st));
         HIPCHK(ctx, ws_launch_cell_starts(sk, N, n_cells, (int32_t*)ctx->cstart.p, st));
@@ -1569,7 +1575,7 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
     a.w_out = with_virial ? wper : nullptr;
     HIPCHK(ctx, launch_prec_gather(a, st));
     HIPCHK(ctx, launch_prec_force(a, st));
-    HIPCHK(ctx, hipMemsetAsync(&f.st->max_grad_bits, 0, sizeof(unsigned), st));
+    if (N == 0) HIPCHK(ctx, hipMemsetAsync(&f.st->max_grad_bits, 0, sizeof(unsigned), st));  // else the gather clears it
     HIPCHK(ctx, launch_fire_grad_max(f, st));
     double h = 0.0;
     if (N > 0) {
@@ -1577,9 +1583,7 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
       HIPCHK(ctx, hipMemcpyAsync(&h, out, 8, hipMemcpyDeviceToHost, st));
     }
     unsigned bits = 0;
-    int32_t mv = 0;
     HIPCHK(ctx, hipMemcpyAsync(&bits, &f.st->max_grad_bits, sizeof(bits), hipMemcpyDeviceToHost, st));
-    HIPCHK(ctx, hipMemcpyAsync(&mv, moved, sizeof(mv), hipMemcpyDeviceToHost, st));
     HIPCHK(ctx, hipStreamSynchronize(st));
     float g;
     std::memcpy(&g, &bits, sizeof(g));
@@ -1587,7 +1591,8 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
     gmax = (double)g;
     return JDEM_OK;
   };
-  HIPCHK(ctx, hipMemsetAsync(moved, 0, sizeof(int32_t), st));
+  // the first evaluation sorts: the flag starts set
+  HIPCHK(ctx, launch_fill_i32(moved, 1, 1, st));
   int rc = eval(false);
   if (rc) return rc;
   double prev = INFINITY;
@@ -1607,15 +1612,17 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
     if (conv_pe || conv_rel || gmax <= force_tol || stalled) break;
     HIPCHK(ctx, launch_fire_power(f, st));
     HIPCHK(ctx, launch_fire_decide(f, st));
-    HIPCHK(ctx, hipMemsetAsync(moved, 0, sizeof(int32_t), st));
+    if (!gated) HIPCHK(ctx, launch_fill_i32(moved, 1, 0, st));
     HIPCHK(ctx, launch_prec_move(f, a, st));
     HIPCHK(ctx, launch_fire_commit(f, st));
     prev = pe;
-    // a sphere that left its cell needs a new cell list before the evaluation
-    int32_t mv = 0;
-    HIPCHK(ctx, hipMemcpyAsync(&mv, moved, sizeof(mv), hipMemcpyDeviceToHost, st));
-    HIPCHK(ctx, hipStreamSynchronize(st));
-    if (mv) sorted = false;
+    if (!gated) {
+      // a sphere that left its cell needs a new cell list before the evaluation
+      int32_t mv = 0;
+      HIPCHK(ctx, hipMemcpyAsync(&mv, moved, sizeof(mv), hipMemcpyDeviceToHost, st));
+      HIPCHK(ctx, hipStreamSynchronize(st));
+      if (mv) sorted = false;
+    }
     rc = eval(false);
     if (rc) return rc;
     ++steps;

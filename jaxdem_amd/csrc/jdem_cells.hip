@@ -95,7 +95,9 @@ constexpr int SC_ITEMS = 16;
 constexpr int SC_CHUNK = kBlock * SC_ITEMS;
 
 __global__ void __launch_bounds__(kBlock) k_scan_totals(const uint32_t* __restrict__ data, int64_t total,
-                                                         uint32_t* __restrict__ sums) {
+                                                         uint32_t* __restrict__ sums,
+                                                         const int32_t* __restrict__ gate = nullptr) {
+  if (gate && *gate == 0) return;
   __shared__ uint32_t sh[kBlock];
   const int tid = threadIdx.x;
   const int64_t base = (int64_t)blockIdx.x * SC_CHUNK;
@@ -115,7 +117,9 @@ __global__ void __launch_bounds__(kBlock) k_scan_totals(const uint32_t* __restri
 }
 
 // exclusive scan of nsums (<= 1024 * 1024) values in one block
-__global__ void __launch_bounds__(1024) k_scan_sums(uint32_t* __restrict__ sums, int nsums) {
+__global__ void __launch_bounds__(1024) k_scan_sums(uint32_t* __restrict__ sums, int nsums,
+                                                     const int32_t* __restrict__ gate = nullptr) {
+  if (gate && *gate == 0) return;
   __shared__ uint32_t part[1024];
   const int tid = threadIdx.x;
   const int per = (nsums + 1023) / 1024;
@@ -140,7 +144,9 @@ __global__ void __launch_bounds__(1024) k_scan_sums(uint32_t* __restrict__ sums,
 }
 
 __global__ void __launch_bounds__(kBlock) k_scan_apply(uint32_t* __restrict__ data, int64_t total,
-                                                        const uint32_t* __restrict__ sums) {
+                                                        const uint32_t* __restrict__ sums,
+                                                        const int32_t* __restrict__ gate = nullptr) {
+  if (gate && *gate == 0) return;
   __shared__ uint32_t tile[SC_CHUNK];
   __shared__ uint32_t tsum[kBlock];
   const int tid = threadIdx.x;
@@ -180,7 +186,9 @@ __global__ void __launch_bounds__(kBlock) k_scan_apply(uint32_t* __restrict__ da
 
 // Exclusive scan of `total` uint32 values in one block of 1024 threads
 // (kept for totals small enough that one launch is cheaper).
-__global__ void __launch_bounds__(1024) k_scan_exclusive(uint32_t* __restrict__ data, int64_t total) {
+__global__ void __launch_bounds__(1024) k_scan_exclusive(uint32_t* __restrict__ data, int64_t total,
+                                                          const int32_t* __restrict__ gate = nullptr) {
+  if (gate && *gate == 0) return;
   __shared__ uint32_t part[1024];
   const int tid = threadIdx.x;
   const int64_t chunk = (total + 1023) / 1024;
@@ -307,7 +315,8 @@ hipError_t radix_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, i
 // rank pass reads a handful of ids per element.
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kBlock) k_cs_zero(int32_t* __restrict__ cnt, int32_t* __restrict__ cur,
-                                                     int64_t n_cells) {
+                                                     int64_t n_cells, const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (c > n_cells) return;
   cnt[c] = 0;
@@ -315,7 +324,8 @@ __global__ void __launch_bounds__(kBlock) k_cs_zero(int32_t* __restrict__ cnt, i
 }
 
 __global__ void __launch_bounds__(kBlock) k_cs_count(const uint32_t* __restrict__ keys, int64_t n,
-                                                      int32_t* __restrict__ cnt) {
+                                                      int32_t* __restrict__ cnt, const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   atomicAdd(&cnt[keys[i]], 1);
@@ -323,7 +333,9 @@ __global__ void __launch_bounds__(kBlock) k_cs_count(const uint32_t* __restrict_
 
 __global__ void __launch_bounds__(kBlock) k_cs_place(const uint32_t* __restrict__ keys, int64_t n,
                                                       const int32_t* __restrict__ starts, int32_t* __restrict__ cur,
-                                                      uint32_t* __restrict__ tkey, int32_t* __restrict__ tid) {
+                                                      uint32_t* __restrict__ tkey, int32_t* __restrict__ tid,
+                                                      const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const uint32_t k = keys[i];
@@ -334,7 +346,9 @@ __global__ void __launch_bounds__(kBlock) k_cs_place(const uint32_t* __restrict_
 
 __global__ void __launch_bounds__(kBlock) k_cs_rank(const uint32_t* __restrict__ tkey, const int32_t* __restrict__ tid,
                                                      int64_t n, const int32_t* __restrict__ starts,
-                                                     uint32_t* __restrict__ sk, int32_t* __restrict__ perm) {
+                                                     uint32_t* __restrict__ sk, int32_t* __restrict__ perm,
+                                                     const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   const int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (j >= n) return;
   const uint32_t k = tkey[j];
@@ -360,7 +374,7 @@ hipError_t launch_starts_to_ends(const int32_t* starts, int64_t n_cells, int32_t
 // keys_in may be ctx->keys[1]: it is read only before the rank pass writes
 // the sorted keys there.
 hipError_t cell_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, int64_t n_cells, uint32_t** keys_out,
-                           int32_t** vals_out, int32_t* starts, hipStream_t s) {
+                           int32_t** vals_out, int32_t* starts, hipStream_t s, const int32_t* gate) {
   const int64_t total = n_cells + 1;
   const int nchunks = (int)((total + SC_CHUNK - 1) / SC_CHUNK);
   const size_t nn = (size_t)(n > 0 ? n : 1);
@@ -374,20 +388,20 @@ hipError_t cell_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, in
   uint32_t* sk = (uint32_t*)ctx->keys[1].p;
   int32_t* perm = (int32_t*)ctx->vals[1].p;
   int32_t* cur = (int32_t*)ctx->ccur.p;
-  hipLaunchKernelGGL(k_cs_zero, dim3(nblocks(total)), dim3(kBlock), 0, s, starts, cur, n_cells);
-  if (n > 0) hipLaunchKernelGGL(k_cs_count, dim3(nblocks(n)), dim3(kBlock), 0, s, keys_in, n, starts);
+  hipLaunchKernelGGL(k_cs_zero, dim3(nblocks(total)), dim3(kBlock), 0, s, starts, cur, n_cells, gate);
+  if (n > 0) hipLaunchKernelGGL(k_cs_count, dim3(nblocks(n)), dim3(kBlock), 0, s, keys_in, n, starts, gate);
   uint32_t* cnt = (uint32_t*)starts;  // counts are non-negative: scan them as uint32
   if (nchunks <= 1) {
-    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, cnt, total);
+    hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, cnt, total, gate);
   } else {
     uint32_t* sums = (uint32_t*)ctx->hist.p;
-    hipLaunchKernelGGL(k_scan_totals, dim3(nchunks), dim3(kBlock), 0, s, cnt, total, sums);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, nchunks);
-    hipLaunchKernelGGL(k_scan_apply, dim3(nchunks), dim3(kBlock), 0, s, cnt, total, sums);
+    hipLaunchKernelGGL(k_scan_totals, dim3(nchunks), dim3(kBlock), 0, s, cnt, total, sums, gate);
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, nchunks, gate);
+    hipLaunchKernelGGL(k_scan_apply, dim3(nchunks), dim3(kBlock), 0, s, cnt, total, sums, gate);
   }
   if (n > 0) {
-    hipLaunchKernelGGL(k_cs_place, dim3(nblocks(n)), dim3(kBlock), 0, s, keys_in, n, starts, cur, tkey, tid);
-    hipLaunchKernelGGL(k_cs_rank, dim3(nblocks(n)), dim3(kBlock), 0, s, tkey, tid, n, starts, sk, perm);
+    hipLaunchKernelGGL(k_cs_place, dim3(nblocks(n)), dim3(kBlock), 0, s, keys_in, n, starts, cur, tkey, tid, gate);
+    hipLaunchKernelGGL(k_cs_rank, dim3(nblocks(n)), dim3(kBlock), 0, s, tkey, tid, n, starts, sk, perm, gate);
   }
   *keys_out = sk;
   *vals_out = perm;

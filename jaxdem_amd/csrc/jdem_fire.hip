@@ -231,6 +231,7 @@ __global__ void __launch_bounds__(kBlock) k_fire_gravity_energy(FireArgs a) {
 
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_prec_keys(PrecArgs a) {
+  if (a.gated && *a.moved == 0) return;  // no sphere changed cell: the last sort stands
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const int32_t* c = a.cidx + D * i;
@@ -242,6 +243,10 @@ __global__ void __launch_bounds__(kBlock) k_prec_keys(PrecArgs a) {
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_prec_gather(PrecArgs a) {
   const int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (kk == 0) {  // after this evaluation's sort, before its max|F| reduction
+    if (a.gmax_bits) *a.gmax_bits = 0u;
+    if (a.gated) *a.moved = 0;
+  }
   if (kk >= a.n) return;
   const int64_t j = a.perm[kk];
   a.sp[kk] = make_float4(a.off[D * j], a.off[D * j + 1], D == 3 ? a.off[D * j + 2] : 0.0f, a.rad[j]);

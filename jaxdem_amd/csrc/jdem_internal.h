@@ -145,8 +145,10 @@ hipError_t radix_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, i
 // per-cell counts, their exclusive scan (written to starts[0..n_cells],
 // starts[n_cells] = n), placement through per-cell cursors, then each
 // element's rank among its cell's ids.  Cost grows with n + n_cells.
+// With gate, every kernel of the sort returns at once while *gate == 0 (a
+// device word), leaving the previous sort's outputs in place.
 hipError_t cell_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, int64_t n_cells, uint32_t** keys_out,
-                           int32_t** vals_out, int32_t* starts, hipStream_t s);
+                           int32_t** vals_out, int32_t* starts, hipStream_t s, const int32_t* gate = nullptr);
 // cend[c] = starts[c + 1] for c < n_cells
 hipError_t launch_starts_to_ends(const int32_t* starts, int64_t n_cells, int32_t* cend, hipStream_t s);
 // whether cell_sort_pairs is the cheaper sort for n keys below n_cells
@@ -469,6 +471,8 @@ struct PrecArgs {
   double* e_out;         // (N) half the pair energies of each sphere
   double* w_out;         // (N, D*D) half the pair virials, or null
   int32_t* moved;        // set when a sphere changed cell
+  unsigned* gmax_bits;   // max|F| word of the FIRE state, cleared by the gather
+  int gated;             // 1: the key kernel runs only while *moved, and the gather clears *moved
 };
 hipError_t launch_prec_keys(const PrecArgs& a, hipStream_t s);
 hipError_t launch_prec_gather(const PrecArgs& a, hipStream_t s);
