@@ -720,3 +720,47 @@ def test_thermal_parts_and_temperature(device):
     assert not torch.equal(s3.vel, st.vel)
     s4 = T.set_temperature(st, 1.5, can_rotate=False, subtract_drift=False, seed=3)
     assert abs(T.compute_temperature(s4, can_rotate=False, subtract_drift=False) - 1.5) < 1e-5 * 1.5
+
+
+@pytest.mark.parametrize("collider", ["naive", "CellList", "NeighborList"])
+def test_empty_and_single_sphere(device, collider):
+    """N = 0: stepping, energies and rollout frames are no-ops of the right
+    shapes (naive collider; a cell list cannot be created for no spheres,
+    as in the reference, whose CellList.Create takes the min of the radii,
+    cell_list.py:400-402).  N = 1 in a periodic box: no pair, so velocity
+    Verlet moves the sphere ballistically, x += v dt per step in float32,
+    bit for bit."""
+    print(f"[edge] empty and single sphere, {collider}", flush=True)
+    box = np.full(3, 6.0, np.float32)
+    for n in (0, 1):
+        pos = np.full((n, 3), 1.25, np.float32)
+        vel = np.tile(np.array([[0.3, -0.7, 1.1]], np.float32), (n, 1))
+        st = jd.State.create(pos=pos, vel=vel, rad=np.full(n, 0.5, np.float32), mass=np.ones(n, np.float32),
+                             device=device)
+        mt = jd.MaterialTable.from_materials([jd.Material.create("elastic", density=1.0, young=1e4, poisson=0.3)])
+        ckw = {}
+        if collider == "CellList":
+            ckw = dict(state=st, cell_size=1.0)
+        if collider == "NeighborList":
+            ckw = dict(state=st, cutoff=1.0, skin=0.1, max_neighbors=8, secondary_collider_type="CellList",
+                       secondary_collider_kw=dict(state=st, cell_size=1.0))
+        kw = dict(dt=1e-2, domain_type="periodic", domain_kw=dict(box_size=box, anchor=np.zeros(3, np.float32)),
+                  collider_type=collider, collider_kw=ckw, force_model_type="spring", mat_table=mt)
+        if n == 0 and collider != "naive":
+            with pytest.raises(ValueError):
+                jd.System.create(st.shape, **kw)
+            continue
+        sy = jd.System.create(st.shape, **kw)
+        s2, y2 = jd.System.step(st, sy, n=25)
+        assert s2.pos_c.shape == (n, 3) and y2.step_count == sy.step_count + 25
+        assert sy.collider.compute_potential_energy(s2, y2)[2] == 0.0
+        want = pos.copy()
+        for _ in range(25):
+            want = want + vel * np.float32(1e-2)
+        assert_bitwise(f"n={n} pos", s2.pos_c.cpu().numpy(), want)
+        assert_bitwise(f"n={n} vel", s2.vel.cpu().numpy(), vel)
+        if collider == "NeighborList":
+            _, _, traj = jd.System.trajectory_rollout(st, sy, strides=np.array([5, 20]), save_fn=jd.save_state)
+            assert traj.pos_c.shape == (2, n, 3)
+            if n:
+                assert_bitwise("frame 1", traj.pos_c[1].cpu().numpy(), want)
