@@ -764,3 +764,25 @@ def test_empty_and_single_sphere(device, collider):
             assert traj.pos_c.shape == (2, n, 3)
             if n:
                 assert_bitwise("frame 1", traj.pos_c[1].cpu().numpy(), want)
+
+
+def test_neighborlist_wide_rows_friction_bitwise(device):
+    """Rows of more than 64 slots (a large skin): the step kernel's 4-word
+    row form and several contact-history words per sphere, across
+    rebuilds, bit for bit against the oracle."""
+    print("[step] neighborlist wide rows", flush=True)
+    w = W.spheres_friction(N=4096, seed=7, skin=0.3)
+    w.vel = w.vel * np.float32(40.0)  # fast enough to rebuild every few steps
+    st, sy = W.make_gpu_system(w, device=device)
+    assert sy.collider.max_neighbors > 64, sy.collider.max_neighbors
+    n = 60
+    st2, sy2 = jd.System.step(st, sy, n=n)
+    nls = B.nl_state_of(st, sy)
+    ost = O.step(B.state_of(st), B.params_of(st, sy), n, nls)
+    _compare_states(st2, ost, "nl-wide")
+    assert sy2.collider.n_build_times == nls.n_build
+    assert nls.n_build >= 3, nls.n_build
+    assert_bitwise("nl", sy2.collider.neighbor_list.cpu().numpy().astype(np.int64), nls.nl)
+    xi, live = B.history_of(sy2.collider)
+    assert_bitwise("live", live, nls.live)
+    assert_bitwise("xi", xi, nls.xi)
