@@ -176,6 +176,30 @@ def test_compute_force_bitwise(device, law, collider, domain, dim):
     assert err < 1e-5, err
 
 
+@pytest.mark.parametrize("dim,n", [(3, 5), (3, 14), (2, 3), (2, 5)])
+def test_compute_force_small_periodic_grid(device, dim, n):
+    """Periodic boxes of two cells per axis, smaller than the 3-cell
+    stencil: each neighbour cell is visited once (cell_list.py:420-427),
+    forces bit for bit against the oracle, and the pair set equals brute
+    force."""
+    print(f"[force] small periodic grid dim={dim} n={n}", flush=True)
+    st, box = packed(n, dim, seed=50 + n, domain="periodic", phi=0.35)
+    sy = make_system(st, box, "periodic", "spring", "CellList")
+    g, _ = sy.collider.grid(sy)
+    assert max(int(g.dims[a]) for a in range(dim)) <= 2, [int(g.dims[a]) for a in range(dim)]
+    st2, _ = sy.collider.compute_force(st, sy)
+    ost = B.state_of(st)
+    P = B.params_of(st, sy)
+    O.compute_force(ost, P)
+    assert_bitwise("force", st2.force.cpu().numpy(), ost.force)
+    cutoff = float(np.float32(1.1))
+    _, _, nl, ovf = sy.collider.create_neighbor_list(st, sy, cutoff, 16)
+    assert not ovf
+    got = nl.cpu().numpy().astype(np.int64)
+    pairs = {(i, int(j)) for i in range(n) for j in got[i] if j >= 0}
+    assert pairs == O.brute_force_pairs(ost, P.domain, cutoff, use_inv=False)
+
+
 def test_excluded_pairs_known_answer(device):
     """tests/test_excluded_pairs.py:11-61 of the reference."""
     st = jd.State.create(pos=np.array([[0.0, 0.0], [1.0, 0.0], [2.0, 0.0]], np.float32),
