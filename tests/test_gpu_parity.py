@@ -200,6 +200,23 @@ def test_compute_force_small_periodic_grid(device, dim, n):
     assert pairs == O.brute_force_pairs(ost, P.domain, cutoff, use_inv=False)
 
 
+@pytest.mark.parametrize("dim,n", [(3, 14), (2, 5)])
+def test_neighborlist_small_periodic_grid(device, dim, n):
+    """The resident NeighborList engine on periodic grids of at most two
+    cells per axis (its stencil runs wrap onto themselves): 40 steps bit for
+    bit against the oracle, rows included."""
+    print(f"[step] neighborlist small periodic grid dim={dim} n={n}", flush=True)
+    st, box = packed(n, dim, seed=60 + n, domain="periodic", phi=0.35)
+    sy = make_system(st, box, "periodic", "springfriction", "NeighborList", skin=0.1)
+    g2, _ = sy.collider.secondary_grid(sy)
+    assert max(int(g2.dims[a]) for a in range(dim)) <= 2
+    st2, sy2 = jd.System.step(st, sy, n=40)
+    nls = B.nl_state_of(st, sy)
+    ost = O.step(B.state_of(st), B.params_of(st, sy), 40, nls)
+    _compare_states(st2, ost, "nl-small")
+    assert_bitwise("nl", sy2.collider.neighbor_list.cpu().numpy().astype(np.int64), nls.nl)
+
+
 def test_excluded_pairs_known_answer(device):
     """tests/test_excluded_pairs.py:11-61 of the reference."""
     st = jd.State.create(pos=np.array([[0.0, 0.0], [1.0, 0.0], [2.0, 0.0]], np.float32),
