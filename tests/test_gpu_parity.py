@@ -631,6 +631,22 @@ def test_batch_3d_mixed_domains_equals_alone(device):
         _kinematics_equal(sa, sb, f"sys{k}")
 
 
+def test_batch_ragged_sizes_equal_alone(device):
+    """Systems of 1, 63, 64, 65 and 200 disks in one batch (boundaries inside
+    and on wavefront edges, a system with no pair): each equals the same
+    system stepped alone, bit for bit, after rebuilds."""
+    print("[batch] ragged sizes", flush=True)
+    ws = [W.disks_hertz_friction(N, seed=70 + k, phi=0.6, skin=0.05) for k, N in enumerate([1, 63, 64, 65, 200])]
+    for w in ws:
+        w.vel = w.vel * np.float32(20.0)  # rebuilds within the run
+    pairs = [W.make_gpu_system(w, device=device) for w in ws]
+    out = jd.Batch.pack(pairs).step(40).unpack()
+    for k, ((s0, y0), (sb, yb)) in enumerate(zip(pairs, out)):
+        sa, ya = jd.System.step(s0, y0, n=40)
+        _kinematics_equal(sa, sb, f"sys{k}")
+        assert ya.collider.n_build_times == yb.collider.n_build_times
+
+
 def test_batch_split_steps(device):
     """batch.step(30) == batch.step(12).step(18)."""
     ws = [W.disks_hertz_friction(N, seed=10 + k, phi=0.7) for k, N in enumerate([500, 900])]
