@@ -843,3 +843,28 @@ def test_neighborlist_wide_rows_friction_bitwise(device):
     xi, live = B.history_of(sy2.collider)
     assert_bitwise("live", live, nls.live)
     assert_bitwise("xi", xi, nls.xi)
+
+
+def test_full_size_properties(device):
+    """The benchmark workload at full size (2^20 frictional spheres), where
+    the oracle is too slow: step(12) then step(18) equals step(30) bit for
+    bit (rebuilds included), rollout frames equal the split steps, and the
+    total momentum (periodic box, pair forces equal and opposite) is kept to
+    float32 rounding of the per-sphere updates."""
+    print("[full size] spheres_1048576_friction", flush=True)
+    w = W.spheres_friction(N=1 << 20, seed=0, skin=0.2)
+    w.vel = w.vel * np.float32(10.0)  # rebuilds within the run
+    st, sy = W.make_gpu_system(w, device=device)
+    a, ya = jd.System.step(st, sy, n=30)
+    b1, yb1 = jd.System.step(st, sy, n=12)
+    b, yb = jd.System.step(b1, yb1, n=18)
+    _kinematics_equal(a, b, "split")
+    assert ya.__dict__["_last_stats"].rebuilds >= 1
+    _, _, traj = jd.System.trajectory_rollout(st, sy, strides=np.array([12, 18]), save_fn=jd.save_state)
+    assert_bitwise("frame 0", traj.pos_c[0].cpu().numpy(), b1.pos_c.cpu().numpy())
+    assert_bitwise("frame 1", traj.pos_c[1].cpu().numpy(), a.pos_c.cpu().numpy())
+    m = st.mass.double()[:, None]
+    p0 = (m * st.vel.double()).sum(dim=0)
+    p1 = (m * a.vel.double()).sum(dim=0)
+    scale = (m * st.vel.double().abs()).sum()
+    assert float((p1 - p0).abs().max()) < 1e-5 * float(scale), (p0, p1)
