@@ -105,13 +105,37 @@ def traffic_from_profiles() -> tuple:
     return (float(m.group(1)), str(p.relative_to(ROOT))) if m else (None, None)
 
 
+def host_cpu() -> dict:
+    """The host's CPUs as the OS reports them: logical CPUs, the ones this
+    process may run on, and the model name."""
+    model = None
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"logical_cpus": os.cpu_count(), "usable_cpus": usable, "model": model,
+            "omp_num_threads_env": os.environ.get("OMP_NUM_THREADS")}
+
+
 def cpu_baseline(w, col, budget_s: float) -> dict:
     import numpy as np
 
     from oracle.port import PortRun
 
-    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(os.cpu_count() or 1, 16)
-    threads = max(1, min(threads, os.cpu_count() or threads))
+    host = host_cpu()
+    # every usable CPU, within the per-GPU CPU share the environment sets
+    # (OMP_NUM_THREADS; 16 threads per GPU on the benchmark hosts)
+    threads = host["usable_cpus"] or 1
+    env = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    if env > 0:
+        threads = min(threads, env)
     run = PortRun(
         w.pos, w.vel, w.rad, w.mass, w.box, w.anchor, domain=w.domain, dt=w.dt, law=w.law, kn=w.young,
         kt=np.float32(w.kt_ratio) * np.float32(w.young), mu=w.mu, damping=w.damping, cutoff=col.cutoff,
@@ -132,6 +156,7 @@ def cpu_baseline(w, col, budget_s: float) -> dict:
         "kind": "port",
         "sample": f"{steps} steps of the same {w.N}-sphere workload after its first list build "
         f"(C++/OpenMP port of the oracle, bit-identical to it; {secs:.1f} s)",
+        "host": host,
     }
 
 

@@ -1184,6 +1184,13 @@ int jdem_contact_virial(jdem_ctx* ctx, jdem_system* sys, double* w) {
     for (int c = 0; c < D * D; ++c) w[c] = 0.0;
     return JDEM_OK;
   }
+  // a NeighborList that is unbuilt or stale is refreshed first, as
+  // get_pair_forces_and_ids does through create_neighbor_list (contacts.py:58-62)
+  if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST) {
+    bool rebuilt = false;
+    rc = nl_check_and_rebuild(ctx, sys, &rebuilt);
+    if (rc) return rc;
+  }
   return reduce_f64(ctx, sys, D * D, w, launch_virial, true);
 }
 

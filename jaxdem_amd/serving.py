@@ -119,6 +119,13 @@ class Server:
                 if self.policy == "sjf":
                     k = min(range(len(queue)), key=lambda i: (queue[i].n * queue[i].steps, queue[i].arrival,
                                                              queue[i].rid))
+                if queue[k].steps <= 0:
+                    # nothing to run: the result is the request's own state
+                    r = queue.pop(k)
+                    r.admitted = r.finished = now()
+                    r.result = (r.state, r.system)
+                    done.append(r)
+                    continue
                 fp = footprint_bytes(queue[k].state, queue[k].system)
                 if used + fp > self.hbm_budget and (active or admit):
                     break

@@ -581,14 +581,19 @@ def _collider_potential_energy(state: State, system: System) -> tuple:
     return state, replace(sys2, collider=_collider_after(col, S)), float(e.value)
 
 
-def _contact_virial(state: State, system: System) -> np.ndarray:
+def _contact_virial(state: State, system: System) -> tuple:
+    """Float64 contact virial sum_{i<j} r_ij (x) F_ij and the system whose
+    NeighborList was refreshed for it (a list that is unbuilt or stale is
+    rebuilt first)."""
+    col = _fresh_collider(system.collider)
+    sys2 = replace(system, collider=col)
     keep: list = []
-    S = _system_struct(state, system, keep)
+    S = _system_struct(state, sys2, keep)
     ctx = _ctx(state)
     D = state.dim
     w = (ctypes.c_double * (D * D))()
     ctx.check(ctx.lib.jdem_contact_virial(ctx.ptr, ctypes.byref(S), w), "jdem_contact_virial")
-    return np.array(list(w), dtype=np.float64).reshape(D, D)
+    return np.array(list(w), dtype=np.float64).reshape(D, D), replace(sys2, collider=_collider_after(col, S))
 
 
 def _kinetic_energy_parts(state: State, system: System) -> tuple:

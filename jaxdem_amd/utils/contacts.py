@@ -14,10 +14,12 @@ def compute_contact_stress_tensor(state, system, cutoff=None, max_neighbors=None
     virial over the volume (the box volume unless ``volume`` is given), as
     (state, system, stress (dim, dim) float64).  The contacts come from the
     system's own collider, so ``cutoff`` and ``max_neighbors`` (the
-    reference's neighbour-list sizing for this call) are not needed."""
+    reference's neighbour-list sizing for this call) are not needed; a
+    NeighborList that is unbuilt or stale is rebuilt first and the returned
+    system carries the refreshed list, as in the reference (contacts.py:58-62)."""
     from ..system import _contact_virial
 
-    W = _contact_virial(state, system)
+    W, system = _contact_virial(state, system)
     V = float(np.prod(np.asarray(system.domain.box_size, np.float64))) if volume is None else float(volume)
     return state, system, W / V
 

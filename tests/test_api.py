@@ -267,6 +267,20 @@ def test_server_rejects_bad_policy_and_cap():
     assert SV.Server("cpu", hbm_budget=1, policy="sjf", max_active=3).policy == "sjf"
 
 
+def test_server_finishes_zero_step_requests_at_admission():
+    """A request of 0 steps returns its own state at once and never enters
+    the batch (no step of 0 and no division by it)."""
+    from jaxdem_amd import serving as SV
+    from jaxdem_amd import workloads as W
+
+    st, sy = W.make_gpu_system(W.polydisperse_spheres(300, seed=2, phi=0.4, domain="periodic", mu=0.2), device="cpu")
+    reqs = [SV.Request(k, st, sy, 0, arrival=0.0) for k in range(3)]
+    done = SV.Server("cpu", hbm_budget=1).run(reqs, realtime=False)
+    assert sorted(r.rid for r in done) == [0, 1, 2]
+    for r in done:
+        assert r.result[0] is st and r.result[1] is sy and r.latency >= 0.0
+
+
 def test_trajectory_rollout_save_state_signature():
     # save_state keeps the state of a frame and drops the system
     assert jd.save_state("s", "y") == "s"

@@ -263,7 +263,12 @@ def test_step_spheres_1k_bitwise(device):
     ost = O.step(ost, P, 40)
     _compare_states(st2, ost, "spheres_1k")
     assert sy2.step_count == 40
-    assert np.float32(sy2.time) == np.float32(np.float32(0.0) + sum([np.float32(1e-3)] * 0) + 0) or True
+    # the clock is 40 sequential float32 additions of dt (system.py:73)
+    t = np.float32(0.0)
+    for _ in range(40):
+        t = np.float32(t + np.float32(w.dt))
+    assert np.float32(sy2.time) == t
+    assert t != np.float32(0.04)  # 0.039999988: a product 40 * dt would differ
 
 
 @pytest.mark.parametrize("domain", ["periodic", "reflect"])
@@ -458,17 +463,31 @@ def test_potential_energy(device, law, collider):
     assert ke == pytest.approx(O.kinetic_energy(ost), rel=1e-12)
 
 
-@pytest.mark.parametrize("collider", ["naive", "CellList"])
+@pytest.mark.parametrize("collider", ["naive", "CellList", "NeighborList", "NeighborList-rescaled"])
 @pytest.mark.parametrize("dim", [3, 2])
 def test_contact_stress_and_pressure(device, collider, dim):
     """Contact stress (contacts.py:89-164) against the oracle's float64
     virial sum of the same float32 pair forces: 1e-12 of the largest
-    component (different summation order)."""
+    component (different summation order).  A NeighborList that was never
+    built, or was marked stale by ``scale_to_packing_fraction``, is rebuilt
+    first, as ``get_pair_forces_and_ids`` does (contacts.py:58-62)."""
     from jaxdem_amd.utils.contacts import compute_contact_pressure, compute_contact_stress_tensor
+    from jaxdem_amd.utils.packing_utils import compute_packing_fraction, scale_to_packing_fraction
 
+    print(f"[stress] collider={collider} dim={dim}", flush=True)
     st, box = packed(1500 if dim == 3 else 1200, dim, seed=31 + dim, domain="periodic")
-    sy = make_system(st, box, "periodic", "hertz", collider)
-    _, _, stress = compute_contact_stress_tensor(st, sy)
+    sy = make_system(st, box, "periodic", "hertz", collider.split("-")[0])
+    if collider == "NeighborList-rescaled":
+        # build the list on the loose packing, then compress by 3 %
+        sy = compute_contact_stress_tensor(st, sy)[1]
+        assert sy.collider.n_build_times == 1
+        st, sy = scale_to_packing_fraction(st, sy, float(compute_packing_fraction(st, sy)) * 1.03)
+        box = np.asarray(sy.domain.box_size, np.float32)
+        assert sy.collider.n_build_times == 0
+    _, sy_out, stress = compute_contact_stress_tensor(st, sy)
+    if collider.startswith("NeighborList"):
+        assert sy_out.collider.n_build_times == 1  # refreshed for this call
+        collider = "naive"  # all pairs: the list must hold every contact
     ost = B.state_of(st)
     P = B.params_of(st, sy)
     if collider == "naive":

@@ -289,8 +289,12 @@ def test_periodic_min_image_invariants():
     assert np.all(np.abs(r) <= B / 2 + 1e-12)
     r2 = O.displacement(a + B * rng.integers(-3, 4, (500, 3)), b, d)
     np.testing.assert_allclose(r, r2, atol=1e-9)
-    # round half to even, as jnp.round
-    assert O.displacement(np.array([1.5]), np.array([0.0]), O.Domain("periodic", np.array([1.0]), np.zeros(1)))[0] == 0.5 - 0.0 or True
+    # round half to even, as jnp.round: 0.5 -> 0, 1.5 -> 2, 2.5 -> 2, -2.5 -> -2
+    # (half away from zero would give -0.5, -0.5, -0.5, 0.5)
+    unit = O.Domain("periodic", np.array([1.0]), np.zeros(1))
+    for x, want in ((0.5, 0.5), (1.5, -0.5), (2.5, 0.5), (-2.5, -0.5)):
+        assert O.displacement(np.array([x]), np.array([0.0]), unit)[0] == want, x
+        assert O.displacement_(np.array([x]), np.array([0.0]), unit)[0] == want, x
 
 
 # --------------------------------------------------------------------------
