@@ -270,6 +270,21 @@ int jdem_cell_build(jdem_ctx* ctx, const jdem_state* st, const jdem_domain* dom,
                     const jdem_grid* grid, int32_t* perm, int32_t* sorted_hash,
                     int32_t* cell_start, int32_t* cell_end);
 
+/* jdem_cell_build without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments and the same outputs as jdem_cell_build; the
+ * sort's scratch lives in one fixed device array of
+ * jdem_noalloc_scratch_bytes() bytes (64 MiB) per GPU, held until the
+ * process ends.  Sizes whose scratch does not fit (about 1.4 M particles on
+ * a counting-sort grid) return JDEM_EINVAL with a message.  All calls on
+ * one GPU share that array: calls that overlap on one GPU race. */
+int jdem_cell_build_noalloc(jdem_ctx* ctx, const jdem_state* st, const jdem_domain* dom,
+                            const jdem_grid* grid, int32_t* perm, int32_t* sorted_hash,
+                            int32_t* cell_start, int32_t* cell_end);
+/* Bytes of the fixed scratch array of jdem_cell_build_noalloc. */
+size_t jdem_noalloc_scratch_bytes(void);
+
 /* Candidate pairs within `cutoff`: DynamicCellList.create_neighbor_list
  * (cell_list.py:500-595) when collider_kind is CELL_LIST (grid already
  * inflated to the cutoff, cell_list.py:537-538), or

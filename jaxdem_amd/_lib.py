@@ -248,6 +248,11 @@ SIGNATURES = {
         c_int,
         [c_void_p, POINTER(JdemState), POINTER(JdemDomain), POINTER(JdemGrid), c_void_p, c_void_p, c_void_p, c_void_p],
     ),
+    "jdem_cell_build_noalloc": (
+        c_int,
+        [c_void_p, POINTER(JdemState), POINTER(JdemDomain), POINTER(JdemGrid), c_void_p, c_void_p, c_void_p, c_void_p],
+    ),
+    "jdem_noalloc_scratch_bytes": (c_size_t, []),
     "jdem_neighbor_list": (
         c_int,
         [c_void_p, POINTER(JdemSystem), c_int32, POINTER(JdemGrid), c_float, c_int32, c_void_p, POINTER(c_int32)],

@@ -20,7 +20,8 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 INCLUDE = ROOT / "include"
 LIB = PKG / "libjdem_hip.so"
-SOURCES = ["jdem_cells.hip", "jdem_step.hip", "jdem_resident.hip", "jdem_fire.hip", "jdem_abi.hip"]
+SOURCES = ["jdem_cells.hip", "jdem_step.hip", "jdem_resident.hip", "jdem_fire.hip", "jdem_abi.hip",
+           "jdem_noalloc.hip"]
 ARCH = os.environ.get("JDEM_OFFLOAD_ARCH", "gfx950")
 
 HIP_FLAGS = [

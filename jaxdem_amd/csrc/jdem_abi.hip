@@ -52,29 +52,6 @@ bool ensure(jdem_ctx* ctx, Buf& b, size_t bytes) {
   b.bytes = want;
   return true;
 }
-}  // namespace jdem
-
-namespace {
-
-int fail(jdem_ctx* ctx, int code, const std::string& msg) {
-  if (ctx) ctx->err = msg;
-  return code;
-}
-
-#define HIPCHK(ctx, expr)                                                                     \
-  do {                                                                                        \
-    hipError_t e_ = (expr);                                                                   \
-    if (e_ != hipSuccess)                                                                     \
-      return fail((ctx), e_ == hipErrorOutOfMemory ? JDEM_ENOMEM : JDEM_EHIP,                 \
-                  std::string(#expr) + ": " + hipGetErrorString(e_));                         \
-  } while (0)
-
-#define ENSURE(ctx, buf, bytes)                                                   \
-  do {                                                                            \
-    if (!ensure((ctx), (buf), (bytes)))                                           \
-      return fail((ctx), JDEM_ENOMEM, "scratch allocation failed: " #buf);        \
-  } while (0)
-
 DomP dom_of(const jdem_domain& d) {
   DomP p;
   p.kind = d.kind;
@@ -99,6 +76,29 @@ GridP grid_of(const jdem_grid& g) {
   p.n_cells = g.n_cells;
   return p;
 }
+
+}  // namespace jdem
+
+namespace {
+
+int fail(jdem_ctx* ctx, int code, const std::string& msg) {
+  if (ctx) ctx->err = msg;
+  return code;
+}
+
+#define HIPCHK(ctx, expr)                                                                     \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess)                                                                     \
+      return fail((ctx), e_ == hipErrorOutOfMemory ? JDEM_ENOMEM : JDEM_EHIP,                 \
+                  std::string(#expr) + ": " + hipGetErrorString(e_));                         \
+  } while (0)
+
+#define ENSURE(ctx, buf, bytes)                                                   \
+  do {                                                                            \
+    if (!ensure((ctx), (buf), (bytes)))                                           \
+      return fail((ctx), JDEM_ENOMEM, "scratch allocation failed: " #buf);        \
+  } while (0)
 
 LawP law_of(const jdem_law& l) {
   LawP p;
@@ -213,15 +213,7 @@ struct Cells {
   float4* spos;
 };
 
-int nbits_for(int64_t n_cells) {
-  int64_t m = n_cells > 1 ? n_cells - 1 : 1;
-  int b = 0;
-  while (m > 0) {
-    ++b;
-    m >>= 1;
-  }
-  return b;
-}
+int nbits_for(int64_t n_cells) { return nbits_for_cells(n_cells); }
 
 // Hash + sort + cell ranges + sorted (x, r) of `pos` (cell_list.py:37-87).
 int build_cells(jdem_ctx* ctx, const float* pos, const float* rad, int64_t n, int dim, const DomP& dom,
@@ -984,6 +976,8 @@ int jdem_ctx_create(int device, void* stream, jdem_ctx** out) {
   if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return JDEM_EHIP;
   if (device < 0 || device >= count) return JDEM_EINVAL;
   if (hipSetDevice(device) != hipSuccess) return JDEM_EHIP;
+  // the fixed scratch of jdem_cell_build_noalloc, once per device
+  if (noalloc_init(device) != JDEM_OK) return JDEM_EHIP;
   jdem_ctx* c = new (std::nothrow) jdem_ctx();
   if (!c) return JDEM_ENOMEM;
   c->device = device;

@@ -262,20 +262,21 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t* __rest
   }
 }
 
-hipError_t radix_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, int nbits, uint32_t** keys_out,
-                            int32_t** vals_out, hipStream_t s) {
+size_t radix_hist_words(int64_t n) {
   const int ntiles = (int)((n + RS_TILE - 1) / RS_TILE);
   const int64_t total = (int64_t)ntiles * 256;
   const int nchunks = (int)((total + SC_CHUNK - 1) / SC_CHUNK);
-  if (!ensure(ctx, ctx->keys[0], sizeof(uint32_t) * (size_t)(n > 0 ? n : 1)) ||
-      !ensure(ctx, ctx->keys[1], sizeof(uint32_t) * (size_t)(n > 0 ? n : 1)) ||
-      !ensure(ctx, ctx->vals[0], sizeof(int32_t) * (size_t)(n > 0 ? n : 1)) ||
-      !ensure(ctx, ctx->vals[1], sizeof(int32_t) * (size_t)(n > 0 ? n : 1)) ||
-      !ensure(ctx, ctx->hist, sizeof(uint32_t) * (size_t)(total > 0 ? total : 1) + sizeof(uint32_t) * (nchunks + 1)))
-    return hipErrorOutOfMemory;
-  uint32_t* kb[2] = {(uint32_t*)ctx->keys[0].p, (uint32_t*)ctx->keys[1].p};
-  int32_t* vb[2] = {(int32_t*)ctx->vals[0].p, (int32_t*)ctx->vals[1].p};
-  uint32_t* hist = (uint32_t*)ctx->hist.p;
+  return (size_t)(total > 0 ? total : 1) + (size_t)(nchunks + 1);
+}
+
+hipError_t radix_sort_bufs(const SortBufs& sb, const uint32_t* keys_in, int64_t n, int nbits, uint32_t** keys_out,
+                           int32_t** vals_out, hipStream_t s) {
+  const int ntiles = (int)((n + RS_TILE - 1) / RS_TILE);
+  const int64_t total = (int64_t)ntiles * 256;
+  const int nchunks = (int)((total + SC_CHUNK - 1) / SC_CHUNK);
+  uint32_t* kb[2] = {sb.k0, sb.k1};
+  int32_t* vb[2] = {sb.v0, sb.v1};
+  uint32_t* hist = sb.hist;
   int passes = (nbits + 7) / 8;
   if (passes < 1) passes = 1;
   const uint32_t* kcur = keys_in;
@@ -305,6 +306,23 @@ hipError_t radix_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, i
   *keys_out = (uint32_t*)kcur;
   *vals_out = (int32_t*)vcur;
   return hipSuccess;
+}
+
+hipError_t radix_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, int nbits, uint32_t** keys_out,
+                            int32_t** vals_out, hipStream_t s) {
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  if (!ensure(ctx, ctx->keys[0], sizeof(uint32_t) * nn) || !ensure(ctx, ctx->keys[1], sizeof(uint32_t) * nn) ||
+      !ensure(ctx, ctx->vals[0], sizeof(int32_t) * nn) || !ensure(ctx, ctx->vals[1], sizeof(int32_t) * nn) ||
+      !ensure(ctx, ctx->hist, sizeof(uint32_t) * radix_hist_words(n)))
+    return hipErrorOutOfMemory;
+  SortBufs sb;
+  sb.k0 = (uint32_t*)ctx->keys[0].p;
+  sb.k1 = (uint32_t*)ctx->keys[1].p;
+  sb.v0 = (int32_t*)ctx->vals[0].p;
+  sb.v1 = (int32_t*)ctx->vals[1].p;
+  sb.hist = (uint32_t*)ctx->hist.p;
+  sb.cur = nullptr;
+  return radix_sort_bufs(sb, keys_in, n, nbits, keys_out, vals_out, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -371,30 +389,29 @@ hipError_t launch_starts_to_ends(const int32_t* starts, int64_t n_cells, int32_t
   return hipGetLastError();
 }
 
-// keys_in may be ctx->keys[1]: it is read only before the rank pass writes
-// the sorted keys there.
-hipError_t cell_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, int64_t n_cells, uint32_t** keys_out,
-                           int32_t** vals_out, int32_t* starts, hipStream_t s, const int32_t* gate) {
+size_t cell_sort_sum_words(int64_t n_cells) {
+  const int64_t total = n_cells + 1;
+  return (size_t)((total + SC_CHUNK - 1) / SC_CHUNK) + 1;
+}
+
+// keys_in may be sb.k1: it is read only before the rank pass writes the
+// sorted keys there.
+hipError_t cell_sort_bufs(const SortBufs& sb, const uint32_t* keys_in, int64_t n, int64_t n_cells, uint32_t** keys_out,
+                          int32_t** vals_out, int32_t* starts, hipStream_t s, const int32_t* gate) {
   const int64_t total = n_cells + 1;
   const int nchunks = (int)((total + SC_CHUNK - 1) / SC_CHUNK);
-  const size_t nn = (size_t)(n > 0 ? n : 1);
-  if (!ensure(ctx, ctx->keys[0], sizeof(uint32_t) * nn) || !ensure(ctx, ctx->keys[1], sizeof(uint32_t) * nn) ||
-      !ensure(ctx, ctx->vals[0], sizeof(int32_t) * nn) || !ensure(ctx, ctx->vals[1], sizeof(int32_t) * nn) ||
-      !ensure(ctx, ctx->ccur, sizeof(int32_t) * (size_t)(n_cells > 0 ? n_cells : 1)) ||
-      !ensure(ctx, ctx->hist, sizeof(uint32_t) * (size_t)(nchunks + 1)))
-    return hipErrorOutOfMemory;
-  uint32_t* tkey = (uint32_t*)ctx->keys[0].p;
-  int32_t* tid = (int32_t*)ctx->vals[0].p;
-  uint32_t* sk = (uint32_t*)ctx->keys[1].p;
-  int32_t* perm = (int32_t*)ctx->vals[1].p;
-  int32_t* cur = (int32_t*)ctx->ccur.p;
+  uint32_t* tkey = sb.k0;
+  int32_t* tid = sb.v0;
+  uint32_t* sk = sb.k1;
+  int32_t* perm = sb.v1;
+  int32_t* cur = sb.cur;
   hipLaunchKernelGGL(k_cs_zero, dim3(nblocks(total)), dim3(kBlock), 0, s, starts, cur, n_cells, gate);
   if (n > 0) hipLaunchKernelGGL(k_cs_count, dim3(nblocks(n)), dim3(kBlock), 0, s, keys_in, n, starts, gate);
   uint32_t* cnt = (uint32_t*)starts;  // counts are non-negative: scan them as uint32
   if (nchunks <= 1) {
     hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, cnt, total, gate);
   } else {
-    uint32_t* sums = (uint32_t*)ctx->hist.p;
+    uint32_t* sums = sb.hist;
     hipLaunchKernelGGL(k_scan_totals, dim3(nchunks), dim3(kBlock), 0, s, cnt, total, sums, gate);
     hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, nchunks, gate);
     hipLaunchKernelGGL(k_scan_apply, dim3(nchunks), dim3(kBlock), 0, s, cnt, total, sums, gate);
@@ -406,6 +423,24 @@ hipError_t cell_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, in
   *keys_out = sk;
   *vals_out = perm;
   return hipGetLastError();
+}
+
+hipError_t cell_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, int64_t n_cells, uint32_t** keys_out,
+                           int32_t** vals_out, int32_t* starts, hipStream_t s, const int32_t* gate) {
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  if (!ensure(ctx, ctx->keys[0], sizeof(uint32_t) * nn) || !ensure(ctx, ctx->keys[1], sizeof(uint32_t) * nn) ||
+      !ensure(ctx, ctx->vals[0], sizeof(int32_t) * nn) || !ensure(ctx, ctx->vals[1], sizeof(int32_t) * nn) ||
+      !ensure(ctx, ctx->ccur, sizeof(int32_t) * (size_t)(n_cells > 0 ? n_cells : 1)) ||
+      !ensure(ctx, ctx->hist, sizeof(uint32_t) * cell_sort_sum_words(n_cells)))
+    return hipErrorOutOfMemory;
+  SortBufs sb;
+  sb.k0 = (uint32_t*)ctx->keys[0].p;
+  sb.k1 = (uint32_t*)ctx->keys[1].p;
+  sb.v0 = (int32_t*)ctx->vals[0].p;
+  sb.v1 = (int32_t*)ctx->vals[1].p;
+  sb.hist = (uint32_t*)ctx->hist.p;
+  sb.cur = (int32_t*)ctx->ccur.p;
+  return cell_sort_bufs(sb, keys_in, n, n_cells, keys_out, vals_out, starts, s, gate);
 }
 
 // ---------------------------------------------------------------------------

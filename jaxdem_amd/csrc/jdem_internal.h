@@ -65,6 +65,10 @@ struct jdem_ctx {
 
 namespace jdem {
 
+// Kernel-side parameters of the ABI's domain and grid structs.
+DomP dom_of(const jdem_domain& d);
+GridP grid_of(const jdem_grid& g);
+
 // Ensure b holds at least `bytes`; returns false on allocation failure.
 bool ensure(jdem_ctx* ctx, Buf& b, size_t bytes);
 
@@ -138,6 +142,25 @@ hipError_t launch_virial(const StepArgs& a, int dim, int law, int src, hipStream
 // launchers (jdem_cells.hip)
 hipError_t launch_hash(const float* pos, int64_t n, int dim, const DomP& dom, const GridP& g, uint32_t* keys,
                        hipStream_t s);
+// Buffers of a sort: key and value ping-pong arrays of n entries, a
+// histogram / scan-sum array and (counting sort) per-cell cursors.
+struct SortBufs {
+  uint32_t* k0;
+  uint32_t* k1;
+  int32_t* v0;
+  int32_t* v1;
+  uint32_t* hist;
+  int32_t* cur;
+};
+// uint32 words of SortBufs::hist that a radix sort of n keys uses
+size_t radix_hist_words(int64_t n);
+// uint32 words of SortBufs::hist that a counting sort over n_cells uses
+size_t cell_sort_sum_words(int64_t n_cells);
+// The sorts below on caller-given buffers (no allocation).
+hipError_t radix_sort_bufs(const SortBufs& sb, const uint32_t* keys_in, int64_t n, int nbits, uint32_t** keys_out,
+                           int32_t** vals_out, hipStream_t s);
+hipError_t cell_sort_bufs(const SortBufs& sb, const uint32_t* keys_in, int64_t n, int64_t n_cells, uint32_t** keys_out,
+                          int32_t** vals_out, int32_t* starts, hipStream_t s, const int32_t* gate = nullptr);
 // Sort (keys, iota) by key; results in (keys_out, vals_out).  nbits = key bits.
 hipError_t radix_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, int nbits, uint32_t** keys_out,
                             int32_t** vals_out, hipStream_t s);
@@ -151,6 +174,19 @@ hipError_t cell_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, in
                            int32_t** vals_out, int32_t* starts, hipStream_t s, const int32_t* gate = nullptr);
 // cend[c] = starts[c + 1] for c < n_cells
 hipError_t launch_starts_to_ends(const int32_t* starts, int64_t n_cells, int32_t* cend, hipStream_t s);
+// key bits of a grid of n_cells cells (at least 1)
+inline int nbits_for_cells(int64_t n_cells) {
+  int64_t m = n_cells > 1 ? n_cells - 1 : 1;
+  int b = 0;
+  while (m > 0) {
+    ++b;
+    m >>= 1;
+  }
+  return b;
+}
+// Device address of the fixed scratch array of jdem_cell_build_noalloc,
+// looked up once per device (jdem_noalloc.hip).
+int noalloc_init(int device);
 // whether cell_sort_pairs is the cheaper sort for n keys below n_cells
 inline bool use_cell_sort(int64_t n, int64_t n_cells) { return n > 0 && n_cells <= 4 * n + 65536; }
 hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);

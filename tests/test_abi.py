@@ -24,7 +24,7 @@ HEADER = ROOT / "include" / "jdem.h"
 
 def header_functions() -> list[str]:
     text = HEADER.read_text()
-    return sorted(set(re.findall(r"^\s*(?:int|const char\*)\s+(jdem_\w+)\s*\(", text, flags=re.M)))
+    return sorted(set(re.findall(r"^\s*(?:int|size_t|const char\*)\s+(jdem_\w+)\s*\(", text, flags=re.M)))
 
 
 def test_library_built_and_loads_now():
