@@ -147,6 +147,13 @@ typedef struct jdem_grid {
   int64_t n_cells;
 } jdem_grid;
 
+/* The resident working set of a NeighborList system between calls (opaque):
+ * its work order, build positions, slot-major list, contact-history bits and
+ * springs, kept on the device in the step kernel's layouts.  Made by
+ * jdem_system_step / jdem_rollout when jdem_nlist.resident_out is set;
+ * immutable afterwards, so any number of later calls may start from it. */
+typedef struct jdem_resident jdem_resident;
+
 /* NeighborList collider buffers (neighbor_list.py:137-286). */
 typedef struct jdem_nlist {
   int32_t max_neighbors; /* M                                            */
@@ -161,6 +168,13 @@ typedef struct jdem_nlist {
                           * stencil reaches cutoff + skin                 */
   float* hist_xi;        /* (N, M, D) tangential spring history or NULL  */
   uint32_t* hist_live;   /* (N, ceil(M/32)) slot-carries-history bits     */
+  /* in: when not NULL, the list, build positions and history are read from
+   * this working set instead of nl / old_pos / hist_* (which may be NULL)  */
+  const jdem_resident* resident_in;
+  /* out: when not NULL, a step of the resident path stores the working set
+   * after the call in a new jdem_resident here and leaves nl / old_pos /
+   * hist_* unwritten; jdem_resident_export converts it on demand          */
+  jdem_resident** resident_out;
 } jdem_nlist;
 
 /* Everything one step reads, i.e. the System pytree (system.py:123-226). */
@@ -355,6 +369,22 @@ typedef struct jdem_frames {
  * NeighborList collider only, not with a FREE domain.  The state and
  * sys->nlist end as after jdem_system_step(sum of steps). */
 int jdem_rollout(jdem_ctx* ctx, jdem_system* sys, const jdem_frames* frames, jdem_step_stats* stats);
+
+/* Release a working set made through jdem_nlist.resident_out.  Its device
+ * buffers go back to the context's pool for later calls (no free on the
+ * hot path).  No counterpart in the reference (JAX frees arrays by
+ * reference counting). */
+int jdem_resident_release(jdem_ctx* ctx, jdem_resident* res);
+/* The NeighborList buffers of a working set in the caller's layouts:
+ * nl (N, M) int32 -1 padded, old_pos (N, D), and, when the system carries
+ * history, hist_xi (N, M, D) and hist_live (N, ceil(M/32)) -- the arrays
+ * jdem_system_step writes when resident_out is NULL
+ * (neighbor_list.py:137-286).  hist_xi / hist_live may be NULL. */
+int jdem_resident_export(jdem_ctx* ctx, const jdem_resident* res, int32_t* nl, float* old_pos, float* hist_xi,
+                         uint32_t* hist_live);
+/* Free the device buffers the context keeps in its pool (released working
+ * sets, grown scratch).  The next call allocates what it needs again. */
+int jdem_ctx_trim(jdem_ctx* ctx);
 
 /* ---- (d) batches of independent systems --------------------------------------- */
 

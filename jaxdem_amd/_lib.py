@@ -9,6 +9,7 @@ point raises.
 from __future__ import annotations
 
 import ctypes
+import sys
 import threading
 from ctypes import POINTER, c_double, c_float, c_int, c_int32, c_int64, c_size_t, c_void_p
 from pathlib import Path
@@ -104,6 +105,8 @@ class JdemNlist(ctypes.Structure):
         ("grid", JdemGrid),
         ("hist_xi", c_void_p),
         ("hist_live", c_void_p),
+        ("resident_in", c_void_p),
+        ("resident_out", c_void_p),
     ]
 
 
@@ -271,6 +274,9 @@ SIGNATURES = {
         [c_void_p, POINTER(JdemPrecise), POINTER(JdemFire), c_int64, c_double, c_double, c_double,
          POINTER(JdemMinimizeStats), c_void_p],
     ),
+    "jdem_resident_release": (c_int, [c_void_p, c_void_p]),
+    "jdem_resident_export": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "jdem_ctx_trim": (c_int, [c_void_p]),
     "jdem_minimize": (
         c_int,
         [c_void_p, POINTER(JdemSystem), POINTER(JdemFire), c_int64, c_double, c_double, c_double,
@@ -341,10 +347,59 @@ class Context:
         self.check(self.lib.jdem_ctx_scratch_bytes(self.ptr, ctypes.byref(n)), "jdem_ctx_scratch_bytes")
         return int(n.value)
 
+    def trim(self) -> None:
+        """Free the device buffers pooled by the context (released working
+        sets, grown scratch)."""
+        self.check(self.lib.jdem_ctx_trim(self.ptr), "jdem_ctx_trim")
+
     def __del__(self):  # pragma: no cover - interpreter shutdown order
         try:
             if self.ptr:
                 self.lib.jdem_ctx_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+class Resident:
+    """A NeighborList working set kept on the device between steps
+    (jdem_resident): immutable once made; released to the context's pool when
+    the last reference goes."""
+
+    def __init__(self, ctx: Context, ptr: int, n: int, M: int, dim: int, hist: bool, device):
+        self.ctx = ctx  # keeps the context alive while the working set is
+        self.ptr = c_void_p(ptr)
+        self.n, self.M, self.dim, self.hist = int(n), int(M), int(dim), bool(hist)
+        self.device = device
+
+    def export(self) -> tuple:
+        """(neighbor_list (N, M) int32, old_pos (N, D), history dict or None)
+        in the reference's layouts (neighbor_list.py:137-286)."""
+        import torch
+
+        N, M, D = self.n, self.M, self.dim
+        nl = torch.empty((N, M), dtype=torch.int32, device=self.device)
+        old = torch.empty((N, D), dtype=torch.float32, device=self.device)
+        hist = None
+        if self.hist:
+            hist = {
+                "xi": torch.zeros((N, M, D), dtype=torch.float32, device=self.device),
+                "live": torch.zeros((N, max((M + 31) // 32, 1)), dtype=torch.int32, device=self.device),
+            }
+        self.ctx.set_stream(torch.cuda.current_stream(self.device).cuda_stream)
+        self.ctx.check(
+            self.ctx.lib.jdem_resident_export(
+                self.ctx.ptr, self.ptr, nl.data_ptr(), old.data_ptr(),
+                hist["xi"].data_ptr() if hist else None, hist["live"].data_ptr() if hist else None,
+            ),
+            "jdem_resident_export",
+        )
+        return nl, old, hist
+
+    def __del__(self):  # pragma: no cover - interpreter shutdown order
+        try:
+            if self.ptr and not sys.is_finalizing():
+                self.ctx.lib.jdem_resident_release(self.ctx.ptr, self.ptr)
+                self.ptr = c_void_p()
         except Exception:
             pass
 

@@ -11,6 +11,7 @@ the box this is the reference's hash.
 
 from __future__ import annotations
 
+import copy
 import ctypes
 import math
 import warnings
@@ -166,12 +167,42 @@ class DynamicCellList(Collider):
         return st, sy, nl, bool(o or ovf)
 
 
+class _OnDevice:
+    """Marks a NeighborList buffer that lives in a resident working set on the
+    device (the step kernel's layouts) until it is read."""
+
+    __slots__ = ()
+
+    def __repr__(self) -> str:
+        return "<in the resident working set>"
+
+
+ON_DEVICE = _OnDevice()
+_RESIDENT_FIELDS = frozenset(("neighbor_list", "old_pos", "history"))
+
+
+def _versions(*ts) -> tuple:
+    out = []
+    for t in ts:
+        if isinstance(t, dict):
+            out.extend((id(v), v._version) for v in t.values())
+        elif t is not None:
+            out.append((id(t), t._version))
+    return tuple(out)
+
+
 @Collider.register("NeighborList")
 @dataclass
 class NeighborList(Collider):
     """Verlet list with skin, rebuilt through a secondary collider when any
     particle moved more than skin/2 since the last build; the carrier of pair
-    history (neighbor_list.py:137-669)."""
+    history (neighbor_list.py:137-669).
+
+    After a step on the GPU the list, build positions and history stay in the
+    step's resident working set (work order, slot-major rows, springs by
+    rank); reading ``neighbor_list``, ``old_pos`` or ``history`` converts them
+    to the reference's layouts then, and the next step starts from the working
+    set as long as those arrays are unmodified."""
 
     secondary_collider: Any = None
     neighbor_list: torch.Tensor | None = None
@@ -253,6 +284,48 @@ class NeighborList(Collider):
             overflow=False,
             max_neighbors=int(max_neighbors),
         )
+
+    def __getattribute__(self, name):
+        if name in _RESIDENT_FIELDS:
+            d = object.__getattribute__(self, "__dict__")
+            if d.get(name) is ON_DEVICE:
+                object.__getattribute__(self, "_export_resident")()
+        return object.__getattribute__(self, name)
+
+    def _export_resident(self) -> None:
+        d = self.__dict__
+        nl, old, hist = d["_resident"].export()
+        d["neighbor_list"], d["old_pos"] = nl, old
+        if d.get("history") is ON_DEVICE:
+            d["history"] = hist
+        d["_resident_versions"] = _versions(nl, old, d["history"])
+
+    def resident_for_step(self):
+        """The working set a step may start from: the one the last step left,
+        unless the exported arrays were replaced or modified since."""
+        d = self.__dict__
+        res = d.get("_resident")
+        if res is None:
+            return None
+        if d.get("neighbor_list") is ON_DEVICE:
+            return res
+        if d.get("_resident_versions") == _versions(d["neighbor_list"], d["old_pos"], d["history"]):
+            return res
+        return None
+
+    def with_resident(self, res, n_build_times: int, overflow: bool) -> "NeighborList":
+        """This collider after a step whose list, build positions and history
+        are the working set ``res``."""
+        out = copy.copy(self)
+        d = out.__dict__
+        d["neighbor_list"] = ON_DEVICE
+        d["old_pos"] = ON_DEVICE
+        d["history"] = ON_DEVICE if res.hist else None
+        d["_resident"] = res
+        d["_resident_versions"] = None
+        d["n_build_times"] = int(n_build_times)
+        d["overflow"] = bool(overflow)
+        return out
 
     def secondary_grid(self, system) -> tuple:
         """Grid of the secondary cell list inflated to cutoff + skin

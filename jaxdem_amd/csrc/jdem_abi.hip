@@ -34,9 +34,54 @@ size_t jdem_ctx::scratch_bytes() {
 }
 
 namespace jdem {
+namespace {
+constexpr size_t kPoolMaxBuffers = 24;
+}
+
+void pool_give(jdem_ctx* ctx, Buf& b) {
+  if (!b.p) return;
+  std::lock_guard<std::mutex> lock(ctx->pool_mu);
+  ctx->pool.push_back(b);
+  b = Buf{};
+  if (ctx->pool.size() > kPoolMaxBuffers) {
+    // drop the smallest pooled buffer; the stream may still use it
+    size_t k = 0;
+    for (size_t i = 1; i < ctx->pool.size(); ++i)
+      if (ctx->pool[i].bytes < ctx->pool[k].bytes) k = i;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ctx->pool[k].p);
+    ctx->pool.erase(ctx->pool.begin() + (long)k);
+  }
+}
+
+void pool_trim(jdem_ctx* ctx) {
+  std::lock_guard<std::mutex> lock(ctx->pool_mu);
+  if (ctx->pool.empty()) return;
+  (void)hipStreamSynchronize(ctx->stream);
+  for (Buf& b : ctx->pool) (void)hipFree(b.p);
+  ctx->pool.clear();
+}
+
+// The smallest pooled buffer of at least `bytes` and at most about four
+// times that (a small request does not take a large buffer).
+static bool pool_take(jdem_ctx* ctx, Buf& b, size_t bytes) {
+  std::lock_guard<std::mutex> lock(ctx->pool_mu);
+  long best = -1;
+  for (size_t i = 0; i < ctx->pool.size(); ++i) {
+    const size_t have = ctx->pool[i].bytes;
+    if (have < bytes || have > 4 * bytes + (size_t(1) << 20)) continue;
+    if (best < 0 || have < ctx->pool[(size_t)best].bytes) best = (long)i;
+  }
+  if (best < 0) return false;
+  b = ctx->pool[(size_t)best];
+  ctx->pool.erase(ctx->pool.begin() + best);
+  return true;
+}
+
 bool ensure(jdem_ctx* ctx, Buf& b, size_t bytes) {
   if (bytes == 0) bytes = 4;
   if (b.bytes >= bytes) return true;
+  if (!b.p && pool_take(ctx, b, bytes)) return true;
   if (b.p) {
     // the stream may still use the old buffer
     if (hipStreamSynchronize(ctx->stream) != hipSuccess) return false;
@@ -116,7 +161,7 @@ LawP law_of(const jdem_law& l) {
   return p;
 }
 
-int check_system(jdem_ctx* ctx, const jdem_system* sys, bool batch = false) {
+int check_system(jdem_ctx* ctx, const jdem_system* sys, bool batch = false, bool allow_resident = false) {
   if (!ctx) return JDEM_EINVAL;
   if (!sys) return fail(ctx, JDEM_EINVAL, "null system");
   const jdem_state& s = sys->state;
@@ -141,14 +186,22 @@ int check_system(jdem_ctx* ctx, const jdem_system* sys, bool batch = false) {
   if (hist_law && sys->collider_kind != JDEM_COLLIDER_NONE) {
     if (sys->collider_kind != JDEM_COLLIDER_NEIGHBOR_LIST)
       return fail(ctx, JDEM_EINVAL, "history laws require the NeighborList collider");
-    if (!sys->nlist.hist_xi || !sys->nlist.hist_live)
+    const jdem_resident* R = sys->nlist.resident_in;
+    if (R ? !R->hist : (!sys->nlist.hist_xi || !sys->nlist.hist_live))
       return fail(ctx, JDEM_EINVAL, "history law without history buffers");
   }
   if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST) {
     const jdem_nlist& L = sys->nlist;
     if (L.max_neighbors < 0) return fail(ctx, JDEM_EINVAL, "max_neighbors < 0");
-    if (s.n > 0 && L.max_neighbors > 0 && (!L.nl || !L.old_pos))
+    if ((L.resident_in || L.resident_out) && (!allow_resident || batch))
+      return fail(ctx, JDEM_EUNSUPPORTED, "resident working sets are used by jdem_system_step and jdem_rollout only");
+    if (L.resident_in) {
+      const jdem_resident* R = L.resident_in;
+      if (R->ctx != ctx || R->n != s.n || R->M != L.max_neighbors || R->dim != s.dim)
+        return fail(ctx, JDEM_EINVAL, "resident working set does not match the system (context, N, M or dim)");
+    } else if (s.n > 0 && L.max_neighbors > 0 && (!L.nl || !L.old_pos)) {
       return fail(ctx, JDEM_EINVAL, "neighbour-list buffers missing");
+    }
     if (L.secondary != JDEM_COLLIDER_CELL_LIST && L.secondary != JDEM_COLLIDER_NAIVE)
       return fail(ctx, JDEM_EINVAL, "secondary collider must be CellList or naive");
   }
@@ -458,7 +511,8 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   const int D = s.dim;
   const int M = L.max_neighbors;
   const int W64 = std::max(1, (M + 63) / 64);
-  const bool hist = L.hist_xi && L.hist_live;
+  const jdem_resident* R = L.resident_in;  // working set of an earlier call
+  const bool hist = R ? R->hist : (L.hist_xi && L.hist_live);
   const bool naive2 = L.secondary == JDEM_COLLIDER_NAIVE;
   if (M > 256) return fail(ctx, JDEM_EUNSUPPORTED, "max_neighbors > 256 on the resident path");
   if (!naive2 && !sg && sys->domain.kind == JDEM_DOMAIN_FREE)
@@ -501,7 +555,10 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   int xc = 0;  // springs: read wXi[xc], write wXi[1 - xc]
 
   // --- entry: order the work set by (cell, original id), convert inputs ---
-  if (!naive2 && !sg) {
+  // (from a resident working set: its work order, list and springs as they are)
+  if (R) {
+    HIPCHK(ctx, hipMemcpyAsync(ctx->wOrig[0].p, R->orig.p, ni, hipMemcpyDeviceToDevice, st));
+  } else if (!naive2 && !sg) {
     HIPCHK(ctx, launch_hash(s.pos, N, D, dom, g, (uint32_t*)ctx->wKeys.p, st));
     uint32_t* sk = nullptr;
     int32_t* perm = nullptr;
@@ -548,11 +605,25 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   io.nl = I32(ctx->wNl[0]);
   io.xi = F4(ctx->wXiRank);
   io.live = U64(ctx->wLive[0]);
-  HIPCHK(ctx, ws_launch_in(io, M > 0, st));
+  if (R) {
+    io.old_user = nullptr;
+    io.old_keep = 1;
+    io.nl_user = nullptr;
+    io.xi_user = nullptr;
+    io.live_user = nullptr;
+    HIPCHK(ctx, hipMemcpyAsync(ctx->wOld.p, R->old4.p, n4, hipMemcpyDeviceToDevice, st));
+    HIPCHK(ctx, ws_launch_in(io, false, st));
+    if (M > 0) {
+      HIPCHK(ctx, ws_launch_copy_rows((const int32_t*)R->nl.p, I32(ctx->wNl[0]), N, M, st));
+      HIPCHK(ctx, hipMemcpyAsync(ctx->wLive[0].p, R->live.p, 8 * (size_t)N * W64, hipMemcpyDeviceToDevice, st));
+    }
+  } else {
+    HIPCHK(ctx, ws_launch_in(io, M > 0, st));
+  }
   // the step kernel keeps springs per wavefront
   if (hist && M > 0)
-    HIPCHK(ctx, ws_launch_pack_springs(N, M, W64, F4(ctx->wXiRank), U64(ctx->wLive[0]), F4(ctx->wXi[0]),
-                                       F4(ctx->wW[0]), nullptr, nullptr, st));
+    HIPCHK(ctx, ws_launch_pack_springs(N, M, W64, R ? (const float4*)R->xi.p : F4(ctx->wXiRank),
+                                       U64(ctx->wLive[0]), F4(ctx->wXi[0]), F4(ctx->wW[0]), nullptr, nullptr, st));
 
   // --- rebuild at the current positions (neighbor_list.py:545-589) ---
   // Batches: only the systems that asked for a rebuild are re-sorted and
@@ -905,6 +976,48 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   if (rebuilt_here) L.overflow = ctx->host_flags[2] != 0;
   S->overflow = L.overflow;
   // --- exit: back to the caller's order and layouts ---
+  if (L.resident_out) {
+    // the state in the caller's order; the list, build positions and springs
+    // stay in a new working set (jdem_resident_export converts on demand)
+    io.orig = I32(ctx->wOrig[set]);
+    io.old4 = F4(ctx->wOld);
+    io.P4 = F4(ctx->wP[cur]);
+    io.V4 = F4(ctx->wV[cur]);
+    io.W4 = F4(ctx->wW[cur]);
+    io.Q4 = F4(ctx->wQ[set]);
+    io.I4 = F4(ctx->wI[set]);
+    io.old_user = nullptr;
+    HIPCHK(ctx, ws_launch_out(io, false, st));
+    if (hist && M > 0)
+      HIPCHK(ctx, ws_launch_unpack_springs(N, M, W64, F4(ctx->wXi[xc]), U64(ctx->wLive[nlc]), F4(ctx->wW[cur]),
+                                           F4(ctx->wXiRank), nullptr, nullptr, st));
+    jdem_resident* res = new (std::nothrow) jdem_resident();
+    if (!res) return fail(ctx, JDEM_ENOMEM, "resident working set");
+    res->ctx = ctx;
+    res->n = N;
+    res->M = M;
+    res->W64 = W64;
+    res->dim = D;
+    res->hist = hist;
+    // hand the buffers over; ensure() refills the emptied slots from the pool
+    std::swap(res->orig, ctx->wOrig[set]);
+    std::swap(res->old4, ctx->wOld);
+    std::swap(res->nl, ctx->wNl[nlc]);
+    std::swap(res->live, ctx->wLive[nlc]);
+    if (hist && M > 0) std::swap(res->xi, ctx->wXiRank);
+    *L.resident_out = res;
+    if (free_done) {
+      for (int a = 0; a < D; ++a) {
+        sys->domain.box_size[a] = fbox[a];
+        sys->domain.anchor[a] = fanc[a];
+        sys->domain.inv_box_size[a] = 1.0f / fbox[a];
+        S->free_box[a] = fbox[a];
+        S->free_anchor[a] = fanc[a];
+      }
+    }
+    S->steps = n;
+    return JDEM_OK;
+  }
   io.orig = I32(ctx->wOrig[set]);
   io.old4 = F4(ctx->wOld);
   io.P4 = F4(ctx->wP[cur]);
@@ -992,6 +1105,7 @@ int jdem_ctx_destroy(jdem_ctx* ctx) {
   (void)hipStreamSynchronize(ctx->stream);
   for (Buf* b : ctx->bufs())
     if (b->p) (void)hipFree(b->p);
+  pool_trim(ctx);
   if (ctx->host_flags) (void)hipHostFree(ctx->host_flags);
   for (auto& s : ctx->spans) {
     (void)hipEventDestroy(s.start);
@@ -1013,6 +1127,50 @@ const char* jdem_last_error(const jdem_ctx* ctx) { return ctx ? ctx->err.c_str()
 int jdem_ctx_scratch_bytes(const jdem_ctx* ctx, size_t* bytes) {
   if (!ctx || !bytes) return JDEM_EINVAL;
   *bytes = const_cast<jdem_ctx*>(ctx)->scratch_bytes();
+  return JDEM_OK;
+}
+
+int jdem_ctx_trim(jdem_ctx* ctx) {
+  if (!ctx) return JDEM_EINVAL;
+  (void)hipSetDevice(ctx->device);
+  pool_trim(ctx);
+  return JDEM_OK;
+}
+
+int jdem_resident_release(jdem_ctx* ctx, jdem_resident* res) {
+  if (!ctx) return JDEM_EINVAL;
+  if (!res) return JDEM_OK;
+  if (res->ctx != ctx) return fail(ctx, JDEM_EINVAL, "working set of another context");
+  for (Buf* b : {&res->orig, &res->old4, &res->nl, &res->live, &res->xi}) pool_give(ctx, *b);
+  delete res;
+  return JDEM_OK;
+}
+
+int jdem_resident_export(jdem_ctx* ctx, const jdem_resident* res, int32_t* nl, float* old_pos, float* hist_xi,
+                         uint32_t* hist_live) {
+  if (!ctx) return JDEM_EINVAL;
+  if (!res) return fail(ctx, JDEM_EINVAL, "null working set");
+  if (res->ctx != ctx) return fail(ctx, JDEM_EINVAL, "working set of another context");
+  if (res->n == 0) return JDEM_OK;
+  (void)hipSetDevice(ctx->device);
+  InOut io;
+  std::memset(&io, 0, sizeof(io));
+  io.n = res->n;
+  io.dim = res->dim;
+  io.M = res->M;
+  io.W64 = res->W64;
+  io.orig = (const int32_t*)res->orig.p;
+  io.old4 = (float4*)res->old4.p;
+  io.nl = (int32_t*)res->nl.p;
+  io.live = (uint64_t*)res->live.p;
+  io.xi = (float4*)res->xi.p;
+  io.old_user = old_pos;
+  io.nl_user = nl;
+  // the springs come with the list they belong to
+  io.xi_user = (res->hist && nl) ? hist_xi : nullptr;
+  io.live_user = (res->hist && nl) ? hist_live : nullptr;
+  if (io.xi_user && !io.live_user) return fail(ctx, JDEM_EINVAL, "hist_xi without hist_live");
+  HIPCHK(ctx, ws_launch_export(io, ctx->stream));
   return JDEM_OK;
 }
 
@@ -1653,7 +1811,7 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
 }
 
 int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats* stats) {
-  int rc = check_system(ctx, sys);
+  int rc = check_system(ctx, sys, false, true);
   if (rc) return rc;
   if (n < 0) return fail(ctx, JDEM_EINVAL, "n < 0");
   jdem_step_stats local;
@@ -1813,7 +1971,7 @@ int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
 }
 
 int jdem_rollout(jdem_ctx* ctx, jdem_system* sys, const jdem_frames* fr, jdem_step_stats* stats) {
-  int rc = check_system(ctx, sys);
+  int rc = check_system(ctx, sys, false, true);
   if (rc) return rc;
   if (!fr || fr->n_frames < 0 || (fr->n_frames > 0 && !fr->steps)) return fail(ctx, JDEM_EINVAL, "bad frames");
   if (sys->collider_kind != JDEM_COLLIDER_NEIGHBOR_LIST)

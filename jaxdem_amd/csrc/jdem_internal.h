@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -58,9 +59,30 @@ struct jdem_ctx {
   bool timing = false;
   std::vector<hipEvent_t> event_pool;
   std::vector<TimedSpan> spans;
+  // Device buffers not in use: released working sets (jdem_resident) and
+  // buffers handed over to them leave empty scratch slots that ensure()
+  // refills from here before it allocates.
+  std::vector<jdem::Buf> pool;
+  std::mutex pool_mu;
   // every scratch buffer of the context, for accounting and release
   std::vector<jdem::Buf*> bufs();
   size_t scratch_bytes();
+};
+
+// The NeighborList working set kept between calls (include/jdem.h): the
+// work order and the buffers of the step kernel's layouts, in work order.
+struct jdem_resident {
+  jdem_ctx* ctx = nullptr;
+  int64_t n = 0;
+  int M = 0;
+  int W64 = 1;
+  int dim = 3;
+  bool hist = false;
+  jdem::Buf orig;  // int32 (N): original index of work slot b
+  jdem::Buf old4;  // float4 (N): positions at the last build
+  jdem::Buf nl;    // int32 slot-major (M, N): rows end at their first -1
+  jdem::Buf live;  // uint64 (W64, N): slots that carry a spring
+  jdem::Buf xi;    // float4 by rank (M, N): k-th live spring of slot b
 };
 
 namespace jdem {
@@ -70,7 +92,12 @@ DomP dom_of(const jdem_domain& d);
 GridP grid_of(const jdem_grid& g);
 
 // Ensure b holds at least `bytes`; returns false on allocation failure.
+// An empty b is first refilled from the context's pool.
 bool ensure(jdem_ctx* ctx, Buf& b, size_t bytes);
+// Give b's memory to the context's pool and leave b empty.
+void pool_give(jdem_ctx* ctx, Buf& b);
+// Free every pooled buffer (after the stream has drained).
+void pool_trim(jdem_ctx* ctx);
 
 // Kernel arguments of the fused step / force / energy kernels.
 struct StepArgs {
@@ -262,6 +289,7 @@ struct InOut {
   int32_t* nl;
   float4* xi;
   uint64_t* live;
+  int old_keep;  // entry: old4 already holds the build positions
 };
 
 // Parameters of one system of a segmented batch (jdem_batch_step).  The
@@ -411,6 +439,11 @@ struct WS {
 
 hipError_t ws_launch_in(const InOut& io, bool with_nl, hipStream_t s);
 hipError_t ws_launch_out(const InOut& io, bool with_nl, hipStream_t s);
+// A working set's list, build positions and springs in the caller's layouts
+// (io.nl_user / old_user / xi_user / live_user; each may be null).
+hipError_t ws_launch_export(const InOut& io, hipStream_t s);
+// dst[s*n + b] = src[s*n + b] for the slots of row b up to its first -1
+hipError_t ws_launch_copy_rows(const int32_t* src, int32_t* dst, int64_t n, int M, hipStream_t s);
 // Tangential springs of the step kernel, per wavefront: the 64 work slots
 // of wave w own the region [64 w M, 64 (w + 1) M) of the spring buffer, and
 // slot b's springs (in slot order of its live bits) start at offset W[b].w

@@ -80,7 +80,7 @@ __global__ void __launch_bounds__(kBlock) k_in_state(InOut a) {
   a.I4[b] = make_float4(I[0], I[1], I[2], __int_as_float(bits));
   a.F4[b] = make_float4(f[0], f[1], f[2], 0.0f);
   a.T4[b] = make_float4(t[0], t[1], t[2], 0.0f);
-  a.old4[b] = make_float4(o[0], o[1], o[2], 0.0f);
+  if (!a.old_keep) a.old4[b] = make_float4(o[0], o[1], o[2], 0.0f);
 }
 
 // user (N, M) list of original ids -> slot-major work list; history likewise
@@ -156,6 +156,29 @@ __global__ void __launch_bounds__(kBlock) k_out_nlist(InOut a) {
   }
   if (a.xi_user)
     for (int w = 0; w < W32 && w < 8; ++w) a.live_user[i * W32 + w] = words[w];
+}
+
+// positions at the last build, work order -> the caller's (N, D)
+__global__ void __launch_bounds__(kBlock) k_out_old(InOut a) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= a.n) return;
+  const int64_t i = a.orig[b];
+  const float4 o = a.old4[b];
+  const float oo[3] = {o.x, o.y, o.z};
+  for (int c = 0; c < a.dim; ++c) a.old_user[a.dim * i + c] = oo[c];
+}
+
+// Slot-major rows up to and including their first -1 (every reader of a work
+// row stops there), so a working set is copied in proportion to its entries.
+__global__ void __launch_bounds__(kBlock) k_copy_rows(const int32_t* __restrict__ src, int32_t* __restrict__ dst,
+                                                       int64_t n, int M) {
+  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (b >= n) return;
+  for (int s = 0; s < M; ++s) {
+    const int32_t v = src[(int64_t)s * n + b];
+    dst[(int64_t)s * n + b] = v;
+    if (v < 0) break;
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1194,6 +1217,18 @@ hipError_t ws_launch_out(const InOut& io, bool with_nl, hipStream_t s) {
   dim3 g(nblocks(io.n)), b(kBlock);
   hipLaunchKernelGGL(k_out_state, g, b, 0, s, io);
   if (with_nl) hipLaunchKernelGGL(k_out_nlist, g, b, 0, s, io);
+  return hipGetLastError();
+}
+hipError_t ws_launch_export(const InOut& io, hipStream_t s) {
+  if (io.n == 0) return hipSuccess;
+  dim3 g(nblocks(io.n)), b(kBlock);
+  if (io.old_user) hipLaunchKernelGGL(k_out_old, g, b, 0, s, io);
+  if (io.nl_user && io.M > 0) hipLaunchKernelGGL(k_out_nlist, g, b, 0, s, io);
+  return hipGetLastError();
+}
+hipError_t ws_launch_copy_rows(const int32_t* src, int32_t* dst, int64_t n, int M, hipStream_t s) {
+  if (n == 0 || M == 0) return hipSuccess;
+  hipLaunchKernelGGL(k_copy_rows, dim3(nblocks(n)), dim3(kBlock), 0, s, src, dst, n, M);
   return hipGetLastError();
 }
 hipError_t ws_launch_inverse(const int32_t* orig, int64_t n, int32_t* wpos, hipStream_t s) {

@@ -359,7 +359,10 @@ def _collider_kind(col: Collider) -> int:
     return int(type(col).kind)
 
 
-def _system_struct(state: State, system: System, keep: list) -> L.JdemSystem:
+def _system_struct(state: State, system: System, keep: list, resident=None, for_collider: bool = True) -> L.JdemSystem:
+    """The jdem_system of (state, system).  With ``resident`` a NeighborList's
+    list, build positions and history come from that working set; without
+    ``for_collider`` the collider is left out (kinetic quantities)."""
     _require_gpu(state)
     S = L.JdemSystem()
     S.state = _state_struct(state)
@@ -383,6 +386,9 @@ def _system_struct(state: State, system: System, keep: list) -> L.JdemSystem:
             keep.append(t)
             setattr(S, cname, t.data_ptr())
     col = system.collider
+    if not for_collider:
+        S.collider_kind = L.COLLIDER_NONE
+        return S
     S.collider_kind = _collider_kind(col)
     if isinstance(col, DynamicCellList):
         g, ovf = col.grid(system)
@@ -394,8 +400,11 @@ def _system_struct(state: State, system: System, keep: list) -> L.JdemSystem:
         nl.max_neighbors = col.max_neighbors
         nl.cutoff = float(col.cutoff)
         nl.skin = float(col.skin)
-        nl.nl = _ptr(col.neighbor_list)
-        nl.old_pos = _ptr(col.old_pos)
+        if resident is None:
+            nl.nl = _ptr(col.neighbor_list)
+            nl.old_pos = _ptr(col.old_pos)
+        else:
+            nl.resident_in = resident.ptr
         nl.n_build_times = int(col.n_build_times)
         nl.overflow = 1 if col.overflow else 0
         sec = col.secondary_collider
@@ -409,7 +418,7 @@ def _system_struct(state: State, system: System, keep: list) -> L.JdemSystem:
             nl.secondary = L.COLLIDER_NAIVE
         else:
             raise NotImplementedError(f"NeighborList secondary collider {type(sec).__name__} on the GPU")
-        if col.history is not None:
+        if resident is None and col.history is not None:
             nl.hist_xi = _ptr(col.history["xi"])
             nl.hist_live = _ptr(col.history["live"])
         S.nlist = nl
@@ -449,15 +458,44 @@ def _fusable(state: State, system: System) -> bool:
     )
 
 
+def _resident_call(state: State, system: System, keep: list):
+    """(struct, output slot, collider) of a NeighborList call that keeps its
+    working set: it starts from the collider's working set when that is
+    current, else from the collider's arrays, which it only reads."""
+    col = system.collider
+    res_in = col.resident_for_step()
+    S = _system_struct(state, system, keep, resident=res_in)
+    slot = ctypes.c_void_p()
+    keep.append(slot)
+    S.nlist.resident_out = ctypes.addressof(slot)
+    hist = res_in.hist if res_in is not None else col.history is not None
+    return S, slot, col, hist
+
+
+def _resident_after(ctx, col, S, slot, hist: bool, state: State):
+    """The collider after a call made by _resident_call."""
+    if not slot.value:  # nothing stepped (no particles, or n = 0)
+        return _collider_after(_fresh_collider(col), S)
+    res = L.Resident(ctx, slot.value, state.N, int(col.max_neighbors), state.dim, hist, state.device)
+    return col.with_resident(res, S.nlist.n_build_times, bool(S.nlist.overflow))
+
+
 def _fused_step(state: State, system: System, n: int) -> tuple:
     new_state = state.with_kinematics()
-    col = _fresh_collider(system.collider)
-    sys2 = replace(system, collider=col, domain=replace(system.domain))
-    keep: list = []
-    S = _system_struct(new_state, sys2, keep)
     stats = L.JdemStepStats()
     ctx = _ctx(new_state)
-    ctx.check(ctx.lib.jdem_system_step(ctx.ptr, ctypes.byref(S), int(n), ctypes.byref(stats)), "jdem_system_step")
+    keep: list = []
+    if isinstance(system.collider, NeighborList):
+        sys2 = replace(system, domain=replace(system.domain))
+        S, slot, col, hist = _resident_call(new_state, sys2, keep)
+        ctx.check(ctx.lib.jdem_system_step(ctx.ptr, ctypes.byref(S), int(n), ctypes.byref(stats)), "jdem_system_step")
+        col_out = _resident_after(ctx, col, S, slot, hist, new_state)
+    else:
+        col = _fresh_collider(system.collider)
+        sys2 = replace(system, collider=col, domain=replace(system.domain))
+        S = _system_struct(new_state, sys2, keep)
+        ctx.check(ctx.lib.jdem_system_step(ctx.ptr, ctypes.byref(S), int(n), ctypes.byref(stats)), "jdem_system_step")
+        col_out = _collider_after(col, S)
     dom = sys2.domain
     if isinstance(dom, FreeDomain) and new_state.N > 0:
         box = np.array([S.domain.box_size[a] for a in range(state.dim)], F32)
@@ -466,7 +504,7 @@ def _fused_step(state: State, system: System, n: int) -> tuple:
     else:
         dom = replace(dom, inv_box_size=(F32(1.0) / np.asarray(dom.box_size, F32)).astype(F32))
     fm = replace(sys2.force_manager, external_force=None, external_force_com=None, external_torque=None)
-    out_sys = replace(sys2, collider=_collider_after(col, S), domain=dom, force_manager=fm)
+    out_sys = replace(sys2, collider=col_out, domain=dom, force_manager=fm)
     out_sys = _advance_clock(out_sys, n)
     out_sys.__dict__["_last_stats"] = stats
     return new_state, out_sys
@@ -477,10 +515,9 @@ def _fused_rollout(state: State, system: System, ks: list) -> tuple:
     segment f runs System.step(ks[f]) on the resident working set and its
     end state is written to frame f."""
     new_state = state.with_kinematics()
-    col = _fresh_collider(system.collider)
-    sys2 = replace(system, collider=col, domain=replace(system.domain))
+    sys2 = replace(system, domain=replace(system.domain))
     keep: list = []
-    S = _system_struct(new_state, sys2, keep)
+    S, slot, col, hist = _resident_call(new_state, sys2, keep)
     nf = len(ks)
 
     def frames_like(t: torch.Tensor) -> torch.Tensor:
@@ -519,7 +556,7 @@ def _fused_rollout(state: State, system: System, ks: list) -> tuple:
     traj = State(**kw)
     dom = replace(sys2.domain, inv_box_size=(F32(1.0) / np.asarray(sys2.domain.box_size, F32)).astype(F32))
     fm = replace(sys2.force_manager, external_force=None, external_force_com=None, external_torque=None)
-    out_sys = replace(sys2, collider=_collider_after(col, S), domain=dom, force_manager=fm)
+    out_sys = replace(sys2, collider=_resident_after(ctx, col, S, slot, hist, new_state), domain=dom, force_manager=fm)
     out_sys = _advance_clock(out_sys, sum(ks))
     out_sys.__dict__["_last_stats"] = stats
     return new_state, out_sys, traj
@@ -598,7 +635,7 @@ def _contact_virial(state: State, system: System) -> tuple:
 
 def _kinetic_energy_parts(state: State, system: System) -> tuple:
     keep: list = []
-    S = _system_struct(state, system, keep)
+    S = _system_struct(state, system, keep, for_collider=False)
     ctx = _ctx(state)
     t, r = ctypes.c_double(), ctypes.c_double()
     ctx.check(ctx.lib.jdem_kinetic_energy_parts(ctx.ptr, ctypes.byref(S), ctypes.byref(t), ctypes.byref(r)),
@@ -608,7 +645,7 @@ def _kinetic_energy_parts(state: State, system: System) -> tuple:
 
 def _kinetic_energy(state: State, system: System) -> float:
     keep: list = []
-    S = _system_struct(state, system, keep)
+    S = _system_struct(state, system, keep, for_collider=False)
     ctx = _ctx(state)
     e = ctypes.c_double()
     ctx.check(ctx.lib.jdem_kinetic_energy(ctx.ptr, ctypes.byref(S), ctypes.byref(e)), "jdem_kinetic_energy")

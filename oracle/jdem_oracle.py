@@ -206,6 +206,8 @@ class Params:
     search_range: int = 1
     gravity: np.ndarray | None = None
     interact_same_bond_id: bool = False
+    # all-pairs rows found through a cell list (naive_near_candidates)
+    naive_by_cells: bool = False
     # NeighborList collider (neighbor_list.py:287-480)
     nl_cutoff: float = 0.0
     nl_skin: float = 0.0
@@ -862,10 +864,33 @@ def drop_far_candidates(st: State, P: Params, C: np.ndarray) -> np.ndarray:
     return out[:, : max(width, 1)]
 
 
+def naive_near_candidates(st: State, P: Params) -> np.ndarray:
+    """The rows of ``drop_far_candidates(naive_candidates(N))`` without the
+    N x N distance matrix.
+
+    A kept pair has ``|r_ij| <= sqrt(1.0001) (r_i + r_j) < 2.0004 r_max``, so a
+    cell list with cells of 2.0004 r_max (search range 1) holds every kept
+    pair; its candidates pruned by the same test and sorted by ``j`` are the
+    all-pairs rows, which list the kept ``j`` in ascending order.
+    """
+    N = st.N
+    if N == 0:
+        return np.full((0, 1), -1, np.int64)
+    T_ = st.pos.dtype
+    cs = np.asarray(2.0004, T_) * np.asarray(st.rad.max(), T_)
+    C, _ = cell_candidates(st.pos, P.domain, cs, 1)
+    C = drop_far_candidates(st, P, C)
+    key = np.where(C < 0, np.iinfo(np.int64).max, C)
+    return np.take_along_axis(C, np.argsort(key, axis=1, kind="stable"), axis=1)
+
+
 def compute_force(st: State, P: Params, nls: NLState | None = None):
     """``collider.compute_force`` for naive / CellList / NeighborList."""
     if P.collider == "naive":
-        C = drop_far_candidates(st, P, naive_candidates(st.N))
+        if st.N > 1024 or P.naive_by_cells:
+            C = naive_near_candidates(st, P)
+        else:
+            C = drop_far_candidates(st, P, naive_candidates(st.N))
         F, T, _ = accumulate(st, P, C)
     elif P.collider == "celllist":
         C, _ = cell_candidates(st.pos, P.domain, P.cell_size, P.search_range)

@@ -55,7 +55,7 @@ STRUCT_FIELDS = {
     "jdem_law": ["kind", "n_mat", "t0", "t1", "t2", "m0", "m1", "m2", "m3", "p0", "p1"],
     "jdem_grid": ["dims", "strides", "cell_size", "search_range", "periodic", "n_cells"],
     "jdem_nlist": ["max_neighbors", "cutoff", "skin", "nl", "old_pos", "n_build_times", "overflow", "secondary",
-                   "grid", "hist_xi", "hist_live"],
+                   "grid", "hist_xi", "hist_live", "resident_in", "resident_out"],
     "jdem_system": ["state", "domain", "law", "lin_kind", "rot_kind", "dt", "interact_same_bond_id", "gravity",
                     "ext_force", "ext_force_com", "ext_torque", "collider_kind", "grid", "nlist"],
     "jdem_step_stats": ["steps", "rebuilds", "host_syncs", "overflow", "reserved", "free_box", "free_anchor"],

@@ -271,6 +271,33 @@ def test_step_spheres_1k_bitwise(device):
     assert t != np.float32(0.04)  # 0.039999988: a product 40 * dt would differ
 
 
+@pytest.mark.parametrize("collider", ["naive", "celllist"])
+def test_spheres_1k_rollout_every_frame(device, collider):
+    """The CPU-runnable plain workload in full: 1000 spheres on a 10^3
+    lattice (spacing 0.5, r 0.1, unit density), N(0, 0.1^2) velocities from
+    seed 0, a reflecting box of 20, spring k = 1e4, dt = 1e-3,
+    trajectory_rollout(n=100, stride=10) = 1000 steps.  Each of the 100
+    frames equals the oracle's state after 10 (f + 1) steps, bit for bit."""
+    print(f"[c1] rollout 100 x 10 collider={collider}", flush=True)
+    w = W.spheres_1k()
+    st, sy = W.make_gpu_system(w, device=device, collider=collider)
+    fs, fy, traj = jd.System.trajectory_rollout(st, sy, n=100, stride=10)
+    fr = traj[0]
+    got = {k: getattr(fr, a).cpu().numpy() for k, a in (("pos", "pos_c"), ("vel", "vel"), ("ang_vel", "ang_vel"),
+                                                       ("force", "force"), ("torque", "torque"))}
+    got["q"] = fr.q.data.cpu().numpy()
+    assert got["pos"].shape == (100, 1000, 3)
+    ost = B.state_of(st)
+    P = B.params_of(st, sy)
+    P.naive_by_cells = True
+    for f in range(100):
+        ost = O.step(ost, P, 10)
+        for k in ("pos", "vel", "q", "ang_vel", "force", "torque"):
+            assert_bitwise(f"frame {f} {k}", got[k][f], getattr(ost, k))
+    _compare_states(fs, ost, "end")
+    assert fy.step_count == 1000
+
+
 @pytest.mark.parametrize("domain", ["periodic", "reflect"])
 @pytest.mark.parametrize("law", ["hertz", "cundallstrack"])
 def test_step_celllist_bitwise(device, domain, law):
@@ -555,6 +582,48 @@ def test_neighborlist_split_calls_equal_one_call(device):
     assert int(sy.collider.n_build_times) == 0
 
 
+def test_resident_working_set_semantics(device):
+    """After a step the list, build positions and springs stay in the step's
+    working set on the device.  The next step starts from it; stepping twice
+    from the same (state, system) gives the same result (the working set is
+    immutable); reading the arrays exports them in the reference's layouts;
+    an exported array modified in place sends the next step back to those
+    arrays, with the same result.  All bit for bit, and equal to the oracle."""
+    print("[resident] semantics", flush=True)
+    w = W.spheres_friction(N=4096, seed=5, skin=0.05)
+    st0, sy0 = W.make_gpu_system(w, device=device)
+    st, sy = jd.System.step(st0, sy0, n=20)
+    assert sy.collider.resident_for_step() is not None
+    a1, ya1 = jd.System.step(st, sy, n=15)
+    a2, ya2 = jd.System.step(st, sy, n=15)
+    c1, yc1 = jd.System.step(st, sy, n=7)
+    c, yc = jd.System.step(c1, yc1, n=8)
+    ref = B.state_of(a1)
+    for other, tag in ((a2, "again"), (c, "7+8")):
+        _compare_states(other, ref, tag)
+    ost = B.state_of(st0)
+    nls = B.nl_state_of(st0, sy0)
+    ost = O.step(ost, B.params_of(st0, sy0), 35, nls)
+    _compare_states(a1, ost, "oracle")
+    assert nls.n_build >= 3
+    for y in (ya1, ya2, yc):
+        assert_bitwise("nl", y.collider.neighbor_list.cpu().numpy().astype(np.int64), nls.nl)
+        xi, live = B.history_of(y.collider)
+        assert_bitwise("live", live, nls.live)
+        assert_bitwise("xi", xi, nls.xi)
+    # exported and unmodified: still the working set
+    _ = sy.collider.neighbor_list
+    assert sy.collider.resident_for_step() is not None
+    e1, _ = jd.System.step(st, sy, n=15)
+    _compare_states(e1, ref, "exported")
+    # modified in place (same values): the next step reads the arrays
+    sy.collider.neighbor_list.add_(0)
+    assert sy.collider.resident_for_step() is None
+    f1, yf1 = jd.System.step(st, sy, n=15)
+    _compare_states(f1, ref, "from arrays")
+    assert torch.equal(yf1.collider.neighbor_list, ya1.collider.neighbor_list)
+
+
 # ---------------------------------------------------------------------------
 # (d) FIRE energy minimisation (minimizers/routines.py:150-383)
 # ---------------------------------------------------------------------------
@@ -676,6 +745,76 @@ def test_batch_split_steps(device):
         _kinematics_equal(sa, sb, f"sys{k}")
 
 
+def _oracle_system(st, sy, n):
+    ost = B.state_of(st)
+    nls = B.nl_state_of(st, sy)
+    return O.step(ost, B.params_of(st, sy), n, nls), nls
+
+
+def test_batching_workload_sample_against_oracle(device):
+    """Systems of the batching workload (4096 2D Hertz-friction disk
+    systems, N log-uniform in [256, 16384], area fraction U[0.3, 0.8]) in one
+    batch: the smallest, the largest and the 10/25/50/75/90 % quantiles by N,
+    80 steps with list rebuilds (skin 0.02).  Each system's state, list and
+    springs equal the oracle's bit for bit."""
+    print("[c3] batching workload sample vs oracle", flush=True)
+    ws = W.batch_disks(4096, seed=0, skin=0.02)
+    Ns = np.array([w.N for w in ws])
+    order = np.argsort(Ns, kind="stable")
+    picks = sorted({int(order[int(round(q * (len(order) - 1)))]) for q in (0.0, 0.1, 0.25, 0.5, 0.75, 0.9, 1.0)})
+    sel = [ws[k] for k in picks]
+    assert max(w.N for w in sel) == Ns.max() and min(w.N for w in sel) == Ns.min()
+    pairs = [W.make_gpu_system(w, device=device) for w in sel]
+    n = 80
+    out = jd.Batch.pack(pairs).step(n).unpack()
+    builds = []
+    for k, ((st, sy), (sb, yb)) in enumerate(zip(pairs, out)):
+        print(f"[c3] system {picks[k]} N={sel[k].N}", flush=True)
+        ost, nls = _oracle_system(st, sy, n)
+        _compare_states(sb, ost, f"system {picks[k]} (N={sel[k].N})")
+        assert yb.collider.n_build_times == nls.n_build
+        assert_bitwise("nl", yb.collider.neighbor_list.cpu().numpy().astype(np.int64), nls.nl)
+        xi, live = B.history_of(yb.collider)
+        assert_bitwise("live", live, nls.live)
+        assert_bitwise("xi", xi, nls.xi)
+        builds.append(nls.n_build)
+    assert max(builds) >= 3  # rebuilds inside the run
+
+
+def test_batching_workload_full_size(device):
+    """The whole batching workload (4096 systems, ~15.6 M disks) in one
+    batch: step(12) then step(18) equals step(30) bit for bit for every
+    disk; every system keeps its momentum (periodic boxes, equal and
+    opposite pair forces) to float32 rounding; sampled systems equal the
+    same system stepped alone."""
+    print("[c3] full batching workload", flush=True)
+    ws = W.batch_disks(4096, seed=0, skin=0.1)
+    pairs = [W.make_gpu_system(w, device="cpu") for w in ws]
+    b = jd.Batch.pack(pairs).to(device)
+    one = b.step(30)
+    two = b.step(12).step(18)
+    for f in ("pos_c", "vel", "ang_vel", "force", "torque"):
+        assert torch.equal(getattr(one.state, f), getattr(two.state, f)), f
+    assert torch.equal(one.state.q.data, two.state.q.data)
+    seg = torch.repeat_interleave(torch.arange(len(ws), device=device),
+                                  torch.as_tensor(np.diff(b.offsets), device=device))
+
+    def momenta(st):
+        p = torch.zeros((len(ws), 2), dtype=torch.float64, device=device)
+        p.index_add_(0, seg, (st.mass[:, None] * st.vel).double())
+        a = torch.zeros(len(ws), dtype=torch.float64, device=device)
+        a.index_add_(0, seg, (st.mass[:, None] * st.vel).double().abs().sum(1))
+        return p, a
+    p0, a0 = momenta(b.state)
+    p1, _ = momenta(one.state)
+    assert float(((p1 - p0).abs().max(1).values / a0).max()) < 1e-5
+    for k in (0, 1234, 4095):
+        sa, _ = jd.System.step(*W.make_gpu_system(ws[k], device=device), n=30)
+        a, z = int(b.offsets[k]), int(b.offsets[k + 1])
+        for f in ("pos_c", "vel", "ang_vel"):
+            assert_bitwise(f"system {k} {f}", getattr(sa, f).cpu().numpy(), getattr(one.state, f)[a:z].cpu().numpy())
+
+
 # ---------------------------------------------------------------------------
 # (f) serving: continuous batching of rollout requests
 # ---------------------------------------------------------------------------
@@ -718,6 +857,56 @@ def test_serving_results_equal_alone(device, budget, policy):
         assert yb.step_count == r.steps
     st = SV.latency_stats(done)
     assert st["p50_latency_s"] <= st["p99_latency_s"] and st["particle_steps"] == sum(N * s for N, s, *_ in spec)
+
+
+def test_serving_workload_requests_against_oracle(device):
+    """Requests of the serving workload (Poisson arrivals; 3D polydisperse
+    spheres, log-normal radii, periodic or reflect, volume fraction
+    U[0.3, 0.55], spring + friction mu U[0, 0.5]) capped at N <= 4000 and
+    <= 80 steps, served together by continuous batching: each result equals
+    the oracle's run of that request bit for bit (state, list, springs)."""
+    from jaxdem_amd import serving as SV
+
+    print("[c5] serving workload vs oracle", flush=True)
+    spec = W.serving_requests(4, rate=50.0, seed=3, n_min=1000, n_max=4000, steps_min=20, steps_max=80)
+    assert {w.domain for _, _, w in spec} == {"periodic", "reflect"}
+    reqs = []
+    for k, (t, steps, w) in enumerate(spec):
+        st, sy = W.make_gpu_system(w, device=device)
+        reqs.append(SV.Request(k, st, sy, steps, arrival=t))
+    server = SV.Server(device, hbm_budget=1 << 36, chunk=16)
+    done = server.run(reqs, realtime=False)
+    assert max(n for _, n, _, _ in server.log) >= 2  # served together
+    for r in sorted(done, key=lambda r: r.rid):
+        q = reqs[r.rid]
+        print(f"[c5] request {r.rid} N={q.state.N} steps={q.steps}", flush=True)
+        ost, nls = _oracle_system(q.state, q.system, q.steps)
+        sb, yb = r.result
+        _compare_states(sb, ost, f"request {r.rid}")
+        assert_bitwise("nl", yb.collider.neighbor_list.cpu().numpy().astype(np.int64), nls.nl)
+        xi, live = B.history_of(yb.collider)
+        assert_bitwise("live", live, nls.live)
+        assert_bitwise("xi", xi, nls.xi)
+
+
+def test_serving_full_size_request_in_a_mix(device):
+    """A full-size request of the serving workload (2e5 spheres) served
+    beside two small ones arriving while it runs: its result equals the same
+    request run alone with System.step, bit for bit."""
+    from jaxdem_amd import serving as SV
+
+    print("[c5] full-size request in a mix", flush=True)
+    big = W.serving_requests(1, rate=1.0, seed=11, n_min=200000, n_max=200000, steps_min=150, steps_max=150)[0][2]
+    small = W.serving_requests(2, rate=1.0, seed=12, n_min=1000, n_max=3000, steps_min=60, steps_max=100)
+    reqs = [SV.Request(0, *W.make_gpu_system(big, device=device), 150, arrival=0.0)]
+    for k, (_, steps, w) in enumerate(small):
+        reqs.append(SV.Request(k + 1, *W.make_gpu_system(w, device=device), steps, arrival=0.0))
+    server = SV.Server(device, hbm_budget=1 << 38, chunk=16)
+    done = {r.rid: r for r in server.run(reqs, realtime=False)}
+    assert max(n for _, n, _, _ in server.log) == 3
+    for k, q in enumerate(reqs):
+        sa, _ = jd.System.step(q.state, q.system, n=q.steps)
+        _kinematics_equal(sa, done[k].result[0], f"request {k} (N={q.state.N})")
 
 
 # ---------------------------------------------------------------------------

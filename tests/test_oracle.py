@@ -356,6 +356,34 @@ def test_drop_far_candidates_keeps_sums():
     assert np.array_equal(F_full, F_c)
 
 
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("domain", ["periodic", "reflect"])
+@pytest.mark.parametrize("box", [1.7, 3.1, 12.0])
+def test_naive_rows_through_cells_equal_all_pairs(dim, domain, box):
+    """naive_near_candidates (the all-pairs rows found through a cell list)
+    equals drop_far_candidates over all pairs, row for row -- with boxes of
+    one, two and many cells per axis, touching and far pairs, polydisperse
+    radii, positions outside a reflecting box and unwrapped periodic ones."""
+    rng = np.random.default_rng(int(10 * box) + dim)
+    N = 700
+    pos = rng.uniform(0, box, (N, dim)).astype(F32)
+    pos[:30] = pos[30:60] + rng.uniform(-0.6, 0.6, (30, dim)).astype(F32)  # touching pairs
+    if domain == "periodic":
+        pos[60:90] += (box * rng.integers(-2, 3, (30, dim))).astype(F32)
+    else:
+        pos[60:90] += rng.uniform(-1.5, 1.5, (30, dim)).astype(F32)
+    st = O.make_state(pos, rng.uniform(0.1, 0.5, N).astype(F32), dtype=F32)
+    P = params("spring", dtype=F32, domain=domain, box=box)
+    P.domain = O.Domain(domain, np.full(dim, box, F32), np.zeros(dim, F32))
+    P.dim = dim
+    want = O.drop_far_candidates(st, P, O.naive_candidates(N))
+    got = O.naive_near_candidates(st, P)
+    K = max(want.shape[1], got.shape[1])
+    pad = lambda a: np.pad(a, ((0, 0), (0, K - a.shape[1])), constant_values=-1)  # noqa: E731
+    assert np.array_equal(pad(got), pad(want))
+    assert (want >= 0).sum() > N  # more than the self pairs
+
+
 def test_excluded_pairs_known_answer():
     """test_excluded_pairs.py:11-61 with the oracle (float64)."""
     st = O.make_state([[0.0, 0.0], [1.0, 0.0], [2.0, 0.0]], [1.1, 1.1, 1.1], bond_id=[[1, -1], [0, 2], [1, -1]],
