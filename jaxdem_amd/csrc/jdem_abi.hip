@@ -629,103 +629,111 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   // Batches: only the systems that asked for a rebuild are re-sorted and
   // rebuilt, each within its own range of work slots; the work order, rows,
   // springs and build positions of the other systems stay where they are.
+  // Batches decide their rebuilds on the device.  After every step kernel a
+  // chain of kernels runs, each returning at once unless the gate word is
+  // set (by the step kernel when a particle of some system moved more than
+  // half its skin, or at the call start by a system without a list); the
+  // chain rebuilds exactly the flagged systems, each inside its own range
+  // of work slots, sized from device counts (flagged particles, compact
+  // cells), and clears the gate.  The work order, rows, springs and build
+  // positions of the other systems stay where they are.  No host round trip.
+  int32_t* gate = (int32_t*)ctx->flags.p + 4;
+  int32_t* events = (int32_t*)ctx->flags.p + 5;
   auto rebuild_batch = [&]() -> int {
     const int sp = span_begin(ctx, 2, 0);
-    HIPCHK(ctx, launch_fill_i32((int32_t*)ctx->flags.p + 2, 1, 0, st));
-    HIPCHK(ctx, ws_launch_seg_modes(sg->n_seg, sg->seg_flag, sg->n_build, sg->seg_mode, st));
+    HIPCHK(ctx, ws_launch_gated_zero(gate, (int32_t*)ctx->flags.p + 2, st));  // overflow of this build
+    HIPCHK(ctx, ws_launch_seg_modes(sg->n_seg, sg->seg_flag, sg->n_build, sg->seg_mode, st, gate, events));
     HIPCHK(ctx, ws_launch_seg_prefix(sg->n_seg, sg->seg_mode, sg->offsets, sg->segp, sg->cpref, sg->ccref, st));
     // springs of the wavefronts that hold a rebuilt system: per wave -> by rank
     if (hist && M > 0)
       HIPCHK(ctx, ws_launch_unpack_springs(N, M, W64, F4(ctx->wXi[xc]), U64(ctx->wLive[nlc]), F4(ctx->wW[cur]),
-                                           F4(ctx->wXiRank), sg->seg, sg->seg_mode, st));
-    int64_t tot[2];
-    HIPCHK(ctx, hipMemcpyAsync(&tot[0], sg->cpref + sg->n_seg, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(ctx, hipMemcpyAsync(&tot[1], sg->ccref + sg->n_seg, 8, hipMemcpyDeviceToHost, st));
-    HIPCHK(ctx, hipStreamSynchronize(st));
-    S->host_syncs++;
-    const int64_t fn = tot[0];
-    if (fn > 0) {
-      RB r;
-      std::memset(&r, 0, sizeof(r));
-      r.n = N;
-      r.fn = fn;
-      r.M = M;
-      r.W64 = W64;
-      r.seg = sg->seg;
-      r.segp = sg->segp;
-      r.seg_mode = sg->seg_mode;
-      r.offsets = sg->offsets;
-      r.cpref = sg->cpref;
-      r.ccref = sg->ccref;
-      r.P = F4(ctx->wP[cur]);
-      r.V = F4(ctx->wV[cur]);
-      r.W = F4(ctx->wW[cur]);
-      r.Q = F4(ctx->wQ[set]);
-      r.I = F4(ctx->wI[set]);
-      r.old4 = F4(ctx->wOld);
-      r.orig = I32(ctx->wOrig[set]);
-      r.nl = I32(ctx->wNl[nlc]);
-      r.live = hist ? U64(ctx->wLive[nlc]) : nullptr;
-      r.xi = F4(ctx->wXiRank);
-      r.keys_c = (uint32_t*)ctx->wKeys.p;
-      r.slot_c = I32(ctx->cend);
-      // staging: buffers that hold nothing live during a batch rebuild
-      r.tP = F4(ctx->wP[1 - cur]);
-      r.tV = F4(ctx->wV[1 - cur]);
-      r.tW = F4(ctx->wW[1 - cur]);
-      r.tQ = F4(ctx->wQ[1 - set]);
-      r.tI = F4(ctx->wI[1 - set]);
-      r.torig = I32(ctx->wOrig[1 - set]);
-      r.cs = I32(ctx->wWpos);
-      r.tnl = I32(ctx->wNl[1 - nlc]);
-      r.tlive = U64(ctx->wLive[1 - nlc]);
-      r.txi = F4(ctx->wXi[1 - xc]);
-      r.inv = I32(ctx->wInv);
-      r.cidx = I32(ctx->wSrc);
-      HIPCHK(ctx, ws_launch_hash_flagged(r, D, st));
-      uint32_t* sk = nullptr;
-      int32_t* perm = nullptr;
-      HIPCHK(ctx, radix_sort_pairs(ctx, r.keys_c, fn, nbits_for(std::max<int64_t>(tot[1], 1)), &sk, &perm, st));
-      r.perm = perm;
-      HIPCHK(ctx, ws_launch_place(r, sk, I32(ctx->cstart), st));
-      if (M > 0) {
-        NLW b;
-        std::memset(&b, 0, sizeof(b));
-        b.n = N;
-        b.M = M;
-        b.isb = sys->interact_same_bond_id;
-        b.bond_w = s.bond_width;
-        b.bond_id = s.bond_id;
-        b.orig = I32(ctx->wOrig[set]);
-        b.P4 = F4(ctx->wP[cur]);
-        b.skeys = sk;  // compact sorted keys: slot b's key is sk[cidx[b]]
-        b.ccref = sg->ccref;
-        b.cidx = I32(ctx->wSrc);
-        b.starts = I32(ctx->cstart);
-        b.grid = g;
-        b.dom = dom;
-        b.seg = sg->seg;
-        b.segp = sg->segp;
-        b.seg_mode = sg->seg_mode;
-        b.nl = I32(ctx->wNl[nlc]);
-        b.overflow = (int32_t*)ctx->flags.p + 2;
-        if (hist) {
-          b.W64 = W64;
-          b.inv = I32(ctx->wInv);
-          b.ostride = fn;
-          b.old_nl = r.tnl;
-          b.old_xi = r.txi;
-          b.old_live = r.tlive;
-          b.xi = F4(ctx->wXiRank);
-          b.live = U64(ctx->wLive[nlc]);
-        }
-        HIPCHK(ctx, ws_launch_nlist(b, D, false, st));
+                                           F4(ctx->wXiRank), sg->seg, sg->seg_mode, st, gate));
+    RB r;
+    std::memset(&r, 0, sizeof(r));
+    r.n = N;
+    r.fn = -1;  // the flagged-particle count is cpref[n_seg] on the device
+    r.n_seg = sg->n_seg;
+    r.gate = gate;
+    r.M = M;
+    r.W64 = W64;
+    r.seg = sg->seg;
+    r.segp = sg->segp;
+    r.seg_mode = sg->seg_mode;
+    r.offsets = sg->offsets;
+    r.cpref = sg->cpref;
+    r.ccref = sg->ccref;
+    r.P = F4(ctx->wP[cur]);
+    r.V = F4(ctx->wV[cur]);
+    r.W = F4(ctx->wW[cur]);
+    r.Q = F4(ctx->wQ[set]);
+    r.I = F4(ctx->wI[set]);
+    r.old4 = F4(ctx->wOld);
+    r.orig = I32(ctx->wOrig[set]);
+    r.nl = I32(ctx->wNl[nlc]);
+    r.live = hist ? U64(ctx->wLive[nlc]) : nullptr;
+    r.xi = F4(ctx->wXiRank);
+    r.keys_c = (uint32_t*)ctx->wKeys.p;
+    r.slot_c = I32(ctx->cend);
+    // staging: buffers that hold nothing live during a batch rebuild
+    r.tP = F4(ctx->wP[1 - cur]);
+    r.tV = F4(ctx->wV[1 - cur]);
+    r.tW = F4(ctx->wW[1 - cur]);
+    r.tQ = F4(ctx->wQ[1 - set]);
+    r.tI = F4(ctx->wI[1 - set]);
+    r.torig = I32(ctx->wOrig[1 - set]);
+    r.cs = I32(ctx->wWpos);
+    r.tnl = I32(ctx->wNl[1 - nlc]);
+    r.tlive = U64(ctx->wLive[1 - nlc]);
+    r.txi = F4(ctx->wXi[1 - xc]);
+    r.inv = I32(ctx->wInv);
+    r.cidx = I32(ctx->wSrc);
+    HIPCHK(ctx, ws_launch_hash_flagged(r, D, st));
+    uint32_t* sk = nullptr;
+    int32_t* perm = nullptr;
+    // sized for every particle and every cell of the batch; the kernels read
+    // the flagged count on the device
+    HIPCHK(ctx, radix_sort_pairs(ctx, r.keys_c, N, nbits_for(std::max<int64_t>(sg->total_cells, 1)), &sk, &perm, st,
+                                 sg->cpref + sg->n_seg, gate));
+    r.perm = perm;
+    HIPCHK(ctx, ws_launch_place(r, sk, I32(ctx->cstart), st));
+    if (M > 0) {
+      NLW b;
+      std::memset(&b, 0, sizeof(b));
+      b.n = N;
+      b.M = M;
+      b.isb = sys->interact_same_bond_id;
+      b.bond_w = s.bond_width;
+      b.bond_id = s.bond_id;
+      b.orig = I32(ctx->wOrig[set]);
+      b.P4 = F4(ctx->wP[cur]);
+      b.skeys = sk;  // compact sorted keys: slot b's key is sk[cidx[b]]
+      b.ccref = sg->ccref;
+      b.cidx = I32(ctx->wSrc);
+      b.starts = I32(ctx->cstart);
+      b.grid = g;
+      b.dom = dom;
+      b.seg = sg->seg;
+      b.segp = sg->segp;
+      b.seg_mode = sg->seg_mode;
+      b.nl = I32(ctx->wNl[nlc]);
+      b.overflow = (int32_t*)ctx->flags.p + 2;
+      b.gate = gate;
+      if (hist) {
+        b.W64 = W64;
+        b.inv = I32(ctx->wInv);
+        b.ostride = N;
+        b.old_nl = r.tnl;
+        b.old_xi = r.txi;
+        b.old_live = r.tlive;
+        b.xi = F4(ctx->wXiRank);
+        b.live = U64(ctx->wLive[nlc]);
       }
+      HIPCHK(ctx, ws_launch_nlist(b, D, false, st));
     }
     if (hist && M > 0)
       HIPCHK(ctx, ws_launch_pack_springs(N, M, W64, F4(ctx->wXiRank), U64(ctx->wLive[nlc]), F4(ctx->wXi[xc]),
-                                         F4(ctx->wW[cur]), sg->seg, sg->seg_mode, st));
-    L.n_build_times += 1;
+                                         F4(ctx->wW[cur]), sg->seg, sg->seg_mode, st, gate));
+    HIPCHK(ctx, ws_launch_gate_end(gate, st));
     span_end(ctx, sp);
     return JDEM_OK;
   };
@@ -869,6 +877,9 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     a.flag_wr = flags + ((t + 1) & 1);
     a.step_idx = (int32_t)t;
     if (sg) {
+      // no speculation: the rebuild chain after this kernel acts on the gate
+      a.flag_rd = nullptr;
+      a.flag_wr = gate;
       a.seg = sg->seg;
       a.segp = sg->segp;
       a.seg_flag = sg->seg_flag;
@@ -887,13 +898,35 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   nseg = ns;
   cur_at.assign((size_t)ns + 2, 0);
   xc_at.assign((size_t)ns + 2, 0);
+  if (sg) {
+    // step kernel, rebuild chain, step kernel, ...: one host read at the end
+    HIPCHK(ctx, launch_fill_i32(flags, 6, 0, st));
+    HIPCHK(ctx, ws_launch_seg_init_flags(sg->n_seg, sg->n_build, sg->seg_flag, st, gate));
+    for (int64_t t = 0; t <= ns; ++t) {
+      int rc2 = launch_t(t);
+      if (rc2) return rc2;
+      if (t < ns) {
+        rc2 = rebuild();
+        if (rc2) return rc2;
+      }
+    }
+    const int rc2 = read_flags(ctx, 6);
+    if (rc2) return rc2;
+    S->host_syncs++;
+    const int32_t ev = ctx->host_flags[5];
+    S->rebuilds += ev;
+    L.n_build_times += ev;
+    if (ev > 0) rebuilt_here = true;
+    return JDEM_OK;
+  }
   HIPCHK(ctx, launch_fill_i32(flags, 4, 0, st));
   int rc = launch_t(0);
   if (rc) return rc;
-  bool pending = sg ? sg->any_unbuilt : L.n_build_times == 0;
-  if (sg) HIPCHK(ctx, ws_launch_seg_init_flags(sg->n_seg, sg->n_build, sg->seg_flag, st));
+  bool pending = L.n_build_times == 0;
   int64_t next = 1;
-  int64_t chunk_len = 8;
+  // speculative chunk: a chunk that runs into a rebuild wastes its remaining
+  // kernels (each returns at once), one that does not saves a host read
+  int64_t chunk_len = 32;
   int64_t last_rebuild_at = 0;
   while (next <= ns) {
     if (pending) {
@@ -1006,6 +1039,16 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     std::swap(res->live, ctx->wLive[nlc]);
     if (hist && M > 0) std::swap(res->xi, ctx->wXiRank);
     *L.resident_out = res;
+    if (!R) {
+      // The first working set of a system reserves its successor's buffers
+      // now: the next call starts from this one while it is still alive, and
+      // its own working set is handed back to the pool only after that call.
+      ENSURE(ctx, ctx->wOrig[set], ni);
+      ENSURE(ctx, ctx->wOld, n4);
+      ENSURE(ctx, ctx->wNl[nlc], 4 * (size_t)N * (size_t)std::max(M, 1));
+      ENSURE(ctx, ctx->wLive[nlc], 8 * (size_t)N * W64);
+      if (hist && M > 0) ENSURE(ctx, ctx->wXiRank, 16 * (size_t)N * (size_t)std::max(M, 1));
+    }
     if (free_done) {
       for (int a = 0; a < D; ++a) {
         sys->domain.box_size[a] = fbox[a];

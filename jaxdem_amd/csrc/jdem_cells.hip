@@ -72,8 +72,14 @@ constexpr int RS_ITEMS = 16;
 constexpr int RS_TILE = kBlock * RS_ITEMS;
 constexpr int RS_WAVES = kBlock / 64;
 
+// n_dev (when set) holds the key count on the device, n is then an upper
+// bound that sized the launch; gate (when set) skips the sort while 0.
 __global__ void __launch_bounds__(kBlock) k_radix_hist(const uint32_t* __restrict__ keys, int64_t n, int shift,
-                                                        uint32_t* __restrict__ hist, int ntiles) {
+                                                        uint32_t* __restrict__ hist, int ntiles,
+                                                        const int64_t* __restrict__ n_dev,
+                                                        const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  if (n_dev) n = *n_dev;
   __shared__ uint32_t bins[256];
   const int tid = threadIdx.x;
   bins[tid] = 0;
@@ -217,7 +223,10 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t* __rest
                                                            const int32_t* __restrict__ vin,
                                                            uint32_t* __restrict__ kout, int32_t* __restrict__ vout,
                                                            int64_t n, int shift, const uint32_t* __restrict__ offs,
-                                                           int ntiles) {
+                                                           int ntiles, const int64_t* __restrict__ n_dev,
+                                                           const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  if (n_dev) n = *n_dev;
   __shared__ uint32_t run[256];
   __shared__ uint32_t wcnt[RS_WAVES][256];
   const int tid = threadIdx.x;
@@ -270,7 +279,7 @@ size_t radix_hist_words(int64_t n) {
 }
 
 hipError_t radix_sort_bufs(const SortBufs& sb, const uint32_t* keys_in, int64_t n, int nbits, uint32_t** keys_out,
-                           int32_t** vals_out, hipStream_t s) {
+                           int32_t** vals_out, hipStream_t s, const int64_t* n_dev, const int32_t* gate) {
   const int ntiles = (int)((n + RS_TILE - 1) / RS_TILE);
   const int64_t total = (int64_t)ntiles * 256;
   const int nchunks = (int)((total + SC_CHUNK - 1) / SC_CHUNK);
@@ -285,17 +294,17 @@ hipError_t radix_sort_bufs(const SortBufs& sb, const uint32_t* keys_in, int64_t 
   for (int p = 0; p < passes; ++p) {
     const int shift = 8 * p;
     if (n > 0) {
-      hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, s, kcur, n, shift, hist, ntiles);
+      hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, s, kcur, n, shift, hist, ntiles, n_dev, gate);
       if (nchunks <= 1) {
-        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, hist, total);
+        hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, hist, total, gate);
       } else {
         uint32_t* sums = hist + total;
-        hipLaunchKernelGGL(k_scan_totals, dim3(nchunks), dim3(kBlock), 0, s, hist, total, sums);
-        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, nchunks);
-        hipLaunchKernelGGL(k_scan_apply, dim3(nchunks), dim3(kBlock), 0, s, hist, total, sums);
+        hipLaunchKernelGGL(k_scan_totals, dim3(nchunks), dim3(kBlock), 0, s, hist, total, sums, gate);
+        hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, nchunks, gate);
+        hipLaunchKernelGGL(k_scan_apply, dim3(nchunks), dim3(kBlock), 0, s, hist, total, sums, gate);
       }
       hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, s, kcur, vcur, kb[out], vb[out], n, shift,
-                         hist, ntiles);
+                         hist, ntiles, n_dev, gate);
     }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
@@ -309,7 +318,7 @@ hipError_t radix_sort_bufs(const SortBufs& sb, const uint32_t* keys_in, int64_t 
 }
 
 hipError_t radix_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, int nbits, uint32_t** keys_out,
-                            int32_t** vals_out, hipStream_t s) {
+                            int32_t** vals_out, hipStream_t s, const int64_t* n_dev, const int32_t* gate) {
   const size_t nn = (size_t)(n > 0 ? n : 1);
   if (!ensure(ctx, ctx->keys[0], sizeof(uint32_t) * nn) || !ensure(ctx, ctx->keys[1], sizeof(uint32_t) * nn) ||
       !ensure(ctx, ctx->vals[0], sizeof(int32_t) * nn) || !ensure(ctx, ctx->vals[1], sizeof(int32_t) * nn) ||
@@ -322,7 +331,7 @@ hipError_t radix_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, i
   sb.v1 = (int32_t*)ctx->vals[1].p;
   sb.hist = (uint32_t*)ctx->hist.p;
   sb.cur = nullptr;
-  return radix_sort_bufs(sb, keys_in, n, nbits, keys_out, vals_out, s);
+  return radix_sort_bufs(sb, keys_in, n, nbits, keys_out, vals_out, s, n_dev, gate);
 }
 
 // ---------------------------------------------------------------------------

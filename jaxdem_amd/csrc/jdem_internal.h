@@ -184,13 +184,17 @@ size_t radix_hist_words(int64_t n);
 // uint32 words of SortBufs::hist that a counting sort over n_cells uses
 size_t cell_sort_sum_words(int64_t n_cells);
 // The sorts below on caller-given buffers (no allocation).
+// With n_dev the key count is read on the device (n bounds it and sizes the
+// launches); with gate every kernel returns at once while *gate == 0.
 hipError_t radix_sort_bufs(const SortBufs& sb, const uint32_t* keys_in, int64_t n, int nbits, uint32_t** keys_out,
-                           int32_t** vals_out, hipStream_t s);
+                           int32_t** vals_out, hipStream_t s, const int64_t* n_dev = nullptr,
+                           const int32_t* gate = nullptr);
 hipError_t cell_sort_bufs(const SortBufs& sb, const uint32_t* keys_in, int64_t n, int64_t n_cells, uint32_t** keys_out,
                           int32_t** vals_out, int32_t* starts, hipStream_t s, const int32_t* gate = nullptr);
 // Sort (keys, iota) by key; results in (keys_out, vals_out).  nbits = key bits.
 hipError_t radix_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, int nbits, uint32_t** keys_out,
-                            int32_t** vals_out, hipStream_t s);
+                            int32_t** vals_out, hipStream_t s, const int64_t* n_dev = nullptr,
+                            const int32_t* gate = nullptr);
 // The same result as radix_sort_pairs for keys < n_cells, by counting:
 // per-cell counts, their exclusive scan (written to starts[0..n_cells],
 // starts[n_cells] = n), placement through per-cell cursors, then each
@@ -347,12 +351,15 @@ struct NLW {
   const int32_t* cidx;
   int64_t ostride;
   const int64_t* ccref;  // compact key of system s's first cell
+  const int32_t* gate;   // when set, the kernel returns at once while *gate == 0
 };
 enum { SEG_KEEP = 0, SEG_FRESH = 1, SEG_REMAP = 2 };
 
 // In-place rebuild of the flagged systems of a batch (jdem_resident.hip).
 struct RB {
-  int64_t n, fn;  // particles; flagged particles
+  int64_t n, fn;  // particles; flagged particles (-1: read on the device, cpref[n_seg])
+  int n_seg;
+  const int32_t* gate;  // when set, every kernel returns at once while *gate == 0
   int M, W64;
   const int32_t* seg;
   const SegP* segp;
@@ -453,10 +460,11 @@ hipError_t ws_launch_copy_rows(const int32_t* src, int32_t* dst, int64_t n, int 
 // unpack: per wave -> by rank.  With seg_mode, only wavefronts holding a
 // slot of a system that is rebuilt (mode != SEG_KEEP) are converted.
 hipError_t ws_launch_pack_springs(int64_t n, int M, int W64, const float4* rank, const uint64_t* live, float4* wave,
-                                  float4* W, const int32_t* seg, const int32_t* seg_mode, hipStream_t s);
+                                  float4* W, const int32_t* seg, const int32_t* seg_mode, hipStream_t s,
+                                  const int32_t* gate = nullptr);
 hipError_t ws_launch_unpack_springs(int64_t n, int M, int W64, const float4* wave, const uint64_t* live,
                                     const float4* W, float4* rank, const int32_t* seg, const int32_t* seg_mode,
-                                    hipStream_t s);
+                                    hipStream_t s, const int32_t* gate = nullptr);
 hipError_t ws_launch_inverse(const int32_t* orig, int64_t n, int32_t* wpos, hipStream_t s);
 hipError_t ws_launch_iota(int32_t* p, int64_t n, hipStream_t s);
 hipError_t ws_launch_copy4(float4* d, const float4* src, int64_t n, hipStream_t s);
@@ -470,8 +478,14 @@ hipError_t ws_launch_permute(int64_t n, const int32_t* new_orig, const int32_t* 
                              int32_t* inv, hipStream_t s, float4* old2 = nullptr);
 // batch rebuilds: per-system mode from the flags (and bump the build counts),
 // and the flags of never-built systems
-hipError_t ws_launch_seg_modes(int n_seg, int32_t* seg_flag, int32_t* n_build, int32_t* seg_mode, hipStream_t s);
-hipError_t ws_launch_seg_init_flags(int n_seg, const int32_t* n_build, int32_t* seg_flag, hipStream_t s);
+// With gate set, *events counts the calls that find *gate != 0 (rebuilds).
+hipError_t ws_launch_seg_modes(int n_seg, int32_t* seg_flag, int32_t* n_build, int32_t* seg_mode, hipStream_t s,
+                               const int32_t* gate = nullptr, int32_t* events = nullptr);
+hipError_t ws_launch_seg_init_flags(int n_seg, const int32_t* n_build, int32_t* seg_flag, hipStream_t s,
+                                   int32_t* gate = nullptr);
+// *gate = 0 (the end of a gated rebuild); *word = 0 when *gate is set
+hipError_t ws_launch_gate_end(int32_t* gate, hipStream_t s);
+hipError_t ws_launch_gated_zero(const int32_t* gate, int32_t* word, hipStream_t s);
 
 hipError_t ws_launch_cell_starts(const uint32_t* skeys, int64_t n, int64_t n_cells, int32_t* starts, hipStream_t s);
 hipError_t ws_launch_nlist(const NLW& a, int dim, bool naive, hipStream_t s);

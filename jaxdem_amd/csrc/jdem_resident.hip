@@ -287,17 +287,25 @@ __global__ void __launch_bounds__(kBlock) k_permute(int64_t n, const int32_t* __
 // rebuild, FRESH for its first build, REMAP otherwise; its build count grows
 // and its flag is cleared.
 __global__ void k_seg_modes(int n_seg, int32_t* __restrict__ seg_flag, int32_t* __restrict__ n_build,
-                            int32_t* __restrict__ seg_mode) {
+                            int32_t* __restrict__ seg_mode, const int32_t* __restrict__ gate,
+                            int32_t* __restrict__ events) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s == 0 && events && gate && *gate) *events += 1;
   if (s >= n_seg) return;
   const int f = seg_flag[s] != 0;
   seg_mode[s] = f ? (n_build[s] > 0 ? SEG_REMAP : SEG_FRESH) : SEG_KEEP;
   n_build[s] += f;
   seg_flag[s] = 0;
 }
-__global__ void k_seg_init_flags(int n_seg, const int32_t* __restrict__ n_build, int32_t* __restrict__ seg_flag) {
+// Call start, before the first step kernel: a system without a list asks
+// for a build (and raises *gate when set), every other flag starts clear.
+__global__ void k_seg_init_flags(int n_seg, const int32_t* __restrict__ n_build, int32_t* __restrict__ seg_flag,
+                                 int32_t* __restrict__ gate) {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < n_seg) seg_flag[s] = n_build[s] == 0 ? 1 : 0;
+  if (s >= n_seg) return;
+  const bool unbuilt = n_build[s] == 0;
+  seg_flag[s] = unbuilt ? 1 : 0;
+  if (unbuilt && gate) *gate = 1;
 }
 
 // ---- batch rebuilds in place -----------------------------------------------
@@ -351,6 +359,7 @@ __global__ void __launch_bounds__(1024) k_seg_prefix(int n_seg, const int32_t* _
 // compact key of every particle of a flagged system, at its compact index
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_hash_flagged(RB r) {
+  if (r.gate && *r.gate == 0) return;
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= r.n) return;
   const int s = r.seg[b];
@@ -368,9 +377,15 @@ __global__ void __launch_bounds__(kBlock) k_hash_flagged(RB r) {
 }
 
 // sorted position k -> staging (k) of the particle's fields and old list
+// Flagged particles of the rebuild: r.fn, or (r.fn < 0) the device count.
+JD int64_t rb_fn(const RB& r) { return r.fn >= 0 ? r.fn : r.cpref[r.n_seg]; }
+
+// The staging arrays hold the compact index k at stride n (rows, live words
+// and springs of k at c * n + k).
 __global__ void __launch_bounds__(kBlock) k_place_gather(RB r) {
+  if (r.gate && *r.gate == 0) return;
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= r.fn) return;
+  if (k >= rb_fn(r)) return;
   const int32_t ob = r.slot_c[r.perm[k]];
   const int s = r.seg[ob];
   const int64_t nb = r.offsets[s] + (k - r.cpref[s]);
@@ -386,25 +401,26 @@ __global__ void __launch_bounds__(kBlock) k_place_gather(RB r) {
     int c = 0;
     for (; c < r.M; ++c) {
       const int32_t j = r.nl[(int64_t)c * r.n + ob];
-      r.tnl[(int64_t)c * r.fn + k] = j;
+      r.tnl[(int64_t)c * r.n + k] = j;
       if (j < 0) break;
     }
     if (r.live) {
       int nlive = 0;
       for (int w = 0; w < r.W64; ++w) {
         const uint64_t bits = r.live[(int64_t)w * r.n + ob];
-        r.tlive[(int64_t)w * r.fn + k] = bits;
+        r.tlive[(int64_t)w * r.n + k] = bits;
         nlive += __popcll(bits);
       }
-      for (int q = 0; q < nlive; ++q) r.txi[(int64_t)q * r.fn + k] = r.xi[(int64_t)q * r.n + ob];
+      for (int q = 0; q < nlive; ++q) r.txi[(int64_t)q * r.n + k] = r.xi[(int64_t)q * r.n + ob];
     }
   }
 }
 
 // staging (k) -> the particle's new work slot in its system's range
 __global__ void __launch_bounds__(kBlock) k_place_scatter(RB r) {
+  if (r.gate && *r.gate == 0) return;
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= r.fn) return;
+  if (k >= rb_fn(r)) return;
   const int s = r.cs[k];
   const int64_t nb = r.offsets[s] + (k - r.cpref[s]);
   const float4 p = r.tP[k];
@@ -423,8 +439,9 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RB r) {
 // of a system also fills to the end of the system's grid)
 __global__ void __launch_bounds__(kBlock) k_cell_starts_flagged(RB r, const uint32_t* __restrict__ sk,
                                                                  int32_t* __restrict__ starts) {
+  if (r.gate && *r.gate == 0) return;
   const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= r.fn) return;
+  if (k >= rb_fn(r)) return;
   const int s = r.cs[k];
   const SegP& sp = r.segp[s];
   const int64_t first = r.cpref[s], last = r.cpref[s + 1] - 1;
@@ -600,6 +617,7 @@ constexpr int kRowLds = 16;
 // batch, where each work slot takes the grid, domain and cutoff of its system.
 template <int D, int PM>
 __global__ void __launch_bounds__(kBlock) k_nlist_work(NLW a) {
+  if (a.gate && *a.gate == 0) return;  // uniform over the grid
   __shared__ int32_t rows[kRowLds][kBlock];
   __shared__ int row_max;
   const int tid = threadIdx.x;
@@ -741,7 +759,9 @@ JD bool wave_touched(int64_t b, int64_t n, const int32_t* seg, const int32_t* se
 __global__ void __launch_bounds__(kBlock) k_pack_springs(int64_t n, int M, int W64, const float4* __restrict__ rank,
                                                           const uint64_t* __restrict__ live, float4* __restrict__ wave,
                                                           float4* __restrict__ W, const int32_t* __restrict__ seg,
-                                                          const int32_t* __restrict__ seg_mode) {
+                                                          const int32_t* __restrict__ seg_mode,
+                                                          const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
   if (!wave_touched(b, n, seg, seg_mode)) return;  // wave-uniform
@@ -759,7 +779,9 @@ __global__ void __launch_bounds__(kBlock) k_unpack_springs(int64_t n, int M, int
                                                             const uint64_t* __restrict__ live,
                                                             const float4* __restrict__ W, float4* __restrict__ rank,
                                                             const int32_t* __restrict__ seg,
-                                                            const int32_t* __restrict__ seg_mode) {
+                                                            const int32_t* __restrict__ seg_mode,
+                                                            const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (!wave_touched(b, n, seg, seg_mode) || b >= n) return;
   int cnt = 0;
@@ -1151,18 +1173,19 @@ hipError_t launch_ws_dim(const WS& a, int law, bool per, hipStream_t s) {
 }
 
 hipError_t ws_launch_pack_springs(int64_t n, int M, int W64, const float4* rank, const uint64_t* live, float4* wave,
-                                  float4* W, const int32_t* seg, const int32_t* seg_mode, hipStream_t s) {
+                                  float4* W, const int32_t* seg, const int32_t* seg_mode, hipStream_t s,
+                                  const int32_t* gate) {
   if (n == 0 || M == 0) return hipSuccess;
   hipLaunchKernelGGL(k_pack_springs, dim3(nblocks(n)), dim3(kBlock), 0, s, n, M, W64, rank, live, wave, W, seg,
-                     seg_mode);
+                     seg_mode, gate);
   return hipGetLastError();
 }
 hipError_t ws_launch_unpack_springs(int64_t n, int M, int W64, const float4* wave, const uint64_t* live,
                                     const float4* W, float4* rank, const int32_t* seg, const int32_t* seg_mode,
-                                    hipStream_t s) {
+                                    hipStream_t s, const int32_t* gate) {
   if (n == 0 || M == 0) return hipSuccess;
   hipLaunchKernelGGL(k_unpack_springs, dim3(nblocks(n)), dim3(kBlock), 0, s, n, M, W64, wave, live, W, rank, seg,
-                     seg_mode);
+                     seg_mode, gate);
   return hipGetLastError();
 }
 
@@ -1217,6 +1240,18 @@ hipError_t ws_launch_out(const InOut& io, bool with_nl, hipStream_t s) {
   dim3 g(nblocks(io.n)), b(kBlock);
   hipLaunchKernelGGL(k_out_state, g, b, 0, s, io);
   if (with_nl) hipLaunchKernelGGL(k_out_nlist, g, b, 0, s, io);
+  return hipGetLastError();
+}
+__global__ void k_gate_end(int32_t* __restrict__ gate) { *gate = 0; }
+__global__ void k_gated_zero(const int32_t* __restrict__ gate, int32_t* __restrict__ word) {
+  if (*gate) *word = 0;
+}
+hipError_t ws_launch_gate_end(int32_t* gate, hipStream_t s) {
+  hipLaunchKernelGGL(k_gate_end, dim3(1), dim3(1), 0, s, gate);
+  return hipGetLastError();
+}
+hipError_t ws_launch_gated_zero(const int32_t* gate, int32_t* word, hipStream_t s) {
+  hipLaunchKernelGGL(k_gated_zero, dim3(1), dim3(1), 0, s, gate, word);
   return hipGetLastError();
 }
 hipError_t ws_launch_export(const InOut& io, hipStream_t s) {
@@ -1288,18 +1323,21 @@ hipError_t ws_launch_hash_flagged(const RB& r, int dim, hipStream_t s) {
 }
 hipError_t ws_launch_place(const RB& r, const uint32_t* sk, int32_t* starts, hipStream_t s) {
   if (r.fn == 0) return hipSuccess;
-  const dim3 g(nblocks(r.fn)), b(kBlock);
+  const dim3 g(nblocks(r.fn >= 0 ? r.fn : r.n)), b(kBlock);  // device count: sized for every particle
   hipLaunchKernelGGL(k_place_gather, g, b, 0, s, r);
   hipLaunchKernelGGL(k_place_scatter, g, b, 0, s, r);
   hipLaunchKernelGGL(k_cell_starts_flagged, g, b, 0, s, r, sk, starts);
   return hipGetLastError();
 }
-hipError_t ws_launch_seg_modes(int n_seg, int32_t* seg_flag, int32_t* n_build, int32_t* seg_mode, hipStream_t s) {
-  hipLaunchKernelGGL(k_seg_modes, dim3(nblocks(n_seg)), dim3(kBlock), 0, s, n_seg, seg_flag, n_build, seg_mode);
+hipError_t ws_launch_seg_modes(int n_seg, int32_t* seg_flag, int32_t* n_build, int32_t* seg_mode, hipStream_t s,
+                               const int32_t* gate, int32_t* events) {
+  hipLaunchKernelGGL(k_seg_modes, dim3(nblocks(n_seg)), dim3(kBlock), 0, s, n_seg, seg_flag, n_build, seg_mode, gate,
+                     events);
   return hipGetLastError();
 }
-hipError_t ws_launch_seg_init_flags(int n_seg, const int32_t* n_build, int32_t* seg_flag, hipStream_t s) {
-  hipLaunchKernelGGL(k_seg_init_flags, dim3(nblocks(n_seg)), dim3(kBlock), 0, s, n_seg, n_build, seg_flag);
+hipError_t ws_launch_seg_init_flags(int n_seg, const int32_t* n_build, int32_t* seg_flag, hipStream_t s,
+                                   int32_t* gate) {
+  hipLaunchKernelGGL(k_seg_init_flags, dim3(nblocks(n_seg)), dim3(kBlock), 0, s, n_seg, n_build, seg_flag, gate);
   return hipGetLastError();
 }
 hipError_t ws_launch_cell_starts(const uint32_t* skeys, int64_t n, int64_t n_cells, int32_t* starts, hipStream_t s) {
