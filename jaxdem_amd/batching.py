@@ -211,6 +211,27 @@ class Batch:
                      self.bond_widths, list(self.n_build))
 
     def _c_arrays(self, keep: list) -> L.JdemBatch:
+        """The jdem_batch of the systems (their domains, grids, dt, cutoffs,
+        skins do not change as they step; built once per Batch)."""
+        cached = self.__dict__.get("_c_cache")
+        if cached is None:
+            cached = self._c_build()
+            self.__dict__["_c_cache"] = cached
+        off, doms, grids, dt, cut, skin = cached
+        nb = np.ascontiguousarray(self.n_build, np.int32)
+        keep += [off, doms, grids, dt, cut, skin, nb]
+        b = L.JdemBatch()
+        b.n_systems = len(self.systems)
+        b.n_build_times = nb.ctypes.data
+        b.offsets = off.ctypes.data
+        b.domains = ctypes.addressof(doms)
+        b.grids = ctypes.addressof(grids)
+        b.dt = dt.ctypes.data
+        b.cutoff = cut.ctypes.data
+        b.skin = skin.ctypes.data
+        return b
+
+    def _c_build(self) -> tuple:
         B = len(self.systems)
         off = np.ascontiguousarray(self.offsets, np.int64)
         doms = (L.JdemDomain * B)()
@@ -227,40 +248,40 @@ class Batch:
             dt[k] = F32(sy.dt)
             cut[k] = F32(sy.collider.cutoff)
             skin[k] = F32(sy.collider.skin)
-        nb = np.ascontiguousarray(self.n_build, np.int32)
-        keep += [off, doms, grids, dt, cut, skin, nb]
-        b = L.JdemBatch()
-        b.n_systems = B
-        b.n_build_times = nb.ctypes.data
-        b.offsets = off.ctypes.data
-        b.domains = ctypes.addressof(doms)
-        b.grids = ctypes.addressof(grids)
-        b.dt = dt.ctypes.data
-        b.cutoff = cut.ctypes.data
-        b.skin = skin.ctypes.data
-        return b
+        return off, doms, grids, dt, cut, skin
 
     def step(self, n: int = 1) -> "Batch":
         """Advance every system ``n`` steps; the batch itself is left
         untouched (functional semantics, like System.step)."""
-        from .system import _advance_clock, _ctx, _fresh_collider, _system_struct
+        from .system import _clock_after, _ctx, _resident_after, _resident_call
 
         st = self.state.with_kinematics()
-        col = _fresh_collider(self.system.collider)
-        sy = replace(self.system, collider=col)
         keep: list = []
-        S = _system_struct(st, sy, keep)
+        # the batch's working set stays on the device between calls, as a
+        # single system's does (unpack / pack export and import it)
+        S, slot, col, hist = _resident_call(st, self.system, keep)
         Bc = self._c_arrays(keep)
         stats = L.JdemStepStats()
         ctx = _ctx(st)
         ctx.check(ctx.lib.jdem_batch_step(ctx.ptr, ctypes.byref(S), ctypes.byref(Bc), int(n), ctypes.byref(stats)),
                   "jdem_batch_step")
         n_build = [int(v) for v in keep[-1]]
-        col2 = replace(col, n_build_times=max(n_build), overflow=bool(S.nlist.overflow))
-        sy2 = replace(sy, collider=col2)
+        col2 = _resident_after(ctx, col, S, slot, hist, st)
+        col2.__dict__["n_build_times"] = max(n_build)
+        sy2 = replace(self.system, collider=col2)
         sy2.__dict__["_last_stats"] = stats
-        systems = [_advance_clock(s, int(n)) for s in self.systems]
-        return Batch(st, sy2, self.offsets, self.mat_offsets, systems, self.bond_widths, n_build)
+        # every system's clock: one float32 accumulation per distinct (time, dt)
+        clocks: dict = {}
+        systems = []
+        for s in self.systems:
+            key = (float(F32(s.time)), float(F32(s.dt)))
+            if key not in clocks:
+                clocks[key] = _clock_after(s.time, s.dt, int(n))
+            systems.append(replace(s, time=clocks[key], step_count=s.step_count + int(n)))
+        out = Batch(st, sy2, self.offsets, self.mat_offsets, systems, self.bond_widths, n_build)
+        if "_c_cache" in self.__dict__:
+            out.__dict__["_c_cache"] = self.__dict__["_c_cache"]
+        return out
 
     def unpack(self) -> list:
         """The systems again, as (State, System) pairs.  A system's list keeps

@@ -193,7 +193,7 @@ int check_system(jdem_ctx* ctx, const jdem_system* sys, bool batch = false, bool
   if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST) {
     const jdem_nlist& L = sys->nlist;
     if (L.max_neighbors < 0) return fail(ctx, JDEM_EINVAL, "max_neighbors < 0");
-    if ((L.resident_in || L.resident_out) && (!allow_resident || batch))
+    if ((L.resident_in || L.resident_out) && !allow_resident)
       return fail(ctx, JDEM_EUNSUPPORTED, "resident working sets are used by jdem_system_step and jdem_rollout only");
     if (L.resident_in) {
       const jdem_resident* R = L.resident_in;
@@ -902,15 +902,67 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     // step kernel, rebuild chain, step kernel, ...: one host read at the end
     HIPCHK(ctx, launch_fill_i32(flags, 6, 0, st));
     HIPCHK(ctx, ws_launch_seg_init_flags(sg->n_seg, sg->n_build, sg->seg_flag, st, gate));
-    for (int64_t t = 0; t <= ns; ++t) {
-      int rc2 = launch_t(t);
-      if (rc2) return rc2;
-      if (t < ns) {
-        rc2 = rebuild();
-        if (rc2) return rc2;
+    auto step_and_chain = [&](int64_t t) -> int {
+      int rc3 = launch_t(t);
+      if (rc3) return rc3;
+      return t < ns ? rebuild() : JDEM_OK;
+    };
+    int64_t t = 0;
+    int rc2 = step_and_chain(t++);
+    if (rc2) return rc2;
+    // The middle steps (force, integration and the next step's start, then
+    // the gated rebuild chain) repeat with the same arguments every two
+    // steps: a chunk of kGraphSteps of them is captured once in a hipGraph
+    // on a private stream and replayed, which removes the host's kernel
+    // launches (~27 per step) from the critical path.  Short calls and
+    // timed calls (HIP events per kernel) launch one by one.
+    constexpr int64_t kGraphSteps = 50;  // even: the ping-pong buffers return to their start
+    if (!ctx->timing && ns - t >= 2 * kGraphSteps) {
+      if (!ctx->gstream) {
+        HIPCHK(ctx, hipStreamCreateWithFlags(&ctx->gstream, hipStreamNonBlocking));
+        // ordinary (fenced) events: they order the two streams' memory traffic
+        HIPCHK(ctx, hipEventCreateWithFlags(&ctx->gevent[0], hipEventDisableTiming));
+        HIPCHK(ctx, hipEventCreateWithFlags(&ctx->gevent[1], hipEventDisableTiming));
       }
+      hipEvent_t ev_in = ctx->gevent[0], ev_out = ctx->gevent[1];
+      HIPCHK(ctx, hipEventRecord(ev_in, st));
+      HIPCHK(ctx, hipStreamWaitEvent(ctx->gstream, ev_in, 0));
+      const hipStream_t caller = st;
+      st = ctx->gstream;
+      hipGraph_t graph = nullptr;
+      hipGraphExec_t exec = nullptr;
+      HIPCHK(ctx, hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+      int rc3 = JDEM_OK;
+      for (int64_t k = 0; k < kGraphSteps && rc3 == JDEM_OK; ++k) rc3 = step_and_chain(t + k);
+      const hipError_t ec = hipStreamEndCapture(st, &graph);
+      if (rc3) {
+        st = caller;
+        if (graph) (void)hipGraphDestroy(graph);
+        return rc3;
+      }
+      if (ec != hipSuccess) {
+        st = caller;
+        return fail(ctx, JDEM_EHIP, std::string("hipStreamEndCapture: ") + hipGetErrorString(ec));
+      }
+      const hipError_t ei = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+      (void)hipGraphDestroy(graph);
+      if (ei != hipSuccess) {
+        st = caller;
+        return fail(ctx, JDEM_EHIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(ei));
+      }
+      hipError_t el = hipSuccess;
+      for (; t + kGraphSteps <= ns - 1 && el == hipSuccess; t += kGraphSteps) el = hipGraphLaunch(exec, st);
+      (void)hipEventRecord(ev_out, st);
+      st = caller;
+      (void)hipStreamWaitEvent(st, ev_out, 0);
+      (void)hipGraphExecDestroy(exec);
+      if (el != hipSuccess) return fail(ctx, JDEM_EHIP, std::string("hipGraphLaunch: ") + hipGetErrorString(el));
     }
-    const int rc2 = read_flags(ctx, 6);
+    for (; t <= ns; ++t) {
+      rc2 = step_and_chain(t);
+      if (rc2) return rc2;
+    }
+    rc2 = read_flags(ctx, 6);
     if (rc2) return rc2;
     S->host_syncs++;
     const int32_t ev = ctx->host_flags[5];
@@ -1149,6 +1201,11 @@ int jdem_ctx_destroy(jdem_ctx* ctx) {
   for (Buf* b : ctx->bufs())
     if (b->p) (void)hipFree(b->p);
   pool_trim(ctx);
+  if (ctx->gstream) {
+    (void)hipStreamDestroy(ctx->gstream);
+    (void)hipEventDestroy(ctx->gevent[0]);
+    (void)hipEventDestroy(ctx->gevent[1]);
+  }
   if (ctx->host_flags) (void)hipHostFree(ctx->host_flags);
   for (auto& s : ctx->spans) {
     (void)hipEventDestroy(s.start);
@@ -1455,7 +1512,7 @@ int jdem_step_stage(jdem_ctx* ctx, jdem_system* sys, int32_t stage) {
 }
 
 int jdem_batch_step(jdem_ctx* ctx, jdem_system* sys, const jdem_batch* batch, int64_t n, jdem_step_stats* stats) {
-  int rc = check_system(ctx, sys, true);
+  int rc = check_system(ctx, sys, true, true);
   if (rc) return rc;
   if (!batch || batch->n_systems < 1 || !batch->offsets || !batch->domains || !batch->grids || !batch->dt ||
       !batch->cutoff || !batch->skin || !batch->n_build_times)

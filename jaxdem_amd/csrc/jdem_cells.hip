@@ -82,9 +82,13 @@ __global__ void __launch_bounds__(kBlock) k_radix_hist(const uint32_t* __restric
   if (n_dev) n = *n_dev;
   __shared__ uint32_t bins[256];
   const int tid = threadIdx.x;
+  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  if (base >= n) {  // a tile past the keys counts nothing
+    hist[(int64_t)tid * ntiles + blockIdx.x] = 0u;
+    return;
+  }
   bins[tid] = 0;
   __syncthreads();
-  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
 #pragma unroll 4
   for (int it = 0; it < RS_ITEMS; ++it) {
     int64_t idx = base + (int64_t)it * kBlock + tid;
@@ -227,6 +231,7 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t* __rest
                                                            const int32_t* __restrict__ gate) {
   if (gate && *gate == 0) return;
   if (n_dev) n = *n_dev;
+  if ((int64_t)blockIdx.x * RS_TILE >= n) return;  // a tile past the keys moves nothing
   __shared__ uint32_t run[256];
   __shared__ uint32_t wcnt[RS_WAVES][256];
   const int tid = threadIdx.x;

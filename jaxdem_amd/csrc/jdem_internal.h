@@ -64,6 +64,9 @@ struct jdem_ctx {
   // refills from here before it allocates.
   std::vector<jdem::Buf> pool;
   std::mutex pool_mu;
+  // private stream on which captured step graphs are replayed (default priority)
+  hipStream_t gstream = nullptr;
+  hipEvent_t gevent[2] = {nullptr, nullptr};
   // every scratch buffer of the context, for accounting and release
   std::vector<jdem::Buf*> bufs();
   size_t scratch_bytes();

@@ -621,9 +621,11 @@ __global__ void __launch_bounds__(kBlock) k_nlist_work(NLW a) {
   __shared__ int32_t rows[kRowLds][kBlock];
   __shared__ int row_max;
   const int tid = threadIdx.x;
+  int64_t b = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + tid;
+  // a batch workgroup whose systems all keep their rows has nothing to do
+  if (PM == 2 && !__syncthreads_or(b < a.n && a.seg_mode[a.seg[b]] != SEG_KEEP)) return;
   if (tid == 0) row_max = 0;
   __syncthreads();
-  int64_t b = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + tid;
   bool remap = a.remap != 0;
   if (PM == 2 && b < a.n) {
     const int mode = a.seg_mode[a.seg[b]];

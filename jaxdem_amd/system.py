@@ -298,12 +298,18 @@ def _alloc_history(collider: NeighborList, state_shape, force_model) -> dict:
     }
 
 
+def _clock_after(t0, dt, n: int):
+    """time after n steps: n sequential float32 additions of dt
+    (system.py:73); ufunc.accumulate adds in order, each sum rounded."""
+    if n <= 0:
+        return F32(t0)
+    seq = np.full(n + 1, F32(dt), F32)
+    seq[0] = F32(t0)
+    return F32(np.add.accumulate(seq, dtype=F32)[-1])
+
+
 def _advance_clock(system: System, n: int) -> System:
-    t = F32(system.time)
-    dt = F32(system.dt)
-    for _ in range(n):
-        t = F32(t + dt)
-    return replace(system, time=t, step_count=system.step_count + n)
+    return replace(system, time=_clock_after(system.time, system.dt, n), step_count=system.step_count + n)
 
 
 # ----------------------------------------------------------------------

@@ -285,3 +285,21 @@ def test_trajectory_rollout_save_state_signature():
     # save_state keeps the state of a frame and drops the system
     assert jd.save_state("s", "y") == "s"
     assert jd.system.save_state is jd.save_state
+
+
+def test_clock_is_sequential_float32_sum():
+    """System.time after n steps is n sequential float32 additions of dt
+    (system.py:73), for any start time and step count."""
+    from jaxdem_amd.system import _clock_after
+
+    rng = np.random.default_rng(7)
+    for _ in range(20):
+        t0 = np.float32(rng.uniform(0, 5))
+        dt = np.float32(10.0 ** rng.uniform(-6, -2))
+        n = int(rng.integers(0, 3000))
+        t = t0
+        for _ in range(n):
+            t = np.float32(t + dt)
+        assert _clock_after(t0, dt, n) == t
+    # a product n * dt would differ
+    assert _clock_after(np.float32(0), np.float32(1e-3), 40) != np.float32(0.04)

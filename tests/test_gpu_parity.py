@@ -735,6 +735,33 @@ def test_batch_ragged_sizes_equal_alone(device):
         assert ya.collider.n_build_times == yb.collider.n_build_times
 
 
+def test_batch_graph_replays_equal_eager(device):
+    """A batch call of 260 steps replays a captured 50-step graph four times
+    (steps 1-200; the rest launch one by one).  Its result equals the same
+    batch advanced in eager calls of 40 steps (no graph), bit for bit, and
+    a second graph call from the same start, after eager work and host
+    synchronisations, gives it again; list rebuilds fall inside the graphs
+    (skin 0.02)."""
+    print("[batch] graph replays", flush=True)
+    ws = [W.disks_hertz_friction(N, seed=40 + k, phi=p, skin=0.02) for k, (N, p) in
+          enumerate([(700, 0.6), (2000, 0.75), (300, 0.4)])]
+    b = jd.Batch.pack([W.make_gpu_system(w, device=device) for w in ws])
+    g1 = b.step(260)
+    assert g1.system.__dict__["_last_stats"].rebuilds >= 5
+    e = b
+    for n in (40, 40, 40, 40, 40, 40, 20):
+        e = e.step(n)
+        torch.cuda.synchronize()
+    g2 = b.step(260)
+    for f in ("pos_c", "vel", "ang_vel", "force", "torque"):
+        assert_bitwise(f"graph vs eager {f}", getattr(g1.state, f).cpu().numpy(), getattr(e.state, f).cpu().numpy())
+        assert_bitwise(f"graph again {f}", getattr(g2.state, f).cpu().numpy(), getattr(g1.state, f).cpu().numpy())
+    assert torch.equal(g1.system.collider.neighbor_list, e.system.collider.neighbor_list)
+    assert g1.n_build == e.n_build == g2.n_build
+    for (sa, _), (sb, _) in zip(g1.unpack(), [jd.System.step(*W.make_gpu_system(w, device=device), n=260) for w in ws]):
+        _kinematics_equal(sa, sb, "graph vs alone")
+
+
 def test_batch_split_steps(device):
     """batch.step(30) == batch.step(12).step(18)."""
     ws = [W.disks_hertz_friction(N, seed=10 + k, phi=0.7) for k, N in enumerate([500, 900])]
