@@ -1748,7 +1748,7 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
   // scratch: FIRE velocities and zero torque / fixed / rotation velocities,
   // per-sphere energies and virials, cell keys and starts
   ENSURE(ctx, ctx->fireV, 4 * Nn * (D + 2 * A) + Nn);
-  ENSURE(ctx, ctx->fireDev, sizeof(FireDev) + 8 * (size_t)nparts + 64);
+  ENSURE(ctx, ctx->fireDev, sizeof(FireDev) + 8 * (size_t)nparts + 64 + sizeof(PrecStop));
   ENSURE(ctx, ctx->red, 8 * (Nn * (1 + D * D) + 256 * D * D + D * D + 8));
   ENSURE(ctx, ctx->wKeys, 4 * Nn);
   ENSURE(ctx, ctx->cstart, 4 * ((size_t)n_cells + 1));
@@ -1833,7 +1833,9 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
     }
     a.w_out = with_virial ? wper : nullptr;
     HIPCHK(ctx, launch_prec_gather(a, st));
+    const int sp = span_begin(ctx, 0, 0);  // k_prec_force, for its roofline
     HIPCHK(ctx, launch_prec_force(a, st));
+    span_end(ctx, sp);
     if (N == 0) HIPCHK(ctx, hipMemsetAsync(&f.st->max_grad_bits, 0, sizeof(unsigned), st));  // else the gather clears it
     HIPCHK(ctx, launch_fire_grad_max(f, st));
     double h = 0.0;
@@ -1861,7 +1863,57 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
   // progress: a new low of max|F|, or of the energy by more than 1e-13 of it
   double best_g = gmax, best_e = pe;
   int64_t best_at = 0;
-  while (steps < max_steps) {
+  if (gated && N > 0) {
+    // The stopping tests run on the device (k_prec_check before every
+    // update; every kernel after a stop returns at once), so the host reads
+    // the loop's state once per chunk of iterations, not once per iteration.
+    PrecStop* ps = (PrecStop*)((char*)ctx->fireDev.p + sizeof(FireDev) + 8 * (size_t)nparts + 64);
+    PrecStop h_ps;
+    std::memset(&h_ps, 0, sizeof(h_ps));
+    h_ps.prev = INFINITY;
+    h_ps.first = 1;
+    HIPCHK(ctx, hipMemcpyAsync(ps, &h_ps, sizeof(h_ps), hipMemcpyHostToDevice, st));
+    PrecStopP pp;
+    pp.pe_tol = pe_tol;
+    pp.pe_diff_tol = pe_diff_tol;
+    pp.force_tol = force_tol;
+    pp.dN = dN;
+    pp.max_steps = max_steps;
+    pp.stall_steps = p->stall_steps;
+    a.stop = &ps->stop;
+    f.stop = &ps->stop;
+    constexpr int kChunk = 64;
+    while (true) {
+      for (int k = 0; k < kChunk; ++k) {
+        HIPCHK(ctx, launch_prec_check(ps, out, &f.st->max_grad_bits, pp, st));
+        HIPCHK(ctx, launch_fire_power(f, st));
+        HIPCHK(ctx, launch_fire_decide(f, st));
+        HIPCHK(ctx, launch_prec_move(f, a, st));
+        HIPCHK(ctx, launch_fire_commit(f, st));
+        // the evaluation (cell list re-sorted while a sphere changed cell)
+        HIPCHK(ctx, launch_prec_keys(a, st));
+        uint32_t* sk = nullptr;
+        int32_t* perm = nullptr;
+        HIPCHK(ctx, cell_sort_pairs(ctx, a.keys, N, n_cells, &sk, &perm, (int32_t*)ctx->cstart.p, st, moved));
+        a.perm = perm;
+        HIPCHK(ctx, launch_prec_gather(a, st));
+        const int sp = span_begin(ctx, 0, k);
+        HIPCHK(ctx, launch_prec_force(a, st));
+        span_end(ctx, sp);
+        HIPCHK(ctx, launch_fire_grad_max(f, st));
+        HIPCHK(ctx, launch_sum_f64(per, N, 1, 1, part, out, 256, st));
+      }
+      HIPCHK(ctx, hipMemcpyAsync(&h_ps, ps, sizeof(h_ps), hipMemcpyDeviceToHost, st));
+      HIPCHK(ctx, hipStreamSynchronize(st));
+      if (h_ps.stop) break;
+    }
+    steps = h_ps.steps;
+    pe = h_ps.pe;
+    gmax = h_ps.gmax;
+    a.stop = nullptr;
+    f.stop = nullptr;
+  }
+  while (!(gated && N > 0) && steps < max_steps) {
     const bool conv_pe = std::fabs(pe / dN) <= pe_tol;
     const double denom = std::max(std::max(std::fabs(pe), std::fabs(prev)), tiny);
     const bool conv_rel = std::fabs(pe - prev) / denom < pe_diff_tol;

@@ -9,6 +9,8 @@
 // domains/free.py:42-58.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "jdem_internal.h"
 
 namespace jdem {
@@ -82,20 +84,24 @@ __global__ void __launch_bounds__(kBlock) k_radix_hist(const uint32_t* __restric
   if (n_dev) n = *n_dev;
   __shared__ uint32_t bins[256];
   const int tid = threadIdx.x;
-  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
-  if (base >= n) {  // a tile past the keys counts nothing
-    hist[(int64_t)tid * ntiles + blockIdx.x] = 0u;
-    return;
-  }
-  bins[tid] = 0;
-  __syncthreads();
+  // one tile per workgroup, or (a grid smaller than the tiles) several in turn
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+    const int64_t base = (int64_t)tile * RS_TILE;
+    if (base >= n) {  // a tile past the keys counts nothing
+      hist[(int64_t)tid * ntiles + tile] = 0u;
+      continue;
+    }
+    bins[tid] = 0;
+    __syncthreads();
 #pragma unroll 4
-  for (int it = 0; it < RS_ITEMS; ++it) {
-    int64_t idx = base + (int64_t)it * kBlock + tid;
-    if (idx < n) atomicAdd(&bins[(keys[idx] >> shift) & 255u], 1u);
+    for (int it = 0; it < RS_ITEMS; ++it) {
+      int64_t idx = base + (int64_t)it * kBlock + tid;
+      if (idx < n) atomicAdd(&bins[(keys[idx] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[(int64_t)tid * ntiles + tile] = bins[tid];
+    __syncthreads();  // the next tile clears bins
   }
-  __syncthreads();
-  hist[(int64_t)tid * ntiles + blockIdx.x] = bins[tid];
 }
 
 // Parallel exclusive scan of `total` uint32 values in three launches:
@@ -231,15 +237,16 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t* __rest
                                                            const int32_t* __restrict__ gate) {
   if (gate && *gate == 0) return;
   if (n_dev) n = *n_dev;
-  if ((int64_t)blockIdx.x * RS_TILE >= n) return;  // a tile past the keys moves nothing
   __shared__ uint32_t run[256];
   __shared__ uint32_t wcnt[RS_WAVES][256];
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const uint64_t lt = (1ull << lane) - 1ull;
-  run[tid] = offs[(int64_t)tid * ntiles + blockIdx.x];
-  const int64_t base = (int64_t)blockIdx.x * RS_TILE;
+  for (int tile = blockIdx.x; tile < ntiles; tile += gridDim.x) {
+  const int64_t base = (int64_t)tile * RS_TILE;
+  if (base >= n) break;  // tiles past the keys move nothing (and later tiles lie further past)
+  run[tid] = offs[(int64_t)tid * ntiles + tile];
   for (int it = 0; it < RS_ITEMS; ++it) {
 #pragma unroll
     for (int w = 0; w < RS_WAVES; ++w) wcnt[w][tid] = 0;
@@ -274,6 +281,7 @@ __global__ void __launch_bounds__(kBlock) k_radix_scatter(const uint32_t* __rest
     run[tid] += add;
     __syncthreads();
   }
+  }
 }
 
 size_t radix_hist_words(int64_t n) {
@@ -299,7 +307,10 @@ hipError_t radix_sort_bufs(const SortBufs& sb, const uint32_t* keys_in, int64_t 
   for (int p = 0; p < passes; ++p) {
     const int shift = 8 * p;
     if (n > 0) {
-      hipLaunchKernelGGL(k_radix_hist, dim3(ntiles), dim3(kBlock), 0, s, kcur, n, shift, hist, ntiles, n_dev, gate);
+      // with the count on the device the launch is capped and the tiles are
+      // walked in turn: an idle (gated or empty) sort dispatches few workgroups
+      const int grid = n_dev ? std::min(ntiles, kCappedGrid) : ntiles;
+      hipLaunchKernelGGL(k_radix_hist, dim3(grid), dim3(kBlock), 0, s, kcur, n, shift, hist, ntiles, n_dev, gate);
       if (nchunks <= 1) {
         hipLaunchKernelGGL(k_scan_exclusive, dim3(1), dim3(1024), 0, s, hist, total, gate);
       } else {
@@ -308,7 +319,7 @@ hipError_t radix_sort_bufs(const SortBufs& sb, const uint32_t* keys_in, int64_t 
         hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(1024), 0, s, sums, nchunks, gate);
         hipLaunchKernelGGL(k_scan_apply, dim3(nchunks), dim3(kBlock), 0, s, hist, total, sums, gate);
       }
-      hipLaunchKernelGGL(k_radix_scatter, dim3(ntiles), dim3(kBlock), 0, s, kcur, vcur, kb[out], vb[out], n, shift,
+      hipLaunchKernelGGL(k_radix_scatter, dim3(grid), dim3(kBlock), 0, s, kcur, vcur, kb[out], vb[out], n, shift,
                          hist, ntiles, n_dev, gate);
     }
     hipError_t e = hipGetLastError();

@@ -84,6 +84,7 @@ struct Leaf {
 
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_fire_power(FireArgs a) {
+  if (a.stop && *a.stop) return;
   constexpr int A = D == 3 ? 3 : 1;
   __shared__ double sh[kBlock];
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -107,6 +108,7 @@ __global__ void __launch_bounds__(kBlock) k_fire_power(FireArgs a) {
 }
 
 __global__ void __launch_bounds__(kBlock) k_fire_decide(FireArgs a) {
+  if (a.stop && *a.stop) return;
   __shared__ double sh[kBlock];
   double acc = 0.0;
   for (int b = threadIdx.x; b < a.nparts; b += kBlock) acc += a.part[b];
@@ -173,13 +175,14 @@ __global__ void __launch_bounds__(kBlock) k_fire_move(FireArgs a) {
 }
 
 __global__ void k_fire_commit(FireArgs a) {
-  if (threadIdx.x != 0) return;
+  if (threadIdx.x != 0 || (a.stop && *a.stop)) return;
   a.st->dt = a.st->new_dt;
   a.st->alpha = a.st->new_alpha;
 }
 
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_fire_grad_max(FireArgs a) {
+  if (a.stop && *a.stop) return;
   constexpr int A = D == 3 ? 3 : 1;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   float m = 0.0f;
@@ -231,7 +234,7 @@ __global__ void __launch_bounds__(kBlock) k_fire_gravity_energy(FireArgs a) {
 
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_prec_keys(PrecArgs a) {
-  if (a.gated && *a.moved == 0) return;  // no sphere changed cell: the last sort stands
+  if ((a.gated && *a.moved == 0) || (a.stop && *a.stop)) return;  // no sphere changed cell: the last sort stands
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const int32_t* c = a.cidx + D * i;
@@ -242,6 +245,7 @@ __global__ void __launch_bounds__(kBlock) k_prec_keys(PrecArgs a) {
 // cell's candidates as one contiguous run
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_prec_gather(PrecArgs a) {
+  if (a.stop && *a.stop) return;
   const int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (kk == 0) {  // after this evaluation's sort, before its max|F| reduction
     if (a.gmax_bits) *a.gmax_bits = 0u;
@@ -253,18 +257,26 @@ __global__ void __launch_bounds__(kBlock) k_prec_gather(PrecArgs a) {
 }
 
 // spring law (spring.py:97-108, 138-147) on the cell stencil; float32 pair
-// arithmetic, float64 energy and virial
+// arithmetic, float64 energy and virial.  Thread kk takes the kk-th sphere
+// in cell order, so a wavefront's spheres are spatial neighbours and their
+// candidates (the cell-sorted (offset, radius) copy) are shared in cache;
+// each sphere sums its candidates in stencil order, as a thread per sphere
+// in any order would.
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_prec_force(PrecArgs a) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+  if (a.stop && *a.stop) return;
+  const int64_t kk0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (kk0 >= a.n) return;
+  const int64_t i = a.perm[kk0];
   int ci[3] = {0, 0, 0};
   float oi[3] = {0, 0, 0};
+  const float4 si = a.sp[kk0];
+  const float osi[3] = {si.x, si.y, si.z};
   for (int q = 0; q < D; ++q) {
     ci[q] = a.cidx[D * i + q];
-    oi[q] = a.off[D * i + q];
+    oi[q] = osi[q];
   }
-  const float ri = a.rad[i];
+  const float ri = si.w;
   float F[3] = {0, 0, 0};
   double e = 0.0, w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   const int zr = D == 3 ? 1 : 0;
@@ -281,7 +293,7 @@ __global__ void __launch_bounds__(kBlock) k_prec_force(PrecArgs a) {
                                  : (int64_t)cc[0] * a.dims[1] + cc[1];
         const int k1 = a.starts[h + 1];
         for (int kk = a.starts[h]; kk < k1; ++kk) {
-          if (a.perm[kk] == i) continue;
+          if (kk == kk0) continue;
           const float4 sj = a.sp[kk];
           const float osj[3] = {sj.x, sj.y, sj.z};
           // r = rh + rl, both float32: two_sum of the cell term (exact, the
@@ -340,6 +352,7 @@ __global__ void __launch_bounds__(kBlock) k_prec_force(PrecArgs a) {
 // torque), then the offset is folded back into [0, h)
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_prec_move(FireArgs f, PrecArgs a) {
+  if (a.stop && *a.stop) return;
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const FireDev s = *f.st;
@@ -367,7 +380,52 @@ __global__ void __launch_bounds__(kBlock) k_prec_move(FireArgs f, PrecArgs a) {
   if (moved) *a.moved = 1;
 }
 
+// The tests at the top of one iteration of minimize's loop
+// (routines.py:284-314: pe_tol on the energy per sphere, pe_diff_tol on the
+// relative change, force_tol on max|F|, max_steps) plus a stall test (no new
+// low of max|F|, or of the energy by more than 1e-13 of it, for stall_steps
+// iterations): sets stop, or counts the update about to run.
+__global__ void k_prec_check(PrecStop* ps, const double* pe_sum, const unsigned* gmax_bits, PrecStopP p) {
+  if (threadIdx.x != 0 || ps->stop) return;
+  const double pe = *pe_sum;
+  float gf;
+  const unsigned bits = *gmax_bits;
+  gf = __uint_as_float(bits);
+  const double g = (double)gf;
+  ps->pe = pe;
+  ps->gmax = g;
+  if (ps->first) {
+    ps->best_g = g;
+    ps->best_e = pe;
+    ps->best_at = 0;
+    ps->first = 0;
+  }
+  if (ps->steps >= p.max_steps) {
+    ps->stop = 1;
+    return;
+  }
+  const double tiny = 2.2250738585072014e-308;
+  const bool conv_pe = fabs(pe / p.dN) <= p.pe_tol;
+  const double denom = fmax(fmax(fabs(pe), fabs(ps->prev)), tiny);
+  const bool conv_rel = fabs(pe - ps->prev) / denom < p.pe_diff_tol;
+  if (g < ps->best_g) ps->best_g = g, ps->best_at = ps->steps;
+  if (pe < ps->best_e - 1e-13 * fabs(ps->best_e)) ps->best_e = pe, ps->best_at = ps->steps;
+  const bool stalled = p.stall_steps > 0 && ps->steps - ps->best_at >= p.stall_steps;
+  if (conv_pe || conv_rel || g <= p.force_tol || stalled) {
+    ps->stop = 1;
+    return;
+  }
+  ps->prev = pe;
+  ps->steps += 1;
+}
+
 }  // namespace
+
+hipError_t launch_prec_check(PrecStop* ps, const double* pe_sum, const unsigned* gmax_bits, const PrecStopP& p,
+                             hipStream_t s) {
+  hipLaunchKernelGGL(k_prec_check, dim3(1), dim3(64), 0, s, ps, pe_sum, gmax_bits, p);
+  return hipGetLastError();
+}
 
 hipError_t launch_prec_keys(const PrecArgs& a, hipStream_t s) {
   if (a.n == 0) return hipSuccess;

@@ -15,6 +15,10 @@
 namespace jdem {
 
 constexpr int kBlock = 256;
+// Workgroups of a launch whose size is known only on the device (gated
+// batch rebuilds): its kernels walk the work in turn, so an idle launch
+// dispatches this many workgroups instead of one per tile.
+constexpr int kCappedGrid = 2048;
 
 // Grid parameters on the device (jdem_grid).
 struct GridP {
@@ -528,6 +532,21 @@ struct FireArgs {
   float g[3];
   const float* mass;
   double* e_out;
+  // when set, every kernel returns at once once *stop is set (device-side
+  // stopping tests of the precision form)
+  const int32_t* stop;
+};
+// Stopping state of a FIRE loop decided on the device (routines.py:284-314
+// tests plus a stall test): k_prec_check runs before every update.
+struct PrecStop {
+  double pe, prev, best_g, best_e, gmax;
+  long long best_at, steps;
+  int32_t stop;
+  int32_t first;
+};
+struct PrecStopP {
+  double pe_tol, pe_diff_tol, force_tol, dN;
+  long long max_steps, stall_steps;
 };
 hipError_t launch_fire_power(const FireArgs& a, hipStream_t s);
 hipError_t launch_fire_decide(const FireArgs& a, hipStream_t s);
@@ -559,7 +578,12 @@ struct PrecArgs {
   int32_t* moved;        // set when a sphere changed cell
   unsigned* gmax_bits;   // max|F| word of the FIRE state, cleared by the gather
   int gated;             // 1: the key kernel runs only while *moved, and the gather clears *moved
+  const int32_t* stop;   // when set, every kernel returns at once once *stop is set
 };
+// the stopping tests of one FIRE iteration from the last evaluation's float64
+// energy sum and max|F| word; stops, or counts the update that follows
+hipError_t launch_prec_check(PrecStop* ps, const double* pe_sum, const unsigned* gmax_bits, const PrecStopP& p,
+                             hipStream_t s);
 hipError_t launch_prec_keys(const PrecArgs& a, hipStream_t s);
 hipError_t launch_prec_gather(const PrecArgs& a, hipStream_t s);
 hipError_t launch_prec_force(const PrecArgs& a, hipStream_t s);

@@ -21,6 +21,7 @@
 // and written once on exit.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstring>
 
 #include "jdem_internal.h"
@@ -360,10 +361,9 @@ __global__ void __launch_bounds__(1024) k_seg_prefix(int n_seg, const int32_t* _
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_hash_flagged(RB r) {
   if (r.gate && *r.gate == 0) return;
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (b >= r.n) return;
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < r.n; b += (int64_t)gridDim.x * blockDim.x) {
   const int s = r.seg[b];
-  if (r.seg_mode[s] == SEG_KEEP) return;
+  if (r.seg_mode[s] == SEG_KEEP) continue;
   const SegP& sp = r.segp[s];
   const float4 p = r.P[b];
   const float x[3] = {p.x, p.y, p.z};
@@ -374,6 +374,7 @@ __global__ void __launch_bounds__(kBlock) k_hash_flagged(RB r) {
   const int64_t i = r.cpref[s] + (r.orig[b] - r.offsets[s]);
   r.keys_c[i] = (uint32_t)(r.ccref[s] + wkey<D>(c, sp.grid));
   r.slot_c[i] = (int32_t)b;
+  }
 }
 
 // sorted position k -> staging (k) of the particle's fields and old list
@@ -384,8 +385,8 @@ JD int64_t rb_fn(const RB& r) { return r.fn >= 0 ? r.fn : r.cpref[r.n_seg]; }
 // and springs of k at c * n + k).
 __global__ void __launch_bounds__(kBlock) k_place_gather(RB r) {
   if (r.gate && *r.gate == 0) return;
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= rb_fn(r)) return;
+  const int64_t fn = rb_fn(r);
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < fn; k += (int64_t)gridDim.x * blockDim.x) {
   const int32_t ob = r.slot_c[r.perm[k]];
   const int s = r.seg[ob];
   const int64_t nb = r.offsets[s] + (k - r.cpref[s]);
@@ -414,13 +415,14 @@ __global__ void __launch_bounds__(kBlock) k_place_gather(RB r) {
       for (int q = 0; q < nlive; ++q) r.txi[(int64_t)q * r.n + k] = r.xi[(int64_t)q * r.n + ob];
     }
   }
+  }
 }
 
 // staging (k) -> the particle's new work slot in its system's range
 __global__ void __launch_bounds__(kBlock) k_place_scatter(RB r) {
   if (r.gate && *r.gate == 0) return;
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= rb_fn(r)) return;
+  const int64_t fn = rb_fn(r);
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < fn; k += (int64_t)gridDim.x * blockDim.x) {
   const int s = r.cs[k];
   const int64_t nb = r.offsets[s] + (k - r.cpref[s]);
   const float4 p = r.tP[k];
@@ -432,6 +434,7 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RB r) {
   r.orig[nb] = r.torig[k];
   r.old4[nb] = p;  // build position
   r.cidx[nb] = (int32_t)k;
+  }
 }
 
 // cell starts of the flagged systems: thread k fills the cells of its system
@@ -440,8 +443,8 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RB r) {
 __global__ void __launch_bounds__(kBlock) k_cell_starts_flagged(RB r, const uint32_t* __restrict__ sk,
                                                                  int32_t* __restrict__ starts) {
   if (r.gate && *r.gate == 0) return;
-  const int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= rb_fn(r)) return;
+  const int64_t fn = rb_fn(r);
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < fn; k += (int64_t)gridDim.x * blockDim.x) {
   const int s = r.cs[k];
   const SegP& sp = r.segp[s];
   const int64_t first = r.cpref[s], last = r.cpref[s + 1] - 1;
@@ -451,6 +454,7 @@ __global__ void __launch_bounds__(kBlock) k_cell_starts_flagged(RB r, const uint
   for (int64_t h = lo; h <= key; ++h) starts[sp.cell_base + h] = slot;
   if (k == last)
     for (int64_t h = key + 1; h <= sp.grid.n_cells; ++h) starts[sp.cell_base + h] = (int32_t)r.offsets[s + 1];
+  }
 }
 
 // neighbour list in work space (cell_list.py:500-595).  The stencil is
@@ -621,9 +625,13 @@ __global__ void __launch_bounds__(kBlock) k_nlist_work(NLW a) {
   __shared__ int32_t rows[kRowLds][kBlock];
   __shared__ int row_max;
   const int tid = threadIdx.x;
-  int64_t b = (int64_t)xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + tid;
+  // one tile of kBlock slots per workgroup, or (a grid smaller than the
+  // tiles, as batches launch it) several in turn
+  const int64_t ntiles = (a.n + kBlock - 1) / kBlock;
+  for (int64_t tile = xcd_tile(blockIdx.x, gridDim.x); tile < ntiles; tile += gridDim.x) {
+  int64_t b = tile * blockDim.x + tid;
   // a batch workgroup whose systems all keep their rows has nothing to do
-  if (PM == 2 && !__syncthreads_or(b < a.n && a.seg_mode[a.seg[b]] != SEG_KEEP)) return;
+  if (PM == 2 && !__syncthreads_or(b < a.n && a.seg_mode[a.seg[b]] != SEG_KEEP)) continue;
   if (tid == 0) row_max = 0;
   __syncthreads();
   bool remap = a.remap != 0;
@@ -733,6 +741,8 @@ __global__ void __launch_bounds__(kBlock) k_nlist_work(NLW a) {
   }
   remap_row(a, b, active, c, [&](int s) { return s < kRowLds ? rows[s][tid] : a.nl[(int64_t)s * a.n + b]; },
             remap);
+  __syncthreads();  // the next tile reuses rows and row_max
+  }
 }
 
 
@@ -764,17 +774,20 @@ __global__ void __launch_bounds__(kBlock) k_pack_springs(int64_t n, int M, int W
                                                           const int32_t* __restrict__ seg_mode,
                                                           const int32_t* __restrict__ gate) {
   if (gate && *gate == 0) return;
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int lane = threadIdx.x & 63;
-  if (!wave_touched(b, n, seg, seg_mode)) return;  // wave-uniform
-  int cnt = 0;
-  if (b < n)
-    for (int w = 0; w < W64; ++w) cnt += __popcll(live[(int64_t)w * n + b]);
-  const int off = wave_exclusive_scan(cnt, lane);
-  if (b >= n) return;
-  float4* dst = wave + (b & ~(int64_t)63) * M + off;
-  for (int k = 0; k < cnt; ++k) dst[k] = rank[(int64_t)k * n + b];
-  W[b].w = __int_as_float(off);
+  const int64_t npad = (n + 63) & ~(int64_t)63;  // whole waves: every lane of a wave loops alike
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < npad; b += stride) {
+    if (!wave_touched(b, n, seg, seg_mode)) continue;  // wave-uniform
+    int cnt = 0;
+    if (b < n)
+      for (int w = 0; w < W64; ++w) cnt += __popcll(live[(int64_t)w * n + b]);
+    const int off = wave_exclusive_scan(cnt, lane);
+    if (b >= n) continue;
+    float4* dst = wave + (b & ~(int64_t)63) * M + off;
+    for (int k = 0; k < cnt; ++k) dst[k] = rank[(int64_t)k * n + b];
+    W[b].w = __int_as_float(off);
+  }
 }
 
 __global__ void __launch_bounds__(kBlock) k_unpack_springs(int64_t n, int M, int W64, const float4* __restrict__ wave,
@@ -784,12 +797,15 @@ __global__ void __launch_bounds__(kBlock) k_unpack_springs(int64_t n, int M, int
                                                             const int32_t* __restrict__ seg_mode,
                                                             const int32_t* __restrict__ gate) {
   if (gate && *gate == 0) return;
-  const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (!wave_touched(b, n, seg, seg_mode) || b >= n) return;
-  int cnt = 0;
-  for (int w = 0; w < W64; ++w) cnt += __popcll(live[(int64_t)w * n + b]);
-  const float4* src = wave + (b & ~(int64_t)63) * M + __float_as_int(W[b].w);
-  for (int k = 0; k < cnt; ++k) rank[(int64_t)k * n + b] = src[k];
+  const int64_t npad = (n + 63) & ~(int64_t)63;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < npad; b += stride) {
+    if (!wave_touched(b, n, seg, seg_mode) || b >= n) continue;
+    int cnt = 0;
+    for (int w = 0; w < W64; ++w) cnt += __popcll(live[(int64_t)w * n + b]);
+    const float4* src = wave + (b & ~(int64_t)63) * M + __float_as_int(W[b].w);
+    for (int k = 0; k < cnt; ++k) rank[(int64_t)k * n + b] = src[k];
+  }
 }
 
 // ---------------------------------------------------------------------------
@@ -1178,16 +1194,16 @@ hipError_t ws_launch_pack_springs(int64_t n, int M, int W64, const float4* rank,
                                   float4* W, const int32_t* seg, const int32_t* seg_mode, hipStream_t s,
                                   const int32_t* gate) {
   if (n == 0 || M == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_pack_springs, dim3(nblocks(n)), dim3(kBlock), 0, s, n, M, W64, rank, live, wave, W, seg,
-                     seg_mode, gate);
+  hipLaunchKernelGGL(k_pack_springs, dim3(gate ? std::min<unsigned>(nblocks(n), (unsigned)kCappedGrid) : nblocks(n)),
+                     dim3(kBlock), 0, s, n, M, W64, rank, live, wave, W, seg, seg_mode, gate);
   return hipGetLastError();
 }
 hipError_t ws_launch_unpack_springs(int64_t n, int M, int W64, const float4* wave, const uint64_t* live,
                                     const float4* W, float4* rank, const int32_t* seg, const int32_t* seg_mode,
                                     hipStream_t s, const int32_t* gate) {
   if (n == 0 || M == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_unpack_springs, dim3(nblocks(n)), dim3(kBlock), 0, s, n, M, W64, wave, live, W, rank, seg,
-                     seg_mode, gate);
+  hipLaunchKernelGGL(k_unpack_springs, dim3(gate ? std::min<unsigned>(nblocks(n), (unsigned)kCappedGrid) : nblocks(n)),
+                     dim3(kBlock), 0, s, n, M, W64, wave, live, W, rank, seg, seg_mode, gate);
   return hipGetLastError();
 }
 
@@ -1317,15 +1333,17 @@ hipError_t ws_launch_seg_prefix(int n_seg, const int32_t* seg_mode, const int64_
 }
 hipError_t ws_launch_hash_flagged(const RB& r, int dim, hipStream_t s) {
   if (r.n == 0) return hipSuccess;
+  const dim3 g(r.gate ? std::min<unsigned>(nblocks(r.n), (unsigned)kCappedGrid) : nblocks(r.n));
   if (dim == 3)
-    hipLaunchKernelGGL(k_hash_flagged<3>, dim3(nblocks(r.n)), dim3(kBlock), 0, s, r);
+    hipLaunchKernelGGL(k_hash_flagged<3>, g, dim3(kBlock), 0, s, r);
   else
-    hipLaunchKernelGGL(k_hash_flagged<2>, dim3(nblocks(r.n)), dim3(kBlock), 0, s, r);
+    hipLaunchKernelGGL(k_hash_flagged<2>, g, dim3(kBlock), 0, s, r);
   return hipGetLastError();
 }
 hipError_t ws_launch_place(const RB& r, const uint32_t* sk, int32_t* starts, hipStream_t s) {
   if (r.fn == 0) return hipSuccess;
-  const dim3 g(nblocks(r.fn >= 0 ? r.fn : r.n)), b(kBlock);  // device count: sized for every particle
+  // a device count: a capped grid that walks the flagged particles in turn
+  const dim3 g(r.fn >= 0 ? nblocks(r.fn) : std::min<unsigned>(nblocks(r.n), (unsigned)kCappedGrid)), b(kBlock);
   hipLaunchKernelGGL(k_place_gather, g, b, 0, s, r);
   hipLaunchKernelGGL(k_place_scatter, g, b, 0, s, r);
   hipLaunchKernelGGL(k_cell_starts_flagged, g, b, 0, s, r, sk, starts);
@@ -1362,10 +1380,11 @@ hipError_t ws_launch_nlist(const NLW& a, int dim, bool naive, hipStream_t s) {
         hipLaunchKernelGGL((k_nlist_work_naive<2, false>), g, b, 0, s, a);
     }
   } else if (a.segp) {
+    const dim3 gs(a.gate ? std::min<unsigned>(g.x, (unsigned)kCappedGrid) : g.x);
     if (dim == 3)
-      hipLaunchKernelGGL((k_nlist_work<3, 2>), g, b, 0, s, a);
+      hipLaunchKernelGGL((k_nlist_work<3, 2>), gs, b, 0, s, a);
     else
-      hipLaunchKernelGGL((k_nlist_work<2, 2>), g, b, 0, s, a);
+      hipLaunchKernelGGL((k_nlist_work<2, 2>), gs, b, 0, s, a);
   } else {
     if (dim == 3) {
       if (per)
