@@ -316,6 +316,19 @@ def precision_bench(args, dev) -> dict:
     secs = time.perf_counter() - t
     print(f"[bench] precision: {r['steps']} steps in {secs:.2f} s, max|F| {r['max_grad']:.3e}", file=sys.stderr,
           flush=True)
+    # roofline of the force kernel: a short run of the same start with HIP
+    # events around every k_prec_force launch
+    from jaxdem_amd import _lib as Lb
+
+    ctx = Lb.context_for(dev)
+    ctx.set_timing(True)
+    PR.minimize_precise(pos, rad, L, k=1.0, dt=1e-2, max_steps=200, pe_tol=0.0, pe_diff_tol=-1.0, force_tol=0.0,
+                        device=dev)
+    tm = ctx.get_timing()
+    ctx.set_timing(False)
+    avg_ms = tm["step_kernel_ms"] / max(tm["step_kernels"], 1)
+    bps = 48.0  # per sphere: cell (12 B) + offset (12) + radius (4) read, force (12) + energy (8, float64) written
+    achieved = args.precision_n * bps / (avg_ms * 1e-3) / 1e9
     return {
         "workload": f"bidisperse_jamming: {args.precision_n} spheres (r 0.5/0.7, 50:50), periodic cube L = {L:.2f} "
         "at phi 0.66, harmonic k = 1, FIRE dt0 = 1e-2, integer cell + float32 offset positions, float64 "
@@ -327,6 +340,59 @@ def precision_bench(args, dev) -> dict:
         "max_grad": r["max_grad"],
         "pe_per_particle": r["pe"],
         "pressure": r["pressure"],
+        "roofline": {"bound": "hbm", "kernel": "k_prec_force", "kernel_avg_us": 1e3 * avg_ms,
+                     "algorithmic_bytes_per_sphere": bps, "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS},
+        "fp64_error": precision_fp64_error(dev),
+    }
+
+
+def _rattlers(pos, rad, L):
+    """Spheres with fewer than dim + 1 contacts, removed until none is left
+    (the contact network that carries the float64 minimum's forces)."""
+    import numpy as np
+
+    N, D = pos.shape
+    d = pos[:, None, :] - pos[None, :, :]
+    d -= L * np.rint(d / L)
+    touch = np.sum(d * d, axis=-1) < (rad[:, None] + rad[None, :]) ** 2
+    np.fill_diagonal(touch, False)
+    keep = np.ones(N, bool)
+    while True:
+        z = (touch & keep[None, :]).sum(axis=1)
+        drop = keep & (z < D + 1)
+        if not drop.any():
+            return ~keep
+        keep &= ~drop
+
+
+def precision_fp64_error(dev) -> dict:
+    """The error of the precision form against a float64 result: FIRE started
+    at the committed float64 minimum of 1024 bidisperse spheres at phi 0.66
+    (tests/golden/jamming_n1024.npz, max|F| < 1e-12, made by float64 FIRE)
+    must stay there; reported are the run's own float64 E/N and pressure
+    against the minimum's, and the non-rattler positions after the mean
+    shift."""
+    import numpy as np
+
+    from jaxdem_amd import precision as PR
+
+    fx = np.load(ROOT / "tests" / "golden" / "jamming_n1024.npz")
+    rad, L, ref = fx["rad"], float(fx["L"]), fx["ref"]
+    r = PR.minimize_precise(ref, rad, L, k=1.0, dt=1e-2, max_steps=20000, pe_tol=0.0, pe_diff_tol=-1.0,
+                            force_tol=1e-12, stall_steps=5000, device=dev)
+    keep = ~_rattlers(ref, rad, L)
+    dx = PR.min_image_diff(r["pos"], ref, L)
+    err = np.abs(dx[keep] - dx[keep].mean(axis=0))
+    return {
+        "reference": "float64 FIRE minimum of 1024 spheres, tests/golden/jamming_n1024.npz",
+        "steps": r["steps"],
+        "max_grad": r["max_grad"],
+        "pe_per_particle_rel_err": abs(r["pe"] - float(fx["ref_pe"])) / float(fx["ref_pe"]),
+        "pressure_rel_err": abs(r["pressure"] - float(fx["ref_pressure"])) / float(fx["ref_pressure"]),
+        "pos_max_err": float(err.max()),
+        "pos_rms_err": float(np.sqrt(np.mean(err ** 2))),
+        "non_rattlers": int(keep.sum()),
     }
 
 

@@ -396,7 +396,12 @@ int jdem_ctx_trim(jdem_ctx* ctx);
  * cell_list.py:537-538), dt, and the NeighborList cutoff and skin. */
 typedef struct jdem_batch {
   int32_t n_systems;
-  int32_t reserved;
+  /* how rebuild decisions reach the kernels: 0 = a gated kernel chain after
+   * every step, long calls replayed from a captured hipGraph; 1 = the same
+   * chain launched step by step; 2 = speculative chunks of step kernels with
+   * one host read per chunk, the chain launched only when a system asked for
+   * a rebuild (fewer launches when rebuilds are rare) */
+  int32_t launch_mode;
   int32_t* n_build_times; /* in/out: NeighborList builds of each system       */
   const int64_t* offsets;
   const jdem_domain* domains;

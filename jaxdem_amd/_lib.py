@@ -168,7 +168,7 @@ class JdemTiming(ctypes.Structure):
 class JdemBatch(ctypes.Structure):
     _fields_ = [
         ("n_systems", c_int32),
-        ("reserved", c_int32),
+        ("launch_mode", c_int32),
         ("n_build_times", c_void_p),
         ("offsets", c_void_p),
         ("domains", c_void_p),

@@ -96,6 +96,10 @@ class Batch:
     systems: list
     bond_widths: list
     n_build: list  # NeighborList builds of each system
+    # how rebuild decisions reach the kernels (jdem_batch.launch_mode): 0 a
+    # gated chain after every step with captured graphs for long calls, 1 the
+    # chain step by step, 2 speculative chunks with one host read per chunk
+    launch_mode: int = 0
 
     def __len__(self) -> int:
         return len(self.systems)
@@ -208,7 +212,7 @@ class Batch:
         hist = None if col.history is None else {k: v.to(device) for k, v in col.history.items()}
         col2 = replace(col, neighbor_list=col.neighbor_list.to(device), old_pos=col.old_pos.to(device), history=hist)
         return Batch(State(**kw), replace(self.system, collider=col2), self.offsets, self.mat_offsets, self.systems,
-                     self.bond_widths, list(self.n_build))
+                     self.bond_widths, list(self.n_build), self.launch_mode)
 
     def _c_arrays(self, keep: list) -> L.JdemBatch:
         """The jdem_batch of the systems (their domains, grids, dt, cutoffs,
@@ -222,6 +226,7 @@ class Batch:
         keep += [off, doms, grids, dt, cut, skin, nb]
         b = L.JdemBatch()
         b.n_systems = len(self.systems)
+        b.launch_mode = int(self.launch_mode)
         b.n_build_times = nb.ctypes.data
         b.offsets = off.ctypes.data
         b.domains = ctypes.addressof(doms)
@@ -278,7 +283,7 @@ class Batch:
             if key not in clocks:
                 clocks[key] = _clock_after(s.time, s.dt, int(n))
             systems.append(replace(s, time=clocks[key], step_count=s.step_count + int(n)))
-        out = Batch(st, sy2, self.offsets, self.mat_offsets, systems, self.bond_widths, n_build)
+        out = Batch(st, sy2, self.offsets, self.mat_offsets, systems, self.bond_widths, n_build, self.launch_mode)
         if "_c_cache" in self.__dict__:
             out.__dict__["_c_cache"] = self.__dict__["_c_cache"]
         return out

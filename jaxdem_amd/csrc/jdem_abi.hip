@@ -497,6 +497,7 @@ struct SegRun {
   int32_t* seg_flag;  // system asks for a rebuild (device)
   int32_t* seg_mode;  // what the current rebuild does to it (device)
   bool any_unbuilt;   // some system has no list yet
+  int launch_mode;    // jdem_batch.launch_mode
 };
 
 // n steps of a NeighborList system on the resident working set.  With
@@ -876,10 +877,12 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     a.flag_rd = t >= 1 ? flags + (t & 1) : nullptr;
     a.flag_wr = flags + ((t + 1) & 1);
     a.step_idx = (int32_t)t;
-    if (sg) {
+    if (sg && sg->launch_mode != 2) {
       // no speculation: the rebuild chain after this kernel acts on the gate
       a.flag_rd = nullptr;
       a.flag_wr = gate;
+    }
+    if (sg) {
       a.seg = sg->seg;
       a.segp = sg->segp;
       a.seg_flag = sg->seg_flag;
@@ -898,6 +901,57 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   nseg = ns;
   cur_at.assign((size_t)ns + 2, 0);
   xc_at.assign((size_t)ns + 2, 0);
+  if (sg && sg->launch_mode == 2) {
+    // Speculative chunks, as for one system: each step kernel checks the
+    // word its predecessor wrote and does nothing once a system asked for a
+    // rebuild; the host reads the words once per chunk, runs the rebuild
+    // chain (its gate raised from the host) and resumes at the flagged step.
+    HIPCHK(ctx, launch_fill_i32(flags, 6, 0, st));
+    HIPCHK(ctx, ws_launch_seg_init_flags(sg->n_seg, sg->n_build, sg->seg_flag, st, gate));
+    int rc2 = launch_t(0);
+    if (rc2) return rc2;
+    if (sg->any_unbuilt) {  // systems without a list build before the first force
+      rc2 = rebuild();
+      if (rc2) return rc2;
+      HIPCHK(ctx, launch_fill_i32(flags, 2, 0, st));  // that chain also served any system kernel 0 flagged
+    }
+    int64_t next = 1, chunk_len = 32, last_rebuild_at = 0;
+    while (next <= ns) {
+      const int64_t cnt = std::min<int64_t>(chunk_len, ns - next + 1);
+      for (int64_t tt = next; tt < next + cnt; ++tt) {
+        rc2 = launch_t(tt);
+        if (rc2) return rc2;
+      }
+      rc2 = read_flags(ctx, 6);
+      if (rc2) return rc2;
+      S->host_syncs++;
+      const int32_t v = std::max(ctx->host_flags[0], ctx->host_flags[1]);
+      if (v == 0) {
+        next += cnt;
+        chunk_len = std::min<int64_t>(chunk_len * 2, 1024);
+        continue;
+      }
+      // kernels v.. of this chunk did no work; the data is where kernel v read it
+      if (v < next + cnt) {
+        cur = cur_at[v];
+        xc = xc_at[v];
+      }
+      HIPCHK(ctx, launch_fill_i32(gate, 1, 1, st));
+      rc2 = rebuild();
+      if (rc2) return rc2;
+      HIPCHK(ctx, launch_fill_i32(flags, 2, 0, st));
+      if (last_rebuild_at > 0) chunk_len = std::max<int64_t>(2, v - last_rebuild_at + 1);
+      last_rebuild_at = v;
+      next = v;
+    }
+    rc2 = read_flags(ctx, 6);
+    if (rc2) return rc2;
+    const int32_t ev = ctx->host_flags[5];
+    S->rebuilds += ev;
+    L.n_build_times += ev;
+    if (ev > 0) rebuilt_here = true;
+    return JDEM_OK;
+  }
   if (sg) {
     // step kernel, rebuild chain, step kernel, ...: one host read at the end
     HIPCHK(ctx, launch_fill_i32(flags, 6, 0, st));
@@ -917,7 +971,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     // launches (~27 per step) from the critical path.  Short calls and
     // timed calls (HIP events per kernel) launch one by one.
     constexpr int64_t kGraphSteps = 50;  // even: the ping-pong buffers return to their start
-    if (!ctx->timing && ns - t >= 2 * kGraphSteps) {
+    if (sg->launch_mode == 0 && !ctx->timing && ns - t >= 2 * kGraphSteps) {
       if (!ctx->gstream) {
         HIPCHK(ctx, hipStreamCreateWithFlags(&ctx->gstream, hipStreamNonBlocking));
         // ordinary (fenced) events: they order the two streams' memory traffic
@@ -1518,6 +1572,7 @@ int jdem_batch_step(jdem_ctx* ctx, jdem_system* sys, const jdem_batch* batch, in
       !batch->cutoff || !batch->skin || !batch->n_build_times)
     return fail(ctx, JDEM_EINVAL, "incomplete batch description");
   if (n < 0) return fail(ctx, JDEM_EINVAL, "negative step count");
+  if (batch->launch_mode < 0 || batch->launch_mode > 2) return fail(ctx, JDEM_EINVAL, "launch_mode must be 0, 1 or 2");
   if (sys->collider_kind != JDEM_COLLIDER_NEIGHBOR_LIST || sys->nlist.secondary != JDEM_COLLIDER_CELL_LIST)
     return fail(ctx, JDEM_EUNSUPPORTED, "batches run the NeighborList collider over a CellList");
   const int B = batch->n_systems;
@@ -1576,6 +1631,7 @@ int jdem_batch_step(jdem_ctx* ctx, jdem_system* sys, const jdem_batch* batch, in
   sg.seg_flag = dbuild + B;
   sg.seg_mode = dbuild + 2 * B;
   sg.any_unbuilt = any_unbuilt;
+  sg.launch_mode = batch->launch_mode;
   jdem_step_stats local;
   jdem_step_stats* S = stats ? stats : &local;
   std::memset(S, 0, sizeof(*S));

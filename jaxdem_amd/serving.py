@@ -78,7 +78,8 @@ class Server:
     ones in the queue."""
 
     def __init__(self, device, hbm_budget: int | None = None, chunk: int = 50, max_active: int = 4096,
-                 clock: Callable[[], float] = time.perf_counter, chunk_ms: float = 20.0, policy: str = "fcfs"):
+                 clock: Callable[[], float] = time.perf_counter, chunk_ms: float = 20.0, policy: str = "fcfs",
+                 launch_mode: int = 2):
         if policy not in ("fcfs", "sjf"):
             raise ValueError(f"unknown admission policy {policy!r} (fcfs, sjf)")
         if int(max_active) < 1:
@@ -93,6 +94,9 @@ class Server:
         self.chunk_ms = float(chunk_ms)
         self.max_active = int(max_active)
         self.clock = clock
+        # a served batch holds a few systems that rebuild rarely: speculative
+        # chunks launch fewer kernels than a rebuild chain after every step
+        self.launch_mode = int(launch_mode)
         self.log: list = []
 
     def run(self, requests: Sequence[Request], realtime: bool = True) -> list:
@@ -138,6 +142,7 @@ class Server:
                 pairs = (batch.unpack() if batch is not None else []) + [(r.state, r.system) for r in admit]
                 active = active + admit
                 batch = Batch.pack(pairs)
+                batch.launch_mode = self.launch_mode
                 ms_per_step = None
             if not active:
                 if pending and realtime:
@@ -166,6 +171,8 @@ class Server:
                         keep.append((r, pair))
                 active = [r for r, _ in keep]
                 batch = Batch.pack([p for _, p in keep]) if keep else None
+                if batch is not None:
+                    batch.launch_mode = self.launch_mode
                 ms_per_step = None
         return done
 

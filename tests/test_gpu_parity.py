@@ -762,6 +762,30 @@ def test_batch_graph_replays_equal_eager(device):
         _kinematics_equal(sa, sb, "graph vs alone")
 
 
+def test_batch_launch_modes_equal(device):
+    """The three ways rebuild decisions reach the kernels -- a gated chain
+    after every step with captured graphs, the same chain launched step by
+    step, speculative chunks with a host read per chunk -- give the same
+    batch bit for bit, with rebuilds inside the run and a system joining
+    without a list."""
+    print("[batch] launch modes", flush=True)
+    ws = [W.disks_hertz_friction(N, seed=60 + k, phi=p, skin=0.02) for k, (N, p) in
+          enumerate([(900, 0.7), (256, 0.35), (3000, 0.6)])]
+    pairs = [W.make_gpu_system(w, device=device) for w in ws]
+    pairs[0] = jd.System.step(*pairs[0], n=7)
+    out = []
+    for mode in (0, 1, 2):
+        b = jd.Batch.pack(pairs)
+        b.launch_mode = mode
+        out.append(b.step(230))
+    for mode, o in zip((1, 2), out[1:]):
+        for f in ("pos_c", "vel", "ang_vel", "force", "torque"):
+            assert_bitwise(f"mode {mode} {f}", getattr(o.state, f).cpu().numpy(), getattr(out[0].state, f).cpu().numpy())
+        assert torch.equal(o.system.collider.neighbor_list, out[0].system.collider.neighbor_list)
+        assert o.n_build == out[0].n_build
+    assert out[2].system.__dict__["_last_stats"].host_syncs > 1
+
+
 def test_batch_split_steps(device):
     """batch.step(30) == batch.step(12).step(18)."""
     ws = [W.disks_hertz_friction(N, seed=10 + k, phi=0.7) for k, N in enumerate([500, 900])]
