@@ -52,12 +52,16 @@ def parse_args():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--batch-skins", type=str, default="0.1,0.2,0.3,0.45,0.6",
                     help="candidate skins of the batch autotuner (empty: keep the workload's 0.1)")
+    ap.add_argument("--batch-cell-factors", type=str, default="1.0,0.5,1.5",
+                    help="cell sizes of the batch list build, in list radii, tried by the autotuner")
     ap.add_argument("--batch-probe-every", type=int, default=4, help="the autotuner probes every k-th system")
     ap.add_argument("--batch-probe-steps", type=int, default=200)
+    ap.add_argument("--batch-probe-warm", type=int, default=300,
+                    help="untimed steps before each autotuner probe (the lattice start melts first)")
     ap.add_argument("--serve-requests", type=int, default=32)
     ap.add_argument("--serve-rate", type=float, default=16.0, help="Poisson arrival rate, requests per second")
     ap.add_argument("--no-serve", action="store_true")
-    ap.add_argument("--serve-max-active", type=int, default=4, help="requests in the serving batch at most")
+    ap.add_argument("--serve-max-active", type=int, default=8, help="requests in the serving batch at most")
     ap.add_argument("--serve-policy", type=str, default="fcfs", help="serving admission: fcfs or sjf")
     ap.add_argument("--serve-loads", type=str, default="",
                     help="also sweep offered loads, as fractions of the one-at-a-time capacity (e.g. 0.3,0.6,0.9)")
@@ -168,27 +172,37 @@ def batch_bench(args, dev) -> dict:
     import torch
 
     import jaxdem_amd as jd
+    from jaxdem_amd import _lib as L
     from jaxdem_amd import workloads as W
 
-    skin, tune = 0.1, []
+    skin, cell_factor, tune = 0.1, 1.0, []
     if args.batch_skins:
-        from jaxdem_amd.batching import autotune_skin
+        from jaxdem_amd.batching import autotune
 
         def sample(sk):
             return [W.make_gpu_system(w, device="cpu")
                     for w in W.batch_disks(args.batch_systems, seed=args.seed, skin=sk, every=args.batch_probe_every)]
 
-        skin, tune = autotune_skin(sample, [float(x) for x in args.batch_skins.split(",") if x],
-                                   probe_steps=args.batch_probe_steps, device=dev)
-        print(f"[bench] batch autotuner: skin {skin} from {tune}", file=sys.stderr, flush=True)
+        cfg, tune = autotune(sample, [float(x) for x in args.batch_skins.split(",") if x],
+                             cell_factors=[float(x) for x in args.batch_cell_factors.split(",") if x],
+                             tiles=(), modes=(), probe_steps=args.batch_probe_steps,
+                             warm_steps=args.batch_probe_warm, device=dev)
+        skin, cell_factor = cfg["skin"], cfg["cell_factor"]
+        print(f"[bench] batch autotuner: skin {skin}, cell factor {cell_factor} from {tune}", file=sys.stderr,
+              flush=True)
+    from jaxdem_amd.batching import tune_launch, with_cell_factor
+
     t0 = time.perf_counter()
     ws = W.batch_disks(args.batch_systems, seed=args.seed, skin=skin)
-    pairs = [W.make_gpu_system(w, device="cpu") for w in ws]
+    pairs = with_cell_factor([W.make_gpu_system(w, device="cpu") for w in ws], cell_factor)
     b = jd.Batch.pack(pairs).to(dev)
     del pairs
     prep = time.perf_counter() - t0
     N = b.n_particles
     print(f"[bench] batch: {len(ws)} systems, {N} disks, packed in {prep:.1f} s", file=sys.stderr, flush=True)
+    launch_cfg, launch_tune = tune_launch(b, probe_steps=args.batch_probe_steps, warm_steps=args.batch_probe_warm)
+    tune = tune + launch_tune
+    print(f"[bench] batch launch autotuner: {launch_cfg} from {launch_tune}", file=sys.stderr, flush=True)
     b = b.step(20)  # first list builds, not timed
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -197,6 +211,20 @@ def batch_bench(args, dev) -> dict:
     secs = time.perf_counter() - t0
     batched = N * args.batch_steps / secs
     stats = b2.system.__dict__.get("_last_stats")
+    # roofline pass: the same steps again with HIP events around every step
+    # kernel and every rebuild chain (kernels launched one by one)
+    ctx = L.context_for(dev)
+    ctx.set_timing(True)
+    n_rl = args.batch_steps
+    b3 = b.step(n_rl)
+    timing = ctx.get_timing()
+    ctx.set_timing(False)
+    torch.cuda.synchronize()
+    zh = live_contacts(b3.system) / N
+    bytes_per_disk = 96.0 + 24.0 * zh
+    avg_ms = timing["step_kernel_ms"] / max(timing["step_kernels"], 1)
+    achieved = N * bytes_per_disk / (avg_ms * 1e-3) / 1e9
+    del b3
     # one at a time: a sample stratified by size, the same number of steps
     order = np.argsort([w.N for w in ws])
     pick = order[np.linspace(0, len(ws) - 1, min(args.plain_sample, len(ws))).astype(int)]
@@ -228,9 +256,27 @@ def batch_bench(args, dev) -> dict:
         "batched_over_plain": batched / plain,
         "rebuilds": int(stats.rebuilds) if stats is not None else None,
         "skin": skin,
-        "autotuner": {"candidates_skin_value_rebuilds": tune,
-                      "probe": f"every {args.batch_probe_every}th system, {args.batch_probe_steps} steps"}
+        "cell_factor": cell_factor,
+        "tile_width": b.tile_width,
+        "launch_mode": b.launch_mode,
+        "autotuner": {"knob_value_rate_rebuilds": tune,
+                      "probe": f"skin and cell factor on every {args.batch_probe_every}th system, tile width and "
+                      f"launch mode on the whole batch; {args.batch_probe_steps} steps after {args.batch_probe_warm}"}
         if tune else None,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_ws_step<2, HERTZ_FRICTION, segmented>",
+            "kernel_avg_us": 1e3 * avg_ms,
+            "algorithmic_bytes_per_disk": bytes_per_disk,
+            "mean_history_contacts": zh,
+            "step_kernels_ms_per_step": timing["step_kernel_ms"] / n_rl,
+            "rebuild_chain_ms_per_step": timing["rebuild_ms"] / n_rl,
+        },
     }
 
 

@@ -409,6 +409,9 @@ typedef struct jdem_batch {
   const float* dt;
   const float* cutoff;
   const float* skin;
+  int32_t tile_width; /* particles per workgroup of the step kernel: 256
+                       * (also for 0) or 128 */
+  int32_t reserved;
 } jdem_batch;
 
 /* System.step for a batch of systems of unequal size in one segmented

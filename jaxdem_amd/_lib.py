@@ -176,6 +176,8 @@ class JdemBatch(ctypes.Structure):
         ("dt", c_void_p),
         ("cutoff", c_void_p),
         ("skin", c_void_p),
+        ("tile_width", c_int32),
+        ("reserved", c_int32),
     ]
 
 

@@ -100,6 +100,8 @@ class Batch:
     # gated chain after every step with captured graphs for long calls, 1 the
     # chain step by step, 2 speculative chunks with one host read per chunk
     launch_mode: int = 0
+    # particles per workgroup of the step kernel (256 or 128)
+    tile_width: int = 256
 
     def __len__(self) -> int:
         return len(self.systems)
@@ -212,7 +214,7 @@ class Batch:
         hist = None if col.history is None else {k: v.to(device) for k, v in col.history.items()}
         col2 = replace(col, neighbor_list=col.neighbor_list.to(device), old_pos=col.old_pos.to(device), history=hist)
         return Batch(State(**kw), replace(self.system, collider=col2), self.offsets, self.mat_offsets, self.systems,
-                     self.bond_widths, list(self.n_build), self.launch_mode)
+                     self.bond_widths, list(self.n_build), self.launch_mode, self.tile_width)
 
     def _c_arrays(self, keep: list) -> L.JdemBatch:
         """The jdem_batch of the systems (their domains, grids, dt, cutoffs,
@@ -227,6 +229,7 @@ class Batch:
         b = L.JdemBatch()
         b.n_systems = len(self.systems)
         b.launch_mode = int(self.launch_mode)
+        b.tile_width = int(self.tile_width)
         b.n_build_times = nb.ctypes.data
         b.offsets = off.ctypes.data
         b.domains = ctypes.addressof(doms)
@@ -283,7 +286,8 @@ class Batch:
             if key not in clocks:
                 clocks[key] = _clock_after(s.time, s.dt, int(n))
             systems.append(replace(s, time=clocks[key], step_count=s.step_count + int(n)))
-        out = Batch(st, sy2, self.offsets, self.mat_offsets, systems, self.bond_widths, n_build, self.launch_mode)
+        out = Batch(st, sy2, self.offsets, self.mat_offsets, systems, self.bond_widths, n_build, self.launch_mode,
+                    self.tile_width)
         if "_c_cache" in self.__dict__:
             out.__dict__["_c_cache"] = self.__dict__["_c_cache"]
         return out
@@ -365,4 +369,94 @@ def autotune_skin(make_pairs: Callable[[float], list], skins: Sequence[float], p
     return best, table
 
 
-__all__ = ["Batch", "autotune_skin"]
+def with_cell_factor(pairs: list, factor: float) -> list:
+    """The pairs with each NeighborList's secondary cell list built on cells
+    of ``factor`` times its list radius r_l = cutoff + skin and a stencil of
+    half-width ceil(1 / factor) cells, so every pair within r_l stays in the
+    stencil (factor 1: 3^D cells of r_l; 0.5: 5^D cells of r_l / 2, a
+    tighter candidate volume; 1.5: 3^D larger cells, fewer and fuller)."""
+    if not factor > 0:
+        raise ValueError(f"cell factor must be positive, got {factor}")
+    sr = max(1, int(np.ceil(1.0 / factor - 1e-6)))
+    r = np.arange(-sr, sr + 1)
+    out = []
+    for st, sy in pairs:
+        col = sy.collider
+        lc = F32(F32(col.cutoff) + F32(col.skin))
+        mesh = np.meshgrid(*([r] * st.dim), indexing="ij")
+        mask = np.stack([m.ravel() for m in mesh], axis=1).astype(np.int64)
+        sec = DynamicCellList(neighbor_mask=mask, cell_size=float(F32(F32(factor) * lc)))
+        out.append((st, replace(sy, collider=replace(col, secondary_collider=sec))))
+    return out
+
+
+def _probe(b: Batch, warm_steps: int, probe_steps: int) -> tuple:
+    """(particle-steps/s, rebuilds) of ``probe_steps`` steps of ``b`` after
+    ``warm_steps`` untimed."""
+    b = b.step(warm_steps)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    b2 = b.step(probe_steps)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    st = b2.system.__dict__.get("_last_stats")
+    return b.n_particles * probe_steps / secs, int(st.rebuilds) if st is not None else -1
+
+
+def tune_launch(batch: Batch, tiles: Sequence[int] = (256, 128), modes: Sequence[int] = (0, 1, 2),
+                probe_steps: int = 200, warm_steps: int = 20) -> tuple:
+    """Tile width of the step kernel, then launch mode (how rebuild decisions
+    reach the kernels), probed on ``batch`` itself: neither changes a bit of
+    the result, so the batch is not repacked.  Sets the best values on
+    ``batch`` and returns ({"tile_width", "launch_mode"}, [(knob, value,
+    particle-steps/s, rebuilds), ...])."""
+    table = []
+    for knob, values in (("tile_width", tiles), ("launch_mode", modes)):
+        best_v, best_r = getattr(batch, knob), -1.0
+        for v in values:
+            setattr(batch, knob, int(v))
+            r, nb = _probe(batch, warm_steps, probe_steps)
+            table.append((knob, int(v), r, nb))
+            if r > best_r:
+                best_v, best_r = int(v), r
+        setattr(batch, knob, best_v)
+    return {"tile_width": batch.tile_width, "launch_mode": batch.launch_mode}, table
+
+
+def autotune(make_pairs: Callable[[float], list], skins: Sequence[float], cell_factors: Sequence[float] = (1.0, 0.5, 1.5),
+             tiles: Sequence[int] = (256, 128), modes: Sequence[int] = (0, 1, 2), probe_steps: int = 200,
+             warm_steps: int = 20, device="cuda") -> tuple:
+    """Autotuner of a batch: rebuild cadence (the skin), cell size of the list
+    build (``with_cell_factor``), then tile width and launch mode
+    (``tune_launch``), each later knob at the best values of the earlier
+    ones.  A probe packs ``make_pairs(skin)``, steps ``warm_steps`` untimed
+    and ``probe_steps`` timed.  Returns (config dict, [(knob, value,
+    particle-steps/s, rebuilds), ...]).  None of the knobs changes which
+    pairs touch; skin and cell size change the order of the sums, tile width
+    and launch mode nothing at all."""
+    table = []
+    cfg = {"skin": float(skins[0]), "cell_factor": float(cell_factors[0])}
+    for knob, values in (("skin", skins), ("cell_factor", cell_factors)):
+        best_v, best_r = cfg[knob], -1.0
+        for v in values:
+            c = dict(cfg)
+            c[knob] = float(v)
+            b = Batch.pack(with_cell_factor(make_pairs(c["skin"]), c["cell_factor"])).to(device)
+            r, nb = _probe(b, warm_steps, probe_steps)
+            del b
+            torch.cuda.empty_cache()
+            table.append((knob, float(v), r, nb))
+            if r > best_r:
+                best_v, best_r = float(v), r
+        cfg[knob] = best_v
+    if tiles or modes:
+        b = Batch.pack(with_cell_factor(make_pairs(cfg["skin"]), cfg["cell_factor"])).to(device)
+        lc, t2 = tune_launch(b, tiles or (b.tile_width,), modes or (b.launch_mode,), probe_steps, warm_steps)
+        cfg.update(lc)
+        table += t2
+        del b
+        torch.cuda.empty_cache()
+    return cfg, table
+
+
+__all__ = ["Batch", "autotune_skin", "autotune", "tune_launch", "with_cell_factor"]

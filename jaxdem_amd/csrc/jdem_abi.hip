@@ -498,6 +498,7 @@ struct SegRun {
   int32_t* seg_mode;  // what the current rebuild does to it (device)
   bool any_unbuilt;   // some system has no list yet
   int launch_mode;    // jdem_batch.launch_mode
+  int tile;           // jdem_batch.tile_width
 };
 
 // n steps of a NeighborList system on the resident working set.  With
@@ -815,6 +816,8 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   std::vector<int> cur_at, xc_at;
   int64_t nseg = n;  // steps of the current segment
 
+  // speculative step kernels (a batch in launch mode 2 while its rebuilds are rare)
+  bool spec = sg && sg->launch_mode == 2;
   auto launch_t = [&](int64_t t) -> int {
     if (!sg && sys->domain.kind == JDEM_DOMAIN_FREE && t == nseg - 1 && !free_done) {
       const int nparts = 64;
@@ -877,12 +880,13 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     a.flag_rd = t >= 1 ? flags + (t & 1) : nullptr;
     a.flag_wr = flags + ((t + 1) & 1);
     a.step_idx = (int32_t)t;
-    if (sg && sg->launch_mode != 2) {
+    if (sg && !spec) {
       // no speculation: the rebuild chain after this kernel acts on the gate
       a.flag_rd = nullptr;
       a.flag_wr = gate;
     }
     if (sg) {
+      a.block = sg->tile;
       a.seg = sg->seg;
       a.segp = sg->segp;
       a.seg_flag = sg->seg_flag;
@@ -896,74 +900,15 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     return JDEM_OK;
   };
 
-  bool rebuilt_here = false;
-  auto run_segment = [&](int64_t ns) -> int {
-  nseg = ns;
-  cur_at.assign((size_t)ns + 2, 0);
-  xc_at.assign((size_t)ns + 2, 0);
-  if (sg && sg->launch_mode == 2) {
-    // Speculative chunks, as for one system: each step kernel checks the
-    // word its predecessor wrote and does nothing once a system asked for a
-    // rebuild; the host reads the words once per chunk, runs the rebuild
-    // chain (its gate raised from the host) and resumes at the flagged step.
-    HIPCHK(ctx, launch_fill_i32(flags, 6, 0, st));
-    HIPCHK(ctx, ws_launch_seg_init_flags(sg->n_seg, sg->n_build, sg->seg_flag, st, gate));
-    int rc2 = launch_t(0);
-    if (rc2) return rc2;
-    if (sg->any_unbuilt) {  // systems without a list build before the first force
-      rc2 = rebuild();
-      if (rc2) return rc2;
-      HIPCHK(ctx, launch_fill_i32(flags, 2, 0, st));  // that chain also served any system kernel 0 flagged
-    }
-    int64_t next = 1, chunk_len = 32, last_rebuild_at = 0;
-    while (next <= ns) {
-      const int64_t cnt = std::min<int64_t>(chunk_len, ns - next + 1);
-      for (int64_t tt = next; tt < next + cnt; ++tt) {
-        rc2 = launch_t(tt);
-        if (rc2) return rc2;
-      }
-      rc2 = read_flags(ctx, 6);
-      if (rc2) return rc2;
-      S->host_syncs++;
-      const int32_t v = std::max(ctx->host_flags[0], ctx->host_flags[1]);
-      if (v == 0) {
-        next += cnt;
-        chunk_len = std::min<int64_t>(chunk_len * 2, 1024);
-        continue;
-      }
-      // kernels v.. of this chunk did no work; the data is where kernel v read it
-      if (v < next + cnt) {
-        cur = cur_at[v];
-        xc = xc_at[v];
-      }
-      HIPCHK(ctx, launch_fill_i32(gate, 1, 1, st));
-      rc2 = rebuild();
-      if (rc2) return rc2;
-      HIPCHK(ctx, launch_fill_i32(flags, 2, 0, st));
-      if (last_rebuild_at > 0) chunk_len = std::max<int64_t>(2, v - last_rebuild_at + 1);
-      last_rebuild_at = v;
-      next = v;
-    }
-    rc2 = read_flags(ctx, 6);
-    if (rc2) return rc2;
-    const int32_t ev = ctx->host_flags[5];
-    S->rebuilds += ev;
-    L.n_build_times += ev;
-    if (ev > 0) rebuilt_here = true;
-    return JDEM_OK;
-  }
-  if (sg) {
-    // step kernel, rebuild chain, step kernel, ...: one host read at the end
-    HIPCHK(ctx, launch_fill_i32(flags, 6, 0, st));
-    HIPCHK(ctx, ws_launch_seg_init_flags(sg->n_seg, sg->n_build, sg->seg_flag, st, gate));
-    auto step_and_chain = [&](int64_t t) -> int {
-      int rc3 = launch_t(t);
+  // Batch, gated chain: step kernel, rebuild chain, step kernel, ... from
+  // step t (the gate and the systems' flags clear), the chain after every
+  // step but the last.
+  auto gated_steps = [&](int64_t t, int64_t ns) -> int {
+    auto step_and_chain = [&](int64_t tt) -> int {
+      int rc3 = launch_t(tt);
       if (rc3) return rc3;
-      return t < ns ? rebuild() : JDEM_OK;
+      return tt < ns ? rebuild() : JDEM_OK;
     };
-    int64_t t = 0;
-    int rc2 = step_and_chain(t++);
-    if (rc2) return rc2;
     // The middle steps (force, integration and the next step's start, then
     // the gated rebuild chain) repeat with the same arguments every two
     // steps: a chunk of kGraphSteps of them is captured once in a hipGraph
@@ -1013,10 +958,93 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       if (el != hipSuccess) return fail(ctx, JDEM_EHIP, std::string("hipGraphLaunch: ") + hipGetErrorString(el));
     }
     for (; t <= ns; ++t) {
-      rc2 = step_and_chain(t);
-      if (rc2) return rc2;
+      const int rc3 = step_and_chain(t);
+      if (rc3) return rc3;
     }
-    rc2 = read_flags(ctx, 6);
+    return JDEM_OK;
+  };
+
+  bool rebuilt_here = false;
+  auto run_segment = [&](int64_t ns) -> int {
+  nseg = ns;
+  cur_at.assign((size_t)ns + 2, 0);
+  xc_at.assign((size_t)ns + 2, 0);
+  if (sg) {
+    // flags[5] counts the chain runs that rebuilt something
+    HIPCHK(ctx, launch_fill_i32(flags, 6, 0, st));
+    HIPCHK(ctx, ws_launch_seg_init_flags(sg->n_seg, sg->n_build, sg->seg_flag, st, gate));
+    int64_t next = 0;
+    spec = sg->launch_mode == 2;
+    if (spec) {
+      // Speculative chunks, as for one system: each step kernel checks the
+      // word its predecessor wrote and does nothing once a system asked for
+      // a rebuild; the host reads the words once per chunk, runs the rebuild
+      // chain (its gate raised from the host) and resumes at the flagged
+      // step.  Once rebuilds come close together (two intervals in a row
+      // of kSpecMin steps or fewer) the rest of the call runs the gated
+      // chain instead, which needs no host read.
+      constexpr int64_t kSpecMin = 8;
+      int rc2 = launch_t(0);
+      if (rc2) return rc2;
+      if (sg->any_unbuilt) {  // systems without a list build before the first force
+        rc2 = rebuild();
+        if (rc2) return rc2;
+        HIPCHK(ctx, launch_fill_i32(flags, 2, 0, st));  // that chain also served any system kernel 0 flagged
+      }
+      next = 1;
+      int64_t chunk_len = 32, last_rebuild_at = 0;
+      int close = 0;
+      while (next <= ns && spec) {
+        const int64_t cnt = std::min<int64_t>(chunk_len, ns - next + 1);
+        for (int64_t tt = next; tt < next + cnt; ++tt) {
+          rc2 = launch_t(tt);
+          if (rc2) return rc2;
+        }
+        rc2 = read_flags(ctx, 2);
+        if (rc2) return rc2;
+        S->host_syncs++;
+        const int32_t v = std::max(ctx->host_flags[0], ctx->host_flags[1]);
+        if (v == 0) {
+          next += cnt;
+          chunk_len = std::min<int64_t>(chunk_len * 2, 1024);
+          continue;
+        }
+        // kernels v.. of this chunk did no work; the data is where kernel v read it
+        if (v < next + cnt) {
+          cur = cur_at[v];
+          xc = xc_at[v];
+        }
+        HIPCHK(ctx, launch_fill_i32(gate, 1, 1, st));
+        rc2 = rebuild();
+        if (rc2) return rc2;
+        HIPCHK(ctx, launch_fill_i32(flags, 2, 0, st));
+        if (last_rebuild_at > 0) {
+          close = v - last_rebuild_at <= kSpecMin ? close + 1 : 0;
+          chunk_len = std::max<int64_t>(2, v - last_rebuild_at + 1);
+        }
+        last_rebuild_at = v;
+        next = v;
+        if (close >= 2) spec = false;  // the chain ran: gate and system flags are clear
+      }
+    }
+    if (next <= ns) {
+      if (next == 0) {
+        // step 0 (integration start, displacement check) then the chain
+        // builds the systems without a list and any that step 0 flagged
+        int rc2 = launch_t(0);
+        if (rc2) return rc2;
+        if (ns > 0) {
+          rc2 = rebuild();
+          if (rc2) return rc2;
+        }
+        next = 1;
+      }
+      if (next <= ns) {
+        const int rc2 = gated_steps(next, ns);
+        if (rc2) return rc2;
+      }
+    }
+    const int rc2 = read_flags(ctx, 6);
     if (rc2) return rc2;
     S->host_syncs++;
     const int32_t ev = ctx->host_flags[5];
@@ -1573,6 +1601,8 @@ int jdem_batch_step(jdem_ctx* ctx, jdem_system* sys, const jdem_batch* batch, in
     return fail(ctx, JDEM_EINVAL, "incomplete batch description");
   if (n < 0) return fail(ctx, JDEM_EINVAL, "negative step count");
   if (batch->launch_mode < 0 || batch->launch_mode > 2) return fail(ctx, JDEM_EINVAL, "launch_mode must be 0, 1 or 2");
+  if (batch->tile_width != 0 && batch->tile_width != 128 && batch->tile_width != 256)
+    return fail(ctx, JDEM_EINVAL, "tile_width must be 128 or 256");
   if (sys->collider_kind != JDEM_COLLIDER_NEIGHBOR_LIST || sys->nlist.secondary != JDEM_COLLIDER_CELL_LIST)
     return fail(ctx, JDEM_EUNSUPPORTED, "batches run the NeighborList collider over a CellList");
   const int B = batch->n_systems;
@@ -1632,6 +1662,7 @@ int jdem_batch_step(jdem_ctx* ctx, jdem_system* sys, const jdem_batch* batch, in
   sg.seg_mode = dbuild + 2 * B;
   sg.any_unbuilt = any_unbuilt;
   sg.launch_mode = batch->launch_mode;
+  sg.tile = batch->tile_width == 128 ? 128 : 256;
   jdem_step_stats local;
   jdem_step_stats* S = stats ? stats : &local;
   std::memset(S, 0, sizeof(*S));

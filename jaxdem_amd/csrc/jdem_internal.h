@@ -449,6 +449,7 @@ struct WS {
   const int32_t* seg;
   const SegP* segp;
   int32_t* seg_flag;
+  int block;  // workgroup size of a batch's step kernel (256 or 128)
 };
 
 hipError_t ws_launch_in(const InOut& io, bool with_nl, hipStream_t s);

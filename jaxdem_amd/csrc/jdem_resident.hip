@@ -851,12 +851,14 @@ JD void wave_sync() {
 
 // MW = 64-slot words of a row the kernel handles (1 when M <= 64, else 4):
 // the per-lane slot masks then take 2 * MW registers
-template <int D, int LAW, int PM, int MW>
-__global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_waves<D, LAW, PM, MW>(), 8))) k_ws_step(WS a) {
-  __shared__ int pdesc[kBlock / 64][kPairCap];
-  __shared__ int pjx[kBlock / 64][kPairCap];
-  __shared__ int pstash[kBlock / 64][kStash][64];
-  __shared__ float4 pres[kBlock / 64][3][64];
+// BS = workgroup size (the tile width: 256, or 128 for batches when the
+// autotuner prefers it)
+template <int D, int LAW, int PM, int MW, int BS = kBlock>
+__global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(ws_waves<D, LAW, PM, MW>(), 8))) k_ws_step(WS a) {
+  __shared__ int pdesc[BS / 64][kPairCap];
+  __shared__ int pjx[BS / 64][kPairCap];
+  __shared__ int pstash[BS / 64][kStash][64];
+  __shared__ float4 pres[BS / 64][3][64];
   if (a.flag_rd && *a.flag_rd) {
     if (a.flag_wr) *a.flag_wr = *a.flag_rd;
     return;
@@ -1155,7 +1157,9 @@ __global__ void __launch_bounds__(kBlock) __attribute__((amdgpu_waves_per_eu(ws_
 template <int D, int LAW, int MW>
 void launch_ws_mw(const WS& a, bool per, hipStream_t s) {
   dim3 g(nblocks(a.n)), b(kBlock);
-  if (a.segp)
+  if (a.segp && a.block == 128)
+    hipLaunchKernelGGL((k_ws_step<D, LAW, 2, MW, 128>), dim3(nblocks(a.n, 128)), dim3(128), 0, s, a);
+  else if (a.segp)
     hipLaunchKernelGGL((k_ws_step<D, LAW, 2, MW>), g, b, 0, s, a);
   else if (per)
     hipLaunchKernelGGL((k_ws_step<D, LAW, 1, MW>), g, b, 0, s, a);

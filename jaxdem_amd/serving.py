@@ -103,8 +103,11 @@ class Server:
         """Serve ``requests`` (their ``arrival`` times are seconds after the
         start).  With ``realtime`` the server waits for arrivals on the wall
         clock; otherwise every request is present from the start.  Returns
-        the requests with their results and timestamps."""
-        pending = sorted(requests, key=lambda r: (r.arrival, r.rid))
+        copies of the requests with their results and timestamps (the
+        requests passed in are left as they were, so they can be served
+        again)."""
+        pending = sorted((Request(r.rid, r.state, r.system, r.steps, r.arrival) for r in requests),
+                         key=lambda r: (r.arrival, r.rid))
         queue: list = []  # arrived, not admitted yet, in arrival order
         active: list = []  # in the batch, in batch order
         batch: Batch | None = None

@@ -765,9 +765,11 @@ def test_batch_graph_replays_equal_eager(device):
 def test_batch_launch_modes_equal(device):
     """The three ways rebuild decisions reach the kernels -- a gated chain
     after every step with captured graphs, the same chain launched step by
-    step, speculative chunks with a host read per chunk -- give the same
-    batch bit for bit, with rebuilds inside the run and a system joining
-    without a list."""
+    step, speculative chunks with a host read per chunk (handing over to the
+    gated chain once rebuilds come close together, as they do at skin 0.02)
+    -- give the same batch bit for bit, with rebuilds inside the run and a
+    system joining without a list; so do speculative chunks that never hand
+    over (skin 0.3, rare rebuilds)."""
     print("[batch] launch modes", flush=True)
     ws = [W.disks_hertz_friction(N, seed=60 + k, phi=p, skin=0.02) for k, (N, p) in
           enumerate([(900, 0.7), (256, 0.35), (3000, 0.6)])]
@@ -784,6 +786,53 @@ def test_batch_launch_modes_equal(device):
         assert torch.equal(o.system.collider.neighbor_list, out[0].system.collider.neighbor_list)
         assert o.n_build == out[0].n_build
     assert out[2].system.__dict__["_last_stats"].host_syncs > 1
+    assert max(out[0].n_build) >= 5
+    ws = [W.disks_hertz_friction(N, seed=64 + k, phi=p, skin=0.3) for k, (N, p) in enumerate([(900, 0.7), (2000, 0.5)])]
+    pairs = [W.make_gpu_system(w, device=device) for w in ws]
+    out = []
+    for mode in (1, 2):
+        b = jd.Batch.pack(pairs)
+        b.launch_mode = mode
+        out.append(b.step(150))
+    for f in ("pos_c", "vel", "ang_vel", "force", "torque"):
+        assert_bitwise(f"rare rebuilds {f}", getattr(out[1].state, f).cpu().numpy(), getattr(out[0].state, f).cpu().numpy())
+    assert torch.equal(out[1].system.collider.neighbor_list, out[0].system.collider.neighbor_list)
+    assert out[1].n_build == out[0].n_build
+
+
+def test_batch_tile_width_and_cell_factor(device):
+    """The autotuner's knobs: the step kernel's tile width (128 or 256
+    particles per workgroup) changes nothing, bit for bit; cells of half the
+    list radius (stencil half-width 2) and of 1.5 list radii give lists and
+    states equal to the oracle's with the same cell lists."""
+    from jaxdem_amd.batching import with_cell_factor
+
+    print("[batch] tile width / cell factor", flush=True)
+    ws = [W.disks_hertz_friction(N, seed=70 + k, phi=p, skin=0.02) for k, (N, p) in
+          enumerate([(900, 0.7), (256, 0.35), (3000, 0.6)])]
+    pairs = [W.make_gpu_system(w, device=device) for w in ws]
+    out = []
+    for tile in (256, 128):
+        b = jd.Batch.pack(pairs)
+        b.tile_width = tile
+        out.append(b.step(120))
+    for f in ("pos_c", "vel", "ang_vel", "force", "torque"):
+        assert_bitwise(f"tile 128 {f}", getattr(out[1].state, f).cpu().numpy(), getattr(out[0].state, f).cpu().numpy())
+    assert torch.equal(out[1].system.collider.neighbor_list, out[0].system.collider.neighbor_list)
+    with pytest.raises(Exception, match="tile_width must be 128 or 256"):
+        b = jd.Batch.pack(pairs)
+        b.tile_width = 64
+        b.step(1)
+    for factor in (0.5, 1.5):
+        fp = with_cell_factor(pairs, factor)
+        assert fp[0][1].collider.secondary_collider.search_range == (2 if factor < 1 else 1)
+        res = jd.Batch.pack(fp).step(80).unpack()
+        for k, ((st, sy), (sb, yb)) in enumerate(zip(fp, res)):
+            print(f"[batch] cell factor {factor} system {k}", flush=True)
+            ost, nls = _oracle_system(st, sy, 80)
+            _compare_states(sb, ost, f"factor {factor} system {k}")
+            assert yb.collider.n_build_times == nls.n_build
+            assert_bitwise("nl", yb.collider.neighbor_list.cpu().numpy().astype(np.int64), nls.nl)
 
 
 def test_batch_split_steps(device):
