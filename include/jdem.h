@@ -50,15 +50,18 @@ enum {
 /* Domain kinds: domains/free.py, domains/periodic.py, domains/reflect.py. */
 enum { JDEM_DOMAIN_FREE = 0, JDEM_DOMAIN_PERIODIC = 1, JDEM_DOMAIN_REFLECT = 2 };
 
-/* Pair laws: forces/spring.py, forces/hertz.py, forces/cundall_strack.py and
- * two history-carrying laws with no reference counterpart. */
+/* Pair laws: forces/spring.py, forces/hertz.py, forces/cundall_strack.py,
+ * two history-carrying laws with no reference counterpart, and ROUTED: the
+ * composite laws ForceRouter (forces/router.py:26-339) and LawCombiner
+ * (forces/law_combiner.py:22-191). */
 enum {
   JDEM_LAW_NONE = 0,
   JDEM_LAW_SPRING = 1,
   JDEM_LAW_HERTZ = 2,
   JDEM_LAW_CUNDALL_STRACK = 3,
   JDEM_LAW_SPRING_FRICTION = 4,
-  JDEM_LAW_HERTZ_FRICTION = 5
+  JDEM_LAW_HERTZ_FRICTION = 5,
+  JDEM_LAW_ROUTED = 6
 };
 
 /* Integrators: integrators/velocity_verlet.py, direct_euler.py,
@@ -121,6 +124,19 @@ typedef struct jdem_domain {
  *                    t0 = beta(e_eff)          (cundall_strack.py:123-193)
  *   SPRING_FRICTION: t0 = k_n, t1 = k_t, t2 = mu_eff; p0 = 2*damping ratio
  *   HERTZ_FRICTION:  t0 = (4/3) E*, t1 = 8 G*, t2 = mu_eff; p0 = 2*damping
+ *   ROUTED:          the pair (i, j) takes the elementary laws
+ *                    route_terms[route_off[p] .. route_off[p + 1]) with
+ *                    p = species[i] * n_species + species[j] (p = 0 for
+ *                    every pair when n_species <= 1), in order: the first
+ *                    term's force, torque and energy, plus each later
+ *                    term's (a LawCombiner entry starts with a NONE term,
+ *                    its zeros, law_combiner.py:72-86; an empty entry is a
+ *                    single NONE term).  Terms are kinds NONE .. HERTZ_FRICTION
+ *                    with their own tables; an entry holds at most one
+ *                    history law, whose tangential spring the pair carries.
+ *                    route_off (n_species^2 + 1, or 2) and route_terms
+ *                    (n_terms) are host arrays read at each call; species
+ *                    is the (N,) device array of State.species_id.
  */
 typedef struct jdem_law {
   int32_t kind;
@@ -134,6 +150,12 @@ typedef struct jdem_law {
   const float* m3;
   float p0;
   float p1;
+  /* ROUTED only */
+  int32_t n_species;
+  int32_t n_terms;
+  const int32_t* route_off;
+  const struct jdem_law* route_terms;
+  const int32_t* species;
 } jdem_law;
 
 /* Cell grid of _grid_params (_partition.py:55-99), computed by the host in

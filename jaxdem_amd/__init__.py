@@ -15,8 +15,10 @@ from .forces import (
     CundallStrackForce,
     ForceManager,
     ForceModel,
+    ForceRouter,
     HertzFrictionForce,
     HertzianForce,
+    LawCombiner,
     SpringForce,
     SpringFrictionForce,
 )
@@ -55,6 +57,8 @@ __all__ = [
     "CundallStrackForce",
     "SpringFrictionForce",
     "HertzFrictionForce",
+    "LawCombiner",
+    "ForceRouter",
     "LinearIntegrator",
     "RotationIntegrator",
     "VelocityVerlet",

@@ -20,7 +20,7 @@ JDEM_OK = 0
 ERRORS = {-1: "JDEM_EINVAL", -2: "JDEM_EHIP", -3: "JDEM_ENOMEM", -4: "JDEM_EUNSUPPORTED"}
 
 DOMAIN_FREE, DOMAIN_PERIODIC, DOMAIN_REFLECT = 0, 1, 2
-LAW_NONE, LAW_SPRING, LAW_HERTZ, LAW_CUNDALL_STRACK, LAW_SPRING_FRICTION, LAW_HERTZ_FRICTION = range(6)
+LAW_NONE, LAW_SPRING, LAW_HERTZ, LAW_CUNDALL_STRACK, LAW_SPRING_FRICTION, LAW_HERTZ_FRICTION, LAW_ROUTED = range(7)
 LIN_NONE, LIN_VERLET, LIN_EULER = 0, 1, 2
 ROT_NONE, ROT_VERLET_SPIRAL, ROT_SPIRAL = 0, 1, 2
 COLLIDER_NONE, COLLIDER_NAIVE, COLLIDER_CELL_LIST, COLLIDER_NEIGHBOR_LIST = 0, 1, 2, 3
@@ -66,19 +66,27 @@ class JdemDomain(ctypes.Structure):
 
 
 class JdemLaw(ctypes.Structure):
-    _fields_ = [
-        ("kind", c_int32),
-        ("n_mat", c_int32),
-        ("t0", c_void_p),
-        ("t1", c_void_p),
-        ("t2", c_void_p),
-        ("m0", c_void_p),
-        ("m1", c_void_p),
-        ("m2", c_void_p),
-        ("m3", c_void_p),
-        ("p0", c_float),
-        ("p1", c_float),
-    ]
+    pass
+
+
+JdemLaw._fields_ = [
+    ("kind", c_int32),
+    ("n_mat", c_int32),
+    ("t0", c_void_p),
+    ("t1", c_void_p),
+    ("t2", c_void_p),
+    ("m0", c_void_p),
+    ("m1", c_void_p),
+    ("m2", c_void_p),
+    ("m3", c_void_p),
+    ("p0", c_float),
+    ("p1", c_float),
+    ("n_species", c_int32),
+    ("n_terms", c_int32),
+    ("route_off", c_void_p),
+    ("route_terms", ctypes.POINTER(JdemLaw)),
+    ("species", c_void_p),
+]
 
 
 class JdemGrid(ctypes.Structure):

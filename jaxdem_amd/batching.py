@@ -142,6 +142,7 @@ class Batch:
                 and sy.interact_same_bond_id == y0.interact_same_bond_id
                 and np.array_equal(np.asarray(sy.force_manager.gravity), np.asarray(y0.force_manager.gravity))
                 and (col.history is None) == (y0.collider.history is None)
+                and (sy.force_model.kind != L.LAW_ROUTED or sy.force_model == y0.force_model)
             )
             if not same:
                 raise ValueError("batched systems must share the force model, integrators and gravity")

@@ -21,7 +21,8 @@ std::vector<Buf*> jdem_ctx::bufs() {
   std::vector<Buf*> all = {&pos2,    &vel2,   &ang2,     &keys[0], &keys[1], &vals[0], &vals[1], &hist,
                            &cstart,  &cend,   &spos,     &nl_new,  &xi_new,  &live_new, &flags,  &red,
                            &nl_copy, &xi_copy, &live_copy, &wF,     &wT,      &wOld,     &wWpos, &wSrc,
-                           &wInv,    &wKeys,  &fireV,    &fireDev, &segp,  &segid, &segflag, &wXiRank};
+                           &wInv,    &wKeys,  &fireV,    &fireDev, &segp,  &segid, &segflag, &wXiRank,
+                           &route};
   for (int k = 0; k < 2; ++k)
     for (Buf* b : {&wP[k], &wV[k], &wW[k], &wQ[k], &wI[k], &wOrig[k], &wNl[k], &wXi[k], &wLive[k]}) all.push_back(b);
   return all;
@@ -145,8 +146,9 @@ int fail(jdem_ctx* ctx, int code, const std::string& msg) {
       return fail((ctx), JDEM_ENOMEM, "scratch allocation failed: " #buf);        \
   } while (0)
 
-LawP law_of(const jdem_law& l) {
+LawP law_of_elem(const jdem_law& l) {
   LawP p;
+  std::memset(&p, 0, sizeof(p));
   p.kind = l.kind;
   p.n_mat = l.n_mat;
   p.t0 = l.t0;
@@ -158,6 +160,76 @@ LawP law_of(const jdem_law& l) {
   p.m3 = l.m3;
   p.p0 = l.p0;
   p.p1 = l.p1;
+  return p;
+}
+
+bool is_history_kind(int k) { return k == JDEM_LAW_SPRING_FRICTION || k == JDEM_LAW_HERTZ_FRICTION; }
+
+// Does the law carry a tangential spring per pair?
+bool law_has_history(const jdem_law& l) {
+  if (l.kind != JDEM_LAW_ROUTED) return is_history_kind(l.kind);
+  for (int k = 0; k < l.n_terms; ++k)
+    if (is_history_kind(l.route_terms[k].kind)) return true;
+  return false;
+}
+
+// Entries of a routed law's species table: n_species^2, or 1 without species.
+int route_entries(const jdem_law& l) { return l.n_species > 1 ? l.n_species * l.n_species : 1; }
+
+// Checks a routed law and copies its terms and species offsets to the
+// context's device table when they differ from the last copy.
+int prepare_route(jdem_ctx* ctx, const jdem_law& l, int64_t n) {
+  if (l.n_species < 0 || l.n_species > kMatMask) return fail(ctx, JDEM_EINVAL, "routed law: n_species out of range");
+  if (l.n_terms < 1 || !l.route_off || !l.route_terms) return fail(ctx, JDEM_EINVAL, "routed law without terms");
+  if (l.n_species > 1 && n > 0 && !l.species) return fail(ctx, JDEM_EINVAL, "routed law: species array missing");
+  const int E = route_entries(l);
+  if (l.route_off[0] != 0 || l.route_off[E] != l.n_terms) return fail(ctx, JDEM_EINVAL, "routed law: bad route_off");
+  for (int e = 0; e < E; ++e) {
+    if (l.route_off[e + 1] <= l.route_off[e]) return fail(ctx, JDEM_EINVAL, "routed law: an entry has no term");
+    int n_hist = 0;
+    for (int k = l.route_off[e]; k < l.route_off[e + 1]; ++k) {
+      const jdem_law& t = l.route_terms[k];
+      if (t.kind < JDEM_LAW_NONE || t.kind > JDEM_LAW_HERTZ_FRICTION)
+        return fail(ctx, JDEM_EINVAL, "routed law: a term is not an elementary law");
+      if (t.kind != JDEM_LAW_NONE) {
+        if (t.n_mat != l.n_mat) return fail(ctx, JDEM_EINVAL, "routed law: a term's n_mat differs from the law's");
+        if (!t.t0 || (t.kind == JDEM_LAW_CUNDALL_STRACK && (!t.m0 || !t.m1 || !t.m2 || !t.m3)) ||
+            (is_history_kind(t.kind) && (!t.t1 || !t.t2)))
+          return fail(ctx, JDEM_EINVAL, "routed law: a term's tables are missing");
+      }
+      n_hist += is_history_kind(t.kind);
+    }
+    if (n_hist > 1) return fail(ctx, JDEM_EUNSUPPORTED, "routed law: an entry with more than one history law");
+  }
+  // device table: the terms (LawP), then the offsets (int32)
+  const size_t tb = sizeof(LawP) * (size_t)l.n_terms;
+  std::vector<char> host(tb + sizeof(int32_t) * (size_t)(E + 1));
+  LawP* terms = reinterpret_cast<LawP*>(host.data());
+  for (int k = 0; k < l.n_terms; ++k) terms[k] = law_of_elem(l.route_terms[k]);
+  std::memcpy(host.data() + tb, l.route_off, sizeof(int32_t) * (size_t)(E + 1));
+  if (host != ctx->route_host || !ctx->route.p) {
+    ENSURE(ctx, ctx->route, host.size());
+    // the stream's earlier kernels may still read the previous table
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+    HIPCHK(ctx, hipMemcpy(ctx->route.p, host.data(), host.size(), hipMemcpyHostToDevice));
+    ctx->route_host.swap(host);
+  }
+  return JDEM_OK;
+}
+
+// Kernel-side law of a call: a routed law points at the context's device
+// table (prepare_route ran in check_system) and takes the kernel kind with
+// or without the history term.
+LawP law_of(jdem_ctx* ctx, const jdem_law& l) {
+  LawP p = law_of_elem(l);
+  if (l.kind == JDEM_LAW_ROUTED) {
+    p.kind = law_has_history(l) ? kLawRoutedHist : kLawRouted;
+    p.n_species = l.n_species;
+    p.route_terms = reinterpret_cast<const LawP*>(ctx->route.p);
+    p.route_off = reinterpret_cast<const int32_t*>(static_cast<const char*>(ctx->route.p) +
+                                                   sizeof(LawP) * (size_t)l.n_terms);
+    p.species = l.n_species > 1 ? l.species : nullptr;
+  }
   return p;
 }
 
@@ -174,15 +246,20 @@ int check_system(jdem_ctx* ctx, const jdem_system* sys, bool batch = false, bool
   if (sys->law.n_mat > 1 && !s.mat_id) return fail(ctx, JDEM_EINVAL, "n_mat > 1 but mat_id is null");
   const int lk = sys->law.kind;
   if (sys->collider_kind != JDEM_COLLIDER_NONE &&
-      !(lk >= JDEM_LAW_SPRING && lk <= JDEM_LAW_HERTZ_FRICTION))
+      !(lk >= JDEM_LAW_SPRING && lk <= JDEM_LAW_ROUTED))
     return fail(ctx, JDEM_EINVAL, "unknown law kind");
-  if (sys->law.n_mat < 1 || !sys->law.t0) {
+  if (sys->law.n_mat > kMatMask) return fail(ctx, JDEM_EINVAL, "more than 32767 materials");
+  if (lk == JDEM_LAW_ROUTED) {
+    if (sys->law.n_mat < 1) return fail(ctx, JDEM_EINVAL, "routed law: n_mat < 1");
+    const int rc = prepare_route(ctx, sys->law, s.n);
+    if (rc) return rc;
+  } else if (sys->law.n_mat < 1 || !sys->law.t0) {
     if (sys->collider_kind != JDEM_COLLIDER_NONE && lk != JDEM_LAW_CUNDALL_STRACK)
       return fail(ctx, JDEM_EINVAL, "law table t0 missing");
   }
   if (sys->domain.kind < JDEM_DOMAIN_FREE || sys->domain.kind > JDEM_DOMAIN_REFLECT)
     return fail(ctx, JDEM_EINVAL, "unknown domain kind");
-  const bool hist_law = lk == JDEM_LAW_SPRING_FRICTION || lk == JDEM_LAW_HERTZ_FRICTION;
+  const bool hist_law = law_has_history(sys->law);
   if (hist_law && sys->collider_kind != JDEM_COLLIDER_NONE) {
     if (sys->collider_kind != JDEM_COLLIDER_NEIGHBOR_LIST)
       return fail(ctx, JDEM_EINVAL, "history laws require the NeighborList collider");
@@ -215,7 +292,7 @@ int check_system(jdem_ctx* ctx, const jdem_system* sys, bool batch = false, bool
   return JDEM_OK;
 }
 
-StepArgs base_args(const jdem_system* sys) {
+StepArgs base_args(jdem_ctx* ctx, const jdem_system* sys) {
   StepArgs a;
   std::memset(&a, 0, sizeof(a));
   const jdem_state& s = sys->state;
@@ -245,7 +322,7 @@ StepArgs base_args(const jdem_system* sys) {
               a.ext_fc || a.ext_t)
                  ? 1
                  : 0;
-  a.law = law_of(sys->law);
+  a.law = law_of(ctx, sys->law);
   a.dom = dom_of(sys->domain);
   a.dt = sys->dt;
   a.lin = sys->lin_kind;
@@ -589,6 +666,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   io.mass = s.mass;
   io.inertia = s.inertia;
   io.mat_id = sys->law.n_mat > 1 ? s.mat_id : nullptr;
+  io.species = sys->law.kind == JDEM_LAW_ROUTED && sys->law.n_species > 1 ? sys->law.species : nullptr;
   io.fixed = s.fixed;
   io.nl_user = L.nl;
   io.old_user = L.old_pos;
@@ -843,7 +921,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     a.n = N;
     a.M = M;
     a.W64 = W64;
-    a.law = law_of(sys->law);
+    a.law = law_of(ctx, sys->law);
     a.dom = dom;
     a.dt = sys->dt;
     a.lin = sys->lin_kind;
@@ -892,7 +970,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       a.seg_flag = sg->seg_flag;
     }
     const int sp = span_begin(ctx, 0, t);
-    hipError_t e = ws_launch_step(a, D, sys->law.kind, st);
+    hipError_t e = ws_launch_step(a, D, a.law.kind, st);
     span_end(ctx, sp);
     if (e != hipSuccess) return fail(ctx, JDEM_EHIP, std::string("resident step kernel: ") + hipGetErrorString(e));
     cur = 1 - cur;
@@ -1451,12 +1529,12 @@ int jdem_compute_force(jdem_ctx* ctx, jdem_system* sys) {
     rc = nl_check_and_rebuild(ctx, sys, &rebuilt);
     if (rc) return rc;
   }
-  StepArgs a = base_args(sys);
+  StepArgs a = base_args(ctx, sys);
   rc = prepare_source(ctx, sys, a, sys->state.pos);
   if (rc) return rc;
   a.do_force = 1;
   a.write_kin = 0;
-  HIPCHK(ctx, launch_step(a, sys->state.dim, sys->law.kind, src_of(sys->collider_kind), ctx->stream));
+  HIPCHK(ctx, launch_step(a, sys->state.dim, a.law.kind, src_of(sys->collider_kind), ctx->stream));
   return JDEM_OK;
 }
 
@@ -1469,7 +1547,7 @@ static int reduce_f64(jdem_ctx* ctx, const jdem_system* sys, int ncomp, double* 
   double* per = (double*)ctx->red.p;
   double* part = per + (size_t)(n > 0 ? n : 1) * ncomp;
   double* out = part + (size_t)nparts * ncomp;
-  StepArgs a = base_args(sys);
+  StepArgs a = base_args(ctx, sys);
   jdem_system* ms = const_cast<jdem_system*>(sys);
   if (by_collider) {
     int rc = prepare_source(ctx, ms, a, s.pos);
@@ -1478,7 +1556,7 @@ static int reduce_f64(jdem_ctx* ctx, const jdem_system* sys, int ncomp, double* 
   a.e_out = per;
   if (n > 0) {
     if (by_collider) {
-      HIPCHK(ctx, launch(a, s.dim, sys->law.kind, src_of(sys->collider_kind), ctx->stream));
+      HIPCHK(ctx, launch(a, s.dim, a.law.kind, src_of(sys->collider_kind), ctx->stream));
     } else {
       HIPCHK(ctx, launch_kinetic(a, s.dim, ctx->stream));
     }
@@ -1588,7 +1666,7 @@ int jdem_step_stage(jdem_ctx* ctx, jdem_system* sys, int32_t stage) {
     }
     return JDEM_OK;
   }
-  StepArgs a = base_args(sys);
+  StepArgs a = base_args(ctx, sys);
   HIPCHK(ctx, launch_stage(a, sys->state.dim, stage, ctx->stream));
   return JDEM_OK;
 }
@@ -1683,8 +1761,7 @@ int jdem_minimize(jdem_ctx* ctx, jdem_system* sys, const jdem_fire* fire, int64_
   if (rc) return rc;
   if (!fire) return fail(ctx, JDEM_EINVAL, "null FIRE arguments");
   if (max_steps < 0) return fail(ctx, JDEM_EINVAL, "max_steps < 0");
-  const bool hist_law = sys->law.kind == JDEM_LAW_SPRING_FRICTION || sys->law.kind == JDEM_LAW_HERTZ_FRICTION;
-  if (hist_law) return fail(ctx, JDEM_EUNSUPPORTED, "minimize with a history law");
+  if (law_has_history(sys->law)) return fail(ctx, JDEM_EUNSUPPORTED, "minimize with a history law");
   if (sys->ext_force || sys->ext_force_com || sys->ext_torque)
     return fail(ctx, JDEM_EUNSUPPORTED, "minimize with external force arrays");
   (void)hipSetDevice(ctx->device);
@@ -1743,7 +1820,7 @@ int jdem_minimize(jdem_ctx* ctx, jdem_system* sys, const jdem_fire* fire, int64_
       int r = nl_check_and_rebuild(ctx, sys, &rebuilt);
       if (r) return r;
     }
-    StepArgs a = base_args(sys);
+    StepArgs a = base_args(ctx, sys);
     if (sys->collider_kind == JDEM_COLLIDER_NONE) {
       HIPCHK(ctx, hipMemsetAsync(s.force, 0, 4 * (size_t)N * D, st));
       HIPCHK(ctx, hipMemsetAsync(s.torque, 0, 4 * (size_t)N * A, st));
@@ -1751,7 +1828,7 @@ int jdem_minimize(jdem_ctx* ctx, jdem_system* sys, const jdem_fire* fire, int64_
       int r = prepare_source(ctx, sys, a, s.pos);
       if (r) return r;
       a.do_force = 1;
-      HIPCHK(ctx, launch_step(a, D, sys->law.kind, src_of(sys->collider_kind), st));
+      HIPCHK(ctx, launch_step(a, D, a.law.kind, src_of(sys->collider_kind), st));
     }
     if (N > 0) HIPCHK(ctx, launch_stage(a, D, JDEM_STAGE_FORCE_MANAGER, st));
     HIPCHK(ctx, hipMemsetAsync(&f.st->max_grad_bits, 0, sizeof(unsigned), st));
@@ -1760,7 +1837,7 @@ int jdem_minimize(jdem_ctx* ctx, jdem_system* sys, const jdem_fire* fire, int64_
     if (N > 0) {
       if (sys->collider_kind != JDEM_COLLIDER_NONE) {
         a.e_out = per;
-        HIPCHK(ctx, launch_energy(a, D, sys->law.kind, src_of(sys->collider_kind), st));
+        HIPCHK(ctx, launch_energy(a, D, a.law.kind, src_of(sys->collider_kind), st));
       } else {
         HIPCHK(ctx, hipMemsetAsync(per, 0, 8 * (size_t)N, st));
       }
@@ -2090,7 +2167,7 @@ int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   float fbox[3] = {0, 0, 0}, fanc[3] = {0, 0, 0};
 
   auto launch_t = [&](int64_t t) -> int {
-    StepArgs a = base_args(sys);
+    StepArgs a = base_args(ctx, sys);
     const int r = rd(t), w = wr(t);
     a.pos_in = P[r];
     a.vel_in = V[r];
@@ -2117,7 +2194,7 @@ int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       span_end(ctx, sp);
     }
     int src_eff = src;
-    int law_eff = sys->law.kind;
+    int law_eff = a.law.kind;
     if (sys->collider_kind == JDEM_COLLIDER_NONE) {
       a.force_zero = 1;  // the "" collider (colliders/__init__.py:218-222)
       src_eff = SRC_NAIVE;

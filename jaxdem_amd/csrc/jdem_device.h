@@ -240,7 +240,21 @@ struct LawP {
   int kind, n_mat;
   const float *t0, *t1, *t2, *m0, *m1, *m2, *m3;
   float p0, p1;
+  // routed laws: the terms of ordered species pair p are
+  // route_terms[route_off[p] .. route_off[p + 1]) (device arrays)
+  int n_species;
+  const int32_t* route_off;
+  const LawP* route_terms;
+  const int32_t* species;
 };
+
+// Kernel kinds of a routed law (jdem_law kind ROUTED): without and with a
+// history-carrying term.
+constexpr int kLawRouted = JDEM_LAW_ROUTED;
+constexpr int kLawRoutedHist = JDEM_LAW_ROUTED + 1;
+// Material and species ids share the per-particle word of the resident
+// layout: fixed | mat << 1 | species << 16.
+constexpr int kMatMask = 0x7fff;
 
 // Data of one side of a pair.
 template <int D>
@@ -248,6 +262,7 @@ struct Side {
   Vec<D> x;
   float r;
   int mat;
+  int sp;  // species (routed laws)
   Vec<D> v;
   Ang<D> w;
   float m;
@@ -255,10 +270,12 @@ struct Side {
 
 template <int LAW>
 struct LawTraits {
+  static constexpr bool routed = (LAW == kLawRouted || LAW == kLawRoutedHist);
   // Does the law read velocities / masses of the partner?
   static constexpr bool kin = (LAW == JDEM_LAW_CUNDALL_STRACK || LAW == JDEM_LAW_SPRING_FRICTION ||
-                               LAW == JDEM_LAW_HERTZ_FRICTION);
-  static constexpr bool hist = (LAW == JDEM_LAW_SPRING_FRICTION || LAW == JDEM_LAW_HERTZ_FRICTION);
+                               LAW == JDEM_LAW_HERTZ_FRICTION || routed);
+  static constexpr bool hist = (LAW == JDEM_LAW_SPRING_FRICTION || LAW == JDEM_LAW_HERTZ_FRICTION ||
+                                LAW == kLawRoutedHist);
   static constexpr bool torque = kin;
 };
 
@@ -410,6 +427,66 @@ JD PairOut<D> law_friction(const LawP& L, const Side<D>& a, const Side<D>& b, Ve
   return o;
 }
 
+// Routed laws (router.py:70-160, law_combiner.py:60-93): the terms of the
+// pair's species entry, in order; the first gives the force and torque, each
+// later one is added to them.  A LawCombiner entry starts with a NONE term
+// (the combiner's zeros); the entry's history term, if any, gives the
+// tangential spring and the contact flag.
+JD int route_of(const LawP& L, int sa, int sb) { return L.n_species > 1 ? sa * L.n_species + sb : 0; }
+
+template <int D, bool H>
+JD PairOut<D> law_routed(const LawP& L, const Side<D>& a, const Side<D>& b, Vec<D> rij, float d_sq, Vec<D> xi_old,
+                         float dt) {
+  const int p = route_of(L, a.sp, b.sp);
+  const int k0 = L.route_off[p], k1 = L.route_off[p + 1];
+  PairOut<D> o;
+  o.F = zero_vec<D>();
+  o.T = zero_ang<D>();
+  o.xi = zero_vec<D>();
+  o.contact = false;
+  for (int k = k0; k < k1; ++k) {
+    const LawP& T = L.route_terms[k];
+    PairOut<D> r;
+    r.F = zero_vec<D>();
+    r.T = zero_ang<D>();
+    switch (T.kind) {
+      case JDEM_LAW_SPRING:
+        r = law_spring<D>(T, a, b, rij, d_sq);
+        break;
+      case JDEM_LAW_HERTZ:
+        r = law_hertz<D>(T, a, b, rij, d_sq);
+        break;
+      case JDEM_LAW_CUNDALL_STRACK:
+        r = law_cundall<D>(T, a, b, rij, d_sq);
+        break;
+      case JDEM_LAW_SPRING_FRICTION:
+        if constexpr (H) {
+          r = law_friction<D, false>(T, a, b, rij, d_sq, xi_old, dt);
+          o.xi = r.xi;
+          o.contact = r.contact;
+        }
+        break;
+      case JDEM_LAW_HERTZ_FRICTION:
+        if constexpr (H) {
+          r = law_friction<D, true>(T, a, b, rij, d_sq, xi_old, dt);
+          o.xi = r.xi;
+          o.contact = r.contact;
+        }
+        break;
+      default:
+        break;
+    }
+    if (k == k0) {
+      o.F = r.F;
+      o.T = r.T;
+    } else {
+      o.F = o.F + r.F;
+      o.T = o.T + r.T;
+    }
+  }
+  return o;
+}
+
 template <int D, int LAW>
 JD PairOut<D> law_eval(const LawP& L, const Side<D>& a, const Side<D>& b, Vec<D> rij, float d_sq, Vec<D> xi_old,
                        float dt) {
@@ -418,6 +495,8 @@ JD PairOut<D> law_eval(const LawP& L, const Side<D>& a, const Side<D>& b, Vec<D>
   if constexpr (LAW == JDEM_LAW_CUNDALL_STRACK) return law_cundall<D>(L, a, b, rij, d_sq);
   if constexpr (LAW == JDEM_LAW_SPRING_FRICTION) return law_friction<D, false>(L, a, b, rij, d_sq, xi_old, dt);
   if constexpr (LAW == JDEM_LAW_HERTZ_FRICTION) return law_friction<D, true>(L, a, b, rij, d_sq, xi_old, dt);
+  if constexpr (LAW == kLawRouted) return law_routed<D, false>(L, a, b, rij, d_sq, xi_old, dt);
+  if constexpr (LAW == kLawRoutedHist) return law_routed<D, true>(L, a, b, rij, d_sq, xi_old, dt);
   PairOut<D> o;
   o.F = zero_vec<D>();
   o.T = zero_ang<D>();
@@ -450,6 +529,33 @@ JD float law_energy(const LawP& L, const Side<D>& a, const Side<D>& b, float d_s
     float Ei = L.m0[a.mat], Ej = L.m0[b.mat];
     float kn = ((((2.0f * Ei) * a.r) * Ej) * b.r) / (Ei * a.r + Ej * b.r);
     return (0.5f * kn) * (delta * delta);
+  }
+  if constexpr (LAW == kLawRouted || LAW == kLawRoutedHist) {
+    // router.py:262-330 selects the entry; law_combiner.py:180-188 sums it
+    const int p = route_of(L, a.sp, b.sp);
+    const int k0 = L.route_off[p], k1 = L.route_off[p + 1];
+    float e = 0.0f;
+    for (int k = k0; k < k1; ++k) {
+      const LawP& T = L.route_terms[k];
+      float ek = 0.0f;
+      switch (T.kind) {
+        case JDEM_LAW_SPRING:
+        case JDEM_LAW_SPRING_FRICTION:
+          ek = law_energy<D, JDEM_LAW_SPRING>(T, a, b, d_sq);
+          break;
+        case JDEM_LAW_HERTZ:
+        case JDEM_LAW_HERTZ_FRICTION:
+          ek = law_energy<D, JDEM_LAW_HERTZ>(T, a, b, d_sq);
+          break;
+        case JDEM_LAW_CUNDALL_STRACK:
+          ek = law_energy<D, JDEM_LAW_CUNDALL_STRACK>(T, a, b, d_sq);
+          break;
+        default:
+          break;
+      }
+      e = (k == k0) ? ek : e + ek;
+    }
+    return e;
   }
   return 0.0f;
 }

@@ -71,6 +71,10 @@ struct jdem_ctx {
   // private stream on which captured step graphs are replayed (default priority)
   hipStream_t gstream = nullptr;
   hipEvent_t gevent[2] = {nullptr, nullptr};
+  // device copy of the last routed law's terms and species offsets, and the
+  // host bytes it was made from
+  jdem::Buf route;
+  std::vector<char> route_host;
   // every scratch buffer of the context, for accounting and release
   std::vector<jdem::Buf*> bufs();
   size_t scratch_bytes();
@@ -281,6 +285,7 @@ struct InOut {
   const float* mass;
   const float* inertia;
   const int32_t* mat_id;
+  const int32_t* species;  // routed laws with species, else null
   const uint8_t* fixed;
   int32_t* nl_user;      // (N, M)
   float* old_user;       // (N, D)

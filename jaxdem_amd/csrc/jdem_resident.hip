@@ -4,7 +4,7 @@
 //
 // Layout in HBM for the duration of one jdem_system_step call:
 //   P4[b] = (x, y, z, r)   V4[b] = (v, m)   W4[b] = (omega)   Q4[b] = q
-//   I4[b] = (inertia, bits: mat_id << 1 | fixed)
+//   I4[b] = (inertia, bits: species << 16 | mat_id << 1 | fixed)
 // in "work" order b: particles sorted by (cell, original id) at every list
 // rebuild, so a particle's candidates are its memory neighbours.  orig[b] is
 // the original index of work slot b.  The neighbour list is slot-major
@@ -73,7 +73,7 @@ __global__ void __launch_bounds__(kBlock) k_in_state(InOut a) {
     t[c] = a.torque[A * i + c];
     I[c] = a.inertia[A * i + c];
   }
-  const int bits = ((a.mat_id ? a.mat_id[i] : 0) << 1) | (a.fixed[i] ? 1 : 0);
+  const int bits = ((a.species ? a.species[i] : 0) << 16) | ((a.mat_id ? a.mat_id[i] : 0) << 1) | (a.fixed[i] ? 1 : 0);
   a.P4[b] = make_float4(p[0], p[1], p[2], a.rad[i]);
   a.V4[b] = make_float4(v[0], v[1], v[2], a.mass[i]);
   a.W4[b] = make_float4(w[0], w[1], w[2], 0.0f);
@@ -819,7 +819,8 @@ __global__ void __launch_bounds__(kBlock) k_unpack_springs(int64_t n, int M, int
 // slots, whose slot masks take four words.
 template <int D, int LAW, int PM, int MW>
 constexpr int ws_waves() {
-  const bool heavy = D == 3 && (LAW == JDEM_LAW_CUNDALL_STRACK || LAW == JDEM_LAW_HERTZ_FRICTION);
+  const bool heavy = D == 3 && (LAW == JDEM_LAW_CUNDALL_STRACK || LAW == JDEM_LAW_HERTZ_FRICTION ||
+                                LawTraits<LAW>::routed);
   const bool seg3 = D == 3 && PM == 2 && LAW != JDEM_LAW_SPRING && LAW != JDEM_LAW_HERTZ;
   const bool wide = MW > 1 && LawTraits<LAW>::hist;
   const int w = heavy ? (PM == 2 ? 4 : 5) : (seg3 ? 5 : 6);
@@ -1030,9 +1031,14 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(ws_wave
           o.r = pj.w;
           mo.mat = 0;
           o.mat = 0;
-          if (a.law.n_mat > 1) {
-            mo.mat = __float_as_int(a.I[bo].w) >> 1;
-            o.mat = __float_as_int(a.I[j].w) >> 1;
+          mo.sp = 0;
+          o.sp = 0;
+          if (a.law.n_mat > 1 || (LawTraits<LAW>::routed && a.law.n_species > 1)) {
+            const int bo_bits = __float_as_int(a.I[bo].w), j_bits = __float_as_int(a.I[j].w);
+            mo.mat = (bo_bits >> 1) & kMatMask;
+            o.mat = (j_bits >> 1) & kMatMask;
+            mo.sp = (int)((unsigned)bo_bits >> 16);
+            o.sp = (int)((unsigned)j_bits >> 16);
           }
           if constexpr (LawTraits<LAW>::kin) {
             mo.v = xyz<D>(vo);
@@ -1189,6 +1195,10 @@ hipError_t launch_ws_dim(const WS& a, int law, bool per, hipStream_t s) {
       return launch_ws_law<D, JDEM_LAW_SPRING_FRICTION>(a, per, s);
     case JDEM_LAW_HERTZ_FRICTION:
       return launch_ws_law<D, JDEM_LAW_HERTZ_FRICTION>(a, per, s);
+    case kLawRouted:
+      return launch_ws_law<D, kLawRouted>(a, per, s);
+    case kLawRoutedHist:
+      return launch_ws_law<D, kLawRoutedHist>(a, per, s);
     default:
       return hipErrorInvalidValue;
   }

@@ -33,9 +33,16 @@ JD bool is_bonded(const StepArgs& a, int64_t i, int32_t j) {
   return false;
 }
 
+template <int LAW>
+JD int species_of(const LawP& L, int64_t i) {
+  if constexpr (LawTraits<LAW>::routed) return L.species ? L.species[i] : 0;
+  return 0;
+}
+
 template <int D, int LAW>
 JD void fill_partner(const StepArgs& a, Side<D>& b, int32_t j) {
   if (a.law.n_mat > 1) b.mat = a.mat_id[j];
+  b.sp = species_of<LAW>(a.law, j);
   if constexpr (LawTraits<LAW>::kin) {
     b.v = ld<D>(a.vel_in, j);
     b.w = lda<D>(a.ang_in, j);
@@ -161,6 +168,7 @@ JD void load_side(const StepArgs& a, int64_t i, Side<D>& me) {
   me.x = ld<D>(a.pos_in, i);
   me.r = a.rad[i];
   me.mat = (a.law.n_mat > 1) ? a.mat_id[i] : 0;
+  me.sp = species_of<LAW>(a.law, i);
   me.v = ld<D>(a.vel_in, i);
   me.w = lda<D>(a.ang_in, i);
   me.m = a.mass[i];
@@ -265,6 +273,7 @@ __global__ void __launch_bounds__(kBlock) k_energy(StepArgs a) {
   me.x = ld<D>(a.pos_in, i);
   me.r = a.rad[i];
   me.mat = (a.law.n_mat > 1) ? a.mat_id[i] : 0;
+  me.sp = species_of<LAW>(a.law, i);
   double acc = 0.0;
   auto one = [&](int32_t j, Vec<D> xj, float rj) {
     if (j == (int32_t)i) return;
@@ -275,6 +284,7 @@ __global__ void __launch_bounds__(kBlock) k_energy(StepArgs a) {
     b.x = xj;
     b.r = rj;
     b.mat = (a.law.n_mat > 1) ? a.mat_id[j] : 0;
+    b.sp = species_of<LAW>(a.law, j);
     const float e = law_energy<D, LAW>(a.law, me, b, d_sq);
     acc += 0.5 * (double)e;
   };
@@ -547,6 +557,10 @@ struct Dispatch;
         return by_src<D, JDEM_LAW_SPRING_FRICTION>(a, src, per, g, b, s);                        \
       case JDEM_LAW_HERTZ_FRICTION:                                                              \
         return by_src<D, JDEM_LAW_HERTZ_FRICTION>(a, src, per, g, b, s);                         \
+      case kLawRouted:                                                                           \
+        return by_src<D, kLawRouted>(a, src, per, g, b, s);                                      \
+      case kLawRoutedHist:                                                                       \
+        return by_src<D, kLawRoutedHist>(a, src, per, g, b, s);                                  \
       default:                                                                                   \
         return hipErrorInvalidValue;                                                             \
     }                                                                                            \

@@ -205,8 +205,101 @@ class HertzFrictionForce(_HistoryLaw):
         }
 
 
+@ForceModel.register("lawcombiner")
+@dataclass
+class LawCombiner(ForceModel):
+    """Sum of elementary laws (law_combiner.py:22-191): force, torque and
+    energy of a pair are the zeros plus each law's, in order; an empty
+    combiner gives none.  At most one of the laws may carry history."""
+
+    laws: tuple = field(default=())
+    kind = L.LAW_ROUTED
+
+    @property
+    def requires_history(self) -> bool:
+        return any(law.requires_history for law in self.laws)
+
+    def init_history(self, shape: tuple) -> Any:
+        return {"shape": tuple(shape)} if self.requires_history else None
+
+    @property
+    def required_material_properties(self) -> tuple:
+        return tuple(sorted({p for law in self.laws for p in law.required_material_properties}))
+
+
+@ForceModel.register("forcerouter")
+@dataclass
+class ForceRouter(ForceModel):
+    """Law by species pair (router.py:26-339): pair (i, j) takes the law at
+    ``table[species_id[i]][species_id[j]]`` of an S x S table, of which the
+    upper triangle is read (``from_dict`` builds it symmetric)."""
+
+    table: tuple = field(default=())
+    kind = L.LAW_ROUTED
+
+    @property
+    def requires_history(self) -> bool:
+        return any(law.requires_history for row in self.table for law in row)
+
+    def init_history(self, shape: tuple) -> Any:
+        return {"shape": tuple(shape)} if self.requires_history else None
+
+    @property
+    def required_material_properties(self) -> tuple:
+        return tuple(sorted({p for row in self.table for law in row for p in law.required_material_properties}))
+
+    @staticmethod
+    def from_dict(S: int, mapping: dict) -> "ForceRouter":
+        """router.py:302-333: the mapping is symmetrised; pairs it does not
+        name get an empty LawCombiner (no force)."""
+        empty = LawCombiner()
+        m = [[empty for _ in range(S)] for _ in range(S)]
+        for (a, b), law in mapping.items():
+            m[a][b] = m[b][a] = law
+        return ForceRouter(table=tuple(tuple(r) for r in m))
+
+
+def _flat_terms(model: ForceModel) -> list:
+    """Elementary laws of one routed entry in evaluation order; None is the
+    zeros a LawCombiner starts from (law_combiner.py:72-86).  Nested
+    combiners flatten exactly: adding the zeros again changes no sum."""
+    if isinstance(model, LawCombiner):
+        out: list = [None]
+        for law in model.laws:
+            out += _flat_terms(law)
+        return out
+    if isinstance(model, ForceRouter):
+        raise NotImplementedError("a ForceRouter inside a ForceRouter or LawCombiner is not supported")
+    if model.kind == L.LAW_NONE:
+        return [None]
+    return [model]
+
+
+def route_layout(model: ForceModel) -> tuple:
+    """(n_species, route_off, terms) of a routed law: the terms of ordered
+    species pair p = s_i * S + s_j are terms[route_off[p] : route_off[p + 1]]
+    (one entry, n_species 0, for a LawCombiner on its own)."""
+    if isinstance(model, ForceRouter):
+        # the router evaluates the upper triangle and uses entry (a, b) for
+        # both orders of the pair (router.py:113-131, 237-246)
+        S = len(model.table)
+        entries = [model.table[min(a, b)][max(a, b)] for a in range(S) for b in range(S)]
+    else:
+        S, entries = 0, [model]
+    off, terms = [0], []
+    for e in entries:
+        t = _flat_terms(e)
+        if sum(1 for x in t if x is not None and x.requires_history) > 1:
+            raise NotImplementedError("a routed entry with more than one history law is not supported")
+        terms += t
+        off.append(len(terms))
+    return S, np.asarray(off, np.int32), terms
+
+
 def law_struct(model: ForceModel, mat_table, device) -> tuple:
     """Build the jdem_law struct; returns (struct, keepalive tensors)."""
+    if model.kind == L.LAW_ROUTED:
+        return _routed_struct(model, mat_table, device)
     law = L.JdemLaw()
     law.kind = model.kind
     law.n_mat = len(mat_table)
@@ -228,6 +321,31 @@ def law_struct(model: ForceModel, mat_table, device) -> tuple:
     law.p0 = float(dev.get("p0", 0.0))
     law.p1 = float(dev.get("p1", 0.0))
     return law, dev
+
+
+def _routed_struct(model: ForceModel, mat_table, device) -> tuple:
+    """jdem_law of a ForceRouter / LawCombiner: host arrays of the species
+    offsets and the elementary terms (each with its own device tables); the
+    caller sets ``species`` to State.species_id when n_species > 1."""
+    S, off, terms = route_layout(model)
+    arr = (L.JdemLaw * len(terms))()
+    keep: list = [off, arr]
+    for k, t in enumerate(terms):
+        if t is None:
+            arr[k].kind = L.LAW_NONE
+            arr[k].n_mat = len(mat_table)
+        else:
+            sub, dev = law_struct(t, mat_table, device)
+            arr[k] = sub
+            keep.append(dev)
+    law = L.JdemLaw()
+    law.kind = L.LAW_ROUTED
+    law.n_mat = len(mat_table)
+    law.n_species = S
+    law.n_terms = len(terms)
+    law.route_off = off.ctypes.data
+    law.route_terms = arr
+    return law, keep
 
 
 # ---------------------------------------------------------------------------

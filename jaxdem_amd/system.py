@@ -30,8 +30,10 @@ from .forces import (
     CundallStrackForce,
     ForceManager,
     ForceModel,
+    ForceRouter,
     HertzFrictionForce,
     HertzianForce,
+    LawCombiner,
     SpringForce,
     SpringFrictionForce,
     law_struct,
@@ -319,7 +321,8 @@ _BUILTIN_LIN = (LinearIntegrator, VelocityVerlet, DirectEuler)
 _BUILTIN_ROT = (RotationIntegrator, VelocityVerletSpiral, Spiral)
 _BUILTIN_DOM = (Domain, FreeDomain, PeriodicDomain, ReflectDomain)
 _BUILTIN_COL = (Collider, NaiveSimulator, DynamicCellList, NeighborList)
-_BUILTIN_LAW = (SpringForce, HertzianForce, CundallStrackForce, SpringFrictionForce, HertzFrictionForce)
+_BUILTIN_LAW = (SpringForce, HertzianForce, CundallStrackForce, SpringFrictionForce, HertzFrictionForce, LawCombiner,
+                ForceRouter)
 
 
 def _require_gpu(state: State) -> None:
@@ -375,6 +378,16 @@ def _system_struct(state: State, system: System, keep: list, resident=None, for_
     S.domain = system.domain.to_c(state.dim)
     law, dev = law_struct(system.force_model, system.mat_table, state.device)
     keep.append(dev)
+    if law.kind == L.LAW_ROUTED and law.n_species > 1:
+        # the kernels index the router's table by species: checked once per
+        # version of the array
+        sp = state.species_id
+        tag = (id(sp), sp._version, int(law.n_species))
+        if state.__dict__.get("_species_checked") != tag:
+            if state.N and (int(sp.min()) < 0 or int(sp.max()) >= law.n_species):
+                raise ValueError(f"species_id outside the router's {law.n_species} species")
+            state.__dict__["_species_checked"] = tag
+        law.species = sp.data_ptr()
     S.law = law
     S.lin_kind = int(type(system.linear_integrator).kind)
     S.rot_kind = int(type(system.rotation_integrator).kind)

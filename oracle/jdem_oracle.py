@@ -191,6 +191,14 @@ class Law:
     mu_eff: np.ndarray | None = None  # (M, M)
     kt_ratio: float = 2.0 / 7.0
     damping: float = 0.0  # normal damping ratio (fraction of critical)
+    # kind "routed": ForceRouter (router.py:26-339) / LawCombiner
+    # (law_combiner.py:22-191).  Ordered species pair p = s_i * S + s_j
+    # (p = 0 for every pair when n_species <= 1) takes the elementary laws
+    # terms[route_off[p] : route_off[p + 1]]; kind "none" is a combiner's
+    # zeros.
+    n_species: int = 0
+    route_off: np.ndarray | None = None
+    terms: tuple = ()
 
 
 @dataclass
@@ -247,6 +255,7 @@ class State:
     mat_id: np.ndarray
     bond_id: np.ndarray
     extra: dict = field(default_factory=dict)
+    species: np.ndarray | None = None  # (N,) species_id, read by routed laws
 
     @property
     def N(self) -> int:
@@ -259,7 +268,7 @@ class State:
     def copy(self) -> "State":
         return State(
             **{
-                k: (v.copy() if isinstance(v, np.ndarray) else dict(v))
+                k: (v.copy() if isinstance(v, np.ndarray) else (dict(v) if isinstance(v, dict) else v))
                 for k, v in self.__dict__.items()
             }
         )
@@ -560,6 +569,48 @@ def _gather_pair(st: State, i: np.ndarray, j: np.ndarray, dom: Domain):
     return rij, d_sq
 
 
+def _route_index(law: Law, st: State, i, j) -> np.ndarray:
+    i, j = np.broadcast_arrays(np.asarray(i), np.asarray(j))
+    if law.n_species <= 1:
+        return np.zeros(i.shape, np.int64)
+    return st.species[i].astype(np.int64) * law.n_species + st.species[j].astype(np.int64)
+
+
+def _routed_force(law: Law, st: State, i, j, dom: Domain, dt, xi_old):
+    """ForceRouter.force_and_history (router.py:70-160): the pair takes the
+    law of its species entry (select_n over the table, router.py:143-148);
+    a LawCombiner entry (law_combiner.py:60-93) adds its laws' forces and
+    torques to zeros, in order, and keeps the history of its history law."""
+    T_ = st.pos.dtype
+    D = st.dim
+    A = 3 if D == 3 else 1
+    p = _route_index(law, st, i, j)
+    F = np.zeros(p.shape + (D,), T_)
+    T = np.zeros(p.shape + (A,), T_)
+    xi = np.zeros(p.shape + (D,), T_)
+    contact = np.zeros(p.shape, bool)
+    for e in np.unique(p):
+        sel = p == e
+        fe = te = None
+        xe, ce = np.zeros_like(xi), np.zeros_like(contact)
+        for sub in law.terms[law.route_off[e] : law.route_off[e + 1]]:
+            if sub.kind == "none":
+                f, t = np.zeros_like(F), np.zeros_like(T)
+            else:
+                f, t, xn, c = law_force(sub, st, i, j, dom, dt, xi_old)
+                if xn is not None:
+                    xe, ce = xn, np.asarray(c, bool)
+            fe = f if fe is None else fe + f
+            te = t if te is None else te + t
+        if fe is None:
+            continue
+        F = np.where(sel[..., None], fe, F)
+        T = np.where(sel[..., None], te, T)
+        xi = np.where(sel[..., None], xe, xi)
+        contact = np.where(sel, ce, contact)
+    return F, T, xi, contact
+
+
 def law_force(law: Law, st: State, i, j, dom: Domain, dt=None, xi_old=None):
     """Force/torque on ``i`` from ``j``; returns (F, T, xi_new, contact).
 
@@ -567,6 +618,8 @@ def law_force(law: Law, st: State, i, j, dom: Domain, dt=None, xi_old=None):
     d*sqrt(d)).  cundallstrack: cundall_strack.py:123-193.  springfriction
     and hertzfriction: tangential spring with Coulomb cap (module docstring).
     """
+    if law.kind == "routed":
+        return _routed_force(law, st, i, j, dom, dt, xi_old)
     T_ = st.pos.dtype
     D = st.dim
     A = 3 if D == 3 else 1
@@ -736,7 +789,20 @@ def law_energy(law: Law, st: State, i, j, dom: Domain) -> np.ndarray:
     """Pair energy: spring.py:138-147, hertz.py:147-162, cundall_strack.py:220-233.
 
     The history laws store their normal part only (elastic normal energy).
+    Routed laws: the entry's energies summed from zeros (law_combiner.py:
+    180-188), selected by species (router.py:262-330).
     """
+    if law.kind == "routed":
+        p = _route_index(law, st, i, j)
+        e_all = np.zeros(p.shape, st.pos.dtype)
+        for e in np.unique(p):
+            ee = None
+            for sub in law.terms[law.route_off[e] : law.route_off[e + 1]]:
+                ek = np.zeros(p.shape, st.pos.dtype) if sub.kind == "none" else law_energy(sub, st, i, j, dom)
+                ee = ek if ee is None else ee + ek
+            if ee is not None:
+                e_all = np.where(p == e, ee, e_all)
+        return e_all
     T_ = st.pos.dtype
     zero = np.asarray(0.0, T_)
     one = np.asarray(1.0, T_)

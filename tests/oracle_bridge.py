@@ -8,8 +8,17 @@ from oracle import jdem_oracle as O
 
 
 def law_of(system) -> O.Law:
-    fm = system.force_model
-    mt = system.mat_table
+    return _law(system.force_model, system.mat_table)
+
+
+def _law(fm, mt) -> O.Law:
+    from jaxdem_amd import _lib as L
+    from jaxdem_amd.forces import route_layout
+
+    if fm.kind == L.LAW_ROUTED:
+        S, off, terms = route_layout(fm)
+        return O.Law(kind="routed", n_species=S, route_off=np.asarray(off, np.int64),
+                     terms=tuple(O.Law(kind="none") if t is None else _law(t, mt) for t in terms))
     kind = fm.type_name
     get = lambda k: (np.asarray(getattr(mt, k), np.float32) if mt.has(k) else None)  # noqa: E731
     return O.Law(
@@ -26,6 +35,18 @@ def law_of(system) -> O.Law:
     )
 
 
+def _cast_law(law: O.Law, dtype) -> O.Law:
+    kw = {}
+    for k, v in law.__dict__.items():
+        if k == "route_off":
+            kw[k] = v
+        elif k == "terms":
+            kw[k] = tuple(_cast_law(t, dtype) for t in v)
+        else:
+            kw[k] = v.astype(dtype) if isinstance(v, np.ndarray) else v
+    return O.Law(**kw)
+
+
 def params_of(state, system, dtype=np.float32) -> O.Params:
     dom = system.domain
     odom = O.Domain(
@@ -36,7 +57,7 @@ def params_of(state, system, dtype=np.float32) -> O.Params:
     )
     law = law_of(system)
     if dtype != np.float32:
-        law = O.Law(**{k: (v.astype(dtype) if isinstance(v, np.ndarray) else v) for k, v in law.__dict__.items()})
+        law = _cast_law(law, dtype)
     col = system.collider
     cname = col.type_name.lower()
     P = O.Params(
@@ -83,6 +104,7 @@ def state_of(state, dtype=np.float32) -> O.State:
         fixed=d["fixed"].astype(bool),
         mat_id=d["mat_id"].astype(np.int64),
         bond_id=bond,
+        species=d["species_id"].astype(np.int64),
     )
 
 

@@ -52,7 +52,8 @@ STRUCT_FIELDS = {
     "jdem_state": ["n", "dim", "bond_width", "pos", "vel", "ang_vel", "q", "force", "torque", "rad", "mass",
                    "inertia", "mat_id", "bond_id", "fixed"],
     "jdem_domain": ["kind", "dim", "box_size", "inv_box_size", "anchor", "restitution"],
-    "jdem_law": ["kind", "n_mat", "t0", "t1", "t2", "m0", "m1", "m2", "m3", "p0", "p1"],
+    "jdem_law": ["kind", "n_mat", "t0", "t1", "t2", "m0", "m1", "m2", "m3", "p0", "p1", "n_species", "n_terms",
+                 "route_off", "route_terms", "species"],
     "jdem_grid": ["dims", "strides", "cell_size", "search_range", "periodic", "n_cells"],
     "jdem_nlist": ["max_neighbors", "cutoff", "skin", "nl", "old_pos", "n_build_times", "overflow", "secondary",
                    "grid", "hist_xi", "hist_live", "resident_in", "resident_out"],
