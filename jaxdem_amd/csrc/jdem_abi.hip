@@ -22,7 +22,7 @@ std::vector<Buf*> jdem_ctx::bufs() {
                            &cstart,  &cend,   &spos,     &nl_new,  &xi_new,  &live_new, &flags,  &red,
                            &nl_copy, &xi_copy, &live_copy, &wF,     &wT,      &wOld,     &wWpos, &wSrc,
                            &wInv,    &wKeys,  &fireV,    &fireDev, &segp,  &segid, &segflag, &wXiRank,
-                           &route};
+                           &route,   &ccnt};
   for (int k = 0; k < 2; ++k)
     for (Buf* b : {&wP[k], &wV[k], &wW[k], &wQ[k], &wI[k], &wOrig[k], &wNl[k], &wXi[k], &wLive[k]}) all.push_back(b);
   return all;
@@ -569,6 +569,7 @@ struct SegRun {
   const int64_t* offsets;  // particle ranges (device, n_seg + 1)
   int64_t* cpref;          // flagged-particle prefix (device, n_seg + 1)
   int64_t* ccref;          // flagged-cell prefix (device, n_seg + 1)
+  int64_t* wpref;          // flagged-wavefront prefix (device, n_seg + 1)
   int64_t total_cells;
   int32_t* n_build;   // builds of each system (device)
   int32_t* seg_flag;  // system asks for a rebuild (device)
@@ -723,11 +724,13 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     const int sp = span_begin(ctx, 2, 0);
     HIPCHK(ctx, ws_launch_gated_zero(gate, (int32_t*)ctx->flags.p + 2, st));  // overflow of this build
     HIPCHK(ctx, ws_launch_seg_modes(sg->n_seg, sg->seg_flag, sg->n_build, sg->seg_mode, st, gate, events));
-    HIPCHK(ctx, ws_launch_seg_prefix(sg->n_seg, sg->seg_mode, sg->offsets, sg->segp, sg->cpref, sg->ccref, st));
+    HIPCHK(ctx, ws_launch_seg_prefix(sg->n_seg, sg->seg_mode, sg->offsets, sg->segp, sg->cpref, sg->ccref, sg->wpref,
+                                     st));
     // springs of the wavefronts that hold a rebuilt system: per wave -> by rank
     if (hist && M > 0)
       HIPCHK(ctx, ws_launch_unpack_springs(N, M, W64, F4(ctx->wXi[xc]), U64(ctx->wLive[nlc]), F4(ctx->wW[cur]),
-                                           F4(ctx->wXiRank), sg->seg, sg->seg_mode, st, gate));
+                                           F4(ctx->wXiRank), sg->seg, sg->seg_mode, st, gate, sg->wpref, sg->offsets,
+                                           sg->n_seg));
     RB r;
     std::memset(&r, 0, sizeof(r));
     r.n = N;
@@ -770,10 +773,11 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     HIPCHK(ctx, ws_launch_hash_flagged(r, D, st));
     uint32_t* sk = nullptr;
     int32_t* perm = nullptr;
-    // sized for every particle and every cell of the batch; the kernels read
-    // the flagged count on the device
-    HIPCHK(ctx, radix_sort_pairs(ctx, r.keys_c, N, nbits_for(std::max<int64_t>(sg->total_cells, 1)), &sk, &perm, st,
-                                 sg->cpref + sg->n_seg, gate));
+    // counting sort by compact cell, sized for every particle and every cell
+    // of the batch; the kernels read the flagged particle and cell counts on
+    // the device
+    HIPCHK(ctx, cell_sort_dev(ctx, r.keys_c, N, std::max<int64_t>(sg->total_cells, 1), sg->cpref + sg->n_seg,
+                              sg->ccref + sg->n_seg, &sk, &perm, st, gate));
     r.perm = perm;
     HIPCHK(ctx, ws_launch_place(r, sk, I32(ctx->cstart), st));
     if (M > 0) {
@@ -798,6 +802,9 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       b.nl = I32(ctx->wNl[nlc]);
       b.overflow = (int32_t*)ctx->flags.p + 2;
       b.gate = gate;
+      b.wpref = sg->wpref;
+      b.offsets = sg->offsets;
+      b.n_seg = sg->n_seg;
       if (hist) {
         b.W64 = W64;
         b.inv = I32(ctx->wInv);
@@ -812,7 +819,8 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     }
     if (hist && M > 0)
       HIPCHK(ctx, ws_launch_pack_springs(N, M, W64, F4(ctx->wXiRank), U64(ctx->wLive[nlc]), F4(ctx->wXi[xc]),
-                                         F4(ctx->wW[cur]), sg->seg, sg->seg_mode, st, gate));
+                                         F4(ctx->wW[cur]), sg->seg, sg->seg_mode, st, gate, sg->wpref, sg->offsets,
+                                         sg->n_seg));
     HIPCHK(ctx, ws_launch_gate_end(gate, st));
     span_end(ctx, sp);
     return JDEM_OK;
@@ -1712,7 +1720,7 @@ int jdem_batch_step(jdem_ctx* ctx, jdem_system* sys, const jdem_batch* batch, in
   rc = ensure_flags(ctx);
   if (rc) return rc;
   hipStream_t st = ctx->stream;
-  ENSURE(ctx, ctx->segp, sizeof(SegP) * (size_t)B + 3 * 8 * ((size_t)B + 1));
+  ENSURE(ctx, ctx->segp, sizeof(SegP) * (size_t)B + 4 * 8 * ((size_t)B + 1));
   ENSURE(ctx, ctx->cend, 4 * (size_t)std::max<int64_t>(N, 1));
   ENSURE(ctx, ctx->segid, 4 * (size_t)std::max<int64_t>(N, 1));
   ENSURE(ctx, ctx->segflag, 3 * 4 * (size_t)B);
@@ -1735,6 +1743,7 @@ int jdem_batch_step(jdem_ctx* ctx, jdem_system* sys, const jdem_batch* batch, in
   sg.offsets = doff;
   sg.cpref = doff + (B + 1);
   sg.ccref = doff + 2 * (B + 1);
+  sg.wpref = doff + 3 * (B + 1);
   sg.n_build = dbuild;
   sg.seg_flag = dbuild + B;
   sg.seg_mode = dbuild + 2 * B;

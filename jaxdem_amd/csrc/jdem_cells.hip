@@ -450,6 +450,201 @@ hipError_t cell_sort_bufs(const SortBufs& sb, const uint32_t* keys_in, int64_t n
   return hipGetLastError();
 }
 
+// ---- counting sort with device-side counts (batch rebuilds) --------------
+// Each kernel returns at once while *gate == 0 and walks its range in turn
+// over a grid capped at kCappedGrid workgroups.
+__global__ void __launch_bounds__(kBlock) k_csd_zero(int32_t* __restrict__ cnt, int32_t* __restrict__ cur,
+                                                      const int64_t* __restrict__ c_dev,
+                                                      const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int64_t nc = *c_dev;
+  for (int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; c <= nc; c += (int64_t)gridDim.x * blockDim.x) {
+    cnt[c] = 0;
+    if (c < nc) cur[c] = 0;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_csd_count(const uint32_t* __restrict__ keys,
+                                                       const int64_t* __restrict__ n_dev, int32_t* __restrict__ cnt,
+                                                       const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int64_t n = *n_dev;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x)
+    atomicAdd(&cnt[keys[i]], 1);
+}
+
+// exclusive scan of *c_dev + 1 values: chunk totals, one block over the
+// totals, chunk scans plus offsets (k_scan_totals / _sums / _apply with the
+// length read on the device)
+__global__ void __launch_bounds__(kBlock) k_scan_totals_dev(const uint32_t* __restrict__ data,
+                                                             const int64_t* __restrict__ c_dev,
+                                                             uint32_t* __restrict__ sums,
+                                                             const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  __shared__ uint32_t sh[kBlock];
+  const int tid = threadIdx.x;
+  const int64_t total = *c_dev + 1;
+  const int64_t nchunks = (total + SC_CHUNK - 1) / SC_CHUNK;
+  for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const int64_t base = ch * SC_CHUNK;
+    uint32_t s = 0;
+#pragma unroll
+    for (int it = 0; it < SC_ITEMS; ++it) {
+      const int64_t k = base + (int64_t)it * kBlock + tid;
+      if (k < total) s += data[k];
+    }
+    sh[tid] = s;
+    __syncthreads();
+    for (int off = kBlock / 2; off > 0; off >>= 1) {
+      if (tid < off) sh[tid] += sh[tid + off];
+      __syncthreads();
+    }
+    if (tid == 0) sums[ch] = sh[0];
+    __syncthreads();  // the next chunk overwrites sh
+  }
+}
+
+__global__ void __launch_bounds__(1024) k_scan_sums_dev(uint32_t* __restrict__ sums, const int64_t* __restrict__ c_dev,
+                                                         const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  __shared__ uint32_t part[1024];
+  const int tid = threadIdx.x;
+  const int nsums = (int)((*c_dev + 1 + SC_CHUNK - 1) / SC_CHUNK);
+  const int per = (nsums + 1023) / 1024;
+  const int b = tid * per;
+  const int e = min(b + per, nsums);
+  uint32_t s = 0;
+  for (int k = b; k < e; ++k) s += sums[k];
+  part[tid] = s;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const uint32_t v = (tid >= off) ? part[tid - off] : 0u;
+    __syncthreads();
+    part[tid] += v;
+    __syncthreads();
+  }
+  uint32_t run = (tid == 0) ? 0u : part[tid - 1];
+  for (int k = b; k < e; ++k) {
+    const uint32_t v = sums[k];
+    sums[k] = run;
+    run += v;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_scan_apply_dev(uint32_t* __restrict__ data,
+                                                            const int64_t* __restrict__ c_dev,
+                                                            const uint32_t* __restrict__ sums,
+                                                            const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  __shared__ uint32_t tile[SC_CHUNK];
+  __shared__ uint32_t tsum[kBlock];
+  const int tid = threadIdx.x;
+  const int64_t total = *c_dev + 1;
+  const int64_t nchunks = (total + SC_CHUNK - 1) / SC_CHUNK;
+  for (int64_t ch = blockIdx.x; ch < nchunks; ch += gridDim.x) {
+    const int64_t base = ch * SC_CHUNK;
+#pragma unroll
+    for (int it = 0; it < SC_ITEMS; ++it) {
+      const int64_t k = base + (int64_t)it * kBlock + tid;
+      tile[it * kBlock + tid] = (k < total) ? data[k] : 0u;
+    }
+    __syncthreads();
+    uint32_t s = 0;
+#pragma unroll
+    for (int it = 0; it < SC_ITEMS; ++it) s += tile[tid * SC_ITEMS + it];
+    tsum[tid] = s;
+    __syncthreads();
+    for (int off = 1; off < kBlock; off <<= 1) {
+      const uint32_t v = (tid >= off) ? tsum[tid - off] : 0u;
+      __syncthreads();
+      tsum[tid] += v;
+      __syncthreads();
+    }
+    uint32_t run = sums[ch] + ((tid == 0) ? 0u : tsum[tid - 1]);
+#pragma unroll
+    for (int it = 0; it < SC_ITEMS; ++it) {
+      const uint32_t v = tile[tid * SC_ITEMS + it];
+      tile[tid * SC_ITEMS + it] = run;
+      run += v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int it = 0; it < SC_ITEMS; ++it) {
+      const int64_t k = base + (int64_t)it * kBlock + tid;
+      if (k < total) data[k] = tile[it * kBlock + tid];
+    }
+    __syncthreads();  // the next chunk overwrites tile and tsum
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_csd_place(const uint32_t* __restrict__ keys,
+                                                       const int64_t* __restrict__ n_dev,
+                                                       const int32_t* __restrict__ starts, int32_t* __restrict__ cur,
+                                                       uint32_t* __restrict__ tkey, int32_t* __restrict__ tid,
+                                                       const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int64_t n = *n_dev;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t k = keys[i];
+    const int32_t slot = starts[k] + atomicAdd(&cur[k], 1);
+    tkey[slot] = k;
+    tid[slot] = (int32_t)i;
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_csd_rank(const uint32_t* __restrict__ tkey,
+                                                      const int32_t* __restrict__ tid,
+                                                      const int64_t* __restrict__ n_dev,
+                                                      const int32_t* __restrict__ starts, uint32_t* __restrict__ sk,
+                                                      int32_t* __restrict__ perm, const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int64_t n = *n_dev;
+  for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < n; j += (int64_t)gridDim.x * blockDim.x) {
+    const uint32_t k = tkey[j];
+    const int32_t id = tid[j];
+    const int32_t s = starts[k], e = starts[k + 1];
+    int32_t r = 0;
+    for (int32_t m = s; m < e; ++m) r += tid[m] < id ? 1 : 0;
+    sk[s + r] = k;
+    perm[s + r] = id;
+  }
+}
+
+hipError_t cell_sort_dev(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, int64_t n_cells, const int64_t* n_dev,
+                         const int64_t* c_dev, uint32_t** keys_out, int32_t** vals_out, hipStream_t s,
+                         const int32_t* gate) {
+  const size_t nn = (size_t)(n > 0 ? n : 1);
+  const int64_t nc = n_cells > 0 ? n_cells : 1;
+  const int64_t max_chunks = (nc + 1 + SC_CHUNK - 1) / SC_CHUNK;
+  if (max_chunks > 1024 * 1024) return hipErrorInvalidValue;
+  if (!ensure(ctx, ctx->keys[0], sizeof(uint32_t) * nn) || !ensure(ctx, ctx->keys[1], sizeof(uint32_t) * nn) ||
+      !ensure(ctx, ctx->vals[0], sizeof(int32_t) * nn) || !ensure(ctx, ctx->vals[1], sizeof(int32_t) * nn) ||
+      !ensure(ctx, ctx->ccur, sizeof(int32_t) * (size_t)nc) ||
+      !ensure(ctx, ctx->ccnt, sizeof(int32_t) * (size_t)(nc + 1)) ||
+      !ensure(ctx, ctx->hist, sizeof(uint32_t) * (size_t)(max_chunks + 1)))
+    return hipErrorOutOfMemory;
+  uint32_t* tkey = (uint32_t*)ctx->keys[0].p;
+  int32_t* tid = (int32_t*)ctx->vals[0].p;
+  uint32_t* sk = (uint32_t*)ctx->keys[1].p;
+  int32_t* perm = (int32_t*)ctx->vals[1].p;
+  int32_t* cnt = (int32_t*)ctx->ccnt.p;
+  int32_t* cur = (int32_t*)ctx->ccur.p;
+  uint32_t* sums = (uint32_t*)ctx->hist.p;
+  const unsigned gc = std::min<unsigned>(nblocks(nc + 1), (unsigned)kCappedGrid);
+  const unsigned gn = std::min<unsigned>(nblocks((int64_t)nn), (unsigned)kCappedGrid);
+  const unsigned gk = (unsigned)std::min<int64_t>(max_chunks, (int64_t)kCappedGrid);
+  hipLaunchKernelGGL(k_csd_zero, dim3(gc), dim3(kBlock), 0, s, cnt, cur, c_dev, gate);
+  hipLaunchKernelGGL(k_csd_count, dim3(gn), dim3(kBlock), 0, s, keys_in, n_dev, cnt, gate);
+  hipLaunchKernelGGL(k_scan_totals_dev, dim3(gk), dim3(kBlock), 0, s, (const uint32_t*)cnt, c_dev, sums, gate);
+  hipLaunchKernelGGL(k_scan_sums_dev, dim3(1), dim3(1024), 0, s, sums, c_dev, gate);
+  hipLaunchKernelGGL(k_scan_apply_dev, dim3(gk), dim3(kBlock), 0, s, (uint32_t*)cnt, c_dev, sums, gate);
+  hipLaunchKernelGGL(k_csd_place, dim3(gn), dim3(kBlock), 0, s, keys_in, n_dev, cnt, cur, tkey, tid, gate);
+  hipLaunchKernelGGL(k_csd_rank, dim3(gn), dim3(kBlock), 0, s, tkey, tid, n_dev, cnt, sk, perm, gate);
+  *keys_out = sk;
+  *vals_out = perm;
+  return hipGetLastError();
+}
+
 hipError_t cell_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, int64_t n_cells, uint32_t** keys_out,
                            int32_t** vals_out, int32_t* starts, hipStream_t s, const int32_t* gate) {
   const size_t nn = (size_t)(n > 0 ? n : 1);

@@ -75,6 +75,8 @@ struct jdem_ctx {
   // host bytes it was made from
   jdem::Buf route;
   std::vector<char> route_host;
+  // per-cell counts / starts of the batch rebuild's counting sort
+  jdem::Buf ccnt;
   // every scratch buffer of the context, for accounting and release
   std::vector<jdem::Buf*> bufs();
   size_t scratch_bytes();
@@ -214,6 +216,15 @@ hipError_t radix_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, i
 // device word), leaving the previous sort's outputs in place.
 hipError_t cell_sort_pairs(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, int64_t n_cells, uint32_t** keys_out,
                            int32_t** vals_out, int32_t* starts, hipStream_t s, const int32_t* gate = nullptr);
+// The counting sort with both counts on the device: the first *n_dev of
+// the keys (n bounds it), every key below *c_dev (n_cells bounds it).  The
+// launches are sized by the bounds and capped (kCappedGrid workgroups that
+// walk the work in turn), so a rebuild of a few systems of a large batch
+// costs what its particles and cells need.  Results as cell_sort_pairs; the
+// cell starts (compact, *c_dev + 1 entries) go to the context's scratch.
+hipError_t cell_sort_dev(jdem_ctx* ctx, const uint32_t* keys_in, int64_t n, int64_t n_cells, const int64_t* n_dev,
+                         const int64_t* c_dev, uint32_t** keys_out, int32_t** vals_out, hipStream_t s,
+                         const int32_t* gate);
 // cend[c] = starts[c + 1] for c < n_cells
 hipError_t launch_starts_to_ends(const int32_t* starts, int64_t n_cells, int32_t* cend, hipStream_t s);
 // key bits of a grid of n_cells cells (at least 1)
@@ -364,6 +375,11 @@ struct NLW {
   int64_t ostride;
   const int64_t* ccref;  // compact key of system s's first cell
   const int32_t* gate;   // when set, the kernel returns at once while *gate == 0
+  // with wpref (flagged systems' wavefront prefix), the launch walks only the
+  // wavefronts that hold a flagged system's slots
+  const int64_t* wpref;
+  const int64_t* offsets;
+  int n_seg;
 };
 enum { SEG_KEEP = 0, SEG_FRESH = 1, SEG_REMAP = 2 };
 
@@ -401,7 +417,7 @@ struct RB {
   int32_t* cidx;
 };
 hipError_t ws_launch_seg_prefix(int n_seg, const int32_t* seg_mode, const int64_t* offsets, const SegP* segp,
-                               int64_t* cpref, int64_t* ccref, hipStream_t s);
+                               int64_t* cpref, int64_t* ccref, int64_t* wpref, hipStream_t s);
 hipError_t ws_launch_hash_flagged(const RB& r, int dim, hipStream_t s);
 hipError_t ws_launch_place(const RB& r, const uint32_t* sk, int32_t* starts, hipStream_t s);
 
@@ -471,13 +487,17 @@ hipError_t ws_launch_copy_rows(const int32_t* src, int32_t* dst, int64_t n, int 
 // and the exit keep springs by rank instead ([k N + b], k-th live slot).
 // pack: by rank -> per wave (offsets = prefix of the live counts);
 // unpack: per wave -> by rank.  With seg_mode, only wavefronts holding a
-// slot of a system that is rebuilt (mode != SEG_KEEP) are converted.
+// slot of a system that is rebuilt (mode != SEG_KEEP) are converted; with
+// wpref (the flagged systems' wavefront prefix of k_seg_prefix) the launch
+// walks those wavefronts only.
 hipError_t ws_launch_pack_springs(int64_t n, int M, int W64, const float4* rank, const uint64_t* live, float4* wave,
                                   float4* W, const int32_t* seg, const int32_t* seg_mode, hipStream_t s,
-                                  const int32_t* gate = nullptr);
+                                  const int32_t* gate = nullptr, const int64_t* wpref = nullptr,
+                                  const int64_t* offsets = nullptr, int n_seg = 0);
 hipError_t ws_launch_unpack_springs(int64_t n, int M, int W64, const float4* wave, const uint64_t* live,
                                     const float4* W, float4* rank, const int32_t* seg, const int32_t* seg_mode,
-                                    hipStream_t s, const int32_t* gate = nullptr);
+                                    hipStream_t s, const int32_t* gate = nullptr, const int64_t* wpref = nullptr,
+                                    const int64_t* offsets = nullptr, int n_seg = 0);
 hipError_t ws_launch_inverse(const int32_t* orig, int64_t n, int32_t* wpos, hipStream_t s);
 hipError_t ws_launch_iota(int32_t* p, int64_t n, hipStream_t s);
 hipError_t ws_launch_copy4(float4* d, const float4* src, int64_t n, hipStream_t s);

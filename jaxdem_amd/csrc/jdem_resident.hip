@@ -318,52 +318,83 @@ __global__ void k_seg_init_flags(int n_seg, const int32_t* __restrict__ n_build,
 
 // Exclusive prefix sums over the systems of the flagged particle and cell
 // counts (one block).
+// Wavefronts (64 work slots) that hold particles of system s.
+JD int64_t seg_waves(const int64_t* __restrict__ offsets, int s) {
+  const int64_t a = offsets[s], e = offsets[s + 1];
+  return e > a ? ((e - 1) >> 6) - (a >> 6) + 1 : 0;
+}
+
+// Exclusive prefix sums over the systems of the flagged particle, cell and
+// wavefront counts (one block).
 __global__ void __launch_bounds__(1024) k_seg_prefix(int n_seg, const int32_t* __restrict__ seg_mode,
                                                       const int64_t* __restrict__ offsets,
                                                       const SegP* __restrict__ segp, int64_t* __restrict__ cpref,
-                                                      int64_t* __restrict__ ccref) {
-  __shared__ int64_t sp[1024], sc[1024];
+                                                      int64_t* __restrict__ ccref, int64_t* __restrict__ wpref) {
+  __shared__ int64_t sp[1024], sc[1024], sw[1024];
   const int t = threadIdx.x;
   const int per = (n_seg + 1023) / 1024;
   const int s0 = t * per, s1 = min(n_seg, s0 + per);
-  int64_t np = 0, nc = 0;
+  int64_t np = 0, nc = 0, nw = 0;
   for (int s = s0; s < s1; ++s)
     if (seg_mode[s] != SEG_KEEP) {
       np += offsets[s + 1] - offsets[s];
       nc += segp[s].grid.n_cells;
+      nw += seg_waves(offsets, s);
     }
   sp[t] = np;
   sc[t] = nc;
+  sw[t] = nw;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {  // inclusive scan
-    const int64_t ap = t >= off ? sp[t - off] : 0, ac = t >= off ? sc[t - off] : 0;
+    const int64_t ap = t >= off ? sp[t - off] : 0, ac = t >= off ? sc[t - off] : 0, aw = t >= off ? sw[t - off] : 0;
     __syncthreads();
     sp[t] += ap;
     sc[t] += ac;
+    sw[t] += aw;
     __syncthreads();
   }
-  int64_t p = sp[t] - np, c = sc[t] - nc;
+  int64_t p = sp[t] - np, c = sc[t] - nc, w = sw[t] - nw;
   for (int s = s0; s < s1; ++s) {
     cpref[s] = p;
     ccref[s] = c;
+    wpref[s] = w;
     if (seg_mode[s] != SEG_KEEP) {
       p += offsets[s + 1] - offsets[s];
       c += segp[s].grid.n_cells;
+      w += seg_waves(offsets, s);
     }
   }
   if (t == 1023) {
     cpref[n_seg] = sp[t];
     ccref[n_seg] = sc[t];
+    wpref[n_seg] = sw[t];
   }
+}
+
+// The flagged system that holds compact index k of a prefix over the
+// systems (cpref, wpref): the last s with pref[s] <= k (systems that are
+// not flagged add nothing to the prefix, so the next flagged one shares
+// their value and comes after them).
+JD int seg_of_compact(const int64_t* __restrict__ pref, int n_seg, int64_t k) {
+  int lo = 0, hi = n_seg - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (pref[mid] <= k)
+      lo = mid;
+    else
+      hi = mid - 1;
+  }
+  return lo;
 }
 
 // compact key of every particle of a flagged system, at its compact index
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_hash_flagged(RB r) {
   if (r.gate && *r.gate == 0) return;
-  for (int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; b < r.n; b += (int64_t)gridDim.x * blockDim.x) {
-  const int s = r.seg[b];
-  if (r.seg_mode[s] == SEG_KEEP) continue;
+  const int64_t fn = r.cpref[r.n_seg];
+  for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < fn; k += (int64_t)gridDim.x * blockDim.x) {
+  const int s = seg_of_compact(r.cpref, r.n_seg, k);
+  const int64_t b = r.offsets[s] + (k - r.cpref[s]);
   const SegP& sp = r.segp[s];
   const float4 p = r.P[b];
   const float x[3] = {p.x, p.y, p.z};
@@ -626,10 +657,22 @@ __global__ void __launch_bounds__(kBlock) k_nlist_work(NLW a) {
   __shared__ int row_max;
   const int tid = threadIdx.x;
   // one tile of kBlock slots per workgroup, or (a grid smaller than the
-  // tiles, as batches launch it) several in turn
-  const int64_t ntiles = (a.n + kBlock - 1) / kBlock;
+  // tiles, as batches launch it) several in turn.  A batch with wpref walks
+  // the wavefronts of the flagged systems only: a tile is four of them.
+  const bool by_wave = PM == 2 && a.wpref;
+  const int64_t nwaves = by_wave ? a.wpref[a.n_seg] : 0;
+  const int64_t ntiles = by_wave ? (nwaves + kBlock / 64 - 1) / (kBlock / 64) : (a.n + kBlock - 1) / kBlock;
   for (int64_t tile = xcd_tile(blockIdx.x, gridDim.x); tile < ntiles; tile += gridDim.x) {
   int64_t b = tile * blockDim.x + tid;
+  if (by_wave) {
+    const int64_t cw = tile * (kBlock / 64) + (tid >> 6);
+    if (cw < nwaves) {
+      const int s = seg_of_compact(a.wpref, a.n_seg, cw);
+      b = ((a.offsets[s] >> 6) + (cw - a.wpref[s])) * 64 + (tid & 63);
+    } else {
+      b = a.n;
+    }
+  }
   // a batch workgroup whose systems all keep their rows has nothing to do
   if (PM == 2 && !__syncthreads_or(b < a.n && a.seg_mode[a.seg[b]] != SEG_KEEP)) continue;
   if (tid == 0) row_max = 0;
@@ -787,6 +830,57 @@ __global__ void __launch_bounds__(kBlock) k_pack_springs(int64_t n, int M, int W
     float4* dst = wave + (b & ~(int64_t)63) * M + off;
     for (int k = 0; k < cnt; ++k) dst[k] = rank[(int64_t)k * n + b];
     W[b].w = __int_as_float(off);
+  }
+}
+
+// The same conversions in a batch rebuild, over the wavefronts that hold a
+// flagged system's particles only (a wavefront shared by two flagged
+// systems is converted twice, to the same values).
+__global__ void __launch_bounds__(kBlock) k_pack_springs_seg(int64_t n, int M, int W64, const float4* __restrict__ rank,
+                                                              const uint64_t* __restrict__ live,
+                                                              float4* __restrict__ wave, float4* __restrict__ W,
+                                                              const int64_t* __restrict__ wpref,
+                                                              const int64_t* __restrict__ offsets, int n_seg,
+                                                              const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t nt = wpref[n_seg] * 64;  // whole waves: every lane of a wave loops alike
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += stride) {
+    const int64_t cw = t >> 6;
+    const int s = seg_of_compact(wpref, n_seg, cw);
+    const int64_t b = ((offsets[s] >> 6) + (cw - wpref[s])) * 64 + lane;
+    int cnt = 0;
+    if (b < n)
+      for (int w = 0; w < W64; ++w) cnt += __popcll(live[(int64_t)w * n + b]);
+    const int off = wave_exclusive_scan(cnt, lane);
+    if (b >= n) continue;
+    float4* dst = wave + (b & ~(int64_t)63) * M + off;
+    for (int k = 0; k < cnt; ++k) dst[k] = rank[(int64_t)k * n + b];
+    W[b].w = __int_as_float(off);
+  }
+}
+
+__global__ void __launch_bounds__(kBlock) k_unpack_springs_seg(int64_t n, int M, int W64,
+                                                                const float4* __restrict__ wave,
+                                                                const uint64_t* __restrict__ live,
+                                                                const float4* __restrict__ W, float4* __restrict__ rank,
+                                                                const int64_t* __restrict__ wpref,
+                                                                const int64_t* __restrict__ offsets, int n_seg,
+                                                                const int32_t* __restrict__ gate) {
+  if (gate && *gate == 0) return;
+  const int lane = threadIdx.x & 63;
+  const int64_t nt = wpref[n_seg] * 64;
+  const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nt; t += stride) {
+    const int64_t cw = t >> 6;
+    const int s = seg_of_compact(wpref, n_seg, cw);
+    const int64_t b = ((offsets[s] >> 6) + (cw - wpref[s])) * 64 + lane;
+    if (b >= n) continue;
+    int cnt = 0;
+    for (int w = 0; w < W64; ++w) cnt += __popcll(live[(int64_t)w * n + b]);
+    const float4* src = wave + (b & ~(int64_t)63) * M + __float_as_int(W[b].w);
+    for (int k = 0; k < cnt; ++k) rank[(int64_t)k * n + b] = src[k];
   }
 }
 
@@ -1206,18 +1300,29 @@ hipError_t launch_ws_dim(const WS& a, int law, bool per, hipStream_t s) {
 
 hipError_t ws_launch_pack_springs(int64_t n, int M, int W64, const float4* rank, const uint64_t* live, float4* wave,
                                   float4* W, const int32_t* seg, const int32_t* seg_mode, hipStream_t s,
-                                  const int32_t* gate) {
+                                  const int32_t* gate, const int64_t* wpref, const int64_t* offsets, int n_seg) {
   if (n == 0 || M == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_pack_springs, dim3(gate ? std::min<unsigned>(nblocks(n), (unsigned)kCappedGrid) : nblocks(n)),
-                     dim3(kBlock), 0, s, n, M, W64, rank, live, wave, W, seg, seg_mode, gate);
+  if (wpref)
+    hipLaunchKernelGGL(k_pack_springs_seg, dim3(std::min<unsigned>(nblocks(n + 64 * (int64_t)n_seg), (unsigned)kCappedGrid)),
+                       dim3(kBlock), 0, s, n, M, W64, rank, live, wave, W, wpref, offsets, n_seg, gate);
+  else
+    hipLaunchKernelGGL(k_pack_springs, dim3(gate ? std::min<unsigned>(nblocks(n), (unsigned)kCappedGrid) : nblocks(n)),
+                       dim3(kBlock), 0, s, n, M, W64, rank, live, wave, W, seg, seg_mode, gate);
   return hipGetLastError();
 }
 hipError_t ws_launch_unpack_springs(int64_t n, int M, int W64, const float4* wave, const uint64_t* live,
                                     const float4* W, float4* rank, const int32_t* seg, const int32_t* seg_mode,
-                                    hipStream_t s, const int32_t* gate) {
+                                    hipStream_t s, const int32_t* gate, const int64_t* wpref,
+                                    const int64_t* offsets, int n_seg) {
   if (n == 0 || M == 0) return hipSuccess;
-  hipLaunchKernelGGL(k_unpack_springs, dim3(gate ? std::min<unsigned>(nblocks(n), (unsigned)kCappedGrid) : nblocks(n)),
-                     dim3(kBlock), 0, s, n, M, W64, wave, live, W, rank, seg, seg_mode, gate);
+  if (wpref)
+    hipLaunchKernelGGL(k_unpack_springs_seg,
+                       dim3(std::min<unsigned>(nblocks(n + 64 * (int64_t)n_seg), (unsigned)kCappedGrid)), dim3(kBlock),
+                       0, s, n, M, W64, wave, live, W, rank, wpref, offsets, n_seg, gate);
+  else
+    hipLaunchKernelGGL(k_unpack_springs,
+                       dim3(gate ? std::min<unsigned>(nblocks(n), (unsigned)kCappedGrid) : nblocks(n)), dim3(kBlock),
+                       0, s, n, M, W64, wave, live, W, rank, seg, seg_mode, gate);
   return hipGetLastError();
 }
 
@@ -1341,8 +1446,8 @@ hipError_t ws_launch_permute(int64_t n, const int32_t* new_orig, const int32_t* 
   return hipGetLastError();
 }
 hipError_t ws_launch_seg_prefix(int n_seg, const int32_t* seg_mode, const int64_t* offsets, const SegP* segp,
-                               int64_t* cpref, int64_t* ccref, hipStream_t s) {
-  hipLaunchKernelGGL(k_seg_prefix, dim3(1), dim3(1024), 0, s, n_seg, seg_mode, offsets, segp, cpref, ccref);
+                               int64_t* cpref, int64_t* ccref, int64_t* wpref, hipStream_t s) {
+  hipLaunchKernelGGL(k_seg_prefix, dim3(1), dim3(1024), 0, s, n_seg, seg_mode, offsets, segp, cpref, ccref, wpref);
   return hipGetLastError();
 }
 hipError_t ws_launch_hash_flagged(const RB& r, int dim, hipStream_t s) {
