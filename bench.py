@@ -63,14 +63,15 @@ def parse_args():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-max-active", type=int, default=8, help="requests in the serving batch at most")
     ap.add_argument("--serve-policy", type=str, default="fcfs", help="serving admission: fcfs or sjf")
-    ap.add_argument("--serve-loads", type=str, default="",
+    ap.add_argument("--serve-loads", type=str, default="0.3,0.6,0.9",
                     help="also sweep offered loads, as fractions of the one-at-a-time capacity (e.g. 0.3,0.6,0.9)")
-    ap.add_argument("--serve-sweep-requests", type=int, default=24)
+    ap.add_argument("--serve-sweep-requests", type=int, default=16)
     ap.add_argument("--precision-n", type=int, default=1 << 18, help="spheres of the precision-form minimisation")
     ap.add_argument("--precision-steps", type=int, default=20000,
                     help="FIRE steps timed (0: run to the float32 floor, which stalls after ~1e5 steps)")
     ap.add_argument("--no-precision", action="store_true")
     ap.add_argument("--rollout-stride", type=int, default=50, help="steps between saved frames of the rollout line")
+    ap.add_argument("--rollout-steps", type=int, default=1000, help="steps of the rollout line")
     ap.add_argument("--no-rollout", action="store_true")
     ap.add_argument(
         "--copy-probe-mib",
@@ -452,7 +453,7 @@ def rollout_bench(args, st, sy, n_particles: int, plain_value: float) -> dict:
     import jaxdem_amd as jd
 
     k = max(1, args.rollout_stride)
-    nf = max(1, args.steps // k)
+    nf = max(1, args.rollout_steps // k)
     out = {"stride": k, "frames": nf, "steps": nf * k}
     for name, fn in (("device_frames", jd.save_state), ("step_per_frame", lambda s, y: (s, y))):
         torch.cuda.synchronize()

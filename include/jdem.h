@@ -422,7 +422,9 @@ typedef struct jdem_batch {
    * every step, long calls replayed from a captured hipGraph; 1 = the same
    * chain launched step by step; 2 = speculative chunks of step kernels with
    * one host read per chunk, the chain launched only when a system asked for
-   * a rebuild (fewer launches when rebuilds are rare) */
+   * a rebuild (fewer launches when rebuilds are rare); once two rebuilds in
+   * a row come spec_min_gap steps or fewer after the previous one, the rest
+   * of the call runs as mode 1 */
   int32_t launch_mode;
   int32_t* n_build_times; /* in/out: NeighborList builds of each system       */
   const int64_t* offsets;
@@ -433,7 +435,7 @@ typedef struct jdem_batch {
   const float* skin;
   int32_t tile_width; /* particles per workgroup of the step kernel: 256
                        * (also for 0) or 128 */
-  int32_t reserved;
+  int32_t spec_min_gap; /* launch mode 2: see above; 0 = 2 steps */
 } jdem_batch;
 
 /* System.step for a batch of systems of unequal size in one segmented

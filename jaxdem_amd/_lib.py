@@ -185,7 +185,7 @@ class JdemBatch(ctypes.Structure):
         ("cutoff", c_void_p),
         ("skin", c_void_p),
         ("tile_width", c_int32),
-        ("reserved", c_int32),
+        ("spec_min_gap", c_int32),
     ]
 
 

@@ -102,6 +102,9 @@ class Batch:
     launch_mode: int = 0
     # particles per workgroup of the step kernel (256 or 128)
     tile_width: int = 256
+    # launch mode 2 runs the rest of a call as mode 1 once two rebuilds in a
+    # row come this many steps or fewer apart (0: 2)
+    spec_min_gap: int = 0
 
     def __len__(self) -> int:
         return len(self.systems)
@@ -215,7 +218,7 @@ class Batch:
         hist = None if col.history is None else {k: v.to(device) for k, v in col.history.items()}
         col2 = replace(col, neighbor_list=col.neighbor_list.to(device), old_pos=col.old_pos.to(device), history=hist)
         return Batch(State(**kw), replace(self.system, collider=col2), self.offsets, self.mat_offsets, self.systems,
-                     self.bond_widths, list(self.n_build), self.launch_mode, self.tile_width)
+                     self.bond_widths, list(self.n_build), self.launch_mode, self.tile_width, self.spec_min_gap)
 
     def _c_arrays(self, keep: list) -> L.JdemBatch:
         """The jdem_batch of the systems (their domains, grids, dt, cutoffs,
@@ -231,6 +234,7 @@ class Batch:
         b.n_systems = len(self.systems)
         b.launch_mode = int(self.launch_mode)
         b.tile_width = int(self.tile_width)
+        b.spec_min_gap = int(self.spec_min_gap)
         b.n_build_times = nb.ctypes.data
         b.offsets = off.ctypes.data
         b.domains = ctypes.addressof(doms)
@@ -288,7 +292,7 @@ class Batch:
                 clocks[key] = _clock_after(s.time, s.dt, int(n))
             systems.append(replace(s, time=clocks[key], step_count=s.step_count + int(n)))
         out = Batch(st, sy2, self.offsets, self.mat_offsets, systems, self.bond_widths, n_build, self.launch_mode,
-                    self.tile_width)
+                    self.tile_width, self.spec_min_gap)
         if "_c_cache" in self.__dict__:
             out.__dict__["_c_cache"] = self.__dict__["_c_cache"]
         return out

@@ -577,6 +577,7 @@ struct SegRun {
   bool any_unbuilt;   // some system has no list yet
   int launch_mode;    // jdem_batch.launch_mode
   int tile;           // jdem_batch.tile_width
+  int spec_min_gap;   // jdem_batch.spec_min_gap (0 -> 2)
 };
 
 // n steps of a NeighborList system on the resident working set.  With
@@ -1067,9 +1068,11 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       // a rebuild; the host reads the words once per chunk, runs the rebuild
       // chain (its gate raised from the host) and resumes at the flagged
       // step.  Once rebuilds come close together (two intervals in a row
-      // of kSpecMin steps or fewer) the rest of the call runs the gated
-      // chain instead, which needs no host read.
-      constexpr int64_t kSpecMin = 8;
+      // of spec_min_gap steps or fewer) the rest of the call runs the gated
+      // chain instead, which needs no host read.  A rebuild found by a host
+      // read costs about two idle chains (the read and the chain), so
+      // speculation pays while rebuilds are more than ~2 steps apart.
+      const int64_t kSpecMin = sg->spec_min_gap;
       int rc2 = launch_t(0);
       if (rc2) return rc2;
       if (sg->any_unbuilt) {  // systems without a list build before the first force
@@ -1689,6 +1692,7 @@ int jdem_batch_step(jdem_ctx* ctx, jdem_system* sys, const jdem_batch* batch, in
   if (batch->launch_mode < 0 || batch->launch_mode > 2) return fail(ctx, JDEM_EINVAL, "launch_mode must be 0, 1 or 2");
   if (batch->tile_width != 0 && batch->tile_width != 128 && batch->tile_width != 256)
     return fail(ctx, JDEM_EINVAL, "tile_width must be 128 or 256");
+  if (batch->spec_min_gap < 0) return fail(ctx, JDEM_EINVAL, "spec_min_gap must be >= 0");
   if (sys->collider_kind != JDEM_COLLIDER_NEIGHBOR_LIST || sys->nlist.secondary != JDEM_COLLIDER_CELL_LIST)
     return fail(ctx, JDEM_EUNSUPPORTED, "batches run the NeighborList collider over a CellList");
   const int B = batch->n_systems;
@@ -1750,6 +1754,7 @@ int jdem_batch_step(jdem_ctx* ctx, jdem_system* sys, const jdem_batch* batch, in
   sg.any_unbuilt = any_unbuilt;
   sg.launch_mode = batch->launch_mode;
   sg.tile = batch->tile_width == 128 ? 128 : 256;
+  sg.spec_min_gap = batch->spec_min_gap > 0 ? batch->spec_min_gap : 2;
   jdem_step_stats local;
   jdem_step_stats* S = stats ? stats : &local;
   std::memset(S, 0, sizeof(*S));

@@ -79,7 +79,7 @@ class Server:
 
     def __init__(self, device, hbm_budget: int | None = None, chunk: int = 50, max_active: int = 4096,
                  clock: Callable[[], float] = time.perf_counter, chunk_ms: float = 20.0, policy: str = "fcfs",
-                 launch_mode: int = 2):
+                 launch_mode: int = 2, spec_min_gap: int = 0):
         if policy not in ("fcfs", "sjf"):
             raise ValueError(f"unknown admission policy {policy!r} (fcfs, sjf)")
         if int(max_active) < 1:
@@ -97,6 +97,7 @@ class Server:
         # a served batch holds a few systems that rebuild rarely: speculative
         # chunks launch fewer kernels than a rebuild chain after every step
         self.launch_mode = int(launch_mode)
+        self.spec_min_gap = int(spec_min_gap)
         self.log: list = []
 
     def run(self, requests: Sequence[Request], realtime: bool = True) -> list:
@@ -145,7 +146,7 @@ class Server:
                 pairs = (batch.unpack() if batch is not None else []) + [(r.state, r.system) for r in admit]
                 active = active + admit
                 batch = Batch.pack(pairs)
-                batch.launch_mode = self.launch_mode
+                batch.launch_mode, batch.spec_min_gap = self.launch_mode, self.spec_min_gap
                 ms_per_step = None
             if not active:
                 if pending and realtime:
@@ -175,7 +176,7 @@ class Server:
                 active = [r for r, _ in keep]
                 batch = Batch.pack([p for _, p in keep]) if keep else None
                 if batch is not None:
-                    batch.launch_mode = self.launch_mode
+                    batch.launch_mode, batch.spec_min_gap = self.launch_mode, self.spec_min_gap
                 ms_per_step = None
         return done
 

@@ -62,7 +62,7 @@ STRUCT_FIELDS = {
     "jdem_step_stats": ["steps", "rebuilds", "host_syncs", "overflow", "reserved", "free_box", "free_anchor"],
     "jdem_timing": ["step_kernel_ms", "step_kernels", "idle_kernel_ms", "idle_kernels", "rebuild_ms", "rebuilds"],
     "jdem_batch": ["n_systems", "launch_mode", "n_build_times", "offsets", "domains", "grids", "dt", "cutoff", "skin",
-                   "tile_width", "reserved"],
+                   "tile_width", "spec_min_gap"],
     "jdem_fire": ["dt", "alpha_init", "f_inc", "f_dec", "f_alpha", "n_min", "n_bad_max", "dt_max_scale",
                   "dt_min_scale"],
     "jdem_minimize_stats": ["steps", "pe", "max_grad", "power", "overflow", "reserved"],
