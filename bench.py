@@ -55,6 +55,9 @@ def parse_args():
     ap.add_argument("--batch-cell-factors", type=str, default="1.0,0.5,1.5",
                     help="cell sizes of the batch list build, in list radii, tried by the autotuner")
     ap.add_argument("--batch-probe-every", type=int, default=4, help="the autotuner probes every k-th system")
+    ap.add_argument("--batch-mode", type=int, default=-1,
+                    help="batch launch mode 0/1/2 (with --batch-tile: no launch autotuning)")
+    ap.add_argument("--batch-tile", type=int, default=0, help="batch step-kernel tile width 128/256")
     ap.add_argument("--batch-probe-steps", type=int, default=200)
     ap.add_argument("--batch-probe-warm", type=int, default=300,
                     help="untimed steps before each autotuner probe (the lattice start melts first)")
@@ -100,10 +103,10 @@ def listed_pairs(sysm) -> int:
     return int((sysm.collider.neighbor_list >= 0).sum().item())
 
 
-def traffic_from_profiles() -> tuple:
-    """HBM bytes per launch of the fused step kernel from the committed
-    rocprofv3 PMC summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)."""
-    p = ROOT / "profiles" / "fused_step_spheres_1m_friction_pmc.txt"
+def traffic_from_profiles(name: str = "fused_step_spheres_1m_friction_pmc.txt") -> tuple:
+    """HBM bytes per launch of a step kernel from a committed rocprofv3 PMC
+    summary (FETCH_SIZE + WRITE_SIZE, copy-kernel scaled)."""
+    p = ROOT / "profiles" / name
     if not p.exists():
         return None, None
     m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
@@ -177,15 +180,18 @@ def batch_bench(args, dev) -> dict:
     from jaxdem_amd import workloads as W
 
     skin, cell_factor, tune = 0.1, 1.0, []
-    if args.batch_skins:
+    skins = [float(x) for x in args.batch_skins.split(",") if x]
+    factors = [float(x) for x in args.batch_cell_factors.split(",") if x] or [1.0]
+    if len(skins) == 1 and len(factors) == 1:
+        skin, cell_factor = skins[0], factors[0]
+    elif skins:
         from jaxdem_amd.batching import autotune
 
         def sample(sk):
             return [W.make_gpu_system(w, device="cpu")
                     for w in W.batch_disks(args.batch_systems, seed=args.seed, skin=sk, every=args.batch_probe_every)]
 
-        cfg, tune = autotune(sample, [float(x) for x in args.batch_skins.split(",") if x],
-                             cell_factors=[float(x) for x in args.batch_cell_factors.split(",") if x],
+        cfg, tune = autotune(sample, skins, cell_factors=factors,
                              tiles=(), modes=(), probe_steps=args.batch_probe_steps,
                              warm_steps=args.batch_probe_warm, device=dev)
         skin, cell_factor = cfg["skin"], cfg["cell_factor"]
@@ -201,9 +207,12 @@ def batch_bench(args, dev) -> dict:
     prep = time.perf_counter() - t0
     N = b.n_particles
     print(f"[bench] batch: {len(ws)} systems, {N} disks, packed in {prep:.1f} s", file=sys.stderr, flush=True)
-    launch_cfg, launch_tune = tune_launch(b, probe_steps=args.batch_probe_steps, warm_steps=args.batch_probe_warm)
-    tune = tune + launch_tune
-    print(f"[bench] batch launch autotuner: {launch_cfg} from {launch_tune}", file=sys.stderr, flush=True)
+    if args.batch_mode >= 0 and args.batch_tile > 0:
+        b.launch_mode, b.tile_width = args.batch_mode, args.batch_tile
+    else:
+        launch_cfg, launch_tune = tune_launch(b, probe_steps=args.batch_probe_steps, warm_steps=args.batch_probe_warm)
+        tune = tune + launch_tune
+        print(f"[bench] batch launch autotuner: {launch_cfg} from {launch_tune}", file=sys.stderr, flush=True)
     b = b.step(20)  # first list builds, not timed
     torch.cuda.synchronize()
     t0 = time.perf_counter()
@@ -270,7 +279,8 @@ def batch_bench(args, dev) -> dict:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": traffic_from_profiles("batch_disks_4096_pmc.txt")[0],
+            "traffic_source": traffic_from_profiles("batch_disks_4096_pmc.txt")[1],
             "kernel": "k_ws_step<2, HERTZ_FRICTION, segmented>",
             "kernel_avg_us": 1e3 * avg_ms,
             "algorithmic_bytes_per_disk": bytes_per_disk,
@@ -363,14 +373,14 @@ def precision_bench(args, dev) -> dict:
     secs = time.perf_counter() - t
     print(f"[bench] precision: {r['steps']} steps in {secs:.2f} s, max|F| {r['max_grad']:.3e}", file=sys.stderr,
           flush=True)
-    # roofline of the force kernel: a short run of the same start with HIP
-    # events around every k_prec_force launch
+    # roofline of the force kernel: the same steps again with HIP events
+    # around every k_prec_force launch
     from jaxdem_amd import _lib as Lb
 
     ctx = Lb.context_for(dev)
     ctx.set_timing(True)
-    PR.minimize_precise(pos, rad, L, k=1.0, dt=1e-2, max_steps=200, pe_tol=0.0, pe_diff_tol=-1.0, force_tol=0.0,
-                        device=dev)
+    PR.minimize_precise(pos, rad, L, k=1.0, dt=1e-2, max_steps=int(r["steps"]), pe_tol=0.0, pe_diff_tol=-1.0,
+                        force_tol=0.0, device=dev)
     tm = ctx.get_timing()
     ctx.set_timing(False)
     avg_ms = tm["step_kernel_ms"] / max(tm["step_kernels"], 1)

@@ -2,8 +2,9 @@
 arrive and advanced together on the GPU (continuous batching).
 
 Every active request is one system of a segmented batch
-(``jaxdem_amd.batching``); its state, cell list and tangential-spring
-history stay in device memory between step chunks.  At each chunk boundary
+(``jaxdem_amd.batching``), or steps alone with ``System.step`` while it is
+the only one; its state, cell list and tangential-spring history stay in
+device memory between step chunks.  At each chunk boundary
 finished requests leave and arrived requests are admitted (first come first
 served, or least work first) while their device footprint fits the memory
 budget and the batch holds fewer than ``max_active`` requests.  A chunk runs
@@ -30,6 +31,7 @@ import numpy as np
 import torch
 
 from .batching import Batch
+from .system import System
 
 
 @dataclass
@@ -116,6 +118,7 @@ class Server:
         now = lambda: self.clock() - t0  # noqa: E731
         used = 0
         done: list = []
+        single = None  # (state, system) of a request that runs alone
         ms_per_step = None  # of the current composition, from the last chunk
         while pending or queue or active:
             t = now()
@@ -143,10 +146,9 @@ class Server:
                 used += fp
                 admit.append(r)
             if admit:
-                pairs = (batch.unpack() if batch is not None else []) + [(r.state, r.system) for r in admit]
+                pairs = self._pairs(batch, single) + [(r.state, r.system) for r in admit]
                 active = active + admit
-                batch = Batch.pack(pairs)
-                batch.launch_mode, batch.spec_min_gap = self.launch_mode, self.spec_min_gap
+                batch, single = self._hold(pairs)
                 ms_per_step = None
             if not active:
                 if pending and realtime:
@@ -155,14 +157,17 @@ class Server:
             want = self.chunk if ms_per_step is None else max(1, int(self.chunk_ms / max(ms_per_step, 1e-6)))
             n = min(want, min(r.steps - r.done_steps for r in active))
             t_c = self.clock()
-            batch = batch.step(n)
+            if single is not None:
+                single = System.step(*single, n=n)
+            else:
+                batch = batch.step(n)
             torch.cuda.synchronize(self.device)
             ms_per_step = (self.clock() - t_c) * 1e3 / n
             for r in active:
                 r.done_steps += n
-            self.log.append((now(), len(active), int(batch.n_particles), n))
+            self.log.append((now(), len(active), int(batch.n_particles if batch is not None else single[0].N), n))
             if any(r.done_steps >= r.steps for r in active):
-                cur = batch.unpack()
+                cur = self._pairs(batch, single)
                 t_f = now()
                 keep = []
                 for r, pair in zip(active, cur):
@@ -174,19 +179,33 @@ class Server:
                     else:
                         keep.append((r, pair))
                 active = [r for r, _ in keep]
-                batch = Batch.pack([p for _, p in keep]) if keep else None
-                if batch is not None:
-                    batch.launch_mode, batch.spec_min_gap = self.launch_mode, self.spec_min_gap
+                batch, single = self._hold([p for _, p in keep])
                 ms_per_step = None
         return done
+
+    @staticmethod
+    def _pairs(batch, single) -> list:
+        if single is not None:
+            return [single]
+        return batch.unpack() if batch is not None else []
+
+    def _hold(self, pairs: list) -> tuple:
+        """(batch, single) holding the active requests: one request steps
+        alone with System.step (a batch of one only adds the segmented
+        kernels' costs), several in one batch."""
+        if not pairs:
+            return None, None
+        if len(pairs) == 1:
+            return None, pairs[0]
+        batch = Batch.pack(pairs)
+        batch.launch_mode, batch.spec_min_gap = self.launch_mode, self.spec_min_gap
+        return batch, None
 
 
 def run_one_at_a_time(requests: Sequence[Request], device, realtime: bool = True,
                       clock: Callable[[], float] = time.perf_counter) -> list:
     """The same requests served one after another with System.step, in
     arrival order (the plain kernel, no batching)."""
-    from .system import System
-
     out = []
     t0 = clock()
     for r0 in sorted(requests, key=lambda r: (r.arrival, r.rid)):
