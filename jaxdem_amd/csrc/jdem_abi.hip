@@ -737,6 +737,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     r.n = N;
     r.fn = -1;  // the flagged-particle count is cpref[n_seg] on the device
     r.n_seg = sg->n_seg;
+    r.dim = D;
     r.gate = gate;
     r.M = M;
     r.W64 = W64;
@@ -848,7 +849,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
       HIPCHK(ctx, ws_launch_permute(N, new_orig, I32(ctx->wWpos), F4(ctx->wP[cur]), F4(ctx->wV[cur]),
                                     F4(ctx->wW[cur]), F4(ctx->wQ[set]), F4(ctx->wI[set]), F4(ctx->wP[1 - cur]),
                                     F4(ctx->wV[1 - cur]), F4(ctx->wW[1 - cur]), F4(ctx->wQ[1 - set]),
-                                    F4(ctx->wI[1 - set]), I32(ctx->wOrig[1 - set]), src, inv, st, F4(ctx->wOld)));
+                                    F4(ctx->wI[1 - set]), I32(ctx->wOrig[1 - set]), src, inv, st, F4(ctx->wOld), D));
       cur = 1 - cur;
       set = 1 - set;
       if (!counted) HIPCHK(ctx, ws_launch_cell_starts(sk, N, n_cells, I32(ctx->cstart), st));

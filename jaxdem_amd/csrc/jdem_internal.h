@@ -387,6 +387,7 @@ enum { SEG_KEEP = 0, SEG_FRESH = 1, SEG_REMAP = 2 };
 struct RB {
   int64_t n, fn;  // particles; flagged particles (-1: read on the device, cpref[n_seg])
   int n_seg;
+  int dim;
   const int32_t* gate;  // when set, every kernel returns at once while *gate == 0
   int M, W64;
   const int32_t* seg;
@@ -508,7 +509,7 @@ hipError_t ws_launch_seg_ids(const int64_t* offsets, int n_seg, int64_t n, int32
 hipError_t ws_launch_permute(int64_t n, const int32_t* new_orig, const int32_t* wpos, const float4* P,
                              const float4* V, const float4* W, const float4* Q, const float4* I, float4* P2,
                              float4* V2, float4* W2, float4* Q2, float4* I2, int32_t* orig2, int32_t* src,
-                             int32_t* inv, hipStream_t s, float4* old2 = nullptr);
+                             int32_t* inv, hipStream_t s, float4* old2 = nullptr, int dim = 3);
 // batch rebuilds: per-system mode from the flags (and bump the build counts),
 // and the flags of never-built systems
 // With gate set, *events counts the calls that find *gate != 0 (rebuilds).

@@ -5,6 +5,10 @@
 // Layout in HBM for the duration of one jdem_system_step call:
 //   P4[b] = (x, y, z, r)   V4[b] = (v, m)   W4[b] = (omega)   Q4[b] = q
 //   I4[b] = (inertia, bits: species << 16 | mat_id << 1 | fixed)
+// 2D records use their free lanes so that the step kernel reads neither I4
+// nor old4: P4 = (x, y, x_build, r), V4 = (vx, vy, y_build, m),
+// W4 = (omega, inertia, bits, spring offset).  I4 and old4 are kept as in 3D
+// for the other kernels.
 // in "work" order b: particles sorted by (cell, original id) at every list
 // rebuild, so a particle's candidates are its memory neighbours.  orig[b] is
 // the original index of work slot b.  The neighbour list is slot-major
@@ -74,9 +78,21 @@ __global__ void __launch_bounds__(kBlock) k_in_state(InOut a) {
     I[c] = a.inertia[A * i + c];
   }
   const int bits = ((a.species ? a.species[i] : 0) << 16) | ((a.mat_id ? a.mat_id[i] : 0) << 1) | (a.fixed[i] ? 1 : 0);
-  a.P4[b] = make_float4(p[0], p[1], p[2], a.rad[i]);
-  a.V4[b] = make_float4(v[0], v[1], v[2], a.mass[i]);
-  a.W4[b] = make_float4(w[0], w[1], w[2], 0.0f);
+  if (D == 2) {
+    float ox = o[0], oy = o[1];
+    if (a.old_keep) {
+      const float4 ok = a.old4[b];
+      ox = ok.x;
+      oy = ok.y;
+    }
+    a.P4[b] = make_float4(p[0], p[1], ox, a.rad[i]);
+    a.V4[b] = make_float4(v[0], v[1], oy, a.mass[i]);
+    a.W4[b] = make_float4(w[0], I[0], __int_as_float(bits), 0.0f);
+  } else {
+    a.P4[b] = make_float4(p[0], p[1], p[2], a.rad[i]);
+    a.V4[b] = make_float4(v[0], v[1], v[2], a.mass[i]);
+    a.W4[b] = make_float4(w[0], w[1], w[2], 0.0f);
+  }
   a.Q4[b] = reinterpret_cast<const float4*>(a.q)[i];
   a.I4[b] = make_float4(I[0], I[1], I[2], __int_as_float(bits));
   a.F4[b] = make_float4(f[0], f[1], f[2], 0.0f);
@@ -267,15 +283,22 @@ __global__ void __launch_bounds__(kBlock) k_permute(int64_t n, const int32_t* __
                                                      float4* __restrict__ W2, float4* __restrict__ Q2,
                                                      float4* __restrict__ I2, int32_t* __restrict__ orig2,
                                                      int32_t* __restrict__ src_out, int32_t* __restrict__ inv_out,
-                                                     float4* __restrict__ old2) {
+                                                     float4* __restrict__ old2, int dim) {
   const int64_t b = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (b >= n) return;
   const int32_t i = new_orig[b];
   const int32_t s = wpos[i];
-  const float4 p = P[s];
+  float4 p = P[s];
+  float4 v = V[s];
+  if (old2) {
+    old2[b] = p;  // the positions of this build
+    if (dim == 2) {  // and in the 2D records' free lanes
+      p.z = p.x;
+      v.z = p.y;
+    }
+  }
   P2[b] = p;
-  if (old2) old2[b] = p;  // the positions of this build
-  V2[b] = V[s];
+  V2[b] = v;
   W2[b] = W[s];
   Q2[b] = Q[s];
   I2[b] = I[s];
@@ -456,14 +479,19 @@ __global__ void __launch_bounds__(kBlock) k_place_scatter(RB r) {
   for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < fn; k += (int64_t)gridDim.x * blockDim.x) {
   const int s = r.cs[k];
   const int64_t nb = r.offsets[s] + (k - r.cpref[s]);
-  const float4 p = r.tP[k];
+  float4 p = r.tP[k];
+  float4 v = r.tV[k];
+  r.old4[nb] = p;  // build position
+  if (r.dim == 2) {  // and in the 2D records' free lanes
+    p.z = p.x;
+    v.z = p.y;
+  }
   r.P[nb] = p;
-  r.V[nb] = r.tV[k];
+  r.V[nb] = v;
   r.W[nb] = r.tW[k];
   r.Q[nb] = r.tQ[k];
   r.I[nb] = r.tI[k];
   r.orig[nb] = r.torig[k];
-  r.old4[nb] = p;  // build position
   r.cidx[nb] = (int32_t)k;
   }
 }
@@ -1128,7 +1156,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(ws_wave
           mo.sp = 0;
           o.sp = 0;
           if (a.law.n_mat > 1 || (LawTraits<LAW>::routed && a.law.n_species > 1)) {
-            const int bo_bits = __float_as_int(a.I[bo].w), j_bits = __float_as_int(a.I[j].w);
+            const int bo_bits = D == 2 ? __float_as_int(a.W[bo].z) : __float_as_int(a.I[bo].w);
+            const int j_bits = D == 2 ? __float_as_int(a.W[j].z) : __float_as_int(a.I[j].w);
             mo.mat = (bo_bits >> 1) & kMatMask;
             o.mat = (j_bits >> 1) & kMatMask;
             mo.sp = (int)((unsigned)bo_bits >> 16);
@@ -1189,7 +1218,8 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(ws_wave
     T = ang4<D>(a.T[b]);
   }
   if (!active) return;
-  const float4 p4b = a.P[b], v4 = a.V[b], w4 = a.W[b], i4 = a.I[b];
+  const float4 p4b = a.P[b], v4 = a.V[b], w4 = a.W[b];
+  const float4 i4 = D == 2 ? make_float4(w4.y, 0.0f, 0.0f, w4.z) : a.I[b];
   const int bits = __float_as_int(i4.w);
   Vec<D> x = xyz<D>(p4b), v = xyz<D>(v4);
   Ang<D> w = ang4<D>(w4);
@@ -1234,9 +1264,16 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(ws_wave
     }
     if (a.rot == JDEM_ROT_VERLET_SPIRAL) vvs_before<D>(q, w, T, I, dt, freef);
   }
-  a.P2[b] = f4(x, p4b.w);
-  a.V2[b] = f4(v, m);
+  float4 p_out = f4(x, p4b.w), v_out = f4(v, m);
   float4 w_out = f4a(w);
+  if constexpr (D == 2) {  // build position, inertia and bits ride along
+    p_out.z = p4b.z;
+    v_out.z = v4.z;
+    w_out.y = w4.y;
+    w_out.z = w4.z;
+  }
+  a.P2[b] = p_out;
+  a.V2[b] = v_out;
   // the springs' place travels with the angular velocity (its free lane)
   w_out.w = __int_as_float(spring_off >= 0 ? spring_off : __float_as_int(w4.w));
   a.W2[b] = w_out;
@@ -1246,7 +1283,12 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(ws_wave
     a.T[b] = f4a(T);
   }
   if (a.do_before && a.check_disp) {
-    const Vec<D> d = x - xyz<D>(a.old4[b]);
+    Vec<D> xb;
+    if constexpr (D == 2)
+      xb = Vec<2>{p4b.z, v4.z};
+    else
+      xb = xyz<D>(a.old4[b]);
+    const Vec<D> d = x - xb;
     if (dot(d, d) > trigger) {
       *a.flag_wr = a.step_idx + 1;
       if (PM == 2) a.seg_flag[a.seg[b]] = 1;
@@ -1440,9 +1482,9 @@ hipError_t ws_launch_seg_ids(const int64_t* offsets, int n_seg, int64_t n, int32
 hipError_t ws_launch_permute(int64_t n, const int32_t* new_orig, const int32_t* wpos, const float4* P,
                              const float4* V, const float4* W, const float4* Q, const float4* I, float4* P2,
                              float4* V2, float4* W2, float4* Q2, float4* I2, int32_t* orig2, int32_t* src,
-                             int32_t* inv, hipStream_t s, float4* old2) {
+                             int32_t* inv, hipStream_t s, float4* old2, int dim) {
   hipLaunchKernelGGL(k_permute, dim3(nblocks(n)), dim3(kBlock), 0, s, n, new_orig, wpos, P, V, W, Q, I, P2, V2, W2,
-                     Q2, I2, orig2, src, inv, old2);
+                     Q2, I2, orig2, src, inv, old2, dim);
   return hipGetLastError();
 }
 hipError_t ws_launch_seg_prefix(int n_seg, const int32_t* seg_mode, const int64_t* offsets, const SegP* segp,
