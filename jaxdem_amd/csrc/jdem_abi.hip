@@ -383,7 +383,8 @@ int build_cells(jdem_ctx* ctx, const float* pos, const float* rad, int64_t n, in
 }
 
 // Device words in ctx->flags: [0], [1] rebuild-pending ping-pong, [2] list
-// overflow, [3] displacement flag of the standalone check.
+// overflow, [3] displacement flag of the standalone check, [4] rebuild gate
+// and [5] rebuild events of a batch, [6] default-inertia flag of a call.
 int ensure_flags(jdem_ctx* ctx) {
   ENSURE(ctx, ctx->flags, 64);
   if (!ctx->host_flags) {
@@ -687,6 +688,14 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   io.nl = I32(ctx->wNl[0]);
   io.xi = F4(ctx->wXiRank);
   io.live = U64(ctx->wLive[0]);
+  // 3D spheres with one material, none fixed and the default inertia skip
+  // the inertia record in the step kernel (flags[6], decided at entry)
+  int32_t* iso = nullptr;
+  if (D == 3 && sys->law.n_mat == 1 && !io.species) {
+    iso = (int32_t*)ctx->flags.p + 6;
+    HIPCHK(ctx, launch_fill_i32(iso, 1, 1, st));
+  }
+  io.iso_flag = iso;
   if (R) {
     io.old_user = nullptr;
     io.old_keep = 1;
@@ -931,6 +940,7 @@ int resident_nl_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     a.n = N;
     a.M = M;
     a.W64 = W64;
+    a.iso = iso;
     a.law = law_of(ctx, sys->law);
     a.dom = dom;
     a.dt = sys->dt;

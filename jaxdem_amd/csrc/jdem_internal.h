@@ -317,6 +317,9 @@ struct InOut {
   float4* xi;
   uint64_t* live;
   int old_keep;  // entry: old4 already holds the build positions
+  // entry: cleared when some sphere is fixed or its inertia differs from
+  // (0.4 m) r^2 in any component (3D; set to 1 before the entry)
+  int32_t* iso_flag;
 };
 
 // Parameters of one system of a segmented batch (jdem_batch_step).  The
@@ -435,6 +438,10 @@ struct WS {
   int64_t n;
   int M;
   int W64;
+  // 3D: device word, nonzero when every sphere is free with inertia (0.4 m) r^2
+  // in each component and one material: the kernel computes the inertia
+  // record instead of reading it
+  const int32_t* iso;
   LawP law;
   DomP dom;
   float dt;

@@ -78,6 +78,10 @@ __global__ void __launch_bounds__(kBlock) k_in_state(InOut a) {
     I[c] = a.inertia[A * i + c];
   }
   const int bits = ((a.species ? a.species[i] : 0) << 16) | ((a.mat_id ? a.mat_id[i] : 0) << 1) | (a.fixed[i] ? 1 : 0);
+  if (a.iso_flag && D == 3) {
+    const float r = a.rad[i], Is = (0.4f * a.mass[i]) * (r * r);
+    if (a.fixed[i] || I[0] != Is || I[1] != Is || I[2] != Is) *a.iso_flag = 0;
+  }
   if (D == 2) {
     float ox = o[0], oy = o[1];
     if (a.old_keep) {
@@ -1219,7 +1223,15 @@ __global__ void __launch_bounds__(BS) __attribute__((amdgpu_waves_per_eu(ws_wave
   }
   if (!active) return;
   const float4 p4b = a.P[b], v4 = a.V[b], w4 = a.W[b];
-  const float4 i4 = D == 2 ? make_float4(w4.y, 0.0f, 0.0f, w4.z) : a.I[b];
+  float4 i4;
+  if constexpr (D == 2) {
+    i4 = make_float4(w4.y, 0.0f, 0.0f, w4.z);
+  } else if (a.iso && *a.iso) {
+    const float Is = (0.4f * v4.w) * (p4b.w * p4b.w);
+    i4 = make_float4(Is, Is, Is, __int_as_float(0));
+  } else {
+    i4 = a.I[b];
+  }
   const int bits = __float_as_int(i4.w);
   Vec<D> x = xyz<D>(p4b), v = xyz<D>(v4);
   Ang<D> w = ang4<D>(w4);
