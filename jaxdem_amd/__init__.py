@@ -33,7 +33,8 @@ from .integrators import (
 from .materials import Material, MaterialMatchmaker, MaterialTable
 from .state import Quaternion, State
 from .system import System, save_state
-from . import batching, minimizers, precision, serving, utils
+from . import batching, minimizers, precision, serving, utils, writers
+from .writers import VTKBaseWriter, VTKWriter
 from .batching import Batch
 from .minimizers import fire, minimize
 
@@ -72,6 +73,8 @@ __all__ = [
     "Quaternion",
     "System",
     "save_state",
+    "VTKWriter",
+    "VTKBaseWriter",
     "Batch",
     "batching",
     "serving",
