@@ -496,6 +496,103 @@ typedef struct jdem_precise {
 int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire, int64_t max_steps, double pe_tol,
                           double pe_diff_tol, double force_tol, jdem_minimize_stats* stats, double* virial);
 
+/* ---- (f) the plain step in float64 -------------------------------------------
+ *
+ * The reference computes in float64 when jax_enable_x64 is set (its own test
+ * suite does, tests/conftest.py:4).  These structs are jdem_state,
+ * jdem_domain, jdem_law, jdem_grid and jdem_system with every floating-point
+ * array and scalar in float64, and the entry points run the kernels of the
+ * float32 calls above instantiated in double, with the same association of
+ * operations.  Colliders: NONE, NAIVE and CELL_LIST; laws: SPRING, HERTZ,
+ * CUNDALL_STRACK and ROUTED over those (a NeighborList and the history laws
+ * return JDEM_EUNSUPPORTED); every domain and integrator. */
+typedef struct jdem_state_f64 {
+  int64_t n;
+  int32_t dim;
+  int32_t bond_width;
+  double* pos;
+  double* vel;
+  double* ang_vel;
+  double* q;
+  double* force;
+  double* torque;
+  const double* rad;
+  const double* mass;
+  const double* inertia;
+  const int32_t* mat_id;
+  const int32_t* bond_id;
+  const uint8_t* fixed;
+} jdem_state_f64;
+
+typedef struct jdem_domain_f64 {
+  int32_t kind;
+  int32_t dim;
+  double box_size[3];
+  double inv_box_size[3];
+  double anchor[3];
+  double restitution;
+} jdem_domain_f64;
+
+typedef struct jdem_law_f64 {
+  int32_t kind;
+  int32_t n_mat;
+  const double* t0;
+  const double* t1;
+  const double* t2;
+  const double* m0;
+  const double* m1;
+  const double* m2;
+  const double* m3;
+  double p0;
+  double p1;
+  int32_t n_species;
+  int32_t n_terms;
+  const int32_t* route_off;
+  const struct jdem_law_f64* route_terms;
+  const int32_t* species;
+} jdem_law_f64;
+
+/* _grid_params (_partition.py:55-99) computed by the host in float64. */
+typedef struct jdem_grid_f64 {
+  int32_t dims[3];
+  int32_t strides[3];
+  double cell_size[3];
+  int32_t search_range;
+  int32_t periodic;
+  int64_t n_cells;
+} jdem_grid_f64;
+
+typedef struct jdem_system_f64 {
+  jdem_state_f64 state;
+  jdem_domain_f64 domain;
+  jdem_law_f64 law;
+  int32_t lin_kind;
+  int32_t rot_kind;
+  double dt;
+  int32_t interact_same_bond_id;
+  double gravity[3];
+  const double* ext_force;
+  const double* ext_force_com;
+  const double* ext_torque;
+  int32_t collider_kind;
+  jdem_grid_f64 grid;
+} jdem_system_f64;
+
+/* System.step in float64 (system.py:701-748), as jdem_system_step.  For a
+ * FREE domain the box and anchor after the last step are written to
+ * sys->domain and stats. */
+int jdem_system_step_f64(jdem_ctx* ctx, jdem_system_f64* sys, int64_t n, jdem_step_stats* stats);
+/* Collider.compute_force in float64 (colliders/__init__.py:56-80), as
+ * jdem_compute_force. */
+int jdem_compute_force_f64(jdem_ctx* ctx, jdem_system_f64* sys);
+/* Collider.compute_potential_energy in float64 (cell_list.py:469-496;
+ * naive.py:76-117): float64 pair terms, float64 sum. */
+int jdem_potential_energy_f64(jdem_ctx* ctx, jdem_system_f64* sys, double* energy);
+/* Kinetic energy parts in float64 (utils/thermal.py:83-122), as
+ * jdem_kinetic_energy_parts. */
+int jdem_kinetic_energy_parts_f64(jdem_ctx* ctx, const jdem_system_f64* sys, double* translational,
+                                  double* rotational);
+
 #ifdef __cplusplus
 }
 #endif

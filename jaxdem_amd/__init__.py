@@ -8,6 +8,7 @@ Public names follow the reference package (jaxdem/__init__.py): ``State``,
 kernels live in ``libjdem_hip.so`` (C ABI: include/jdem.h).
 """
 
+from .config import config
 from .colliders import Collider, DynamicCellList, NaiveSimulator, NeighborList, refresh_collider
 from .domains import Domain, FreeDomain, PeriodicDomain, ReflectDomain
 from .factory import Factory
@@ -41,6 +42,7 @@ from .minimizers import fire, minimize
 __version__ = "0.1.0"
 
 __all__ = [
+    "config",
     "Collider",
     "DynamicCellList",
     "NaiveSimulator",

@@ -230,7 +230,81 @@ class JdemPrecise(ctypes.Structure):
     ]
 
 
+# ---- the plain step in float64 (jdem_*_f64) ---------------------------------
+class JdemStateF64(ctypes.Structure):
+    _fields_ = JdemState._fields_
+
+
+class JdemDomainF64(ctypes.Structure):
+    _fields_ = [
+        ("kind", c_int32),
+        ("dim", c_int32),
+        ("box_size", c_double * 3),
+        ("inv_box_size", c_double * 3),
+        ("anchor", c_double * 3),
+        ("restitution", c_double),
+    ]
+
+
+class JdemLawF64(ctypes.Structure):
+    pass
+
+
+JdemLawF64._fields_ = [
+    ("kind", c_int32),
+    ("n_mat", c_int32),
+    ("t0", c_void_p),
+    ("t1", c_void_p),
+    ("t2", c_void_p),
+    ("m0", c_void_p),
+    ("m1", c_void_p),
+    ("m2", c_void_p),
+    ("m3", c_void_p),
+    ("p0", c_double),
+    ("p1", c_double),
+    ("n_species", c_int32),
+    ("n_terms", c_int32),
+    ("route_off", c_void_p),
+    ("route_terms", ctypes.POINTER(JdemLawF64)),
+    ("species", c_void_p),
+]
+
+
+class JdemGridF64(ctypes.Structure):
+    _fields_ = [
+        ("dims", c_int32 * 3),
+        ("strides", c_int32 * 3),
+        ("cell_size", c_double * 3),
+        ("search_range", c_int32),
+        ("periodic", c_int32),
+        ("n_cells", c_int64),
+    ]
+
+
+class JdemSystemF64(ctypes.Structure):
+    _fields_ = [
+        ("state", JdemStateF64),
+        ("domain", JdemDomainF64),
+        ("law", JdemLawF64),
+        ("lin_kind", c_int32),
+        ("rot_kind", c_int32),
+        ("dt", c_double),
+        ("interact_same_bond_id", c_int32),
+        ("gravity", c_double * 3),
+        ("ext_force", c_void_p),
+        ("ext_force_com", c_void_p),
+        ("ext_torque", c_void_p),
+        ("collider_kind", c_int32),
+        ("grid", JdemGridF64),
+    ]
+
+
 STRUCTS = {
+    "jdem_state_f64": JdemStateF64,
+    "jdem_domain_f64": JdemDomainF64,
+    "jdem_law_f64": JdemLawF64,
+    "jdem_grid_f64": JdemGridF64,
+    "jdem_system_f64": JdemSystemF64,
     "jdem_timing": JdemTiming,
     "jdem_state": JdemState,
     "jdem_domain": JdemDomain,
@@ -292,6 +366,11 @@ SIGNATURES = {
         [c_void_p, POINTER(JdemSystem), POINTER(JdemFire), c_int64, c_double, c_double, c_double,
          POINTER(JdemMinimizeStats)],
     ),
+    "jdem_system_step_f64": (c_int, [c_void_p, POINTER(JdemSystemF64), c_int64, POINTER(JdemStepStats)]),
+    "jdem_compute_force_f64": (c_int, [c_void_p, POINTER(JdemSystemF64)]),
+    "jdem_potential_energy_f64": (c_int, [c_void_p, POINTER(JdemSystemF64), POINTER(c_double)]),
+    "jdem_kinetic_energy_parts_f64": (
+        c_int, [c_void_p, POINTER(JdemSystemF64), POINTER(c_double), POINTER(c_double)]),
 }
 
 _lib = None

@@ -129,6 +129,8 @@ class Batch:
                 raise ValueError("batched systems must share the device and the dimension")
             if not st.is_sphere_state:
                 raise NotImplementedError("batches hold sphere states only")
+            if st.pos_c.dtype != torch.float32:
+                raise NotImplementedError("batches run in float32 (the float64 kernels cover System.step)")
             if not _fusable(st, sy):
                 raise NotImplementedError("batched systems must use built-in plugins without force functions")
             col = sy.collider

@@ -27,27 +27,29 @@ from .factory import Factory, _normalize_key
 F32 = np.float32
 
 
-def grid_params(box_size: np.ndarray, cell_size, periodic: bool):
-    """``_grid_params`` (_partition.py:55-99) in float32.
+def grid_params(box_size: np.ndarray, cell_size, periodic: bool, ft=F32):
+    """``_grid_params`` (_partition.py:55-99) in float32 (``ft`` float64
+    for the float64 step).
 
-    Returns (dims int64 (D,), strides int64 (D,), cell float32 (D,), overflow).
+    Returns (dims int64 (D,), strides int64 (D,), cell (D,), overflow).
     """
-    box = np.asarray(box_size, F32)
-    cs = F32(cell_size)
+    box = np.asarray(box_size, ft)
+    cs = ft(cell_size)
     if periodic:
         dims = np.maximum(np.floor(box / cs).astype(np.int64), 1)
-        cell = (box / dims.astype(F32)).astype(F32)
+        cell = (box / dims.astype(ft)).astype(ft)
     else:
         dims = np.maximum(np.ceil(box / cs).astype(np.int64), 1)
-        cell = np.full(box.shape, cs, F32)
+        cell = np.full(box.shape, cs, ft)
     strides = np.concatenate([[1], np.cumprod(dims[:-1])]).astype(np.int64)
     overflow = float(np.prod(dims.astype(np.float64))) > float(np.iinfo(np.int32).max)
     return dims, strides, cell, overflow
 
 
-def grid_struct(box_size, cell_size, periodic: bool, search_range: int) -> tuple:
-    dims, strides, cell, overflow = grid_params(box_size, cell_size, periodic)
-    g = L.JdemGrid()
+def grid_struct(box_size, cell_size, periodic: bool, search_range: int, f64: bool = False) -> tuple:
+    """jdem_grid (jdem_grid_f64 with ``f64``) and the overflow flag."""
+    dims, strides, cell, overflow = grid_params(box_size, cell_size, periodic, np.float64 if f64 else F32)
+    g = L.JdemGridF64() if f64 else L.JdemGrid()
     D = len(dims)
     for a in range(3):
         g.dims[a] = int(dims[a]) if a < D else 1
@@ -127,32 +129,33 @@ class DynamicCellList(Collider):
 
     @classmethod
     def Create(cls, state, cell_size=None, search_range=None, box_size=None):
-        """cell_list.py:375-432."""
-        rad = state._rad.detach().cpu().numpy().astype(F32)
-        min_rad, max_rad = F32(rad.min()), F32(rad.max())
+        """cell_list.py:375-432, in the dtype of the radii."""
+        ft = np.float64 if state._rad.dtype == torch.float64 else F32
+        rad = state._rad.detach().cpu().numpy().astype(ft)
+        min_rad, max_rad = ft(rad.min()), ft(rad.max())
         alpha = max_rad / min_rad
         if cell_size is None:
-            cell_size = F32(2.0) * max_rad if alpha < 2.5 else F32(0.5) * max_rad
-        cell_size = F32(cell_size)
+            cell_size = ft(2.0) * max_rad if alpha < 2.5 else ft(0.5) * max_rad
+        cell_size = ft(cell_size)
         if box_size is not None:
-            box = np.asarray(box_size, F32)
+            box = np.asarray(box_size, ft)
             for _ in range(2):
                 if search_range is None:
-                    sr = max(1, int(np.ceil(F32(2.0) * max_rad / cell_size)))
+                    sr = max(1, int(np.ceil(ft(2.0) * max_rad / cell_size)))
                 else:
                     sr = int(search_range)
                 dims = np.maximum(np.floor(box / cell_size).astype(np.int64), 2 * sr + 1)
-                cell_size = F32(np.min(box / dims.astype(F32)))
+                cell_size = ft(np.min(box / dims.astype(ft)))
         if search_range is None:
-            search_range = max(1, int(np.ceil(F32(2.0) * max_rad / cell_size)))
+            search_range = max(1, int(np.ceil(ft(2.0) * max_rad / cell_size)))
         sr = int(search_range)
         r = np.arange(-sr, sr + 1)
         mesh = np.meshgrid(*([r] * state.dim), indexing="ij")
         mask = np.stack([m.ravel() for m in mesh], axis=1).astype(np.int64)
         return cls(neighbor_mask=mask, cell_size=float(cell_size))
 
-    def grid(self, system) -> tuple:
-        return grid_struct(system.domain.box_size, self.cell_size, system.domain.periodic, self.search_range)
+    def grid(self, system, f64: bool = False) -> tuple:
+        return grid_struct(system.domain.box_size, self.cell_size, system.domain.periodic, self.search_range, f64)
 
     @staticmethod
     def create_neighbor_list(state, system, cutoff: float, max_neighbors: int):

@@ -123,6 +123,31 @@ GridP grid_of(const jdem_grid& g) {
   return p;
 }
 
+DomP64 dom_of(const jdem_domain_f64& d) {
+  DomP64 p;
+  p.kind = d.kind;
+  for (int a = 0; a < 3; ++a) {
+    p.box[a] = d.box_size[a];
+    p.inv_box[a] = d.inv_box_size[a];
+    p.anchor[a] = d.anchor[a];
+  }
+  p.restitution = d.restitution;
+  return p;
+}
+
+GridP64 grid_of(const jdem_grid_f64& g) {
+  GridP64 p;
+  for (int a = 0; a < 3; ++a) {
+    p.dims[a] = g.dims[a] > 0 ? g.dims[a] : 1;
+    p.strides[a] = g.strides[a];
+    p.cell[a] = g.cell_size[a];
+  }
+  p.sr = g.search_range;
+  p.periodic = g.periodic;
+  p.n_cells = g.n_cells;
+  return p;
+}
+
 }  // namespace jdem
 
 namespace {
@@ -146,8 +171,23 @@ int fail(jdem_ctx* ctx, int code, const std::string& msg) {
       return fail((ctx), JDEM_ENOMEM, "scratch allocation failed: " #buf);        \
   } while (0)
 
-LawP law_of_elem(const jdem_law& l) {
-  LawP p;
+// Scalar type of an ABI law struct: jdem_law is float32, jdem_law_f64 float64.
+template <class L>
+struct LawScalar;
+template <>
+struct LawScalar<jdem_law> {
+  using T = float;
+};
+template <>
+struct LawScalar<jdem_law_f64> {
+  using T = double;
+};
+template <class L>
+using LawOf = LawPT<typename LawScalar<L>::T>;
+
+template <class L>
+LawOf<L> law_of_elem(const L& l) {
+  LawOf<L> p;
   std::memset(&p, 0, sizeof(p));
   p.kind = l.kind;
   p.n_mat = l.n_mat;
@@ -166,7 +206,8 @@ LawP law_of_elem(const jdem_law& l) {
 bool is_history_kind(int k) { return k == JDEM_LAW_SPRING_FRICTION || k == JDEM_LAW_HERTZ_FRICTION; }
 
 // Does the law carry a tangential spring per pair?
-bool law_has_history(const jdem_law& l) {
+template <class L>
+bool law_has_history(const L& l) {
   if (l.kind != JDEM_LAW_ROUTED) return is_history_kind(l.kind);
   for (int k = 0; k < l.n_terms; ++k)
     if (is_history_kind(l.route_terms[k].kind)) return true;
@@ -174,11 +215,15 @@ bool law_has_history(const jdem_law& l) {
 }
 
 // Entries of a routed law's species table: n_species^2, or 1 without species.
-int route_entries(const jdem_law& l) { return l.n_species > 1 ? l.n_species * l.n_species : 1; }
+template <class L>
+int route_entries(const L& l) {
+  return l.n_species > 1 ? l.n_species * l.n_species : 1;
+}
 
 // Checks a routed law and copies its terms and species offsets to the
 // context's device table when they differ from the last copy.
-int prepare_route(jdem_ctx* ctx, const jdem_law& l, int64_t n) {
+template <class L>
+int prepare_route(jdem_ctx* ctx, const L& l, int64_t n) {
   if (l.n_species < 0 || l.n_species > kMatMask) return fail(ctx, JDEM_EINVAL, "routed law: n_species out of range");
   if (l.n_terms < 1 || !l.route_off || !l.route_terms) return fail(ctx, JDEM_EINVAL, "routed law without terms");
   if (l.n_species > 1 && n > 0 && !l.species) return fail(ctx, JDEM_EINVAL, "routed law: species array missing");
@@ -188,7 +233,7 @@ int prepare_route(jdem_ctx* ctx, const jdem_law& l, int64_t n) {
     if (l.route_off[e + 1] <= l.route_off[e]) return fail(ctx, JDEM_EINVAL, "routed law: an entry has no term");
     int n_hist = 0;
     for (int k = l.route_off[e]; k < l.route_off[e + 1]; ++k) {
-      const jdem_law& t = l.route_terms[k];
+      const L& t = l.route_terms[k];
       if (t.kind < JDEM_LAW_NONE || t.kind > JDEM_LAW_HERTZ_FRICTION)
         return fail(ctx, JDEM_EINVAL, "routed law: a term is not an elementary law");
       if (t.kind != JDEM_LAW_NONE) {
@@ -201,10 +246,11 @@ int prepare_route(jdem_ctx* ctx, const jdem_law& l, int64_t n) {
     }
     if (n_hist > 1) return fail(ctx, JDEM_EUNSUPPORTED, "routed law: an entry with more than one history law");
   }
-  // device table: the terms (LawP), then the offsets (int32)
-  const size_t tb = sizeof(LawP) * (size_t)l.n_terms;
+  // device table: the terms (LawPT of the law's scalar type), then the
+  // offsets (int32)
+  const size_t tb = sizeof(LawOf<L>) * (size_t)l.n_terms;
   std::vector<char> host(tb + sizeof(int32_t) * (size_t)(E + 1));
-  LawP* terms = reinterpret_cast<LawP*>(host.data());
+  LawOf<L>* terms = reinterpret_cast<LawOf<L>*>(host.data());
   for (int k = 0; k < l.n_terms; ++k) terms[k] = law_of_elem(l.route_terms[k]);
   std::memcpy(host.data() + tb, l.route_off, sizeof(int32_t) * (size_t)(E + 1));
   if (host != ctx->route_host || !ctx->route.p) {
@@ -220,14 +266,15 @@ int prepare_route(jdem_ctx* ctx, const jdem_law& l, int64_t n) {
 // Kernel-side law of a call: a routed law points at the context's device
 // table (prepare_route ran in check_system) and takes the kernel kind with
 // or without the history term.
-LawP law_of(jdem_ctx* ctx, const jdem_law& l) {
-  LawP p = law_of_elem(l);
+template <class L>
+LawOf<L> law_of(jdem_ctx* ctx, const L& l) {
+  LawOf<L> p = law_of_elem(l);
   if (l.kind == JDEM_LAW_ROUTED) {
     p.kind = law_has_history(l) ? kLawRoutedHist : kLawRouted;
     p.n_species = l.n_species;
-    p.route_terms = reinterpret_cast<const LawP*>(ctx->route.p);
+    p.route_terms = reinterpret_cast<const LawOf<L>*>(ctx->route.p);
     p.route_off = reinterpret_cast<const int32_t*>(static_cast<const char*>(ctx->route.p) +
-                                                   sizeof(LawP) * (size_t)l.n_terms);
+                                                   sizeof(LawOf<L>) * (size_t)l.n_terms);
     p.species = l.n_species > 1 ? l.species : nullptr;
   }
   return p;
@@ -345,9 +392,12 @@ struct Cells {
 
 int nbits_for(int64_t n_cells) { return nbits_for_cells(n_cells); }
 
-// Hash + sort + cell ranges + sorted (x, r) of `pos` (cell_list.py:37-87).
-int build_cells(jdem_ctx* ctx, const float* pos, const float* rad, int64_t n, int dim, const DomP& dom,
-                const GridP& g, Cells* out) {
+// Hash + sort + cell ranges + sorted (x, r) of `pos` (cell_list.py:37-87);
+// in float64 the kernels read the partners' own arrays and no (x, r) copy
+// is made.
+template <class Real>
+int build_cells(jdem_ctx* ctx, const Real* pos, const Real* rad, int64_t n, int dim, const DomPT<Real>& dom,
+                const GridPT<Real>& g, Cells* out) {
   const size_t nn = (size_t)(n > 0 ? n : 1);
   ENSURE(ctx, ctx->keys[0], 4 * nn);
   ENSURE(ctx, ctx->keys[1], 4 * nn);
@@ -373,7 +423,8 @@ int build_cells(jdem_ctx* ctx, const float* pos, const float* rad, int64_t n, in
     HIPCHK(ctx, launch_fill_i32(ce, g.n_cells, 0, ctx->stream));
     HIPCHK(ctx, launch_cell_ranges(sk, n, cs, ce, ctx->stream));
   }
-  HIPCHK(ctx, launch_gather_spos(perm, pos, rad, n, dim, (float4*)ctx->spos.p, ctx->stream));
+  if constexpr (std::is_same<Real, float>::value)
+    HIPCHK(ctx, launch_gather_spos(perm, pos, rad, n, dim, (float4*)ctx->spos.p, ctx->stream));
   out->perm = perm;
   out->skeys = (int32_t*)sk;
   out->cstart = cs;
@@ -2339,6 +2390,290 @@ int jdem_rollout(jdem_ctx* ctx, jdem_system* sys, const jdem_frames* fr, jdem_st
   rc = ensure_flags(ctx);
   if (rc) return rc;
   return resident_nl_step(ctx, sys, n, S, nullptr, fr);
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// (f) the plain step in float64: the kernels of jdem_system_step,
+// jdem_compute_force and the energies instantiated in double.
+// ---------------------------------------------------------------------------
+namespace {
+
+int check_system64(jdem_ctx* ctx, const jdem_system_f64* sys) {
+  if (!ctx) return JDEM_EINVAL;
+  if (!sys) return fail(ctx, JDEM_EINVAL, "null system");
+  const jdem_state_f64& s = sys->state;
+  if (s.dim != 2 && s.dim != 3) return fail(ctx, JDEM_EINVAL, "dim must be 2 or 3");
+  if (s.n < 0 || s.n > (int64_t)INT32_MAX) return fail(ctx, JDEM_EINVAL, "particle count out of range");
+  if (s.n > 0 && (!s.pos || !s.vel || !s.ang_vel || !s.q || !s.force || !s.torque || !s.rad || !s.mass ||
+                  !s.inertia || !s.fixed))
+    return fail(ctx, JDEM_EINVAL, "null state array");
+  if (s.bond_width > 0 && !s.bond_id) return fail(ctx, JDEM_EINVAL, "bond_width > 0 but bond_id is null");
+  if (sys->law.n_mat > 1 && !s.mat_id) return fail(ctx, JDEM_EINVAL, "n_mat > 1 but mat_id is null");
+  const int ck = sys->collider_kind;
+  if (ck == JDEM_COLLIDER_NEIGHBOR_LIST)
+    return fail(ctx, JDEM_EUNSUPPORTED, "float64: the NeighborList collider runs in float32 only");
+  if (ck != JDEM_COLLIDER_NONE && ck != JDEM_COLLIDER_NAIVE && ck != JDEM_COLLIDER_CELL_LIST)
+    return fail(ctx, JDEM_EINVAL, "unknown collider kind");
+  const int lk = sys->law.kind;
+  if (ck != JDEM_COLLIDER_NONE && !(lk >= JDEM_LAW_SPRING && lk <= JDEM_LAW_ROUTED))
+    return fail(ctx, JDEM_EINVAL, "unknown law kind");
+  if (sys->law.n_mat > kMatMask) return fail(ctx, JDEM_EINVAL, "more than 32767 materials");
+  if (lk == JDEM_LAW_ROUTED) {
+    if (sys->law.n_mat < 1) return fail(ctx, JDEM_EINVAL, "routed law: n_mat < 1");
+    const int rc = prepare_route(ctx, sys->law, s.n);
+    if (rc) return rc;
+  } else if (sys->law.n_mat < 1 || !sys->law.t0) {
+    if (ck != JDEM_COLLIDER_NONE && lk != JDEM_LAW_CUNDALL_STRACK)
+      return fail(ctx, JDEM_EINVAL, "law table t0 missing");
+  }
+  if (ck != JDEM_COLLIDER_NONE && law_has_history(sys->law))
+    return fail(ctx, JDEM_EUNSUPPORTED, "float64: the history laws run in float32 only");
+  if (sys->domain.kind < JDEM_DOMAIN_FREE || sys->domain.kind > JDEM_DOMAIN_REFLECT)
+    return fail(ctx, JDEM_EINVAL, "unknown domain kind");
+  if (ck == JDEM_COLLIDER_CELL_LIST) {
+    const jdem_grid_f64& g = sys->grid;
+    if (g.n_cells <= 0 || g.n_cells > (int64_t)INT32_MAX)
+      return fail(ctx, JDEM_EUNSUPPORTED, "cell grid size out of the int32 hash range");
+    if (g.search_range < 1) return fail(ctx, JDEM_EINVAL, "search_range must be >= 1");
+  }
+  return JDEM_OK;
+}
+
+StepArgs64 base_args64(jdem_ctx* ctx, const jdem_system_f64* sys) {
+  StepArgs64 a;
+  std::memset(&a, 0, sizeof(a));
+  const jdem_state_f64& s = sys->state;
+  a.n = s.n;
+  a.bond_w = s.bond_width;
+  a.isb = sys->interact_same_bond_id;
+  a.pos_in = s.pos;
+  a.vel_in = s.vel;
+  a.ang_in = s.ang_vel;
+  a.pos_out = s.pos;
+  a.vel_out = s.vel;
+  a.ang_out = s.ang_vel;
+  a.q = s.q;
+  a.force = s.force;
+  a.torque = s.torque;
+  a.rad = s.rad;
+  a.mass = s.mass;
+  a.inertia = s.inertia;
+  a.mat_id = s.mat_id;
+  a.bond_id = s.bond_id;
+  a.fixed = s.fixed;
+  a.ext_f = sys->ext_force;
+  a.ext_fc = sys->ext_force_com;
+  a.ext_t = sys->ext_torque;
+  for (int c = 0; c < 3; ++c) a.g[c] = sys->gravity[c];
+  a.has_fm = (sys->gravity[0] != 0.0 || sys->gravity[1] != 0.0 || sys->gravity[2] != 0.0 || a.ext_f || a.ext_fc ||
+              a.ext_t)
+                 ? 1
+                 : 0;
+  a.law = law_of(ctx, sys->law);
+  a.dom = dom_of(sys->domain);
+  a.dt = sys->dt;
+  a.lin = sys->lin_kind;
+  a.rot = sys->rot_kind;
+  return a;
+}
+
+int prepare_source64(jdem_ctx* ctx, const jdem_system_f64* sys, StepArgs64& a, const double* pos) {
+  if (sys->collider_kind == JDEM_COLLIDER_CELL_LIST) {
+    Cells c;
+    a.grid = grid_of(sys->grid);
+    int rc = build_cells(ctx, pos, sys->state.rad, sys->state.n, sys->state.dim, a.dom, a.grid, &c);
+    if (rc) return rc;
+    a.perm = c.perm;
+    a.skeys = c.skeys;
+    a.cstart = c.cstart;
+    a.cend = c.cend;
+    a.spos = nullptr;
+  }
+  return JDEM_OK;
+}
+
+// FreeDomain.apply (free.py:42-58) in float64.
+int free_box64(jdem_ctx* ctx, const double* pos, const jdem_system_f64* sys, double box[3], double anchor[3]) {
+  const jdem_state_f64& s = sys->state;
+  const int nparts = 64;
+  ENSURE(ctx, ctx->red, 8 * 6 * (size_t)nparts);
+  HIPCHK(ctx, launch_box_minmax(pos, s.rad, s.n, s.dim, (double*)ctx->red.p, nparts, ctx->stream));
+  std::vector<double> hp(6 * nparts);
+  HIPCHK(ctx, hipMemcpyAsync(hp.data(), ctx->red.p, 8 * hp.size(), hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  for (int a = 0; a < s.dim; ++a) {
+    double lo = INFINITY, hi = -INFINITY;
+    for (int b = 0; b < nparts; ++b) {
+      lo = std::min(lo, hp[6 * b + a]);
+      hi = std::max(hi, hp[6 * b + 3 + a]);
+    }
+    box[a] = hi - lo;
+    anchor[a] = lo;
+  }
+  return JDEM_OK;
+}
+
+// Per-particle float64 terms summed on the device: the potential energy
+// (collider pairs) or the two kinetic parts.
+int reduce64(jdem_ctx* ctx, const jdem_system_f64* sys, int ncomp, double* host_out, bool kinetic) {
+  const jdem_state_f64& s = sys->state;
+  const int64_t n = s.n;
+  const int nparts = 256;
+  ENSURE(ctx, ctx->red, 8 * ((size_t)(n > 0 ? n : 1) * ncomp + (size_t)nparts * ncomp + ncomp));
+  double* per = (double*)ctx->red.p;
+  double* part = per + (size_t)(n > 0 ? n : 1) * ncomp;
+  double* out = part + (size_t)nparts * ncomp;
+  StepArgs64 a = base_args64(ctx, sys);
+  if (!kinetic) {
+    int rc = prepare_source64(ctx, sys, a, s.pos);
+    if (rc) return rc;
+  }
+  a.e_out = per;
+  if (n > 0) {
+    if (kinetic) {
+      HIPCHK(ctx, launch_kinetic(a, s.dim, ctx->stream));
+    } else {
+      HIPCHK(ctx, launch_energy(a, s.dim, a.law.kind, src_of(sys->collider_kind), ctx->stream));
+    }
+    HIPCHK(ctx, launch_sum_f64(per, n, ncomp, ncomp, part, out, nparts, ctx->stream));
+    HIPCHK(ctx, hipMemcpyAsync(host_out, out, 8 * (size_t)ncomp, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  } else {
+    for (int c = 0; c < ncomp; ++c) host_out[c] = 0.0;
+  }
+  return JDEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int jdem_system_step_f64(jdem_ctx* ctx, jdem_system_f64* sys, int64_t n, jdem_step_stats* stats) {
+  int rc = check_system64(ctx, sys);
+  if (rc) return rc;
+  if (n < 0) return fail(ctx, JDEM_EINVAL, "n < 0");
+  jdem_step_stats local;
+  std::memset(&local, 0, sizeof(local));
+  jdem_step_stats* S = stats ? stats : &local;
+  std::memset(S, 0, sizeof(*S));
+  if (n == 0) return JDEM_OK;
+  if (sys->collider_kind == JDEM_COLLIDER_CELL_LIST && sys->domain.kind == JDEM_DOMAIN_FREE)
+    return fail(ctx, JDEM_EUNSUPPORTED, "CellList with a FreeDomain (per-step grid) is not implemented");
+  (void)hipSetDevice(ctx->device);
+  jdem_state_f64& s = sys->state;
+  const int64_t N = s.n;
+  const int D = s.dim;
+  const int A = D == 3 ? 3 : 1;
+  if (N == 0) {
+    S->steps = n;
+    return JDEM_OK;
+  }
+  ENSURE(ctx, ctx->pos2, 8 * (size_t)N * D);
+  ENSURE(ctx, ctx->vel2, 8 * (size_t)N * D);
+  ENSURE(ctx, ctx->ang2, 8 * (size_t)N * A);
+  double* P[2] = {s.pos, (double*)ctx->pos2.p};
+  double* V[2] = {s.vel, (double*)ctx->vel2.p};
+  double* W[2] = {s.ang_vel, (double*)ctx->ang2.p};
+  // kernel t (0..n) writes buffer wr(t), so that the last one writes the
+  // caller's arrays; it reads the buffer kernel t-1 wrote (as
+  // jdem_system_step).
+  auto wr = [&](int64_t t) { return (int)((n - t) & 1); };
+  auto rd = [&](int64_t t) { return t == 0 ? 0 : wr(t - 1); };
+  bool free_done = false;
+  double fbox[3] = {0, 0, 0}, fanc[3] = {0, 0, 0};
+  for (int64_t t = 0; t <= n; ++t) {
+    StepArgs64 a = base_args64(ctx, sys);
+    const int r = rd(t), w = wr(t);
+    a.pos_in = P[r];
+    a.vel_in = V[r];
+    a.ang_in = W[r];
+    a.pos_out = P[w];
+    a.vel_out = V[w];
+    a.ang_out = W[w];
+    a.do_force = t >= 1;
+    a.do_after = t >= 1;
+    a.do_before = t < n;
+    a.write_kin = 1;
+    a.step_idx = (int32_t)t;
+    if (sys->domain.kind == JDEM_DOMAIN_FREE && t == n - 1 && !free_done) {
+      // the last FreeDomain.apply (step n) sees the positions kernel n-1 reads
+      rc = free_box64(ctx, P[r], sys, fbox, fanc);
+      if (rc) return rc;
+      free_done = true;
+      S->host_syncs++;
+    }
+    if (a.do_force && sys->collider_kind != JDEM_COLLIDER_NONE) {
+      rc = prepare_source64(ctx, sys, a, P[r]);
+      if (rc) return rc;
+    }
+    int src_eff = src_of(sys->collider_kind);
+    int law_eff = a.law.kind;
+    if (sys->collider_kind == JDEM_COLLIDER_NONE) {
+      a.force_zero = 1;  // the "" collider (colliders/__init__.py:218-222)
+      src_eff = SRC_NAIVE;
+      law_eff = JDEM_LAW_SPRING;
+    }
+    const hipError_t e = launch_step(a, D, law_eff, src_eff, ctx->stream);
+    if (e != hipSuccess) return fail(ctx, JDEM_EHIP, std::string("step kernel: ") + hipGetErrorString(e));
+  }
+  if (free_done) {
+    for (int a = 0; a < D; ++a) {
+      sys->domain.box_size[a] = fbox[a];
+      sys->domain.anchor[a] = fanc[a];
+      sys->domain.inv_box_size[a] = 1.0 / fbox[a];
+      S->free_box[a] = fbox[a];
+      S->free_anchor[a] = fanc[a];
+    }
+  }
+  S->steps = n;
+  return JDEM_OK;
+}
+
+int jdem_compute_force_f64(jdem_ctx* ctx, jdem_system_f64* sys) {
+  int rc = check_system64(ctx, sys);
+  if (rc) return rc;
+  (void)hipSetDevice(ctx->device);
+  const jdem_state_f64& s = sys->state;
+  if (sys->collider_kind == JDEM_COLLIDER_NONE) {
+    HIPCHK(ctx, hipMemsetAsync(s.force, 0, 8 * (size_t)s.n * s.dim, ctx->stream));
+    HIPCHK(ctx, hipMemsetAsync(s.torque, 0, 8 * (size_t)s.n * (s.dim == 3 ? 3 : 1), ctx->stream));
+    return JDEM_OK;
+  }
+  StepArgs64 a = base_args64(ctx, sys);
+  rc = prepare_source64(ctx, sys, a, s.pos);
+  if (rc) return rc;
+  a.do_force = 1;
+  a.write_kin = 0;
+  HIPCHK(ctx, launch_step(a, s.dim, a.law.kind, src_of(sys->collider_kind), ctx->stream));
+  return JDEM_OK;
+}
+
+int jdem_potential_energy_f64(jdem_ctx* ctx, jdem_system_f64* sys, double* energy) {
+  int rc = check_system64(ctx, sys);
+  if (rc) return rc;
+  if (!energy) return fail(ctx, JDEM_EINVAL, "null output");
+  (void)hipSetDevice(ctx->device);
+  if (sys->collider_kind == JDEM_COLLIDER_NONE) {
+    *energy = 0.0;
+    return JDEM_OK;
+  }
+  return reduce64(ctx, sys, 1, energy, false);
+}
+
+int jdem_kinetic_energy_parts_f64(jdem_ctx* ctx, const jdem_system_f64* sys, double* translational,
+                                  double* rotational) {
+  int rc = check_system64(ctx, sys);
+  if (rc) return rc;
+  if (!translational || !rotational) return fail(ctx, JDEM_EINVAL, "null output");
+  (void)hipSetDevice(ctx->device);
+  double parts[2] = {0.0, 0.0};
+  rc = reduce64(ctx, sys, 2, parts, true);
+  if (rc) return rc;
+  *translational = parts[0];
+  *rotational = parts[1];
+  return JDEM_OK;
 }
 
 }  // extern "C"

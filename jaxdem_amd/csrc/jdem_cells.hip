@@ -18,23 +18,23 @@ namespace jdem {
 // ---------------------------------------------------------------------------
 // cell coordinates (cell_list.py:55-60), clamped into [0, dims-1]
 // ---------------------------------------------------------------------------
-template <bool PER>
-JD int cell_coord(float x, int a, const DomP& dom, const GridP& g) {
-  float f;
+template <bool PER, class Real>
+JD int cell_coord(Real x, int a, const DomPT<Real>& dom, const GridPT<Real>& g) {
+  Real f;
   if (PER) {
-    float t = (x - dom.anchor[a]) / dom.box[a];
-    t = t - floorf(t);  // jnp.remainder(t, 1.0)
-    f = floorf(t * (float)g.dims[a]);
+    Real t = (x - dom.anchor[a]) / dom.box[a];
+    t = t - floor_(t);  // jnp.remainder(t, 1.0)
+    f = floor_(t * (Real)g.dims[a]);
   } else {
-    f = floorf((x - dom.anchor[a]) / g.cell[a]);
+    f = floor_((x - dom.anchor[a]) / g.cell[a]);
   }
-  f = fminf(fmaxf(f, 0.0f), (float)(g.dims[a] - 1));
+  f = fmin_(fmax_(f, Real(0)), (Real)(g.dims[a] - 1));
   return (int)f;
 }
 
-template <int D, bool PER>
-__global__ void __launch_bounds__(kBlock) k_hash(const float* __restrict__ pos, int64_t n, DomP dom, GridP g,
-                                                  uint32_t* __restrict__ keys) {
+template <int D, bool PER, class Real>
+__global__ void __launch_bounds__(kBlock) k_hash(const Real* __restrict__ pos, int64_t n, DomPT<Real> dom,
+                                                  GridPT<Real> g, uint32_t* __restrict__ keys) {
   int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   int h = 0;
@@ -43,23 +43,32 @@ __global__ void __launch_bounds__(kBlock) k_hash(const float* __restrict__ pos, 
   keys[i] = (uint32_t)h;
 }
 
-hipError_t launch_hash(const float* pos, int64_t n, int dim, const DomP& dom, const GridP& g, uint32_t* keys,
-                       hipStream_t s) {
+template <class Real>
+hipError_t hash_run(const Real* pos, int64_t n, int dim, const DomPT<Real>& dom, const GridPT<Real>& g,
+                    uint32_t* keys, hipStream_t s) {
   if (n == 0) return hipSuccess;
   dim3 grd(nblocks(n)), blk(kBlock);
   bool per = g.periodic != 0;
   if (dim == 3) {
     if (per)
-      hipLaunchKernelGGL((k_hash<3, true>), grd, blk, 0, s, pos, n, dom, g, keys);
+      hipLaunchKernelGGL((k_hash<3, true, Real>), grd, blk, 0, s, pos, n, dom, g, keys);
     else
-      hipLaunchKernelGGL((k_hash<3, false>), grd, blk, 0, s, pos, n, dom, g, keys);
+      hipLaunchKernelGGL((k_hash<3, false, Real>), grd, blk, 0, s, pos, n, dom, g, keys);
   } else {
     if (per)
-      hipLaunchKernelGGL((k_hash<2, true>), grd, blk, 0, s, pos, n, dom, g, keys);
+      hipLaunchKernelGGL((k_hash<2, true, Real>), grd, blk, 0, s, pos, n, dom, g, keys);
     else
-      hipLaunchKernelGGL((k_hash<2, false>), grd, blk, 0, s, pos, n, dom, g, keys);
+      hipLaunchKernelGGL((k_hash<2, false, Real>), grd, blk, 0, s, pos, n, dom, g, keys);
   }
   return hipGetLastError();
+}
+hipError_t launch_hash(const float* pos, int64_t n, int dim, const DomP& dom, const GridP& g, uint32_t* keys,
+                       hipStream_t s) {
+  return hash_run(pos, n, dim, dom, g, keys, s);
+}
+hipError_t launch_hash(const double* pos, int64_t n, int dim, const DomP64& dom, const GridP64& g, uint32_t* keys,
+                       hipStream_t s) {
+  return hash_run(pos, n, dim, dom, g, keys, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -907,17 +916,19 @@ hipError_t launch_remap_hist(int64_t n, int M, int W32, int dim, const int32_t* 
 // reductions
 // ---------------------------------------------------------------------------
 // FreeDomain.apply (free.py:42-58): per-block min(x - r) / max(x + r).
-__global__ void __launch_bounds__(kBlock) k_box_minmax(const float* __restrict__ pos, const float* __restrict__ rad,
-                                                        int64_t n, int dim, float* __restrict__ part) {
-  __shared__ float smin[3][kBlock], smax[3][kBlock];
+template <class Real>
+__global__ void __launch_bounds__(kBlock) k_box_minmax(const Real* __restrict__ pos, const Real* __restrict__ rad,
+                                                        int64_t n, int dim, Real* __restrict__ part) {
+  __shared__ Real smin[3][kBlock], smax[3][kBlock];
   const int tid = threadIdx.x;
-  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  Real lo[3] = {Real(INFINITY), Real(INFINITY), Real(INFINITY)};
+  Real hi[3] = {-Real(INFINITY), -Real(INFINITY), -Real(INFINITY)};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + tid; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const float r = rad[i];
+    const Real r = rad[i];
     for (int a = 0; a < dim; ++a) {
-      const float x = pos[dim * i + a];
-      lo[a] = fminf(lo[a], x - r);
-      hi[a] = fmaxf(hi[a], x + r);
+      const Real x = pos[dim * i + a];
+      lo[a] = fmin_(lo[a], x - r);
+      hi[a] = fmax_(hi[a], x + r);
     }
   }
   for (int a = 0; a < 3; ++a) {
@@ -928,8 +939,8 @@ __global__ void __launch_bounds__(kBlock) k_box_minmax(const float* __restrict__
   for (int off = kBlock / 2; off > 0; off >>= 1) {
     if (tid < off)
       for (int a = 0; a < 3; ++a) {
-        smin[a][tid] = fminf(smin[a][tid], smin[a][tid + off]);
-        smax[a][tid] = fmaxf(smax[a][tid], smax[a][tid + off]);
+        smin[a][tid] = fmin_(smin[a][tid], smin[a][tid + off]);
+        smax[a][tid] = fmax_(smax[a][tid], smax[a][tid + off]);
       }
     __syncthreads();
   }
@@ -942,7 +953,12 @@ __global__ void __launch_bounds__(kBlock) k_box_minmax(const float* __restrict__
 
 hipError_t launch_box_minmax(const float* pos, const float* rad, int64_t n, int dim, float* part, int nparts,
                              hipStream_t s) {
-  hipLaunchKernelGGL(k_box_minmax, dim3(nparts), dim3(kBlock), 0, s, pos, rad, n, dim, part);
+  hipLaunchKernelGGL((k_box_minmax<float>), dim3(nparts), dim3(kBlock), 0, s, pos, rad, n, dim, part);
+  return hipGetLastError();
+}
+hipError_t launch_box_minmax(const double* pos, const double* rad, int64_t n, int dim, double* part, int nparts,
+                             hipStream_t s) {
+  hipLaunchKernelGGL((k_box_minmax<double>), dim3(nparts), dim3(kBlock), 0, s, pos, rad, n, dim, part);
   return hipGetLastError();
 }
 

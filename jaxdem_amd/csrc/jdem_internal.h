@@ -20,15 +20,17 @@ constexpr int kBlock = 256;
 // dispatches this many workgroups instead of one per tile.
 constexpr int kCappedGrid = 2048;
 
-// Grid parameters on the device (jdem_grid).
-struct GridP {
+// Grid parameters on the device (jdem_grid; jdem_grid_f64 for double).
+template <class Real = float>
+struct GridPT {
   int dims[3];
   int strides[3];
-  float cell[3];
+  Real cell[3];
   int sr;
   int periodic;
   int64_t n_cells;
 };
+using GridP = GridPT<float>;
 
 // One grow-only device allocation of the context.
 struct Buf {
@@ -103,6 +105,8 @@ namespace jdem {
 // Kernel-side parameters of the ABI's domain and grid structs.
 DomP dom_of(const jdem_domain& d);
 GridP grid_of(const jdem_grid& g);
+DomPT<double> dom_of(const jdem_domain_f64& d);
+GridPT<double> grid_of(const jdem_grid_f64& g);
 
 // Ensure b holds at least `bytes`; returns false on allocation failure.
 // An empty b is first refilled from the context's pool.
@@ -113,33 +117,34 @@ void pool_give(jdem_ctx* ctx, Buf& b);
 void pool_trim(jdem_ctx* ctx);
 
 // Kernel arguments of the fused step / force / energy kernels.
-struct StepArgs {
+template <class Real = float>
+struct StepArgsT {
   int64_t n;
   int bond_w;
   int isb;
-  const float* pos_in;
-  const float* vel_in;
-  const float* ang_in;
-  float* pos_out;
-  float* vel_out;
-  float* ang_out;
-  float* q;
-  float* force;
-  float* torque;
-  const float* rad;
-  const float* mass;
-  const float* inertia;
+  const Real* pos_in;
+  const Real* vel_in;
+  const Real* ang_in;
+  Real* pos_out;
+  Real* vel_out;
+  Real* ang_out;
+  Real* q;
+  Real* force;
+  Real* torque;
+  const Real* rad;
+  const Real* mass;
+  const Real* inertia;
   const int32_t* mat_id;
   const int32_t* bond_id;
   const uint8_t* fixed;
-  const float* ext_f;
-  const float* ext_fc;
-  const float* ext_t;
-  float g[3];
+  const Real* ext_f;
+  const Real* ext_fc;
+  const Real* ext_t;
+  Real g[3];
   int has_fm;
-  LawP law;
-  DomP dom;
-  float dt;
+  LawPT<Real> law;
+  DomPT<Real> dom;
+  Real dt;
   int lin;
   int rot;
   // cell-list source
@@ -147,12 +152,12 @@ struct StepArgs {
   const int32_t* skeys;
   const int32_t* cstart;
   const int32_t* cend;
-  const float4* spos;
-  GridP grid;
+  const float4* spos;  // float only: sorted (x, r)
+  GridPT<Real> grid;
   // neighbour-list source
   const int32_t* nl;
   int M;
-  float* hist_xi;
+  Real* hist_xi;
   uint32_t* hist_live;
   int W32;
   // phases
@@ -162,8 +167,8 @@ struct StepArgs {
   int do_before;
   int write_kin;
   // neighbour-list displacement check (neighbor_list.py:78-85)
-  const float* old_pos;
-  float trigger;
+  const Real* old_pos;
+  Real trigger;
   int check_disp;
   const int32_t* flag_rd;
   int32_t* flag_wr;
@@ -171,6 +176,12 @@ struct StepArgs {
   // energy / virial outputs (per particle, float64)
   double* e_out;
 };
+using StepArgs = StepArgsT<float>;
+// The float64 form of the plain step (jdem_system_step_f64 and friends).
+using StepArgs64 = StepArgsT<double>;
+using DomP64 = DomPT<double>;
+using GridP64 = GridPT<double>;
+using LawP64 = LawPT<double>;
 
 enum Src { SRC_NAIVE = 1, SRC_CELLS = 2, SRC_NLIST = 3 };
 
@@ -178,9 +189,15 @@ enum Src { SRC_NAIVE = 1, SRC_CELLS = 2, SRC_NLIST = 3 };
 hipError_t launch_step(const StepArgs& a, int dim, int law, int src, hipStream_t s);
 hipError_t launch_energy(const StepArgs& a, int dim, int law, int src, hipStream_t s);
 hipError_t launch_virial(const StepArgs& a, int dim, int law, int src, hipStream_t s);
+// float64 instances: colliders naive / CellList, laws without history
+hipError_t launch_step(const StepArgs64& a, int dim, int law, int src, hipStream_t s);
+hipError_t launch_energy(const StepArgs64& a, int dim, int law, int src, hipStream_t s);
+hipError_t launch_kinetic(const StepArgs64& a, int dim, hipStream_t s);
 
 // launchers (jdem_cells.hip)
 hipError_t launch_hash(const float* pos, int64_t n, int dim, const DomP& dom, const GridP& g, uint32_t* keys,
+                       hipStream_t s);
+hipError_t launch_hash(const double* pos, int64_t n, int dim, const DomP64& dom, const GridP64& g, uint32_t* keys,
                        hipStream_t s);
 // Buffers of a sort: key and value ping-pong arrays of n entries, a
 // histogram / scan-sum array and (counting sort) per-cell cursors.
@@ -271,6 +288,8 @@ hipError_t launch_remap_hist(int64_t n, int M, int W32, int dim, const int32_t* 
                              const float* old_xi, const int32_t* new_nl, uint32_t* new_live, float* new_xi,
                              hipStream_t s);
 hipError_t launch_box_minmax(const float* pos, const float* rad, int64_t n, int dim, float* part, int nparts,
+                             hipStream_t s);
+hipError_t launch_box_minmax(const double* pos, const double* rad, int64_t n, int dim, double* part, int nparts,
                              hipStream_t s);
 hipError_t launch_sum_f64(const double* in, int64_t n, int stride, int ncomp, double* partial, double* out,
                           int nparts, hipStream_t s);

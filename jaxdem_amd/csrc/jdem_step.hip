@@ -27,20 +27,21 @@ JD Vec<3> v4s<3>(float4 a) { return {a.x, a.y, a.z}; }
 template <>
 JD Vec<2> v4s<2>(float4 a) { return {a.x, a.y}; }
 
-JD bool is_bonded(const StepArgs& a, int64_t i, int32_t j) {
+template <class Real>
+JD bool is_bonded(const StepArgsT<Real>& a, int64_t i, int32_t j) {
   for (int w = 0; w < a.bond_w; ++w)
     if (a.bond_id[i * a.bond_w + w] == j) return true;
   return false;
 }
 
-template <int LAW>
-JD int species_of(const LawP& L, int64_t i) {
+template <int LAW, class Real>
+JD int species_of(const LawPT<Real>& L, int64_t i) {
   if constexpr (LawTraits<LAW>::routed) return L.species ? L.species[i] : 0;
   return 0;
 }
 
-template <int D, int LAW>
-JD void fill_partner(const StepArgs& a, Side<D>& b, int32_t j) {
+template <int D, int LAW, class Real>
+JD void fill_partner(const StepArgsT<Real>& a, Side<D, Real>& b, int32_t j) {
   if (a.law.n_mat > 1) b.mat = a.mat_id[j];
   b.sp = species_of<LAW>(a.law, j);
   if constexpr (LawTraits<LAW>::kin) {
@@ -51,23 +52,23 @@ JD void fill_partner(const StepArgs& a, Side<D>& b, int32_t j) {
 }
 
 // Evaluate the pair (i <- j) and accumulate.  Returns true if in contact.
-template <int D, int LAW, bool PER>
-JD bool visit(const StepArgs& a, const Side<D>& me, int64_t i, int32_t j, const Vec<D>& xj, float rj, Vec<D>& F,
-              Ang<D>& T, const Vec<D>& xi_old, Vec<D>& xi_new) {
-  const Vec<D> rij = disp_<PER>(me.x, xj, a.dom);
-  const float d_sq = dot(rij, rij);
-  const float R = me.r + rj;
+template <int D, int LAW, bool PER, class Real>
+JD bool visit(const StepArgsT<Real>& a, const Side<D, Real>& me, int64_t i, int32_t j, const Vec<D, Real>& xj,
+              Real rj, Vec<D, Real>& F, Ang<D, Real>& T, const Vec<D, Real>& xi_old, Vec<D, Real>& xi_new) {
+  const Vec<D, Real> rij = disp_<PER>(me.x, xj, a.dom);
+  const Real d_sq = dot(rij, rij);
+  const Real R = me.r + rj;
   // No contact is possible: delta of every law is exactly 0 (|rij| exceeds
   // R by far more than the rounding of d_sq * rsqrt(d_sq)).
-  if (d_sq > (R * R) * 1.0001f) return false;
+  if (d_sq > (R * R) * Real(1.0001)) return false;
   if (j == (int32_t)i) return false;
   if (!a.isb && a.bond_w > 0 && is_bonded(a, i, j)) return false;
-  Side<D> b;
+  Side<D, Real> b;
   b.x = xj;
   b.r = rj;
   b.mat = 0;
   fill_partner<D, LAW>(a, b, j);
-  PairOut<D> o = law_eval<D, LAW>(a.law, me, b, rij, d_sq, xi_old, a.dt);
+  PairOut<D, Real> o = law_eval<D, LAW>(a.law, me, b, rij, d_sq, xi_old, a.dt);
   F = F + o.F;
   if constexpr (LawTraits<LAW>::torque) T = T + o.T;
   if constexpr (LawTraits<LAW>::hist) xi_new = o.xi;
@@ -75,11 +76,12 @@ JD bool visit(const StepArgs& a, const Side<D>& me, int64_t i, int32_t j, const 
 }
 
 // Per-particle force from the three candidate sources.
-template <int D, int LAW, int SRC, bool PER>
-JD void particle_force(const StepArgs& a, int64_t idx, int64_t i, const Side<D>& me, Vec<D>& F, Ang<D>& T) {
-  F = zero_vec<D>();
-  T = zero_ang<D>();
-  Vec<D> xo = zero_vec<D>(), xn;
+template <int D, int LAW, int SRC, bool PER, class Real>
+JD void particle_force(const StepArgsT<Real>& a, int64_t idx, int64_t i, const Side<D, Real>& me, Vec<D, Real>& F,
+                       Ang<D, Real>& T) {
+  F = zero_vec<D, Real>();
+  T = zero_ang<D, Real>();
+  Vec<D, Real> xo = zero_vec<D, Real>(), xn;
   if constexpr (SRC == SRC_NAIVE) {
     for (int64_t j = 0; j < a.n; ++j) {
       visit<D, LAW, PER>(a, me, i, (int32_t)j, ld<D>(a.pos_in, j), a.rad[j], F, T, xo, xn);
@@ -121,8 +123,13 @@ JD void particle_force(const StepArgs& a, int64_t idx, int64_t i, const Side<D>&
           const int h = cx * a.grid.strides[0] + cy * a.grid.strides[1] + (D == 3 ? cz * a.grid.strides[2] : 0);
           const int k1 = a.cend[h];
           for (int k = a.cstart[h]; k < k1; ++k) {
-            const float4 p = a.spos[k];
-            visit<D, LAW, PER>(a, me, i, a.perm[k], v4s<D>(p), p.w, F, T, xo, xn);
+            if constexpr (std::is_same<Real, float>::value) {
+              const float4 p = a.spos[k];
+              visit<D, LAW, PER>(a, me, i, a.perm[k], v4s<D>(p), p.w, F, T, xo, xn);
+            } else {  // float64: the partner's own arrays
+              const int32_t j = a.perm[k];
+              visit<D, LAW, PER>(a, me, i, j, ld<D>(a.pos_in, j), a.rad[j], F, T, xo, xn);
+            }
           }
         }
       }
@@ -140,11 +147,11 @@ JD void particle_force(const StepArgs& a, int64_t idx, int64_t i, const Side<D>&
           done = true;
           break;
         }
-        Vec<D> xold = zero_vec<D>();
+        Vec<D, Real> xold = zero_vec<D, Real>();
         if constexpr (LawTraits<LAW>::hist) {
           if ((live_old >> (sl & 31)) & 1u) xold = ld<D>(a.hist_xi, i * a.M + sl);
         }
-        Vec<D> xnew;
+        Vec<D, Real> xnew;
         const bool c = visit<D, LAW, PER>(a, me, i, j, ld<D>(a.pos_in, j), a.rad[j], F, T, xold, xnew);
         if constexpr (LawTraits<LAW>::hist) {
           if (c) {
@@ -163,8 +170,8 @@ JD void particle_force(const StepArgs& a, int64_t idx, int64_t i, const Side<D>&
   }
 }
 
-template <int D, int LAW>
-JD void load_side(const StepArgs& a, int64_t i, Side<D>& me) {
+template <int D, int LAW, class Real>
+JD void load_side(const StepArgsT<Real>& a, int64_t i, Side<D, Real>& me) {
   me.x = ld<D>(a.pos_in, i);
   me.r = a.rad[i];
   me.mat = (a.law.n_mat > 1) ? a.mat_id[i] : 0;
@@ -178,8 +185,8 @@ JD void load_side(const StepArgs& a, int64_t i, Side<D>& me) {
 // current positions (else they are read from force/torque); do_after runs
 // ForceManager + step_after_force of step k; do_before runs domain.apply +
 // step_before_force of step k+1 and the NeighborList displacement check.
-template <int D, int LAW, int SRC, bool PER>
-__global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
+template <int D, int LAW, int SRC, bool PER, class Real>
+__global__ void __launch_bounds__(kBlock) k_step(StepArgsT<Real> a) {
   if (a.flag_rd && *a.flag_rd) {  // a neighbour-list rebuild is pending
     if (a.flag_wr) *a.flag_wr = *a.flag_rd;
     return;
@@ -187,13 +194,13 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= a.n) return;
   const int64_t i = (SRC == SRC_CELLS && a.do_force) ? (int64_t)a.perm[idx] : idx;
-  Side<D> me;
+  Side<D, Real> me;
   load_side<D, LAW>(a, i, me);
-  Vec<D> F;
-  Ang<D> T;
+  Vec<D, Real> F;
+  Ang<D, Real> T;
   if (a.do_force && a.force_zero) {
-    F = zero_vec<D>();
-    T = zero_ang<D>();
+    F = zero_vec<D, Real>();
+    T = zero_ang<D, Real>();
   } else if (a.do_force) {
     particle_force<D, LAW, SRC, PER>(a, idx, i, me, F, T);
   } else {
@@ -205,33 +212,33 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     sta(a.torque, i, T);
     return;
   }
-  Vec<D> x = me.x, v = me.v;
-  Ang<D> w = me.w;
-  Quat q = ldq(a.q, i);
-  const Ang<D> I = lda<D>(a.inertia, i);
-  const float m = me.m;
+  Vec<D, Real> x = me.x, v = me.v;
+  Ang<D, Real> w = me.w;
+  QuatT<Real> q = ldq(a.q, i);
+  const Ang<D, Real> I = lda<D>(a.inertia, i);
+  const Real m = me.m;
   const bool fixed = a.fixed[i] != 0;
-  const float freef = fixed ? 0.0f : 1.0f;
+  const Real freef = fixed ? Real(0) : Real(1);
   if (a.do_after) {
     // ForceManager.apply for spheres (force_manager.py:358-424)
     if (a.has_fm) {
-      Vec<D> fp = a.ext_f ? ld<D>(a.ext_f, i) : zero_vec<D>();
-      Vec<D> fc = a.ext_fc ? ld<D>(a.ext_fc, i) : zero_vec<D>();
-      Vec<D> g;
+      Vec<D, Real> fp = a.ext_f ? ld<D>(a.ext_f, i) : zero_vec<D, Real>();
+      Vec<D, Real> fc = a.ext_fc ? ld<D>(a.ext_fc, i) : zero_vec<D, Real>();
+      Vec<D, Real> g;
       if constexpr (D == 3)
-        g = Vec<3>{a.g[0], a.g[1], a.g[2]};
+        g = Vec<3, Real>{a.g[0], a.g[1], a.g[2]};
       else
-        g = Vec<2>{a.g[0], a.g[1]};
-      fc = fc + g * (m / 1.0f);
+        g = Vec<2, Real>{a.g[0], a.g[1]};
+      fc = fc + g * (m / Real(1));
       F = (F + fp) + fc;
       if (a.ext_t) T = T + lda<D>(a.ext_t, i);
     }
     // step_after_force
     if (a.lin == JDEM_LIN_VERLET) {
-      const float s = (a.dt * 0.5f) / m;
+      const Real s = (a.dt * Real(0.5)) / m;
       v = v + (F * s) * freef;
     } else if (a.lin == JDEM_LIN_EULER) {
-      const float s = a.dt / m;
+      const Real s = a.dt / m;
       v = v + (F * s) * freef;
       x = x + v * a.dt;
     }
@@ -245,7 +252,7 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
     if (a.dom.kind == JDEM_DOMAIN_REFLECT) reflect_apply<D>(a.dom, x, v, q, F, m, me.r, fixed, a.dt);
     // step_before_force of step k+1
     if (a.lin == JDEM_LIN_VERLET) {
-      const float s = (a.dt * 0.5f) / m;
+      const Real s = (a.dt * Real(0.5)) / m;
       v = v + (F * s) * freef;
       x = x + v * a.dt;
     }
@@ -258,34 +265,34 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgs a) {
   st(a.force, i, F);
   sta(a.torque, i, T);
   if (a.do_before && a.check_disp) {
-    const Vec<D> d = x - ld<D>(a.old_pos, i);
+    const Vec<D, Real> d = x - ld<D>(a.old_pos, i);
     if (dot(d, d) > a.trigger) *a.flag_wr = a.step_idx + 1;
   }
 }
 
 // Potential energy: 1/2 sum of pair energies per particle, float64 sum.
-template <int D, int LAW, int SRC, bool PER>
-__global__ void __launch_bounds__(kBlock) k_energy(StepArgs a) {
+template <int D, int LAW, int SRC, bool PER, class Real>
+__global__ void __launch_bounds__(kBlock) k_energy(StepArgsT<Real> a) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= a.n) return;
   const int64_t i = (SRC == SRC_CELLS) ? (int64_t)a.perm[idx] : idx;
-  Side<D> me;
+  Side<D, Real> me;
   me.x = ld<D>(a.pos_in, i);
   me.r = a.rad[i];
   me.mat = (a.law.n_mat > 1) ? a.mat_id[i] : 0;
   me.sp = species_of<LAW>(a.law, i);
   double acc = 0.0;
-  auto one = [&](int32_t j, Vec<D> xj, float rj) {
+  auto one = [&](int32_t j, Vec<D, Real> xj, Real rj) {
     if (j == (int32_t)i) return;
     if (!a.isb && a.bond_w > 0 && is_bonded(a, i, j)) return;
-    const Vec<D> rij = disp_<PER>(me.x, xj, a.dom);
-    const float d_sq = dot(rij, rij);
-    Side<D> b;
+    const Vec<D, Real> rij = disp_<PER>(me.x, xj, a.dom);
+    const Real d_sq = dot(rij, rij);
+    Side<D, Real> b;
     b.x = xj;
     b.r = rj;
     b.mat = (a.law.n_mat > 1) ? a.mat_id[j] : 0;
     b.sp = species_of<LAW>(a.law, j);
-    const float e = law_energy<D, LAW>(a.law, me, b, d_sq);
+    const Real e = law_energy<D, LAW>(a.law, me, b, d_sq);
     acc += 0.5 * (double)e;
   };
   if constexpr (SRC == SRC_NAIVE) {
@@ -324,8 +331,13 @@ __global__ void __launch_bounds__(kBlock) k_energy(StepArgs a) {
           }
           if (!ok) continue;
           for (int k = a.cstart[h]; k < a.cend[h]; ++k) {
-            const float4 p = a.spos[k];
-            one(a.perm[k], v4s<D>(p), p.w);
+            if constexpr (std::is_same<Real, float>::value) {
+              const float4 p = a.spos[k];
+              one(a.perm[k], v4s<D>(p), p.w);
+            } else {
+              const int32_t j = a.perm[k];
+              one(j, ld<D>(a.pos_in, j), a.rad[j]);
+            }
           }
         }
   }
@@ -334,39 +346,39 @@ __global__ void __launch_bounds__(kBlock) k_energy(StepArgs a) {
 
 // Contact virial sum_{i<j} r_ij (x) F_ij (contacts.py:89-164): each i-side
 // pair contributes 1/2 r_ij (x) F_ij, float64.
-template <int D, int LAW, int SRC, bool PER>
-__global__ void __launch_bounds__(kBlock) k_virial(StepArgs a) {
+template <int D, int LAW, int SRC, bool PER, class Real>
+__global__ void __launch_bounds__(kBlock) k_virial(StepArgsT<Real> a) {
   const int64_t idx = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= a.n) return;
   const int64_t i = (SRC == SRC_CELLS) ? (int64_t)a.perm[idx] : idx;
-  Side<D> me;
+  Side<D, Real> me;
   load_side<D, LAW>(a, i, me);
   double W[D * D];
   for (int c = 0; c < D * D; ++c) W[c] = 0.0;
-  auto one = [&](int32_t j, Vec<D> xj, float rj, Vec<D> xold) {
+  auto one = [&](int32_t j, Vec<D, Real> xj, Real rj, Vec<D, Real> xold) {
     if (j == (int32_t)i) return;
     if (!a.isb && a.bond_w > 0 && is_bonded(a, i, j)) return;
-    const Vec<D> rij = disp_<PER>(me.x, xj, a.dom);
-    const float d_sq = dot(rij, rij);
-    Side<D> b;
+    const Vec<D, Real> rij = disp_<PER>(me.x, xj, a.dom);
+    const Real d_sq = dot(rij, rij);
+    Side<D, Real> b;
     b.x = xj;
     b.r = rj;
     b.mat = 0;
     fill_partner<D, LAW>(a, b, j);
-    PairOut<D> o = law_eval<D, LAW>(a.law, me, b, rij, d_sq, xold, a.dt);
-    const float* r = &rij.x;
-    const float* f = &o.F.x;
+    PairOut<D, Real> o = law_eval<D, LAW>(a.law, me, b, rij, d_sq, xold, a.dt);
+    const Real* r = &rij.x;
+    const Real* f = &o.F.x;
     for (int p = 0; p < D; ++p)
       for (int c = 0; c < D; ++c) W[p * D + c] += 0.5 * ((double)r[p] * (double)f[c]);
   };
   if constexpr (SRC == SRC_NAIVE) {
-    for (int64_t j = 0; j < a.n; ++j) one((int32_t)j, ld<D>(a.pos_in, j), a.rad[j], zero_vec<D>());
+    for (int64_t j = 0; j < a.n; ++j) one((int32_t)j, ld<D>(a.pos_in, j), a.rad[j], zero_vec<D, Real>());
   } else if constexpr (SRC == SRC_NLIST) {
     const int32_t* row = a.nl + i * a.M;
     for (int sl = 0; sl < a.M; ++sl) {
       const int32_t j = row[sl];
       if (j < 0) break;
-      Vec<D> xold = zero_vec<D>();
+      Vec<D, Real> xold = zero_vec<D, Real>();
       if constexpr (LawTraits<LAW>::hist) {
         if ((a.hist_live[i * a.W32 + (sl >> 5)] >> (sl & 31)) & 1u) xold = ld<D>(a.hist_xi, i * a.M + sl);
       }
@@ -399,8 +411,13 @@ __global__ void __launch_bounds__(kBlock) k_virial(StepArgs a) {
           }
           if (!ok) continue;
           for (int k = a.cstart[h]; k < a.cend[h]; ++k) {
-            const float4 p = a.spos[k];
-            one(a.perm[k], v4s<D>(p), p.w, zero_vec<D>());
+            if constexpr (std::is_same<Real, float>::value) {
+              const float4 p = a.spos[k];
+              one(a.perm[k], v4s<D>(p), p.w, zero_vec<D>());
+            } else {
+              const int32_t j = a.perm[k];
+              one(j, ld<D>(a.pos_in, j), a.rad[j], zero_vec<D, Real>());
+            }
           }
         }
   }
@@ -408,8 +425,8 @@ __global__ void __launch_bounds__(kBlock) k_virial(StepArgs a) {
 }
 
 // Kinetic energy per particle (utils/thermal.py:83-122), float64.
-template <int D>
-__global__ void __launch_bounds__(kBlock) k_kinetic(StepArgs a) {
+template <int D, class Real>
+__global__ void __launch_bounds__(kBlock) k_kinetic(StepArgsT<Real> a) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
   const double m = a.mass[i];
@@ -512,40 +529,45 @@ __global__ void __launch_bounds__(kBlock) k_stage(StepArgs a, int stage) {
 }
 
 // ---------------------------------------------------------------------------
-// dispatch
+// dispatch: every collider source and law in float; in double (the float64
+// form of the plain step) the sources without a neighbour list and the laws
+// without history.
 // ---------------------------------------------------------------------------
-template <template <int, int, int, bool> class K>
-struct Dispatch;
+template <class Real>
+constexpr bool kIsF32 = std::is_same<Real, float>::value;
 
 #define JDEM_DISPATCH_BODY(KERNEL)                                                               \
   template <int D, int LAW>                                                                      \
-  static hipError_t by_src(const StepArgs& a, int src, bool per, dim3 g, dim3 b, hipStream_t s) { \
+  static hipError_t by_src(const StepArgsT<Real>& a, int src, bool per, dim3 g, dim3 b, hipStream_t s) { \
     switch (src) {                                                                               \
       case SRC_NAIVE:                                                                            \
         if (per)                                                                                 \
-          hipLaunchKernelGGL((KERNEL<D, LAW, SRC_NAIVE, true>), g, b, 0, s, a);                  \
+          hipLaunchKernelGGL((KERNEL<D, LAW, SRC_NAIVE, true, Real>), g, b, 0, s, a);            \
         else                                                                                     \
-          hipLaunchKernelGGL((KERNEL<D, LAW, SRC_NAIVE, false>), g, b, 0, s, a);                 \
+          hipLaunchKernelGGL((KERNEL<D, LAW, SRC_NAIVE, false, Real>), g, b, 0, s, a);           \
         break;                                                                                   \
       case SRC_CELLS:                                                                            \
         if (per)                                                                                 \
-          hipLaunchKernelGGL((KERNEL<D, LAW, SRC_CELLS, true>), g, b, 0, s, a);                  \
+          hipLaunchKernelGGL((KERNEL<D, LAW, SRC_CELLS, true, Real>), g, b, 0, s, a);            \
         else                                                                                     \
-          hipLaunchKernelGGL((KERNEL<D, LAW, SRC_CELLS, false>), g, b, 0, s, a);                 \
+          hipLaunchKernelGGL((KERNEL<D, LAW, SRC_CELLS, false, Real>), g, b, 0, s, a);           \
         break;                                                                                   \
       case SRC_NLIST:                                                                            \
-        if (per)                                                                                 \
-          hipLaunchKernelGGL((KERNEL<D, LAW, SRC_NLIST, true>), g, b, 0, s, a);                  \
-        else                                                                                     \
-          hipLaunchKernelGGL((KERNEL<D, LAW, SRC_NLIST, false>), g, b, 0, s, a);                 \
-        break;                                                                                   \
+        if constexpr (kIsF32<Real>) {                                                            \
+          if (per)                                                                               \
+            hipLaunchKernelGGL((KERNEL<D, LAW, SRC_NLIST, true, Real>), g, b, 0, s, a);          \
+          else                                                                                   \
+            hipLaunchKernelGGL((KERNEL<D, LAW, SRC_NLIST, false, Real>), g, b, 0, s, a);         \
+          break;                                                                                 \
+        }                                                                                        \
+        return hipErrorInvalidValue;                                                             \
       default:                                                                                   \
         return hipErrorInvalidValue;                                                             \
     }                                                                                            \
     return hipGetLastError();                                                                    \
   }                                                                                              \
   template <int D>                                                                               \
-  static hipError_t by_law(const StepArgs& a, int law, int src, bool per, dim3 g, dim3 b, hipStream_t s) { \
+  static hipError_t by_law(const StepArgsT<Real>& a, int law, int src, bool per, dim3 g, dim3 b, hipStream_t s) { \
     switch (law) {                                                                               \
       case JDEM_LAW_SPRING:                                                                      \
         return by_src<D, JDEM_LAW_SPRING>(a, src, per, g, b, s);                                 \
@@ -553,52 +575,68 @@ struct Dispatch;
         return by_src<D, JDEM_LAW_HERTZ>(a, src, per, g, b, s);                                  \
       case JDEM_LAW_CUNDALL_STRACK:                                                              \
         return by_src<D, JDEM_LAW_CUNDALL_STRACK>(a, src, per, g, b, s);                         \
-      case JDEM_LAW_SPRING_FRICTION:                                                             \
-        return by_src<D, JDEM_LAW_SPRING_FRICTION>(a, src, per, g, b, s);                        \
-      case JDEM_LAW_HERTZ_FRICTION:                                                              \
-        return by_src<D, JDEM_LAW_HERTZ_FRICTION>(a, src, per, g, b, s);                         \
       case kLawRouted:                                                                           \
         return by_src<D, kLawRouted>(a, src, per, g, b, s);                                      \
+      case JDEM_LAW_SPRING_FRICTION:                                                             \
+        if constexpr (kIsF32<Real>) return by_src<D, JDEM_LAW_SPRING_FRICTION>(a, src, per, g, b, s); \
+        return hipErrorInvalidValue;                                                             \
+      case JDEM_LAW_HERTZ_FRICTION:                                                              \
+        if constexpr (kIsF32<Real>) return by_src<D, JDEM_LAW_HERTZ_FRICTION>(a, src, per, g, b, s); \
+        return hipErrorInvalidValue;                                                             \
       case kLawRoutedHist:                                                                       \
-        return by_src<D, kLawRoutedHist>(a, src, per, g, b, s);                                  \
+        if constexpr (kIsF32<Real>) return by_src<D, kLawRoutedHist>(a, src, per, g, b, s);     \
+        return hipErrorInvalidValue;                                                             \
       default:                                                                                   \
         return hipErrorInvalidValue;                                                             \
     }                                                                                            \
   }                                                                                              \
-  static hipError_t run(const StepArgs& a, int dim, int law, int src, hipStream_t s) {           \
+  static hipError_t run(const StepArgsT<Real>& a, int dim, int law, int src, hipStream_t s) {    \
     if (a.n == 0) return hipSuccess;                                                             \
     dim3 g(nblocks(a.n)), b(kBlock);                                                             \
     const bool per = a.dom.kind == JDEM_DOMAIN_PERIODIC;                                         \
     return dim == 3 ? by_law<3>(a, law, src, per, g, b, s) : by_law<2>(a, law, src, per, g, b, s); \
   }
 
+template <class Real>
 struct StepDispatch {
   JDEM_DISPATCH_BODY(k_step)
 };
+template <class Real>
 struct EnergyDispatch {
   JDEM_DISPATCH_BODY(k_energy)
 };
+template <class Real>
 struct VirialDispatch {
   JDEM_DISPATCH_BODY(k_virial)
 };
 
-hipError_t launch_step(const StepArgs& a, int dim, int law, int src, hipStream_t s) {
-  return StepDispatch::run(a, dim, law, src, s);
-}
-hipError_t launch_energy(const StepArgs& a, int dim, int law, int src, hipStream_t s) {
-  return EnergyDispatch::run(a, dim, law, src, s);
-}
-hipError_t launch_virial(const StepArgs& a, int dim, int law, int src, hipStream_t s) {
-  return VirialDispatch::run(a, dim, law, src, s);
-}
-hipError_t launch_kinetic(const StepArgs& a, int dim, hipStream_t s) {
+template <class Real>
+hipError_t kinetic_run(const StepArgsT<Real>& a, int dim, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   if (dim == 3)
-    hipLaunchKernelGGL(k_kinetic<3>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_kinetic<3, Real>), dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
   else
-    hipLaunchKernelGGL(k_kinetic<2>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_kinetic<2, Real>), dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
+
+hipError_t launch_step(const StepArgs& a, int dim, int law, int src, hipStream_t s) {
+  return StepDispatch<float>::run(a, dim, law, src, s);
+}
+hipError_t launch_energy(const StepArgs& a, int dim, int law, int src, hipStream_t s) {
+  return EnergyDispatch<float>::run(a, dim, law, src, s);
+}
+hipError_t launch_virial(const StepArgs& a, int dim, int law, int src, hipStream_t s) {
+  return VirialDispatch<float>::run(a, dim, law, src, s);
+}
+hipError_t launch_kinetic(const StepArgs& a, int dim, hipStream_t s) { return kinetic_run(a, dim, s); }
+hipError_t launch_step(const StepArgs64& a, int dim, int law, int src, hipStream_t s) {
+  return StepDispatch<double>::run(a, dim, law, src, s);
+}
+hipError_t launch_energy(const StepArgs64& a, int dim, int law, int src, hipStream_t s) {
+  return EnergyDispatch<double>::run(a, dim, law, src, s);
+}
+hipError_t launch_kinetic(const StepArgs64& a, int dim, hipStream_t s) { return kinetic_run(a, dim, s); }
 hipError_t launch_stage(const StepArgs& a, int dim, int stage, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   if (dim == 3)

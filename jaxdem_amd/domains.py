@@ -1,8 +1,8 @@
 """Simulation domains (jaxdem/domains/__init__.py:28-215, free.py, periodic.py,
 reflect.py).
 
-A domain carries ``box_size``, ``inv_box_size`` and ``anchor`` as float32
-host arrays of shape (dim,).  Its per-particle effects run on the GPU: the
+A domain carries ``box_size``, ``inv_box_size`` and ``anchor`` as host
+arrays of shape (dim,), float32 (float64 with ``jax_enable_x64``).  Its per-particle effects run on the GPU: the
 periodic minimum image inside the pair kernels, the reflective wall impulse
 and the free-domain bounding box inside the fused step.
 """
@@ -16,15 +16,16 @@ import numpy as np
 import torch
 
 from . import _lib as L
+from .config import real_dtype
 from .factory import Factory
 
 
 def _arr(x, dim: int, default: float, name: str) -> np.ndarray:
     if x is None:
-        return np.full(dim, default, np.float32)
+        return np.full(dim, default, real_dtype())
     if isinstance(x, torch.Tensor):
         x = x.detach().cpu().numpy()
-    a = np.asarray(x, dtype=np.float32)
+    a = np.asarray(x, dtype=real_dtype())
     if a.shape != (dim,):
         raise ValueError(f"{name} must have shape ({dim},), got shape {a.shape}.")
     return a
@@ -49,7 +50,8 @@ class Domain(Factory):
         """domains/__init__.py:62-120."""
         box = _arr(box_size, dim, 1.0, "box_size")
         anc = _arr(anchor, dim, 0.0, "anchor")
-        return cls(box_size=box, inv_box_size=(np.float32(1.0) / box).astype(np.float32), anchor=anc, **kw)
+        ft = box.dtype.type
+        return cls(box_size=box, inv_box_size=(ft(1.0) / box).astype(ft), anchor=anc, **kw)
 
     @staticmethod
     def displacement(ri: torch.Tensor, rj: torch.Tensor, system) -> torch.Tensor:
@@ -71,8 +73,9 @@ class Domain(Factory):
     def shift(state, system):
         return state, system
 
-    def to_c(self, dim: int) -> L.JdemDomain:
-        d = L.JdemDomain()
+    def to_c(self, dim: int, f64: bool = False):
+        """jdem_domain (jdem_domain_f64 with ``f64``)."""
+        d = L.JdemDomainF64() if f64 else L.JdemDomain()
         d.kind = self.kind
         d.dim = dim
         for a in range(dim):
@@ -146,7 +149,8 @@ class ReflectDomain(Domain):
         if not (0.0 < restitution_coefficient <= 1.0):
             raise ValueError(f"restitution_coefficient must be in (0, 1], got {restitution_coefficient}.")
         return super(ReflectDomain, cls).Create(
-            dim, box_size=box_size, anchor=anchor, restitution_coefficient=float(np.float32(restitution_coefficient)), **kw
+            dim, box_size=box_size, anchor=anchor, restitution_coefficient=float(real_dtype()(restitution_coefficient)),
+            **kw
         )
 
 

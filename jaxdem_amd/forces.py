@@ -18,7 +18,8 @@ ForceModel (forces/__init__.py:20-150) registers the laws by key:
 
 Each law converts a MaterialTable into the float32 device tables the kernels
 read (``tables``), computed with the same float32 operations as the
-reference's per-pair formulas so the kernels reproduce them bit for bit.
+reference's per-pair formulas so the kernels reproduce them bit for bit
+(float64 tables, by the same operations, for the float64 step).
 """
 
 from __future__ import annotations
@@ -30,19 +31,20 @@ import numpy as np
 import torch
 
 from . import _lib as L
+from .config import real_dtype
 from .factory import Factory
 
 F32 = np.float32
 
 
-def _hertz_estar(young: np.ndarray, poisson: np.ndarray) -> np.ndarray:
-    """E* = 1/((1-nu_i^2)/E_i + (1-nu_j^2)/E_j) (hertz.py:107), float32."""
-    one = F32(1.0)
-    E = young.astype(F32)
-    nu = poisson.astype(F32)
+def _hertz_estar(young: np.ndarray, poisson: np.ndarray, ft=F32) -> np.ndarray:
+    """E* = 1/((1-nu_i^2)/E_i + (1-nu_j^2)/E_j) (hertz.py:107), in ``ft``."""
+    one = ft(1.0)
+    E = young.astype(ft)
+    nu = poisson.astype(ft)
     Ei, Ej = E[:, None], E[None, :]
     ni, nj = nu[:, None], nu[None, :]
-    return (one / ((one - ni * ni) / Ei + (one - nj * nj) / Ej)).astype(F32)
+    return (one / ((one - ni * ni) / Ei + (one - nj * nj) / Ej)).astype(ft)
 
 
 @dataclass
@@ -91,8 +93,8 @@ class SpringForce(ForceModel):
     def required_material_properties(self) -> tuple:
         return ("young_eff",)
 
-    def tables(self, mt) -> dict:
-        return {"t0": np.asarray(mt.young_eff, F32)}
+    def tables(self, mt, ft=F32) -> dict:
+        return {"t0": np.asarray(mt.young_eff, ft)}
 
 
 @ForceModel.register("hertz")
@@ -106,8 +108,8 @@ class HertzianForce(ForceModel):
     def required_material_properties(self) -> tuple:
         return ("young", "poisson")
 
-    def tables(self, mt) -> dict:
-        return {"t0": (F32(4.0 / 3.0) * _hertz_estar(np.asarray(mt.young), np.asarray(mt.poisson))).astype(F32)}
+    def tables(self, mt, ft=F32) -> dict:
+        return {"t0": (ft(4.0 / 3.0) * _hertz_estar(np.asarray(mt.young), np.asarray(mt.poisson), ft)).astype(ft)}
 
 
 @ForceModel.register("cundallstrack")
@@ -122,23 +124,23 @@ class CundallStrackForce(ForceModel):
     def required_material_properties(self) -> tuple:
         return ("young", "poisson", "e", "mu", "mu_r")
 
-    def tables(self, mt) -> dict:
-        one = F32(1.0)
-        E = np.asarray(mt.young, F32)
-        nu = np.asarray(mt.poisson, F32)
-        G = (E / (2.0 * (one + nu))).astype(F32)
-        e = np.asarray(mt.e, F32)
+    def tables(self, mt, ft=F32) -> dict:
+        one = ft(1.0)
+        E = np.asarray(mt.young, ft)
+        nu = np.asarray(mt.poisson, ft)
+        G = (E / (2.0 * (one + nu))).astype(ft)
+        e = np.asarray(mt.e, ft)
         e_eff = np.minimum(e[:, None], e[None, :])
         e_safe = np.where(e_eff > 0.0, e_eff, one)
         ln_e = np.log(e_safe)
-        pi2 = F32(np.pi**2)
-        beta = np.where(e_eff > 0.0, -ln_e / np.sqrt(pi2 + ln_e * ln_e), one).astype(F32)
+        pi2 = ft(np.pi**2)
+        beta = np.where(e_eff > 0.0, -ln_e / np.sqrt(pi2 + ln_e * ln_e), one).astype(ft)
         return {
             "t0": beta,
             "m0": E,
             "m1": G,
-            "m2": np.asarray(mt.mu, F32),
-            "m3": np.asarray(mt.mu_r, F32),
+            "m2": np.asarray(mt.mu, ft),
+            "m3": np.asarray(mt.mu_r, ft),
         }
 
 
@@ -170,13 +172,13 @@ class SpringFrictionForce(_HistoryLaw):
     def required_material_properties(self) -> tuple:
         return ("young_eff", "mu_eff")
 
-    def tables(self, mt) -> dict:
-        kn = np.asarray(mt.young_eff, F32)
+    def tables(self, mt, ft=F32) -> dict:
+        kn = np.asarray(mt.young_eff, ft)
         return {
             "t0": kn,
-            "t1": (F32(self.kt_ratio) * kn).astype(F32),
-            "t2": np.asarray(mt.mu_eff, F32),
-            "p0": float(F32(2.0 * self.damping)),
+            "t1": (ft(self.kt_ratio) * kn).astype(ft),
+            "t2": np.asarray(mt.mu_eff, ft),
+            "p0": float(ft(2.0 * self.damping)),
         }
 
 
@@ -191,17 +193,17 @@ class HertzFrictionForce(_HistoryLaw):
     def required_material_properties(self) -> tuple:
         return ("young", "poisson", "mu_eff")
 
-    def tables(self, mt) -> dict:
-        one = F32(1.0)
-        E = np.asarray(mt.young, F32)
-        nu = np.asarray(mt.poisson, F32)
-        G = (E / (2.0 * (one + nu))).astype(F32)
-        G_star = (one / ((2.0 - nu[:, None]) / G[:, None] + (2.0 - nu[None, :]) / G[None, :])).astype(F32)
+    def tables(self, mt, ft=F32) -> dict:
+        one = ft(1.0)
+        E = np.asarray(mt.young, ft)
+        nu = np.asarray(mt.poisson, ft)
+        G = (E / (2.0 * (one + nu))).astype(ft)
+        G_star = (one / ((2.0 - nu[:, None]) / G[:, None] + (2.0 - nu[None, :]) / G[None, :])).astype(ft)
         return {
-            "t0": (F32(4.0 / 3.0) * _hertz_estar(E, nu)).astype(F32),
-            "t1": (F32(8.0) * G_star).astype(F32),
-            "t2": np.asarray(mt.mu_eff, F32),
-            "p0": float(F32(2.0 * self.damping)),
+            "t0": (ft(4.0 / 3.0) * _hertz_estar(E, nu, ft)).astype(ft),
+            "t1": (ft(8.0) * G_star).astype(ft),
+            "t2": np.asarray(mt.mu_eff, ft),
+            "p0": float(ft(2.0 * self.damping)),
         }
 
 
@@ -296,23 +298,25 @@ def route_layout(model: ForceModel) -> tuple:
     return S, np.asarray(off, np.int32), terms
 
 
-def law_struct(model: ForceModel, mat_table, device) -> tuple:
-    """Build the jdem_law struct; returns (struct, keepalive tensors)."""
+def law_struct(model: ForceModel, mat_table, device, f64: bool = False) -> tuple:
+    """Build the jdem_law struct (jdem_law_f64 with ``f64``); returns
+    (struct, keepalive tensors)."""
     if model.kind == L.LAW_ROUTED:
-        return _routed_struct(model, mat_table, device)
-    law = L.JdemLaw()
+        return _routed_struct(model, mat_table, device, f64)
+    ft = np.float64 if f64 else F32
+    law = L.JdemLawF64() if f64 else L.JdemLaw()
     law.kind = model.kind
     law.n_mat = len(mat_table)
     cache = mat_table._device_cache
-    key = (type(model).__name__, getattr(model, "kt_ratio", None), getattr(model, "damping", None), str(device))
+    key = (type(model).__name__, getattr(model, "kt_ratio", None), getattr(model, "damping", None), str(device), f64)
     if key not in cache:
-        tabs = model.tables(mat_table)
+        tabs = model.tables(mat_table, ft)
         dev = {}
         for k, v in tabs.items():
             if k in ("p0", "p1"):
                 dev[k] = v
             else:
-                dev[k] = torch.as_tensor(np.ascontiguousarray(v, F32), device=device)
+                dev[k] = torch.as_tensor(np.ascontiguousarray(v, ft), device=device)
         cache[key] = dev
     dev = cache[key]
     for k in ("t0", "t1", "t2", "m0", "m1", "m2", "m3"):
@@ -323,22 +327,23 @@ def law_struct(model: ForceModel, mat_table, device) -> tuple:
     return law, dev
 
 
-def _routed_struct(model: ForceModel, mat_table, device) -> tuple:
+def _routed_struct(model: ForceModel, mat_table, device, f64: bool = False) -> tuple:
     """jdem_law of a ForceRouter / LawCombiner: host arrays of the species
     offsets and the elementary terms (each with its own device tables); the
     caller sets ``species`` to State.species_id when n_species > 1."""
     S, off, terms = route_layout(model)
-    arr = (L.JdemLaw * len(terms))()
+    cls = L.JdemLawF64 if f64 else L.JdemLaw
+    arr = (cls * len(terms))()
     keep: list = [off, arr]
     for k, t in enumerate(terms):
         if t is None:
             arr[k].kind = L.LAW_NONE
             arr[k].n_mat = len(mat_table)
         else:
-            sub, dev = law_struct(t, mat_table, device)
+            sub, dev = law_struct(t, mat_table, device, f64)
             arr[k] = sub
             keep.append(dev)
-    law = L.JdemLaw()
+    law = cls()
     law.kind = L.LAW_ROUTED
     law.n_mat = len(mat_table)
     law.n_species = S
@@ -376,7 +381,8 @@ class ForceManager:
     def create(state_shape: tuple, *, gravity=None, force_functions: Sequence = ()) -> "ForceManager":
         """force_manager.py:82-190."""
         dim = state_shape[-1]
-        g = np.zeros(dim, F32) if gravity is None else np.asarray(gravity, F32).reshape(dim)
+        ft = real_dtype()
+        g = np.zeros(dim, ft) if gravity is None else np.asarray(gravity, ft).reshape(dim)
         funcs, energies, com = [], [], []
         for entry in force_functions:
             args = tuple(entry) if isinstance(entry, (tuple, list)) else (entry,)
@@ -414,7 +420,7 @@ class ForceManager:
     @staticmethod
     def add_force(state, system, force, *, is_com: bool = False):
         """force_manager.py:113-149 (spheres: clump count 1)."""
-        f = torch.as_tensor(force, dtype=torch.float32, device=state.device).expand(state.N, state.dim)
+        f = torch.as_tensor(force, dtype=state.pos_c.dtype, device=state.device).expand(state.N, state.dim)
         fm = system.force_manager
         if is_com:
             fm.external_force_com = (fm.external_force_com if fm.external_force_com is not None else 0) + f
@@ -426,15 +432,15 @@ class ForceManager:
     def add_force_at(state, system, force, idx, *, is_com: bool = False):
         """force_manager.py:153-187."""
         idx_t = torch.as_tensor(idx, dtype=torch.long, device=state.device)
-        add = torch.zeros((state.N, state.dim), dtype=torch.float32, device=state.device)
-        add.index_add_(0, idx_t.reshape(-1), torch.as_tensor(force, dtype=torch.float32, device=state.device).reshape(-1, state.dim))
+        add = torch.zeros((state.N, state.dim), dtype=state.pos_c.dtype, device=state.device)
+        add.index_add_(0, idx_t.reshape(-1), torch.as_tensor(force, dtype=state.pos_c.dtype, device=state.device).reshape(-1, state.dim))
         return ForceManager.add_force(state, system, add, is_com=is_com)
 
     @staticmethod
     def add_torque(state, system, torque):
         """force_manager.py:191-221."""
         A = 3 if state.dim == 3 else 1
-        t = torch.as_tensor(torque, dtype=torch.float32, device=state.device).expand(state.N, A)
+        t = torch.as_tensor(torque, dtype=state.pos_c.dtype, device=state.device).expand(state.N, A)
         fm = system.force_manager
         fm.external_torque = (fm.external_torque if fm.external_torque is not None else 0) + t
         return system
@@ -443,8 +449,8 @@ class ForceManager:
     def add_torque_at(state, system, torque, idx):
         A = 3 if state.dim == 3 else 1
         idx_t = torch.as_tensor(idx, dtype=torch.long, device=state.device)
-        add = torch.zeros((state.N, A), dtype=torch.float32, device=state.device)
-        add.index_add_(0, idx_t.reshape(-1), torch.as_tensor(torque, dtype=torch.float32, device=state.device).reshape(-1, A))
+        add = torch.zeros((state.N, A), dtype=state.pos_c.dtype, device=state.device)
+        add.index_add_(0, idx_t.reshape(-1), torch.as_tensor(torque, dtype=state.pos_c.dtype, device=state.device).reshape(-1, A))
         return ForceManager.add_torque(state, system, add)
 
     @staticmethod
@@ -458,8 +464,8 @@ class ForceManager:
             pos = state.pos
             for func, com in zip(fm.force_functions, fm.is_com_force):
                 f, t = func(pos, state, system)
-                f = torch.as_tensor(f, dtype=torch.float32, device=state.device)
-                t = torch.as_tensor(t, dtype=torch.float32, device=state.device)
+                f = torch.as_tensor(f, dtype=state.pos_c.dtype, device=state.device)
+                t = torch.as_tensor(t, dtype=state.pos_c.dtype, device=state.device)
                 if com:
                     fm.external_force_com = (fm.external_force_com if fm.external_force_com is not None else 0) + f
                 else:

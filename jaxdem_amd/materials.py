@@ -3,7 +3,8 @@
 Reference: materials/__init__.py (Material), materials/elastic_mats.py
 (``elastic``, ``elasticfrict``), material_matchmakers/harmonic.py:39 and
 linear.py:32, materials/material_table.py:79-196.  Tables hold float32 numpy
-arrays on the host; the force laws turn them into small device tables.
+arrays on the host (float64 with ``jax_enable_x64``); the force laws turn them
+into small device tables.
 """
 
 from __future__ import annotations
@@ -13,6 +14,7 @@ from typing import Any, Sequence
 
 import numpy as np
 
+from .config import real_dtype
 from .factory import Factory
 
 
@@ -54,10 +56,11 @@ class HarmonicMaterialMatchmaker(MaterialMatchmaker):
 
     @staticmethod
     def get_effective_property(p1: np.ndarray, p2: np.ndarray) -> np.ndarray:
+        ft = np.result_type(p1, p2).type
         z = (p1 == 0.0) | (p2 == 0.0)
-        s1 = np.where(z, np.float32(1.0), p1)
-        s2 = np.where(z, np.float32(1.0), p2)
-        return np.where(z, np.float32(0.0), 2.0 * s1 * s2 / (s1 + s2)).astype(np.float32)
+        s1 = np.where(z, ft(1.0), p1)
+        s2 = np.where(z, ft(1.0), p2)
+        return np.where(z, ft(0.0), 2.0 * s1 * s2 / (s1 + s2)).astype(ft)
 
 
 @MaterialMatchmaker.register("linear")
@@ -66,7 +69,7 @@ class LinearMaterialMatchmaker(MaterialMatchmaker):
 
     @staticmethod
     def get_effective_property(p1: np.ndarray, p2: np.ndarray) -> np.ndarray:
-        return ((p1 + p2) / 2).astype(np.float32)
+        return ((p1 + p2) / 2).astype(np.result_type(p1, p2))
 
 
 class MaterialTable:
@@ -90,7 +93,7 @@ class MaterialTable:
             for f in fields(m):
                 if f.name not in keys:
                     keys.append(f.name)
-        scal = {k: np.asarray([getattr(m, k, fill) for m in mats], dtype=np.float32) for k in keys}
+        scal = {k: np.asarray([getattr(m, k, fill) for m in mats], dtype=real_dtype()) for k in keys}
         if matcher is None:
             matcher = MaterialMatchmaker.create("harmonic")
         pair = {f"{k}_eff": matcher.get_effective_property(a[:, None], a[None, :]) for k, a in scal.items()}

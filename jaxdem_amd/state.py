@@ -5,7 +5,8 @@ Field names, shapes and defaults follow the reference ``State``: ``pos_c``,
 ``ang_vel``, ``torque`` and ``inertia`` are (N, 3) in 3D and (N, 1) in 2D;
 ``rad``, ``_rad``, ``volume``, ``mass``, ``clump_id``, ``mat_id``,
 ``species_id`` and ``fixed`` are (N,); ``bond_id`` is (N, W) padded with -1.
-Floats are float32 (the reference's default precision), integers int32.
+Floats are float32 (the reference's default precision; float64 when
+``jax_enable_x64`` is set in ``jaxdem_amd.config``), integers int32.
 
 The GPU time step covers spheres: ``pos_p == 0`` and every particle its own
 clump.  ``State.create`` records whether that holds (``is_sphere_state``);
@@ -20,6 +21,8 @@ from typing import Any, Sequence
 
 import numpy as np
 import torch
+
+from .config import real_dtype, torch_real
 
 
 def default_device() -> torch.device:
@@ -43,11 +46,12 @@ class Quaternion:
     @staticmethod
     def create(w=None, xyz=None, *, device=None) -> "Quaternion":
         dev = device or default_device()
-        w = torch.ones((1, 1)) if w is None else torch.as_tensor(np.asarray(w), dtype=torch.float32)
+        ft = torch_real()
+        w = torch.ones((1, 1)) if w is None else torch.as_tensor(np.asarray(w), dtype=ft)
         if w.ndim == 0:
             w = w[None]
-        xyz = torch.zeros((1, 3)) if xyz is None else torch.as_tensor(np.asarray(xyz), dtype=torch.float32)
-        return Quaternion(torch.cat([w.to(torch.float32), xyz.to(torch.float32)], dim=-1).to(dev))
+        xyz = torch.zeros((1, 3)) if xyz is None else torch.as_tensor(np.asarray(xyz), dtype=ft)
+        return Quaternion(torch.cat([w.to(ft), xyz.to(ft)], dim=-1).to(dev))
 
     @property
     def w(self) -> torch.Tensor:
@@ -210,54 +214,54 @@ class State:
         device=None,
     ) -> "State":
         """``State.create`` (state.py:371-867) for spheres."""
-        f32 = np.float32
+        ft = real_dtype()  # float32, or float64 with jax_enable_x64
         if pos is None:
             d = 3 if dim is None else int(dim)
-            pos_c = np.zeros((0, d), f32)
+            pos_c = np.zeros((0, d), ft)
         else:
-            pos_c = _np(pos, f32)
+            pos_c = _np(pos, ft)
         if pos_c.ndim != 2 or pos_c.shape[-1] not in (2, 3):
             raise ValueError(f"`pos` must have shape (N, 2|3); got {pos_c.shape}")
         N, D = pos_c.shape
         A = 1 if D == 2 else 3
-        pos_p_np = np.zeros_like(pos_c) if pos_p is None else _np(pos_p, f32)
-        vel_np = np.zeros_like(pos_c) if vel is None else _np(vel, f32)
-        force_np = np.zeros_like(pos_c) if force is None else _np(force, f32)
+        pos_p_np = np.zeros_like(pos_c) if pos_p is None else _np(pos_p, ft)
+        vel_np = np.zeros_like(pos_c) if vel is None else _np(vel, ft)
+        force_np = np.zeros_like(pos_c) if force is None else _np(force, ft)
         if q is None:
-            q_np = np.zeros((N, 4), f32)
+            q_np = np.zeros((N, 4), ft)
             q_np[:, 0] = 1.0
         elif isinstance(q, Quaternion):
-            q_np = _np(q.data, f32)
+            q_np = _np(q.data, ft)
         else:
-            q_np = _np(q, f32)
+            q_np = _np(q, ft)
             if q_np.ndim < 2 or q_np.shape[-1] != 4:
                 raise ValueError(f"`q` array must have shape (..., N, 4). Got shape={q_np.shape}.")
-        ang_np = np.zeros((N, A), f32) if ang_vel is None else _np(ang_vel, f32).reshape(N, A)
-        torque_np = np.zeros((N, A), f32) if torque is None else _np(torque, f32).reshape(N, A)
-        rad_np = np.ones(N, f32) if rad is None else _np(rad, f32).reshape(N)
-        vol_np = _hypersphere_volume(rad_np.astype(f32), D).astype(f32) if volume is None else _np(volume, f32)
+        ang_np = np.zeros((N, A), ft) if ang_vel is None else _np(ang_vel, ft).reshape(N, A)
+        torque_np = np.zeros((N, A), ft) if torque is None else _np(torque, ft).reshape(N, A)
+        rad_np = np.ones(N, ft) if rad is None else _np(rad, ft).reshape(N)
+        vol_np = _hypersphere_volume(rad_np.astype(ft), D).astype(ft) if volume is None else _np(volume, ft)
         clump_np = np.arange(N, dtype=np.int64) if clump_id is None else _np(clump_id, np.int64)
         bond_np = np.full((N, 1), -1, np.int64) if bond_id is None else _symmetric_bonds(bond_id, N)
         mat_np = np.zeros(N, np.int64) if mat_id is None else _np(mat_id, np.int64).reshape(N)
         spec_np = np.zeros(N, np.int64) if species_id is None else _np(species_id, np.int64).reshape(N)
         fixed_np = np.zeros(N, bool) if fixed is None else np.asarray(fixed, bool).reshape(N)
         if mat_table is not None:
-            dens = np.asarray(mat_table.density, f32)
-            mass_np = (dens[mat_np] * vol_np).astype(f32)
+            dens = np.asarray(mat_table.density, ft)
+            mass_np = (dens[mat_np] * vol_np).astype(ft)
         else:
-            mass_np = np.ones(N, f32) if mass is None else _np(mass, f32).reshape(N)
-        coeff = np.float32(0.5 if D == 2 else 0.4)
+            mass_np = np.ones(N, ft) if mass is None else _np(mass, ft).reshape(N)
+        coeff = ft(0.5 if D == 2 else 0.4)
         if inertia is None:
-            inertia_np = ((coeff * mass_np * rad_np**2)[:, None] * np.ones((N, A), f32)).astype(f32)
+            inertia_np = ((coeff * mass_np * rad_np**2)[:, None] * np.ones((N, A), ft)).astype(ft)
         else:
-            inertia_np = _np(inertia, f32).reshape(N, A)
+            inertia_np = _np(inertia, ft).reshape(N, A)
         if N > 0:
             _, clump_np = np.unique(clump_np, return_inverse=True)
-        _rad_np = rad_np.copy() if _rad is None else _np(_rad, f32)
+        _rad_np = rad_np.copy() if _rad is None else _np(_rad, ft)
         sphere = bool(np.all(pos_p_np == 0)) and (N == 0 or np.unique(clump_np).size == N)
         dev = torch.device(device) if device is not None else default_device()
 
-        def T(a, dt=torch.float32):
+        def T(a, dt=torch_real()):
             return torch.as_tensor(np.ascontiguousarray(a)).to(device=dev, dtype=dt)
 
         return State(

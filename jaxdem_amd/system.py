@@ -46,6 +46,7 @@ from .integrators import (
     VelocityVerlet,
     VelocityVerletSpiral,
 )
+from .config import real_dtype
 from .materials import Material, MaterialTable
 from .state import Quaternion, State
 
@@ -193,8 +194,8 @@ class System:
             bonded_force_model=None,
             force_model=force_model,
             mat_table=mat_table,
-            dt=F32(dt),
-            time=F32(time),
+            dt=real_dtype()(dt),
+            time=real_dtype()(time),
             dim=dim,
             step_count=0,
             key=key,
@@ -301,13 +302,18 @@ def _alloc_history(collider: NeighborList, state_shape, force_model) -> dict:
 
 
 def _clock_after(t0, dt, n: int):
-    """time after n steps: n sequential float32 additions of dt
-    (system.py:73); ufunc.accumulate adds in order, each sum rounded."""
+    """time after n steps: n sequential additions of dt in the dtype of dt,
+    float32 or float64 (system.py:73); ufunc.accumulate adds in order, each
+    sum rounded."""
+    if isinstance(dt, (np.float32, np.float64)):
+        ft = type(dt)
+    else:  # a Python number takes the default precision, as a weak type in jnp
+        ft = real_dtype()
     if n <= 0:
-        return F32(t0)
-    seq = np.full(n + 1, F32(dt), F32)
-    seq[0] = F32(t0)
-    return F32(np.add.accumulate(seq, dtype=F32)[-1])
+        return ft(t0)
+    seq = np.full(n + 1, ft(dt), ft)
+    seq[0] = ft(t0)
+    return ft(np.add.accumulate(seq, dtype=ft)[-1])
 
 
 def _advance_clock(system: System, n: int) -> System:
@@ -343,8 +349,9 @@ def _ptr(t: torch.Tensor | None) -> int | None:
     return t.data_ptr()
 
 
-def _state_struct(state: State) -> L.JdemState:
-    s = L.JdemState()
+def _state_struct(state: State, cls=L.JdemState):
+    """jdem_state (``cls`` L.JdemStateF64 for jdem_state_f64)."""
+    s = cls()
     s.n = state.N
     s.dim = state.dim
     bw = int(state.bond_id.shape[-1]) if state.has_bonds else 0
@@ -373,21 +380,13 @@ def _system_struct(state: State, system: System, keep: list, resident=None, for_
     list, build positions and history come from that working set; without
     ``for_collider`` the collider is left out (kinetic quantities)."""
     _require_gpu(state)
+    _require_f32(state, "this call")
     S = L.JdemSystem()
     S.state = _state_struct(state)
     S.domain = system.domain.to_c(state.dim)
     law, dev = law_struct(system.force_model, system.mat_table, state.device)
     keep.append(dev)
-    if law.kind == L.LAW_ROUTED and law.n_species > 1:
-        # the kernels index the router's table by species: checked once per
-        # version of the array
-        sp = state.species_id
-        tag = (id(sp), sp._version, int(law.n_species))
-        if state.__dict__.get("_species_checked") != tag:
-            if state.N and (int(sp.min()) < 0 or int(sp.max()) >= law.n_species):
-                raise ValueError(f"species_id outside the router's {law.n_species} species")
-            state.__dict__["_species_checked"] = tag
-        law.species = sp.data_ptr()
+    _set_species(law, state)
     S.law = law
     S.lin_kind = int(type(system.linear_integrator).kind)
     S.rot_kind = int(type(system.rotation_integrator).kind)
@@ -442,6 +441,97 @@ def _system_struct(state: State, system: System, keep: list, resident=None, for_
             nl.hist_live = _ptr(col.history["live"])
         S.nlist = nl
     return S
+
+
+def _set_species(law, state: State) -> None:
+    """Point a routed law with several species at State.species_id; the
+    kernels index the router's table by species, so the array is checked
+    once per version."""
+    if law.kind == L.LAW_ROUTED and law.n_species > 1:
+        sp = state.species_id
+        tag = (id(sp), sp._version, int(law.n_species))
+        if state.__dict__.get("_species_checked") != tag:
+            if state.N and (int(sp.min()) < 0 or int(sp.max()) >= law.n_species):
+                raise ValueError(f"species_id outside the router's {law.n_species} species")
+            state.__dict__["_species_checked"] = tag
+        law.species = sp.data_ptr()
+
+
+_F64_FIELDS = ("pos_c", "vel", "ang_vel", "force", "torque", "rad", "mass", "inertia")
+
+
+def _is_f64(state: State) -> bool:
+    """Whether the state steps through the float64 kernels (a float64 state,
+    as State.create makes with jax_enable_x64)."""
+    if state.pos_c.dtype != torch.float64:
+        return False
+    bad = [f for f in _F64_FIELDS if getattr(state, f).dtype != torch.float64]
+    if state.q.data.dtype != torch.float64:
+        bad.append("q")
+    if bad:
+        raise TypeError(f"a float64 state needs every float field in float64; {bad} are not")
+    return True
+
+
+def _require_f32(state: State, what: str) -> None:
+    if state.pos_c.dtype != torch.float32:
+        raise NotImplementedError(
+            f"{what} runs in float32; the float64 kernels cover System.step with built-in plugins, the naive and "
+            "CellList colliders and the laws without history, compute_force and the energies"
+        )
+
+
+def _system_struct64(state: State, system: System, keep: list, for_collider: bool = True) -> L.JdemSystemF64:
+    """The jdem_system_f64 of a float64 (state, system): domain, law tables,
+    dt and gravity in float64."""
+    _require_gpu(state)
+    S = L.JdemSystemF64()
+    S.state = _state_struct(state, L.JdemStateF64)
+    S.domain = system.domain.to_c(state.dim, f64=True)
+    law, dev = law_struct(system.force_model, system.mat_table, state.device, f64=True)
+    keep.append(dev)
+    _set_species(law, state)
+    S.law = law
+    S.lin_kind = int(type(system.linear_integrator).kind)
+    S.rot_kind = int(type(system.rotation_integrator).kind)
+    S.dt = float(system.dt)
+    S.interact_same_bond_id = 1 if system.interact_same_bond_id else 0
+    g = np.zeros(3, np.float64)
+    g[: state.dim] = np.asarray(system.force_manager.gravity, np.float64).reshape(-1)[: state.dim]
+    for a in range(3):
+        S.gravity[a] = float(g[a])
+    fm = system.force_manager
+    for name, cname in (("external_force", "ext_force"), ("external_force_com", "ext_force_com"), ("external_torque", "ext_torque")):
+        t = getattr(fm, name)
+        if t is not None:
+            t = torch.as_tensor(t, dtype=torch.float64, device=state.device).contiguous()
+            keep.append(t)
+            setattr(S, cname, t.data_ptr())
+    col = system.collider
+    if not for_collider:
+        S.collider_kind = L.COLLIDER_NONE
+        return S
+    if isinstance(col, NeighborList):
+        raise NotImplementedError("the NeighborList collider runs in float32; the float64 step covers the naive "
+                                  "and CellList colliders")
+    S.collider_kind = _collider_kind(col)
+    if isinstance(col, DynamicCellList):
+        g, ovf = col.grid(system, f64=True)
+        if ovf:
+            raise NotImplementedError("cell grid exceeds the int32 hash range (reference: hash_overflow)")
+        S.grid = g
+    return S
+
+
+def _domain_after(dom, S, state: State, f64: bool):
+    """The domain after a fused call: a FreeDomain takes the box the call
+    fitted, every domain the inverse box of its box, in the call's dtype."""
+    ft = np.float64 if f64 else F32
+    if isinstance(dom, FreeDomain) and state.N > 0:
+        box = np.array([S.domain.box_size[a] for a in range(state.dim)], ft)
+        anc = np.array([S.domain.anchor[a] for a in range(state.dim)], ft)
+        return replace(dom, box_size=box, anchor=anc, inv_box_size=(ft(1.0) / box).astype(ft))
+    return replace(dom, inv_box_size=(ft(1.0) / np.asarray(dom.box_size, ft)).astype(ft))
 
 
 def _ctx(state: State) -> L.Context:
@@ -504,6 +594,16 @@ def _fused_step(state: State, system: System, n: int) -> tuple:
     stats = L.JdemStepStats()
     ctx = _ctx(new_state)
     keep: list = []
+    if _is_f64(new_state):
+        sys2 = replace(system, domain=replace(system.domain))
+        S = _system_struct64(new_state, sys2, keep)
+        ctx.check(ctx.lib.jdem_system_step_f64(ctx.ptr, ctypes.byref(S), int(n), ctypes.byref(stats)),
+                  "jdem_system_step_f64")
+        fm = replace(sys2.force_manager, external_force=None, external_force_com=None, external_torque=None)
+        out_sys = replace(sys2, domain=_domain_after(sys2.domain, S, new_state, True), force_manager=fm)
+        out_sys = _advance_clock(out_sys, n)
+        out_sys.__dict__["_last_stats"] = stats
+        return new_state, out_sys
     if isinstance(system.collider, NeighborList):
         sys2 = replace(system, domain=replace(system.domain))
         S, slot, col, hist = _resident_call(new_state, sys2, keep)
@@ -515,13 +615,7 @@ def _fused_step(state: State, system: System, n: int) -> tuple:
         S = _system_struct(new_state, sys2, keep)
         ctx.check(ctx.lib.jdem_system_step(ctx.ptr, ctypes.byref(S), int(n), ctypes.byref(stats)), "jdem_system_step")
         col_out = _collider_after(col, S)
-    dom = sys2.domain
-    if isinstance(dom, FreeDomain) and new_state.N > 0:
-        box = np.array([S.domain.box_size[a] for a in range(state.dim)], F32)
-        anc = np.array([S.domain.anchor[a] for a in range(state.dim)], F32)
-        dom = replace(dom, box_size=box, anchor=anc, inv_box_size=(F32(1.0) / box).astype(F32))
-    else:
-        dom = replace(dom, inv_box_size=(F32(1.0) / np.asarray(dom.box_size, F32)).astype(F32))
+    dom = _domain_after(sys2.domain, S, new_state, False)
     fm = replace(sys2.force_manager, external_force=None, external_force_com=None, external_torque=None)
     out_sys = replace(sys2, collider=col_out, domain=dom, force_manager=fm)
     out_sys = _advance_clock(out_sys, n)
@@ -601,6 +695,7 @@ def _step_once(state: State, system: System) -> tuple:
 
 def _run_stage(state: State, system: System, stage: int) -> tuple:
     """One built-in stage as a GPU kernel on fresh copies of the state."""
+    _require_f32(state, "a step with user plugins or actions (stage by stage)")
     new_state = state.with_kinematics()
     keep: list = []
     S = _system_struct(new_state, system, keep)
@@ -620,8 +715,12 @@ def _collider_compute_force(state: State, system: System) -> tuple:
     col = _fresh_collider(system.collider)
     sys2 = replace(system, collider=col)
     keep: list = []
-    S = _system_struct(new_state, sys2, keep)
     ctx = _ctx(new_state)
+    if _is_f64(new_state):
+        S = _system_struct64(new_state, sys2, keep)
+        ctx.check(ctx.lib.jdem_compute_force_f64(ctx.ptr, ctypes.byref(S)), "jdem_compute_force_f64")
+        return new_state, sys2
+    S = _system_struct(new_state, sys2, keep)
     ctx.check(ctx.lib.jdem_compute_force(ctx.ptr, ctypes.byref(S)), "jdem_compute_force")
     return new_state, replace(sys2, collider=_collider_after(col, S))
 
@@ -630,9 +729,14 @@ def _collider_potential_energy(state: State, system: System) -> tuple:
     col = _fresh_collider(system.collider)
     sys2 = replace(system, collider=col)
     keep: list = []
-    S = _system_struct(state, sys2, keep)
     ctx = _ctx(state)
     e = ctypes.c_double()
+    if _is_f64(state):
+        S = _system_struct64(state, sys2, keep)
+        ctx.check(ctx.lib.jdem_potential_energy_f64(ctx.ptr, ctypes.byref(S), ctypes.byref(e)),
+                  "jdem_potential_energy_f64")
+        return state, sys2, float(e.value)
+    S = _system_struct(state, sys2, keep)
     ctx.check(ctx.lib.jdem_potential_energy(ctx.ptr, ctypes.byref(S), ctypes.byref(e)), "jdem_potential_energy")
     return state, replace(sys2, collider=_collider_after(col, S)), float(e.value)
 
@@ -641,6 +745,7 @@ def _contact_virial(state: State, system: System) -> tuple:
     """Float64 contact virial sum_{i<j} r_ij (x) F_ij and the system whose
     NeighborList was refreshed for it (a list that is unbuilt or stale is
     rebuilt first)."""
+    _require_f32(state, "the contact virial")
     col = _fresh_collider(system.collider)
     sys2 = replace(system, collider=col)
     keep: list = []
@@ -654,15 +759,23 @@ def _contact_virial(state: State, system: System) -> tuple:
 
 def _kinetic_energy_parts(state: State, system: System) -> tuple:
     keep: list = []
-    S = _system_struct(state, system, keep, for_collider=False)
     ctx = _ctx(state)
     t, r = ctypes.c_double(), ctypes.c_double()
+    if _is_f64(state):
+        S = _system_struct64(state, system, keep, for_collider=False)
+        ctx.check(ctx.lib.jdem_kinetic_energy_parts_f64(ctx.ptr, ctypes.byref(S), ctypes.byref(t), ctypes.byref(r)),
+                  "jdem_kinetic_energy_parts_f64")
+        return float(t.value), float(r.value)
+    S = _system_struct(state, system, keep, for_collider=False)
     ctx.check(ctx.lib.jdem_kinetic_energy_parts(ctx.ptr, ctypes.byref(S), ctypes.byref(t), ctypes.byref(r)),
               "jdem_kinetic_energy_parts")
     return float(t.value), float(r.value)
 
 
 def _kinetic_energy(state: State, system: System) -> float:
+    if _is_f64(state):
+        t, r = _kinetic_energy_parts(state, system)
+        return t + r
     keep: list = []
     S = _system_struct(state, system, keep, for_collider=False)
     ctx = _ctx(state)
@@ -673,6 +786,7 @@ def _kinetic_energy(state: State, system: System) -> float:
 
 def _neighbor_list(state: State, system: System, kind: int, grid, cutoff: float, M: int) -> tuple:
     _require_gpu(state)
+    _require_f32(state, "create_neighbor_list")
     M = int(M)
     nl = torch.full((state.N, M), -1, dtype=torch.int32, device=state.device)
     if M == 0 or state.N == 0:

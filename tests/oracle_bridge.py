@@ -7,20 +7,20 @@ import numpy as np
 from oracle import jdem_oracle as O
 
 
-def law_of(system) -> O.Law:
-    return _law(system.force_model, system.mat_table)
+def law_of(system, dtype=np.float32) -> O.Law:
+    return _law(system.force_model, system.mat_table, dtype)
 
 
-def _law(fm, mt) -> O.Law:
+def _law(fm, mt, dtype=np.float32) -> O.Law:
     from jaxdem_amd import _lib as L
     from jaxdem_amd.forces import route_layout
 
     if fm.kind == L.LAW_ROUTED:
         S, off, terms = route_layout(fm)
         return O.Law(kind="routed", n_species=S, route_off=np.asarray(off, np.int64),
-                     terms=tuple(O.Law(kind="none") if t is None else _law(t, mt) for t in terms))
+                     terms=tuple(O.Law(kind="none") if t is None else _law(t, mt, dtype) for t in terms))
     kind = fm.type_name
-    get = lambda k: (np.asarray(getattr(mt, k), np.float32) if mt.has(k) else None)  # noqa: E731
+    get = lambda k: (np.asarray(getattr(mt, k), dtype) if mt.has(k) else None)  # noqa: E731
     return O.Law(
         kind=kind,
         young_eff=get("young_eff"),
@@ -35,18 +35,6 @@ def _law(fm, mt) -> O.Law:
     )
 
 
-def _cast_law(law: O.Law, dtype) -> O.Law:
-    kw = {}
-    for k, v in law.__dict__.items():
-        if k == "route_off":
-            kw[k] = v
-        elif k == "terms":
-            kw[k] = tuple(_cast_law(t, dtype) for t in v)
-        else:
-            kw[k] = v.astype(dtype) if isinstance(v, np.ndarray) else v
-    return O.Law(**kw)
-
-
 def params_of(state, system, dtype=np.float32) -> O.Params:
     dom = system.domain
     odom = O.Domain(
@@ -55,14 +43,12 @@ def params_of(state, system, dtype=np.float32) -> O.Params:
         anchor=np.asarray(dom.anchor, dtype).copy(),
         restitution=float(getattr(dom, "restitution_coefficient", 1.0)),
     )
-    law = law_of(system)
-    if dtype != np.float32:
-        law = _cast_law(law, dtype)
+    law = law_of(system, dtype)
     col = system.collider
     cname = col.type_name.lower()
     P = O.Params(
         dim=state.dim,
-        dt=float(np.float32(system.dt)),
+        dt=float(dtype(system.dt)),
         domain=odom,
         law=law,
         lin=system.linear_integrator.type_name,

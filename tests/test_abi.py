@@ -68,6 +68,14 @@ STRUCT_FIELDS = {
     "jdem_minimize_stats": ["steps", "pe", "max_grad", "power", "overflow", "reserved"],
     "jdem_precise": ["n", "dim", "dims", "cell", "cell_lo", "cidx", "off", "rad", "force", "k", "stall_steps"],
     "jdem_frames": ["n_frames", "steps", "pos", "vel", "ang_vel", "q", "force", "torque"],
+    "jdem_state_f64": ["n", "dim", "bond_width", "pos", "vel", "ang_vel", "q", "force", "torque", "rad", "mass",
+                       "inertia", "mat_id", "bond_id", "fixed"],
+    "jdem_domain_f64": ["kind", "dim", "box_size", "inv_box_size", "anchor", "restitution"],
+    "jdem_law_f64": ["kind", "n_mat", "t0", "t1", "t2", "m0", "m1", "m2", "m3", "p0", "p1", "n_species", "n_terms",
+                     "route_off", "route_terms", "species"],
+    "jdem_grid_f64": ["dims", "strides", "cell_size", "search_range", "periodic", "n_cells"],
+    "jdem_system_f64": ["state", "domain", "law", "lin_kind", "rot_kind", "dt", "interact_same_bond_id", "gravity",
+                        "ext_force", "ext_force_com", "ext_torque", "collider_kind", "grid"],
 }
 
 
