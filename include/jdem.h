@@ -503,8 +503,9 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
  * jdem_domain, jdem_law, jdem_grid and jdem_system with every floating-point
  * array and scalar in float64, and the entry points run the kernels of the
  * float32 calls above instantiated in double, with the same association of
- * operations.  Colliders: NONE, NAIVE and CELL_LIST; laws: SPRING, HERTZ,
- * CUNDALL_STRACK and ROUTED over those (a NeighborList and the history laws
+ * operations.  Colliders: NONE, NAIVE, CELL_LIST and NEIGHBOR_LIST (list,
+ * skin and rebuild as jdem_nlist, without the resident working set); laws:
+ * SPRING, HERTZ, CUNDALL_STRACK and ROUTED over those (the history laws
  * return JDEM_EUNSUPPORTED); every domain and integrator. */
 typedef struct jdem_state_f64 {
   int64_t n;
@@ -562,6 +563,19 @@ typedef struct jdem_grid_f64 {
   int64_t n_cells;
 } jdem_grid_f64;
 
+/* NeighborList buffers (neighbor_list.py:137-286), float64 positions. */
+typedef struct jdem_nlist_f64 {
+  int32_t max_neighbors;
+  double cutoff;
+  double skin;
+  int32_t* nl;      /* (N, M) in/out, -1 padded                          */
+  double* old_pos;  /* (N, D) in/out: positions at the last build        */
+  int32_t n_build_times;
+  int32_t overflow;
+  int32_t secondary; /* JDEM_COLLIDER_CELL_LIST or _NAIVE                 */
+  jdem_grid_f64 grid;
+} jdem_nlist_f64;
+
 typedef struct jdem_system_f64 {
   jdem_state_f64 state;
   jdem_domain_f64 domain;
@@ -576,14 +590,16 @@ typedef struct jdem_system_f64 {
   const double* ext_torque;
   int32_t collider_kind;
   jdem_grid_f64 grid;
+  jdem_nlist_f64 nlist;
 } jdem_system_f64;
 
 /* System.step in float64 (system.py:701-748), as jdem_system_step.  For a
  * FREE domain the box and anchor after the last step are written to
- * sys->domain and stats. */
+ * sys->domain and stats; for NEIGHBOR_LIST sys->nlist is updated in place
+ * (the list is checked after every step). */
 int jdem_system_step_f64(jdem_ctx* ctx, jdem_system_f64* sys, int64_t n, jdem_step_stats* stats);
 /* Collider.compute_force in float64 (colliders/__init__.py:56-80), as
- * jdem_compute_force. */
+ * jdem_compute_force (a NeighborList is checked and rebuilt first). */
 int jdem_compute_force_f64(jdem_ctx* ctx, jdem_system_f64* sys);
 /* Collider.compute_potential_energy in float64 (cell_list.py:469-496;
  * naive.py:76-117): float64 pair terms, float64 sum. */

@@ -281,6 +281,20 @@ class JdemGridF64(ctypes.Structure):
     ]
 
 
+class JdemNlistF64(ctypes.Structure):
+    _fields_ = [
+        ("max_neighbors", c_int32),
+        ("cutoff", c_double),
+        ("skin", c_double),
+        ("nl", c_void_p),
+        ("old_pos", c_void_p),
+        ("n_build_times", c_int32),
+        ("overflow", c_int32),
+        ("secondary", c_int32),
+        ("grid", JdemGridF64),
+    ]
+
+
 class JdemSystemF64(ctypes.Structure):
     _fields_ = [
         ("state", JdemStateF64),
@@ -296,6 +310,7 @@ class JdemSystemF64(ctypes.Structure):
         ("ext_torque", c_void_p),
         ("collider_kind", c_int32),
         ("grid", JdemGridF64),
+        ("nlist", JdemNlistF64),
     ]
 
 
@@ -304,6 +319,7 @@ STRUCTS = {
     "jdem_domain_f64": JdemDomainF64,
     "jdem_law_f64": JdemLawF64,
     "jdem_grid_f64": JdemGridF64,
+    "jdem_nlist_f64": JdemNlistF64,
     "jdem_system_f64": JdemSystemF64,
     "jdem_timing": JdemTiming,
     "jdem_state": JdemState,

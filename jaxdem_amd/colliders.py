@@ -277,13 +277,14 @@ class NeighborList(Collider):
         if _normalize_key(secondary_collider_type) == "celllist" and "cell_size" not in skw:
             skw["cell_size"] = list_cutoff
         sec = Collider.create(secondary_collider_type, **skw)
+        ft = np.float64 if state.pos_c.dtype == torch.float64 else F32  # the state's precision
         return cls(
             secondary_collider=sec,
             neighbor_list=torch.full((N, max_neighbors), -1, dtype=torch.int32, device=state.device),
             old_pos=state.pos.clone(),
             n_build_times=0,
-            cutoff=float(F32(cutoff)),
-            skin=float(F32(skin_val)),
+            cutoff=float(ft(cutoff)),
+            skin=float(ft(skin_val)),
             overflow=False,
             max_neighbors=int(max_neighbors),
         )
@@ -330,14 +331,15 @@ class NeighborList(Collider):
         d["overflow"] = bool(overflow)
         return out
 
-    def secondary_grid(self, system) -> tuple:
+    def secondary_grid(self, system, f64: bool = False) -> tuple:
         """Grid of the secondary cell list inflated to cutoff + skin
-        (cell_list.py:537-538)."""
+        (cell_list.py:537-538), in float32 (float64 with ``f64``)."""
+        ft = np.float64 if f64 else F32
         sec = self.secondary_collider
-        lc = F32(F32(self.cutoff) + F32(self.skin))
+        lc = ft(ft(self.cutoff) + ft(self.skin))
         sr = sec.search_range
-        cs = max(F32(sec.cell_size), F32(lc / F32(sr)))
-        return grid_struct(system.domain.box_size, cs, system.domain.periodic, sr)
+        cs = max(ft(sec.cell_size), ft(lc / ft(sr)))
+        return grid_struct(system.domain.box_size, cs, system.domain.periodic, sr, f64)
 
     @staticmethod
     def create_neighbor_list(state, system, cutoff: float, max_neighbors: int):

@@ -526,15 +526,16 @@ int nl_rebuild(jdem_ctx* ctx, jdem_system* sys, const float* pos) {
   return JDEM_OK;
 }
 
-__global__ void k_disp_check(const float* __restrict__ pos, const float* __restrict__ old, int64_t n, int dim,
-                             float trigger, int32_t* __restrict__ flag) {
+template <class Real>
+__global__ void k_disp_check(const Real* __restrict__ pos, const Real* __restrict__ old, int64_t n, int dim,
+                             Real trigger, int32_t* __restrict__ flag) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  float d2;
-  const float dx = pos[dim * i] - old[dim * i];
-  const float dy = pos[dim * i + 1] - old[dim * i + 1];
+  Real d2;
+  const Real dx = pos[dim * i] - old[dim * i];
+  const Real dy = pos[dim * i + 1] - old[dim * i + 1];
   if (dim == 3) {
-    const float dz = pos[dim * i + 2] - old[dim * i + 2];
+    const Real dz = pos[dim * i + 2] - old[dim * i + 2];
     d2 = (dx * dx + dy * dy) + dz * dz;
   } else {
     d2 = dx * dx + dy * dy;
@@ -557,7 +558,7 @@ int nl_check_and_rebuild(jdem_ctx* ctx, jdem_system* sys, bool* rebuilt) {
   bool need = L.n_build_times == 0;
   if (!need && s.n > 0) {
     HIPCHK(ctx, launch_fill_i32((int32_t*)ctx->flags.p + 3, 1, 0, ctx->stream));
-    hipLaunchKernelGGL(k_disp_check, dim3(nblocks(s.n)), dim3(kBlock), 0, ctx->stream, s.pos, L.old_pos, s.n, s.dim,
+    hipLaunchKernelGGL(k_disp_check<float>, dim3(nblocks(s.n)), dim3(kBlock), 0, ctx->stream, s.pos, L.old_pos, s.n, s.dim,
                        nl_trigger(L), (int32_t*)ctx->flags.p + 3);
     HIPCHK(ctx, hipGetLastError());
     rc = read_flags(ctx, 4);
@@ -2412,10 +2413,17 @@ int check_system64(jdem_ctx* ctx, const jdem_system_f64* sys) {
   if (s.bond_width > 0 && !s.bond_id) return fail(ctx, JDEM_EINVAL, "bond_width > 0 but bond_id is null");
   if (sys->law.n_mat > 1 && !s.mat_id) return fail(ctx, JDEM_EINVAL, "n_mat > 1 but mat_id is null");
   const int ck = sys->collider_kind;
-  if (ck == JDEM_COLLIDER_NEIGHBOR_LIST)
-    return fail(ctx, JDEM_EUNSUPPORTED, "float64: the NeighborList collider runs in float32 only");
-  if (ck != JDEM_COLLIDER_NONE && ck != JDEM_COLLIDER_NAIVE && ck != JDEM_COLLIDER_CELL_LIST)
+  if (ck != JDEM_COLLIDER_NONE && ck != JDEM_COLLIDER_NAIVE && ck != JDEM_COLLIDER_CELL_LIST &&
+      ck != JDEM_COLLIDER_NEIGHBOR_LIST)
     return fail(ctx, JDEM_EINVAL, "unknown collider kind");
+  if (ck == JDEM_COLLIDER_NEIGHBOR_LIST) {
+    const jdem_nlist_f64& L = sys->nlist;
+    if (L.max_neighbors < 0) return fail(ctx, JDEM_EINVAL, "max_neighbors < 0");
+    if (s.n > 0 && L.max_neighbors > 0 && (!L.nl || !L.old_pos))
+      return fail(ctx, JDEM_EINVAL, "neighbour-list buffers missing");
+    if (L.secondary != JDEM_COLLIDER_CELL_LIST && L.secondary != JDEM_COLLIDER_NAIVE)
+      return fail(ctx, JDEM_EINVAL, "secondary collider must be CellList or naive");
+  }
   const int lk = sys->law.kind;
   if (ck != JDEM_COLLIDER_NONE && !(lk >= JDEM_LAW_SPRING && lk <= JDEM_LAW_ROUTED))
     return fail(ctx, JDEM_EINVAL, "unknown law kind");
@@ -2432,8 +2440,9 @@ int check_system64(jdem_ctx* ctx, const jdem_system_f64* sys) {
     return fail(ctx, JDEM_EUNSUPPORTED, "float64: the history laws run in float32 only");
   if (sys->domain.kind < JDEM_DOMAIN_FREE || sys->domain.kind > JDEM_DOMAIN_REFLECT)
     return fail(ctx, JDEM_EINVAL, "unknown domain kind");
-  if (ck == JDEM_COLLIDER_CELL_LIST) {
-    const jdem_grid_f64& g = sys->grid;
+  if (ck == JDEM_COLLIDER_CELL_LIST ||
+      (ck == JDEM_COLLIDER_NEIGHBOR_LIST && sys->nlist.secondary == JDEM_COLLIDER_CELL_LIST)) {
+    const jdem_grid_f64& g = ck == JDEM_COLLIDER_CELL_LIST ? sys->grid : sys->nlist.grid;
     if (g.n_cells <= 0 || g.n_cells > (int64_t)INT32_MAX)
       return fail(ctx, JDEM_EUNSUPPORTED, "cell grid size out of the int32 hash range");
     if (g.search_range < 1) return fail(ctx, JDEM_EINVAL, "search_range must be >= 1");
@@ -2476,7 +2485,96 @@ StepArgs64 base_args64(jdem_ctx* ctx, const jdem_system_f64* sys) {
   a.dt = sys->dt;
   a.lin = sys->lin_kind;
   a.rot = sys->rot_kind;
+  a.M = sys->nlist.max_neighbors;
+  a.W32 = (a.M + 31) / 32;
+  a.nl = sys->nlist.nl;
   return a;
+}
+
+// Candidate pairs within `cutoff` into nl_out (N, M) through the secondary
+// collider `kind` (cell_list.py:500-595, naive.py:122-185), float64.
+int build_nlist64(jdem_ctx* ctx, const jdem_system_f64* sys, int kind, const jdem_grid_f64& grid, const double* pos,
+                  double cutoff, int M, int32_t* nl_out) {
+  const jdem_state_f64& s = sys->state;
+  NLBuildT<double> b;
+  std::memset(&b, 0, sizeof(b));
+  b.n = s.n;
+  b.dim = s.dim;
+  b.M = M;
+  b.cutoff_sq = cutoff * cutoff;
+  b.isb = sys->interact_same_bond_id;
+  b.bond_w = s.bond_width;
+  b.bond_id = s.bond_id;
+  b.pos = pos;
+  b.dom = dom_of(sys->domain);
+  b.nl = nl_out;
+  b.overflow = (int32_t*)ctx->flags.p + 2;
+  if (kind == JDEM_COLLIDER_CELL_LIST) {
+    Cells c;
+    b.grid = grid_of(grid);
+    int rc = build_cells(ctx, pos, s.rad, s.n, s.dim, b.dom, b.grid, &c);
+    if (rc) return rc;
+    b.perm = c.perm;
+    b.skeys = c.skeys;
+    b.cstart = c.cstart;
+    b.cend = c.cend;
+    HIPCHK(ctx, launch_nlist_cells(b, ctx->stream));
+  } else {
+    HIPCHK(ctx, launch_nlist_naive(b, ctx->stream));
+  }
+  return JDEM_OK;
+}
+
+// NeighborList rebuild at positions `pos` in float64 (neighbor_list.py:
+// 58-131, 545-589): new list, old_pos <- pos, n_build_times += 1; flags[2]
+// receives the overflow.
+int nl_rebuild64(jdem_ctx* ctx, jdem_system_f64* sys, const double* pos) {
+  jdem_nlist_f64& L = sys->nlist;
+  const jdem_state_f64& s = sys->state;
+  const int64_t n = s.n;
+  const int M = L.max_neighbors;
+  const double lc = L.cutoff + L.skin;
+  HIPCHK(ctx, launch_fill_i32((int32_t*)ctx->flags.p + 2, 1, 0, ctx->stream));
+  if (M > 0 && n > 0) {
+    ENSURE(ctx, ctx->nl_new, 4 * (size_t)n * M);
+    int32_t* nl_new = (int32_t*)ctx->nl_new.p;
+    int rc = build_nlist64(ctx, sys, L.secondary, L.grid, pos, lc, M, nl_new);
+    if (rc) return rc;
+    HIPCHK(ctx, hipMemcpyAsync(L.nl, nl_new, 4 * (size_t)n * M, hipMemcpyDeviceToDevice, ctx->stream));
+  }
+  if (n > 0 && pos != L.old_pos)
+    HIPCHK(ctx, hipMemcpyAsync(L.old_pos, pos, 8 * (size_t)n * s.dim, hipMemcpyDeviceToDevice, ctx->stream));
+  L.n_build_times += 1;
+  return JDEM_OK;
+}
+
+// (skin / 2)^2 of the displacement check (neighbor_list.py:78-85).
+double nl_trigger64(const jdem_nlist_f64& L) { return (L.skin * L.skin) / 4.0; }
+
+// _check_and_rebuild on its own (neighbor_list.py:58-131), float64.
+int nl_check_and_rebuild64(jdem_ctx* ctx, jdem_system_f64* sys) {
+  jdem_nlist_f64& L = sys->nlist;
+  const jdem_state_f64& s = sys->state;
+  int rc = ensure_flags(ctx);
+  if (rc) return rc;
+  bool need = L.n_build_times == 0;
+  if (!need && s.n > 0) {
+    HIPCHK(ctx, launch_fill_i32((int32_t*)ctx->flags.p + 3, 1, 0, ctx->stream));
+    hipLaunchKernelGGL(k_disp_check<double>, dim3(nblocks(s.n)), dim3(kBlock), 0, ctx->stream, s.pos, L.old_pos, s.n,
+                       s.dim, nl_trigger64(L), (int32_t*)ctx->flags.p + 3);
+    HIPCHK(ctx, hipGetLastError());
+    rc = read_flags(ctx, 4);
+    if (rc) return rc;
+    need = ctx->host_flags[3] != 0;
+  }
+  if (need) {
+    rc = nl_rebuild64(ctx, sys, s.pos);
+    if (rc) return rc;
+    rc = read_flags(ctx, 3);
+    if (rc) return rc;
+    L.overflow = ctx->host_flags[2] != 0;
+  }
+  return JDEM_OK;
 }
 
 int prepare_source64(jdem_ctx* ctx, const jdem_system_f64* sys, StepArgs64& a, const double* pos) {
@@ -2583,7 +2681,34 @@ int jdem_system_step_f64(jdem_ctx* ctx, jdem_system_f64* sys, int64_t n, jdem_st
   auto rd = [&](int64_t t) { return t == 0 ? 0 : wr(t - 1); };
   bool free_done = false;
   double fbox[3] = {0, 0, 0}, fanc[3] = {0, 0, 0};
+  // NeighborList: kernel t-1 checks the positions it writes against the
+  // build positions (flags[3]); the host reads that word before kernel t and
+  // rebuilds the list at those positions when it is set (neighbor_list.py:
+  // 58-131 runs the same check inside compute_force).
+  const bool is_nl = sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST;
+  bool rebuilt_here = false;
+  if (is_nl) {
+    rc = ensure_flags(ctx);
+    if (rc) return rc;
+    HIPCHK(ctx, launch_fill_i32((int32_t*)ctx->flags.p, 4, 0, ctx->stream));
+  }
   for (int64_t t = 0; t <= n; ++t) {
+    if (is_nl && t >= 1) {
+      bool need = sys->nlist.n_build_times == 0;
+      if (!need) {
+        rc = read_flags(ctx, 4);
+        if (rc) return rc;
+        S->host_syncs++;
+        need = ctx->host_flags[3] != 0;
+      }
+      if (need) {
+        rc = nl_rebuild64(ctx, sys, P[rd(t)]);
+        if (rc) return rc;
+        HIPCHK(ctx, launch_fill_i32((int32_t*)ctx->flags.p + 3, 1, 0, ctx->stream));
+        S->rebuilds++;
+        rebuilt_here = true;
+      }
+    }
     StepArgs64 a = base_args64(ctx, sys);
     const int r = rd(t), w = wr(t);
     a.pos_in = P[r];
@@ -2615,8 +2740,23 @@ int jdem_system_step_f64(jdem_ctx* ctx, jdem_system_f64* sys, int64_t n, jdem_st
       src_eff = SRC_NAIVE;
       law_eff = JDEM_LAW_SPRING;
     }
+    if (is_nl && a.do_before) {
+      a.check_disp = 1;
+      a.old_pos = sys->nlist.old_pos;
+      a.trigger = nl_trigger64(sys->nlist);
+      a.flag_wr = (int32_t*)ctx->flags.p + 3;
+    }
     const hipError_t e = launch_step(a, D, law_eff, src_eff, ctx->stream);
     if (e != hipSuccess) return fail(ctx, JDEM_EHIP, std::string("step kernel: ") + hipGetErrorString(e));
+  }
+  if (is_nl) {
+    // overflow of the most recent build (neighbor_list.py:582-589)
+    if (rebuilt_here) {
+      rc = read_flags(ctx, 3);
+      if (rc) return rc;
+      sys->nlist.overflow = ctx->host_flags[2] != 0;
+    }
+    S->overflow = sys->nlist.overflow;
   }
   if (free_done) {
     for (int a = 0; a < D; ++a) {
@@ -2641,6 +2781,10 @@ int jdem_compute_force_f64(jdem_ctx* ctx, jdem_system_f64* sys) {
     HIPCHK(ctx, hipMemsetAsync(s.torque, 0, 8 * (size_t)s.n * (s.dim == 3 ? 3 : 1), ctx->stream));
     return JDEM_OK;
   }
+  if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST) {
+    rc = nl_check_and_rebuild64(ctx, sys);
+    if (rc) return rc;
+  }
   StepArgs64 a = base_args64(ctx, sys);
   rc = prepare_source64(ctx, sys, a, s.pos);
   if (rc) return rc;
@@ -2658,6 +2802,10 @@ int jdem_potential_energy_f64(jdem_ctx* ctx, jdem_system_f64* sys, double* energ
   if (sys->collider_kind == JDEM_COLLIDER_NONE) {
     *energy = 0.0;
     return JDEM_OK;
+  }
+  if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST) {
+    rc = nl_check_and_rebuild64(ctx, sys);
+    if (rc) return rc;
   }
   return reduce64(ctx, sys, 1, energy, false);
 }

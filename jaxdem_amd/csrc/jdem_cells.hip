@@ -754,12 +754,17 @@ JD Vec<2> v4<2>(float4 a) { return {a.x, a.y}; }
 
 // DynamicCellList.create_neighbor_list (cell_list.py:500-595): thread per
 // sorted index; rows in stencil order, ascending id within a cell.
-template <int D, bool PER>
-__global__ void __launch_bounds__(kBlock) k_nlist_cells(NLBuild b) {
+template <int D, bool PER, class Real>
+__global__ void __launch_bounds__(kBlock) k_nlist_cells(NLBuildT<Real> b) {
+  constexpr bool kF32 = std::is_same<Real, float>::value;
   const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (s >= b.n) return;
   const int32_t i = b.perm[s];
-  const Vec<D> xi = v4<D>(b.spos[s]);
+  Vec<D, Real> xi;
+  if constexpr (kF32)
+    xi = v4<D>(b.spos[s]);
+  else  // float64: the particles' own positions
+    xi = ld<D>(b.pos, i);
   const uint32_t key = (uint32_t)b.skeys[s];
   int c[3];
   c[0] = (int)(key % (uint32_t)b.grid.dims[0]);
@@ -801,7 +806,12 @@ __global__ void __launch_bounds__(kBlock) k_nlist_cells(NLBuild b) {
         for (int k = k0; k < k1; ++k) {
           const int32_t j = b.perm[k];
           if (j == i) continue;
-          if (!(dist_sq<PER>(xi, v4<D>(b.spos[k]), b.dom) <= b.cutoff_sq)) continue;
+          Vec<D, Real> xj;
+          if constexpr (kF32)
+            xj = v4<D>(b.spos[k]);
+          else
+            xj = ld<D>(b.pos, j);
+          if (!(dist_sq<PER>(xi, xj, b.dom) <= b.cutoff_sq)) continue;
           if (chk_bond && bonded(b.bond_id, b.bond_w, i, j)) continue;
           if (count < b.M) row[count] = j;
           ++count;
@@ -815,17 +825,17 @@ __global__ void __launch_bounds__(kBlock) k_nlist_cells(NLBuild b) {
 
 // NaiveSimulator.create_neighbor_list (naive.py:122-185): descending ids,
 // _displacement (multiply by the inverse box).
-template <int D, bool PER>
-__global__ void __launch_bounds__(kBlock) k_nlist_naive(NLBuild b) {
+template <int D, bool PER, class Real>
+__global__ void __launch_bounds__(kBlock) k_nlist_naive(NLBuildT<Real> b) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= b.n) return;
-  const Vec<D> xi = ld<D>(b.pos, i);
+  const Vec<D, Real> xi = ld<D>(b.pos, i);
   int32_t* row = b.nl + i * b.M;
   int count = 0;
   const bool chk_bond = (!b.isb) && b.bond_w > 0;
   for (int64_t j = b.n - 1; j >= 0; --j) {
     if (j == i) continue;
-    const Vec<D> d = disp_<PER>(xi, ld<D>(b.pos, j), b.dom);
+    const Vec<D, Real> d = disp_<PER>(xi, ld<D>(b.pos, j), b.dom);
     if (!(dot(d, d) <= b.cutoff_sq)) continue;
     if (chk_bond && bonded(b.bond_id, b.bond_w, i, (int32_t)j)) continue;
     if (count < b.M) row[count] = (int32_t)j;
@@ -835,41 +845,48 @@ __global__ void __launch_bounds__(kBlock) k_nlist_naive(NLBuild b) {
   if (count > b.M) atomicOr(b.overflow, 1);
 }
 
-hipError_t launch_nlist_cells(const NLBuild& b, hipStream_t s) {
+template <class Real>
+hipError_t nlist_cells_run(const NLBuildT<Real>& b, hipStream_t s) {
   if (b.n == 0) return hipSuccess;
   dim3 grd(nblocks(b.n)), blk(kBlock);
   bool per = b.grid.periodic != 0;
   if (b.dim == 3) {
     if (per)
-      hipLaunchKernelGGL((k_nlist_cells<3, true>), grd, blk, 0, s, b);
+      hipLaunchKernelGGL((k_nlist_cells<3, true, Real>), grd, blk, 0, s, b);
     else
-      hipLaunchKernelGGL((k_nlist_cells<3, false>), grd, blk, 0, s, b);
+      hipLaunchKernelGGL((k_nlist_cells<3, false, Real>), grd, blk, 0, s, b);
   } else {
     if (per)
-      hipLaunchKernelGGL((k_nlist_cells<2, true>), grd, blk, 0, s, b);
+      hipLaunchKernelGGL((k_nlist_cells<2, true, Real>), grd, blk, 0, s, b);
     else
-      hipLaunchKernelGGL((k_nlist_cells<2, false>), grd, blk, 0, s, b);
+      hipLaunchKernelGGL((k_nlist_cells<2, false, Real>), grd, blk, 0, s, b);
   }
   return hipGetLastError();
 }
 
-hipError_t launch_nlist_naive(const NLBuild& b, hipStream_t s) {
+template <class Real>
+hipError_t nlist_naive_run(const NLBuildT<Real>& b, hipStream_t s) {
   if (b.n == 0) return hipSuccess;
   dim3 grd(nblocks(b.n)), blk(kBlock);
   bool per = b.dom.kind == JDEM_DOMAIN_PERIODIC;
   if (b.dim == 3) {
     if (per)
-      hipLaunchKernelGGL((k_nlist_naive<3, true>), grd, blk, 0, s, b);
+      hipLaunchKernelGGL((k_nlist_naive<3, true, Real>), grd, blk, 0, s, b);
     else
-      hipLaunchKernelGGL((k_nlist_naive<3, false>), grd, blk, 0, s, b);
+      hipLaunchKernelGGL((k_nlist_naive<3, false, Real>), grd, blk, 0, s, b);
   } else {
     if (per)
-      hipLaunchKernelGGL((k_nlist_naive<2, true>), grd, blk, 0, s, b);
+      hipLaunchKernelGGL((k_nlist_naive<2, true, Real>), grd, blk, 0, s, b);
     else
-      hipLaunchKernelGGL((k_nlist_naive<2, false>), grd, blk, 0, s, b);
+      hipLaunchKernelGGL((k_nlist_naive<2, false, Real>), grd, blk, 0, s, b);
   }
   return hipGetLastError();
 }
+
+hipError_t launch_nlist_cells(const NLBuild& b, hipStream_t s) { return nlist_cells_run(b, s); }
+hipError_t launch_nlist_naive(const NLBuild& b, hipStream_t s) { return nlist_naive_run(b, s); }
+hipError_t launch_nlist_cells(const NLBuildT<double>& b, hipStream_t s) { return nlist_cells_run(b, s); }
+hipError_t launch_nlist_naive(const NLBuildT<double>& b, hipStream_t s) { return nlist_naive_run(b, s); }
 
 // _remap_history_array (neighbor_list.py:29-54): a new slot inherits the
 // history of the first old slot with the same partner id.

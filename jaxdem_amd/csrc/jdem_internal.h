@@ -189,7 +189,7 @@ enum Src { SRC_NAIVE = 1, SRC_CELLS = 2, SRC_NLIST = 3 };
 hipError_t launch_step(const StepArgs& a, int dim, int law, int src, hipStream_t s);
 hipError_t launch_energy(const StepArgs& a, int dim, int law, int src, hipStream_t s);
 hipError_t launch_virial(const StepArgs& a, int dim, int law, int src, hipStream_t s);
-// float64 instances: colliders naive / CellList, laws without history
+// float64 instances: every collider source, laws without history
 hipError_t launch_step(const StepArgs64& a, int dim, int law, int src, hipStream_t s);
 hipError_t launch_energy(const StepArgs64& a, int dim, int law, int src, hipStream_t s);
 hipError_t launch_kinetic(const StepArgs64& a, int dim, hipStream_t s);
@@ -263,27 +263,31 @@ hipError_t launch_fill_i32(int32_t* p, int64_t n, int32_t v, hipStream_t s);
 hipError_t launch_cell_ranges(const uint32_t* skeys, int64_t n, int32_t* cstart, int32_t* cend, hipStream_t s);
 hipError_t launch_gather_spos(const int32_t* perm, const float* pos, const float* rad, int64_t n, int dim,
                               float4* spos, hipStream_t s);
-struct NLBuild {
+template <class Real = float>
+struct NLBuildT {
   int64_t n;
   int dim;
   int M;
-  float cutoff_sq;
+  Real cutoff_sq;
   int isb;
   int bond_w;
   const int32_t* bond_id;
-  const float* pos;
+  const Real* pos;
   const int32_t* perm;
   const int32_t* skeys;
   const int32_t* cstart;
   const int32_t* cend;
-  const float4* spos;
-  GridP grid;
-  DomP dom;
+  const float4* spos;  // float only: sorted (x, r)
+  GridPT<Real> grid;
+  DomPT<Real> dom;
   int32_t* nl;
   int32_t* overflow;
 };
+using NLBuild = NLBuildT<float>;
 hipError_t launch_nlist_cells(const NLBuild& b, hipStream_t s);
 hipError_t launch_nlist_naive(const NLBuild& b, hipStream_t s);
+hipError_t launch_nlist_cells(const NLBuildT<double>& b, hipStream_t s);
+hipError_t launch_nlist_naive(const NLBuildT<double>& b, hipStream_t s);
 hipError_t launch_remap_hist(int64_t n, int M, int W32, int dim, const int32_t* old_nl, const uint32_t* old_live,
                              const float* old_xi, const int32_t* new_nl, uint32_t* new_live, float* new_xi,
                              hipStream_t s);

@@ -530,8 +530,7 @@ __global__ void __launch_bounds__(kBlock) k_stage(StepArgs a, int stage) {
 
 // ---------------------------------------------------------------------------
 // dispatch: every collider source and law in float; in double (the float64
-// form of the plain step) the sources without a neighbour list and the laws
-// without history.
+// form of the plain step) the laws without history.
 // ---------------------------------------------------------------------------
 template <class Real>
 constexpr bool kIsF32 = std::is_same<Real, float>::value;
@@ -553,14 +552,11 @@ constexpr bool kIsF32 = std::is_same<Real, float>::value;
           hipLaunchKernelGGL((KERNEL<D, LAW, SRC_CELLS, false, Real>), g, b, 0, s, a);           \
         break;                                                                                   \
       case SRC_NLIST:                                                                            \
-        if constexpr (kIsF32<Real>) {                                                            \
-          if (per)                                                                               \
-            hipLaunchKernelGGL((KERNEL<D, LAW, SRC_NLIST, true, Real>), g, b, 0, s, a);          \
-          else                                                                                   \
-            hipLaunchKernelGGL((KERNEL<D, LAW, SRC_NLIST, false, Real>), g, b, 0, s, a);         \
-          break;                                                                                 \
-        }                                                                                        \
-        return hipErrorInvalidValue;                                                             \
+        if (per)                                                                                 \
+          hipLaunchKernelGGL((KERNEL<D, LAW, SRC_NLIST, true, Real>), g, b, 0, s, a);            \
+        else                                                                                     \
+          hipLaunchKernelGGL((KERNEL<D, LAW, SRC_NLIST, false, Real>), g, b, 0, s, a);           \
+        break;                                                                                   \
       default:                                                                                   \
         return hipErrorInvalidValue;                                                             \
     }                                                                                            \

@@ -476,8 +476,8 @@ def _is_f64(state: State) -> bool:
 def _require_f32(state: State, what: str) -> None:
     if state.pos_c.dtype != torch.float32:
         raise NotImplementedError(
-            f"{what} runs in float32; the float64 kernels cover System.step with built-in plugins, the naive and "
-            "CellList colliders and the laws without history, compute_force and the energies"
+            f"{what} runs in float32; the float64 kernels cover System.step with built-in plugins and the laws "
+            "without history (every collider), compute_force and the energies"
         )
 
 
@@ -511,15 +511,37 @@ def _system_struct64(state: State, system: System, keep: list, for_collider: boo
     if not for_collider:
         S.collider_kind = L.COLLIDER_NONE
         return S
-    if isinstance(col, NeighborList):
-        raise NotImplementedError("the NeighborList collider runs in float32; the float64 step covers the naive "
-                                  "and CellList colliders")
     S.collider_kind = _collider_kind(col)
     if isinstance(col, DynamicCellList):
         g, ovf = col.grid(system, f64=True)
         if ovf:
             raise NotImplementedError("cell grid exceeds the int32 hash range (reference: hash_overflow)")
         S.grid = g
+    if isinstance(col, NeighborList):
+        if col.history is not None:
+            raise NotImplementedError("the history laws run in float32")
+        if col.old_pos.dtype != torch.float64:
+            raise TypeError("a float64 NeighborList needs float64 build positions (create it from the float64 state)")
+        nl = L.JdemNlistF64()
+        nl.max_neighbors = col.max_neighbors
+        nl.cutoff = float(col.cutoff)
+        nl.skin = float(col.skin)
+        nl.nl = _ptr(col.neighbor_list)
+        nl.old_pos = _ptr(col.old_pos)
+        nl.n_build_times = int(col.n_build_times)
+        nl.overflow = 1 if col.overflow else 0
+        sec = col.secondary_collider
+        if isinstance(sec, DynamicCellList):
+            nl.secondary = L.COLLIDER_CELL_LIST
+            g2, ovf = col.secondary_grid(system, f64=True)
+            if ovf:
+                raise NotImplementedError("secondary cell grid exceeds the int32 hash range")
+            nl.grid = g2
+        elif isinstance(sec, NaiveSimulator):
+            nl.secondary = L.COLLIDER_NAIVE
+        else:
+            raise NotImplementedError(f"NeighborList secondary collider {type(sec).__name__} on the GPU")
+        S.nlist = nl
     return S
 
 
@@ -595,12 +617,14 @@ def _fused_step(state: State, system: System, n: int) -> tuple:
     ctx = _ctx(new_state)
     keep: list = []
     if _is_f64(new_state):
-        sys2 = replace(system, domain=replace(system.domain))
+        col = _fresh_collider(system.collider)
+        sys2 = replace(system, collider=col, domain=replace(system.domain))
         S = _system_struct64(new_state, sys2, keep)
         ctx.check(ctx.lib.jdem_system_step_f64(ctx.ptr, ctypes.byref(S), int(n), ctypes.byref(stats)),
                   "jdem_system_step_f64")
         fm = replace(sys2.force_manager, external_force=None, external_force_com=None, external_torque=None)
-        out_sys = replace(sys2, domain=_domain_after(sys2.domain, S, new_state, True), force_manager=fm)
+        out_sys = replace(sys2, collider=_collider_after(col, S), domain=_domain_after(sys2.domain, S, new_state, True),
+                          force_manager=fm)
         out_sys = _advance_clock(out_sys, n)
         out_sys.__dict__["_last_stats"] = stats
         return new_state, out_sys
@@ -719,7 +743,7 @@ def _collider_compute_force(state: State, system: System) -> tuple:
     if _is_f64(new_state):
         S = _system_struct64(new_state, sys2, keep)
         ctx.check(ctx.lib.jdem_compute_force_f64(ctx.ptr, ctypes.byref(S)), "jdem_compute_force_f64")
-        return new_state, sys2
+        return new_state, replace(sys2, collider=_collider_after(col, S))
     S = _system_struct(new_state, sys2, keep)
     ctx.check(ctx.lib.jdem_compute_force(ctx.ptr, ctypes.byref(S)), "jdem_compute_force")
     return new_state, replace(sys2, collider=_collider_after(col, S))
@@ -735,7 +759,7 @@ def _collider_potential_energy(state: State, system: System) -> tuple:
         S = _system_struct64(state, sys2, keep)
         ctx.check(ctx.lib.jdem_potential_energy_f64(ctx.ptr, ctypes.byref(S), ctypes.byref(e)),
                   "jdem_potential_energy_f64")
-        return state, sys2, float(e.value)
+        return state, replace(sys2, collider=_collider_after(col, S)), float(e.value)
     S = _system_struct(state, sys2, keep)
     ctx.check(ctx.lib.jdem_potential_energy(ctx.ptr, ctypes.byref(S), ctypes.byref(e)), "jdem_potential_energy")
     return state, replace(sys2, collider=_collider_after(col, S)), float(e.value)

@@ -94,11 +94,11 @@ def state_of(state, dtype=np.float32) -> O.State:
     )
 
 
-def nl_state_of(state, system) -> O.NLState:
+def nl_state_of(state, system, dtype=np.float32) -> O.NLState:
     col = system.collider
     nls = O.NLState(
         nl=col.neighbor_list.cpu().numpy().astype(np.int64),
-        old_pos=col.old_pos.cpu().numpy().astype(np.float32),
+        old_pos=col.old_pos.cpu().numpy().astype(dtype),
         n_build=int(col.n_build_times),
         overflow=bool(col.overflow),
     )

@@ -74,8 +74,10 @@ STRUCT_FIELDS = {
     "jdem_law_f64": ["kind", "n_mat", "t0", "t1", "t2", "m0", "m1", "m2", "m3", "p0", "p1", "n_species", "n_terms",
                      "route_off", "route_terms", "species"],
     "jdem_grid_f64": ["dims", "strides", "cell_size", "search_range", "periodic", "n_cells"],
+    "jdem_nlist_f64": ["max_neighbors", "cutoff", "skin", "nl", "old_pos", "n_build_times", "overflow", "secondary",
+                       "grid"],
     "jdem_system_f64": ["state", "domain", "law", "lin_kind", "rot_kind", "dt", "interact_same_bond_id", "gravity",
-                        "ext_force", "ext_force_com", "ext_torque", "collider_kind", "grid"],
+                        "ext_force", "ext_force_com", "ext_torque", "collider_kind", "grid", "nlist"],
 }
 
 

@@ -2,7 +2,8 @@
 precision, tests/conftest.py:4) against the float64 oracle on the same seeded
 inputs: forces and torques within 1e-12 relative (max |GPU - oracle| over the
 largest component), whole steps of every integrator pair and domain with the
-naive and CellList colliders, energies to 1e-12, routed laws, gravity.
+naive, CellList and NeighborList colliders (list rebuilds inside the run),
+energies to 1e-12, routed laws, gravity.
 Each test also checks that the result is not a float32 computation: the
 float32 oracle differs from it by far more than 1e-12.
 """
@@ -155,12 +156,48 @@ def test_f64_routed_and_gravity(device):
     compare_states(st2, ost, "routed")
 
 
+@pytest.mark.parametrize("secondary", ["CellList", "naive"])
+@pytest.mark.parametrize("dim", [3, 2])
+def test_f64_neighborlist_step(device, dim, secondary):
+    """60 steps on a float64 NeighborList with a small skin (rebuilds inside
+    the run): state, list, build positions and build count."""
+    print(f"[f64] neighborlist {secondary} dim={dim}", flush=True)
+    n = 60
+    with x64():
+        st, box = packed(2000 if secondary == "CellList" else 600, dim, seed=13, domain="periodic", phi=0.6)
+        ckw = dict(state=st) if secondary == "CellList" else {}
+        sy = jd.System.create(
+            st.shape, dt=1e-4, domain_type="periodic", domain_kw=dict(box_size=box, anchor=np.zeros(dim)),
+            collider_type="NeighborList", force_model_type="cundallstrack",
+            collider_kw=dict(state=st, cutoff=float(2 * st.rad.max().item()), skin=0.02,
+                             secondary_collider_type=secondary, secondary_collider_kw=ckw),
+            mat_table=jd.MaterialTable.from_materials(
+                [jd.Material.create("elasticfrict", density=1.0, young=1e4, poisson=0.3, mu=0.4, e=0.7, mu_r=0.05)]))
+        assert sy.collider.old_pos.dtype == torch.float64
+        st2, sy2 = jd.System.step(st, sy, n=n)
+        f_only, _ = sy2.collider.compute_force(st2, sy2)
+    ost = B.state_of(st, np.float64)
+    P = B.params_of(st, sy, np.float64)
+    nls = B.nl_state_of(st, sy, np.float64)
+    ost = O.step(ost, P, n, nls)
+    ost32 = O.step(B.state_of(st, np.float32), B.params_of(st, sy, np.float32), n, B.nl_state_of(st, sy))
+    compare_states(st2, ost, f"nl/{secondary}", ost32)
+    assert sy2.collider.n_build_times == nls.n_build and nls.n_build >= 2
+    assert np.array_equal(sy2.collider.neighbor_list.cpu().numpy().astype(np.int64), nls.nl)
+    check("old_pos", sy2.collider.old_pos.cpu().numpy(), nls.old_pos)
+    # compute_force on the stepped state (a list check first; velocities after the last kick)
+    oc = ost.copy()
+    O.compute_force(oc, P, nls)
+    check("compute_force", f_only.force.cpu().numpy(), oc.force)
+    check("compute_force torque", f_only.torque.cpu().numpy(), oc.torque)
+
+
 def test_f64_unsupported_paths_raise(device):
     with x64():
         st, box = packed(200, 3, seed=1)
-        nl = make_system(st, box, "periodic", "spring", "NeighborList")
+        hist = make_system(st, box, "periodic", "springfriction", "NeighborList")
         with pytest.raises(NotImplementedError, match="float32"):
-            jd.System.step(st, nl, n=1)
+            jd.System.step(st, hist, n=1)
         sy = make_system(st, box, "periodic", "spring", "naive")
     import dataclasses
 

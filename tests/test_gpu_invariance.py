@@ -2,7 +2,8 @@
 the HIP path (tests/test_colliders_invariance.py:365-468 and
 tests/test_energy_conservation.py:34-89 of JaxDEM, restated).
 
-Invariance: 64 disks (8 x 8) or 27 spheres (3 x 3 x 3) on a lattice (spacing 1.2, radii U[0.5/p, 0.5] for
+Invariance (float32, and float64 with jax_enable_x64 as the reference runs
+it): 64 disks (8 x 8) or 27 spheres (3 x 3 x 3) on a lattice (spacing 1.2, radii U[0.5/p, 0.5] for
 polydispersity p, velocities U[-1, 1]), spring k = young = 1e3, dt = 1e-3,
 200 steps with the naive collider and with every other collider
 (CellList, NeighborList over naive, NeighborList over CellList, skin 0.5);
@@ -63,22 +64,36 @@ def set_up_spheres(dim, collider, domain, polydispersity=1.0, n=64, seed=42, ski
     return st, sy
 
 
+@contextlib.contextmanager
+def precision(name: str):
+    """jax_enable_x64 on for "float64" while the block runs."""
+    old = jd.config.jax_enable_x64
+    jd.config.update("jax_enable_x64", name == "float64")
+    try:
+        yield
+    finally:
+        jd.config.update("jax_enable_x64", old)
+
+
 def _close(a, b, what):
     np.testing.assert_allclose(a, b, atol=5e-2, rtol=5e-2, err_msg=f"{what} mismatch")
 
 
+@pytest.mark.parametrize("prec", ["float32", "float64"])
 @pytest.mark.parametrize("domain", ["periodic", "reflect", "free"])
 @pytest.mark.parametrize("polydispersity", [1.0, 2.0, 3.0])
 @pytest.mark.parametrize("collider", ["celllist", "nl_naive", "nl_celllist"])
 @pytest.mark.parametrize("dim", [2, 3])
-def test_spheres_invariance(device, dim, collider, polydispersity, domain):
+def test_spheres_invariance(device, dim, collider, polydispersity, domain, prec):
     if domain == "free" and "celllist" in collider:
         pytest.skip("a FreeDomain with a cell list (per-step grid) is not on the GPU path")
-    print(f"[invariance] dim={dim} {collider} p={polydispersity} {domain}", flush=True)
-    s1, y1 = set_up_spheres(dim, "naive", domain, polydispersity, device=device)
-    s2, y2 = set_up_spheres(dim, collider, domain, polydispersity, device=device)
-    s1, y1 = jd.System.step(s1, y1, n=200)
-    s2, y2 = jd.System.step(s2, y2, n=200)
+    print(f"[invariance] dim={dim} {collider} p={polydispersity} {domain} {prec}", flush=True)
+    with precision(prec):
+        s1, y1 = set_up_spheres(dim, "naive", domain, polydispersity, device=device)
+        s2, y2 = set_up_spheres(dim, collider, domain, polydispersity, device=device)
+        s1, y1 = jd.System.step(s1, y1, n=200)
+        s2, y2 = jd.System.step(s2, y2, n=200)
+    assert s2.pos_c.dtype == (torch.float64 if prec == "float64" else torch.float32)
     for f in ("pos_c", "vel", "force", "ang_vel", "torque"):
         _close(getattr(s1, f).cpu().numpy(), getattr(s2, f).cpu().numpy(), f)
     _close(s1.q.data.cpu().numpy(), s2.q.data.cpu().numpy(), "q")
@@ -100,17 +115,6 @@ def _conservation_slope(st, sy, lin, rot, dts, total_time=5.0, n_samples=100):
         stds.append(float(np.std(np.asarray(e, np.float64))))
     slope, _ = np.polyfit(np.log(dts), np.log(stds), 1)
     return slope, stds
-
-
-@contextlib.contextmanager
-def precision(name: str):
-    """jax_enable_x64 on for "float64" while the block runs."""
-    old = jd.config.jax_enable_x64
-    jd.config.update("jax_enable_x64", name == "float64")
-    try:
-        yield
-    finally:
-        jd.config.update("jax_enable_x64", old)
 
 
 @pytest.mark.parametrize("prec", ["float32", "float64"])
