@@ -104,6 +104,15 @@ typedef struct jdem_state {
   const int32_t* mat_id;  /* (N,)                                      */
   const int32_t* bond_id; /* (N, W), -1 padded, or NULL                */
   const uint8_t* fixed;   /* (N,) 0/1                                  */
+  /* Clumps (rigid bodies of spheres; NULL, NULL for spheres): pos_p (N, D)
+   * body-frame offsets from the clump centre pos, clump_id (N,) in [0, N).
+   * Every member holds its clump's pos, vel, q, ang_vel, mass (the total)
+   * and principal inertia.  Pairs of one clump do not interact; member
+   * forces act at pos + R(q) pos_p, and ForceManager sums them over the
+   * clump (force_manager.py:359-412).  Colliders NAIVE and CELL_LIST,
+   * FREE and PERIODIC domains, laws without history. */
+  const float* pos_p;
+  const int32_t* clump_id;
 } jdem_state;
 
 /* Domain parameters (domains/__init__.py:28-120). */
@@ -523,6 +532,8 @@ typedef struct jdem_state_f64 {
   const int32_t* mat_id;
   const int32_t* bond_id;
   const uint8_t* fixed;
+  const double* pos_p;
+  const int32_t* clump_id;
 } jdem_state_f64;
 
 typedef struct jdem_domain_f64 {

@@ -51,6 +51,8 @@ class JdemState(ctypes.Structure):
         ("mat_id", c_void_p),
         ("bond_id", c_void_p),
         ("fixed", c_void_p),
+        ("pos_p", c_void_p),
+        ("clump_id", c_void_p),
     ]
 
 

@@ -22,7 +22,7 @@ std::vector<Buf*> jdem_ctx::bufs() {
                            &cstart,  &cend,   &spos,     &nl_new,  &xi_new,  &live_new, &flags,  &red,
                            &nl_copy, &xi_copy, &live_copy, &wF,     &wT,      &wOld,     &wWpos, &wSrc,
                            &wInv,    &wKeys,  &fireV,    &fireDev, &segp,  &segid, &segflag, &wXiRank,
-                           &route,   &ccnt};
+                           &route,   &ccnt,   &cl_start, &cl_members, &plab, &ppr, &ftot, &ttot};
   for (int k = 0; k < 2; ++k)
     for (Buf* b : {&wP[k], &wV[k], &wW[k], &wQ[k], &wI[k], &wOrig[k], &wNl[k], &wXi[k], &wLive[k]}) all.push_back(b);
   return all;
@@ -291,6 +291,14 @@ int check_system(jdem_ctx* ctx, const jdem_system* sys, bool batch = false, bool
     return fail(ctx, JDEM_EINVAL, "null state array");
   if (s.bond_width > 0 && !s.bond_id) return fail(ctx, JDEM_EINVAL, "bond_width > 0 but bond_id is null");
   if (sys->law.n_mat > 1 && !s.mat_id) return fail(ctx, JDEM_EINVAL, "n_mat > 1 but mat_id is null");
+  if (s.clump_id || s.pos_p) {
+    if (s.n > 0 && (!s.clump_id || !s.pos_p)) return fail(ctx, JDEM_EINVAL, "clumps need both pos_p and clump_id");
+    if (batch) return fail(ctx, JDEM_EUNSUPPORTED, "batches with clumps");
+    if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST)
+      return fail(ctx, JDEM_EUNSUPPORTED, "clumps with the NeighborList collider");
+    if (sys->domain.kind == JDEM_DOMAIN_REFLECT)
+      return fail(ctx, JDEM_EUNSUPPORTED, "clumps in a reflective domain");
+  }
   const int lk = sys->law.kind;
   if (sys->collider_kind != JDEM_COLLIDER_NONE &&
       !(lk >= JDEM_LAW_SPRING && lk <= JDEM_LAW_ROUTED))
@@ -1387,6 +1395,20 @@ int src_of(int collider_kind) {
   }
 }
 
+// Clumps (defined with the float64 helpers at the end of this file).
+int clump_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats* S);
+int clump_force(jdem_ctx* ctx, jdem_system* sys);
+int clump_reduce(jdem_ctx* ctx, jdem_system* sys, int ncomp, double* host_out, bool kinetic);
+int clump_step(jdem_ctx* ctx, jdem_system_f64* sys, int64_t n, jdem_step_stats* S);
+int clump_force(jdem_ctx* ctx, jdem_system_f64* sys);
+int clump_reduce(jdem_ctx* ctx, jdem_system_f64* sys, int ncomp, double* host_out, bool kinetic);
+
+// The calls that do not take clumps refuse them.
+int reject_clumps(jdem_ctx* ctx, const jdem_state& s, const char* what) {
+  if (s.clump_id || s.pos_p) return fail(ctx, JDEM_EUNSUPPORTED, std::string(what) + " with clumps");
+  return JDEM_OK;
+}
+
 // Fill the source fields of `a` for the collider (building cells if needed).
 int prepare_source(jdem_ctx* ctx, jdem_system* sys, StepArgs& a, const float* pos) {
   if (sys->collider_kind == JDEM_COLLIDER_CELL_LIST) {
@@ -1566,6 +1588,8 @@ int jdem_neighbor_list(jdem_ctx* ctx, const jdem_system* sys, int32_t collider_k
   tmp.collider_kind = JDEM_COLLIDER_NONE;
   int rc = check_system(ctx, &tmp);
   if (rc) return rc;
+  rc = reject_clumps(ctx, sys->state, "jdem_neighbor_list");
+  if (rc) return rc;
   if (max_neighbors < 0 || (max_neighbors > 0 && sys->state.n > 0 && !nl))
     return fail(ctx, JDEM_EINVAL, "bad neighbour-list output");
   if (collider_kind != JDEM_COLLIDER_CELL_LIST && collider_kind != JDEM_COLLIDER_NAIVE)
@@ -1598,6 +1622,7 @@ int jdem_compute_force(jdem_ctx* ctx, jdem_system* sys) {
     HIPCHK(ctx, hipMemsetAsync(s.torque, 0, 4 * (size_t)s.n * (s.dim == 3 ? 3 : 1), ctx->stream));
     return JDEM_OK;
   }
+  if (sys->state.clump_id) return clump_force(ctx, sys);
   if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST) {
     bool rebuilt = false;
     rc = nl_check_and_rebuild(ctx, sys, &rebuilt);
@@ -1657,11 +1682,14 @@ int jdem_potential_energy(jdem_ctx* ctx, jdem_system* sys, double* energy) {
     rc = nl_check_and_rebuild(ctx, sys, &rebuilt);
     if (rc) return rc;
   }
+  if (sys->state.clump_id) return clump_reduce(ctx, sys, 1, energy, false);
   return reduce_f64(ctx, sys, 1, energy, launch_energy, true);
 }
 
 int jdem_contact_virial(jdem_ctx* ctx, jdem_system* sys, double* w) {
   int rc = check_system(ctx, sys);
+  if (rc) return rc;
+  rc = reject_clumps(ctx, sys->state, "the contact virial");
   if (rc) return rc;
   if (!w) return fail(ctx, JDEM_EINVAL, "null output");
   (void)hipSetDevice(ctx->device);
@@ -1686,7 +1714,8 @@ int jdem_kinetic_energy_parts(jdem_ctx* ctx, const jdem_system* sys, double* tra
   if (!translational || !rotational) return fail(ctx, JDEM_EINVAL, "null output");
   (void)hipSetDevice(ctx->device);
   double parts[2] = {0.0, 0.0};
-  rc = reduce_f64(ctx, sys, 2, parts, nullptr, false);
+  rc = sys->state.clump_id ? clump_reduce(ctx, const_cast<jdem_system*>(sys), 2, parts, true)
+                           : reduce_f64(ctx, sys, 2, parts, nullptr, false);
   if (rc) return rc;
   *translational = parts[0];
   *rotational = parts[1];
@@ -1706,7 +1735,7 @@ static int free_box(jdem_ctx* ctx, const float* pos, const jdem_system* sys, flo
   const jdem_state& s = sys->state;
   const int nparts = 64;
   ENSURE(ctx, ctx->red, 4 * 6 * (size_t)nparts);
-  HIPCHK(ctx, launch_box_minmax(pos, s.rad, s.n, s.dim, (float*)ctx->red.p, nparts, ctx->stream));
+  HIPCHK(ctx, launch_box_minmax(pos, s.rad, s.n, s.dim, (float*)ctx->red.p, nparts, ctx->stream, s.pos_p));
   float hp[6 * 64];
   HIPCHK(ctx, hipMemcpyAsync(hp, ctx->red.p, sizeof(hp), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -1724,6 +1753,8 @@ static int free_box(jdem_ctx* ctx, const float* pos, const jdem_system* sys, flo
 
 int jdem_step_stage(jdem_ctx* ctx, jdem_system* sys, int32_t stage) {
   int rc = check_system(ctx, sys);
+  if (rc) return rc;
+  rc = reject_clumps(ctx, sys->state, "jdem_step_stage");
   if (rc) return rc;
   if (stage < JDEM_STAGE_DOMAIN_APPLY || stage > JDEM_STAGE_ROT_AFTER)
     return fail(ctx, JDEM_EINVAL, "unknown stage");
@@ -1838,6 +1869,8 @@ int jdem_minimize(jdem_ctx* ctx, jdem_system* sys, const jdem_fire* fire, int64_
   if (rc) return rc;
   if (!fire) return fail(ctx, JDEM_EINVAL, "null FIRE arguments");
   if (max_steps < 0) return fail(ctx, JDEM_EINVAL, "max_steps < 0");
+  rc = reject_clumps(ctx, sys->state, "minimize");
+  if (rc) return rc;
   if (law_has_history(sys->law)) return fail(ctx, JDEM_EUNSUPPORTED, "minimize with a history law");
   if (sys->ext_force || sys->ext_force_com || sys->ext_torque)
     return fail(ctx, JDEM_EUNSUPPORTED, "minimize with external force arrays");
@@ -2225,6 +2258,7 @@ int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     S->steps = n;
     return JDEM_OK;
   }
+  if (s.clump_id) return clump_step(ctx, sys, n, S);
   if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST) return resident_nl_step(ctx, sys, n, S);
   ENSURE(ctx, ctx->pos2, 4 * (size_t)N * D);
   ENSURE(ctx, ctx->vel2, 4 * (size_t)N * D);
@@ -2412,6 +2446,13 @@ int check_system64(jdem_ctx* ctx, const jdem_system_f64* sys) {
     return fail(ctx, JDEM_EINVAL, "null state array");
   if (s.bond_width > 0 && !s.bond_id) return fail(ctx, JDEM_EINVAL, "bond_width > 0 but bond_id is null");
   if (sys->law.n_mat > 1 && !s.mat_id) return fail(ctx, JDEM_EINVAL, "n_mat > 1 but mat_id is null");
+  if (s.clump_id || s.pos_p) {
+    if (s.n > 0 && (!s.clump_id || !s.pos_p)) return fail(ctx, JDEM_EINVAL, "clumps need both pos_p and clump_id");
+    if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST)
+      return fail(ctx, JDEM_EUNSUPPORTED, "clumps with the NeighborList collider");
+    if (sys->domain.kind == JDEM_DOMAIN_REFLECT)
+      return fail(ctx, JDEM_EUNSUPPORTED, "clumps in a reflective domain");
+  }
   const int ck = sys->collider_kind;
   if (ck != JDEM_COLLIDER_NONE && ck != JDEM_COLLIDER_NAIVE && ck != JDEM_COLLIDER_CELL_LIST &&
       ck != JDEM_COLLIDER_NEIGHBOR_LIST)
@@ -2597,7 +2638,7 @@ int free_box64(jdem_ctx* ctx, const double* pos, const jdem_system_f64* sys, dou
   const jdem_state_f64& s = sys->state;
   const int nparts = 64;
   ENSURE(ctx, ctx->red, 8 * 6 * (size_t)nparts);
-  HIPCHK(ctx, launch_box_minmax(pos, s.rad, s.n, s.dim, (double*)ctx->red.p, nparts, ctx->stream));
+  HIPCHK(ctx, launch_box_minmax(pos, s.rad, s.n, s.dim, (double*)ctx->red.p, nparts, ctx->stream, s.pos_p));
   std::vector<double> hp(6 * nparts);
   HIPCHK(ctx, hipMemcpyAsync(hp.data(), ctx->red.p, 8 * hp.size(), hipMemcpyDeviceToHost, ctx->stream));
   HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
@@ -2668,6 +2709,7 @@ int jdem_system_step_f64(jdem_ctx* ctx, jdem_system_f64* sys, int64_t n, jdem_st
     S->steps = n;
     return JDEM_OK;
   }
+  if (s.clump_id) return clump_step(ctx, sys, n, S);
   ENSURE(ctx, ctx->pos2, 8 * (size_t)N * D);
   ENSURE(ctx, ctx->vel2, 8 * (size_t)N * D);
   ENSURE(ctx, ctx->ang2, 8 * (size_t)N * A);
@@ -2785,6 +2827,7 @@ int jdem_compute_force_f64(jdem_ctx* ctx, jdem_system_f64* sys) {
     rc = nl_check_and_rebuild64(ctx, sys);
     if (rc) return rc;
   }
+  if (s.clump_id) return clump_force(ctx, sys);
   StepArgs64 a = base_args64(ctx, sys);
   rc = prepare_source64(ctx, sys, a, s.pos);
   if (rc) return rc;
@@ -2807,6 +2850,7 @@ int jdem_potential_energy_f64(jdem_ctx* ctx, jdem_system_f64* sys, double* energ
     rc = nl_check_and_rebuild64(ctx, sys);
     if (rc) return rc;
   }
+  if (sys->state.clump_id) return clump_reduce(ctx, sys, 1, energy, false);
   return reduce64(ctx, sys, 1, energy, false);
 }
 
@@ -2817,7 +2861,8 @@ int jdem_kinetic_energy_parts_f64(jdem_ctx* ctx, const jdem_system_f64* sys, dou
   if (!translational || !rotational) return fail(ctx, JDEM_EINVAL, "null output");
   (void)hipSetDevice(ctx->device);
   double parts[2] = {0.0, 0.0};
-  rc = reduce64(ctx, sys, 2, parts, true);
+  rc = sys->state.clump_id ? clump_reduce(ctx, const_cast<jdem_system_f64*>(sys), 2, parts, true)
+                           : reduce64(ctx, sys, 2, parts, true);
   if (rc) return rc;
   *translational = parts[0];
   *rotational = parts[1];
@@ -2825,3 +2870,220 @@ int jdem_kinetic_energy_parts_f64(jdem_ctx* ctx, const jdem_system_f64* sys, dou
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// Clumps: rigid bodies of spheres on the plain step (state.py:230-304,
+// force_manager.py:341-425, naive.py:210-233, cell_list.py:437-464), for
+// both scalar types.  A step is five launches: members' lab positions and
+// levers, the pair forces at the lab positions (same-clump pairs masked,
+// lever torque added), ForceManager's member totals, their sum over each
+// clump, and the integrators (after-force of this step, domain.apply and
+// before-force of the next) on the clump state every member holds.
+// ---------------------------------------------------------------------------
+namespace {
+
+StepArgs args_of(jdem_ctx* ctx, const jdem_system* sys) { return base_args(ctx, sys); }
+StepArgs64 args_of(jdem_ctx* ctx, const jdem_system_f64* sys) { return base_args64(ctx, sys); }
+int source_of(jdem_ctx* ctx, jdem_system* sys, StepArgs& a, const float* pos) {
+  return prepare_source(ctx, sys, a, pos);
+}
+int source_of(jdem_ctx* ctx, jdem_system_f64* sys, StepArgs64& a, const double* pos) {
+  return prepare_source64(ctx, sys, a, pos);
+}
+int box_of(jdem_ctx* ctx, const float* pos, const jdem_system* sys, float box[3], float anchor[3]) {
+  return free_box(ctx, pos, sys, box, anchor);
+}
+int box_of(jdem_ctx* ctx, const double* pos, const jdem_system_f64* sys, double box[3], double anchor[3]) {
+  return free_box64(ctx, pos, sys, box, anchor);
+}
+
+// R(q) pos_p and pos_c + R(q) pos_p of every member into the context's
+// buffers (a.pos_in holds pos_c).
+template <class Real, class Sys>
+int clump_lab(jdem_ctx* ctx, const Sys* sys, StepArgsT<Real>& a) {
+  const auto& s = sys->state;
+  const size_t nd = sizeof(Real) * (size_t)s.n * s.dim;
+  ENSURE(ctx, ctx->plab, nd);
+  ENSURE(ctx, ctx->ppr, nd);
+  a.plab = (Real*)ctx->plab.p;
+  a.ppr = (Real*)ctx->ppr.p;
+  a.clump = s.clump_id;
+  a.pos_p = s.pos_p;
+  HIPCHK(ctx, launch_clump_lab(a, s.dim, ctx->stream));
+  return JDEM_OK;
+}
+
+// The members of each clump in ascending index (a stable counting sort of
+// the ids, which lie in [0, N)) and the buffers of the member totals.
+template <class Real, class Sys>
+int clump_members(jdem_ctx* ctx, const Sys* sys, StepArgsT<Real>& a) {
+  const auto& s = sys->state;
+  const int64_t N = s.n;
+  const int A = s.dim == 3 ? 3 : 1;
+  ENSURE(ctx, ctx->cl_start, 4 * ((size_t)N + 1));
+  ENSURE(ctx, ctx->cl_members, 4 * (size_t)N);
+  ENSURE(ctx, ctx->ftot, sizeof(Real) * (size_t)N * s.dim);
+  ENSURE(ctx, ctx->ttot, sizeof(Real) * (size_t)N * A);
+  uint32_t* sk = nullptr;
+  int32_t* perm = nullptr;
+  HIPCHK(ctx, cell_sort_pairs(ctx, reinterpret_cast<const uint32_t*>(s.clump_id), N, N, &sk, &perm,
+                              (int32_t*)ctx->cl_start.p, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(ctx->cl_members.p, perm, 4 * (size_t)N, hipMemcpyDeviceToDevice, ctx->stream));
+  a.clump = s.clump_id;
+  a.cl_start = (const int32_t*)ctx->cl_start.p;
+  a.cl_members = (const int32_t*)ctx->cl_members.p;
+  a.ftot = (Real*)ctx->ftot.p;
+  a.ttot = (Real*)ctx->ttot.p;
+  return JDEM_OK;
+}
+
+// Pair forces of the members at their lab positions (a from clump_lab).
+template <class Real, class Sys>
+int clump_pair_forces(jdem_ctx* ctx, Sys* sys, const StepArgsT<Real>& a0) {
+  StepArgsT<Real> f = a0;
+  f.pos_in = a0.plab;
+  f.do_force = 1;
+  f.do_after = 0;
+  f.do_before = 0;
+  f.write_kin = 0;
+  int law = f.law.kind, src = src_of(sys->collider_kind);
+  if (sys->collider_kind == JDEM_COLLIDER_NONE) {
+    f.force_zero = 1;  // the "" collider (colliders/__init__.py:218-222)
+    law = JDEM_LAW_SPRING;
+    src = SRC_NAIVE;
+  } else if (sys->collider_kind == JDEM_COLLIDER_CELL_LIST) {
+    int rc = source_of(ctx, sys, f, a0.plab);
+    if (rc) return rc;
+  }
+  HIPCHK(ctx, launch_step(f, sys->state.dim, law, src, ctx->stream));
+  return JDEM_OK;
+}
+
+template <class Real, class Sys>
+int clump_step_t(jdem_ctx* ctx, Sys* sys, int64_t n, jdem_step_stats* S) {
+  auto& s = sys->state;
+  const int D = s.dim;
+  StepArgsT<Real> a0 = args_of(ctx, sys);
+  int rc = clump_members(ctx, sys, a0);
+  if (rc) return rc;
+  const size_t nd = sizeof(Real) * (size_t)s.n * D;
+  ENSURE(ctx, ctx->plab, nd);
+  ENSURE(ctx, ctx->ppr, nd);
+  a0.plab = (Real*)ctx->plab.p;
+  a0.ppr = (Real*)ctx->ppr.p;
+  a0.pos_p = s.pos_p;
+  const bool none = sys->collider_kind == JDEM_COLLIDER_NONE;
+  const bool free_dom = sys->domain.kind == JDEM_DOMAIN_FREE;
+  bool free_done = false;
+  Real fbox[3] = {0, 0, 0}, fanc[3] = {0, 0, 0};
+  // the integrators on the clump state, in place (no pair reads)
+  auto integrate = [&](bool after, bool before) -> int {
+    StepArgsT<Real> a = a0;
+    a.do_force = 0;
+    a.do_after = after;
+    a.do_before = before;
+    a.write_kin = 1;
+    a.has_fm = 0;  // ForceManager ran in the clump kernels
+    a.ppr = nullptr;
+    const hipError_t e =
+        launch_step(a, D, none ? JDEM_LAW_SPRING : a.law.kind, none ? SRC_NAIVE : src_of(sys->collider_kind),
+                    ctx->stream);
+    if (e != hipSuccess) return fail(ctx, JDEM_EHIP, std::string("step kernel: ") + hipGetErrorString(e));
+    return JDEM_OK;
+  };
+  // FreeDomain.apply of step n sees the positions after step n - 1
+  auto free_at = [&]() -> int {
+    int r2 = box_of(ctx, s.pos, sys, fbox, fanc);
+    if (r2) return r2;
+    free_done = true;
+    S->host_syncs++;
+    return JDEM_OK;
+  };
+  if (free_dom && n == 1) {
+    rc = free_at();
+    if (rc) return rc;
+  }
+  rc = integrate(false, true);  // domain.apply + before-force of step 1
+  if (rc) return rc;
+  for (int64_t t = 1; t <= n; ++t) {
+    HIPCHK(ctx, launch_clump_lab(a0, D, ctx->stream));
+    rc = clump_pair_forces(ctx, sys, a0);
+    if (rc) return rc;
+    HIPCHK(ctx, launch_clump_forces(a0, D, ctx->stream));
+    if (free_dom && t == n - 1) {
+      rc = free_at();
+      if (rc) return rc;
+    }
+    rc = integrate(true, t < n);
+    if (rc) return rc;
+  }
+  if (free_done) {
+    for (int a = 0; a < D; ++a) {
+      sys->domain.box_size[a] = fbox[a];
+      sys->domain.anchor[a] = fanc[a];
+      sys->domain.inv_box_size[a] = Real(1) / fbox[a];
+      S->free_box[a] = fbox[a];
+      S->free_anchor[a] = fanc[a];
+    }
+  }
+  S->steps = n;
+  return JDEM_OK;
+}
+
+template <class Real, class Sys>
+int clump_force_t(jdem_ctx* ctx, Sys* sys) {
+  StepArgsT<Real> a = args_of(ctx, sys);
+  int rc = clump_lab(ctx, sys, a);
+  if (rc) return rc;
+  return clump_pair_forces(ctx, sys, a);
+}
+
+template <class Real, class Sys>
+int clump_reduce_t(jdem_ctx* ctx, Sys* sys, int ncomp, double* host_out, bool kinetic) {
+  const auto& s = sys->state;
+  const int64_t n = s.n;
+  const int nparts = 256;
+  if (n == 0) {
+    for (int c = 0; c < ncomp; ++c) host_out[c] = 0.0;
+    return JDEM_OK;
+  }
+  StepArgsT<Real> a = args_of(ctx, sys);
+  int rc = kinetic ? clump_members(ctx, sys, a) : clump_lab(ctx, sys, a);
+  if (rc) return rc;
+  ENSURE(ctx, ctx->red, 8 * ((size_t)n * ncomp + (size_t)nparts * ncomp + ncomp));
+  double* per = (double*)ctx->red.p;
+  double* part = per + (size_t)n * ncomp;
+  double* out = part + (size_t)nparts * ncomp;
+  a.e_out = per;
+  if (kinetic) {
+    HIPCHK(ctx, launch_kinetic(a, s.dim, ctx->stream));
+  } else {
+    a.pos_in = a.plab;
+    if (sys->collider_kind == JDEM_COLLIDER_CELL_LIST) {
+      rc = source_of(ctx, sys, a, a.plab);
+      if (rc) return rc;
+    }
+    HIPCHK(ctx, launch_energy(a, s.dim, a.law.kind, src_of(sys->collider_kind), ctx->stream));
+  }
+  HIPCHK(ctx, launch_sum_f64(per, n, ncomp, ncomp, part, out, nparts, ctx->stream));
+  HIPCHK(ctx, hipMemcpyAsync(host_out, out, 8 * (size_t)ncomp, hipMemcpyDeviceToHost, ctx->stream));
+  HIPCHK(ctx, hipStreamSynchronize(ctx->stream));
+  return JDEM_OK;
+}
+
+int clump_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats* S) {
+  return clump_step_t<float>(ctx, sys, n, S);
+}
+int clump_force(jdem_ctx* ctx, jdem_system* sys) { return clump_force_t<float>(ctx, sys); }
+int clump_reduce(jdem_ctx* ctx, jdem_system* sys, int ncomp, double* host_out, bool kinetic) {
+  return clump_reduce_t<float>(ctx, sys, ncomp, host_out, kinetic);
+}
+int clump_step(jdem_ctx* ctx, jdem_system_f64* sys, int64_t n, jdem_step_stats* S) {
+  return clump_step_t<double>(ctx, sys, n, S);
+}
+int clump_force(jdem_ctx* ctx, jdem_system_f64* sys) { return clump_force_t<double>(ctx, sys); }
+int clump_reduce(jdem_ctx* ctx, jdem_system_f64* sys, int ncomp, double* host_out, bool kinetic) {
+  return clump_reduce_t<double>(ctx, sys, ncomp, host_out, kinetic);
+}
+
+}  // namespace

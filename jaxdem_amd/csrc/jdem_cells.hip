@@ -935,13 +935,15 @@ hipError_t launch_remap_hist(int64_t n, int M, int W32, int dim, const int32_t* 
 // FreeDomain.apply (free.py:42-58): per-block min(x - r) / max(x + r).
 template <class Real>
 __global__ void __launch_bounds__(kBlock) k_box_minmax(const Real* __restrict__ pos, const Real* __restrict__ rad,
-                                                        int64_t n, int dim, Real* __restrict__ part) {
+                                                        int64_t n, int dim, Real* __restrict__ part,
+                                                        const Real* __restrict__ pos_p) {
   __shared__ Real smin[3][kBlock], smax[3][kBlock];
   const int tid = threadIdx.x;
   Real lo[3] = {Real(INFINITY), Real(INFINITY), Real(INFINITY)};
   Real hi[3] = {-Real(INFINITY), -Real(INFINITY), -Real(INFINITY)};
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + tid; i < n; i += (int64_t)gridDim.x * blockDim.x) {
-    const Real r = rad[i];
+    Real r = rad[i];
+    if (pos_p) r = (dim == 3 ? norm_(ld<3>(pos_p, i)) : norm_(ld<2>(pos_p, i))) + r;  // clumps: |pos_p| + rad
     for (int a = 0; a < dim; ++a) {
       const Real x = pos[dim * i + a];
       lo[a] = fmin_(lo[a], x - r);
@@ -969,13 +971,13 @@ __global__ void __launch_bounds__(kBlock) k_box_minmax(const Real* __restrict__ 
 }
 
 hipError_t launch_box_minmax(const float* pos, const float* rad, int64_t n, int dim, float* part, int nparts,
-                             hipStream_t s) {
-  hipLaunchKernelGGL((k_box_minmax<float>), dim3(nparts), dim3(kBlock), 0, s, pos, rad, n, dim, part);
+                             hipStream_t s, const float* pos_p) {
+  hipLaunchKernelGGL((k_box_minmax<float>), dim3(nparts), dim3(kBlock), 0, s, pos, rad, n, dim, part, pos_p);
   return hipGetLastError();
 }
 hipError_t launch_box_minmax(const double* pos, const double* rad, int64_t n, int dim, double* part, int nparts,
-                             hipStream_t s) {
-  hipLaunchKernelGGL((k_box_minmax<double>), dim3(nparts), dim3(kBlock), 0, s, pos, rad, n, dim, part);
+                             hipStream_t s, const double* pos_p) {
+  hipLaunchKernelGGL((k_box_minmax<double>), dim3(nparts), dim3(kBlock), 0, s, pos, rad, n, dim, part, pos_p);
   return hipGetLastError();
 }
 

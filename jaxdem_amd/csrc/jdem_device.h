@@ -288,6 +288,20 @@ JD Vec<3, Real> q_rotate(QuatT<Real> q, Vec<3, Real> v) {
   Vec<3, Real> t = cross(qv(q), v) * Real(2);
   return (v + t * q.w) + cross(qv(q), t);
 }
+// quaternion.py:226-236: a 2D vector turns about z
+template <class Real>
+JD Vec<2, Real> q_rotate(QuatT<Real> q, Vec<2, Real> v) {
+  Real c = q.w * q.w - q.z * q.z;
+  Real s = (Real(2) * q.w) * q.z;
+  return {c * v.x - s * v.y, s * v.x + c * v.y};
+}
+// utils.norm (linalg.py:113-133)
+template <class V>
+JD auto norm_(V v) {
+  auto n2 = dot(v, v);
+  using Real = decltype(n2);
+  return (n2 == Real(0)) ? Real(0) : sqrt_(fmax_(n2, Real(1e-16)));
+}
 // quaternion.py:247-268
 template <class Real>
 JD Vec<3, Real> q_rotate_back(QuatT<Real> q, Vec<3, Real> v) {

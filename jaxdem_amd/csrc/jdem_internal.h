@@ -79,6 +79,8 @@ struct jdem_ctx {
   std::vector<char> route_host;
   // per-cell counts / starts of the batch rebuild's counting sort
   jdem::Buf ccnt;
+  // clumps: members per clump (CSR), lab positions, levers, member totals
+  jdem::Buf cl_start, cl_members, plab, ppr, ftot, ttot;
   // every scratch buffer of the context, for accounting and release
   std::vector<jdem::Buf*> bufs();
   size_t scratch_bytes();
@@ -175,6 +177,18 @@ struct StepArgsT {
   int32_t step_idx;
   // energy / virial outputs (per particle, float64)
   double* e_out;
+  // clumps (null for spheres): ids (the pair mask), R(q) pos_p per member
+  // (the torque lever), the members of each clump (CSR: cl_start (N + 1),
+  // cl_members (N) in ascending index), body offsets, and the per-member
+  // force / torque totals of ForceManager before the clump sum
+  const int32_t* clump;
+  Real* ppr;
+  const int32_t* cl_start;
+  const int32_t* cl_members;
+  const Real* pos_p;
+  Real* plab;
+  Real* ftot;
+  Real* ttot;
 };
 using StepArgs = StepArgsT<float>;
 // The float64 form of the plain step (jdem_system_step_f64 and friends).
@@ -193,6 +207,12 @@ hipError_t launch_virial(const StepArgs& a, int dim, int law, int src, hipStream
 hipError_t launch_step(const StepArgs64& a, int dim, int law, int src, hipStream_t s);
 hipError_t launch_energy(const StepArgs64& a, int dim, int law, int src, hipStream_t s);
 hipError_t launch_kinetic(const StepArgs64& a, int dim, hipStream_t s);
+// clumps: lab positions and levers; ForceManager totals; the sum over each
+// clump broadcast to its members (both scalar types)
+hipError_t launch_clump_lab(const StepArgs& a, int dim, hipStream_t s);
+hipError_t launch_clump_lab(const StepArgs64& a, int dim, hipStream_t s);
+hipError_t launch_clump_forces(const StepArgs& a, int dim, hipStream_t s);
+hipError_t launch_clump_forces(const StepArgs64& a, int dim, hipStream_t s);
 
 // launchers (jdem_cells.hip)
 hipError_t launch_hash(const float* pos, int64_t n, int dim, const DomP& dom, const GridP& g, uint32_t* keys,
@@ -291,10 +311,11 @@ hipError_t launch_nlist_naive(const NLBuildT<double>& b, hipStream_t s);
 hipError_t launch_remap_hist(int64_t n, int M, int W32, int dim, const int32_t* old_nl, const uint32_t* old_live,
                              const float* old_xi, const int32_t* new_nl, uint32_t* new_live, float* new_xi,
                              hipStream_t s);
+// pos_p (clumps, or null): the bounding radius is |pos_p| + rad
 hipError_t launch_box_minmax(const float* pos, const float* rad, int64_t n, int dim, float* part, int nparts,
-                             hipStream_t s);
+                             hipStream_t s, const float* pos_p = nullptr);
 hipError_t launch_box_minmax(const double* pos, const double* rad, int64_t n, int dim, double* part, int nparts,
-                             hipStream_t s);
+                             hipStream_t s, const double* pos_p = nullptr);
 hipError_t launch_sum_f64(const double* in, int64_t n, int stride, int ncomp, double* partial, double* out,
                           int nparts, hipStream_t s);
 hipError_t launch_kinetic(const StepArgs& a, int dim, hipStream_t s);

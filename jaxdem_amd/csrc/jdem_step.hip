@@ -61,7 +61,7 @@ JD bool visit(const StepArgsT<Real>& a, const Side<D, Real>& me, int64_t i, int3
   // No contact is possible: delta of every law is exactly 0 (|rij| exceeds
   // R by far more than the rounding of d_sq * rsqrt(d_sq)).
   if (d_sq > (R * R) * Real(1.0001)) return false;
-  if (j == (int32_t)i) return false;
+  if (a.clump ? a.clump[j] == a.clump[i] : j == (int32_t)i) return false;  // valid_interaction_mask
   if (!a.isb && a.bond_w > 0 && is_bonded(a, i, j)) return false;
   Side<D, Real> b;
   b.x = xj;
@@ -203,6 +203,9 @@ __global__ void __launch_bounds__(kBlock) k_step(StepArgsT<Real> a) {
     T = zero_ang<D, Real>();
   } else if (a.do_force) {
     particle_force<D, LAW, SRC, PER>(a, idx, i, me, F, T);
+    // clump member: the lever torque of its summed force (naive.py:231,
+    // cell_list.py:462)
+    if (a.ppr) T = T + cross_t(ld<D>(a.ppr, i), F);
   } else {
     F = ld<D>(a.force, i);
     T = lda<D>(a.torque, i);
@@ -283,7 +286,7 @@ __global__ void __launch_bounds__(kBlock) k_energy(StepArgsT<Real> a) {
   me.sp = species_of<LAW>(a.law, i);
   double acc = 0.0;
   auto one = [&](int32_t j, Vec<D, Real> xj, Real rj) {
-    if (j == (int32_t)i) return;
+    if (a.clump ? a.clump[j] == a.clump[i] : j == (int32_t)i) return;
     if (!a.isb && a.bond_w > 0 && is_bonded(a, i, j)) return;
     const Vec<D, Real> rij = disp_<PER>(me.x, xj, a.dom);
     const Real d_sq = dot(rij, rij);
@@ -429,7 +432,9 @@ template <int D, class Real>
 __global__ void __launch_bounds__(kBlock) k_kinetic(StepArgsT<Real> a) {
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
-  const double m = a.mass[i];
+  // a clump member counts 1/(member count) of its clump (thermal.py:20-77)
+  const double cnt = a.clump ? (double)(a.cl_start[a.clump[i] + 1] - a.cl_start[a.clump[i]]) : 1.0;
+  const double m = (double)a.mass[i] / cnt;
   double ke = 0.0;  // rotational part
   double v2 = 0.0;
   for (int c = 0; c < D; ++c) {
@@ -454,6 +459,7 @@ __global__ void __launch_bounds__(kBlock) k_kinetic(StepArgsT<Real> a) {
     const double w = a.ang_in[i];
     ke += 0.5 * (double)a.inertia[i] * w * w;
   }
+  ke = ke / cnt;
   // translational and rotational parts, summed separately
   a.e_out[2 * i] = ke_t;
   a.e_out[2 * i + 1] = ke;
@@ -524,6 +530,62 @@ __global__ void __launch_bounds__(kBlock) k_stage(StepArgs a, int stage) {
   st(a.vel_out, i, v);
   sta(a.ang_out, i, w);
   stq(a.q, i, q);
+  st(a.force, i, F);
+  sta(a.torque, i, T);
+}
+
+// ---------------------------------------------------------------------------
+// clumps (rigid bodies of spheres; every member holds its clump's pos_c, q,
+// velocities, mass and inertia)
+// ---------------------------------------------------------------------------
+// State._pos_p_rot = R(q) pos_p and State.pos = pos_c + R(q) pos_p
+// (state.py:230-304)
+template <int D, class Real>
+__global__ void __launch_bounds__(kBlock) k_clump_lab(StepArgsT<Real> a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const QuatT<Real> q = ldq(a.q, i);
+  const Vec<D, Real> r = q_rotate(q, ld<D>(a.pos_p, i));
+  st(a.ppr, i, r);
+  st(a.plab, i, ld<D>(a.pos_in, i) + r);
+}
+
+// ForceManager.apply for clumps (force_manager.py:359-412): per member,
+// gravity with mass / member count, the lever torque of particle forces,
+// the totals; then each member's force and torque are its clump's sums of
+// the totals, in ascending member index.
+template <int D, class Real>
+__global__ void __launch_bounds__(kBlock) k_clump_total(StepArgsT<Real> a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int c = a.clump[i];
+  const Real cnt = (Real)(a.cl_start[c + 1] - a.cl_start[c]);
+  const Vec<D, Real> fp = a.ext_f ? ld<D>(a.ext_f, i) : zero_vec<D, Real>();
+  Vec<D, Real> fc = a.ext_fc ? ld<D>(a.ext_fc, i) : zero_vec<D, Real>();
+  Ang<D, Real> tp = a.ext_t ? lda<D>(a.ext_t, i) : zero_ang<D, Real>();
+  Vec<D, Real> g;
+  if constexpr (D == 3)
+    g = Vec<3, Real>{a.g[0], a.g[1], a.g[2]};
+  else
+    g = Vec<2, Real>{a.g[0], a.g[1]};
+  fc = fc + g * (a.mass[i] / cnt);
+  tp = tp + cross_t(ld<D>(a.ppr, i), fp);
+  st(a.ftot, i, (ld<D>(a.force, i) + fp) + fc);
+  sta(a.ttot, i, lda<D>(a.torque, i) + tp);
+}
+
+template <int D, class Real>
+__global__ void __launch_bounds__(kBlock) k_clump_sum(StepArgsT<Real> a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int c = a.clump[i];
+  Vec<D, Real> F = zero_vec<D, Real>();
+  Ang<D, Real> T = zero_ang<D, Real>();
+  for (int k = a.cl_start[c]; k < a.cl_start[c + 1]; ++k) {
+    const int32_t m = a.cl_members[k];
+    F = F + ld<D>(a.ftot, m);
+    T = T + lda<D>(a.ttot, m);
+  }
   st(a.force, i, F);
   sta(a.torque, i, T);
 }
@@ -633,6 +695,32 @@ hipError_t launch_energy(const StepArgs64& a, int dim, int law, int src, hipStre
   return EnergyDispatch<double>::run(a, dim, law, src, s);
 }
 hipError_t launch_kinetic(const StepArgs64& a, int dim, hipStream_t s) { return kinetic_run(a, dim, s); }
+template <class Real>
+hipError_t clump_lab_run(const StepArgsT<Real>& a, int dim, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  if (dim == 3)
+    hipLaunchKernelGGL((k_clump_lab<3, Real>), dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL((k_clump_lab<2, Real>), dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+template <class Real>
+hipError_t clump_forces_run(const StepArgsT<Real>& a, int dim, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  if (dim == 3) {
+    hipLaunchKernelGGL((k_clump_total<3, Real>), dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_clump_sum<3, Real>), dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  } else {
+    hipLaunchKernelGGL((k_clump_total<2, Real>), dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+    hipLaunchKernelGGL((k_clump_sum<2, Real>), dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  }
+  return hipGetLastError();
+}
+hipError_t launch_clump_lab(const StepArgs& a, int dim, hipStream_t s) { return clump_lab_run(a, dim, s); }
+hipError_t launch_clump_lab(const StepArgs64& a, int dim, hipStream_t s) { return clump_lab_run(a, dim, s); }
+hipError_t launch_clump_forces(const StepArgs& a, int dim, hipStream_t s) { return clump_forces_run(a, dim, s); }
+hipError_t launch_clump_forces(const StepArgs64& a, int dim, hipStream_t s) { return clump_forces_run(a, dim, s); }
+
 hipError_t launch_stage(const StepArgs& a, int dim, int stage, hipStream_t s) {
   if (a.n == 0) return hipSuccess;
   if (dim == 3)

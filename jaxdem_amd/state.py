@@ -8,9 +8,13 @@ Field names, shapes and defaults follow the reference ``State``: ``pos_c``,
 Floats are float32 (the reference's default precision; float64 when
 ``jax_enable_x64`` is set in ``jaxdem_amd.config``), integers int32.
 
-The GPU time step covers spheres: ``pos_p == 0`` and every particle its own
-clump.  ``State.create`` records whether that holds (``is_sphere_state``);
-clump and mesh builders of the reference are out of scope.
+The GPU time step covers spheres (``pos_p == 0`` and every particle its own
+clump; ``State.create`` records whether that holds in ``is_sphere_state``)
+and clumps of spheres: rigid bodies whose members share a ``clump_id`` and
+carry their offsets ``pos_p`` from the clump centre, each member holding the
+clump's ``pos_c``, velocities, ``q``, total mass and principal inertia (naive
+and CellList colliders, free and periodic domains).  The reference's clump,
+facet and mesh builders (``add_clump`` and friends) are out of scope.
 """
 
 from __future__ import annotations

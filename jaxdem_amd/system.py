@@ -337,8 +337,21 @@ def _require_gpu(state: State) -> None:
             "jaxdem_amd runs the DEM time step as HIP kernels on a GPU; the state tensors are on "
             f"{state.pos_c.device} (create the State with device='cuda')"
         )
-    if not state.is_sphere_state:
-        raise NotImplementedError("the GPU time step covers spheres (pos_p == 0, one particle per clump)")
+
+
+def _clump_arrays(state: State) -> tuple:
+    """(pos_p, clump_id) pointers of a clump state, (None, None) for spheres.
+    The kernels index per-clump tables by clump id, so the ids are checked
+    to lie in [0, N) once per version of the array."""
+    if state.is_sphere_state:
+        return None, None
+    cid = state.clump_id
+    tag = (id(cid), cid._version)
+    if state.__dict__.get("_clumps_checked") != tag:
+        if state.N and (int(cid.min()) < 0 or int(cid.max()) >= state.N):
+            raise ValueError("clump_id outside [0, N)")
+        state.__dict__["_clumps_checked"] = tag
+    return _ptr(state.pos_p), _ptr(cid)
 
 
 def _ptr(t: torch.Tensor | None) -> int | None:
@@ -368,6 +381,7 @@ def _state_struct(state: State, cls=L.JdemState):
     s.mat_id = _ptr(state.mat_id)
     s.bond_id = _ptr(state.bond_id) if bw else None
     s.fixed = _ptr(state.fixed)
+    s.pos_p, s.clump_id = _clump_arrays(state)
     return s
 
 

@@ -39,9 +39,14 @@ Documented deviations from the reference (each also pinned by a test):
   spring in the form of Luding (2008); parity against the reference is
   unpinned and the laws are pinned by closed-form tests instead.
 
-Sphere-only scope: ``pos_p == 0`` and every particle is its own clump, so the
-clump segment operations of ``ForceManager.apply`` and ``ReflectDomain.apply``
-are identities (``force_manager.py:409-412``; ``reflect.py:176-181,234``).
+Clumps (rigid bodies of spheres, ``State.clump`` ids and body-frame offsets
+``State.pos_p``): pair forces at the members' lab positions
+``pos_c + R(q) pos_p`` with same-clump pairs masked, the torque lever
+``R(q) pos_p x F`` added per member, and ``ForceManager.apply``'s segment sum
+over the clump (force_manager.py:392-412), taken in ascending member index.
+A state without ``clump`` is a sphere state: every particle its own clump and
+``pos_p == 0``, for which those operations are identities.  The reflective
+domain's clump segment operations (reflect.py:165-297) are not restated.
 """
 
 from __future__ import annotations
@@ -133,6 +138,23 @@ def q_rotate(q: np.ndarray, v: np.ndarray) -> np.ndarray:
         return np.concatenate([c * vx - s * vy, s * vx + c * vy], axis=-1)
     t = 2.0 * cross3(q[..., 1:], v)
     return (v + q[..., 0:1] * t) + cross3(q[..., 1:], t)
+
+
+def pos_p_rot(st) -> np.ndarray:
+    """``State._pos_p_rot`` = R(q) pos_p (state.py:230-273, quaternion.py:193-242)."""
+    return q_rotate(st.q, st.pos_p)
+
+
+def lab_pos(st) -> np.ndarray:
+    """``State.pos`` = pos_c + R(q) pos_p (state.py:295-304)."""
+    return st.pos + pos_p_rot(st)
+
+
+def norm(v: np.ndarray) -> np.ndarray:
+    """``utils.norm`` (linalg.py:113-133): 0 at 0, else sqrt(max(|v|^2, 1e-16))."""
+    n2 = dot(v, v)
+    T_ = v.dtype
+    return np.where(n2 == 0.0, np.asarray(0.0, T_), np.sqrt(np.maximum(n2, np.asarray(1e-16, T_))))
 
 
 def q_rotate_back(q: np.ndarray, v: np.ndarray) -> np.ndarray:
@@ -256,6 +278,9 @@ class State:
     bond_id: np.ndarray
     extra: dict = field(default_factory=dict)
     species: np.ndarray | None = None  # (N,) species_id, read by routed laws
+    # clumps: (N,) ids in [0, K) and (N, D) body-frame offsets; None for spheres
+    clump: np.ndarray | None = None
+    pos_p: np.ndarray | None = None
 
     @property
     def N(self) -> int:
@@ -356,12 +381,11 @@ def periodic_shift(pos: np.ndarray, dom: Domain) -> np.ndarray:
 
 
 def valid_mask(i: np.ndarray, j: np.ndarray, st: State, isb: bool) -> np.ndarray:
-    """``valid_interaction_mask`` for spheres (clump_id == arange).
-
-    ``clump_i != clump_j`` reduces to ``i != j``; the bond test checks
-    whether ``j`` is listed in ``bond_id[i]``.
+    """``valid_interaction_mask`` (colliders/__init__.py:225-243):
+    ``clump_i != clump_j`` (``i != j`` for spheres, clump_id == arange); the
+    bond test checks whether ``j`` is listed in ``bond_id[i]``.
     """
-    m = i != j
+    m = (st.clump[i] != st.clump[j]) if st.clump is not None else (i != j)
     if not isb and st.bond_id.shape[1] > 0:
         bonded = np.any(st.bond_id[i] == j[..., None], axis=-1)
         m = m & ~bonded
@@ -952,6 +976,9 @@ def naive_near_candidates(st: State, P: Params) -> np.ndarray:
 
 def compute_force(st: State, P: Params, nls: NLState | None = None):
     """``collider.compute_force`` for naive / CellList / NeighborList."""
+    if st.clump is not None:
+        _compute_force_clumps(st, P, nls)
+        return
     if P.collider == "naive":
         if st.N > 1024 or P.naive_by_cells:
             C = naive_near_candidates(st, P)
@@ -975,6 +1002,28 @@ def compute_force(st: State, P: Params, nls: NLState | None = None):
     else:
         raise ValueError(P.collider)
     st.force, st.torque = F, T
+
+
+def _compute_force_clumps(st: State, P: Params, nls: NLState | None):
+    """Clump members: forces at the lab positions (state.pos), same-clump pairs
+    masked, then ``torque += pos_p_rot x force`` per member (naive.py:210-233,
+    cell_list.py:461-462)."""
+    r = pos_p_rot(st)
+    lab = st.copy()
+    lab.pos = st.pos + r
+    if P.collider == "naive":
+        if st.N > 1024 or P.naive_by_cells:
+            C = naive_near_candidates(lab, P)
+        else:
+            C = drop_far_candidates(lab, P, naive_candidates(st.N))
+    elif P.collider == "celllist":
+        C, _ = cell_candidates(lab.pos, P.domain, P.cell_size, P.search_range)
+        C = drop_far_candidates(lab, P, C)
+    else:
+        raise NotImplementedError(f"clumps with the {P.collider} collider are not restated")
+    F, T, _ = accumulate(lab, P, C)
+    lever = cross3(r, F) if st.dim == 3 else cross2(r, F)
+    st.force, st.torque = F, T + lever
 
 
 def nl_init(st: State, P: Params, history: bool) -> NLState:
@@ -1033,10 +1082,26 @@ def force_manager_apply(st: State, P: Params, ext: dict | None = None):
     F_com = np.zeros((N, D), T_) if F_com is None else F_com
     T_part = np.zeros((N, A), T_) if T_part is None else T_part
     g = np.zeros(D, T_) if P.gravity is None else np.asarray(P.gravity, T_)
-    F_com = F_com + g * (st.mass / np.asarray(1.0, T_))[:, None]
+    if st.clump is None:
+        F_com = F_com + g * (st.mass / np.asarray(1.0, T_))[:, None]
+        F_total = (st.force + F_part) + F_com
+        T_total = st.torque + T_part
+        st.force, st.torque = F_total, T_total
+        return
+    # clumps (force_manager.py:392-412): every member holds the clump's total
+    # mass, so gravity takes mass / member count; particle forces add their
+    # lever torque; the totals are summed over the clump and broadcast
+    count = np.bincount(st.clump, minlength=N)[st.clump].astype(T_)
+    r = pos_p_rot(st)
+    F_com = F_com + g * (st.mass / count)[:, None]
+    T_part = T_part + (cross3(r, F_part) if D == 3 else cross2(r, F_part))
     F_total = (st.force + F_part) + F_com
     T_total = st.torque + T_part
-    st.force, st.torque = F_total, T_total
+    F_c = np.zeros((N, D), T_)
+    T_c = np.zeros((N, A), T_)
+    np.add.at(F_c, st.clump, F_total)  # sequential, ascending member index
+    np.add.at(T_c, st.clump, T_total)
+    st.force, st.torque = F_c[st.clump], T_c[st.clump]
 
 
 # --------------------------------------------------------------------------
@@ -1184,8 +1249,8 @@ def domain_apply(st: State, P: Params):
     dom = P.domain
     T_ = st.pos.dtype
     if dom.kind == "free":
-        # free.py:42-58 (pos_p == 0 so the bounding radius is rad)
-        br = st.rad[:, None]
+        # free.py:42-58: bounding radius |pos_p| + rad (rad for spheres)
+        br = st.rad[:, None] if st.pos_p is None else (norm(st.pos_p) + st.rad)[:, None]
         pmin = np.min(st.pos - br, axis=0)
         pmax = np.max(st.pos + br, axis=0)
         dom.box = pmax - pmin
@@ -1193,6 +1258,8 @@ def domain_apply(st: State, P: Params):
         return
     if dom.kind != "reflect":
         return
+    if st.clump is not None:
+        raise NotImplementedError("the reflective domain's clump segment operations are not restated")
     dt = np.asarray(P.dt, T_)
     zero = np.asarray(0.0, T_)
     one = np.asarray(1.0, T_)
@@ -1250,7 +1317,13 @@ def step(st: State, P: Params, n: int, nls: NLState | None = None) -> State:
 
 def potential_energy(st: State, P: Params, C: np.ndarray | None = None) -> float:
     """``compute_potential_energy`` (cell_list.py:469-496, naive.py:76-117):
-    1/2 sum of pair energies; pair terms in the state dtype, sum in f64."""
+    1/2 sum of pair energies; pair terms in the state dtype, sum in f64.
+    Clump members interact at their lab positions (state.pos)."""
+    if st.clump is not None:
+        lab = st.copy()
+        lab.pos = lab_pos(st)
+        lab.clump = st.clump
+        st = lab
     if C is None:
         if P.collider == "celllist":
             C, _ = cell_candidates(st.pos, P.domain, P.cell_size, P.search_range)
@@ -1270,8 +1343,10 @@ def potential_energy(st: State, P: Params, C: np.ndarray | None = None) -> float
 
 
 def kinetic_energy(st: State) -> float:
-    """Translational + rotational KE (utils/thermal.py:83-122) in f64."""
+    """Translational + rotational KE (utils/thermal.py:20-122) in f64; a
+    clump member counts 1/(member count) of its clump's energy."""
     m = st.mass.astype(np.float64)
+    cnt = np.ones(st.N) if st.clump is None else np.bincount(st.clump, minlength=st.N)[st.clump].astype(np.float64)
     v = st.vel.astype(np.float64)
     w = st.ang_vel.astype(np.float64)
     if st.dim == 3:
@@ -1279,7 +1354,7 @@ def kinetic_energy(st: State) -> float:
     else:
         wb = w
     I = st.inertia.astype(np.float64)
-    return float(0.5 * np.sum(m * np.sum(v * v, axis=1)) + 0.5 * np.sum(I * wb * wb))
+    return float(0.5 * np.sum((m / cnt) * np.sum(v * v, axis=1)) + 0.5 * np.sum(np.sum(I * wb * wb, axis=1) / cnt))
 
 
 def pair_virial(st: State, P: Params, C: np.ndarray) -> np.ndarray:

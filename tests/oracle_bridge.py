@@ -91,6 +91,8 @@ def state_of(state, dtype=np.float32) -> O.State:
         mat_id=d["mat_id"].astype(np.int64),
         bond_id=bond,
         species=d["species_id"].astype(np.int64),
+        clump=None if state.is_sphere_state else d["clump_id"].astype(np.int64),
+        pos_p=None if state.is_sphere_state else d["pos_p"].astype(dtype),
     )
 
 

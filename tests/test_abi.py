@@ -50,7 +50,7 @@ def test_version_and_target():
 
 STRUCT_FIELDS = {
     "jdem_state": ["n", "dim", "bond_width", "pos", "vel", "ang_vel", "q", "force", "torque", "rad", "mass",
-                   "inertia", "mat_id", "bond_id", "fixed"],
+                   "inertia", "mat_id", "bond_id", "fixed", "pos_p", "clump_id"],
     "jdem_domain": ["kind", "dim", "box_size", "inv_box_size", "anchor", "restitution"],
     "jdem_law": ["kind", "n_mat", "t0", "t1", "t2", "m0", "m1", "m2", "m3", "p0", "p1", "n_species", "n_terms",
                  "route_off", "route_terms", "species"],
@@ -69,7 +69,7 @@ STRUCT_FIELDS = {
     "jdem_precise": ["n", "dim", "dims", "cell", "cell_lo", "cidx", "off", "rad", "force", "k", "stall_steps"],
     "jdem_frames": ["n_frames", "steps", "pos", "vel", "ang_vel", "q", "force", "torque"],
     "jdem_state_f64": ["n", "dim", "bond_width", "pos", "vel", "ang_vel", "q", "force", "torque", "rad", "mass",
-                       "inertia", "mat_id", "bond_id", "fixed"],
+                       "inertia", "mat_id", "bond_id", "fixed", "pos_p", "clump_id"],
     "jdem_domain_f64": ["kind", "dim", "box_size", "inv_box_size", "anchor", "restitution"],
     "jdem_law_f64": ["kind", "n_mat", "t0", "t1", "t2", "m0", "m1", "m2", "m3", "p0", "p1", "n_species", "n_terms",
                      "route_off", "route_terms", "species"],

@@ -1,0 +1,171 @@
+"""Clumps (rigid bodies of spheres) on the GPU against the oracle on the same
+seeded inputs: dimers, trimers and single spheres with random orientations
+and spins; forces and torques bit for bit (naive and CellList, 2D and 3D),
+whole steps of every integrator pair with gravity (periodic and free
+domains), potential and kinetic energy to 1e-12, the float64 form within
+1e-12 of the float64 oracle; the combinations off the GPU path refuse."""
+
+from __future__ import annotations
+
+import contextlib
+import math
+
+import numpy as np
+import pytest
+import torch
+
+import jaxdem_amd as jd
+from oracle import jdem_oracle as O
+from tests import oracle_bridge as B
+from tests.test_gpu_parity import assert_bitwise
+
+pytestmark = pytest.mark.gpu
+
+
+def clump_scene(n_side, dim, seed, spacing=1.7, r=0.5, d=0.45, device="cuda", dtype=np.float32):
+    """A jittered lattice of bodies: dimers, trimers and single spheres with
+    random orientations, velocities and spins.  Returns (state, box)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    A = 3 if dim == 3 else 1
+    pos, pp, cid, mass, inertia, q, vel, w = [], [], [], [], [], [], [], []
+    k = 0
+    for idx in np.ndindex(*([n_side] * dim)):
+        c = (np.array(idx) + 0.5) * spacing + rng.uniform(-0.25, 0.25, dim)
+        kind = k % 3  # 0 sphere, 1 dimer, 2 trimer
+        offs = {0: [[0.0, 0.0, 0.0]], 1: [[-d, 0.0, 0.0], [d, 0.0, 0.0]],
+                2: [[-d, 0.0, 0.0], [d, 0.0, 0.0], [0.0, d, 0.0]]}[kind]
+        offs = np.array(offs)[:, :dim]
+        offs = offs - offs.mean(0)
+        m1 = (4.0 / 3.0 * math.pi * r**3) if dim == 3 else math.pi * r * r
+        M = m1 * len(offs)
+        if dim == 3:
+            qq = rng.standard_normal(4)
+            qq /= np.linalg.norm(qq)
+            # principal inertia of the body about its centre (member spheres + parallel axes), body frame
+            I = np.zeros(3)
+            for o in offs:
+                I += 0.4 * m1 * r * r + m1 * (np.dot(o, o) - o * o)
+        else:
+            th = rng.uniform(0, 2 * np.pi)
+            qq = np.array([np.cos(th / 2), 0.0, 0.0, np.sin(th / 2)])
+            I = np.array([sum(0.5 * m1 * r * r + m1 * np.dot(o, o) for o in offs)])
+        v = rng.standard_normal(dim)
+        om = rng.standard_normal(A)
+        for o in offs:
+            pos.append(c)
+            pp.append(o)
+            cid.append(k)
+            mass.append(M)
+            inertia.append(I)
+            q.append(qq)
+            vel.append(v)
+            w.append(om)
+        k += 1
+    box = np.full(dim, n_side * spacing)
+    st = jd.State.create(pos=np.array(pos, dtype), pos_p=np.array(pp, dtype), clump_id=np.array(cid),
+                         mass=np.array(mass, dtype), inertia=np.array(inertia, dtype), q=np.array(q, dtype),
+                         vel=np.array(vel, dtype), ang_vel=np.array(w, dtype), rad=np.full(len(pos), r, dtype),
+                         device=device)
+    assert not st.is_sphere_state
+    return st, box
+
+
+def clump_system(st, box, law, collider, domain="periodic", dt=1e-4, lin="verlet", rot="verletspiral", g=None):
+    mats = [jd.Material.create("elasticfrict", density=1.0, young=1e4, poisson=0.3, mu=0.4, e=0.7, mu_r=0.05)]
+    ckw = dict(state=st) if collider == "CellList" else {}
+    fm = {"gravity": g} if g is not None else None
+    return jd.System.create(st.shape, dt=dt, domain_type=domain,
+                            domain_kw=dict(box_size=box, anchor=np.zeros(st.dim)), collider_type=collider,
+                            collider_kw=ckw, force_model_type=law, mat_table=jd.MaterialTable.from_materials(mats),
+                            linear_integrator_type=lin, rotation_integrator_type=rot, force_manager_kw=fm)
+
+
+@pytest.mark.parametrize("collider", ["naive", "CellList"])
+@pytest.mark.parametrize("law", ["spring", "cundallstrack"])
+@pytest.mark.parametrize("dim", [3, 2])
+def test_clump_compute_force_bitwise(device, dim, law, collider):
+    print(f"[clumps] force {law} {collider} dim={dim}", flush=True)
+    st, box = clump_scene(7 if dim == 3 else 16, dim, seed=4)
+    sy = clump_system(st, box, law, collider)
+    st2, _ = sy.collider.compute_force(st, sy)
+    ost = B.state_of(st)
+    O.compute_force(ost, B.params_of(st, sy))
+    assert np.abs(ost.force).max() > 0
+    assert_bitwise("force", st2.force.cpu().numpy(), ost.force)
+    assert_bitwise("torque", st2.torque.cpu().numpy(), ost.torque)
+
+
+CASES = [("naive", "periodic", "verlet", "verletspiral"), ("naive", "free", "verlet", "verletspiral"),
+         ("CellList", "periodic", "euler", "spiral"), ("CellList", "periodic", "verlet", "verletspiral")]
+
+
+@pytest.mark.parametrize("collider,domain,lin,rot", CASES)
+@pytest.mark.parametrize("dim", [3, 2])
+def test_clump_steps_bitwise(device, dim, collider, domain, lin, rot):
+    """30 steps with gravity: every member's state bit for bit, members of a
+    clump identical, the free box fitted with |pos_p| + rad."""
+    print(f"[clumps] step {collider} {domain} {lin}/{rot} dim={dim}", flush=True)
+    st, box = clump_scene(6 if dim == 3 else 14, dim, seed=7)
+    g = np.array([0.0, -2.0, 0.0][:dim])
+    sy = clump_system(st, box, "cundallstrack", collider, domain, lin=lin, rot=rot, g=g)
+    n = 30
+    st2, sy2 = jd.System.step(st, sy, n=n)
+    P = B.params_of(st, sy)
+    ost = O.step(B.state_of(st), P, n)
+    for f, of in (("pos_c", "pos"), ("vel", "vel"), ("ang_vel", "ang_vel"), ("force", "force"),
+                  ("torque", "torque")):
+        assert_bitwise(f, getattr(st2, f).cpu().numpy(), getattr(ost, of))
+    assert_bitwise("q", st2.q.data.cpu().numpy(), ost.q)
+    cid = st.clump_id.cpu().numpy()
+    F = st2.force.cpu().numpy()
+    for c in np.unique(cid)[:20]:
+        assert np.all(F[cid == c] == F[cid == c][0])
+    if domain == "free":
+        np.testing.assert_array_equal(sy2.domain.box_size, P.domain.box)
+        np.testing.assert_array_equal(sy2.domain.anchor, P.domain.anchor)
+
+
+def test_clump_energies(device):
+    print("[clumps] energies", flush=True)
+    st, box = clump_scene(7, 3, seed=9)
+    sy = clump_system(st, box, "spring", "CellList")
+    e = sy.potential_energy(st)
+    ke = jd.utils.compute_kinetic_energy(st, sy)
+    ost = B.state_of(st)
+    P = B.params_of(st, sy)
+    lab = ost.copy()
+    lab.pos = O.lab_pos(ost)
+    C, _ = O.cell_candidates(lab.pos, P.domain, P.cell_size, P.search_range)
+    oe = O.potential_energy(ost, P, C)
+    assert e > 0 and e == pytest.approx(oe, rel=1e-12)
+    assert ke == pytest.approx(O.kinetic_energy(ost), rel=1e-12)
+
+
+def test_clump_f64(device):
+    """The float64 form: 20 steps within 1e-12 of the float64 oracle."""
+    print("[clumps] float64", flush=True)
+    old = jd.config.jax_enable_x64
+    jd.config.update("jax_enable_x64", True)
+    try:
+        st, box = clump_scene(6, 3, seed=12, dtype=np.float64)
+        assert st.pos_c.dtype == torch.float64
+        sy = clump_system(st, box, "hertz", "naive", g=np.array([0.0, 0.0, -1.0]))
+        st2, _ = jd.System.step(st, sy, n=20)
+    finally:
+        jd.config.update("jax_enable_x64", old)
+    ost = O.step(B.state_of(st, np.float64), B.params_of(st, sy, np.float64), 20)
+    for f, of in (("pos_c", "pos"), ("vel", "vel"), ("ang_vel", "ang_vel"), ("force", "force")):
+        a = getattr(st2, f).cpu().numpy()
+        b = getattr(ost, of)
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max(), f
+
+
+def test_clump_unsupported_combinations_raise(device):
+    st, box = clump_scene(4, 3, seed=1)
+    with pytest.raises(jd._lib.JdemError, match="reflective"):
+        jd.System.step(st, clump_system(st, box, "spring", "naive", "reflect"), n=1)
+    nl = jd.System.create(st.shape, dt=1e-4, domain_type="periodic", domain_kw=dict(box_size=box),
+                          collider_type="NeighborList",
+                          collider_kw=dict(state=st, cutoff=1.0, skin=0.1, secondary_collider_type="naive"))
+    with pytest.raises(jd._lib.JdemError, match="NeighborList"):
+        jd.System.step(st, nl, n=1)
