@@ -8,6 +8,7 @@ domains), potential and kinetic energy to 1e-12, the float64 form within
 from __future__ import annotations
 
 import contextlib
+import dataclasses
 import math
 
 import numpy as np
@@ -169,3 +170,107 @@ def test_clump_unsupported_combinations_raise(device):
                           collider_kw=dict(state=st, cutoff=1.0, skin=0.1, secondary_collider_type="naive"))
     with pytest.raises(jd._lib.JdemError, match="NeighborList"):
         jd.System.step(st, nl, n=1)
+
+
+# ---------------------------------------------------------------------------
+# The reference's clump tests (test_colliders_invariance.py:113-233, 405-434;
+# test_energy_conservation.py:108-124), restated: clumps of nv asperities of
+# radius 0.1 on a ring (2D, nv = 16) or a Fibonacci sphere (3D, nv = 42) of
+# radius 0.3, total mass 1 per clump, on a lattice of spacing 1.0 with the
+# lattice velocities of grid_state(seed 42).  The reference takes volume,
+# centre, principal inertia and frame of the asperity union from a Monte
+# Carlo estimate (utils/clumps.py), a builder outside this package; here the
+# centre is the asperities' mean, the frame the lab frame and the inertia that
+# of a uniform ball (disk) of radius 0.4 -- a consistent rigid body, which is
+# all these tests check.
+# ---------------------------------------------------------------------------
+def set_up_clump(dim, N=350, collider="naive", domain="periodic", seed=42, dt=1e-3, device="cuda"):
+    if dim == 3:
+        N = int(np.ceil(N * 1.5))
+    nv = 16 if dim == 2 else 42
+    per = int((N // nv) ** (1.0 / dim))
+    n = per**dim
+    R_p, r_a = 0.4, 0.1
+    if dim == 2:
+        t = 2 * np.pi * np.arange(nv) / nv
+        shell = np.stack([np.cos(t), np.sin(t)], 1)
+    else:
+        k = np.arange(nv) + 0.5
+        phi = np.arccos(1 - 2 * k / nv)
+        th = np.pi * (1 + 5**0.5) * k
+        shell = np.stack([np.cos(th) * np.sin(phi), np.sin(th) * np.sin(phi), np.cos(phi)], 1)
+    shell = (R_p - r_a) * shell
+    shell -= shell.mean(0)
+    spacing = 2.0 * (R_p + r_a)
+    grid = jd.utils.grid_state(n_per_axis=(per,) * dim, spacing=spacing, radius_range=(R_p, R_p),
+                               vel_range=(-1.0, 1.0), seed=seed, device="cpu")
+    centres = grid.pos_c.numpy()[:n].astype(np.float64)
+    vel_c = grid.vel.numpy()[:n]
+    I = (0.4 if dim == 3 else 0.5) * 1.0 * R_p * R_p
+    A = 3 if dim == 3 else 1
+    st = jd.State.create(pos=np.repeat(centres, nv, 0), pos_p=np.tile(shell, (n, 1)),
+                         vel=np.repeat(vel_c, nv, 0), rad=np.full(n * nv, r_a), mass=np.ones(n * nv),
+                         inertia=np.full((n * nv, A), I), clump_id=np.repeat(np.arange(n), nv), device=device)
+    mt = jd.MaterialTable.from_materials([jd.Material.create("elastic", density=0.27, young=1.0e3, poisson=0.3)])
+    ckw = dict(state=st) if collider == "CellList" else {}
+    sy = jd.System.create(st.shape, domain_type=domain,
+                          domain_kw=dict(box_size=np.full(dim, spacing * per), anchor=np.full(dim, -R_p - r_a)),
+                          collider_type=collider, collider_kw=ckw, dt=dt, mat_table=mt)
+    return st, sy
+
+
+@contextlib.contextmanager
+def precision(name):
+    old = jd.config.jax_enable_x64
+    jd.config.update("jax_enable_x64", name == "float64")
+    try:
+        yield
+    finally:
+        jd.config.update("jax_enable_x64", old)
+
+
+@pytest.mark.parametrize("prec", ["float32", "float64"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_clumps_invariance(device, dim, prec):
+    """naive vs CellList over 200 steps, atol = rtol = 5e-2 (periodic: the
+    domains the CellList takes with clumps)."""
+    print(f"[clumps] invariance dim={dim} {prec}", flush=True)
+    with precision(prec):
+        s1, y1 = set_up_clump(dim, N=128, collider="naive")
+        s2, y2 = set_up_clump(dim, N=128, collider="CellList")
+        s1, y1 = jd.System.step(s1, y1, n=200)
+        s2, y2 = jd.System.step(s2, y2, n=200)
+        pe1 = jd.utils.compute_potential_energy(s1, y1)
+        pe2 = jd.utils.compute_potential_energy(s2, y2)
+    for f in ("pos", "vel", "force", "ang_vel", "torque"):
+        np.testing.assert_allclose(getattr(s1, f).cpu().numpy(), getattr(s2, f).cpu().numpy(), atol=5e-2, rtol=5e-2,
+                                   err_msg=f)
+    np.testing.assert_allclose(s1.q.data.cpu().numpy(), s2.q.data.cpu().numpy(), atol=5e-2, rtol=5e-2)
+    np.testing.assert_allclose(pe1, pe2, atol=5e-2, rtol=5e-2)
+
+
+@pytest.mark.parametrize("domain", ["periodic", "free"])
+@pytest.mark.parametrize("integrator", ["euler", "verlet"])
+@pytest.mark.parametrize("dim", [2, 3])
+def test_clumps_energy_conservation(device, dim, integrator, domain):
+    """std(E) over 100 frames at five dt in [3e-5, 2e-3], total time 5, in
+    float64 as the reference runs it: slope > 1.8 (velocity Verlet +
+    VV-SPIRAL), > 0.7 (Euler + SPIRAL)."""
+    print(f"[clumps] energy dim={dim} {integrator} {domain}", flush=True)
+    from jaxdem_amd.utils import compute_energy
+
+    lin, rot = ("euler", "spiral") if integrator == "euler" else ("verlet", "verletspiral")
+    dts = np.logspace(np.log10(2e-3), np.log10(3e-5), 5)
+    stds = []
+    with precision("float64"):
+        st, sy = set_up_clump(dim, domain=domain)
+        for dt in dts:
+            steps = int(5.0 / dt)
+            stride = max(steps // 100, 1)
+            s = dataclasses.replace(sy, dt=float(dt), linear_integrator=jd.LinearIntegrator.create(lin),
+                                    rotation_integrator=jd.RotationIntegrator.create(rot))
+            _, _, e = jd.System.trajectory_rollout(st, s, n=100, stride=stride, save_fn=compute_energy)
+            stds.append(float(np.std(np.asarray(e, np.float64))))
+    slope, _ = np.polyfit(np.log(dts), np.log(stds), 1)
+    print(f"[clumps] energy slope {slope:.3f}", flush=True)
+    assert slope > (0.7 if integrator == "euler" else 1.8), (slope, stds)
