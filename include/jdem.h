@@ -109,8 +109,9 @@ typedef struct jdem_state {
    * Every member holds its clump's pos, vel, q, ang_vel, mass (the total)
    * and principal inertia.  Pairs of one clump do not interact; member
    * forces act at pos + R(q) pos_p, and ForceManager sums them over the
-   * clump (force_manager.py:359-412).  Colliders NAIVE and CELL_LIST,
-   * FREE and PERIODIC domains, laws without history. */
+   * clump (force_manager.py:359-412).  Colliders NAIVE, CELL_LIST and
+   * NEIGHBOR_LIST (on the caller's rows, checked and built on the lab
+   * positions), FREE and PERIODIC domains. */
   const float* pos_p;
   const int32_t* clump_id;
 } jdem_state;

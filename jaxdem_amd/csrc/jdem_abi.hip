@@ -294,8 +294,8 @@ int check_system(jdem_ctx* ctx, const jdem_system* sys, bool batch = false, bool
   if (s.clump_id || s.pos_p) {
     if (s.n > 0 && (!s.clump_id || !s.pos_p)) return fail(ctx, JDEM_EINVAL, "clumps need both pos_p and clump_id");
     if (batch) return fail(ctx, JDEM_EUNSUPPORTED, "batches with clumps");
-    if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST)
-      return fail(ctx, JDEM_EUNSUPPORTED, "clumps with the NeighborList collider");
+    if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST && (sys->nlist.resident_in || sys->nlist.resident_out))
+      return fail(ctx, JDEM_EUNSUPPORTED, "clumps with a resident NeighborList working set");
     if (sys->domain.kind == JDEM_DOMAIN_REFLECT)
       return fail(ctx, JDEM_EUNSUPPORTED, "clumps in a reflective domain");
   }
@@ -480,6 +480,7 @@ int build_nlist(jdem_ctx* ctx, const jdem_system* sys, int kind, const jdem_grid
   b.dom = dom_of(sys->domain);
   b.nl = nl_out;
   b.overflow = (int32_t*)ctx->flags.p + 2;
+  b.clump = s.clump_id;
   if (kind == JDEM_COLLIDER_CELL_LIST) {
     Cells c;
     b.grid = grid_of(grid);
@@ -1677,12 +1678,12 @@ int jdem_potential_energy(jdem_ctx* ctx, jdem_system* sys, double* energy) {
     *energy = 0.0;
     return JDEM_OK;
   }
+  if (sys->state.clump_id) return clump_reduce(ctx, sys, 1, energy, false);
   if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST) {
     bool rebuilt = false;
     rc = nl_check_and_rebuild(ctx, sys, &rebuilt);
     if (rc) return rc;
   }
-  if (sys->state.clump_id) return clump_reduce(ctx, sys, 1, energy, false);
   return reduce_f64(ctx, sys, 1, energy, launch_energy, true);
 }
 
@@ -2448,8 +2449,6 @@ int check_system64(jdem_ctx* ctx, const jdem_system_f64* sys) {
   if (sys->law.n_mat > 1 && !s.mat_id) return fail(ctx, JDEM_EINVAL, "n_mat > 1 but mat_id is null");
   if (s.clump_id || s.pos_p) {
     if (s.n > 0 && (!s.clump_id || !s.pos_p)) return fail(ctx, JDEM_EINVAL, "clumps need both pos_p and clump_id");
-    if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST)
-      return fail(ctx, JDEM_EUNSUPPORTED, "clumps with the NeighborList collider");
     if (sys->domain.kind == JDEM_DOMAIN_REFLECT)
       return fail(ctx, JDEM_EUNSUPPORTED, "clumps in a reflective domain");
   }
@@ -2550,6 +2549,7 @@ int build_nlist64(jdem_ctx* ctx, const jdem_system_f64* sys, int kind, const jde
   b.dom = dom_of(sys->domain);
   b.nl = nl_out;
   b.overflow = (int32_t*)ctx->flags.p + 2;
+  b.clump = s.clump_id;
   if (kind == JDEM_COLLIDER_CELL_LIST) {
     Cells c;
     b.grid = grid_of(grid);
@@ -2823,11 +2823,11 @@ int jdem_compute_force_f64(jdem_ctx* ctx, jdem_system_f64* sys) {
     HIPCHK(ctx, hipMemsetAsync(s.torque, 0, 8 * (size_t)s.n * (s.dim == 3 ? 3 : 1), ctx->stream));
     return JDEM_OK;
   }
+  if (s.clump_id) return clump_force(ctx, sys);
   if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST) {
     rc = nl_check_and_rebuild64(ctx, sys);
     if (rc) return rc;
   }
-  if (s.clump_id) return clump_force(ctx, sys);
   StepArgs64 a = base_args64(ctx, sys);
   rc = prepare_source64(ctx, sys, a, s.pos);
   if (rc) return rc;
@@ -2846,11 +2846,11 @@ int jdem_potential_energy_f64(jdem_ctx* ctx, jdem_system_f64* sys, double* energ
     *energy = 0.0;
     return JDEM_OK;
   }
+  if (sys->state.clump_id) return clump_reduce(ctx, sys, 1, energy, false);
   if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST) {
     rc = nl_check_and_rebuild64(ctx, sys);
     if (rc) return rc;
   }
-  if (sys->state.clump_id) return clump_reduce(ctx, sys, 1, energy, false);
   return reduce64(ctx, sys, 1, energy, false);
 }
 
@@ -2937,6 +2937,45 @@ int clump_members(jdem_ctx* ctx, const Sys* sys, StepArgsT<Real>& a) {
   return JDEM_OK;
 }
 
+int rebuild_of(jdem_ctx* ctx, jdem_system* sys, const float* pos) { return nl_rebuild(ctx, sys, pos); }
+int rebuild_of(jdem_ctx* ctx, jdem_system_f64* sys, const double* pos) { return nl_rebuild64(ctx, sys, pos); }
+float trigger_of(const jdem_nlist& L) { return nl_trigger(L); }
+double trigger_of(const jdem_nlist_f64& L) { return nl_trigger64(L); }
+
+// _check_and_rebuild on the members' lab positions (neighbor_list.py:58-131
+// compares state.pos, the lab positions, with the build positions).
+template <class Real, class Sys>
+int clump_nl_check(jdem_ctx* ctx, Sys* sys, const Real* plab, bool* rebuilt) {
+  auto& L = sys->nlist;
+  const auto& s = sys->state;
+  int rc = ensure_flags(ctx);
+  if (rc) return rc;
+  bool need = L.n_build_times == 0;
+  if (!need) {
+    HIPCHK(ctx, launch_fill_i32((int32_t*)ctx->flags.p + 3, 1, 0, ctx->stream));
+    hipLaunchKernelGGL(k_disp_check<Real>, dim3(nblocks(s.n)), dim3(kBlock), 0, ctx->stream, plab, L.old_pos, s.n,
+                       s.dim, trigger_of(L), (int32_t*)ctx->flags.p + 3);
+    HIPCHK(ctx, hipGetLastError());
+    rc = read_flags(ctx, 4);
+    if (rc) return rc;
+    need = ctx->host_flags[3] != 0;
+  }
+  if (need) {
+    rc = rebuild_of(ctx, sys, plab);
+    if (rc) return rc;
+    *rebuilt = true;
+  }
+  return JDEM_OK;
+}
+
+template <class Sys>
+int clump_nl_overflow(jdem_ctx* ctx, Sys* sys) {
+  int rc = read_flags(ctx, 3);
+  if (rc) return rc;
+  sys->nlist.overflow = ctx->host_flags[2] != 0;
+  return JDEM_OK;
+}
+
 // Pair forces of the members at their lab positions (a from clump_lab).
 template <class Real, class Sys>
 int clump_pair_forces(jdem_ctx* ctx, Sys* sys, const StepArgsT<Real>& a0) {
@@ -3005,8 +3044,20 @@ int clump_step_t(jdem_ctx* ctx, Sys* sys, int64_t n, jdem_step_stats* S) {
   }
   rc = integrate(false, true);  // domain.apply + before-force of step 1
   if (rc) return rc;
+  const bool is_nl = sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST;
+  bool rebuilt = false;
   for (int64_t t = 1; t <= n; ++t) {
     HIPCHK(ctx, launch_clump_lab(a0, D, ctx->stream));
+    if (is_nl) {
+      bool r2 = false;
+      rc = clump_nl_check(ctx, sys, a0.plab, &r2);
+      if (rc) return rc;
+      S->host_syncs++;
+      if (r2) {
+        S->rebuilds++;
+        rebuilt = true;
+      }
+    }
     rc = clump_pair_forces(ctx, sys, a0);
     if (rc) return rc;
     HIPCHK(ctx, launch_clump_forces(a0, D, ctx->stream));
@@ -3016,6 +3067,13 @@ int clump_step_t(jdem_ctx* ctx, Sys* sys, int64_t n, jdem_step_stats* S) {
     }
     rc = integrate(true, t < n);
     if (rc) return rc;
+  }
+  if (is_nl) {
+    if (rebuilt) {
+      rc = clump_nl_overflow(ctx, sys);  // of the most recent build
+      if (rc) return rc;
+    }
+    S->overflow = sys->nlist.overflow;
   }
   if (free_done) {
     for (int a = 0; a < D; ++a) {
@@ -3035,6 +3093,15 @@ int clump_force_t(jdem_ctx* ctx, Sys* sys) {
   StepArgsT<Real> a = args_of(ctx, sys);
   int rc = clump_lab(ctx, sys, a);
   if (rc) return rc;
+  if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST) {
+    bool rebuilt = false;
+    rc = clump_nl_check(ctx, sys, a.plab, &rebuilt);
+    if (rc) return rc;
+    if (rebuilt) {
+      rc = clump_nl_overflow(ctx, sys);
+      if (rc) return rc;
+    }
+  }
   return clump_pair_forces(ctx, sys, a);
 }
 
@@ -3062,6 +3129,14 @@ int clump_reduce_t(jdem_ctx* ctx, Sys* sys, int ncomp, double* host_out, bool ki
     if (sys->collider_kind == JDEM_COLLIDER_CELL_LIST) {
       rc = source_of(ctx, sys, a, a.plab);
       if (rc) return rc;
+    } else if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST) {
+      bool rebuilt = false;
+      rc = clump_nl_check(ctx, sys, a.plab, &rebuilt);
+      if (rc) return rc;
+      if (rebuilt) {
+        rc = clump_nl_overflow(ctx, sys);
+        if (rc) return rc;
+      }
     }
     HIPCHK(ctx, launch_energy(a, s.dim, a.law.kind, src_of(sys->collider_kind), ctx->stream));
   }

@@ -805,7 +805,7 @@ __global__ void __launch_bounds__(kBlock) k_nlist_cells(NLBuildT<Real> b) {
         const int k0 = b.cstart[h], k1 = b.cend[h];
         for (int k = k0; k < k1; ++k) {
           const int32_t j = b.perm[k];
-          if (j == i) continue;
+          if (b.clump ? b.clump[j] == b.clump[i] : j == i) continue;
           Vec<D, Real> xj;
           if constexpr (kF32)
             xj = v4<D>(b.spos[k]);
@@ -834,7 +834,7 @@ __global__ void __launch_bounds__(kBlock) k_nlist_naive(NLBuildT<Real> b) {
   int count = 0;
   const bool chk_bond = (!b.isb) && b.bond_w > 0;
   for (int64_t j = b.n - 1; j >= 0; --j) {
-    if (j == i) continue;
+    if (b.clump ? b.clump[j] == b.clump[i] : j == i) continue;
     const Vec<D, Real> d = disp_<PER>(xi, ld<D>(b.pos, j), b.dom);
     if (!(dot(d, d) <= b.cutoff_sq)) continue;
     if (chk_bond && bonded(b.bond_id, b.bond_w, i, (int32_t)j)) continue;

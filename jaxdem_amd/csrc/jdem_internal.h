@@ -302,6 +302,7 @@ struct NLBuildT {
   DomPT<Real> dom;
   int32_t* nl;
   int32_t* overflow;
+  const int32_t* clump;  // clump ids (pairs of one clump are not listed), or null
 };
 using NLBuild = NLBuildT<float>;
 hipError_t launch_nlist_cells(const NLBuild& b, hipStream_t s);

@@ -253,6 +253,8 @@ class System:
             and state.N > 0
             and isinstance(system.collider, NeighborList)
             and not isinstance(system.domain, FreeDomain)
+            and state.is_sphere_state
+            and state.pos_c.dtype == torch.float32
             and _fusable(state, system)
         ):
             return _fused_rollout(state, system, ks)
@@ -642,7 +644,7 @@ def _fused_step(state: State, system: System, n: int) -> tuple:
         out_sys = _advance_clock(out_sys, n)
         out_sys.__dict__["_last_stats"] = stats
         return new_state, out_sys
-    if isinstance(system.collider, NeighborList):
+    if isinstance(system.collider, NeighborList) and new_state.is_sphere_state:
         sys2 = replace(system, domain=replace(system.domain))
         S, slot, col, hist = _resident_call(new_state, sys2, keep)
         ctx.check(ctx.lib.jdem_system_step(ctx.ptr, ctypes.byref(S), int(n), ctypes.byref(stats)), "jdem_system_step")

@@ -1007,21 +1007,30 @@ def compute_force(st: State, P: Params, nls: NLState | None = None):
 def _compute_force_clumps(st: State, P: Params, nls: NLState | None):
     """Clump members: forces at the lab positions (state.pos), same-clump pairs
     masked, then ``torque += pos_p_rot x force`` per member (naive.py:210-233,
-    cell_list.py:461-462)."""
+    cell_list.py:461-462, neighbor_list.py:624-648)."""
     r = pos_p_rot(st)
     lab = st.copy()
     lab.pos = st.pos + r
-    if P.collider == "naive":
-        if st.N > 1024 or P.naive_by_cells:
-            C = naive_near_candidates(lab, P)
-        else:
-            C = drop_far_candidates(lab, P, naive_candidates(st.N))
-    elif P.collider == "celllist":
-        C, _ = cell_candidates(lab.pos, P.domain, P.cell_size, P.search_range)
-        C = drop_far_candidates(lab, P, C)
+    if P.collider == "neighborlist":
+        # the list is checked and built on the lab positions, with the clump
+        # mask (neighbor_list.py:58-131, 594-669)
+        nl_check_and_rebuild(lab, P, nls)
+        hist = (nls.xi, nls.live) if nls.xi is not None else None
+        F, T, h = accumulate(lab, P, nls.nl, hist)
+        if h is not None:
+            nls.xi, nls.live = h
     else:
-        raise NotImplementedError(f"clumps with the {P.collider} collider are not restated")
-    F, T, _ = accumulate(lab, P, C)
+        if P.collider == "naive":
+            if st.N > 1024 or P.naive_by_cells:
+                C = naive_near_candidates(lab, P)
+            else:
+                C = drop_far_candidates(lab, P, naive_candidates(st.N))
+        elif P.collider == "celllist":
+            C, _ = cell_candidates(lab.pos, P.domain, P.cell_size, P.search_range)
+            C = drop_far_candidates(lab, P, C)
+        else:
+            raise ValueError(P.collider)
+        F, T, _ = accumulate(lab, P, C)
     lever = cross3(r, F) if st.dim == 3 else cross2(r, F)
     st.force, st.torque = F, T + lever
 

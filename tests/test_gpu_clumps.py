@@ -2,8 +2,9 @@
 seeded inputs: dimers, trimers and single spheres with random orientations
 and spins; forces and torques bit for bit (naive and CellList, 2D and 3D),
 whole steps of every integrator pair with gravity (periodic and free
-domains), potential and kinetic energy to 1e-12, the float64 form within
-1e-12 of the float64 oracle; the combinations off the GPU path refuse."""
+domains), NeighborList runs with rebuilds on the lab positions, potential and
+kinetic energy to 1e-12, the float64 form within 1e-12 of the float64
+oracle; the combinations off the GPU path refuse."""
 
 from __future__ import annotations
 
@@ -161,15 +162,77 @@ def test_clump_f64(device):
         assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max(), f
 
 
+def _nl_clump_system(st, box, secondary, law="cundallstrack", skin=0.05, g=None):
+    mats = [jd.Material.create("elasticfrict", density=1.0, young=1e4, poisson=0.3, mu=0.4, e=0.7, mu_r=0.05)]
+    skw = dict(state=st) if secondary == "CellList" else {}
+    return jd.System.create(
+        st.shape, dt=1e-4, domain_type="periodic", domain_kw=dict(box_size=box, anchor=np.zeros(st.dim)),
+        collider_type="NeighborList", force_model_type=law, mat_table=jd.MaterialTable.from_materials(mats),
+        collider_kw=dict(state=st, cutoff=float(2 * st.rad.max().item()), skin=skin,
+                         secondary_collider_type=secondary, secondary_collider_kw=skw),
+        force_manager_kw={"gravity": g} if g is not None else None)
+
+
+@pytest.mark.parametrize("secondary", ["CellList", "naive"])
+@pytest.mark.parametrize("dim", [3, 2])
+def test_clump_neighborlist_bitwise(device, dim, secondary):
+    """60 steps on a NeighborList built and checked on the members' lab
+    positions (pairs of one clump unlisted), with rebuilds: state, list rows,
+    build positions and build count bit for bit."""
+    print(f"[clumps] neighborlist {secondary} dim={dim}", flush=True)
+    st, box = clump_scene(6 if dim == 3 else 14, dim, seed=21)
+    sy = _nl_clump_system(st, box, secondary, g=np.array([0.0, -1.0, 0.0][:dim]))
+    n = 60
+    st2, sy2 = jd.System.step(st, sy, n=n)
+    ost = B.state_of(st)
+    P = B.params_of(st, sy)
+    nls = B.nl_state_of(st, sy)
+    ost = O.step(ost, P, n, nls)
+    for f, of in (("pos_c", "pos"), ("vel", "vel"), ("ang_vel", "ang_vel"), ("force", "force"),
+                  ("torque", "torque")):
+        assert_bitwise(f, getattr(st2, f).cpu().numpy(), getattr(ost, of))
+    assert sy2.collider.n_build_times == nls.n_build and nls.n_build >= 2
+    assert_bitwise("nl", sy2.collider.neighbor_list.cpu().numpy().astype(np.int64), nls.nl)
+    assert_bitwise("old_pos", sy2.collider.old_pos.cpu().numpy(), nls.old_pos)
+    # no row lists a member of the particle's own clump
+    nl = sy2.collider.neighbor_list.cpu().numpy()
+    cid = st.clump_id.cpu().numpy()
+    i, k = np.nonzero(nl >= 0)
+    assert not np.any(cid[i] == cid[nl[i, k]])
+
+
+def test_clump_neighborlist_f64(device):
+    print("[clumps] neighborlist float64", flush=True)
+    old = jd.config.jax_enable_x64
+    jd.config.update("jax_enable_x64", True)
+    try:
+        st, box = clump_scene(6, 3, seed=22, dtype=np.float64)
+        sy = _nl_clump_system(st, box, "CellList", law="hertz")
+        st2, sy2 = jd.System.step(st, sy, n=40)
+        _, _, frames = jd.System.trajectory_rollout(st, sy, n=2, stride=20)
+    finally:
+        jd.config.update("jax_enable_x64", old)
+    nls = B.nl_state_of(st, sy, np.float64)
+    ost = O.step(B.state_of(st, np.float64), B.params_of(st, sy, np.float64), 40, nls)
+    for f, of in (("pos_c", "pos"), ("vel", "vel"), ("ang_vel", "ang_vel"), ("force", "force")):
+        a = getattr(st2, f).cpu().numpy()
+        b = getattr(ost, of)
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max(), f
+    assert sy2.collider.n_build_times == nls.n_build
+    # a rollout with save_state frames (two segments of 20) ends where 40 steps do
+    states, _ = frames
+    assert torch.equal(states.pos_c[-1], st2.pos_c)
+
+
 def test_clump_unsupported_combinations_raise(device):
     st, box = clump_scene(4, 3, seed=1)
     with pytest.raises(jd._lib.JdemError, match="reflective"):
         jd.System.step(st, clump_system(st, box, "spring", "naive", "reflect"), n=1)
-    nl = jd.System.create(st.shape, dt=1e-4, domain_type="periodic", domain_kw=dict(box_size=box),
-                          collider_type="NeighborList",
-                          collider_kw=dict(state=st, cutoff=1.0, skin=0.1, secondary_collider_type="naive"))
-    with pytest.raises(jd._lib.JdemError, match="NeighborList"):
-        jd.System.step(st, nl, n=1)
+    mats = jd.MaterialTable.from_materials([jd.Material.create("elastic", density=1.0, young=1e4, poisson=0.3)])
+    sy = jd.System.create(st.shape, dt=1e-4, domain_type="periodic", domain_kw=dict(box_size=box),
+                          collider_type="naive", mat_table=mats)
+    with pytest.raises(NotImplementedError, match="spheres only"):
+        jd.minimize(st, sy, max_steps=2)
 
 
 # ---------------------------------------------------------------------------
@@ -212,10 +275,15 @@ def set_up_clump(dim, N=350, collider="naive", domain="periodic", seed=42, dt=1e
                          vel=np.repeat(vel_c, nv, 0), rad=np.full(n * nv, r_a), mass=np.ones(n * nv),
                          inertia=np.full((n * nv, A), I), clump_id=np.repeat(np.arange(n), nv), device=device)
     mt = jd.MaterialTable.from_materials([jd.Material.create("elastic", density=0.27, young=1.0e3, poisson=0.3)])
-    ckw = dict(state=st) if collider == "CellList" else {}
+    ckw = dict(state=st) if collider.endswith("CellList") else {}
+    ctype = collider
+    if collider.startswith("nl_"):
+        ckw = dict(state=st, cutoff=float(2 * r_a), skin=0.5, secondary_collider_type=collider[3:],
+                   secondary_collider_kw=ckw)
+        ctype = "NeighborList"
     sy = jd.System.create(st.shape, domain_type=domain,
                           domain_kw=dict(box_size=np.full(dim, spacing * per), anchor=np.full(dim, -R_p - r_a)),
-                          collider_type=collider, collider_kw=ckw, dt=dt, mat_table=mt)
+                          collider_type=ctype, collider_kw=ckw, dt=dt, mat_table=mt)
     return st, sy
 
 
@@ -230,14 +298,17 @@ def precision(name):
 
 
 @pytest.mark.parametrize("prec", ["float32", "float64"])
+@pytest.mark.parametrize("collider,domain", [("CellList", "periodic"), ("nl_naive", "periodic"),
+                                             ("nl_CellList", "periodic"), ("nl_naive", "free")])
 @pytest.mark.parametrize("dim", [2, 3])
-def test_clumps_invariance(device, dim, prec):
-    """naive vs CellList over 200 steps, atol = rtol = 5e-2 (periodic: the
-    domains the CellList takes with clumps)."""
-    print(f"[clumps] invariance dim={dim} {prec}", flush=True)
+def test_clumps_invariance(device, dim, collider, domain, prec):
+    """naive vs CellList and NeighborList over 200 steps, atol = rtol = 5e-2
+    (test_colliders_invariance.py:405-434; the domains each collider takes
+    with clumps)."""
+    print(f"[clumps] invariance dim={dim} {collider} {domain} {prec}", flush=True)
     with precision(prec):
-        s1, y1 = set_up_clump(dim, N=128, collider="naive")
-        s2, y2 = set_up_clump(dim, N=128, collider="CellList")
+        s1, y1 = set_up_clump(dim, N=128, collider="naive", domain=domain)
+        s2, y2 = set_up_clump(dim, N=128, collider=collider, domain=domain)
         s1, y1 = jd.System.step(s1, y1, n=200)
         s2, y2 = jd.System.step(s2, y2, n=200)
         pe1 = jd.utils.compute_potential_energy(s1, y1)
