@@ -273,14 +273,11 @@ JD QuatT<Real> q_unit(QuatT<Real> q) {
   Real inv = rsq(safe);
   return {q.w * inv, q.x * inv, q.y * inv, q.z * inv};
 }
-// quaternion.py:150-180
+// Quaternion.conj (quaternion.py:143-162): (w, -xyz), what rotate_back
+// applies (quaternion.py:267-268)
 template <class Real>
 JD QuatT<Real> q_conj(QuatT<Real> q) {
-  Vec<3, Real> v = qv(q);
-  Real n2 = q.w * q.w + dot(v, v);
-  Real safe = (n2 == Real(0)) ? Real(1) : n2;
-  Real inv_n2 = Real(1) / safe;
-  return {q.w * inv_n2, (-q.x) * inv_n2, (-q.y) * inv_n2, (-q.z) * inv_n2};
+  return {q.w, -q.x, -q.y, -q.z};
 }
 // quaternion.py:193-242 (3D)
 template <class Real>

@@ -298,8 +298,13 @@ __global__ void __launch_bounds__(kBlock) k_prec_force(PrecArgs a) {
           const float osj[3] = {sj.x, sj.y, sj.z};
           // r = rh + rl, both float32: two_sum of the cell term (exact, the
           // stencil offset is -1, 0 or 1) and the offset difference, whose
-          // own rounding error two_sum recovers too
+          // own rounding error two_sum recovers too.  The contact test reads
+          // rh only, so the low parts are formed for contacts alone.
           float r[3] = {0, 0, 0}, rl[3] = {0, 0, 0};
+          for (int q = 0; q < D; ++q) r[q] = (float)(-o[q]) * a.h[q] + (oi[q] - osj[q]);
+          const float d2 = D == 3 ? (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
+          const float R = ri + sj.w;
+          if (!(d2 < R * R) || d2 == 0.0f) continue;
           for (int q = 0; q < D; ++q) {
             const float oj = osj[q];
             const float b = oi[q] - oj;
@@ -309,12 +314,8 @@ __global__ void __launch_bounds__(kBlock) k_prec_force(PrecArgs a) {
             const float t = c + b;
             const float tv = t - c;
             const float te = (c - (t - tv)) + (b - tv);
-            r[q] = t;
             rl[q] = (te + be) + (float)(-o[q]) * a.hlo[q];
           }
-          const float d2 = D == 3 ? (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
-          const float R = ri + sj.w;
-          if (!(d2 < R * R) || d2 == 0.0f) continue;
           const float inv = 1.0f / sqrtf(d2);
           // delta = (R^2 - |r|^2) / (R + |r|) with |r|^2 = sum rh^2 + 2 rh rl
           // and the squares split exactly into hi + lo with fma, so the

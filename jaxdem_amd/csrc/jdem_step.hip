@@ -445,9 +445,8 @@ __global__ void __launch_bounds__(kBlock) k_kinetic(StepArgsT<Real> a) {
   if constexpr (D == 3) {
     // body-frame angular velocity in float64
     const double qw = a.q[4 * i], qx = a.q[4 * i + 1], qy = a.q[4 * i + 2], qz = a.q[4 * i + 3];
-    const double n2 = qw * qw + ((qx * qx + qy * qy) + qz * qz);
-    const double inv = 1.0 / (n2 == 0.0 ? 1.0 : n2);
-    const double cw = qw * inv, cx = -qx * inv, cy = -qy * inv, cz = -qz * inv;
+    // rotate_back: the rotation by the conjugate (quaternion.py:267-268)
+    const double cw = qw, cx = -qx, cy = -qy, cz = -qz;
     const double wx = a.ang_in[3 * i], wy = a.ang_in[3 * i + 1], wz = a.ang_in[3 * i + 2];
     const double tx = 2.0 * (cy * wz - cz * wy), ty = 2.0 * (cz * wx - cx * wz), tz = 2.0 * (cx * wy - cy * wx);
     const double bx = (wx + cw * tx) + (cy * tz - cz * ty);

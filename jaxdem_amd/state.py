@@ -81,11 +81,8 @@ class Quaternion:
 
     @staticmethod
     def rotate_back(q: "Quaternion", v: torch.Tensor) -> torch.Tensor:
-        """Lab -> body rotation (quaternion.py:247-268)."""
-        n2 = q.w[..., 0] ** 2 + (q.xyz * q.xyz).sum(-1)
-        n2 = torch.where(n2 == 0, torch.ones_like(n2), n2)
-        inv = (1.0 / n2)[..., None]
-        return Quaternion.rotate(Quaternion(torch.cat([q.w * inv, -q.xyz * inv], dim=-1)), v)
+        """Lab -> body rotation: rotate by the conjugate (quaternion.py:247-268)."""
+        return Quaternion.rotate(Quaternion(torch.cat([q.w, -q.xyz], dim=-1)), v)
 
     def __repr__(self) -> str:
         return f"Quaternion(shape={tuple(self.data.shape)})"

@@ -118,13 +118,10 @@ def q_unit(q: np.ndarray) -> np.ndarray:
 
 
 def q_conj(q: np.ndarray) -> np.ndarray:
-    """``Quaternion.conj`` (quaternion.py:150-180): inverse, w/n2, -xyz/n2."""
-    n2 = q[..., 0] * q[..., 0] + dot(q[..., 1:], q[..., 1:])
-    safe = np.where(n2 == 0.0, np.asarray(1.0, q.dtype), n2)
-    inv_n2 = np.asarray(1.0, q.dtype) / safe
-    w = q[..., 0] * inv_n2
-    xyz = (-q[..., 1:]) * inv_n2[..., None]
-    return np.concatenate([w[..., None], xyz], axis=-1)
+    """``Quaternion.conj`` (quaternion.py:143-162): (w, -xyz), what
+    ``rotate_back`` applies (quaternion.py:267-268); not ``inv``, which
+    divides by |q|^2 (quaternion.py:164-187)."""
+    return np.concatenate([q[..., 0:1], -q[..., 1:]], axis=-1)
 
 
 def q_rotate(q: np.ndarray, v: np.ndarray) -> np.ndarray:

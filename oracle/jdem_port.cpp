@@ -75,12 +75,8 @@ inline Q qunit(Q q) {
   float inv = 1.0f / std::sqrt(s);
   return {q.w * inv, q.x * inv, q.y * inv, q.z * inv};
 }
-inline Q qconj(Q q) {
-  float n2 = q.w * q.w + dot(qv(q), qv(q));
-  float s = n2 == 0.0f ? 1.0f : n2;
-  float inv = 1.0f / s;
-  return {q.w * inv, (-q.x) * inv, (-q.y) * inv, (-q.z) * inv};
-}
+// Quaternion.conj (quaternion.py:143-162), applied by rotate_back
+inline Q qconj(Q q) { return {q.w, -q.x, -q.y, -q.z}; }
 inline V3 qrot(Q q, V3 v) {
   V3 t = mul(cross(qv(q), v), 2.0f);
   return add(add(v, mul(t, q.w)), cross(qv(q), t));
