@@ -111,7 +111,7 @@ typedef struct jdem_state {
    * forces act at pos + R(q) pos_p, and ForceManager sums them over the
    * clump (force_manager.py:359-412).  Colliders NAIVE, CELL_LIST and
    * NEIGHBOR_LIST (on the caller's rows, checked and built on the lab
-   * positions), FREE and PERIODIC domains. */
+   * positions); every domain (reflect.py:160-297 for REFLECT). */
   const float* pos_p;
   const int32_t* clump_id;
 } jdem_state;

@@ -22,7 +22,7 @@ std::vector<Buf*> jdem_ctx::bufs() {
                            &cstart,  &cend,   &spos,     &nl_new,  &xi_new,  &live_new, &flags,  &red,
                            &nl_copy, &xi_copy, &live_copy, &wF,     &wT,      &wOld,     &wWpos, &wSrc,
                            &wInv,    &wKeys,  &fireV,    &fireDev, &segp,  &segid, &segflag, &wXiRank,
-                           &route,   &ccnt,   &cl_start, &cl_members, &plab, &ppr, &ftot, &ttot};
+                           &route,   &ccnt,   &cl_start, &cl_members, &plab, &ppr, &ftot, &ttot, &crefl};
   for (int k = 0; k < 2; ++k)
     for (Buf* b : {&wP[k], &wV[k], &wW[k], &wQ[k], &wI[k], &wOrig[k], &wNl[k], &wXi[k], &wLive[k]}) all.push_back(b);
   return all;
@@ -296,8 +296,6 @@ int check_system(jdem_ctx* ctx, const jdem_system* sys, bool batch = false, bool
     if (batch) return fail(ctx, JDEM_EUNSUPPORTED, "batches with clumps");
     if (sys->collider_kind == JDEM_COLLIDER_NEIGHBOR_LIST && (sys->nlist.resident_in || sys->nlist.resident_out))
       return fail(ctx, JDEM_EUNSUPPORTED, "clumps with a resident NeighborList working set");
-    if (sys->domain.kind == JDEM_DOMAIN_REFLECT)
-      return fail(ctx, JDEM_EUNSUPPORTED, "clumps in a reflective domain");
   }
   const int lk = sys->law.kind;
   if (sys->collider_kind != JDEM_COLLIDER_NONE &&
@@ -2449,8 +2447,6 @@ int check_system64(jdem_ctx* ctx, const jdem_system_f64* sys) {
   if (sys->law.n_mat > 1 && !s.mat_id) return fail(ctx, JDEM_EINVAL, "n_mat > 1 but mat_id is null");
   if (s.clump_id || s.pos_p) {
     if (s.n > 0 && (!s.clump_id || !s.pos_p)) return fail(ctx, JDEM_EINVAL, "clumps need both pos_p and clump_id");
-    if (sys->domain.kind == JDEM_DOMAIN_REFLECT)
-      return fail(ctx, JDEM_EUNSUPPORTED, "clumps in a reflective domain");
   }
   const int ck = sys->collider_kind;
   if (ck != JDEM_COLLIDER_NONE && ck != JDEM_COLLIDER_NAIVE && ck != JDEM_COLLIDER_CELL_LIST &&
@@ -3013,11 +3009,38 @@ int clump_step_t(jdem_ctx* ctx, Sys* sys, int64_t n, jdem_step_stats* S) {
   a0.pos_p = s.pos_p;
   const bool none = sys->collider_kind == JDEM_COLLIDER_NONE;
   const bool free_dom = sys->domain.kind == JDEM_DOMAIN_FREE;
+  const bool reflect = sys->domain.kind == JDEM_DOMAIN_REFLECT;
+  if (reflect) {
+    ENSURE(ctx, ctx->crefl, sizeof(Real) * (size_t)s.n * refl_scratch_words(D));
+    a0.rscr = (Real*)ctx->crefl.p;
+  }
   bool free_done = false;
   Real fbox[3] = {0, 0, 0}, fanc[3] = {0, 0, 0};
-  // the integrators on the clump state, in place (no pair reads)
+  // the integrators on the clump state, in place (no pair reads); with a
+  // reflective domain, domain.apply runs as the clump kernels before the
+  // before-force half (the sphere form in the step kernel is skipped)
   auto integrate = [&](bool after, bool before) -> int {
+    if (reflect && before) {
+      if (after) {
+        int r2 = [&]() -> int {
+          StepArgsT<Real> a = a0;
+          a.do_force = 0;
+          a.do_after = 1;
+          a.do_before = 0;
+          a.write_kin = 1;
+          a.has_fm = 0;
+          a.ppr = nullptr;
+          HIPCHK(ctx, launch_step(a, D, none ? JDEM_LAW_SPRING : a.law.kind,
+                                  none ? SRC_NAIVE : src_of(sys->collider_kind), ctx->stream));
+          return JDEM_OK;
+        }();
+        if (r2) return r2;
+        after = false;
+      }
+      HIPCHK(ctx, launch_clump_reflect(a0, D, ctx->stream));
+    }
     StepArgsT<Real> a = a0;
+    if (reflect) a.dom.kind = JDEM_DOMAIN_FREE;
     a.do_force = 0;
     a.do_after = after;
     a.do_before = before;

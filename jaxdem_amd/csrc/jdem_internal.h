@@ -80,7 +80,7 @@ struct jdem_ctx {
   // per-cell counts / starts of the batch rebuild's counting sort
   jdem::Buf ccnt;
   // clumps: members per clump (CSR), lab positions, levers, member totals
-  jdem::Buf cl_start, cl_members, plab, ppr, ftot, ttot;
+  jdem::Buf cl_start, cl_members, plab, ppr, ftot, ttot, crefl;
   // every scratch buffer of the context, for accounting and release
   std::vector<jdem::Buf*> bufs();
   size_t scratch_bytes();
@@ -189,6 +189,7 @@ struct StepArgsT {
   Real* plab;
   Real* ftot;
   Real* ttot;
+  Real* rscr;  // reflective domain with clumps: per-member scratch (kReflW words)
 };
 using StepArgs = StepArgsT<float>;
 // The float64 form of the plain step (jdem_system_step_f64 and friends).
@@ -213,6 +214,11 @@ hipError_t launch_clump_lab(const StepArgs& a, int dim, hipStream_t s);
 hipError_t launch_clump_lab(const StepArgs64& a, int dim, hipStream_t s);
 hipError_t launch_clump_forces(const StepArgs& a, int dim, hipStream_t s);
 hipError_t launch_clump_forces(const StepArgs64& a, int dim, hipStream_t s);
+// ReflectDomain.apply for clumps (four launches; scratch a.rscr of
+// refl_scratch_words(dim) words per member)
+hipError_t launch_clump_reflect(const StepArgs& a, int dim, hipStream_t s);
+hipError_t launch_clump_reflect(const StepArgs64& a, int dim, hipStream_t s);
+inline int refl_scratch_words(int dim) { return 8 * dim + 2 + (dim == 3 ? 3 : 1); }
 
 // launchers (jdem_cells.hip)
 hipError_t launch_hash(const float* pos, int64_t n, int dim, const DomP& dom, const GridP& g, uint32_t* keys,

@@ -589,6 +589,259 @@ __global__ void __launch_bounds__(kBlock) k_clump_sum(StepArgsT<Real> a) {
   sta(a.torque, i, T);
 }
 
+// ReflectDomain.apply for clumps (reflect.py:160-297).  Per member: its
+// overlap past each wall at its lab position; the deepest member of its clump
+// per axis takes the impulse with the effective mass of the rigid body; the
+// clump's time-of-collision fraction is its members' minimum; the members'
+// impulses are summed over the clump into the linear and angular velocity
+// changes, and position and orientation are corrected over the remaining
+// time.  Matrix products sum in index order.  Scratch per member (rscr):
+// [0, D) over_lo, [D, 2D) over_hi, [2D, 3D) wall sign, [3D, 4D) active,
+// [4D, 5D) denominator, [5D, 6D) contact velocity, [6D, 7D) contact
+// acceleration, 7D alpha minimum, 7D + 1 clump alpha, [7D + 2, 8D + 2)
+// impulse / mass, then the angular velocity change (A words).
+template <int D, class Real>
+JD Real* refl_row(const StepArgsT<Real>& a, int64_t i) {
+  return a.rscr + (int64_t)(8 * D + 2 + (D == 3 ? 3 : 1)) * i;
+}
+template <class Real>
+JD void to_arr_a(Vec<3, Real> v, Real* x) {
+  to_arr(v, x);
+}
+template <class Real>
+JD void to_arr_a(Ang1T<Real> v, Real* x) {
+  x[0] = v.z;
+}
+// rows of R (body -> lab): row k = R^T e_k = rotate_back(q, e_k)
+template <int D, class Real>
+JD void rot_rows(QuatT<Real> q, Real R[D][D]) {
+#pragma unroll
+  for (int k = 0; k < D; ++k) {
+    Real e[D];
+#pragma unroll
+    for (int c = 0; c < D; ++c) e[c] = (c == k) ? Real(1) : Real(0);
+    Vec<D, Real> ek;
+    from_arr(ek, e);
+    Vec<D, Real> rk;
+    if constexpr (D == 3)
+      rk = q_rotate_back(q, ek);
+    else
+      rk = q_rotate(q_conj(q), ek);
+    to_arr(rk, R[k]);
+  }
+}
+// R^T v (T = true) or R v, summing over j in index order
+template <int D, bool T, class Real>
+JD void rmul(const Real R[D][D], const Real* v, Real* out) {
+#pragma unroll
+  for (int i = 0; i < D; ++i) {
+    Real acc = (T ? R[0][i] : R[i][0]) * v[0];
+#pragma unroll
+    for (int j = 1; j < D; ++j) acc = acc + (T ? R[j][i] : R[i][j]) * v[j];
+    out[i] = acc;
+  }
+}
+
+template <int D, class Real>
+__global__ void __launch_bounds__(kBlock) k_crefl_over(StepArgsT<Real> a) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const Vec<D, Real> r = q_rotate(ldq(a.q, i), ld<D>(a.pos_p, i));
+  Real pos[D];
+  to_arr(ld<D>(a.pos_in, i) + r, pos);
+  const Real rad = a.rad[i];
+  Real* s = refl_row<D>(a, i);
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    const Real lo = a.dom.anchor[c] + rad;
+    const Real hi = (a.dom.anchor[c] + a.dom.box[c]) - rad;
+    s[c] = fmax_(Real(0), lo - pos[c]);
+    s[D + c] = fmax_(Real(0), pos[c] - hi);
+  }
+}
+
+template <int D, class Real>
+__global__ void __launch_bounds__(kBlock) k_crefl_alpha(StepArgsT<Real> a) {
+  constexpr int A = D == 3 ? 3 : 1;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int c0 = a.clump[i];
+  Real mlo[D], mhi[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) mlo[c] = mhi[c] = -Real(INFINITY);
+  for (int k = a.cl_start[c0]; k < a.cl_start[c0 + 1]; ++k) {
+    const Real* sm = refl_row<D>(a, a.cl_members[k]);
+#pragma unroll
+    for (int c = 0; c < D; ++c) {
+      mlo[c] = fmax_(mlo[c], sm[c]);
+      mhi[c] = fmax_(mhi[c], sm[D + c]);
+    }
+  }
+  Real* s = refl_row<D>(a, i);
+  const QuatT<Real> q = ldq(a.q, i);
+  const Vec<D, Real> pp = ld<D>(a.pos_p, i);
+  const Vec<D, Real> r = q_rotate(q, pp);
+  const Real inv_mass = Real(1) / a.mass[i];
+  Real I[A], invI[A];
+  to_arr_a(lda<D>(a.inertia, i), I);
+#pragma unroll
+  for (int k = 0; k < A; ++k) invI[k] = Real(1) / I[k];
+  Real R[D][D];
+  rot_rows<D>(q, R);
+  Real delta[D], ws[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    delta[c] = fmax_(mlo[c], mhi[c]);
+    const bool dlo = (s[c] > Real(0)) && (s[c] == mlo[c]);
+    const bool dhi = (s[D + c] > Real(0)) && (s[D + c] == mhi[c]);
+    ws[c] = (dlo ? Real(1) : Real(0)) - (dhi ? Real(1) : Real(0));
+    s[2 * D + c] = ws[c];
+    s[3 * D + c] = fabs_(ws[c]);
+    // effective mass along axis c: 1/m + sum_k invI_k ((pos_p x n'_c)_k)^2
+    Vec<D, Real> nc;
+    from_arr(nc, R[c]);
+    Real x[A];
+    to_arr_a(cross_t(pp, nc), x);
+    Real drot = (x[0] * x[0]) * invI[0];
+#pragma unroll
+    for (int k = 1; k < A; ++k) drot = drot + (x[k] * x[k]) * invI[k];
+    Real den = inv_mass + drot;
+    s[4 * D + c] = (den == Real(0)) ? Real(1) : den;
+  }
+  const Vec<D, Real> vcs = ld<D>(a.vel_in, i) + cross_wr(lda<D>(a.ang_in, i), r);
+  const Vec<D, Real> acl = ld<D>(a.force, i) * inv_mass;
+  Ang<D, Real> arot;
+  if constexpr (D == 3) {
+    Real t[3], w[3], tb[3], wb[3];
+    to_arr(lda<3>(a.torque, i), t);
+    to_arr(lda<3>(a.ang_in, i), w);
+    rmul<3, true>(R, t, tb);
+    rmul<3, true>(R, w, wb);
+    Vec<3, Real> wbv, tbv, Iv;
+    from_arr(wbv, wb);
+    from_arr(tbv, tb);
+    from_arr(Iv, I);
+    const Vec<3, Real> ab = (tbv - cross(wbv, Iv * wbv)) * Vec<3, Real>{invI[0], invI[1], invI[2]};
+    Real abv[3], ar[3];
+    to_arr(ab, abv);
+    rmul<3, false>(R, abv, ar);
+    from_arr(arot, ar);
+  } else {
+    arot = lda<2>(a.torque, i) * Ang1T<Real>{invI[0]};
+  }
+  const Vec<D, Real> acc = acl + cross_wr(arot, r);
+  Real v[D], ac[D];
+  to_arr(vcs, v);
+  to_arr(acc, ac);
+  Real amin = Real(1);
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    s[5 * D + c] = v[c];
+    s[6 * D + c] = ac[c];
+    const Real al = toc_alpha(v[c], ac[c], delta[c], ws[c], a.dt);
+    if (fabs_(ws[c]) > Real(0)) amin = fmin_(amin, al);
+  }
+  s[7 * D] = amin;
+}
+
+template <int D, class Real>
+__global__ void __launch_bounds__(kBlock) k_crefl_impulse(StepArgsT<Real> a) {
+  constexpr int A = D == 3 ? 3 : 1;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int c0 = a.clump[i];
+  Real acl = Real(INFINITY), cnt[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) cnt[c] = Real(0);
+  for (int k = a.cl_start[c0]; k < a.cl_start[c0 + 1]; ++k) {
+    const Real* sm = refl_row<D>(a, a.cl_members[k]);
+    acl = fmin_(acl, sm[7 * D]);
+#pragma unroll
+    for (int c = 0; c < D; ++c) cnt[c] = cnt[c] + sm[3 * D + c];
+  }
+  Real* s = refl_row<D>(a, i);
+  s[7 * D + 1] = acl;
+  const QuatT<Real> q = ldq(a.q, i);
+  const Real inv_mass = Real(1) / a.mass[i];
+  Real I[A];
+  to_arr_a(lda<D>(a.inertia, i), I);
+  Real R[D][D];
+  rot_rows<D>(q, R);
+  const Real e = a.dom.restitution;
+  const Real dt_factor = (acl - Real(1)) * a.dt;
+  Real j[D];
+#pragma unroll
+  for (int c = 0; c < D; ++c) {
+    const Real act = s[3 * D + c];
+    const Real weight = act / fmax_(cnt[c], Real(1));
+    const Real vc = s[5 * D + c] + dt_factor * s[6 * D + c];
+    Real jj = ((-(Real(1) + e)) * vc) / s[4 * D + c];
+    const Real closing = ((vc * s[2 * D + c]) < Real(0)) ? Real(1) : Real(0);
+    jj = jj * (weight * closing);
+    j[c] = jj;
+    s[7 * D + 2 + c] = jj * inv_mass;
+  }
+  Real jb[D];
+  rmul<D, true>(R, j, jb);
+  Vec<D, Real> jbv;
+  from_arr(jbv, jb);
+  Real m[A], dw[A];
+  to_arr_a(cross_t(ld<D>(a.pos_p, i), jbv), m);
+  if constexpr (D == 3) {
+    Real mb[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) mb[k] = m[k] * (Real(1) / I[k]);
+    rmul<3, false>(R, mb, dw);
+  } else {
+    dw[0] = m[0] * (Real(1) / I[0]);
+  }
+#pragma unroll
+  for (int k = 0; k < A; ++k) s[8 * D + 2 + k] = dw[k];
+}
+
+template <int D, class Real>
+__global__ void __launch_bounds__(kBlock) k_crefl_apply(StepArgsT<Real> a) {
+  constexpr int A = D == 3 ? 3 : 1;
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= a.n) return;
+  const int c0 = a.clump[i];
+  Real dv[D], dw[A];
+#pragma unroll
+  for (int c = 0; c < D; ++c) dv[c] = Real(0);
+#pragma unroll
+  for (int k = 0; k < A; ++k) dw[k] = Real(0);
+  for (int k = a.cl_start[c0]; k < a.cl_start[c0 + 1]; ++k) {
+    const Real* sm = refl_row<D>(a, a.cl_members[k]);
+#pragma unroll
+    for (int c = 0; c < D; ++c) dv[c] = dv[c] + sm[7 * D + 2 + c];
+#pragma unroll
+    for (int kk = 0; kk < A; ++kk) dw[kk] = dw[kk] + sm[8 * D + 2 + kk];
+  }
+  if (a.fixed[i]) {
+#pragma unroll
+    for (int c = 0; c < D; ++c) dv[c] = Real(0);
+#pragma unroll
+    for (int k = 0; k < A; ++k) dw[k] = Real(0);
+  }
+  const Real dt_rem = (Real(1) - refl_row<D>(a, i)[7 * D + 1]) * a.dt;
+  Vec<D, Real> dvv;
+  from_arr(dvv, dv);
+  Ang<D, Real> dwv;
+  if constexpr (D == 3)
+    from_arr(dwv, dw);
+  else
+    dwv = Ang1T<Real>{dw[0]};
+  st(a.vel_out, i, ld<D>(a.vel_in, i) + dvv);
+  sta(a.ang_out, i, lda<D>(a.ang_in, i) + dwv);
+  Vec<3, Real> th;
+  if constexpr (D == 3)
+    th = dwv * dt_rem;
+  else
+    th = Vec<3, Real>{Real(0), Real(0), dw[0] * dt_rem};
+  stq(a.q, i, q_unit(q_mul(q_from_small_rotvec(th), ldq(a.q, i))));
+  st(a.pos_out, i, ld<D>(a.pos_in, i) + dvv * dt_rem);
+}
+
 // ---------------------------------------------------------------------------
 // dispatch: every collider source and law in float; in double (the float64
 // form of the plain step) the laws without history.
@@ -715,6 +968,25 @@ hipError_t clump_forces_run(const StepArgsT<Real>& a, int dim, hipStream_t s) {
   }
   return hipGetLastError();
 }
+template <class Real>
+hipError_t clump_reflect_run(const StepArgsT<Real>& a, int dim, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  const dim3 g(nblocks(a.n)), b(kBlock);
+  if (dim == 3) {
+    hipLaunchKernelGGL((k_crefl_over<3, Real>), g, b, 0, s, a);
+    hipLaunchKernelGGL((k_crefl_alpha<3, Real>), g, b, 0, s, a);
+    hipLaunchKernelGGL((k_crefl_impulse<3, Real>), g, b, 0, s, a);
+    hipLaunchKernelGGL((k_crefl_apply<3, Real>), g, b, 0, s, a);
+  } else {
+    hipLaunchKernelGGL((k_crefl_over<2, Real>), g, b, 0, s, a);
+    hipLaunchKernelGGL((k_crefl_alpha<2, Real>), g, b, 0, s, a);
+    hipLaunchKernelGGL((k_crefl_impulse<2, Real>), g, b, 0, s, a);
+    hipLaunchKernelGGL((k_crefl_apply<2, Real>), g, b, 0, s, a);
+  }
+  return hipGetLastError();
+}
+hipError_t launch_clump_reflect(const StepArgs& a, int dim, hipStream_t s) { return clump_reflect_run(a, dim, s); }
+hipError_t launch_clump_reflect(const StepArgs64& a, int dim, hipStream_t s) { return clump_reflect_run(a, dim, s); }
 hipError_t launch_clump_lab(const StepArgs& a, int dim, hipStream_t s) { return clump_lab_run(a, dim, s); }
 hipError_t launch_clump_lab(const StepArgs64& a, int dim, hipStream_t s) { return clump_lab_run(a, dim, s); }
 hipError_t launch_clump_forces(const StepArgs& a, int dim, hipStream_t s) { return clump_forces_run(a, dim, s); }

@@ -13,7 +13,7 @@ clump; ``State.create`` records whether that holds in ``is_sphere_state``)
 and clumps of spheres: rigid bodies whose members share a ``clump_id`` and
 carry their offsets ``pos_p`` from the clump centre, each member holding the
 clump's ``pos_c``, velocities, ``q``, total mass and principal inertia (every
-collider, free and periodic domains).  The reference's clump,
+collider and domain).  The reference's clump,
 facet and mesh builders (``add_clump`` and friends) are out of scope.
 """
 

@@ -43,10 +43,10 @@ Clumps (rigid bodies of spheres, ``State.clump`` ids and body-frame offsets
 ``State.pos_p``): pair forces at the members' lab positions
 ``pos_c + R(q) pos_p`` with same-clump pairs masked, the torque lever
 ``R(q) pos_p x F`` added per member, and ``ForceManager.apply``'s segment sum
-over the clump (force_manager.py:392-412), taken in ascending member index.
+over the clump (force_manager.py:392-412), taken in ascending member index,
+and the reflective domain's clump segment operations (reflect.py:160-297).
 A state without ``clump`` is a sphere state: every particle its own clump and
-``pos_p == 0``, for which those operations are identities.  The reflective
-domain's clump segment operations (reflect.py:165-297) are not restated.
+``pos_p == 0``, for which those operations are identities.
 """
 
 from __future__ import annotations
@@ -1265,7 +1265,8 @@ def domain_apply(st: State, P: Params):
     if dom.kind != "reflect":
         return
     if st.clump is not None:
-        raise NotImplementedError("the reflective domain's clump segment operations are not restated")
+        _reflect_clumps(st, P)
+        return
     dt = np.asarray(P.dt, T_)
     zero = np.asarray(0.0, T_)
     one = np.asarray(1.0, T_)
@@ -1296,6 +1297,110 @@ def domain_apply(st: State, P: Params):
     st.vel = st.vel + dv
     st.q = q_unit(st.q)
     dt_rem = (1.0 - a_min) * dt
+    st.pos = st.pos + dv * dt_rem
+
+
+def _seg(op, x: np.ndarray, ids: np.ndarray, init) -> np.ndarray:
+    """``jax.ops.segment_{max,min,sum}`` over clump ids, broadcast back to the
+    members; sums in ascending member index."""
+    out = np.full((ids.max() + 1,) + x.shape[1:], init, x.dtype)
+    op.at(out, ids, x)
+    return out[ids]
+
+
+def _reflect_clumps(st: State, P: Params):
+    """``ReflectDomain.apply`` for clumps (reflect.py:160-297): each member's
+    overlap past a wall, the deepest member per clump and axis takes the
+    impulse, the clump's time of collision is its members' minimum, and the
+    impulses are summed over the clump into linear and angular velocity
+    changes with the position and orientation corrected over the remaining
+    time.  The matrix products (R^T v, R v, the einsum of the effective
+    mass) sum in index order."""
+    dom = P.domain
+    T_ = st.pos.dtype
+    N, D = st.pos.shape
+    A = 3 if D == 3 else 1
+    dt = np.asarray(P.dt, T_)
+    zero = np.asarray(0.0, T_)
+    one = np.asarray(1.0, T_)
+    e = np.asarray(dom.restitution, T_)
+    ids = st.clump
+    r = pos_p_rot(st)
+    pos = st.pos + r
+    rad = st.rad[:, None]
+    lo = dom.anchor + rad
+    hi = (dom.anchor + dom.box) - rad
+    over_lo = np.maximum(zero, lo - pos)
+    over_hi = np.maximum(zero, pos - hi)
+    max_lo = _seg(np.maximum, over_lo, ids, -np.inf)
+    max_hi = _seg(np.maximum, over_hi, ids, -np.inf)
+    inv_mass = one / st.mass
+    inv_I = one / st.inertia
+    # n_prime[:, k] = R^T e_k, which is row k of R (body -> lab)
+    eye = np.eye(D, dtype=T_)
+    Rm = np.stack([q_rotate_back(st.q, np.broadcast_to(eye[k], (N, D))) for k in range(D)], axis=1)
+    cr = cross3 if D == 3 else cross2
+    rpn = np.stack([cr(st.pos_p, Rm[:, k]) for k in range(D)], axis=1)  # (N, D, A)
+    sq = rpn * rpn
+    denom_rot = sq[..., 0] * inv_I[:, None, 0]
+    for i in range(1, A):
+        denom_rot = denom_rot + sq[..., i] * inv_I[:, None, i]
+    denom = inv_mass[:, None] + denom_rot
+    denom = np.where(denom == 0.0, one, denom)
+    delta = np.maximum(max_lo, max_hi)
+    deep_lo = (over_lo > 0) & (over_lo == max_lo)
+    deep_hi = (over_hi > 0) & (over_hi == max_hi)
+    ws = deep_lo.astype(T_) - deep_hi.astype(T_)
+    active = np.abs(ws)
+    v_step = st.vel + cross_w_r(st.ang_vel, r)
+    acc_clump = st.force * inv_mass[:, None]
+
+    def mat(M, v, transpose):  # R v or R^T v, summing over j in order
+        out = []
+        for i in range(D):
+            acc = None
+            for j in range(D):
+                m = M[:, j, i] if transpose else M[:, i, j]
+                t = m * v[:, j]
+                acc = t if acc is None else acc + t
+            out.append(acc)
+        return np.stack(out, axis=1)
+
+    if D == 3:
+        tb = mat(Rm, st.torque, True)
+        wb = mat(Rm, st.ang_vel, True)
+        ab = (tb - cross3(wb, st.inertia * wb)) * inv_I
+        alpha_rot = mat(Rm, ab, False)
+    else:
+        alpha_rot = st.torque * inv_I
+    acc_contact = acc_clump + cross_w_r(alpha_rot, r)
+    alpha = verlet_collision_fraction(v_step, acc_contact, delta, ws, dt)
+    a_min = np.min(np.where(active > 0, alpha, one), axis=-1, keepdims=True)
+    a_clump = _seg(np.minimum, a_min, ids, np.inf)
+    dt_factor = (a_clump - 1.0) * dt
+    v_contact = v_step + dt_factor * acc_contact
+    j = ((-(one + e)) * v_contact) / denom
+    closing = ((v_contact * ws) < 0.0).astype(T_)
+    count = _seg(np.add, active, ids, 0.0)
+    weight = active / np.maximum(count, one)
+    j = j * (weight * closing)
+    dv = _seg(np.add, j * inv_mass[:, None], ids, 0.0)
+    dv = np.where(st.fixed[:, None], zero, dv)
+    st.vel = st.vel + dv
+    jb = mat(Rm, j, True)
+    moment = cr(st.pos_p, jb)
+    if D == 2:
+        dw = moment[..., -1:] * inv_I
+    else:
+        dw = mat(Rm, moment * inv_I, False)
+    dw = _seg(np.add, dw, ids, 0.0)
+    dw = np.where(st.fixed[:, None], zero, dw)
+    st.ang_vel = st.ang_vel + dw
+    dt_rem = (1.0 - a_clump) * dt
+    dth = dw * dt_rem
+    if D == 2:
+        dth = np.concatenate([np.zeros_like(dth), np.zeros_like(dth), dth], axis=-1)
+    st.q = q_unit(q_mul(q_from_small_rotvec(dth), st.q))
     st.pos = st.pos + dv * dt_rem
 
 

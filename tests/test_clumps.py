@@ -12,7 +12,11 @@ missing segment sum or gravity counted per member fails:
 * two spinning dimers colliding in free space keep the total linear and
   angular momentum (orbital + spin) while the spin changes by O(1);
 * the free domain's box bounds |pos_p| + rad; the kinetic energy counts each
-  clump once.
+  clump once;
+* the reflective domain: singleton clumps bounce as spheres do, and a tilted
+  dimer striking the floor with one sphere leaves with its contact point's
+  normal velocity reversed times e, spinning, its kinetic energy kept at
+  e = 1 (a wrong effective mass or lever sign breaks both).
 """
 
 from __future__ import annotations
@@ -193,3 +197,52 @@ def test_free_box_and_kinetic_energy_of_clumps():
     # 1/2 M V^2 + 1/2 I_zz w^2, counted once per clump
     M, Izz = 2.0, 2 * (0.4 * 0.25) + 2 * 0.25
     assert O.kinetic_energy(st) == pytest.approx(0.5 * M * 0.25 + 0.5 * Izz * 4.0, rel=1e-14)
+
+
+def test_singleton_clumps_reflect_like_spheres():
+    """The clump form of the reflective domain reduces to the sphere form."""
+    rng = np.random.Generator(np.random.PCG64(8))
+    N, D = 200, 3
+    L = 8.0
+    pos = rng.uniform(-0.3, L + 0.3, (N, D)).astype(np.float32)  # some past the walls
+    st = O.make_state(pos, np.full(N, 0.4, np.float32), vel=rng.standard_normal((N, D)).astype(np.float32),
+                      ang_vel=rng.standard_normal((N, D)).astype(np.float32))
+    st.force = rng.standard_normal((N, D)).astype(np.float32)
+    dom = O.Domain("reflect", np.full(D, L, np.float32), np.zeros(D, np.float32), 0.8)
+    P = O.Params(dim=D, dt=1e-2, domain=dom, law=O.Law("spring", young_eff=np.ones((1, 1), np.float32)))
+    c = st.copy()
+    c.clump = np.arange(N)
+    c.pos_p = np.zeros((N, D), np.float32)
+    O.domain_apply(st, P)
+    O.domain_apply(c, P)
+    for f in ("pos", "vel", "ang_vel", "q"):
+        assert np.array_equal(getattr(st, f), getattr(c, f)), f
+
+
+@pytest.mark.parametrize("dim", [3, 2])
+def test_tilted_dimer_bounce_is_elastic_at_e1(dim):
+    """A tilted dimer whose lower sphere reaches the floor: with e = 1 the
+    impulse keeps the kinetic energy (translation + rotation), reverses the
+    contact point's normal velocity and spins the body; with e = 0.5 the
+    contact point leaves at half its approach speed."""
+    for e in (1.0, 0.5):
+        th = 0.6
+        c = np.array([3.0, 0.5 + 0.45 * np.sin(th) - 0.01, 3.0][:dim])  # lower sphere 0.01 into the floor
+        w0 = np.zeros(3 if dim == 3 else 1)
+        st = _dimers([(c, np.array([0.3, -1.0, 0.0][:dim]), w0, _quat_z(th))], d=0.45, dim=dim)
+        st.pos = np.tile(c, (2, 1))
+        dom = O.Domain("reflect", np.full(dim, 6.0), np.zeros(dim), e)
+        P = O.Params(dim=dim, dt=1e-3, domain=dom, law=O.Law("spring", young_eff=np.ones((1, 1))))
+        ke0 = O.kinetic_energy(st)
+        lab0 = O.lab_pos(st)
+        low = int(np.argmin(lab0[:, 1]))
+        r0 = O.pos_p_rot(st)[low:low + 1]  # the lever at the impulse (before the orientation correction)
+        vp0 = st.vel[low] + O.cross_w_r(st.ang_vel[low:low + 1], r0)[0]
+        O.domain_apply(st, P)
+        vp1 = st.vel[low] + O.cross_w_r(st.ang_vel[low:low + 1], r0)[0]
+        # the members stay one body
+        assert np.array_equal(st.vel[0], st.vel[1]) and np.array_equal(st.ang_vel[0], st.ang_vel[1])
+        assert np.abs(st.ang_vel).max() > 0.1
+        assert vp1[1] == pytest.approx(-e * vp0[1], rel=1e-12)
+        if e == 1.0:
+            assert O.kinetic_energy(st) == pytest.approx(ke0, rel=1e-12)

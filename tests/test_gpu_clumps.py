@@ -1,8 +1,8 @@
 """Clumps (rigid bodies of spheres) on the GPU against the oracle on the same
 seeded inputs: dimers, trimers and single spheres with random orientations
 and spins; forces and torques bit for bit (naive and CellList, 2D and 3D),
-whole steps of every integrator pair with gravity (periodic and free
-domains), NeighborList runs with rebuilds on the lab positions, potential and
+whole steps of every integrator pair with gravity (periodic, free and
+reflective domains), NeighborList runs with rebuilds on the lab positions, potential and
 kinetic energy to 1e-12, the float64 form within 1e-12 of the float64
 oracle; the combinations off the GPU path refuse."""
 
@@ -98,7 +98,8 @@ def test_clump_compute_force_bitwise(device, dim, law, collider):
 
 
 CASES = [("naive", "periodic", "verlet", "verletspiral"), ("naive", "free", "verlet", "verletspiral"),
-         ("CellList", "periodic", "euler", "spiral"), ("CellList", "periodic", "verlet", "verletspiral")]
+         ("CellList", "periodic", "euler", "spiral"), ("CellList", "periodic", "verlet", "verletspiral"),
+         ("naive", "reflect", "verlet", "verletspiral"), ("CellList", "reflect", "euler", "spiral")]
 
 
 @pytest.mark.parametrize("collider,domain,lin,rot", CASES)
@@ -226,8 +227,6 @@ def test_clump_neighborlist_f64(device):
 
 def test_clump_unsupported_combinations_raise(device):
     st, box = clump_scene(4, 3, seed=1)
-    with pytest.raises(jd._lib.JdemError, match="reflective"):
-        jd.System.step(st, clump_system(st, box, "spring", "naive", "reflect"), n=1)
     mats = jd.MaterialTable.from_materials([jd.Material.create("elastic", density=1.0, young=1e4, poisson=0.3)])
     sy = jd.System.create(st.shape, dt=1e-4, domain_type="periodic", domain_kw=dict(box_size=box),
                           collider_type="naive", mat_table=mats)
@@ -299,7 +298,8 @@ def precision(name):
 
 @pytest.mark.parametrize("prec", ["float32", "float64"])
 @pytest.mark.parametrize("collider,domain", [("CellList", "periodic"), ("nl_naive", "periodic"),
-                                             ("nl_CellList", "periodic"), ("nl_naive", "free")])
+                                             ("nl_CellList", "periodic"), ("nl_naive", "free"),
+                                             ("CellList", "reflect"), ("nl_CellList", "reflect")])
 @pytest.mark.parametrize("dim", [2, 3])
 def test_clumps_invariance(device, dim, collider, domain, prec):
     """naive vs CellList and NeighborList over 200 steps, atol = rtol = 5e-2
@@ -320,7 +320,7 @@ def test_clumps_invariance(device, dim, collider, domain, prec):
     np.testing.assert_allclose(pe1, pe2, atol=5e-2, rtol=5e-2)
 
 
-@pytest.mark.parametrize("domain", ["periodic", "free"])
+@pytest.mark.parametrize("domain", ["periodic", "reflect", "free"])
 @pytest.mark.parametrize("integrator", ["euler", "verlet"])
 @pytest.mark.parametrize("dim", [2, 3])
 def test_clumps_energy_conservation(device, dim, integrator, domain):
