@@ -73,6 +73,8 @@ def parse_args():
     ap.add_argument("--precision-steps", type=int, default=20000,
                     help="FIRE steps timed (0: run to the float32 floor, which stalls after ~1e5 steps)")
     ap.add_argument("--no-precision", action="store_true")
+    ap.add_argument("--precision-skin", type=float, default=0.05,
+                    help="Verlet-list skin of the precision form (0: the cell stencil every step)")
     ap.add_argument("--rollout-stride", type=int, default=50, help="steps between saved frames of the rollout line")
     ap.add_argument("--rollout-steps", type=int, default=1000, help="steps of the rollout line")
     ap.add_argument("--no-rollout", action="store_true")
@@ -363,12 +365,13 @@ def precision_bench(args, dev) -> dict:
     steps = args.precision_steps if args.precision_steps > 0 else 400000
     stall = 0 if args.precision_steps > 0 else 20000
     # warm-up: a few steps, so the timed run does not include the first allocations
+    sk = max(args.precision_skin, 0.0)
     PR.minimize_precise(pos, rad, L, k=1.0, dt=1e-2, max_steps=10, pe_tol=0.0, pe_diff_tol=-1.0, force_tol=0.0,
-                        device=dev)
+                        skin=sk, device=dev)
     torch.cuda.synchronize()
     t = time.perf_counter()
     r = PR.minimize_precise(pos, rad, L, k=1.0, dt=1e-2, max_steps=steps, pe_tol=0.0, pe_diff_tol=-1.0,
-                            force_tol=1e-12, stall_steps=stall, device=dev)
+                            force_tol=1e-12, stall_steps=stall, skin=sk, device=dev)
     torch.cuda.synchronize()
     secs = time.perf_counter() - t
     print(f"[bench] precision: {r['steps']} steps in {secs:.2f} s, max|F| {r['max_grad']:.3e}", file=sys.stderr,
@@ -380,7 +383,7 @@ def precision_bench(args, dev) -> dict:
     ctx = Lb.context_for(dev)
     ctx.set_timing(True)
     PR.minimize_precise(pos, rad, L, k=1.0, dt=1e-2, max_steps=int(r["steps"]), pe_tol=0.0, pe_diff_tol=-1.0,
-                        force_tol=0.0, device=dev)
+                        force_tol=0.0, skin=sk, device=dev)
     tm = ctx.get_timing()
     ctx.set_timing(False)
     avg_ms = tm["step_kernel_ms"] / max(tm["step_kernels"], 1)
@@ -389,7 +392,7 @@ def precision_bench(args, dev) -> dict:
     return {
         "workload": f"bidisperse_jamming: {args.precision_n} spheres (r 0.5/0.7, 50:50), periodic cube L = {L:.2f} "
         "at phi 0.66, harmonic k = 1, FIRE dt0 = 1e-2, integer cell + float32 offset positions, float64 "
-        "energy / virial / power / residual",
+        "energy / virial / power / residual, " + (f"Verlet list skin {sk}" if sk > 0 else "27-cell stencil"),
         "steps": r["steps"],
         "seconds": secs,
         "ms_per_step": secs * 1e3 / max(r["steps"], 1),
@@ -397,7 +400,7 @@ def precision_bench(args, dev) -> dict:
         "max_grad": r["max_grad"],
         "pe_per_particle": r["pe"],
         "pressure": r["pressure"],
-        "roofline": {"bound": "hbm", "kernel": "k_prec_force", "kernel_avg_us": 1e3 * avg_ms,
+        "roofline": {"bound": "hbm", "kernel": "k_prec_vforce" if sk > 0 else "k_prec_force", "kernel_avg_us": 1e3 * avg_ms,
                      "algorithmic_bytes_per_sphere": bps, "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS},
         "fp64_error": precision_fp64_error(dev),

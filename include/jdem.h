@@ -497,6 +497,12 @@ typedef struct jdem_precise {
                           nor the energy (by more than 1e-13 of it) has
                           reached a new low for this many steps (0: never);
                           float32 positions have a residual floor         */
+  float skin;            /* Verlet list: pairs within r_i + r_j + skin, rebuilt
+                            (with the cell re-sort) once a sphere has moved
+                            skin / 2; 0 or max_neighbors 0: the cell stencil
+                            every evaluation                               */
+  int32_t max_neighbors; /* list row length; a bound on the spheres within
+                            2 r_max + skin of a centre, or the call fails  */
 } jdem_precise;
 
 /* minimize (minimizers/routines.py:150-383) with FIRE on a jdem_precise

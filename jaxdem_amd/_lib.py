@@ -229,6 +229,8 @@ class JdemPrecise(ctypes.Structure):
         ("force", c_void_p),
         ("k", c_float),
         ("stall_steps", c_int32),
+        ("skin", c_float),
+        ("max_neighbors", c_int32),
     ]
 
 

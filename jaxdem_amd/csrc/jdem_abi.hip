@@ -22,7 +22,7 @@ std::vector<Buf*> jdem_ctx::bufs() {
                            &cstart,  &cend,   &spos,     &nl_new,  &xi_new,  &live_new, &flags,  &red,
                            &nl_copy, &xi_copy, &live_copy, &wF,     &wT,      &wOld,     &wWpos, &wSrc,
                            &wInv,    &wKeys,  &fireV,    &fireDev, &segp,  &segid, &segflag, &wXiRank,
-                           &route,   &ccnt,   &cl_start, &cl_members, &plab, &ppr, &ftot, &ttot, &crefl};
+                           &route,   &ccnt,   &cl_start, &cl_members, &plab, &ppr, &ftot, &ttot, &crefl, &pvl, &pvaux};
   for (int k = 0; k < 2; ++k)
     for (Buf* b : {&wP[k], &wV[k], &wW[k], &wQ[k], &wI[k], &wOrig[k], &wNl[k], &wXi[k], &wLive[k]}) all.push_back(b);
   return all;
@@ -2088,6 +2088,39 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
   // of the flag.  The radix fallback keeps the host-side decision.
   const bool gated = use_cell_sort(N, n_cells);
   a.gated = gated ? 1 : 0;
+  // The Verlet list (with the counting sort only): pairs within r_i + r_j +
+  // skin; a partner is at most 1 + ceil(skin / h) cells away on an axis
+  // when h >= 2 r_max, and the build stencil must not wrap onto itself.
+  int32_t* vovf = nullptr;
+  if (gated && N > 0 && p->skin > 0.0f && p->max_neighbors > 0) {
+    float hmin = p->cell[0];
+    for (int q = 1; q < D; ++q) hmin = std::min(hmin, p->cell[q]);
+    const int sr = 1 + (int)std::ceil(p->skin / hmin);
+    bool fits = sr <= 4;
+    for (int q = 0; q < D; ++q) fits = fits && p->dims[q] >= 2 * sr + 1;
+    if (fits) {
+      ENSURE(ctx, ctx->pvl, 4 * Nn * (size_t)p->max_neighbors);
+      ENSURE(ctx, ctx->pvaux, 4 * Nn * (3 + 2 * (size_t)D) + 64);
+      a.vl = (int32_t*)ctx->pvl.p;
+      a.vm = p->max_neighbors;
+      a.vsr = sr;
+      a.vcut = p->skin;
+      a.vtrig = 0.25f * p->skin * p->skin;
+      a.scidx = (int32_t*)ctx->pvaux.p;
+      a.bcidx = a.scidx + 3 * Nn;
+      a.boff = (float*)(a.bcidx + D * Nn);
+      vovf = (int32_t*)(a.boff + D * Nn);
+      a.vovf = vovf;
+      HIPCHK(ctx, launch_fill_i32(vovf, 1, 0, st));
+    }
+  }
+  auto list_overflow = [&]() -> int {
+    int32_t ov = 0;
+    if (!vovf) return JDEM_OK;
+    HIPCHK(ctx, hipMemcpyAsync(&ov, vovf, sizeof(ov), hipMemcpyDeviceToHost, st));
+    HIPCHK(ctx, hipStreamSynchronize(st));
+    return ov ? fail(ctx, JDEM_EINVAL, "a neighbour row exceeds max_neighbors") : JDEM_OK;
+  };
   bool sorted = false;
   double pe = 0.0, gmax = 0.0;
   auto eval = [&](bool with_virial) -> int {
@@ -2106,8 +2139,9 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
     }
     a.w_out = with_virial ? wper : nullptr;
     HIPCHK(ctx, launch_prec_gather(a, st));
-    const int sp = span_begin(ctx, 0, 0);  // k_prec_force, for its roofline
-    HIPCHK(ctx, launch_prec_force(a, st));
+    if (a.vl) HIPCHK(ctx, launch_prec_vbuild(a, st));
+    const int sp = span_begin(ctx, 0, 0);  // the force kernel, for its roofline
+    HIPCHK(ctx, a.vl ? launch_prec_vforce(a, st) : launch_prec_force(a, st));
     span_end(ctx, sp);
     if (N == 0) HIPCHK(ctx, hipMemsetAsync(&f.st->max_grad_bits, 0, sizeof(unsigned), st));  // else the gather clears it
     HIPCHK(ctx, launch_fire_grad_max(f, st));
@@ -2123,7 +2157,7 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
     std::memcpy(&g, &bits, sizeof(g));
     pe = h;
     gmax = (double)g;
-    return JDEM_OK;
+    return list_overflow();
   };
   // the first evaluation sorts: the flag starts set
   HIPCHK(ctx, launch_fill_i32(moved, 1, 1, st));
@@ -2170,14 +2204,17 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
         HIPCHK(ctx, cell_sort_pairs(ctx, a.keys, N, n_cells, &sk, &perm, (int32_t*)ctx->cstart.p, st, moved));
         a.perm = perm;
         HIPCHK(ctx, launch_prec_gather(a, st));
+        if (a.vl) HIPCHK(ctx, launch_prec_vbuild(a, st));
         const int sp = span_begin(ctx, 0, k);
-        HIPCHK(ctx, launch_prec_force(a, st));
+        HIPCHK(ctx, a.vl ? launch_prec_vforce(a, st) : launch_prec_force(a, st));
         span_end(ctx, sp);
         HIPCHK(ctx, launch_fire_grad_max(f, st));
         HIPCHK(ctx, launch_sum_f64(per, N, 1, 1, part, out, 256, st));
       }
       HIPCHK(ctx, hipMemcpyAsync(&h_ps, ps, sizeof(h_ps), hipMemcpyDeviceToHost, st));
       HIPCHK(ctx, hipStreamSynchronize(st));
+      rc = list_overflow();
+      if (rc) return rc;
       if (h_ps.stop) break;
     }
     steps = h_ps.steps;

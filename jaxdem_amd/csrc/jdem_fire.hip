@@ -249,11 +249,63 @@ __global__ void __launch_bounds__(kBlock) k_prec_gather(PrecArgs a) {
   const int64_t kk = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (kk == 0) {  // after this evaluation's sort, before its max|F| reduction
     if (a.gmax_bits) *a.gmax_bits = 0u;
-    if (a.gated) *a.moved = 0;
+    if (a.gated && !a.vl) *a.moved = 0;
   }
   if (kk >= a.n) return;
   const int64_t j = a.perm[kk];
   a.sp[kk] = make_float4(a.off[D * j], a.off[D * j + 1], D == 3 ? a.off[D * j + 2] : 0.0f, a.rad[j]);
+  if (a.vl)
+    for (int q = 0; q < 3; ++q) a.scidx[3 * kk + q] = q < D ? a.cidx[D * j + q] : 0;
+}
+
+// Verlet list of the precision form (rebuilds only, gated by *moved): the
+// partners of the sphere in sorted slot kk0 within r_i + r_j + skin, by
+// sorted slot, in stencil order (half-width vsr); the build positions.
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_prec_vbuild(PrecArgs a) {
+  if (*a.moved == 0 || (a.stop && *a.stop)) return;
+  const int64_t kk0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (kk0 >= a.n) return;
+  const int64_t i = a.perm[kk0];
+  const float4 si = a.sp[kk0];
+  const float oi[3] = {si.x, si.y, si.z};
+  int ci[3] = {0, 0, 0};
+  for (int q = 0; q < D; ++q) {
+    ci[q] = a.cidx[D * i + q];
+    a.bcidx[D * i + q] = ci[q];
+    a.boff[D * i + q] = oi[q];
+  }
+  int32_t* row = a.vl + kk0 * a.vm;
+  int cnt = 0;
+  const int sr = a.vsr, zr = D == 3 ? sr : 0;
+  for (int ox = -sr; ox <= sr; ++ox)
+    for (int oy = -sr; oy <= sr; ++oy)
+      for (int oz = -zr; oz <= zr; ++oz) {
+        const int o[3] = {ox, oy, oz};
+        int cc[3] = {0, 0, 0};
+        for (int q = 0; q < D; ++q) {
+          cc[q] = (ci[q] + o[q]) % a.dims[q];
+          if (cc[q] < 0) cc[q] += a.dims[q];
+        }
+        const int64_t h = D == 3 ? ((int64_t)cc[0] * a.dims[1] + cc[1]) * a.dims[2] + cc[2]
+                                 : (int64_t)cc[0] * a.dims[1] + cc[1];
+        for (int kk = a.starts[h]; kk < a.starts[h + 1]; ++kk) {
+          if (kk == kk0) continue;
+          const float4 sj = a.sp[kk];
+          const float osj[3] = {sj.x, sj.y, sj.z};
+          float d2 = 0.0f;
+          for (int q = 0; q < D; ++q) {
+            const float r = (float)(-o[q]) * a.h[q] + (oi[q] - osj[q]);
+            d2 = d2 + r * r;
+          }
+          const float R = (si.w + sj.w) + a.vcut;
+          if (!(d2 < R * R)) continue;
+          if (cnt < a.vm) row[cnt] = kk;
+          ++cnt;
+        }
+      }
+  for (int t = cnt; t < a.vm; ++t) row[t] = -1;
+  if (cnt > a.vm) *a.vovf = 1;
 }
 
 // spring law (spring.py:97-108, 138-147) on the cell stencil; float32 pair
@@ -349,6 +401,81 @@ __global__ void __launch_bounds__(kBlock) k_prec_force(PrecArgs a) {
     for (int q = 0; q < D * D; ++q) a.w_out[(int64_t)D * D * i + q] = w[q];
 }
 
+// The spring force of k_prec_force over the Verlet list: the pair's cell
+// difference comes from the cells of the two slots now (wrapped to the
+// nearest image), so spheres may change cell between builds; a contact has
+// |dc| <= 1 on every axis, where the displacement is the same float pair as
+// on the stencil.  Partners are summed in the list's order (the stencil
+// order of the build).
+template <int D>
+__global__ void __launch_bounds__(kBlock) k_prec_vforce(PrecArgs a) {
+  if (a.stop && *a.stop) return;
+  const int64_t kk0 = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (kk0 == 0) *a.moved = 0;  // the list of this evaluation is built
+  if (kk0 >= a.n) return;
+  const int64_t i = a.perm[kk0];
+  const float4 si = a.sp[kk0];
+  const float oi[3] = {si.x, si.y, si.z};
+  const int ci[3] = {a.scidx[3 * kk0], a.scidx[3 * kk0 + 1], a.scidx[3 * kk0 + 2]};
+  const float ri = si.w;
+  float F[3] = {0, 0, 0};
+  double e = 0.0, w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+  const int32_t* row = a.vl + kk0 * a.vm;
+  for (int t = 0; t < a.vm; ++t) {
+    const int kk = row[t];
+    if (kk < 0) break;
+    const float4 sj = a.sp[kk];
+    const float osj[3] = {sj.x, sj.y, sj.z};
+    int dc[3] = {0, 0, 0};
+    for (int q = 0; q < D; ++q) {
+      int d = ci[q] - a.scidx[3 * kk + q];
+      if (2 * d > a.dims[q]) d -= a.dims[q];
+      if (2 * d < -a.dims[q]) d += a.dims[q];
+      dc[q] = d;
+    }
+    float r[3] = {0, 0, 0}, rl[3] = {0, 0, 0};
+    for (int q = 0; q < D; ++q) r[q] = (float)dc[q] * a.h[q] + (oi[q] - osj[q]);
+    const float d2 = D == 3 ? (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
+    const float R = ri + sj.w;
+    if (!(d2 < R * R) || d2 == 0.0f) continue;
+    for (int q = 0; q < D; ++q) {
+      const float oj = osj[q];
+      const float b = oi[q] - oj;
+      const float bv = b - oi[q];
+      const float be = (oi[q] - (b - bv)) + (-oj - bv);
+      const float c = (float)dc[q] * a.h[q];
+      const float tt = c + b;
+      const float tv = tt - c;
+      const float te = (c - (tt - tv)) + (b - tv);
+      rl[q] = (te + be) + (float)dc[q] * a.hlo[q];
+    }
+    const float inv = 1.0f / sqrtf(d2);
+    float sh = 0.0f, sl = 0.0f;
+    for (int q = 0; q < D; ++q) {
+      const float p = r[q] * r[q];
+      const float pe = fmaf(r[q], r[q], -p);
+      const float tt = sh + p;
+      const float bp = tt - sh;
+      sl = sl + ((((sh - (tt - bp)) + (p - bp)) + pe) + 2.0f * r[q] * rl[q]);
+      sh = tt;
+    }
+    const float RR = R * R;
+    const float diff = (RR - sh) + (fmaf(R, R, -RR) - sl);
+    const float delta = diff / (R + d2 * inv);
+    const float s = (a.k * delta) * inv;
+    for (int q = 0; q < D; ++q) F[q] = F[q] + s * r[q];
+    const double dd = (double)delta;
+    e += 0.25 * (double)a.k * dd * dd;
+    if (a.w_out)
+      for (int p = 0; p < D; ++p)
+        for (int q = 0; q < D; ++q) w[D * p + q] += 0.5 * (double)r[p] * (double)(s * r[q]);
+  }
+  for (int q = 0; q < D; ++q) a.force[D * i + q] = F[q];
+  a.e_out[i] = e;
+  if (a.w_out)
+    for (int q = 0; q < D * D; ++q) a.w_out[(int64_t)D * D * i + q] = w[q];
+}
+
 // the FIRE update of the position leaf (no rotation: the spring law has no
 // torque), then the offset is folded back into [0, h)
 template <int D>
@@ -377,6 +504,18 @@ __global__ void __launch_bounds__(kBlock) k_prec_move(FireArgs f, PrecArgs a) {
       moved = true;
     }
     a.off[D * i + q] = o;
+  }
+  if (a.vl) {  // list mode: a rebuild once the sphere has moved skin / 2
+    float d2 = 0.0f;
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      int dc = a.cidx[D * i + q] - a.bcidx[D * i + q];
+      if (2 * dc > a.dims[q]) dc -= a.dims[q];
+      if (2 * dc < -a.dims[q]) dc += a.dims[q];
+      const float d = (float)dc * a.h[q] + (a.off[D * i + q] - a.boff[D * i + q]);
+      d2 = d2 + d * d;
+    }
+    moved = d2 > a.vtrig;
   }
   if (moved) *a.moved = 1;
 }
@@ -450,6 +589,22 @@ hipError_t launch_prec_force(const PrecArgs& a, hipStream_t s) {
     hipLaunchKernelGGL(k_prec_force<3>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
   else
     hipLaunchKernelGGL(k_prec_force<2>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_prec_vbuild(const PrecArgs& a, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  if (a.dim == 3)
+    hipLaunchKernelGGL(k_prec_vbuild<3>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_prec_vbuild<2>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  return hipGetLastError();
+}
+hipError_t launch_prec_vforce(const PrecArgs& a, hipStream_t s) {
+  if (a.n == 0) return hipSuccess;
+  if (a.dim == 3)
+    hipLaunchKernelGGL(k_prec_vforce<3>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
+  else
+    hipLaunchKernelGGL(k_prec_vforce<2>, dim3(nblocks(a.n)), dim3(kBlock), 0, s, a);
   return hipGetLastError();
 }
 hipError_t launch_prec_move(const FireArgs& f, const PrecArgs& a, hipStream_t s) {

@@ -81,6 +81,8 @@ struct jdem_ctx {
   jdem::Buf ccnt;
   // clumps: members per clump (CSR), lab positions, levers, member totals
   jdem::Buf cl_start, cl_members, plab, ppr, ftot, ttot, crefl;
+  // the precision form's Verlet list and its per-sphere build state
+  jdem::Buf pvl, pvaux;
   // every scratch buffer of the context, for accounting and release
   std::vector<jdem::Buf*> bufs();
   size_t scratch_bytes();
@@ -664,6 +666,18 @@ struct PrecArgs {
   unsigned* gmax_bits;   // max|F| word of the FIRE state, cleared by the gather
   int gated;             // 1: the key kernel runs only while *moved, and the gather clears *moved
   const int32_t* stop;   // when set, every kernel returns at once once *stop is set
+  // Verlet list (vl set): *moved means "rebuild the list", raised when a
+  // sphere has moved skin / 2 since the build; the sort, the list build and
+  // the build positions run only then, and the list force clears *moved
+  int32_t* vl;           // (N, vm) by sorted slot: sorted slots of the partners, -1 ends a row
+  int vm;
+  int vsr;               // stencil half-width of the list build
+  float vcut;            // list reach beyond r_i + r_j (the skin)
+  float vtrig;           // (skin / 2)^2
+  int32_t* scidx;        // (N, 3) cell of each sorted slot, gathered per evaluation
+  int32_t* bcidx;        // (N, D) cell and offset of each sphere at the build
+  float* boff;
+  int32_t* vovf;         // set when a row would exceed vm
 };
 // the stopping tests of one FIRE iteration from the last evaluation's float64
 // energy sum and max|F| word; stops, or counts the update that follows
@@ -672,6 +686,8 @@ hipError_t launch_prec_check(PrecStop* ps, const double* pe_sum, const unsigned*
 hipError_t launch_prec_keys(const PrecArgs& a, hipStream_t s);
 hipError_t launch_prec_gather(const PrecArgs& a, hipStream_t s);
 hipError_t launch_prec_force(const PrecArgs& a, hipStream_t s);
+hipError_t launch_prec_vbuild(const PrecArgs& a, hipStream_t s);
+hipError_t launch_prec_vforce(const PrecArgs& a, hipStream_t s);
 hipError_t launch_prec_move(const FireArgs& f, const PrecArgs& a, hipStream_t s);
 
 inline unsigned nblocks(int64_t n, int b = kBlock) { return (unsigned)((n + b - 1) / b); }

@@ -66,7 +66,8 @@ def from_cell_offset(cidx, off, box, dims) -> np.ndarray:
 
 def minimize_precise(pos, rad, box, k: float = 1.0, fire_cfg: FireConfig | None = None, dt: float = 1e-2,
                      max_steps: int = 10000, pe_tol: float = 1e-16, pe_diff_tol: float = 1e-16,
-                     force_tol: float = 0.0, stall_steps: int = 0, device="cuda") -> dict:
+                     force_tol: float = 0.0, stall_steps: int = 0, skin: float = 0.0,
+                     max_neighbors: int | None = None, device="cuda") -> dict:
     """FIRE (minimizers/routines.py:150-383) of harmonic spheres in a periodic
     box on the precision-form representation.  Returns float64 positions, the
     energy per particle, the largest force component, the contact virial and
@@ -75,7 +76,11 @@ def minimize_precise(pos, rad, box, k: float = 1.0, fire_cfg: FireConfig | None 
     The loop stops on the reference's three tests (pe_tol, pe_diff_tol,
     force_tol) or at ``max_steps``; with ``stall_steps`` > 0 it also stops
     once neither max |F| nor the energy (by more than 1e-13 of it) has
-    reached a new low for that many steps.  The float32
+    reached a new low for that many steps.  With ``skin`` > 0 the pairs come
+    from a Verlet list of reach r_i + r_j + skin, rebuilt (with the cell
+    re-sort) once a sphere has moved skin / 2; ``max_neighbors`` defaults to
+    a packing bound, ((3 r_max + skin) / r_min)^D spheres of radius >= r_min
+    around a centre.  The float32
     pair arithmetic and the float32 offsets leave a residual floor near
     1e-7 k h, below which force_tol cannot be met."""
     pos = np.asarray(pos, np.float64)
@@ -103,6 +108,11 @@ def minimize_precise(pos, rad, box, k: float = 1.0, fire_cfg: FireConfig | None 
     p.force = tf.data_ptr()
     p.k = float(F32(k))
     p.stall_steps = int(stall_steps)
+    p.skin = float(F32(max(skin, 0.0)))
+    if skin > 0 and max_neighbors is None and N > 0:
+        rmin, rmax = float(rad.min()), float(rad.max())
+        max_neighbors = int(np.ceil(((3 * rmax + skin) / rmin) ** D))
+    p.max_neighbors = int(max_neighbors or 0) if skin > 0 else 0
     cfg = fire_cfg or fire(dt)
     fs = cfg.struct()
     stats = L.JdemMinimizeStats()

@@ -118,7 +118,7 @@ def fp64_run(pos, rad, L, force_tol=1e-12, max_steps=400000, threads=None) -> di
     return r
 
 
-def gpu_precise(pos, rad, L, max_steps=400000, stall_steps=20000, device="cuda") -> dict:
+def gpu_precise(pos, rad, L, max_steps=400000, stall_steps=20000, device="cuda", skin=0.0) -> dict:
     import torch
 
     from jaxdem_amd import precision as PR
@@ -126,7 +126,7 @@ def gpu_precise(pos, rad, L, max_steps=400000, stall_steps=20000, device="cuda")
     torch.cuda.synchronize()
     t = time.perf_counter()
     r = PR.minimize_precise(pos, rad, L, k=K, dt=DT, max_steps=max_steps, pe_tol=0.0, pe_diff_tol=-1.0,
-                            force_tol=1e-12, stall_steps=stall_steps, device=device)
+                            force_tol=1e-12, stall_steps=stall_steps, skin=skin, device=device)
     torch.cuda.synchronize()
     r["seconds"] = time.perf_counter() - t
     return r

@@ -66,7 +66,8 @@ STRUCT_FIELDS = {
     "jdem_fire": ["dt", "alpha_init", "f_inc", "f_dec", "f_alpha", "n_min", "n_bad_max", "dt_max_scale",
                   "dt_min_scale"],
     "jdem_minimize_stats": ["steps", "pe", "max_grad", "power", "overflow", "reserved"],
-    "jdem_precise": ["n", "dim", "dims", "cell", "cell_lo", "cidx", "off", "rad", "force", "k", "stall_steps"],
+    "jdem_precise": ["n", "dim", "dims", "cell", "cell_lo", "cidx", "off", "rad", "force", "k", "stall_steps",
+                     "skin", "max_neighbors"],
     "jdem_frames": ["n_frames", "steps", "pos", "vel", "ang_vel", "q", "force", "torque"],
     "jdem_state_f64": ["n", "dim", "bond_width", "pos", "vel", "ang_vel", "q", "force", "torque", "rad", "mass",
                        "inertia", "mat_id", "bond_id", "fixed", "pos_p", "clump_id"],

@@ -1012,7 +1012,8 @@ def test_serving_full_size_request_in_a_mix(device):
 # ---------------------------------------------------------------------------
 # (g) precision form: integer cell + float32 offset, float64 reductions
 # ---------------------------------------------------------------------------
-def test_precise_fire_against_fp64(device):
+@pytest.mark.parametrize("skin", [0.0, 0.1])
+def test_precise_fire_against_fp64(device, skin):
     """FIRE on the integer-cell + float32-offset positions started at the
     float64 minimum of tests/golden/jamming_n1024.npz (1024 bidisperse
     spheres at volume fraction 0.66, max|F| < 1e-12): the run must stay at
@@ -1021,13 +1022,14 @@ def test_precise_fire_against_fp64(device):
     P 2e-5 relative; the run's own float64 reductions of float32 pair terms
     E/N and P 2e-5 (the float32 radii alone shift E/N by 4.4e-6 and P by
     3.3e-6); non-rattler positions 1e-5 after the mean shift; residual floor
-    (no new low of max|F| or of the energy for 5000 steps) below 1e-5."""
+    (no new low of max|F| or of the energy for 5000 steps) below 1e-5.
+    skin 0: the 27-cell stencil every evaluation; skin 0.1: the Verlet list."""
     from tests import precision_vs_fp64 as PV
 
     fx = np.load(Path(__file__).parent / "golden" / "jamming_n1024.npz")
     rad, L = fx["rad"], float(fx["L"])
-    print("[precision] N=1024 from the float64 minimum", flush=True)
-    got = PV.gpu_precise(fx["ref"], rad, L, max_steps=20000, stall_steps=5000, device=device)
+    print(f"[precision] N=1024 from the float64 minimum, skin {skin}", flush=True)
+    got = PV.gpu_precise(fx["ref"], rad, L, max_steps=20000, stall_steps=5000, device=device, skin=skin)
     res = PV.compare({"pos": fx["ref"]}, got, rad, L)
     print(f"[precision] steps {got['steps']} max|F| {got['max_grad']:.2e} {res}", flush=True)
     assert got["max_grad"] < 1e-5
@@ -1035,6 +1037,29 @@ def test_precise_fire_against_fp64(device):
     assert res["reported_pe_rel_err"] < 2e-5 and res["reported_pressure_rel_err"] < 2e-5
     assert res["pos_max_err"] < 1e-5
     assert abs(res["ref_pe"] - float(fx["ref_pe"])) <= 1e-12 * float(fx["ref_pe"])
+
+
+def test_precise_verlet_list_matches_stencil(device):
+    """The precision form's Verlet list against its cell stencil on FIRE from
+    the random bidisperse start (4096 spheres, many list rebuilds in 200
+    steps): the same pairs in another summation order, so positions agree to
+    float32 rounding of the forces (1e-5 of the cell edge), and the energy
+    to 1e-5 relative; a row bound too small for the packing is an error."""
+    from jaxdem_amd import _lib as Lb
+    from jaxdem_amd import precision as PR
+
+    pos, rad, L = W.bidisperse_jamming(4096, seed=3, phi=0.66)
+    kw = dict(k=1.0, dt=1e-2, max_steps=200, pe_tol=0.0, pe_diff_tol=-1.0, force_tol=0.0, device=device)
+    print("[precision] list vs stencil, 4096 spheres, 200 steps", flush=True)
+    a = PR.minimize_precise(pos, rad, L, **kw)
+    b = PR.minimize_precise(pos, rad, L, skin=0.05, **kw)
+    assert a["steps"] == b["steps"] == 200
+    h = L / a["dims"][0]
+    dx = PR.min_image_diff(a["pos"], b["pos"], L)
+    assert np.abs(dx).max() < 1e-5 * h, np.abs(dx).max()
+    assert abs(a["pe"] - b["pe"]) <= 1e-5 * abs(a["pe"])
+    with pytest.raises(Lb.JdemError, match="max_neighbors"):
+        PR.minimize_precise(pos, rad, L, skin=0.05, max_neighbors=4, **kw)
 
 
 def test_skin_autotuner_system(device):
