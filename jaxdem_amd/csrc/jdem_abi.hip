@@ -2090,25 +2090,25 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
   a.gated = gated ? 1 : 0;
   // The Verlet list (with the counting sort only): pairs within r_i + r_j +
   // skin; a partner is at most 1 + ceil(skin / h) cells away on an axis
-  // when h >= 2 r_max, and the build stencil must not wrap onto itself.
+  // when h >= 2 r_max, so skin <= h keeps the build stencil at half-width
+  // 2 (cell differences in one byte of the entry), which must not wrap onto
+  // itself; slots take the entry's low 24 bits.
   int32_t* vovf = nullptr;
-  if (gated && N > 0 && p->skin > 0.0f && p->max_neighbors > 0) {
+  if (gated && N > 0 && N < (int64_t)1 << 24 && p->skin > 0.0f && p->max_neighbors > 0) {
     float hmin = p->cell[0];
     for (int q = 1; q < D; ++q) hmin = std::min(hmin, p->cell[q]);
     const int sr = 1 + (int)std::ceil(p->skin / hmin);
-    bool fits = sr <= 4;
+    bool fits = sr <= 2;
     for (int q = 0; q < D; ++q) fits = fits && p->dims[q] >= 2 * sr + 1;
     if (fits) {
       ENSURE(ctx, ctx->pvl, 4 * Nn * (size_t)p->max_neighbors);
-      ENSURE(ctx, ctx->pvaux, 4 * Nn * (3 + 2 * (size_t)D) + 64);
+      ENSURE(ctx, ctx->pvaux, 4 * Nn * (size_t)D + 64);
       a.vl = (int32_t*)ctx->pvl.p;
       a.vm = p->max_neighbors;
       a.vsr = sr;
       a.vcut = p->skin;
       a.vtrig = 0.25f * p->skin * p->skin;
-      a.scidx = (int32_t*)ctx->pvaux.p;
-      a.bcidx = a.scidx + 3 * Nn;
-      a.boff = (float*)(a.bcidx + D * Nn);
+      a.boff = (float*)ctx->pvaux.p;
       vovf = (int32_t*)(a.boff + D * Nn);
       a.vovf = vovf;
       HIPCHK(ctx, launch_fill_i32(vovf, 1, 0, st));
@@ -2248,8 +2248,14 @@ int jdem_precise_minimize(jdem_ctx* ctx, jdem_precise* p, const jdem_fire* fire,
     if (rc) return rc;
     ++steps;
   }
+  if (a.vl) {
+    // list mode: fold the offsets back into [0, h) (a rebuild) for the caller
+    HIPCHK(ctx, launch_fill_i32(moved, 1, 1, st));
+    rc = eval(virial != nullptr);
+    if (rc) return rc;
+  }
   if (virial) {
-    rc = eval(true);  // same configuration: the virial of the final state
+    if (!a.vl) rc = eval(true);  // same configuration: the virial of the final state
     if (rc) return rc;
     double w9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     if (N > 0) {

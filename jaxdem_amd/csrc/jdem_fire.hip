@@ -232,11 +232,36 @@ __global__ void __launch_bounds__(kBlock) k_fire_gravity_energy(FireArgs a) {
 // o_j), is formed as a float pair too (error-free sums), so it is exact to
 // ~1e-14 |r| rather than rounded to ~6e-8 |r|.
 
+// x = c h + o with o folded back into [0, h) (c wrapped periodically);
+// returns whether the cell changed
+template <int D>
+__device__ __forceinline__ bool prec_fold(const PrecArgs& a, int64_t i) {
+  bool moved = false;
+#pragma unroll
+  for (int q = 0; q < D; ++q) {
+    float o = a.off[D * i + q];
+    if (!(o >= 0.0f && o < a.h[q])) {
+      int c = a.cidx[D * i + q];
+      const float m = floorf(o / a.h[q]);
+      o = (o - m * a.h[q]) - m * a.hlo[q];
+      if (o < 0.0f) o = 0.0f;  // rounding of o - m h
+      if (o >= a.h[q]) o = 0.0f, c += 1;
+      c = (c + (int)m) % a.dims[q];
+      if (c < 0) c += a.dims[q];
+      a.cidx[D * i + q] = c;
+      a.off[D * i + q] = o;
+      moved = true;
+    }
+  }
+  return moved;
+}
+
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_prec_keys(PrecArgs a) {
   if ((a.gated && *a.moved == 0) || (a.stop && *a.stop)) return;  // no sphere changed cell: the last sort stands
   const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= a.n) return;
+  if (a.vl) prec_fold<D>(a, i);  // list mode: cells are kept between list builds
   const int32_t* c = a.cidx + D * i;
   a.keys[i] = D == 3 ? (uint32_t)((c[0] * a.dims[1] + c[1]) * a.dims[2] + c[2]) : (uint32_t)(c[0] * a.dims[1] + c[1]);
 }
@@ -254,13 +279,15 @@ __global__ void __launch_bounds__(kBlock) k_prec_gather(PrecArgs a) {
   if (kk >= a.n) return;
   const int64_t j = a.perm[kk];
   a.sp[kk] = make_float4(a.off[D * j], a.off[D * j + 1], D == 3 ? a.off[D * j + 2] : 0.0f, a.rad[j]);
-  if (a.vl)
-    for (int q = 0; q < 3; ++q) a.scidx[3 * kk + q] = q < D ? a.cidx[D * j + q] : 0;
 }
 
 // Verlet list of the precision form (rebuilds only, gated by *moved): the
-// partners of the sphere in sorted slot kk0 within r_i + r_j + skin, by
-// sorted slot, in stencil order (half-width vsr); the build positions.
+// partners of the sphere in sorted slot kk0 within r_i + r_j + skin, in
+// stencil order (half-width vsr <= 2), each entry the partner's sorted slot
+// (low 24 bits) and the cell difference c_i - c_j (high 8 bits, base-5
+// digits + 2); the build offsets.  Cells stay fixed until the next build
+// (offsets may leave [0, h) by up to skin / 2 meanwhile), so the entry's
+// cell difference holds for the list's lifetime.
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_prec_vbuild(PrecArgs a) {
   if (*a.moved == 0 || (a.stop && *a.stop)) return;
@@ -272,7 +299,6 @@ __global__ void __launch_bounds__(kBlock) k_prec_vbuild(PrecArgs a) {
   int ci[3] = {0, 0, 0};
   for (int q = 0; q < D; ++q) {
     ci[q] = a.cidx[D * i + q];
-    a.bcidx[D * i + q] = ci[q];
     a.boff[D * i + q] = oi[q];
   }
   int32_t* row = a.vl + kk0 * a.vm;
@@ -300,7 +326,8 @@ __global__ void __launch_bounds__(kBlock) k_prec_vbuild(PrecArgs a) {
           }
           const float R = (si.w + sj.w) + a.vcut;
           if (!(d2 < R * R)) continue;
-          if (cnt < a.vm) row[cnt] = kk;
+          const int code = (2 - ox) + 5 * ((2 - oy) + 5 * (2 - oz));
+          if (cnt < a.vm) row[cnt] = (int32_t)((uint32_t)kk | ((uint32_t)code << 24));
           ++cnt;
         }
       }
@@ -401,12 +428,11 @@ __global__ void __launch_bounds__(kBlock) k_prec_force(PrecArgs a) {
     for (int q = 0; q < D * D; ++q) a.w_out[(int64_t)D * D * i + q] = w[q];
 }
 
-// The spring force of k_prec_force over the Verlet list: the pair's cell
-// difference comes from the cells of the two slots now (wrapped to the
-// nearest image), so spheres may change cell between builds; a contact has
-// |dc| <= 1 on every axis, where the displacement is the same float pair as
-// on the stencil.  Partners are summed in the list's order (the stencil
-// order of the build).
+// The spring force of k_prec_force over the Verlet list, the pair's cell
+// difference read from the entry.  With h >= 2 r_max and offsets within
+// skin / 2 < h / 2 of [0, h), a contact has |c_i - c_j| <= 2 per axis, so
+// (c_i - c_j) h is still exact in float32.  Partners are summed in the
+// list's order (the stencil order of the build).
 template <int D>
 __global__ void __launch_bounds__(kBlock) k_prec_vforce(PrecArgs a) {
   if (a.stop && *a.stop) return;
@@ -416,23 +442,17 @@ __global__ void __launch_bounds__(kBlock) k_prec_vforce(PrecArgs a) {
   const int64_t i = a.perm[kk0];
   const float4 si = a.sp[kk0];
   const float oi[3] = {si.x, si.y, si.z};
-  const int ci[3] = {a.scidx[3 * kk0], a.scidx[3 * kk0 + 1], a.scidx[3 * kk0 + 2]};
   const float ri = si.w;
   float F[3] = {0, 0, 0};
   double e = 0.0, w[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
   const int32_t* row = a.vl + kk0 * a.vm;
   for (int t = 0; t < a.vm; ++t) {
-    const int kk = row[t];
-    if (kk < 0) break;
-    const float4 sj = a.sp[kk];
+    const int32_t ent = row[t];
+    if (ent < 0) break;
+    const float4 sj = a.sp[ent & 0xFFFFFF];
     const float osj[3] = {sj.x, sj.y, sj.z};
-    int dc[3] = {0, 0, 0};
-    for (int q = 0; q < D; ++q) {
-      int d = ci[q] - a.scidx[3 * kk + q];
-      if (2 * d > a.dims[q]) d -= a.dims[q];
-      if (2 * d < -a.dims[q]) d += a.dims[q];
-      dc[q] = d;
-    }
+    const int code = (int)((uint32_t)ent >> 24);
+    const int dc[3] = {code % 5 - 2, (code / 5) % 5 - 2, code / 25 - 2};
     float r[3] = {0, 0, 0}, rl[3] = {0, 0, 0};
     for (int q = 0; q < D; ++q) r[q] = (float)dc[q] * a.h[q] + (oi[q] - osj[q]);
     const float d2 = D == 3 ? (r[0] * r[0] + r[1] * r[1]) + r[2] * r[2] : r[0] * r[0] + r[1] * r[1];
@@ -488,6 +508,18 @@ __global__ void __launch_bounds__(kBlock) k_prec_move(FireArgs f, PrecArgs a) {
   lp.load(f.force, f.vel_pos, i, 1.0f, s.dt);
   float up[D];
   lp.mix(s, f.vel_pos, i, up);
+  if (a.vl) {  // list mode: cells stay; a rebuild once the sphere has moved skin / 2
+    float d2 = 0.0f;
+#pragma unroll
+    for (int q = 0; q < D; ++q) {
+      const float o = a.off[D * i + q] + up[q];
+      a.off[D * i + q] = o;
+      const float d = o - a.boff[D * i + q];
+      d2 = d2 + d * d;
+    }
+    if (d2 > a.vtrig) *a.moved = 1;
+    return;
+  }
   bool moved = false;
 #pragma unroll
   for (int q = 0; q < D; ++q) {
@@ -504,18 +536,6 @@ __global__ void __launch_bounds__(kBlock) k_prec_move(FireArgs f, PrecArgs a) {
       moved = true;
     }
     a.off[D * i + q] = o;
-  }
-  if (a.vl) {  // list mode: a rebuild once the sphere has moved skin / 2
-    float d2 = 0.0f;
-#pragma unroll
-    for (int q = 0; q < D; ++q) {
-      int dc = a.cidx[D * i + q] - a.bcidx[D * i + q];
-      if (2 * dc > a.dims[q]) dc -= a.dims[q];
-      if (2 * dc < -a.dims[q]) dc += a.dims[q];
-      const float d = (float)dc * a.h[q] + (a.off[D * i + q] - a.boff[D * i + q]);
-      d2 = d2 + d * d;
-    }
-    moved = d2 > a.vtrig;
   }
   if (moved) *a.moved = 1;
 }

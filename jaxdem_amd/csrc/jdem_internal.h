@@ -667,16 +667,14 @@ struct PrecArgs {
   int gated;             // 1: the key kernel runs only while *moved, and the gather clears *moved
   const int32_t* stop;   // when set, every kernel returns at once once *stop is set
   // Verlet list (vl set): *moved means "rebuild the list", raised when a
-  // sphere has moved skin / 2 since the build; the sort, the list build and
-  // the build positions run only then, and the list force clears *moved
-  int32_t* vl;           // (N, vm) by sorted slot: sorted slots of the partners, -1 ends a row
+  // sphere has moved skin / 2 since the build; the fold of the offsets, the
+  // sort and the list build run only then, and the list force clears *moved
+  int32_t* vl;           // (N, vm) by sorted slot: partner slot | cell difference << 24, -1 ends a row
   int vm;
-  int vsr;               // stencil half-width of the list build
+  int vsr;               // stencil half-width of the list build (1 or 2)
   float vcut;            // list reach beyond r_i + r_j (the skin)
   float vtrig;           // (skin / 2)^2
-  int32_t* scidx;        // (N, 3) cell of each sorted slot, gathered per evaluation
-  int32_t* bcidx;        // (N, D) cell and offset of each sphere at the build
-  float* boff;
+  float* boff;           // (N, D) offsets at the build
   int32_t* vovf;         // set when a row would exceed vm
 };
 // the stopping tests of one FIRE iteration from the last evaluation's float64
