@@ -2287,8 +2287,9 @@ int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   jdem_step_stats* S = stats ? stats : &local;
   std::memset(S, 0, sizeof(*S));
   if (n == 0) return JDEM_OK;
-  if (sys->collider_kind == JDEM_COLLIDER_CELL_LIST && sys->domain.kind == JDEM_DOMAIN_FREE)
-    return fail(ctx, JDEM_EUNSUPPORTED, "CellList with a FreeDomain (per-step grid) is not implemented");
+  // A CellList in a FreeDomain hashes on the box FreeDomain.apply fits to the
+  // spheres every step (free.py:42-58), so its grid is rebuilt per step.
+  const bool free_cells = sys->collider_kind == JDEM_COLLIDER_CELL_LIST && sys->domain.kind == JDEM_DOMAIN_FREE;
   (void)hipSetDevice(ctx->device);
   rc = ensure_flags(ctx);
   if (rc) return rc;
@@ -2319,9 +2320,48 @@ int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
   bool free_done = false;
   float fbox[3] = {0, 0, 0}, fanc[3] = {0, 0, 0};
 
+  jdem_system fsys;  // free_cells: the system with this step's fitted box and grid
+  float bx[3] = {0, 0, 0}, an[3] = {0, 0, 0};
   auto launch_t = [&](int64_t t) -> int {
-    StepArgs a = base_args(ctx, sys);
     const int r = rd(t), w = wr(t);
+    jdem_system* ss = sys;
+    if (free_cells && N > 0 && t <= 1) {
+      // step 1's box: the input positions, before kernel 0 (which may drift
+      // them in place) runs
+      if (t == 0) {
+        int rc2 = free_box(ctx, P[r], sys, bx, an);
+        if (rc2) return rc2;
+        S->host_syncs++;
+      }
+    } else if (free_cells && N > 0) {
+      // step t applies the domain before its drift (system.py:230-260): the
+      // box of the positions kernel t-1 read, still held in this kernel's
+      // output buffer for t >= 2
+      int rc2 = free_box(ctx, P[w], sys, bx, an);
+      if (rc2) return rc2;
+      S->host_syncs++;
+    }
+    if (free_cells && N > 0 && t >= 1) {
+      // the grid on the fitted box (_partition.py:55-99, non-periodic:
+      // ceil(box / cell) cells)
+      fsys = *sys;
+      jdem_grid& g = fsys.grid;
+      double cells = 1.0;
+      for (int q = 0; q < D; ++q) {
+        fsys.domain.box_size[q] = bx[q];
+        fsys.domain.inv_box_size[q] = 1.0f / bx[q];
+        fsys.domain.anchor[q] = an[q];
+        const float c = std::ceil(bx[q] / g.cell_size[q]);
+        g.dims[q] = c >= 1.0f ? (int32_t)std::min(c, 2.0e9f) : 1;
+        cells *= (double)g.dims[q];
+      }
+      if (cells > (double)INT32_MAX) return fail(ctx, JDEM_EUNSUPPORTED, "cell grid size out of the int32 hash range");
+      g.strides[0] = 1;
+      for (int q = 1; q < D; ++q) g.strides[q] = g.strides[q - 1] * g.dims[q - 1];
+      g.n_cells = (int64_t)cells;
+      ss = &fsys;
+    }
+    StepArgs a = base_args(ctx, ss);
     a.pos_in = P[r];
     a.vel_in = V[r];
     a.ang_in = W[r];
@@ -2342,7 +2382,7 @@ int jdem_system_step(jdem_ctx* ctx, jdem_system* sys, int64_t n, jdem_step_stats
     }
     if (a.do_force && sys->collider_kind != JDEM_COLLIDER_NONE) {
       const int sp = sys->collider_kind == JDEM_COLLIDER_CELL_LIST ? span_begin(ctx, 2, t) : -1;
-      int rc2 = prepare_source(ctx, sys, a, P[r]);
+      int rc2 = prepare_source(ctx, ss, a, P[r]);
       if (rc2) return rc2;
       span_end(ctx, sp);
     }
@@ -2737,8 +2777,9 @@ int jdem_system_step_f64(jdem_ctx* ctx, jdem_system_f64* sys, int64_t n, jdem_st
   jdem_step_stats* S = stats ? stats : &local;
   std::memset(S, 0, sizeof(*S));
   if (n == 0) return JDEM_OK;
-  if (sys->collider_kind == JDEM_COLLIDER_CELL_LIST && sys->domain.kind == JDEM_DOMAIN_FREE)
-    return fail(ctx, JDEM_EUNSUPPORTED, "CellList with a FreeDomain (per-step grid) is not implemented");
+  // A CellList in a FreeDomain hashes on the box FreeDomain.apply fits to the
+  // spheres every step (free.py:42-58), so its grid is rebuilt per step.
+  const bool free_cells = sys->collider_kind == JDEM_COLLIDER_CELL_LIST && sys->domain.kind == JDEM_DOMAIN_FREE;
   (void)hipSetDevice(ctx->device);
   jdem_state_f64& s = sys->state;
   const int64_t N = s.n;
