@@ -2777,9 +2777,8 @@ int jdem_system_step_f64(jdem_ctx* ctx, jdem_system_f64* sys, int64_t n, jdem_st
   jdem_step_stats* S = stats ? stats : &local;
   std::memset(S, 0, sizeof(*S));
   if (n == 0) return JDEM_OK;
-  // A CellList in a FreeDomain hashes on the box FreeDomain.apply fits to the
-  // spheres every step (free.py:42-58), so its grid is rebuilt per step.
-  const bool free_cells = sys->collider_kind == JDEM_COLLIDER_CELL_LIST && sys->domain.kind == JDEM_DOMAIN_FREE;
+  if (sys->collider_kind == JDEM_COLLIDER_CELL_LIST && sys->domain.kind == JDEM_DOMAIN_FREE)
+    return fail(ctx, JDEM_EUNSUPPORTED, "CellList with a FreeDomain (per-step grid) in float64 is not implemented");
   (void)hipSetDevice(ctx->device);
   jdem_state_f64& s = sys->state;
   const int64_t N = s.n;

@@ -17,6 +17,7 @@ import pytest
 import torch
 
 import jaxdem_amd as jd
+from jaxdem_amd import _lib as L
 from oracle import jdem_oracle as O
 from tests import oracle_bridge as B
 from tests.test_gpu_parity import make_system, packed
@@ -199,6 +200,10 @@ def test_f64_unsupported_paths_raise(device):
         with pytest.raises(NotImplementedError, match="float32"):
             jd.System.step(st, hist, n=1)
         sy = make_system(st, box, "periodic", "spring", "naive")
+        stf, boxf = packed(200, 3, seed=1, domain="free")
+        fc = make_system(stf, boxf, "free", "spring", "CellList")
+        with pytest.raises(L.JdemError, match="FreeDomain"):
+            jd.System.step(stf, fc, n=2)
     import dataclasses
 
     mixed = dataclasses.replace(st, vel=st.vel.float())
